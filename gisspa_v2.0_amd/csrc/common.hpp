@@ -1,0 +1,54 @@
+// Shared helpers for the GisSPA HIP library: error handling and complex arithmetic.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+namespace gs {
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+// Status codes mirrored by include/gisspa.h.
+enum : int { kOk = 0, kInvalid = 1, kHip = 2, kUnsupported = 3, kNoMem = 4, kState = 5 };
+
+[[noreturn]] inline void hip_fail(hipError_t e, const char* what, const char* file, int line) {
+  throw Error(e == hipErrorOutOfMemory ? kNoMem : kHip,
+              std::string(what) + ": " + hipGetErrorString(e) + " (" + file + ":" +
+                  std::to_string(line) + ")");
+}
+
+}  // namespace gs
+
+#define GS_HIP(x)                                                   \
+  do {                                                              \
+    hipError_t e_ = (x);                                            \
+    if (e_ != hipSuccess) ::gs::hip_fail(e_, #x, __FILE__, __LINE__); \
+  } while (0)
+
+#define GS_CHECK_LAUNCH() GS_HIP(hipGetLastError())
+
+#define GS_REQUIRE(cond, code, msg)                 \
+  do {                                              \
+    if (!(cond)) throw ::gs::Error((code), (msg));  \
+  } while (0)
+
+__device__ __forceinline__ float2 c_add(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 c_sub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 c_mul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 c_conj(float2 a) { return make_float2(a.x, -a.y); }
+__device__ __forceinline__ float2 c_scale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+// a * conj(b)
+__device__ __forceinline__ float2 c_mulconj(float2 a, float2 b) {
+  return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+// i^k * a for the sign convention of the caller (s = +1 multiplies by +i, -1 by -i)
+__device__ __forceinline__ float2 c_muli(float2 a, int s) {
+  return s > 0 ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+}

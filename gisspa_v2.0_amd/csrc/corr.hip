@@ -1,0 +1,463 @@
+// Correlation hot path: the inverse 2-D transform of Ihat_b * conj(That_t) for every (template,
+// block) plane, fused with the reductions the scoring needs.
+//
+// Reference: compute_corner_CCG (k:645-678) + cufftExecC2C INVERSE (src/norm.cu:536,568;
+// src/nonorm.cu:478) + add_CCG_to_sum / update_CCG / get_peak_pos (k:681-793) and
+// get_peak_and_SUM (k:725-763).
+//
+// Layout (half spectra of real planes, H = L/2):
+//   That[t][kx 0..H][ky 0..L)   Ihat[b][kx 0..H][ky 0..L)   (column-major: a column is contiguous)
+//   Z[t][b][y][kx' 0..H)        inter-pass buffer; kx'=0 packs the real DC and Nyquist columns
+//                               as (re, im), rows are already shifted by L/2.
+// Column pass: one workgroup = COLS adjacent packed columns of one (t, b) plane.  It loads
+// Ihat*conj(That) (the DC and Nyquist columns are Hermitian along ky, so their inverse
+// transforms are real and share one complex transform), runs length-L inverse FFTs in LDS and
+// writes rows of Z.  The (-1)^(x+y) checkerboard of the reference is a shift by L/2 of the
+// output; rows shift here, columns in the row pass, and the extra sign for odd L/2 is folded
+// into the spectra by the caller.
+// Row pass: one workgroup = ROWS rows of one block, looping over the templates of the launch,
+// TPI at a time: each row becomes a length-H complex sequence (real-to-complex packing of the
+// inverse real transform), one LDS FFT gives the L real outputs, and every thread folds the
+// pixels it owns into registers -- sum, sum of squares, max and argmax for norm_type=1 (the
+// per-pixel best record), or per-plane peak/sum partials for norm_type=0.
+#include <cmath>
+#include <mutex>
+#include <vector>
+
+#include "common.hpp"
+#include "fft_tile.hpp"
+#include "kernels.hpp"
+
+namespace gs {
+
+template <int L>
+struct CorrCfg;
+
+template <>
+struct CorrCfg<320> {
+  static constexpr int H = 160;
+  static constexpr int COLS = 16, CT = 256, TPC = 16;
+  using ColPlan = ct::RadixList<4, 4, 4, 5>;
+  static constexpr int ROWS = 8, TPI = 4, RT = 256, TPT = 8;
+  using RowPlan = ct::RadixList<4, 4, 10>;
+};
+
+template <>
+struct CorrCfg<720> {
+  static constexpr int H = 360;
+  static constexpr int COLS = 8, CT = 256, TPC = 32;
+  using ColPlan = ct::RadixList<4, 4, 5, 9>;
+  static constexpr int ROWS = 4, TPI = 2, RT = 360, TPT = 45;
+  using RowPlan = ct::RadixList<4, 9, 10>;
+};
+
+namespace {
+
+template <int L>
+__global__ void __launch_bounds__(CorrCfg<L>::CT)
+    k_colpass(const float2* __restrict__ that, const float2* __restrict__ ihat,
+              const float2* __restrict__ twL, float2* __restrict__ Z, int t0, int tc, int nb) {
+  using C = CorrCfg<L>;
+  constexpr int H = C::H, COLS = C::COLS, NT = C::CT, TPC = C::TPC, LP = L + 1;
+  constexpr int NCG = H / COLS;
+  static_assert(COLS * TPC == NT, "column pass thread mapping");
+  static_assert(H % COLS == 0, "column groups");
+  constexpr long long PLANE = (long long)(H + 1) * L;
+  __shared__ float2 buf[COLS * LP];
+  __shared__ float2 tw[L];
+
+  // Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a contiguous range of
+  // logical work so the template columns it streams stay in its own L2.
+  const unsigned G = gridDim.x, bid = blockIdx.x;
+  const unsigned xcd = bid % 8u, idx = bid / 8u, per = G / 8u, rem = G % 8u;
+  const unsigned logical = (xcd < rem ? xcd * (per + 1u) : rem * (per + 1u) + (xcd - rem) * per) + idx;
+  const int cg = (int)(logical % NCG);
+  const unsigned rest = logical / NCG;
+  const int b = (int)(rest % (unsigned)nb);
+  const int tl = (int)(rest / (unsigned)nb);
+  const int tid = threadIdx.x;
+
+  for (int i = tid; i < L; i += NT) tw[i] = twL[i];
+  const float2* Tp = that + (long long)(t0 + tl) * PLANE;
+  const float2* Ip = ihat + (long long)b * PLANE;
+  const int kx0 = cg * COLS;
+  for (int q = tid; q < COLS * L; q += NT) {
+    const int c = q / L, ky = q - c * L;
+    const long long off = (long long)kx0 * L + q;
+    float2 p = c_mulconj(Ip[off], Tp[off]);
+    if (kx0 == 0 && c == 0) {  // pack the Nyquist column as the imaginary part
+      const float2 pn = c_mulconj(Ip[(long long)H * L + ky], Tp[(long long)H * L + ky]);
+      p = make_float2(p.x - pn.y, p.y + pn.x);
+    }
+    buf[c * LP + ky] = p;
+  }
+  __syncthreads();
+  ct::run_stockham<L, TPC, +1>(buf + (tid / TPC) * LP, tw, tid % TPC, typename C::ColPlan{});
+
+  float2* Zp = Z + ((long long)tl * nb + b) * (long long)L * H;
+  for (int q = tid; q < COLS * L; q += NT) {
+    const int y = q / COLS, cc = q - y * COLS;
+    int py = y + L / 2;
+    if (py >= L) py -= L;
+    Zp[(long long)py * H + kx0 + cc] = buf[cc * LP + y];
+  }
+}
+
+struct PlaneBest {
+  float v;
+  int i;
+  float s, s2;
+};
+__device__ __forceinline__ PlaneBest plane_combine(PlaneBest a, PlaneBest b) {
+  PlaneBest o;
+  const bool take = (b.v > a.v) || (b.v == a.v && b.i < a.i);
+  o.v = take ? b.v : a.v;
+  o.i = take ? b.i : a.i;
+  o.s = a.s + b.s;
+  o.s2 = a.s2 + b.s2;
+  return o;
+}
+
+template <int L, bool PLANE>
+__global__ void __launch_bounds__(CorrCfg<L>::RT)
+    k_rowpass(const float2* __restrict__ Z, const float2* __restrict__ twL,
+              const float2* __restrict__ twH, int tc, int nb, double* __restrict__ S,
+              double* __restrict__ SS, float* __restrict__ best, int32_t* __restrict__ arg,
+              int arg_base, float4* __restrict__ partial, float border) {
+  using C = CorrCfg<L>;
+  constexpr int H = C::H, ROWS = C::ROWS, TPI = C::TPI, NT = C::RT, TPT = C::TPT;
+  constexpr int NTR = ROWS * TPI;
+  constexpr int PPT = ROWS * L / NT;
+  static_assert(PPT * NT == ROWS * L, "pixels per thread");
+  static_assert(NTR * TPT == NT, "row pass thread mapping");
+  __shared__ float2 buf[NTR * H];
+  __shared__ float2 twh[H];
+  __shared__ float2 twc[H];
+  __shared__ PlaneBest red[PLANE ? 512 : 1];
+
+  const int rg = blockIdx.x, b = blockIdx.y, tid = threadIdx.x;
+  const int groups = L / ROWS;
+  for (int i = tid; i < H; i += NT) {
+    twh[i] = twH[i];
+    const float2 w = twL[i];
+    twc[i] = make_float2(w.x, -w.y);  // exp(+2 pi i k / L)
+  }
+  const int y0 = rg * ROWS;
+
+  float s[PPT], ss[PPT], bs[PPT];
+  int ag[PPT];
+#pragma unroll
+  for (int i = 0; i < PPT; ++i) {
+    s[i] = 0.f;
+    ss[i] = 0.f;
+    bs[i] = -INFINITY;
+    ag[i] = -1;
+  }
+  __syncthreads();
+
+  for (int t = 0; t < tc; t += TPI) {
+    // Real inverse transform of each row as a half-length complex one:
+    // Zc(k) = (X(k) + conj X(H-k)) + i e^{2 pi i k/L} (X(k) - conj X(H-k)),
+    // whose inverse gives out(2m) + i out(2m+1).
+    for (int e = tid; e < NTR * H; e += NT) {
+      const int k = e % H, tr = e / H;
+      const int ti = tr / ROWS, row = tr - ti * ROWS;
+      float2 zc = make_float2(0.f, 0.f);
+      if (t + ti < tc) {
+        const float2* zr = Z + ((((long long)(t + ti) * nb + b) * L) + y0 + row) * H;
+        float2 A, B;
+        if (k == 0) {
+          const float2 z0 = zr[0];
+          A = make_float2(z0.x, 0.f);
+          B = make_float2(z0.y, 0.f);
+        } else {
+          A = zr[k];
+          const float2 m = zr[H - k];
+          B = make_float2(m.x, -m.y);
+        }
+        const float2 sum = c_add(A, B), dif = c_sub(A, B);
+        const float2 wd = c_mul(twc[k], dif);
+        zc = make_float2(sum.x - wd.y, sum.y + wd.x);
+      }
+      buf[tr * H + k] = zc;
+    }
+    __syncthreads();
+    ct::run_stockham<H, TPT, +1>(buf + (tid / TPT) * H, twh, tid % TPT, typename C::RowPlan{});
+
+    const float* out = reinterpret_cast<const float*>(buf);
+    if constexpr (!PLANE) {
+#pragma unroll
+      for (int i = 0; i < PPT; ++i) {
+        const int q = tid + i * NT;
+        const int row = q / L, x = q - row * L;
+#pragma unroll
+        for (int ti = 0; ti < TPI; ++ti) {
+          if (t + ti < tc) {
+            const float v = out[(ti * ROWS + row) * L + x];
+            s[i] += v;
+            ss[i] = fmaf(v, v, ss[i]);
+            if (v > bs[i]) {
+              bs[i] = v;
+              ag[i] = t + ti;
+            }
+          }
+        }
+      }
+    } else {
+      for (int ti = 0; ti < TPI; ++ti) {
+        PlaneBest pb{0.f, 0x7fffffff, 0.f, 0.f};
+        if (t + ti < tc) {
+#pragma unroll
+          for (int i = 0; i < PPT; ++i) {
+            const int q = tid + i * NT;
+            const int row = q / L, x = q - row * L;
+            const float v = out[(ti * ROWS + row) * L + x];
+            pb.s += v;
+            pb.s2 = fmaf(v, v, pb.s2);
+            int px = x + L / 2;
+            if (px >= L) px -= L;
+            const int py = y0 + row;
+            const bool edge = px < border || px > L - border || py < border || py > L - border;
+            const int id = py * L + px;
+            if (!edge && v > 0.f && (v > pb.v || (v == pb.v && id < pb.i))) {
+              pb.v = v;
+              pb.i = id;
+            }
+          }
+        }
+        red[tid] = pb;
+        for (int j = tid + NT; j < 512; j += NT) red[j] = PlaneBest{0.f, 0x7fffffff, 0.f, 0.f};
+        __syncthreads();
+        for (int h = 256; h > 0; h >>= 1) {
+          if (tid < h) red[tid] = plane_combine(red[tid], red[tid + h]);
+          __syncthreads();
+        }
+        if (tid == 0 && t + ti < tc) {
+          const PlaneBest r = red[0];
+          partial[((long long)(t + ti) * nb + b) * groups + rg] =
+              make_float4(r.v, __int_as_float(r.i), r.s, r.s2);
+        }
+        __syncthreads();
+      }
+    }
+    __syncthreads();
+  }
+
+  if constexpr (!PLANE) {
+#pragma unroll
+    for (int i = 0; i < PPT; ++i) {
+      const int q = tid + i * NT;
+      const int row = q / L, x = q - row * L;
+      int px = x + L / 2;
+      if (px >= L) px -= L;
+      const long long gi = ((long long)b * L + (y0 + row)) * L + px;
+      S[gi] += (double)s[i];
+      SS[gi] += (double)ss[i];
+      if (bs[i] > best[gi]) {
+        best[gi] = bs[i];
+        arg[gi] = arg_base + ag[i];
+      }
+    }
+  }
+}
+
+__global__ void k_init_acc(double* S, double* SS, float* best, int32_t* arg, long long count) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  S[i] = 0.0;
+  SS[i] = 0.0;
+  best[i] = -INFINITY;
+  arg[i] = -1;
+}
+
+// set_CCG_mean + update_CCG + the host threshold of PickParticles (k:697-721,
+// src/norm.cu:583-597) applied to the per-pixel best record.
+__global__ void k_finalize_norm(const double* __restrict__ S, const double* __restrict__ SS,
+                                const float* __restrict__ best, const int32_t* __restrict__ arg,
+                                int nb, int l, double total_n, float thresh,
+                                const int* __restrict__ offx, const int* __restrict__ offy, int bx,
+                                int W, int H, float d_m, double* __restrict__ mu,
+                                double* __restrict__ sigma, Cand* __restrict__ cand,
+                                int* __restrict__ count, int max_cand) {
+  const long long L2 = (long long)l * l;
+  const long long gi = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gi >= L2 * nb) return;
+  const int b = (int)(gi / L2);
+  const int k = (int)(gi - (long long)b * L2);
+  const double m = S[gi] / total_n;
+  const double sd = sqrt(SS[gi] / total_n - m * m);
+  const double z = sd > 0 ? ((double)best[gi] - m) / sd : 0.0;
+  if (mu) mu[gi] = m;
+  if (sigma) sigma[gi] = sd;
+  if (z >= (double)thresh) {
+    const int cx = offx[b % bx] + k % l, cy = offy[b / bx] + k / l;
+    if (cx >= d_m && cx < W - d_m && cy >= d_m && cy < H - d_m) {
+      const int slot = atomicAdd(count, 1);
+      if (slot < max_cand) cand[slot] = Cand{(float)cx, (float)cy, (float)z, arg[gi], b, k};
+    }
+  }
+}
+
+// Per (template, block): merge row-group partials in row order (strict max, lowest index on
+// ties; sums in double).
+__global__ void k_finalize_plane(const float4* __restrict__ partial, int tc, int nb, int groups,
+                                 double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= tc * nb) return;
+  float v = 0.f;
+  int id = 0x7fffffff;
+  double s = 0, s2 = 0;
+  for (int g = 0; g < groups; ++g) {
+    const float4 p = partial[(long long)i * groups + g];
+    const int pi = __float_as_int(p.y);
+    if (p.x > v || (p.x == v && pi < id)) {
+      v = p.x;
+      id = pi;
+    }
+    s += p.z;
+    s2 += p.w;
+  }
+  out[4LL * i + 0] = v;
+  out[4LL * i + 1] = (id == 0x7fffffff) ? 0.0 : (double)id;
+  out[4LL * i + 2] = s;
+  out[4LL * i + 3] = s2;
+}
+
+}  // namespace
+
+// Twiddle tables exp(-2 pi i e/N) for the compile-time transforms, built once per process.
+struct TwTables {
+  float2* t320 = nullptr;
+  float2* t160 = nullptr;
+  float2* t720 = nullptr;
+  float2* t360 = nullptr;
+};
+static TwTables& tw_tables(int device);
+
+static float2* make_table(int n) {
+  std::vector<float2> h(n);
+  for (int e = 0; e < n; ++e) {
+    const double a = 2.0 * M_PI * (double)e / (double)n;
+    h[e] = make_float2((float)std::cos(a), (float)-std::sin(a));
+  }
+  float2* d = nullptr;
+  GS_HIP(hipMalloc(&d, sizeof(float2) * n));
+  GS_HIP(hipMemcpy(d, h.data(), sizeof(float2) * n, hipMemcpyHostToDevice));
+  return d;
+}
+
+static TwTables& tw_tables(int device) {
+  static TwTables tabs[64];
+  static bool ready[64] = {};
+  static std::mutex mu;
+  std::lock_guard<std::mutex> lock(mu);
+  GS_REQUIRE(device >= 0 && device < 64, kInvalid, "device index out of range");
+  if (!ready[device]) {
+    tabs[device].t320 = make_table(320);
+    tabs[device].t160 = make_table(160);
+    tabs[device].t720 = make_table(720);
+    tabs[device].t360 = make_table(360);
+    ready[device] = true;
+  }
+  return tabs[device];
+}
+
+static int current_device() {
+  int d = 0;
+  GS_HIP(hipGetDevice(&d));
+  return d;
+}
+
+bool corr_supported(int L) { return L == 320 || L == 720; }
+
+int rowpass_groups(int L) {
+  if (L == 320) return 320 / CorrCfg<320>::ROWS;
+  if (L == 720) return 720 / CorrCfg<720>::ROWS;
+  return 0;
+}
+
+template <int L>
+static void colpass_impl(const CorrArgs& a, const float2* twL, hipStream_t s) {
+  using C = CorrCfg<L>;
+  const long long G = (long long)(C::H / C::COLS) * a.tc * a.nb;
+  GS_REQUIRE(G > 0 && G < (1LL << 31), kInvalid, "column pass grid out of range");
+  k_colpass<L><<<(unsigned)G, C::CT, 0, s>>>(a.that, a.ihat, twL, a.Z, a.t0, a.tc, a.nb);
+  GS_CHECK_LAUNCH();
+}
+
+bool launch_colpass(int L, const CorrArgs& a, hipStream_t s) {
+  TwTables& T = tw_tables(current_device());
+  if (L == 320) {
+    colpass_impl<320>(a, T.t320, s);
+    return true;
+  }
+  if (L == 720) {
+    colpass_impl<720>(a, T.t720, s);
+    return true;
+  }
+  return false;
+}
+
+template <int L, bool PLANE>
+static void rowpass_impl(const float2* Z, int tc, int nb, double* S, double* SS, float* best,
+                         int32_t* arg, int arg_base, float4* partial, float border,
+                         const float2* twL, const float2* twH, hipStream_t s) {
+  using C = CorrCfg<L>;
+  dim3 grid(L / C::ROWS, nb);
+  k_rowpass<L, PLANE><<<grid, C::RT, 0, s>>>(Z, twL, twH, tc, nb, S, SS, best, arg, arg_base,
+                                             partial, border);
+  GS_CHECK_LAUNCH();
+}
+
+bool launch_rowpass_pixel(int L, const float2* Z, int tc, int nb, double* S, double* SS,
+                          float* best, int32_t* arg, int arg_base, hipStream_t s) {
+  TwTables& T = tw_tables(current_device());
+  if (L == 320) {
+    rowpass_impl<320, false>(Z, tc, nb, S, SS, best, arg, arg_base, nullptr, 0.f, T.t320, T.t160, s);
+    return true;
+  }
+  if (L == 720) {
+    rowpass_impl<720, false>(Z, tc, nb, S, SS, best, arg, arg_base, nullptr, 0.f, T.t720, T.t360, s);
+    return true;
+  }
+  return false;
+}
+
+bool launch_rowpass_plane(int L, const float2* Z, int tc, int nb, float4* partial, float border,
+                          hipStream_t s) {
+  TwTables& T = tw_tables(current_device());
+  if (L == 320) {
+    rowpass_impl<320, true>(Z, tc, nb, nullptr, nullptr, nullptr, nullptr, 0, partial, border, T.t320, T.t160, s);
+    return true;
+  }
+  if (L == 720) {
+    rowpass_impl<720, true>(Z, tc, nb, nullptr, nullptr, nullptr, nullptr, 0, partial, border, T.t720, T.t360, s);
+    return true;
+  }
+  return false;
+}
+
+void launch_finalize_plane(const float4* partial, int tc, int nb, int groups, double* out, hipStream_t s) {
+  const int n = tc * nb;
+  k_finalize_plane<<<(n + 255) / 256, 256, 0, s>>>(partial, tc, nb, groups, out);
+  GS_CHECK_LAUNCH();
+}
+
+void launch_init_accumulators(double* S, double* SS, float* best, int32_t* arg, long long count,
+                              hipStream_t s) {
+  k_init_acc<<<(unsigned)((count + 255) / 256), 256, 0, s>>>(S, SS, best, arg, count);
+  GS_CHECK_LAUNCH();
+}
+
+void launch_finalize_norm(const double* S, const double* SS, const float* best, const int32_t* arg,
+                          int nb, int l, double total_n, float thresh, const int* offx,
+                          const int* offy, int bx, int W, int H, float d_m, double* mu,
+                          double* sigma, Cand* cand, int* count, int max_cand, hipStream_t s) {
+  const long long n = (long long)nb * l * l;
+  k_finalize_norm<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(S, SS, best, arg, nb, l, total_n, thresh,
+                                                              offx, offy, bx, W, H, d_m, mu, sigma,
+                                                              cand, count, max_cand);
+  GS_CHECK_LAUNCH();
+}
+
+}  // namespace gs

@@ -1,0 +1,900 @@
+// Search engine: orchestration of the device pipeline and the C ABI of include/gisspa.h.
+//
+// One engine = one device + one HIP stream (default priority), as one SearchNorm/SearchNoNorm
+// per host thread in the reference (src/norm.cu:48-123, src/nonorm.cu:50-122).  Device buffers
+// grow on demand and persist across micrographs instead of being rebuilt per micrograph.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/gisspa.h"
+#include "common.hpp"
+#include "fft_generic.hpp"
+#include "kernels.hpp"
+
+namespace gs {
+
+constexpr double kRefPi = 3.1415926535;  // include/utils/constants.h:3
+constexpr int kTile = 720;               // include/utils/constants.h:5
+
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  T* ensure(size_t count) {
+    if (count <= n && p) return p;
+    release();
+    GS_HIP(hipMalloc(&p, sizeof(T) * std::max<size_t>(count, 1)));
+    n = count;
+    return p;
+  }
+};
+
+// blocks_one_axis of src/norm.cu:68-81.
+static std::vector<int> block_offsets(int length, int l, int overlap) {
+  std::vector<int> o{0};
+  int off = 0;
+  while (off + l < length) {
+    off += l - overlap;
+    if (off + l >= length) off = length - l;
+    o.push_back(off);
+  }
+  return o;
+}
+
+// The float angle loop of src/norm.cu:642 / src/nonorm.cu:582.
+static std::vector<float> angle_list(float phistep) {
+  std::vector<float> a;
+  for (float e = 0.0f; e < 360.0f; e += phistep) a.push_back(e);
+  return a;
+}
+
+// Exact radial bin, identical to the device rbin.
+static int rbin_host(long long dx, long long dy) {
+  long long n = dx * dx + dy * dy;
+  long long k = (long long)std::sqrt((double)n);
+  while (k * k > n) --k;
+  while ((k + 1) * (k + 1) <= n) ++k;
+  if (n > k * k + k) ++k;
+  return (int)k - 1;
+}
+
+// Pixel counts per radial bin over the half plane x <= nx/2 (the rb of SQRSum_by_circle,
+// k:115-125): purely geometric, so computed exactly on the host.
+static std::vector<double> radial_counts(int nx, int ny) {
+  const int nbins = std::max(nx, ny) / 2;
+  std::vector<double> rb(nbins, 0.0);
+  for (int y = 0; y < ny; ++y)
+    for (int x = 0; x <= nx / 2; ++x) {
+      const int r = rbin_host(std::min(x, nx - x), std::min(y, ny - y));
+      if (r >= 0 && r < nbins) rb[r] += 1;
+    }
+  return rb;
+}
+
+// LoadImage host clipping (src/norm.cu:156-181): float sequential sums, int-truncated bounds.
+static void clip_image_host(float* img, size_t N, int nx, int ny, int invert) {
+#pragma clang fp contract(off)
+  float sum = 0, sum_s2 = 0;
+  for (size_t i = 0; i < N; ++i) {
+    const float cur = img[i];
+    sum += cur / nx / ny;
+    sum_s2 += (cur * cur / nx / ny);
+  }
+  const float avg = sum;
+  const float var = std::sqrt(sum_s2 - avg * avg);
+  const int up = (int)(avg + 6 * var);
+  const int low = (int)(avg - 6 * var);
+  if (sum_s2 > avg * avg)
+    for (size_t i = 0; i < N; ++i)
+      if (img[i] > up || img[i] < low) img[i] = avg;
+  if (invert)
+    for (size_t i = 0; i < N; ++i) img[i] = -img[i];
+}
+
+// TileImages geometry (src/utils/tileimages.cpp:18-70), upper-left corners in x-major order.
+static void tile_geometry(int W, int H, std::vector<int>& ulx, std::vector<int>& uly) {
+  const int cols = (int)std::ceil((double)W / kTile), rows = (int)std::ceil((double)H / kTile);
+  const int x_step = W / cols, y_step = H / rows;
+  for (int j = 0; j < cols; ++j) {
+    const int cx = j == 0 ? kTile / 2 : (j == cols - 1 ? W - kTile / 2 : x_step * (2 * j + 1) / 2);
+    for (int i = 0; i < rows; ++i) {
+      const int cy = i == 0 ? kTile / 2 : (i == rows - 1 ? H - kTile / 2 : y_step * (2 * i + 1) / 2);
+      ulx.push_back(std::max(cx - kTile / 2, 0));
+      uly.push_back(std::max(cy - kTile / 2, 0));
+    }
+  }
+}
+
+static bool record_before(const gs_record& a, const gs_record& b) {
+  if (a.score != b.score) return a.score > b.score;
+  if (a.y != b.y) return a.y < b.y;
+  if (a.x != b.x) return a.x < b.x;
+  if (a.template_id != b.template_id) return a.template_id < b.template_id;
+  if (a.angle_index != b.angle_index) return a.angle_index < b.angle_index;
+  if (a.block != b.block) return a.block < b.block;
+  return a.pixel < b.pixel;
+}
+
+// Greedy non-maximum suppression (no reference implementation: the reference leaves it to
+// external scripts, README.md:81): best first, drop anything strictly closer than radius.
+static void nms(std::vector<gs_record>& v, float radius) {
+  std::stable_sort(v.begin(), v.end(), record_before);
+  if (radius <= 0) return;
+  const double R2 = (double)radius * radius;
+  std::vector<gs_record> kept;
+  for (const auto& c : v) {
+    bool ok = true;
+    for (const auto& k : kept) {
+      const double dx = (double)c.x - k.x, dy = (double)c.y - k.y;
+      if (dx * dx + dy * dy < R2) {
+        ok = false;
+        break;
+      }
+    }
+    if (ok) kept.push_back(c);
+  }
+  v.swap(kept);
+}
+
+struct KernelClock {
+  hipEvent_t a = nullptr, b = nullptr;
+  double total_ms = 0;
+  long long launches = 0;
+};
+
+class Engine {
+ public:
+  Engine(const gs_params& p, int device) : P_(p), device_(device) {
+    validate();
+    int count = 0;
+    GS_HIP(hipGetDeviceCount(&count));
+    GS_REQUIRE(device >= 0 && device < count, kInvalid, "device index out of range");
+    GS_HIP(hipSetDevice(device_));
+    GS_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    GS_HIP(hipEventCreate(&ev_start_));
+    GS_HIP(hipEventCreate(&ev_corr0_));
+    GS_HIP(hipEventCreate(&ev_corr1_));
+    GS_HIP(hipEventCreate(&ev_end_));
+  }
+  ~Engine() {
+    (void)hipSetDevice(device_);
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    for (hipEvent_t e : {ev_start_, ev_corr0_, ev_corr1_, ev_end_})
+      if (e) (void)hipEventDestroy(e);
+    for (KernelClock* k : {&clk_col_, &clk_row_}) {
+      if (k->a) (void)hipEventDestroy(k->a);
+      if (k->b) (void)hipEventDestroy(k->b);
+    }
+    if (stream_) (void)hipStreamDestroy(stream_);
+  }
+
+  void load_templates(const float* data, int n, int w, int h, const float* alt_az) {
+    GS_REQUIRE(data && n > 0 && w > 0 && h > 0, kInvalid, "empty template stack");
+    const int l = P_.window_size;
+    GS_REQUIRE(w <= l && h <= l, kInvalid, "templates larger than window_size");
+    bind();
+    n_ = n;
+    w_ = w;
+    h_ = h;
+    alt_.assign(n, 0.f);
+    az_.assign(n, 0.f);
+    if (alt_az)
+      for (int i = 0; i < n; ++i) {
+        alt_[i] = alt_az[2 * i];
+        az_[i] = alt_az[2 * i + 1];
+      }
+    d_raw_.ensure((size_t)n * w * h);
+    GS_HIP(hipMemcpyAsync(d_raw_.p, data, sizeof(float) * n * w * h, hipMemcpyHostToDevice, stream_));
+    GS_HIP(hipStreamSynchronize(stream_));
+  }
+
+  void set_image(const float* img, int W, int H, const gs_ctf& ctf) {
+    GS_REQUIRE(img && W > 0 && H > 0, kInvalid, "empty image");
+    const int l = P_.window_size;
+    if (P_.norm_type) {
+      GS_REQUIRE(W >= l && H >= l, kUnsupported, "norm_type=1 needs an image at least window_size wide and high");
+    } else {
+      GS_REQUIRE(W >= kTile && H >= kTile, kUnsupported, "norm_type=0 needs an image of at least 720 x 720");
+    }
+    bind();
+    W_ = W;
+    H_ = H;
+    ctf_ = ctf;
+    host_img_.assign(img, img + (size_t)W * H);
+    if (P_.norm_type) clip_image_host(host_img_.data(), host_img_.size(), W, H, P_.invert);
+    d_img_.ensure((size_t)W * H);
+    GS_HIP(hipMemcpyAsync(d_img_.p, host_img_.data(), sizeof(float) * W * H, hipMemcpyHostToDevice, stream_));
+    GS_HIP(hipStreamSynchronize(stream_));
+    have_image_ = true;
+  }
+
+  void run() {
+    GS_REQUIRE(n_ > 0, kState, "no templates loaded");
+    GS_REQUIRE(have_image_, kState, "no image set");
+    bind();
+    set_params();
+    records_.clear();
+    GS_HIP(hipEventRecord(ev_start_, stream_));
+    if (P_.norm_type)
+      run_norm();
+    else
+      run_nonorm();
+    GS_HIP(hipEventRecord(ev_end_, stream_));
+    GS_HIP(hipStreamSynchronize(stream_));
+    float ms = 0;
+    GS_HIP(hipEventElapsedTime(&ms, ev_start_, ev_end_));
+    total_ms_ = ms;
+    GS_HIP(hipEventElapsedTime(&ms, ev_corr0_, ev_corr1_));
+    corr_ms_ = ms;
+    if (P_.norm_type)
+      collect_norm();
+    else
+      collect_nonorm();
+    nms(records_, P_.nms_radius);
+    ran_ = true;
+  }
+
+  const std::vector<gs_record>& records() const { return records_; }
+
+  size_t stage_bytes(int stage) const {
+    const size_t l = P_.window_size, hx = l / 2 + 1;
+    switch (stage) {
+      case GS_STAGE_TEMPLATES:
+        return P_.norm_type ? d_tprep_.n * sizeof(float) : (size_t)n_ * hx * l * sizeof(float2);
+      case GS_STAGE_IMAGE: return P_.norm_type ? (size_t)W_ * H_ * sizeof(float) : 0;
+      case GS_STAGE_BLOCKS: return P_.norm_type ? (size_t)nb_ * hx * l * sizeof(float2) : 0;
+      case GS_STAGE_ROTATED: return P_.norm_type ? (size_t)n_ * hx * l * sizeof(float2) : 0;
+      case GS_STAGE_MU:
+      case GS_STAGE_SIGMA: return P_.norm_type ? (size_t)nb_ * l * l * sizeof(double) : 0;
+      case GS_STAGE_BEST: return P_.norm_type ? (size_t)nb_ * l * l * sizeof(float) : 0;
+      case GS_STAGE_ARG: return P_.norm_type ? (size_t)nb_ * l * l * sizeof(int32_t) : 0;
+      case GS_STAGE_PLANES: return P_.norm_type ? 0 : planes_host_.size() * sizeof(double);
+      default: return 0;
+    }
+  }
+
+  void fetch(int stage, void* out, size_t bytes) {
+    GS_REQUIRE(ran_, kState, "no completed run");
+    const size_t need = stage_bytes(stage);
+    GS_REQUIRE(need > 0 && bytes == need, kInvalid, "stage unavailable or size mismatch");
+    bind();
+    const void* src = nullptr;
+    switch (stage) {
+      case GS_STAGE_TEMPLATES: src = P_.norm_type ? (const void*)d_tprep_.p : (const void*)d_that_.p; break;
+      case GS_STAGE_IMAGE: src = d_imgpre_.p; break;
+      case GS_STAGE_BLOCKS: src = d_ihat_.p; break;
+      case GS_STAGE_ROTATED: src = d_that_.p; break;
+      case GS_STAGE_MU: src = d_mu_.p; break;
+      case GS_STAGE_SIGMA: src = d_sigma_.p; break;
+      case GS_STAGE_BEST: src = d_best_.p; break;
+      case GS_STAGE_ARG: src = d_arg_.p; break;
+      case GS_STAGE_PLANES: std::memcpy(out, planes_host_.data(), bytes); return;
+    }
+    GS_HIP(hipMemcpy(out, src, bytes, hipMemcpyDeviceToHost));
+  }
+
+  void set_limits(int na, int nt) {
+    lim_angles_ = na;
+    lim_templates_ = nt;
+  }
+  void set_profiling(bool on) { profile_ = on; }
+  void timing(double* total, double* corr) const {
+    if (total) *total = total_ms_;
+    if (corr) *corr = corr_ms_;
+  }
+  void kernel_timing(int which, double* avg_ms, long long* count) const {
+    const KernelClock& k = which == 0 ? clk_col_ : clk_row_;
+    if (avg_ms) *avg_ms = k.launches ? k.total_ms / k.launches : 0.0;
+    if (count) *count = k.launches;
+  }
+  void set_chunk(int tc) { chunk_ = tc; }
+
+  const gs_params& params() const { return P_; }
+  float alt(int j) const { return (j >= 0 && j < (int)alt_.size()) ? alt_[j] : 0.f; }
+  float az(int j) const { return (j >= 0 && j < (int)az_.size()) ? az_[j] : 0.f; }
+
+ private:
+  void validate() {
+    GS_REQUIRE(P_.window_size > 0 && P_.window_size % 2 == 0, kInvalid, "window_size must be even and positive");
+    GS_REQUIRE(corr_supported(P_.window_size), kUnsupported, "window_size must be 320 or 720 in this build");
+    GS_REQUIRE(P_.phistep > 0, kInvalid, "phistep must be positive");
+    GS_REQUIRE(P_.angpix > 0, kInvalid, "angpix must be positive");
+    GS_REQUIRE(P_.overlap >= 0 && P_.overlap < P_.window_size, kInvalid, "overlap out of range");
+    GS_REQUIRE(P_.norm_type || P_.window_size <= kTile, kInvalid, "window_size must be <= 720 for norm_type=0");
+  }
+
+  void bind() { GS_HIP(hipSetDevice(device_)); }
+
+  void set_params() {  // SetParams, src/norm.cu:183-192
+    DevParams& d = dp_;
+    d.apix = P_.angpix;
+    d.kk = P_.kk;
+    d.highres = P_.highres;
+    d.lowres = P_.lowres;
+    d.d_m = P_.diameter;
+    d.thresh = P_.threshold;
+    const double def = ctf_.defocus;
+    d.dfdiff = ctf_.dfdiff;
+    d.dfang = ctf_.dfang;
+    d.dfu = -def + ctf_.dfdiff;
+    d.dfv = -def - ctf_.dfdiff;
+    const double E = P_.energy;
+    d.lambda = 12.2639 / std::sqrt(E * 1000.0 + 0.97845 * E * E);
+    d.cs = P_.cs;
+    d.ampconst = (double)0.07f;
+    d.ehw = 4.0f;
+    d.a = -10.81f;
+    d.b = 1.f;
+    d.b2 = 0.32f;
+    d.bf = -18.17f;
+    d.bf2 = -15.22f;
+    d.bf3 = 1.72f;
+    const int l = P_.window_size;
+    d.hr_t = (float)l * P_.angpix / P_.highres;
+    d.lr_t = (float)l * P_.angpix / P_.lowres;
+    d.ds = 1.0f / (P_.angpix * (float)l);
+  }
+
+  void set_image_band(int nx, int ny) {
+    const int l = std::max(nx, ny);
+    dp_.hr_i = (float)l * P_.angpix / P_.highres;
+    dp_.lr_i = (float)l * P_.angpix / P_.lowres;
+    dp_.hi_thr = (float)l * P_.angpix / 6;
+  }
+
+  const double* radial_counts_dev(int nx, int ny) {
+    const long long key = (long long)nx * 100000 + ny;
+    auto it = rb_cache_.find(key);
+    if (it != rb_cache_.end()) return it->second->p;
+    auto rb = radial_counts(nx, ny);
+    auto buf = std::make_unique<DevBuf<double>>();
+    buf->ensure(rb.size());
+    GS_HIP(hipMemcpy(buf->p, rb.data(), sizeof(double) * rb.size(), hipMemcpyHostToDevice));
+    const double* p = buf->p;
+    rb_cache_.emplace(key, std::move(buf));
+    return p;
+  }
+
+  // Whole-frame (or tile) preprocessing: src/norm.cu:356-402 / src/nonorm.cu:346-389.
+  void prep_frame(const float* in, int nx, int ny, bool dc_zero, float* out) {
+    const size_t N = (size_t)nx * ny;
+    float2* F = d_frame_.ensure(N);
+    float2* S = d_frame2_.ensure(N);
+    launch_real_to_complex(in, F, (long long)N, 1.f, stream_);
+    fft2_batched(fft_, F, S, 1, ny, nx, -1, (float)(1.0 / (double)N), stream_);
+    set_image_band(nx, ny);
+    launch_phase_flip(F, nx, ny, dp_, stream_);
+    const int nbins = std::max(nx, ny) / 2;
+    double* ra = d_ra_img_.ensure(nbins);
+    GS_HIP(hipMemsetAsync(ra, 0, sizeof(double) * nbins, stream_));
+    launch_radial_power_image(F, nx, ny, ra, stream_);
+    launch_whiten_filter_image(F, nx, ny, ra, radial_counts_dev(nx, ny), dp_, stream_);
+    if (dc_zero) launch_zero_dc(F, stream_);
+    fft2_batched(fft_, F, S, 1, ny, nx, +1, 1.f, stream_);
+    launch_extract_real(F, out, (long long)N, stream_);
+  }
+
+  // Template preprocessing into full complex planes in work_ (src/norm.cu:194-354 or
+  // src/nonorm.cu:178-344).  norm: ends in real space; nonorm: ends in Fourier space.
+  void prep_templates() {
+    const int l = P_.window_size, n = n_;
+    const size_t L2 = (size_t)l * l;
+    float2* T = d_work_.ensure((size_t)n * L2);
+    float2* S = d_scratch_.ensure((size_t)n * L2);
+    launch_pad_templates(d_raw_.p, n, w_, h_, l, T, stream_);
+    double* mom = d_mom_.ensure((size_t)n * 4);
+    launch_template_moments(T, n, l, mom, stream_);
+    launch_template_standardise(T, n, l, mom, stream_);
+    fft2_batched(fft_, T, S, n, l, l, -1, (float)(1.0 / (double)L2), stream_);
+    double* ra = d_ra_.ensure((size_t)n * (l / 2));
+    launch_radial_power_planes(T, n, l, ra, stream_);
+    launch_whiten_planes(T, n, l, ra, radial_counts_dev(l, l), stream_);
+    if (!P_.norm_type) {  // src/nonorm.cu:291-299
+      fft2_batched(fft_, T, S, n, l, l, +1, 1.f, stream_);
+      launch_mask_planes(T, n, l, P_.diameter, 4.0f, stream_);
+      fft2_batched(fft_, T, S, n, l, l, -1, (float)(1.0 / (double)L2), stream_);
+    }
+    launch_weight_planes(T, n, l, dp_, stream_);
+    double* m = d_m_.ensure(n);
+    launch_area_power_full(T, n, l, m, stream_);
+    const float sign = ((l / 2) % 2) ? -1.f : 1.f;  // checkerboard sign for odd l/2 (k:669-677)
+    launch_scale_planes(T, n, (long long)L2, m, P_.norm_type ? 1.f : sign, stream_);
+    if (P_.norm_type) {
+      fft2_batched(fft_, T, S, n, l, l, +1, 1.f, stream_);
+      launch_extract_real(T, d_tprep_.ensure((size_t)n * L2), (long long)n * L2, stream_);
+    } else {
+      const int hx = l / 2 + 1;
+      transpose_batched(T, d_that_.ensure((size_t)n * hx * l), n, l, l, hx, stream_);
+    }
+  }
+
+  // Half spectra [p][kx 0..l/2][ky] scaled by `scale` of `np` complex l x l planes in `planes`
+  // (the real-input forward transform of the reference's C2C + scale, src/norm.cu:516-519).
+  void half_spectra(float2* planes, int np, float2* out, float scale) {
+    const int l = P_.window_size, hx = l / 2 + 1;
+    float2* S = d_scratch_.ensure((size_t)np * l * l);
+    fft_rows(fft_, planes, S, l, (long long)np * l, -1, 1.f, stream_);
+    transpose_batched(planes, out, np, l, l, hx, stream_);
+    fft_rows(fft_, out, S, l, (long long)np * hx, -1, scale, stream_);
+  }
+
+  void clock_begin(KernelClock& k) {
+    if (!profile_) return;
+    if (!k.a) {
+      GS_HIP(hipEventCreate(&k.a));
+      GS_HIP(hipEventCreate(&k.b));
+    }
+    GS_HIP(hipEventRecord(k.a, stream_));
+  }
+  void clock_end(KernelClock& k) {
+    if (!profile_) return;
+    GS_HIP(hipEventRecord(k.b, stream_));
+    GS_HIP(hipEventSynchronize(k.b));
+    float ms = 0;
+    GS_HIP(hipEventElapsedTime(&ms, k.a, k.b));
+    k.total_ms += ms;
+    k.launches += 1;
+  }
+
+  int chunk_for(int nb) const {
+    if (chunk_ > 0) return chunk_;
+    // keep the inter-pass buffer near 3 GB at l=320 (32 templates x 234 blocks)
+    const double plane = (double)P_.window_size * (P_.window_size / 2) * sizeof(float2);
+    int tc = (int)std::max(4.0, std::min(64.0, 3.2e9 / (plane * std::max(nb, 1))));
+    return tc;
+  }
+
+  // norm_type = 1 (SearchNorm::work, src/norm.cu:620-660) with one correlation pass.
+  void run_norm() {
+    const int l = P_.window_size, hx = l / 2 + 1, n = n_;
+    const size_t L2 = (size_t)l * l;
+    prep_templates();
+    float* pre = d_imgpre_.ensure((size_t)W_ * H_);
+    prep_frame(d_img_.p, W_, H_, true, pre);
+    // SplitImage (src/norm.cu:404-477)
+    const auto ox = block_offsets(W_, l, P_.overlap), oy = block_offsets(H_, l, P_.overlap);
+    bx_ = (int)ox.size();
+    nb_ = (int)(ox.size() * oy.size());
+    offx_ = ox;
+    offy_ = oy;
+    int* doffx = d_offx_.ensure(ox.size());
+    int* doffy = d_offy_.ensure(oy.size());
+    GS_HIP(hipMemcpyAsync(doffx, ox.data(), sizeof(int) * ox.size(), hipMemcpyHostToDevice, stream_));
+    GS_HIP(hipMemcpyAsync(doffy, oy.data(), sizeof(int) * oy.size(), hipMemcpyHostToDevice, stream_));
+    float2* blk = d_blocks_.ensure((size_t)nb_ * L2);
+    launch_split_blocks(pre, W_, H_, l, doffx, doffy, bx_, nb_, blk, stream_);
+    float2* ihat = d_ihat_.ensure((size_t)nb_ * hx * l);
+    half_spectra(blk, nb_, ihat, (float)(1.0 / (double)L2));
+    double* m = d_m_.ensure(std::max(n, nb_));
+    launch_area_power_half(ihat, nb_, l, m, stream_);
+    // fold the 1/l^2 of add_CCG_to_sum (k:691) and the odd-l/2 checkerboard sign into Ihat
+    const float sign = ((l / 2) % 2) ? -1.f : 1.f;
+    launch_scale_planes(ihat, nb_, (long long)hx * l, m, (float)(sign / (double)L2), stream_);
+
+    const long long npix = (long long)nb_ * L2;
+    launch_init_accumulators(d_S_.ensure(npix), d_SS_.ensure(npix), d_best_.ensure(npix),
+                             d_arg_.ensure(npix), npix, stream_);
+    auto angles = angle_list(P_.phistep);
+    int na = (int)angles.size();
+    if (lim_angles_ > 0) na = std::min(na, lim_angles_);
+    const int nt = lim_templates_ > 0 ? std::min(n, lim_templates_) : n;
+    angles_ = angles;
+    const int tc_max = chunk_for(nb_);
+    float2* Z = d_Z_.ensure((size_t)tc_max * nb_ * l * (l / 2));
+    float* rot = d_rot_.ensure((size_t)n * L2);
+    double* part = d_part_.ensure((size_t)n * l * 2);
+    float* stats = d_stats_.ensure((size_t)n * 2);
+    float2* that = d_that_.ensure((size_t)n * hx * l);
+    float2* work = d_work_.ensure((size_t)n * L2);
+    GS_HIP(hipEventRecord(ev_corr0_, stream_));
+    for (int a = 0; a < na; ++a) {
+      // RotateTemplate (src/norm.cu:479-521)
+      const float cose = (float)std::cos((double)angles[a] * kRefPi / 180);
+      const float sine = (float)std::sin((double)angles[a] * kRefPi / 180);
+      launch_rotate_mask(d_tprep_.p, nt, l, cose, sine, P_.diameter, 4.0f, rot, part, stream_);
+      launch_rotation_stats(part, nt, l, stats, stream_);
+      launch_standardise_to_complex(rot, nt, l, stats, work, stream_);
+      half_spectra(work, nt, that, (float)(1.0 / (double)L2));
+      // correlation + per-pixel statistics (ComputeCCGSum / PickParticles in one pass)
+      for (int t0 = 0; t0 < nt; t0 += tc_max) {
+        const int tc = std::min(tc_max, nt - t0);
+        CorrArgs ca{that, ihat, Z, t0, tc, nb_};
+        clock_begin(clk_col_);
+        GS_REQUIRE(launch_colpass(l, ca, stream_), kUnsupported, "window_size not supported");
+        clock_end(clk_col_);
+        clock_begin(clk_row_);
+        launch_rowpass_pixel(l, Z, tc, nb_, d_S_.p, d_SS_.p, d_best_.p, d_arg_.p, a * n + t0, stream_);
+        clock_end(clk_row_);
+      }
+    }
+    GS_HIP(hipEventRecord(ev_corr1_, stream_));
+    // set_CCG_mean: n = N_t * int(360/phistep) (src/norm.cu:549-550)
+    double total_n = (double)n * (int)(360 / P_.phistep);
+    if (lim_angles_ > 0 || lim_templates_ > 0) total_n = (double)nt * na;
+    const int max_cand = 1 << 20;
+    Cand* cand = d_cand_.ensure(max_cand);
+    int* count = d_count_.ensure(1);
+    GS_HIP(hipMemsetAsync(count, 0, sizeof(int), stream_));
+    launch_finalize_norm(d_S_.p, d_SS_.p, d_best_.p, d_arg_.p, nb_, l, total_n, P_.threshold, doffx,
+                         doffy, bx_, W_, H_, P_.diameter, d_mu_.ensure(npix), d_sigma_.ensure(npix),
+                         cand, count, max_cand, stream_);
+  }
+
+  void collect_norm() {
+    int count = 0;
+    GS_HIP(hipMemcpy(&count, d_count_.p, sizeof(int), hipMemcpyDeviceToHost));
+    GS_REQUIRE(count <= (1 << 20), kUnsupported, "too many candidates above threshold");
+    std::vector<Cand> c(count);
+    if (count) GS_HIP(hipMemcpy(c.data(), d_cand_.p, sizeof(Cand) * count, hipMemcpyDeviceToHost));
+    const int l = P_.window_size;
+    records_.reserve(count);
+    for (const Cand& k : c) {
+      gs_record r{};
+      r.x = k.x;
+      r.y = k.y;
+      r.score = k.score;
+      r.template_id = k.arg % n_;
+      r.angle_index = k.arg / n_;
+      float e = 360.0f - angles_[r.angle_index];  // src/norm.cu:654-655
+      if (e >= 360.0f) e -= 360.0f;
+      r.psi = e;
+      r.block = k.block;
+      r.pixel = k.pixel;
+      r.local_x = (float)(k.pixel % l);
+      r.local_y = (float)(k.pixel / l);
+      records_.push_back(r);
+    }
+  }
+
+  // norm_type = 0 (SearchNoNorm::work, src/nonorm.cu:559-589).
+  void run_nonorm() {
+    const int l = P_.window_size, hx = l / 2 + 1, n = n_;
+    const size_t L2 = (size_t)l * l;
+    prep_templates();  // -> d_that_ [n][hx][l]
+    ulx_.clear();
+    uly_.clear();
+    tile_geometry(W_, H_, ulx_, uly_);
+    const int ntiles = (int)ulx_.size();
+    nbk_ = (int)block_offsets(kTile, l, P_.overlap).size();
+    const int nbb = nbk_ * nbk_;
+    std::vector<int> off(nbk_);
+    for (int i = 0; i < nbk_; ++i) off[i] = i * (l - P_.overlap);  // split_IMG (k:615-620)
+    int* doff = d_offx_.ensure(nbk_);
+    GS_HIP(hipMemcpyAsync(doff, off.data(), sizeof(int) * nbk_, hipMemcpyHostToDevice, stream_));
+    auto angles = angle_list(P_.phistep);
+    int na = (int)angles.size();
+    if (lim_angles_ > 0) na = std::min(na, lim_angles_);
+    angles_ = angles;
+    na_ = na;
+    const int groups = rowpass_groups(l);
+    const int tc_max = std::min(n, 64);
+    float2* Z = d_Z_.ensure((size_t)tc_max * nbb * l * (l / 2));
+    float4* part = reinterpret_cast<float4*>(d_plane_part_.ensure((size_t)tc_max * nbb * groups * 4));
+    const size_t per_angle = (size_t)n * nbb * 4;
+    double* planes = d_planes_.ensure((size_t)ntiles * na * per_angle);
+    float* tile = d_tile_.ensure((size_t)kTile * kTile);
+    float* tpre = d_imgpre_.ensure((size_t)kTile * kTile);
+    float2* blk = d_blocks_.ensure((size_t)nbb * L2);
+    float2* rotb = d_work2_.ensure((size_t)nbb * L2);
+    float2* ihat = d_ihat_.ensure((size_t)nbb * hx * l);
+    double* m = d_m_.ensure(std::max(n, nbb));
+    const float border = P_.diameter / 4;
+    GS_HIP(hipEventRecord(ev_corr0_, stream_));
+    for (int tI = 0; tI < ntiles; ++tI) {
+      launch_extract_tile(d_img_.p, W_, ulx_[tI], uly_[tI], kTile, P_.invert ? -1.f : 1.f, tile, stream_);
+      const float* src = tile;
+      if (P_.phase_flip == 1) {
+        prep_frame(tile, kTile, kTile, false, tpre);
+        src = tpre;
+      }
+      launch_split_blocks(src, kTile, kTile, l, doff, doff, nbk_, nbb, blk, stream_);
+      if (P_.phase_flip == 1) {  // src/nonorm.cu:405-447
+        float2* S = d_scratch_.ensure((size_t)nbb * L2);
+        fft2_batched(fft_, blk, S, nbb, l, l, -1, (float)(1.0 / (double)L2), stream_);
+        launch_area_power_full(blk, nbb, l, m, stream_);
+        launch_scale_planes(blk, nbb, (long long)L2, m, 1.f, stream_);
+        fft2_batched(fft_, blk, S, nbb, l, l, +1, 1.f, stream_);
+      }
+      for (int a = 0; a < na; ++a) {
+        const float cose = (float)std::cos((double)angles[a] * kRefPi / 180);
+        const float sine = (float)std::sin((double)angles[a] * kRefPi / 180);
+        launch_rotate_complex(blk, nbb, l, cose, sine, rotb, stream_);
+        half_spectra(rotb, nbb, ihat, (float)(1.0 / (double)L2));
+        for (int t0 = 0; t0 < n; t0 += tc_max) {
+          const int tc = std::min(tc_max, n - t0);
+          CorrArgs ca{d_that_.p, ihat, Z, t0, tc, nbb};
+          clock_begin(clk_col_);
+          GS_REQUIRE(launch_colpass(l, ca, stream_), kUnsupported, "window_size not supported");
+          clock_end(clk_col_);
+          clock_begin(clk_row_);
+          launch_rowpass_plane(l, Z, tc, nbb, part, border, stream_);
+          clock_end(clk_row_);
+          // results for templates t0.. of this (tile, angle): layout [tile][angle][t][b][4]
+          launch_finalize_plane(part, tc, nbb, groups,
+                                planes + ((size_t)tI * na + a) * per_angle + (size_t)t0 * nbb * 4, stream_);
+        }
+      }
+    }
+    GS_HIP(hipEventRecord(ev_corr1_, stream_));
+    planes_host_.resize((size_t)ntiles * na * per_angle);
+  }
+
+  void collect_nonorm() {
+    GS_HIP(hipMemcpy(planes_host_.data(), d_planes_.p, sizeof(double) * planes_host_.size(),
+                     hipMemcpyDeviceToHost));
+    const int l = P_.window_size, n = n_, nbb = nbk_ * nbk_;
+    const size_t per_angle = (size_t)n * nbb * 4;
+    const double rc = (double)l * l - 1;
+    for (int tI = 0; tI < (int)ulx_.size(); ++tI)
+      for (int a = 0; a < na_; ++a) {
+        const double e = angles_[a];
+        const double ce = std::cos(e * kRefPi / 180), se = std::sin(e * kRefPi / 180);
+        const double* P = planes_host_.data() + ((size_t)tI * na_ + a) * per_angle;
+        for (int t = 0; t < n; ++t) {
+          double sc = 0, cxs = 0, cys = 0;
+          int pix = 0, blkid = 0;
+          for (int b = 0; b < nbb; ++b) {  // PickParticles host scoring, src/nonorm.cu:509-534
+            const double* q = P + ((size_t)t * nbb + b) * 4;
+            const double peak = q[0], pos = q[1], sum = q[2], sum2 = q[3];
+            const double ra = sum - peak, rb = sum2 - peak * peak;
+            const double sd = std::sqrt(rb / rc - (ra / rc) * (ra / rc));
+            const double score = sd == 0 ? 0.0 : peak / sd;
+            const int cx = (int)pos % l, cy = (int)pos / l;
+            const int bi = b % nbk_, bj = b / nbk_;
+            const float centerx = (float)(bi * (l - P_.overlap) + (cx - l / 2) * ce + (cy - l / 2) * se + l / 2);
+            const float centery = (float)(bj * (l - P_.overlap) + (cy - l / 2) * ce - (cx - l / 2) * se + l / 2);
+            if (sc < score && centerx >= 0 && centerx < kTile && centery >= 0 && centery < kTile) {
+              sc = score;
+              cxs = centerx;
+              cys = centery;
+              pix = (int)pos;
+              blkid = b;
+            }
+          }
+          if (sc >= P_.threshold) {  // OutputScore, src/nonorm.cu:539-557
+            gs_record r{};
+            r.local_x = (float)cxs;
+            r.local_y = (float)cys;
+            r.x = (float)(ulx_[tI] + cxs);
+            r.y = (float)(uly_[tI] + cys);
+            r.template_id = t;
+            r.angle_index = a;
+            r.psi = angles_[a];
+            r.score = (float)sc;
+            r.block = tI;
+            r.pixel = blkid * (int)(l * l) + pix;
+            records_.push_back(r);
+          }
+        }
+      }
+  }
+
+  gs_params P_;
+  int device_;
+  hipStream_t stream_ = nullptr;
+  hipEvent_t ev_start_ = nullptr, ev_corr0_ = nullptr, ev_corr1_ = nullptr, ev_end_ = nullptr;
+  KernelClock clk_col_, clk_row_;
+  bool profile_ = false;
+  FftCache fft_;
+  DevParams dp_{};
+  int n_ = 0, w_ = 0, h_ = 0;
+  std::vector<float> alt_, az_;
+  int W_ = 0, H_ = 0;
+  gs_ctf ctf_{};
+  bool have_image_ = false, ran_ = false;
+  std::vector<float> host_img_;
+  int nb_ = 0, bx_ = 0, nbk_ = 0, na_ = 0;
+  std::vector<int> offx_, offy_, ulx_, uly_;
+  std::vector<float> angles_;
+  int lim_angles_ = 0, lim_templates_ = 0, chunk_ = 0;
+  double total_ms_ = 0, corr_ms_ = 0;
+  std::vector<gs_record> records_;
+  std::vector<double> planes_host_;
+  std::map<long long, std::unique_ptr<DevBuf<double>>> rb_cache_;
+
+  DevBuf<float> d_raw_, d_img_, d_imgpre_, d_tprep_, d_rot_, d_stats_, d_best_, d_tile_;
+  DevBuf<float2> d_work_, d_work2_, d_scratch_, d_frame_, d_frame2_, d_blocks_, d_ihat_, d_that_, d_Z_;
+  DevBuf<double> d_mom_, d_ra_, d_ra_img_, d_m_, d_part_, d_S_, d_SS_, d_mu_, d_sigma_, d_planes_;
+  DevBuf<float> d_plane_part_;
+  DevBuf<int32_t> d_arg_;
+  DevBuf<int> d_offx_, d_offy_, d_count_;
+  DevBuf<Cand> d_cand_;
+};
+
+}  // namespace gs
+
+// ------------------------------------------------------------------ C ABI
+struct gs_engine {
+  std::unique_ptr<gs::Engine> impl;
+};
+
+namespace {
+thread_local std::string g_last_error;
+
+gs_status fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return (gs_status)code;
+}
+
+template <typename F>
+gs_status guarded(F&& f) {
+  try {
+    f();
+    g_last_error.clear();
+    return GS_OK;
+  } catch (const gs::Error& e) {
+    return fail(e.code, e.what());
+  } catch (const std::bad_alloc&) {
+    return fail(GS_ERR_NOMEM, "host allocation failed");
+  } catch (const std::exception& e) {
+    return fail(GS_ERR_INVALID, e.what());
+  } catch (...) {
+    return fail(GS_ERR_INVALID, "unknown error");
+  }
+}
+}  // namespace
+
+extern "C" {
+
+gs_status gs_create(const gs_params* params, int device, gs_engine** out) {
+  if (!params || !out) return fail(GS_ERR_INVALID, "null argument");
+  return guarded([&] {
+    auto e = std::make_unique<gs_engine>();
+    e->impl = std::make_unique<gs::Engine>(*params, device);
+    *out = e.release();
+  });
+}
+
+gs_status gs_load_templates(gs_engine* e, const float* data, int n, int w, int h, const float* alt_az) {
+  if (!e) return fail(GS_ERR_INVALID, "null engine");
+  return guarded([&] { e->impl->load_templates(data, n, w, h, alt_az); });
+}
+
+gs_status gs_set_image(gs_engine* e, const float* image, int width, int height, const gs_ctf* ctf) {
+  if (!e || !ctf) return fail(GS_ERR_INVALID, "null argument");
+  return guarded([&] { e->impl->set_image(image, width, height, *ctf); });
+}
+
+gs_status gs_run(gs_engine* e) {
+  if (!e) return fail(GS_ERR_INVALID, "null engine");
+  return guarded([&] { e->impl->run(); });
+}
+
+gs_status gs_get_records(gs_engine* e, gs_record** out, size_t* n_out) {
+  if (!e || !out || !n_out) return fail(GS_ERR_INVALID, "null argument");
+  return guarded([&] {
+    const auto& r = e->impl->records();
+    gs_record* p = (gs_record*)std::malloc(sizeof(gs_record) * std::max<size_t>(r.size(), 1));
+    if (!p) throw std::bad_alloc();
+    if (!r.empty()) std::memcpy(p, r.data(), sizeof(gs_record) * r.size());
+    *out = p;
+    *n_out = r.size();
+  });
+}
+
+gs_status gs_search(gs_engine* e, const float* image, int width, int height, const gs_ctf* ctf,
+                    gs_record** out, size_t* n_out) {
+  gs_status s = gs_set_image(e, image, width, height, ctf);
+  if (s != GS_OK) return s;
+  s = gs_run(e);
+  if (s != GS_OK) return s;
+  return gs_get_records(e, out, n_out);
+}
+
+int gs_format_record(const gs_engine* e, const gs_record* r, int image_index, const char* path,
+                     const gs_ctf* ctf, int tile_local, char* buf, size_t cap) {
+  if (!e || !r || !ctf) return -1;
+  const float cx = tile_local ? r->local_x : r->x, cy = tile_local ? r->local_y : r->y;
+  // float fields as the reference stores them (Parameters is float, src/norm.cu:185-187)
+  return std::snprintf(buf, cap,
+                       "%d\t%s\tdefocus=%f\tdfdiff=%f\tdfang=%f\teuler=%f,%f,%f\tcenter=%f,%f\tscore=%f\n",
+                       image_index, path ? path : "", (float)ctf->defocus, (float)ctf->dfdiff,
+                       (float)ctf->dfang, e->impl->alt(r->template_id), e->impl->az(r->template_id),
+                       r->psi, cx, cy, r->score);
+}
+
+void gs_free(void* p) { std::free(p); }
+
+void gs_destroy(gs_engine* e) {
+  if (!e) return;
+  try {
+    delete e;
+  } catch (...) {
+  }
+}
+
+const char* gs_last_error(void) { return g_last_error.c_str(); }
+
+size_t gs_stage_bytes(const gs_engine* e, int stage) {
+  if (!e) return 0;
+  try {
+    return e->impl->stage_bytes(stage);
+  } catch (...) {
+    return 0;
+  }
+}
+
+gs_status gs_fetch_stage(gs_engine* e, int stage, void* out, size_t bytes) {
+  if (!e || !out) return fail(GS_ERR_INVALID, "null argument");
+  return guarded([&] { e->impl->fetch(stage, out, bytes); });
+}
+
+gs_status gs_set_limits(gs_engine* e, int n_angles, int n_templates) {
+  if (!e) return fail(GS_ERR_INVALID, "null engine");
+  return guarded([&] { e->impl->set_limits(n_angles, n_templates); });
+}
+
+gs_status gs_fft_c2c(int device, float* data, int n, int batch, int sign) {
+  if (!data || n < 1 || batch < 1 || (sign != 1 && sign != -1)) return fail(GS_ERR_INVALID, "bad FFT arguments");
+  return guarded([&] {
+    GS_HIP(hipSetDevice(device));
+    gs::FftCache cache;
+    const size_t N = (size_t)n * batch;
+    gs::DevBuf<float2> a, b;
+    a.ensure(N);
+    b.ensure(N);
+    GS_HIP(hipMemcpy(a.p, data, sizeof(float2) * N, hipMemcpyHostToDevice));
+    gs::fft_rows(cache, a.p, b.p, n, batch, sign, 1.f, nullptr);
+    GS_HIP(hipMemcpy(data, a.p, sizeof(float2) * N, hipMemcpyDeviceToHost));
+  });
+}
+
+gs_status gs_fft2_c2c(int device, float* data, int ny, int nx, int batch, int sign) {
+  if (!data || nx < 1 || ny < 1 || batch < 1 || (sign != 1 && sign != -1))
+    return fail(GS_ERR_INVALID, "bad FFT arguments");
+  return guarded([&] {
+    GS_HIP(hipSetDevice(device));
+    gs::FftCache cache;
+    const size_t N = (size_t)nx * ny * batch;
+    gs::DevBuf<float2> a, b;
+    a.ensure(N);
+    b.ensure(N);
+    GS_HIP(hipMemcpy(a.p, data, sizeof(float2) * N, hipMemcpyHostToDevice));
+    gs::fft2_batched(cache, a.p, b.p, batch, ny, nx, sign, 1.f, nullptr);
+    GS_HIP(hipMemcpy(data, a.p, sizeof(float2) * N, hipMemcpyDeviceToHost));
+  });
+}
+
+gs_status gs_last_timing(const gs_engine* e, double* total_ms, double* corr_ms) {
+  if (!e) return fail(GS_ERR_INVALID, "null engine");
+  e->impl->timing(total_ms, corr_ms);
+  return GS_OK;
+}
+
+
+gs_status gs_set_profiling(gs_engine* e, int on) {
+  if (!e) return fail(GS_ERR_INVALID, "null engine");
+  e->impl->set_profiling(on != 0);
+  return GS_OK;
+}
+
+gs_status gs_kernel_timing(const gs_engine* e, int which, double* avg_ms, long long* count) {
+  if (!e) return fail(GS_ERR_INVALID, "null engine");
+  e->impl->kernel_timing(which, avg_ms, count);
+  return GS_OK;
+}
+
+gs_status gs_set_chunk(gs_engine* e, int templates_per_launch) {
+  if (!e) return fail(GS_ERR_INVALID, "null engine");
+  e->impl->set_chunk(templates_per_launch);
+  return GS_OK;
+}
+
+}  // extern "C"

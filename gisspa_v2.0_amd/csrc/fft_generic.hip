@@ -1,0 +1,190 @@
+// Runtime-planned mixed-radix FFT (see fft_generic.hpp).
+#include <cmath>
+
+#include "common.hpp"
+#include "fft_generic.hpp"
+#include "fft_tile.hpp"
+
+namespace gs {
+
+FftPlanGpu::~FftPlanGpu() {
+  if (tw) (void)hipFree(tw);
+}
+
+static std::vector<int> factor_radices(int n) {
+  std::vector<int> f;
+  while (n % 4 == 0) { f.push_back(4); n /= 4; }
+  while (n % 2 == 0) { f.push_back(2); n /= 2; }
+  for (int p = 3; p * p <= n; p += 2)
+    while (n % p == 0) { f.push_back(p); n /= p; }
+  if (n > 1) f.push_back(n);
+  return f;
+}
+
+const FftPlanGpu& FftCache::plan(int n) {
+  auto it = plans_.find(n);
+  if (it != plans_.end()) return *it->second;
+  GS_REQUIRE(n >= 1, kInvalid, "FFT length must be positive");
+  auto p = std::make_unique<FftPlanGpu>();
+  p->n = n;
+  p->radices = factor_radices(n);
+  std::vector<float2> h(n);
+  for (int e = 0; e < n; ++e) {
+    const double a = 2.0 * M_PI * (double)e / (double)n;
+    h[e] = make_float2((float)std::cos(a), (float)-std::sin(a));
+  }
+  GS_HIP(hipMalloc(&p->tw, sizeof(float2) * n));
+  GS_HIP(hipMemcpy(p->tw, h.data(), sizeof(float2) * n, hipMemcpyHostToDevice));
+  auto& ref = *p;
+  plans_.emplace(n, std::move(p));
+  return ref;
+}
+
+template <int SIGN>
+__device__ __forceinline__ float2 twv(const float2* tw, long long e) {
+  float2 w = tw[e];
+  return SIGN < 0 ? w : make_float2(w.x, -w.y);
+}
+
+template <int R, int SIGN>
+__device__ __forceinline__ void stage_fixed(const float2* __restrict__ src, float2* __restrict__ dst,
+                                            const float2* __restrict__ tw, int n, int Ns, int j,
+                                            float scale) {
+  const int nb = n / R;
+  const int jm = j % Ns;
+  const int step = n / (Ns * R);
+  float2 v[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    float2 a = src[j + r * nb];
+    if (r > 0 && jm > 0) a = c_mul(a, twv<SIGN>(tw, (long long)r * jm * step));
+    v[r] = a;
+  }
+  ct::Dft<R, SIGN>::run(v);
+  const int idxD = (j / Ns) * Ns * R + jm;
+#pragma unroll
+  for (int r = 0; r < R; ++r) dst[idxD + r * Ns] = c_scale(v[r], scale);
+}
+
+template <int SIGN>
+__device__ __forceinline__ void stage_direct(const float2* __restrict__ src, float2* __restrict__ dst,
+                                             const float2* __restrict__ tw, int n, int R, int Ns,
+                                             int j, float scale) {
+  const int nb = n / R;
+  const int jm = j % Ns;
+  const int step = n / (Ns * R);
+  const int idxD = (j / Ns) * Ns * R + jm;
+  const int nR = n / R;  // table stride of W_R
+  for (int k = 0; k < R; ++k) {
+    float2 acc = make_float2(0.f, 0.f);
+    for (int r = 0; r < R; ++r) {
+      float2 a = src[j + r * nb];
+      if (r > 0 && jm > 0) a = c_mul(a, twv<SIGN>(tw, (long long)r * jm * step));
+      const float2 w = twv<SIGN>(tw, (long long)((r * k) % R) * nR);
+      acc = c_add(acc, c_mul(a, w));
+    }
+    dst[idxD + k * Ns] = c_scale(acc, scale);
+  }
+}
+
+template <int SIGN>
+__global__ void k_fft_stage(const float2* __restrict__ src, float2* __restrict__ dst,
+                            const float2* __restrict__ tw, int n, int R, int Ns, long long nrows,
+                            float scale) {
+  const int nb = n / R;
+  const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= nrows * nb) return;
+  const long long row = gid / nb;
+  const int j = (int)(gid - row * nb);
+  const float2* s = src + row * n;
+  float2* d = dst + row * n;
+  switch (R) {
+    case 2: stage_fixed<2, SIGN>(s, d, tw, n, Ns, j, scale); break;
+    case 3: stage_fixed<3, SIGN>(s, d, tw, n, Ns, j, scale); break;
+    case 4: stage_fixed<4, SIGN>(s, d, tw, n, Ns, j, scale); break;
+    case 5: stage_fixed<5, SIGN>(s, d, tw, n, Ns, j, scale); break;
+    case 7: stage_fixed<7, SIGN>(s, d, tw, n, Ns, j, scale); break;
+    default: stage_direct<SIGN>(s, d, tw, n, R, Ns, j, scale); break;
+  }
+}
+
+__global__ void k_copy_scale(const float2* __restrict__ src, float2* __restrict__ dst, long long n,
+                             float scale) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = c_scale(src[i], scale);
+}
+
+static inline unsigned nblk(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+void fft_rows(FftCache& cache, float2* data, float2* scratch, int n, long long nrows, int sign,
+              float scale, hipStream_t s) {
+  if (nrows <= 0) return;
+  const FftPlanGpu& P = cache.plan(n);
+  if (n == 1) {
+    if (scale != 1.f) {
+      k_copy_scale<<<nblk(nrows, 256), 256, 0, s>>>(data, data, nrows, scale);
+      GS_CHECK_LAUNCH();
+    }
+    return;
+  }
+  float2* src = data;
+  float2* dst = scratch;
+  int Ns = 1;
+  const int nst = (int)P.radices.size();
+  for (int st = 0; st < nst; ++st) {
+    const int R = P.radices[st];
+    const float sc = (st == nst - 1) ? scale : 1.f;
+    const long long work = nrows * (n / R);
+    if (sign < 0)
+      k_fft_stage<-1><<<nblk(work, 256), 256, 0, s>>>(src, dst, P.tw, n, R, Ns, nrows, sc);
+    else
+      k_fft_stage<+1><<<nblk(work, 256), 256, 0, s>>>(src, dst, P.tw, n, R, Ns, nrows, sc);
+    GS_CHECK_LAUNCH();
+    std::swap(src, dst);
+    Ns *= R;
+  }
+  if (src != data) {
+    k_copy_scale<<<nblk((long long)n * nrows, 256), 256, 0, s>>>(src, data, (long long)n * nrows, 1.f);
+    GS_CHECK_LAUNCH();
+  }
+}
+
+// 32x32 tiles through LDS; grid (ceil(cols/32), ceil(rows/32), batch).
+__global__ void k_transpose(const float2* __restrict__ in, float2* __restrict__ out, int rows,
+                            int cols, int out_rows) {
+  __shared__ float2 tile[32][33];
+  const long long b = blockIdx.z;
+  const float2* src = in + b * (long long)rows * cols;
+  float2* dst = out + b * (long long)out_rows * rows;
+  const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
+  const int tx = threadIdx.x, ty = threadIdx.y;  // 32 x 8
+  for (int k = 0; k < 32; k += 8) {
+    const int r = r0 + ty + k, c = c0 + tx;
+    if (r < rows && c < cols) tile[ty + k][tx] = src[(long long)r * cols + c];
+  }
+  __syncthreads();
+  for (int k = 0; k < 32; k += 8) {
+    const int c = c0 + ty + k, r = r0 + tx;  // output row c, column r
+    if (c < out_rows && c < cols && r < rows) dst[(long long)c * rows + r] = tile[tx][ty + k];
+  }
+}
+
+void transpose_batched(const float2* in, float2* out, int batch, int rows, int cols, int out_rows,
+                       hipStream_t s) {
+  if (batch <= 0) return;
+  dim3 grid((unsigned)((std::min(cols, out_rows) + 31) / 32), (unsigned)((rows + 31) / 32),
+            (unsigned)batch);
+  k_transpose<<<grid, dim3(32, 8), 0, s>>>(in, out, rows, cols, out_rows);
+  GS_CHECK_LAUNCH();
+}
+
+void fft2_batched(FftCache& cache, float2* data, float2* scratch, int batch, int ny, int nx,
+                  int sign, float scale, hipStream_t s) {
+  // rows (length nx), transpose to [nx][ny], rows (length ny), transpose back
+  fft_rows(cache, data, scratch, nx, (long long)batch * ny, sign, 1.f, s);
+  transpose_batched(data, scratch, batch, ny, nx, nx, s);
+  fft_rows(cache, scratch, data, ny, (long long)batch * nx, sign, scale, s);
+  transpose_batched(scratch, data, batch, nx, ny, ny, s);
+}
+
+}  // namespace gs

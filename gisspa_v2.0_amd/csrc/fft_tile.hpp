@@ -1,0 +1,291 @@
+// Compile-time FFT building blocks for the l x l correlation transforms.
+//
+// * dft<R, SIGN>: length-R DFT held in registers (explicit 2/3/4/5, split composite radices
+//   R = A*B by an in-register Cooley-Tukey step, direct sums for other primes).  Twiddles are
+//   constexpr values, so every butterfly compiles to straight-line FMA code.
+// * stockham<N, T, SIGN, R...>: self-sorting Stockham stages over an LDS-resident transform
+//   of length N shared by T threads (Govindaraju et al., SC'08).  Each stage reads all of its
+//   inputs into registers, synchronises, writes its outputs in place and synchronises again, so
+//   one LDS buffer per transform suffices.  Stages whose butterfly count divides T compile
+//   without predication.
+// SIGN = -1 is the forward transform exp(-2 pi i k n / N); +1 the unnormalised inverse.
+#pragma once
+#include "common.hpp"
+
+namespace gs {
+namespace ct {
+
+// constexpr sin/cos of 2*pi*num/den for exact small-angle twiddles.
+constexpr double kPi = 3.14159265358979323846264338327950288;
+constexpr double cx_sin_red(double x) {  // |x| <= pi/4
+  double term = x, sum = x;
+  for (int i = 1; i < 14; ++i) {
+    term *= -x * x / ((2.0 * i) * (2.0 * i + 1.0));
+    sum += term;
+  }
+  return sum;
+}
+constexpr double cx_cos_red(double x) {
+  double term = 1.0, sum = 1.0;
+  for (int i = 1; i < 14; ++i) {
+    term *= -x * x / ((2.0 * i - 1.0) * (2.0 * i));
+    sum += term;
+  }
+  return sum;
+}
+// angle = 2 pi num/den; exact octant reduction on the rational argument.
+constexpr void cx_sincos_frac(long long num, long long den, double& s, double& c) {
+  num %= den;
+  if (num < 0) num += den;
+  // work in units of den/8 octants
+  const long long n8 = num * 8;
+  const long long oct = n8 / den;             // 0..7
+  const double rem = (double)(n8 - oct * den) / (double)den;  // [0,1) of an octant
+  double x = rem * (kPi / 4);               // [0, pi/4)
+  double sx = cx_sin_red(x), cxv = cx_cos_red(x);
+  // angle = oct*pi/4 + x
+  double so = 0, co = 1;
+  switch (oct) {
+    case 0: so = sx; co = cxv; break;
+    case 1: { double y = kPi / 4 - x; so = cx_cos_red(y); co = cx_sin_red(y); break; }
+    case 2: so = cxv; co = -sx; break;
+    case 3: { double y = kPi / 4 - x; so = cx_sin_red(y); co = -cx_cos_red(y); break; }
+    case 4: so = -sx; co = -cxv; break;
+    case 5: { double y = kPi / 4 - x; so = -cx_cos_red(y); co = -cx_sin_red(y); break; }
+    case 6: so = -cxv; co = sx; break;
+    default: { double y = kPi / 4 - x; so = -cx_sin_red(y); co = cx_cos_red(y); break; }
+  }
+  s = so;
+  c = co;
+}
+constexpr double cx_cos_frac(long long num, long long den) {
+  double s = 0, c = 0;
+  cx_sincos_frac(num, den, s, c);
+  return c;
+}
+constexpr double cx_sin_frac(long long num, long long den) {
+  double s = 0, c = 0;
+  cx_sincos_frac(num, den, s, c);
+  return s;
+}
+
+// W_R^k with sign: exp(SIGN * 2 pi i k / R)
+template <int R, int K, int SIGN>
+struct Tw {
+  static constexpr float re = (float)cx_cos_frac(K, R);
+  static constexpr float im = (float)(SIGN * cx_sin_frac(K, R));
+};
+
+template <int R, int K, int SIGN>
+__device__ __forceinline__ float2 twmul(float2 a) {
+  constexpr int k = ((K % R) + R) % R;
+  if constexpr (k == 0) {
+    return a;
+  } else if constexpr (4 * k == R) {  // SIGN*i
+    return c_muli(a, SIGN);
+  } else if constexpr (2 * k == R) {
+    return make_float2(-a.x, -a.y);
+  } else if constexpr (4 * k == 3 * R) {
+    return c_muli(a, -SIGN);
+  } else {
+    constexpr float wr = Tw<R, k, SIGN>::re, wi = Tw<R, k, SIGN>::im;
+    return make_float2(a.x * wr - a.y * wi, a.x * wi + a.y * wr);
+  }
+}
+
+constexpr int smallest_factor(int n) {
+  for (int p = 2; p * p <= n; ++p)
+    if (n % p == 0) return p;
+  return n;
+}
+
+template <int R, int SIGN>
+struct Dft;
+
+template <int SIGN>
+struct Dft<1, SIGN> {
+  __device__ __forceinline__ static void run(float2*) {}
+};
+
+template <int SIGN>
+struct Dft<2, SIGN> {
+  __device__ __forceinline__ static void run(float2* v) {
+    float2 a = v[0], b = v[1];
+    v[0] = c_add(a, b);
+    v[1] = c_sub(a, b);
+  }
+};
+
+template <int SIGN>
+struct Dft<3, SIGN> {
+  __device__ __forceinline__ static void run(float2* v) {
+    constexpr float c = -0.5f;
+    constexpr float s = (float)(SIGN * 0.86602540378443864676);
+    float2 t1 = c_add(v[1], v[2]), t2 = c_sub(v[1], v[2]);
+    float2 m1 = make_float2(v[0].x + c * t1.x, v[0].y + c * t1.y);
+    float2 m2 = make_float2(-s * t2.y, s * t2.x);
+    v[0] = c_add(v[0], t1);
+    v[1] = c_add(m1, m2);
+    v[2] = c_sub(m1, m2);
+  }
+};
+
+template <int SIGN>
+struct Dft<4, SIGN> {
+  __device__ __forceinline__ static void run(float2* v) {
+    float2 a0 = c_add(v[0], v[2]), a1 = c_sub(v[0], v[2]);
+    float2 b0 = c_add(v[1], v[3]), b1 = c_muli(c_sub(v[1], v[3]), SIGN);
+    v[0] = c_add(a0, b0);
+    v[2] = c_sub(a0, b0);
+    v[1] = c_add(a1, b1);
+    v[3] = c_sub(a1, b1);
+  }
+};
+
+template <int SIGN>
+struct Dft<5, SIGN> {
+  __device__ __forceinline__ static void run(float2* v) {
+    constexpr float c1 = 0.30901699437494742410f, c2 = -0.80901699437494742410f;
+    constexpr float s1 = (float)(SIGN * 0.95105651629515357212);
+    constexpr float s2 = (float)(SIGN * 0.58778525229247312917);
+    float2 a = c_add(v[1], v[4]), b = c_sub(v[1], v[4]);
+    float2 c = c_add(v[2], v[3]), d = c_sub(v[2], v[3]);
+    float2 x0 = v[0];
+    float2 p1 = make_float2(x0.x + c1 * a.x + c2 * c.x, x0.y + c1 * a.y + c2 * c.y);
+    float2 p2 = make_float2(x0.x + c2 * a.x + c1 * c.x, x0.y + c2 * a.y + c1 * c.y);
+    // q1 = s1*b + s2*d ; q2 = s2*b - s1*d  (then multiplied by i)
+    float2 q1 = make_float2(s1 * b.x + s2 * d.x, s1 * b.y + s2 * d.y);
+    float2 q2 = make_float2(s2 * b.x - s1 * d.x, s2 * b.y - s1 * d.y);
+    v[0] = make_float2(x0.x + a.x + c.x, x0.y + a.y + c.y);
+    v[1] = make_float2(p1.x - q1.y, p1.y + q1.x);
+    v[4] = make_float2(p1.x + q1.y, p1.y - q1.x);
+    v[2] = make_float2(p2.x - q2.y, p2.y + q2.x);
+    v[3] = make_float2(p2.x + q2.y, p2.y - q2.x);
+  }
+};
+
+// Composite R = A * B (A = smallest factor) or a prime handled by direct summation.
+template <int R, int SIGN>
+struct Dft {
+  static constexpr int A = smallest_factor(R);
+  static constexpr int B = R / A;
+
+  template <int K, int N>
+  __device__ __forceinline__ static void direct_acc(const float2* v, float2& acc) {
+    if constexpr (N < R) {
+      float2 t = twmul<R, (K * N) % R, SIGN>(v[N]);
+      acc.x += t.x;
+      acc.y += t.y;
+      direct_acc<K, N + 1>(v, acc);
+    }
+  }
+  template <int K>
+  __device__ __forceinline__ static void direct_all(const float2* v, float2* o) {
+    if constexpr (K < R) {
+      float2 acc = make_float2(0.f, 0.f);
+      direct_acc<K, 0>(v, acc);
+      o[K] = acc;
+      direct_all<K + 1>(v, o);
+    }
+  }
+  template <int a, int k2>
+  __device__ __forceinline__ static void twiddle_grid(float2 (&y)[A][B]) {
+    if constexpr (a < A) {
+      if constexpr (k2 < B) {
+        y[a][k2] = twmul<R, a * k2, SIGN>(y[a][k2]);
+        twiddle_grid<a, k2 + 1>(y);
+      } else {
+        twiddle_grid<a + 1, 0>(y);
+      }
+    }
+  }
+
+  __device__ __forceinline__ static void run(float2* v) {
+    if constexpr (A == R) {  // prime
+      float2 o[R];
+      direct_all<0>(v, o);
+#pragma unroll
+      for (int k = 0; k < R; ++k) v[k] = o[k];
+    } else {
+      // X[k2 + B k1] = sum_a W_A^{a k1} W_R^{a k2} sum_b v[a + A b] W_B^{b k2}
+      float2 y[A][B];
+#pragma unroll
+      for (int a = 0; a < A; ++a) {
+#pragma unroll
+        for (int b = 0; b < B; ++b) y[a][b] = v[a + A * b];
+        Dft<B, SIGN>::run(y[a]);
+      }
+      twiddle_grid<0, 0>(y);
+#pragma unroll
+      for (int k2 = 0; k2 < B; ++k2) {
+        float2 z[A];
+#pragma unroll
+        for (int a = 0; a < A; ++a) z[a] = y[a][k2];
+        Dft<A, SIGN>::run(z);
+#pragma unroll
+        for (int k1 = 0; k1 < A; ++k1) v[k2 + B * k1] = z[k1];
+      }
+    }
+  }
+};
+
+// Twiddle for stage inputs: exp(SIGN 2 pi i e / N) from an LDS table tw[e] = exp(-2 pi i e/N).
+template <int SIGN>
+__device__ __forceinline__ float2 tw_lookup(const float2* tw, int e) {
+  float2 w = tw[e];
+  return SIGN < 0 ? w : make_float2(w.x, -w.y);
+}
+
+// One Stockham stage (radix R, NS = product of earlier radices) over a length-N transform in
+// buf, shared by T threads (tid in [0,T)); then recurse into the remaining radices.
+template <int N, int T, int SIGN, int NS, int R, int... REST>
+__device__ __forceinline__ void stockham(float2* buf, const float2* tw, int tid) {
+  constexpr int NB = N / R;
+  constexpr int IT = (NB + T - 1) / T;
+  constexpr bool even = (NB % T) == 0;
+  float2 v[IT][R];
+#pragma unroll
+  for (int it = 0; it < IT; ++it) {
+    const int j = tid + it * T;
+    if (even || j < NB) {
+      const int jm = j % NS;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        float2 a = buf[j + r * NB];
+        if constexpr (NS > 1) {
+          if (r > 0) a = c_mul(a, tw_lookup<SIGN>(tw, r * jm * (N / (NS * R))));
+        }
+        v[it][r] = a;
+      }
+      Dft<R, SIGN>::run(v[it]);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int it = 0; it < IT; ++it) {
+    const int j = tid + it * T;
+    if (even || j < NB) {
+      const int jm = j % NS;
+      const int idxD = (j / NS) * NS * R + jm;
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[idxD + r * NS] = v[it][r];
+    }
+  }
+  __syncthreads();
+  if constexpr (sizeof...(REST) > 0) stockham<N, T, SIGN, NS * R, REST...>(buf, tw, tid);
+}
+
+template <int... RS>
+struct RadixList {
+  static constexpr int product() { return (RS * ... * 1); }
+};
+
+template <int N, int T, int SIGN, int... RS>
+__device__ __forceinline__ void run_stockham(float2* buf, const float2* tw, int tid,
+                                             RadixList<RS...>) {
+  static_assert(RadixList<RS...>::product() == N, "radix plan must multiply to N");
+  stockham<N, T, SIGN, 1, RS...>(buf, tw, tid);
+}
+
+}  // namespace ct
+}  // namespace gs

@@ -1,0 +1,96 @@
+// Device parameter block and kernel launchers of the GisSPA pipeline.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace gs {
+
+// Per-micrograph parameters (reference Parameters, include/EMReader/DataReader2.h:73-83, with
+// SetParams, src/norm.cu:183-192).  CTF quantities are kept in double: the CTF phase reaches
+// tens of radians on whole frames, and these kernels run once per micrograph.
+struct DevParams {
+  float apix, kk, highres, lowres, d_m, thresh;
+  double lambda, dfu, dfv, dfdiff, dfang, cs, ampconst;
+  float ehw, a, b, b2, bf, bf2, bf3;
+  // Band-pass radii and thresholds: float expressions of config values in the reference
+  // (k:179,183-189,265-273), evaluated once on the host.
+  float hr_t, lr_t;          // template band at l
+  float hr_i, lr_i, hi_thr;  // whole-frame band at max(nx, ny)
+  float ds;                  // 1/(apix*l), src/norm.cu:191
+};
+
+// Candidate pick written by the finalise kernel (norm_type=1).
+struct Cand {
+  float x, y, score;
+  int32_t arg;    // angle_index * n_templates + template
+  int32_t block;  // block index
+  int32_t pixel;  // in-block pixel index
+};
+
+// ---- template preparation (src/norm.cu:194-354, src/nonorm.cu:178-344) ----
+void launch_pad_templates(const float* raw, int n, int w, int h, int l, float2* out, hipStream_t s);
+void launch_template_moments(const float2* T, int n, int l, double* mom, hipStream_t s);
+void launch_template_standardise(float2* T, int n, int l, const double* mom, hipStream_t s);
+void launch_radial_power_planes(const float2* F, int n, int l, double* ra, hipStream_t s);
+void launch_whiten_planes(float2* F, int n, int l, const double* ra, const double* rb, hipStream_t s);
+void launch_weight_planes(float2* F, int n, int l, const DevParams& p, hipStream_t s);
+void launch_mask_planes(float2* T, int n, int l, float d_m, float ehw, hipStream_t s);
+void launch_area_power_full(const float2* F, int n, int l, double* m, hipStream_t s);
+void launch_area_power_half(const float2* F, int n, int l, double* m, hipStream_t s);
+void launch_scale_planes(float2* F, int n, long long plane, const double* m, float extra,
+                         hipStream_t s);
+void launch_extract_real(const float2* F, float* out, long long count, hipStream_t s);
+
+// ---- whole-frame preparation (src/norm.cu:356-402, src/nonorm.cu:346-389) ----
+void launch_real_to_complex(const float* in, float2* out, long long count, float sign, hipStream_t s);
+void launch_phase_flip(float2* F, int nx, int ny, const DevParams& p, hipStream_t s);
+void launch_radial_power_image(const float2* F, int nx, int ny, double* ra, hipStream_t s);
+void launch_whiten_filter_image(float2* F, int nx, int ny, const double* ra, const double* rb,
+                                const DevParams& p, hipStream_t s);
+void launch_zero_dc(float2* F, hipStream_t s);
+void launch_extract_tile(const float* img, int W, int ulx, int uly, int tile, float sign,
+                         float* out, hipStream_t s);
+
+// ---- blocks (src/norm.cu:404-477, src/nonorm.cu:391-450) ----
+void launch_split_blocks(const float* img, int nx, int ny, int l, const int* offx, const int* offy,
+                         int bx, int nb, float2* out, hipStream_t s);
+
+// ---- per-angle rotation (src/norm.cu:479-521, src/nonorm.cu:457) ----
+void launch_rotate_mask(const float* tp, int n, int l, float cose, float sine, float d_m, float ehw,
+                        float* out, double* partial, hipStream_t s);
+void launch_rotation_stats(const double* partial, int n, int l, float* stats, hipStream_t s);
+void launch_standardise_to_complex(const float* R, int n, int l, const float* stats, float2* out,
+                                   hipStream_t s);
+void launch_rotate_complex(const float2* in, int n, int l, float cose, float sine, float2* out,
+                           hipStream_t s);
+
+// ---- correlation (north-star kernels b, c, d, e) ----
+struct CorrArgs {
+  const float2* that;  // [>=t0+tc][H+1][L] template spectra
+  const float2* ihat;  // [nb][H+1][L] image spectra
+  float2* Z;           // [tc][nb][L][H] inter-pass buffer
+  int t0, tc, nb;
+};
+// Column pass: packed conj-multiply + inverse column FFTs into Z (returns false if L unsupported).
+bool launch_colpass(int L, const CorrArgs& a, hipStream_t s);
+// Row pass, norm_type=1: C2R rows + per-(block,pixel) running moments and best record.
+bool launch_rowpass_pixel(int L, const float2* Z, int tc, int nb, double* S, double* SS,
+                          float* best, int32_t* arg, int arg_base, hipStream_t s);
+// Row pass, norm_type=0: C2R rows + per-(template,block,row group) peak/sum partials.
+bool launch_rowpass_plane(int L, const float2* Z, int tc, int nb, float4* partial, float border,
+                          hipStream_t s);
+int rowpass_groups(int L);
+bool corr_supported(int L);
+// Reduce plane partials to (peak, index, sum, sum2) per (template, block).
+void launch_finalize_plane(const float4* partial, int tc, int nb, int groups, double* out,
+                           hipStream_t s);
+void launch_init_accumulators(double* S, double* SS, float* best, int32_t* arg, long long count,
+                              hipStream_t s);
+// Per block pixel: mu, sigma, z of the best record; candidates above threshold.
+void launch_finalize_norm(const double* S, const double* SS, const float* best, const int32_t* arg,
+                          int nb, int l, double total_n, float thresh, const int* offx,
+                          const int* offy, int bx, int W, int H, float d_m, double* mu,
+                          double* sigma, Cand* cand, int* count, int max_cand, hipStream_t s);
+
+}  // namespace gs

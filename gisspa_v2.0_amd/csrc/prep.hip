@@ -1,0 +1,577 @@
+// Preprocessing kernels: templates, whole frame, blocks and per-angle rotation.
+//
+// These stages run once per micrograph (or once per angle on the template side) and are small
+// next to the correlation.  Reductions are deterministic (fixed-order double accumulation)
+// except the radial power spectra, which use double-precision LDS atomics.
+#include <cmath>
+
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace gs {
+
+namespace {
+
+constexpr double kRefPi = 3.1415926535;  // include/utils/constants.h:3
+constexpr int BS = 256;
+
+inline unsigned nblk(long long n, int bs = BS) { return (unsigned)((n + bs - 1) / bs); }
+
+// floor(hypot(dx,dy)+0.5)-1 evaluated exactly on the integer squared radius.
+__device__ __forceinline__ int rbin(int dx, int dy) {
+  const int n = dx * dx + dy * dy;
+  int k = (int)sqrtf((float)n);
+  while (k * k > n) --k;
+  while ((k + 1) * (k + 1) <= n) ++k;
+  if (n > k * k + k) ++k;
+  return k - 1;
+}
+
+// CTF_AST (src/kernels.cu:290-317), mode 0 sign / 2 magnitude / else signed.
+__device__ double ctf_ast(int x1, int y1, int nx, int ny, const DevParams& p, int mode) {
+  y1 = x1 > nx / 2 ? ny - y1 : y1;
+  x1 = min(x1, nx - x1);
+  const double dx = min(x1, nx - x1);
+  const double dy = y1 - ny / 2;
+  const double ss = (dx * dx / (double)(nx * nx) + dy * dy / (double)(ny * ny)) /
+                    ((double)p.apix * (double)p.apix);
+  const double ag = atan2((double)(y1 - ny / 2), (double)x1);
+  const double df_ast = 0.5 * (p.dfu + p.dfv + 2 * p.dfdiff * cos(2 * (p.dfang * kRefPi / 180 - ag)));
+  const double lam = p.lambda;
+  const double gamma = -2 * kRefPi * (p.cs * 2.5e6 * lam * lam * lam * ss * ss + df_ast * 5000.0 * lam * ss);
+  const double v = sqrt(1.0 - p.ampconst * p.ampconst) * sin(gamma) + p.ampconst * cos(gamma);
+  if (mode == 0) return v > 0 ? 1.0 : -1.0;
+  if (mode == 2) return fabs(v);
+  return v;
+}
+
+__device__ __forceinline__ double band_template(int r, float hr, float lr) {  // k:263-275
+  if (r < hr && r >= 0) return 1.0;
+  if (r >= hr && r < hr + 8) return 0.5 * cos(kRefPi * (r - (double)hr) / 16) + 0.5;
+  if (r >= (lr - 8) && r < lr && r >= 0) return 0.5 * cos(kRefPi * ((double)lr - r) / 16) + 0.5;
+  return 0.0;
+}
+
+__device__ __forceinline__ double band_image(int r, float hr, float lr) {  // k:183-189
+  if (r < hr && r >= lr) return 1.0;
+  if (r >= hr && r < hr + 8) return 0.5 * cos(kRefPi * (r - (double)hr) / 16) + 0.5;
+  if (r >= (lr - 8) && r < lr && r >= 0) return 0.5 * cos(kRefPi * ((double)lr - r) / 16) + 0.5;
+  return 0.0;
+}
+
+// Block-wide sum of a double (blockDim.x a multiple of 64, at most 1024).
+__device__ double block_sum(double v) {
+  __shared__ double red[16];
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
+  __syncthreads();
+  if (lane == 0) red[w] = v;
+  __syncthreads();
+  double t = 0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < (int)(blockDim.x / 64); ++i) t += red[i];
+  return t;  // valid in thread 0
+}
+
+// ---------------------------------------------------------------- templates
+__global__ void k_pad_templates(const float* __restrict__ raw, int n, int w, int h, int l,
+                                float2* __restrict__ out) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long L2 = (long long)l * l;
+  if (i >= L2 * n) return;
+  const int t = (int)(i / L2);
+  const int p = (int)(i - (long long)t * L2);
+  const int x = p % l, y = p / l;
+  const int sx = (l - w) / 2, sy = (l - h) / 2;  // src/norm.cu:137-138
+  float v = 0.f;
+  if (x >= sx && x < sx + w && y >= sy && y < sy + h)
+    v = raw[(long long)t * w * h + (long long)(y - sy) * w + (x - sx)];
+  out[i] = make_float2(v, 0.f);
+}
+
+// Ring sum/count (generate_mask + multiCount_dot, k:28-80) and box sums (UpdateSigma, k:4-26).
+__global__ void k_template_moments(const float2* __restrict__ T, int l, double* __restrict__ mom) {
+  const int t = blockIdx.x;
+  const float2* P = T + (long long)t * l * l;
+  const float r = l / 2.f - 2.f;
+  float up = 0.f, low = 0.f;
+  if (r > 1) {
+    up = (r + 1) * (r + 1);
+    low = (r - 1) * (r - 1);
+  }
+  double ring = 0, cnt = 0, s = 0, s2 = 0;
+  for (int p = threadIdx.x; p < l * l; p += blockDim.x) {
+    const int x = p % l, y = p / l;
+    const float rr = (float)((x - l / 2) * (x - l / 2) + (y - l / 2) * (y - l / 2));
+    const double v = P[p].x;
+    if (rr >= low && rr <= up) {
+      ring += v;
+      cnt += 1;
+    }
+    s += v;
+    s2 += v * v;
+  }
+  ring = block_sum(ring);
+  cnt = block_sum(cnt);
+  s = block_sum(s);
+  s2 = block_sum(s2);
+  if (threadIdx.x == 0) {
+    mom[4 * t + 0] = ring;
+    mom[4 * t + 1] = cnt;
+    mom[4 * t + 2] = s;
+    mom[4 * t + 3] = s2;
+  }
+}
+
+// scale_each (k:82-90) with the host sigma of src/norm.cu:268-273.
+__global__ void k_template_standardise(float2* __restrict__ T, int n, int l, const double* __restrict__ mom) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long L2 = (long long)l * l;
+  if (i >= L2 * n) return;
+  const int t = (int)(i / L2);
+  const double ring = mom[4 * t], cnt = mom[4 * t + 1], s = mom[4 * t + 2], s2 = mom[4 * t + 3];
+  const double em = cnt != 0 ? ring / cnt : ring;
+  const double mean = s / (double)L2;
+  double sigma = sqrt(s2 / (double)L2 - mean * mean);
+  if (sigma < 0 || !isfinite(sigma)) sigma = 0;
+  if (sigma < 1e-8 && sigma > -1e-8) return;
+  T[i].x = (float)(((double)T[i].x - em) / sigma);
+}
+
+// Radial power over the half plane x <= l/2 (SQRSum_by_circle, k:94-126); one block per plane.
+__global__ void k_radial_power_planes(const float2* __restrict__ F, int l, double* __restrict__ ra) {
+  extern __shared__ double hist[];
+  const int t = blockIdx.x, nbins = l / 2;
+  for (int i = threadIdx.x; i < nbins; i += blockDim.x) hist[i] = 0.0;
+  __syncthreads();
+  const float2* P = F + (long long)t * l * l;
+  const int hx = l / 2 + 1;
+  for (int p = threadIdx.x; p < l * hx; p += blockDim.x) {
+    const int y = p / hx, x = p % hx;
+    const int r = rbin(min(x, l - x), min(y, l - y));
+    if (r >= 0 && r < nbins) {
+      const float2 v = P[(long long)y * l + x];
+      atomicAdd(&hist[r], (double)v.x * v.x + (double)v.y * v.y);
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < nbins; i += blockDim.x) ra[(long long)t * nbins + i] = hist[i];
+}
+
+// whiten_Tmp (k:128-149): |F| /= sqrt(ra[r]/rb[r]) for r in [0, l/2).
+__global__ void k_whiten_planes(float2* __restrict__ F, int n, int l, const double* __restrict__ ra,
+                                const double* __restrict__ rb) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long L2 = (long long)l * l;
+  if (i >= L2 * n) return;
+  const int t = (int)(i / L2);
+  const int p = (int)(i - (long long)t * L2);
+  const int x = p % l, y = p / l;
+  const int r = rbin(min(x, l - x), min(y, l - y));
+  if (r >= 0 && r < l / 2) {
+    const double f = 1.0 / sqrt(ra[(long long)t * (l / 2) + r] / rb[r]);
+    F[i] = make_float2((float)(F[i].x * f), (float)(F[i].y * f));
+  }
+}
+
+// apply_weighting_function (k:243-288): band-pass and |CTF| Wiener-like weighting.
+__global__ void k_weight_planes(float2* __restrict__ F, int n, int l, DevParams p) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long L2 = (long long)l * l;
+  if (i >= L2 * n) return;
+  const int q = (int)(i % L2);
+  const int x = q % l, y = q / l;
+  const int fx = min(x, l - x), fy = min(y, l - y);
+  const int rr = rbin(fx, fy);
+  double f = band_template(rr, p.hr_t, p.lr_t);
+  if (rr < l / 2 && rr >= 0) {
+    const double r = hypot((double)fy, (double)fx);
+    const double ss = r * (double)p.ds;
+    const double v = ctf_ast(x, (y + l / 2) % l, l, l, p, 2);
+    const double signal = exp(p.bf * ss * ss + p.bf2 * ss + p.bf3) / ((double)p.kk + 1);
+    const double Ncurve = exp(p.a * ss * ss + p.b * ss + p.b2) / signal;
+    f *= v * sqrt(1 / (Ncurve + (double)p.kk * v * v));
+  }
+  F[i] = make_float2((float)(F[i].x * f), (float)(F[i].y * f));
+}
+
+// apply_mask (k:225-241) on the real part.
+__global__ void k_mask_planes(float2* __restrict__ T, int n, int l, float rin, float rout, float ehw) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long L2 = (long long)l * l;
+  if (i >= L2 * n) return;
+  const int q = (int)(i % L2);
+  const int x = q % l, y = q / l;
+  const double r = hypot((double)(x - l / 2), (double)(y - l / 2));
+  if (r > rout)
+    T[i].x = 0.f;
+  else if (r >= rin)
+    T[i].x = (float)(T[i].x * (0.5 * cos(kRefPi * (r - rin) / (2.0 * ehw)) + 0.5));
+}
+
+// compute_area_sum_ofSQR (k:319-382) + host mean (src/norm.cu:334-342); full [y][x] planes.
+__global__ void k_area_power_full(const float2* __restrict__ F, int l, double* __restrict__ m) {
+  const int t = blockIdx.x;
+  const float2* P = F + (long long)t * l * l;
+  const int hx = l / 2 + 1;
+  double s = 0, c = 0;
+  for (int p = threadIdx.x; p < l * hx; p += blockDim.x) {
+    const int y = p / hx, x = p % hx;
+    const int r = rbin(min(x, l - x), min(y, l - y));
+    if (r < l / 2 && r >= 0) {
+      const float2 v = P[(long long)y * l + x];
+      s += (double)v.x * v.x + (double)v.y * v.y;
+      c += 1;
+    }
+  }
+  s = block_sum(s);
+  c = block_sum(c);
+  if (threadIdx.x == 0) m[t] = sqrt(s / (c * c));
+}
+
+// Same over half spectra stored column-major [kx 0..l/2][ky].
+__global__ void k_area_power_half(const float2* __restrict__ F, int l, double* __restrict__ m) {
+  const int t = blockIdx.x;
+  const int hx = l / 2 + 1;
+  const float2* P = F + (long long)t * hx * l;
+  double s = 0, c = 0;
+  for (int p = threadIdx.x; p < l * hx; p += blockDim.x) {
+    const int x = p / l, y = p % l;
+    const int r = rbin(min(x, l - x), min(y, l - y));
+    if (r < l / 2 && r >= 0) {
+      const float2 v = P[p];
+      s += (double)v.x * v.x + (double)v.y * v.y;
+      c += 1;
+    }
+  }
+  s = block_sum(s);
+  c = block_sum(c);
+  if (threadIdx.x == 0) m[t] = sqrt(s / (c * c));
+}
+
+// normalize (k:445-458): F /= m (when m != 0), times an extra factor.
+__global__ void k_scale_planes(float2* __restrict__ F, int n, long long plane, const double* __restrict__ m,
+                               float extra) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= plane * n) return;
+  const int t = (int)(i / plane);
+  const double mt = m[t];
+  const double f = (mt != 0 ? 1.0 / mt : 1.0) * (double)extra;
+  F[i] = make_float2((float)(F[i].x * f), (float)(F[i].y * f));
+}
+
+__global__ void k_extract_real(const float2* __restrict__ F, float* __restrict__ out, long long count) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) out[i] = F[i].x;
+}
+
+// ---------------------------------------------------------------- whole frame
+__global__ void k_real_to_complex(const float* __restrict__ in, float2* __restrict__ out, long long count,
+                                  float sign) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) out[i] = make_float2(sign * in[i], 0.f);
+}
+
+// do_phase_flip (k:828-839)
+__global__ void k_phase_flip(float2* __restrict__ F, int nx, int ny, DevParams p) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long long)nx * ny) return;
+  const int x = (int)(i % nx), y = (int)(i / nx);
+  const float v = (float)ctf_ast(x, (y + ny / 2) % ny, nx, ny, p, 0);
+  F[i] = make_float2(F[i].x * v, F[i].y * v);
+}
+
+// SQRSum_by_circle mode 1 (k:94-126) over the whole frame: per-block LDS histogram, then global.
+__global__ void k_radial_power_image(const float2* __restrict__ F, int nx, int ny, double* __restrict__ ra) {
+  extern __shared__ double hist[];
+  const int nbins = max(nx, ny) / 2;
+  for (int i = threadIdx.x; i < nbins; i += blockDim.x) hist[i] = 0.0;
+  __syncthreads();
+  const int hx = nx / 2 + 1;
+  const long long total = (long long)ny * hx;
+  const long long per = (total + gridDim.x - 1) / gridDim.x;
+  const long long beg = (long long)blockIdx.x * per, end = min(total, beg + per);
+  for (long long p = beg + threadIdx.x; p < end; p += blockDim.x) {
+    const int y = (int)(p / hx), x = (int)(p % hx);
+    const int r = rbin(min(x, nx - x), min(y, ny - y));
+    if (r < nbins && r >= 0) {
+      const float2 v = F[(long long)y * nx + x];
+      atomicAdd(&hist[r], (double)v.x * v.x + (double)v.y * v.y);
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < nbins; i += blockDim.x)
+    if (hist[i] != 0.0) atomicAdd(&ra[i], hist[i]);
+}
+
+// whiten_filter_weight_Img (k:151-202)
+__global__ void k_whiten_filter_image(float2* __restrict__ F, int nx, int ny, const double* __restrict__ ra,
+                                      const double* __restrict__ rb, DevParams p) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long long)nx * ny) return;
+  const int x = (int)(i % nx), y = (int)(i / nx);
+  const int dxi = min(x, nx - x), dyi = min(y, ny - y);
+  const double dx = dxi, dy = dyi;
+  const int r = rbin(dxi, dyi);
+  const double ss = sqrt((dx * dx / (double)(nx * nx) + dy * dy / (double)(ny * ny)) /
+                         ((double)p.apix * (double)p.apix));
+  const int l = max(nx, ny);
+  double f = 1.0, v = 0, signal = 0, Ncurve = 0;
+  if (r < l / 2 && r >= 0) {
+    v = ctf_ast(x, (y + ny / 2) % ny, nx, ny, p, 2);
+    signal = exp(p.bf * ss * ss + p.bf2 * ss + p.bf3);
+    Ncurve = exp(p.a * ss * ss + p.b * ss + p.b2);
+    f *= sqrt((signal * v * v + Ncurve) / signal) / sqrt(ra[r] / rb[r]);
+    if (r > p.hi_thr) f *= exp(-100 * ss * ss);
+  }
+  f *= band_image(r, p.hr_i, p.lr_i);
+  if (r < l / 2 && r >= 0) {
+    signal /= ((double)p.kk + 1);
+    Ncurve /= signal;
+    f *= sqrt(1 / (Ncurve + (double)p.kk * v * v));
+  }
+  F[i] = make_float2((float)(F[i].x * f), (float)(F[i].y * f));
+}
+
+__global__ void k_zero_dc(float2* F) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) F[0] = make_float2(0.f, 0.f);
+}
+
+// Tile copy of TileImages (src/utils/tileimages.cpp:66-71) with optional inversion.
+__global__ void k_extract_tile(const float* __restrict__ img, int W, int ulx, int uly, int tile,
+                               float sign, float* __restrict__ out) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long long)tile * tile) return;
+  const int r = (int)(i / tile), c = (int)(i % tile);
+  out[i] = sign * img[(long long)(uly + r) * W + ulx + c];
+}
+
+// ---------------------------------------------------------------- blocks
+// split_IMG (k:605-643): block b = (i, j), i = b % bx, at (offx[i], offy[j]); zero outside.
+__global__ void k_split_blocks(const float* __restrict__ img, int nx, int ny, int l,
+                               const int* __restrict__ offx, const int* __restrict__ offy, int bx,
+                               int nb, float2* __restrict__ out) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long L2 = (long long)l * l;
+  if (i >= L2 * nb) return;
+  const int b = (int)(i / L2);
+  const int p = (int)(i - (long long)b * L2);
+  const int x = p % l, y = p / l;
+  const int X = offx[b % bx] + x, Y = offy[b / bx] + y;
+  const float v = (X < nx && Y < ny) ? img[(long long)Y * nx + X] : 0.f;
+  out[i] = make_float2(v, 0.f);
+}
+
+// ---------------------------------------------------------------- rotation
+// Bilinear sample geometry of rotate_subIMG (k:534-603), float arithmetic with explicit FMA.
+struct Bil {
+  int k0, k1, k2, k3;
+  float t, u;
+  bool ok;
+};
+__device__ __forceinline__ Bil bil_coords(int i, int j, int l, float cose, float sine) {
+  Bil o;
+  const float x = (float)(i - l / 2), y = (float)(j - l / 2);
+  const float half = (float)(l / 2);
+  const float x2 = __fmaf_rn(cose, x, sine * y) + half;
+  const float y2 = __fmaf_rn(-sine, x, cose * y) + half;
+  o.ok = !(x2 < 0 || x2 > l - 1.0f || y2 < 0 || y2 > l - 1.0f);
+  if (!o.ok) return o;
+  const int ii = (int)floorf(x2), jj = (int)floorf(y2);
+  o.k0 = ii + jj * l;
+  o.k1 = o.k0 + 1;
+  o.k2 = o.k0 + l + 1;
+  o.k3 = o.k0 + l;
+  if (ii == l - 1) { o.k1 -= 1; o.k2 -= 1; }
+  if (jj == l - 1) { o.k2 -= l; o.k3 -= l; }
+  o.t = x2 - (float)ii;
+  o.u = y2 - (float)jj;
+  return o;
+}
+
+// Rotate + mask one template row per block; per-row double partial sums for mean/var.
+__global__ void k_rotate_mask(const float* __restrict__ tp, int l, float cose, float sine, float rin,
+                              float rout, float ehw, float* __restrict__ out, double* __restrict__ partial) {
+  const int j = blockIdx.x, t = blockIdx.y;
+  const float* P = tp + (long long)t * l * l;
+  double s = 0, s2 = 0;
+  for (int i = threadIdx.x; i < l; i += blockDim.x) {
+    float res = 0.f;
+    const Bil b = bil_coords(i, j, l, cose, sine);
+    if (b.ok) {
+      const float tt = 1.0f - b.t, uu = 1.0f - b.u;
+      res = P[b.k0] * tt * uu + P[b.k1] * b.t * uu + P[b.k2] * b.t * b.u + P[b.k3] * tt * b.u;
+    }
+    const float r = hypotf((float)(i - l / 2), (float)(j - l / 2));
+    if (r > rout)
+      res = 0.f;
+    else if (r >= rin)
+      res *= 0.5f * cosf((float)kRefPi * (r - rin) / (2.0f * ehw)) + 0.5f;
+    out[((long long)t * l + j) * l + i] = res;
+    s += res;
+    s2 += (double)res * res;
+  }
+  s = block_sum(s);
+  s2 = block_sum(s2);
+  if (threadIdx.x == 0) {
+    partial[((long long)t * l + j) * 2 + 0] = s;
+    partial[((long long)t * l + j) * 2 + 1] = s2;
+  }
+}
+
+// Host reduction of RotateTemplate (src/norm.cu:498-507): mean and E[x^2]-mean^2 per template.
+__global__ void k_rotation_stats(const double* __restrict__ partial, int n, int l, float* __restrict__ stats) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  double s = 0, s2 = 0;
+  for (int j = 0; j < l; ++j) {
+    s += partial[((long long)t * l + j) * 2];
+    s2 += partial[((long long)t * l + j) * 2 + 1];
+  }
+  const double L2 = (double)l * l;
+  const double mean = s / L2;
+  stats[2 * t] = (float)mean;
+  stats[2 * t + 1] = (float)(s2 / L2 - mean * mean);
+}
+
+// substract_by_mean + divided_by_var (k:460-477) into a complex plane.
+__global__ void k_standardise_to_complex(const float* __restrict__ R, int n, int l, const float* __restrict__ stats,
+                                         float2* __restrict__ out) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long L2 = (long long)l * l;
+  if (i >= L2 * n) return;
+  const int t = (int)(i / L2);
+  const float mean = stats[2 * t], var = stats[2 * t + 1];
+  float v = R[i];
+  if (mean != 0) v = v - mean;
+  if (var != 0) v = v / sqrtf(var);
+  out[i] = make_float2(v, 0.f);
+}
+
+// rotate_subIMG (k:534-603) on complex planes (both parts).
+__global__ void k_rotate_complex(const float2* __restrict__ in, int n, int l, float cose, float sine,
+                                 float2* __restrict__ out) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long L2 = (long long)l * l;
+  if (i >= L2 * n) return;
+  const int t = (int)(i / L2);
+  const int q = (int)(i - (long long)t * L2);
+  const float2* P = in + (long long)t * L2;
+  const Bil b = bil_coords(q % l, q / l, l, cose, sine);
+  float2 res = make_float2(0.f, 0.f);
+  if (b.ok) {
+    const float tt = 1.0f - b.t, uu = 1.0f - b.u;
+    const float w0 = tt * uu, w1 = b.t * uu, w2 = b.t * b.u, w3 = tt * b.u;
+    res.x = P[b.k0].x * w0 + P[b.k1].x * w1 + P[b.k2].x * w2 + P[b.k3].x * w3;
+    res.y = P[b.k0].y * w0 + P[b.k1].y * w1 + P[b.k2].y * w2 + P[b.k3].y * w3;
+  }
+  out[i] = res;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- launchers
+void launch_pad_templates(const float* raw, int n, int w, int h, int l, float2* out, hipStream_t s) {
+  k_pad_templates<<<nblk((long long)n * l * l), BS, 0, s>>>(raw, n, w, h, l, out);
+  GS_CHECK_LAUNCH();
+}
+void launch_template_moments(const float2* T, int n, int l, double* mom, hipStream_t s) {
+  k_template_moments<<<n, BS, 0, s>>>(T, l, mom);
+  GS_CHECK_LAUNCH();
+}
+void launch_template_standardise(float2* T, int n, int l, const double* mom, hipStream_t s) {
+  k_template_standardise<<<nblk((long long)n * l * l), BS, 0, s>>>(T, n, l, mom);
+  GS_CHECK_LAUNCH();
+}
+void launch_radial_power_planes(const float2* F, int n, int l, double* ra, hipStream_t s) {
+  k_radial_power_planes<<<n, BS, sizeof(double) * (l / 2), s>>>(F, l, ra);
+  GS_CHECK_LAUNCH();
+}
+void launch_whiten_planes(float2* F, int n, int l, const double* ra, const double* rb, hipStream_t s) {
+  k_whiten_planes<<<nblk((long long)n * l * l), BS, 0, s>>>(F, n, l, ra, rb);
+  GS_CHECK_LAUNCH();
+}
+void launch_weight_planes(float2* F, int n, int l, const DevParams& p, hipStream_t s) {
+  k_weight_planes<<<nblk((long long)n * l * l), BS, 0, s>>>(F, n, l, p);
+  GS_CHECK_LAUNCH();
+}
+void launch_mask_planes(float2* T, int n, int l, float d_m, float ehw, hipStream_t s) {
+  const float dm = 1.5f * d_m;  // apply_mask: d_m = 1.5 * d_m (k:233)
+  const float rin = dm / 2, rout = dm / 2 + 2 * ehw;
+  k_mask_planes<<<nblk((long long)n * l * l), BS, 0, s>>>(T, n, l, rin, rout, ehw);
+  GS_CHECK_LAUNCH();
+}
+void launch_area_power_full(const float2* F, int n, int l, double* m, hipStream_t s) {
+  k_area_power_full<<<n, BS, 0, s>>>(F, l, m);
+  GS_CHECK_LAUNCH();
+}
+void launch_area_power_half(const float2* F, int n, int l, double* m, hipStream_t s) {
+  k_area_power_half<<<n, BS, 0, s>>>(F, l, m);
+  GS_CHECK_LAUNCH();
+}
+void launch_scale_planes(float2* F, int n, long long plane, const double* m, float extra, hipStream_t s) {
+  k_scale_planes<<<nblk(plane * n), BS, 0, s>>>(F, n, plane, m, extra);
+  GS_CHECK_LAUNCH();
+}
+void launch_extract_real(const float2* F, float* out, long long count, hipStream_t s) {
+  k_extract_real<<<nblk(count), BS, 0, s>>>(F, out, count);
+  GS_CHECK_LAUNCH();
+}
+void launch_real_to_complex(const float* in, float2* out, long long count, float sign, hipStream_t s) {
+  k_real_to_complex<<<nblk(count), BS, 0, s>>>(in, out, count, sign);
+  GS_CHECK_LAUNCH();
+}
+void launch_phase_flip(float2* F, int nx, int ny, const DevParams& p, hipStream_t s) {
+  k_phase_flip<<<nblk((long long)nx * ny), BS, 0, s>>>(F, nx, ny, p);
+  GS_CHECK_LAUNCH();
+}
+void launch_radial_power_image(const float2* F, int nx, int ny, double* ra, hipStream_t s) {
+  const int nbins = std::max(nx, ny) / 2;
+  GS_REQUIRE(nbins * sizeof(double) <= 48 * 1024, kUnsupported, "frame too large for radial histogram");
+  const long long work = (long long)ny * (nx / 2 + 1);
+  const unsigned grid = (unsigned)std::min<long long>(1024, (work + 4095) / 4096);
+  k_radial_power_image<<<grid, BS, sizeof(double) * nbins, s>>>(F, nx, ny, ra);
+  GS_CHECK_LAUNCH();
+}
+void launch_whiten_filter_image(float2* F, int nx, int ny, const double* ra, const double* rb,
+                                const DevParams& p, hipStream_t s) {
+  k_whiten_filter_image<<<nblk((long long)nx * ny), BS, 0, s>>>(F, nx, ny, ra, rb, p);
+  GS_CHECK_LAUNCH();
+}
+void launch_zero_dc(float2* F, hipStream_t s) {
+  k_zero_dc<<<1, 64, 0, s>>>(F);
+  GS_CHECK_LAUNCH();
+}
+void launch_extract_tile(const float* img, int W, int ulx, int uly, int tile, float sign, float* out,
+                         hipStream_t s) {
+  k_extract_tile<<<nblk((long long)tile * tile), BS, 0, s>>>(img, W, ulx, uly, tile, sign, out);
+  GS_CHECK_LAUNCH();
+}
+void launch_split_blocks(const float* img, int nx, int ny, int l, const int* offx, const int* offy,
+                         int bx, int nb, float2* out, hipStream_t s) {
+  k_split_blocks<<<nblk((long long)nb * l * l), BS, 0, s>>>(img, nx, ny, l, offx, offy, bx, nb, out);
+  GS_CHECK_LAUNCH();
+}
+void launch_rotate_mask(const float* tp, int n, int l, float cose, float sine, float d_m, float ehw,
+                        float* out, double* partial, hipStream_t s) {
+  const float dm = 1.5f * d_m;
+  const float rin = dm / 2, rout = dm / 2 + 2 * ehw;
+  k_rotate_mask<<<dim3(l, n), BS, 0, s>>>(tp, l, cose, sine, rin, rout, ehw, out, partial);
+  GS_CHECK_LAUNCH();
+}
+void launch_rotation_stats(const double* partial, int n, int l, float* stats, hipStream_t s) {
+  k_rotation_stats<<<nblk(n, 64), 64, 0, s>>>(partial, n, l, stats);
+  GS_CHECK_LAUNCH();
+}
+void launch_standardise_to_complex(const float* R, int n, int l, const float* stats, float2* out,
+                                   hipStream_t s) {
+  k_standardise_to_complex<<<nblk((long long)n * l * l), BS, 0, s>>>(R, n, l, stats, out);
+  GS_CHECK_LAUNCH();
+}
+void launch_rotate_complex(const float2* in, int n, int l, float cose, float sine, float2* out,
+                           hipStream_t s) {
+  k_rotate_complex<<<nblk((long long)n * l * l), BS, 0, s>>>(in, n, l, cose, sine, out);
+  GS_CHECK_LAUNCH();
+}
+
+}  // namespace gs

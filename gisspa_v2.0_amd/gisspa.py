@@ -1,0 +1,242 @@
+"""ctypes binding of libgisspa.so (the C ABI declared in include/gisspa.h).
+
+The search itself runs in the HIP library; this module only marshals numpy arrays across the
+C ABI.  It raises if the library has not been built: there is no fallback path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, fields
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libgisspa.so")
+
+GS_STAGE_TEMPLATES, GS_STAGE_IMAGE, GS_STAGE_BLOCKS, GS_STAGE_ROTATED = 0, 1, 2, 3
+GS_STAGE_MU, GS_STAGE_SIGMA, GS_STAGE_BEST, GS_STAGE_ARG, GS_STAGE_PLANES = 4, 5, 6, 7, 8
+
+
+class Params(C.Structure):
+    """gs_params: the reference config keys (include/EMReader/DataReader2.h:17-44)."""
+
+    _fields_ = [
+        ("angpix", C.c_float), ("phistep", C.c_float), ("kk", C.c_float), ("energy", C.c_float),
+        ("cs", C.c_float), ("highres", C.c_float), ("lowres", C.c_float), ("diameter", C.c_float),
+        ("threshold", C.c_float), ("window_size", C.c_int32), ("overlap", C.c_int32),
+        ("norm_type", C.c_int32), ("invert", C.c_int32), ("phase_flip", C.c_int32),
+        ("nms_radius", C.c_float),
+    ]
+
+
+class Ctf(C.Structure):
+    _fields_ = [("defocus", C.c_double), ("dfdiff", C.c_double), ("dfang", C.c_double)]
+
+
+class Record(C.Structure):
+    _fields_ = [
+        ("x", C.c_float), ("y", C.c_float), ("template_id", C.c_int32), ("psi", C.c_float),
+        ("score", C.c_float), ("angle_index", C.c_int32), ("block", C.c_int32),
+        ("pixel", C.c_int32), ("local_x", C.c_float), ("local_y", C.c_float),
+    ]
+
+
+RECORD_DTYPE = np.dtype([
+    ("x", "<f4"), ("y", "<f4"), ("template_id", "<i4"), ("psi", "<f4"), ("score", "<f4"),
+    ("angle_index", "<i4"), ("block", "<i4"), ("pixel", "<i4"), ("local_x", "<f4"),
+    ("local_y", "<f4"),
+])
+
+# Entry points declared in include/gisspa.h, with ctypes signatures.
+_P = C.c_void_p
+SIGNATURES = {
+    "gs_create": (C.c_int, [C.POINTER(Params), C.c_int, C.POINTER(_P)]),
+    "gs_load_templates": (C.c_int, [_P, _P, C.c_int, C.c_int, C.c_int, _P]),
+    "gs_set_image": (C.c_int, [_P, _P, C.c_int, C.c_int, C.POINTER(Ctf)]),
+    "gs_run": (C.c_int, [_P]),
+    "gs_get_records": (C.c_int, [_P, C.POINTER(_P), C.POINTER(C.c_size_t)]),
+    "gs_search": (C.c_int, [_P, _P, C.c_int, C.c_int, C.POINTER(Ctf), C.POINTER(_P), C.POINTER(C.c_size_t)]),
+    "gs_format_record": (C.c_int, [_P, C.POINTER(Record), C.c_int, C.c_char_p, C.POINTER(Ctf), C.c_int, C.c_char_p, C.c_size_t]),
+    "gs_free": (None, [_P]),
+    "gs_destroy": (None, [_P]),
+    "gs_last_error": (C.c_char_p, []),
+    "gs_stage_bytes": (C.c_size_t, [_P, C.c_int]),
+    "gs_fetch_stage": (C.c_int, [_P, C.c_int, _P, C.c_size_t]),
+    "gs_set_limits": (C.c_int, [_P, C.c_int, C.c_int]),
+    "gs_fft_c2c": (C.c_int, [C.c_int, _P, C.c_int, C.c_int, C.c_int]),
+    "gs_fft2_c2c": (C.c_int, [C.c_int, _P, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "gs_last_timing": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "gs_set_profiling": (C.c_int, [_P, C.c_int]),
+    "gs_kernel_timing": (C.c_int, [_P, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]),
+    "gs_set_chunk": (C.c_int, [_P, C.c_int]),
+}
+
+_lib = None
+
+
+def load(path: str = LIB_PATH):
+    """Load libgisspa.so; raises OSError when it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise OSError(f"{path} not built: run __graft_entry__.build() or make -C gisspa_v2.0_amd")
+    lib = C.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+class GisspaError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"gisspa status {status}: {msg}")
+        self.status = status
+
+
+def _check(st: int):
+    if st != 0:
+        raise GisspaError(st, load().gs_last_error().decode(errors="replace"))
+
+
+@dataclass
+class SearchConfig:
+    """Search parameters with the reference's key names and defaults
+    (include/EMReader/DataReader2.h:17-44)."""
+
+    angpix: float = 1.0
+    phistep: float = 10.0
+    kk: float = 3.0
+    energy: float = 300.0
+    cs: float = 2.7
+    highres: float = 8.0
+    lowres: float = 100.0
+    diameter: float = 128.0
+    threshold: float = 7.0
+    window_size: int = 320
+    overlap: int = 0
+    norm_type: int = 1
+    invert: int = 0
+    phase_flip: int = 0
+    nms_radius: float = 64.0
+
+    def to_c(self) -> Params:
+        return Params(**{f.name: getattr(self, f.name) for f in fields(self)})
+
+
+def _f32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+class Engine:
+    """One search engine bound to a device (gs_create ... gs_destroy)."""
+
+    def __init__(self, cfg: SearchConfig, device: int = 0):
+        lib = load()
+        self.cfg = cfg
+        self._h = C.c_void_p()
+        self._params = cfg.to_c()
+        _check(lib.gs_create(C.byref(self._params), device, C.byref(self._h)))
+
+    def close(self):
+        if self._h:
+            load().gs_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def load_templates(self, tmpl: np.ndarray, alt_az: np.ndarray | None = None):
+        t = _f32(tmpl)
+        assert t.ndim == 3
+        self._tmpl = t
+        aa = None if alt_az is None else _f32(alt_az).reshape(-1, 2)
+        self._altaz = aa
+        _check(load().gs_load_templates(self._h, t.ctypes.data, t.shape[0], t.shape[2], t.shape[1],
+                                        None if aa is None else aa.ctypes.data))
+
+    def set_image(self, image: np.ndarray, ctf=(1.5, 0.0, 0.0)):
+        img = _f32(image)
+        self._ctf = Ctf(*ctf)
+        _check(load().gs_set_image(self._h, img.ctypes.data, img.shape[1], img.shape[0], C.byref(self._ctf)))
+
+    def run(self):
+        _check(load().gs_run(self._h))
+
+    def records(self) -> np.ndarray:
+        p = C.c_void_p()
+        n = C.c_size_t()
+        _check(load().gs_get_records(self._h, C.byref(p), C.byref(n)))
+        try:
+            buf = (C.c_char * (n.value * RECORD_DTYPE.itemsize)).from_address(p.value) if n.value else b""
+            return np.frombuffer(bytes(buf), dtype=RECORD_DTYPE).copy()
+        finally:
+            load().gs_free(p)
+
+    def search(self, image, ctf=(1.5, 0.0, 0.0)) -> np.ndarray:
+        self.set_image(image, ctf)
+        self.run()
+        return self.records()
+
+    def set_limits(self, n_angles: int = 0, n_templates: int = 0):
+        _check(load().gs_set_limits(self._h, n_angles, n_templates))
+
+    def set_profiling(self, on: bool):
+        _check(load().gs_set_profiling(self._h, 1 if on else 0))
+
+    def set_chunk(self, tc: int):
+        _check(load().gs_set_chunk(self._h, tc))
+
+    def kernel_timing(self, which: int):
+        avg = C.c_double()
+        cnt = C.c_longlong()
+        _check(load().gs_kernel_timing(self._h, which, C.byref(avg), C.byref(cnt)))
+        return avg.value, cnt.value
+
+    def timing(self):
+        a, b = C.c_double(), C.c_double()
+        _check(load().gs_last_timing(self._h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def stage(self, which: int, dtype) -> np.ndarray:
+        nbytes = load().gs_stage_bytes(self._h, which)
+        if nbytes == 0:
+            raise GisspaError(5, f"stage {which} unavailable")
+        out = np.empty(nbytes // np.dtype(dtype).itemsize, dtype=dtype)
+        _check(load().gs_fetch_stage(self._h, which, out.ctypes.data, nbytes))
+        return out
+
+    def format_record(self, rec, image_index: int, path: str, tile_local: bool = False) -> str:
+        r = Record(*[rec[name] for name in RECORD_DTYPE.names])
+        buf = C.create_string_buffer(1024)
+        load().gs_format_record(self._h, C.byref(r), image_index, path.encode(), C.byref(self._ctf),
+                                1 if tile_local else 0, buf, 1024)
+        return buf.value.decode()
+
+
+def fft_c2c(x: np.ndarray, sign: int, device: int = 0) -> np.ndarray:
+    """Batched 1-D complex transform along the last axis on the GPU (unnormalised)."""
+    a = np.array(x, dtype=np.complex64, order="C", copy=True)
+    n = a.shape[-1]
+    _check(load().gs_fft_c2c(device, a.ctypes.data, n, a.size // n, sign))
+    return a
+
+
+def fft2_c2c(x: np.ndarray, sign: int, device: int = 0) -> np.ndarray:
+    """Batched 2-D complex transform over the last two axes on the GPU (unnormalised)."""
+    a = np.array(x, dtype=np.complex64, order="C", copy=True)
+    ny, nx = a.shape[-2:]
+    _check(load().gs_fft2_c2c(device, a.ctypes.data, ny, nx, a.size // (nx * ny), sign))
+    return a

@@ -1,0 +1,149 @@
+/*
+ * gisspa.h -- C ABI of the MI355X template-matching search library (libgisspa.so).
+ *
+ * The reference has no plugin or FFI: its boundary is the C++ engine pair
+ *   SearchNorm   (include/norm.cuh:12-55,   src/norm.cu:48-123, 620-660)  norm_type = 1
+ *   SearchNoNorm (include/nonorm.cuh:12-49, src/nonorm.cu:50-122, 559-589) norm_type = 0
+ * constructed per micrograph by main (src/main.cu:42-48) from a Config, the EulerData and a
+ * device id, and driven by work(Templates, Image|TileImages, ostream).  This header exposes the
+ * same capabilities as plain C entry points: an engine bound to one device and one host
+ * thread, template loading, a per-micrograph search, and the records the reference prints.
+ *
+ * Conventions: every pointer argument is caller-owned host memory and is never modified;
+ * arrays returned by the library are freed with gs_free.  Functions return a gs_status and
+ * never exit or throw; gs_last_error() describes the last failure on the calling thread.
+ * Separate engines may be used concurrently from separate threads.
+ */
+#ifndef GISSPA_H
+#define GISSPA_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  GS_OK = 0,
+  GS_ERR_INVALID = 1,     /* bad argument or configuration */
+  GS_ERR_HIP = 2,         /* a HIP runtime call failed */
+  GS_ERR_UNSUPPORTED = 3, /* valid request this build does not handle */
+  GS_ERR_NOMEM = 4,       /* device or host allocation failed */
+  GS_ERR_STATE = 5        /* call out of order (e.g. search before templates) */
+} gs_status;
+
+/* Search parameters.  Each field is the reference config key of the same meaning
+ * (include/EMReader/DataReader2.h:17-44 keys, :73-83 Parameters):
+ *   angpix, phistep, kk, energy, cs, Highres, Lowres, diameter, threshold,
+ *   window_size, overlap, norm_type, invert, phase_flip.
+ * nms_radius is new: radius (px) of the greedy non-maximum suppression applied to the
+ * records before they are returned (0 returns every record above threshold). */
+typedef struct {
+  float angpix, phistep, kk, energy, cs, highres, lowres, diameter, threshold;
+  int32_t window_size, overlap, norm_type, invert, phase_flip;
+  float nms_radius;
+} gs_params;
+
+/* Per-micrograph CTF of an LST line (src/EMReader/DataReader2.cpp:80-101):
+ * defocus and dfdiff in micrometres, dfang in degrees. */
+typedef struct {
+  double defocus, dfdiff, dfang;
+} gs_ctf;
+
+/* One output record: the fields the reference prints per line (src/norm.cu:610-615,
+ * src/nonorm.cu:548-553): centre, template index J (Euler angles alt[J], az[J]), in-plane angle
+ * psi (norm: 360-theta; nonorm: theta) and score; plus bookkeeping (angle loop index, block or
+ * tile, in-block pixel) and, for norm_type=0, the tile-local centre the reference prints. */
+typedef struct {
+  float x, y;
+  int32_t template_id;
+  float psi;
+  float score;
+  int32_t angle_index;
+  int32_t block;
+  int32_t pixel;
+  float local_x, local_y;
+} gs_record;
+
+typedef struct gs_engine gs_engine;
+
+/* Replaces SearchNorm/SearchNoNorm construction (src/norm.cu:48-123, src/nonorm.cu:50-122):
+ * binds an engine to `device` and validates the parameters. */
+gs_status gs_create(const gs_params* params, int device, gs_engine** out);
+
+/* Replaces Templates + LoadTemplate (src/utils/templates.cpp:11-31, src/norm.cu:132-154):
+ * n templates of w x h float32, row-major, stacked; alt_az holds n (alt, az) pairs from the
+ * Euler file (src/EMReader/DataReader2.cpp:55-78) and may be NULL. */
+gs_status gs_load_templates(gs_engine* e, const float* data, int n, int w, int h,
+                            const float* alt_az);
+
+/* Replaces Image/LoadImage + SetParams (src/utils/image.cpp:5-18, src/norm.cu:156-192): takes
+ * one W x H float32 micrograph and its CTF, applies the reference's host-side clipping and
+ * inversion, and uploads it.  The caller's pixels are not modified. */
+gs_status gs_set_image(gs_engine* e, const float* image, int width, int height,
+                       const gs_ctf* ctf);
+
+/* Replaces work() (src/norm.cu:620-660, src/nonorm.cu:559-589) minus the text output: runs the
+ * whole device pipeline on the image set by gs_set_image.  Asynchronous work completes before
+ * return. */
+gs_status gs_run(gs_engine* e);
+
+/* Records of the last gs_run, sorted by descending score, after NMS when nms_radius > 0.
+ * *out is allocated by the library (free with gs_free); *n_out receives the count. */
+gs_status gs_get_records(gs_engine* e, gs_record** out, size_t* n_out);
+
+/* gs_set_image + gs_run + gs_get_records. */
+gs_status gs_search(gs_engine* e, const float* image, int width, int height, const gs_ctf* ctf,
+                    gs_record** out, size_t* n_out);
+
+/* Formats one record as the reference's output line (src/norm.cu:610-615):
+ * "%d\t%s\tdefocus=%f\tdfdiff=%f\tdfang=%f\teuler=%f,%f,%f\tcenter=%f,%f\tscore=%f\n".
+ * Writes at most cap bytes including the terminator; returns the untruncated length. */
+int gs_format_record(const gs_engine* e, const gs_record* r, int image_index, const char* path,
+                     const gs_ctf* ctf, int tile_local, char* buf, size_t cap);
+
+void gs_free(void* p);
+void gs_destroy(gs_engine* e);
+const char* gs_last_error(void);
+
+/* ---- inspection entry points (no counterpart in the reference; used by the parity tests) ---- */
+typedef enum {
+  GS_STAGE_TEMPLATES = 0,  /* norm: processed real-space templates, n*l*l float32;
+                              nonorm: Fourier-space templates, n*(l/2+1)*l complex64 [t][kx][ky] */
+  GS_STAGE_IMAGE = 1,      /* preprocessed micrograph (norm) W*H float32 */
+  GS_STAGE_BLOCKS = 2,     /* norm: block spectra nb*(l/2+1)*l complex64 [b][kx][ky], scaled by 1/l^2 */
+  GS_STAGE_ROTATED = 3,    /* norm: template spectra of the last angle run, n*(l/2+1)*l complex64 */
+  GS_STAGE_MU = 4,         /* norm: per block pixel mean, nb*l*l float64 */
+  GS_STAGE_SIGMA = 5,      /* norm: per block pixel sigma, nb*l*l float64 */
+  GS_STAGE_BEST = 6,       /* norm: per block pixel best correlation, nb*l*l float32 */
+  GS_STAGE_ARG = 7,        /* norm: per block pixel argmax angle_index*n+template, nb*l*l int32 */
+  GS_STAGE_PLANES = 8      /* nonorm: per (tile, angle, block, template) peak, index, sum, sum2 float64 */
+} gs_stage;
+
+/* Size in bytes of a stage buffer after the last run (0 if not available). */
+size_t gs_stage_bytes(const gs_engine* e, int stage);
+/* Copies a stage buffer to host memory (bytes must equal gs_stage_bytes). */
+gs_status gs_fetch_stage(gs_engine* e, int stage, void* out, size_t bytes);
+/* Restricts the next gs_run to the first n_angles angles and n_templates templates (<= 0 means
+ * all); mu and sigma then use the restricted counts.  For tests and timing samples. */
+gs_status gs_set_limits(gs_engine* e, int n_angles, int n_templates);
+/* Standalone transform of `batch` contiguous complex rows of length n on `device` (host
+ * buffers, interleaved re/im float32, in place, unnormalised; sign -1 forward, +1 inverse). */
+gs_status gs_fft_c2c(int device, float* data, int n, int batch, int sign);
+/* Standalone 2-D transform of `batch` row-major ny x nx complex planes (in place). */
+gs_status gs_fft2_c2c(int device, float* data, int ny, int nx, int batch, int sign);
+/* Wall time of the device pipeline of the last gs_run, and of its angle loop, in ms. */
+gs_status gs_last_timing(const gs_engine* e, double* total_ms, double* corr_ms);
+/* When on, every correlation launch is bracketed by HIP events on the engine's stream. */
+gs_status gs_set_profiling(gs_engine* e, int on);
+/* Average duration (ms) and count of the timed launches: which = 0 column pass, 1 row pass. */
+gs_status gs_kernel_timing(const gs_engine* e, int which, double* avg_ms, long long* count);
+/* Templates per correlation launch (0 = automatic); a result-invariant tuning knob. */
+gs_status gs_set_chunk(gs_engine* e, int templates_per_launch);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GISSPA_H */

@@ -1,0 +1,840 @@
+// FP64 CPU oracle of the GisSPA search (TEST INFRASTRUCTURE ONLY; see gisspa_oracle.h).
+//
+// Every function restates one reference routine in double precision.  Parameter-level scalars
+// that drive integer decisions (band-pass radii, rotation sample coordinates) are evaluated in
+// float exactly as the HIP path evaluates them, so both sides take the same branches; all
+// data arithmetic is FP64.  Radial bins are computed exactly from integer squared radii.
+#include "gisspa_oracle.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <stdexcept>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "fft64.hpp"
+
+namespace {
+using orc::cd;
+
+constexpr double PI = 3.1415926535;  // include/utils/constants.h:3
+constexpr double EPS = 1e-8;         // include/utils/constants.h:4
+constexpr int TILE = 720;            // include/utils/constants.h:5
+
+// Physics constants of Parameters (include/EMReader/DataReader2.h:76-77), as the float values
+// the reference stores.
+struct Phys {
+  double edge_half_width = 4.0f, ampconst = 0.07f;
+  double a = -10.81f, b = 1.f, b2 = 0.32f, bf = -18.17f, bf2 = -15.22f, bf3 = 1.72f;
+};
+
+struct P {
+  // config (float in the reference)
+  float apix, kk, energy, cs, highres, lowres, d_m, thresh;
+  // per micrograph (SetParams, src/norm.cu:183-192)
+  double lambda, dfu, dfv, dfdiff, dfang, defocus;
+  Phys ph;
+};
+
+P make_params(const orc_config* c, const double* ctf3) {
+  P p{};
+  p.apix = c->angpix; p.kk = c->kk; p.energy = c->energy; p.cs = c->cs;
+  p.highres = c->highres; p.lowres = c->lowres; p.d_m = c->diameter; p.thresh = c->threshold;
+  const double def = ctf3[0], dfdiff = ctf3[1], dfang = ctf3[2];
+  p.defocus = -def;
+  p.dfdiff = dfdiff;
+  p.dfang = dfang;
+  p.dfu = -def + dfdiff;  // src/norm.cu:188
+  p.dfv = -def - dfdiff;  // src/norm.cu:189
+  const double E = p.energy;
+  p.lambda = 12.2639 / std::sqrt(E * 1000.0 + 0.97845 * E * E);  // src/norm.cu:190
+  return p;
+}
+
+// floor(hypot(dx,dy) + 0.5) - 1 computed exactly (reference: k:118,136,165,264,329).
+inline int rbin(long long dx, long long dy) {
+  long long n = dx * dx + dy * dy;
+  long long k = (long long)std::sqrt((double)n);
+  while (k * k > n) --k;
+  while ((k + 1) * (k + 1) <= n) ++k;
+  if (n > k * k + k) ++k;  // sqrt(n) >= k + 1/2  <=>  n > k^2 + k
+  return (int)k - 1;
+}
+
+// CTF_AST (src/kernels.cu:290-317) in FP64.
+double ctf_ast(int x1, int y1, int nx, int ny, const P& p, int mode) {
+  y1 = x1 > nx / 2 ? ny - y1 : y1;
+  x1 = std::min(x1, nx - x1);
+  const double dx = std::min(x1, nx - x1);
+  const double dy = y1 - ny / 2;
+  const double ss = (dx * dx / (double)(nx * nx) + dy * dy / (double)(ny * ny)) /
+                    ((double)p.apix * (double)p.apix);
+  const double ag = std::atan2((double)(y1 - ny / 2), (double)x1);
+  const double df_ast =
+      0.5 * (p.dfu + p.dfv + 2 * p.dfdiff * std::cos(2 * (p.dfang * PI / 180 - ag)));
+  const double lam = p.lambda;
+  const double gamma =
+      -2 * PI * ((double)p.cs * 2.5e6 * lam * lam * lam * ss * ss + df_ast * 5000.0 * lam * ss);
+  const double A = p.ph.ampconst;
+  const double v = std::sqrt(1.0 - A * A) * std::sin(gamma) + A * std::cos(gamma);
+  if (mode == 0) return v > 0 ? 1.0 : -1.0;
+  if (mode == 2) return std::fabs(v);
+  return v;
+}
+
+// Band-pass radii are float expressions of config values in the reference (k:183-189,265-273).
+struct Band {
+  float hr, lr;
+};
+Band band(int l, const P& p) {
+  Band b;
+  b.hr = (float)l * p.apix / p.highres;
+  b.lr = (float)l * p.apix / p.lowres;
+  return b;
+}
+
+// Template band-pass factor of apply_weighting_function (k:263-275).
+double bandpass_template(int r, const Band& b) {
+  if (r < b.hr && r >= 0) return 1.0;
+  if (r >= b.hr && r < b.hr + 8) return 0.5 * std::cos(PI * (r - (double)b.hr) / (2 * 8)) + 0.5;
+  if (r >= (b.lr - 8) && r < b.lr && r >= 0)
+    return 0.5 * std::cos(PI * ((double)b.lr - r) / (2 * 8)) + 0.5;
+  return 0.0;
+}
+
+// Image band-pass factor of whiten_filter_weight_Img (k:183-189).
+double bandpass_image(int r, const Band& b) {
+  if (r < b.hr && r >= b.lr) return 1.0;
+  if (r >= b.hr && r < b.hr + 8) return 0.5 * std::cos(PI * (r - (double)b.hr) / (2 * 8)) + 0.5;
+  if (r >= (b.lr - 8) && r < b.lr && r >= 0)
+    return 0.5 * std::cos(PI * ((double)b.lr - r) / (2 * 8)) + 0.5;
+  return 0.0;
+}
+
+// Zero-padded centring of LoadTemplate (src/norm.cu:132-154).
+void pad_template(const float* t, int w, int h, int l, cd* out) {
+  std::fill(out, out + (size_t)l * l, cd(0));
+  const int sx = (l - w) / 2, sy = (l - h) / 2;
+  for (int j = 0; j < h; ++j)
+    for (int i = 0; i < w; ++i) out[(size_t)(sy + j) * l + (sx + i)] = cd(t[(size_t)j * w + i], 0);
+}
+
+// Edge-mean / box-sigma standardisation: generate_mask, multiCount_dot, UpdateSigma,
+// scale_each (k:4-90) with the host reductions of src/norm.cu:194-280.
+void standardise_template(cd* t, int l) {
+  const float r = l / 2.f - 2.f;
+  float up = 0, low = 0;
+  if (r > 1) {
+    up = (r + 1) * (r + 1);
+    low = (r - 1) * (r - 1);
+  }
+  double ring = 0, cnt = 0, s = 0, s2 = 0;
+  for (int y = 0; y < l; ++y)
+    for (int x = 0; x < l; ++x) {
+      const float rr = (float)((x - l / 2) * (x - l / 2) + (y - l / 2) * (y - l / 2));
+      const double v = t[(size_t)y * l + x].real();
+      if (rr >= low && rr <= up) {
+        ring += v;
+        cnt += 1;
+      }
+      s += v;
+      s2 += v * v;
+    }
+  const double em = cnt != 0 ? ring / cnt : ring;
+  const double mean = s / ((double)l * l);
+  double sigma = std::sqrt(s2 / ((double)l * l) - mean * mean);
+  if (sigma < 0 || !std::isfinite(sigma)) sigma = 0;
+  if (sigma < EPS && sigma > -EPS) return;
+  for (size_t i = 0; i < (size_t)l * l; ++i) t[i] = cd((t[i].real() - em) / sigma, t[i].imag());
+}
+
+// SQRSum_by_circle (k:94-126) + whiten_Tmp (k:128-149) on one l x l spectrum.
+void whiten_template(cd* F, int l) {
+  std::vector<double> ra(l / 2, 0.0), rb(l / 2, 0.0);
+  for (int y = 0; y < l; ++y)
+    for (int x = 0; x <= l / 2; ++x) {
+      const int r = rbin(std::min(x, l - x), std::min(y, l - y));
+      if (r >= 0 && r < l / 2) {
+        ra[r] += std::norm(F[(size_t)y * l + x]);
+        rb[r] += 1;
+      }
+    }
+  for (int y = 0; y < l; ++y)
+    for (int x = 0; x < l; ++x) {
+      const int r = rbin(std::min(x, l - x), std::min(y, l - y));
+      if (r >= 0 && r < l / 2) F[(size_t)y * l + x] /= std::sqrt(ra[r] / rb[r]);
+    }
+}
+
+// apply_weighting_function (k:243-288): band-pass then |CTF| Wiener-like weight.
+void weight_template(cd* F, int l, const P& p) {
+  const Band b = band(l, p);
+  const double ds = (double)(1.0f / (p.apix * (float)l));  // para.ds, src/norm.cu:191
+  for (int y = 0; y < l; ++y)
+    for (int x = 0; x < l; ++x) {
+      const int fx = std::min(x, l - x), fy = std::min(y, l - y);
+      const double r = std::hypot((double)fy, (double)fx);
+      const int rr = rbin(fx, fy);
+      double f = bandpass_template(rr, b);
+      if (rr < l / 2 && rr >= 0) {
+        const double ss = r * ds;
+        const double v = ctf_ast(x, (y + l / 2) % l, l, l, p, 2);
+        const double signal =
+            std::exp(p.ph.bf * ss * ss + p.ph.bf2 * ss + p.ph.bf3) / ((double)p.kk + 1);
+        const double Ncurve = std::exp(p.ph.a * ss * ss + p.ph.b * ss + p.ph.b2) / signal;
+        f *= v * std::sqrt(1 / (Ncurve + (double)p.kk * v * v));
+      }
+      F[(size_t)y * l + x] *= f;
+    }
+}
+
+// compute_area_sum_ofSQR (k:319-382) + host mean (src/norm.cu:334-342) + normalize (k:445-458).
+void spectral_normalise(cd* F, int l) {
+  double s = 0, cnt = 0;
+  for (int y = 0; y < l; ++y)
+    for (int x = 0; x <= l / 2; ++x) {
+      const int r = rbin(std::min(x, l - x), std::min(y, l - y));
+      if (r < l / 2 && r >= 0) {
+        s += std::norm(F[(size_t)y * l + x]);
+        cnt += 1;
+      }
+    }
+  const double m = std::sqrt(s / (cnt * cnt));
+  if (m != 0)
+    for (size_t i = 0; i < (size_t)l * l; ++i) F[i] /= m;
+}
+
+// apply_mask (k:225-241): soft disc on the real part.
+void apply_mask(cd* t, int l, double d_m_cfg, double ehw) {
+  const float d_m = 1.5f * (float)d_m_cfg;
+  const double rout = (double)(d_m / 2 + 2 * (float)ehw), rin = (double)(d_m / 2);
+  for (int y = 0; y < l; ++y)
+    for (int x = 0; x < l; ++x) {
+      const double r = std::hypot((double)(x - l / 2), (double)(y - l / 2));
+      cd& v = t[(size_t)y * l + x];
+      if (r > rout)
+        v = cd(0, v.imag());
+      else if (r >= rin)
+        v = cd(v.real() * (0.5 * std::cos(PI * (r - rin) / (2 * ehw)) + 0.5), v.imag());
+    }
+}
+
+void scale_all(cd* x, size_t n, double s) {
+  for (size_t i = 0; i < n; ++i) x[i] *= s;
+}
+
+// Rotation geometry of rotate_subIMG (k:534-603), evaluated in float exactly as the HIP path
+// does; returns false outside [0,l-1]^2.
+struct Bil {
+  int k0, k1, k2, k3;
+  float t, u;
+};
+inline bool bilinear_coords(int i, int j, int l, float cose, float sine, Bil& o) {
+  const float x = (float)(i - l / 2), y = (float)(j - l / 2);
+  const float half = (float)(l / 2);
+  const float x2 = std::fmaf(cose, x, sine * y) + half;
+  const float y2 = std::fmaf(-sine, x, cose * y) + half;
+  if (x2 < 0 || x2 > l - 1.0f || y2 < 0 || y2 > l - 1.0f) return false;
+  const int ii = (int)std::floor(x2), jj = (int)std::floor(y2);
+  o.k0 = ii + jj * l;
+  o.k1 = o.k0 + 1;
+  o.k2 = o.k0 + l + 1;
+  o.k3 = o.k0 + l;
+  if (ii == l - 1) { o.k1 -= 1; o.k2 -= 1; }
+  if (jj == l - 1) { o.k2 -= l; o.k3 -= l; }
+  o.t = x2 - (float)ii;
+  o.u = y2 - (float)jj;
+  return true;
+}
+
+// cos/sin of e*PI/180 rounded to float (k:538-539: double cos of a double argument).
+inline void rot_cs(float e, float& c, float& s) {
+  c = (float)std::cos((double)e * PI / 180);
+  s = (float)std::sin((double)e * PI / 180);
+}
+
+void rotate_plane(const cd* in, cd* out, int l, float theta) {
+  float cose, sine;
+  rot_cs(theta, cose, sine);
+  for (int j = 0; j < l; ++j)
+    for (int i = 0; i < l; ++i) {
+      Bil b;
+      cd r = 0;
+      if (bilinear_coords(i, j, l, cose, sine, b)) {
+        const double t = b.t, u = b.u, tt = 1.0 - t, uu = 1.0 - u;
+        r = in[b.k0] * (tt * uu) + in[b.k1] * (t * uu) + in[b.k3] * (tt * u) + in[b.k2] * (t * u);
+      }
+      out[(size_t)j * l + i] = r;
+    }
+}
+
+// Checkerboard of compute_corner_CCG (k:669-677): -1 where the reference negates.
+inline double checker(int x, int y, int l) {
+  const int of = (l / 2) % 2;
+  const int st = (of == y % 2) ? 1 : 0;
+  return ((x - st) % 2 == 0) ? -1.0 : 1.0;
+}
+
+std::vector<int> block_offsets(int length, int l, int overlap) {  // src/norm.cu:68-81
+  std::vector<int> o{0};
+  int off = 0;
+  while (off + l < length) {
+    off += l - overlap;
+    if (off + l >= length) off = length - l;
+    o.push_back(off);
+  }
+  return o;
+}
+
+std::vector<float> angle_list(float phistep) {  // src/norm.cu:642
+  std::vector<float> a;
+  for (float e = 0.0f; e < 360.0f; e += phistep) a.push_back(e);
+  return a;
+}
+
+// Template preprocessing, norm (src/norm.cu:194-354) or nonorm (src/nonorm.cu:178-344).
+void prep_templates(const orc_config* c, const P& p, const float* tmpl, int n, int w, int h,
+                    cd* out) {
+  const int l = c->window_size;
+  const size_t L2 = (size_t)l * l;
+#pragma omp parallel for schedule(dynamic)
+  for (int t = 0; t < n; ++t) {
+    cd* T = out + (size_t)t * L2;
+    pad_template(tmpl + (size_t)t * w * h, w, h, l, T);
+    standardise_template(T, l);
+    orc::fft2(T, l, l, -1);
+    scale_all(T, L2, 1.0 / (double)L2);
+    whiten_template(T, l);
+    if (!c->norm_type) {  // src/nonorm.cu:291-299
+      orc::fft2(T, l, l, +1);
+      apply_mask(T, l, p.d_m, p.ph.edge_half_width);
+      orc::fft2(T, l, l, -1);
+      scale_all(T, L2, 1.0 / (double)L2);
+    }
+    weight_template(T, l, p);
+    spectral_normalise(T, l);
+    if (c->norm_type) orc::fft2(T, l, l, +1);  // src/norm.cu:350
+  }
+}
+
+// Whole-frame preprocessing: src/norm.cu:356-402 (dc_zero=1) / src/nonorm.cu:346-389.
+void prep_image(const P& p, const float* img, int nx, int ny, bool dc_zero, double* out) {
+  const size_t N = (size_t)nx * ny;
+  std::vector<cd> F(N);
+  for (size_t i = 0; i < N; ++i) F[i] = cd(img[i], 0);
+  orc::fft2(F.data(), ny, nx, -1);
+  scale_all(F.data(), N, 1.0 / (double)N);
+  const int l = std::max(nx, ny);
+  // do_phase_flip (k:828-839)
+#pragma omp parallel for schedule(static)
+  for (int y = 0; y < ny; ++y)
+    for (int x = 0; x < nx; ++x)
+      F[(size_t)y * nx + x] *= ctf_ast(x, (y + ny / 2) % ny, nx, ny, p, 0);
+  // SQRSum_by_circle mode 1 (k:94-126)
+  std::vector<double> ra(l / 2, 0.0), rb(l / 2, 0.0);
+  for (int y = 0; y < ny; ++y)
+    for (int x = 0; x <= nx / 2; ++x) {
+      const int r = rbin(std::min(x, nx - x), std::min(y, ny - y));
+      if (r < l / 2 && r >= 0) {
+        ra[r] += std::norm(F[(size_t)y * nx + x]);
+        rb[r] += 1;
+      }
+    }
+  // whiten_filter_weight_Img (k:151-202)
+  const Band b = band(l, p);
+  const float hi_thr = (float)l * p.apix / 6;
+#pragma omp parallel for schedule(static)
+  for (int y = 0; y < ny; ++y)
+    for (int x = 0; x < nx; ++x) {
+      const double dx = std::min(x, nx - x), dy = std::min(y, ny - y);
+      const int r = rbin((long long)dx, (long long)dy);
+      const double ss =
+          std::sqrt((dx * dx / (double)(nx * nx) + dy * dy / (double)(ny * ny)) /
+                    ((double)p.apix * p.apix));
+      double f = 1.0, v = 0, signal = 0, Ncurve = 0;
+      if (r < l / 2 && r >= 0) {
+        v = ctf_ast(x, (y + ny / 2) % ny, nx, ny, p, 2);
+        signal = std::exp(p.ph.bf * ss * ss + p.ph.bf2 * ss + p.ph.bf3);
+        Ncurve = std::exp(p.ph.a * ss * ss + p.ph.b * ss + p.ph.b2);
+        f *= std::sqrt((signal * v * v + Ncurve) / signal) / std::sqrt(ra[r] / rb[r]);
+        if (r > hi_thr) f *= std::exp(-100 * ss * ss);
+      }
+      f *= bandpass_image(r, b);
+      if (r < l / 2 && r >= 0) {
+        signal /= ((double)p.kk + 1);
+        Ncurve /= signal;
+        f *= std::sqrt(1 / (Ncurve + (double)p.kk * v * v));
+      }
+      F[(size_t)y * nx + x] *= f;
+    }
+  if (dc_zero) F[0] = 0;  // set_0Hz_to_0_at_RI (k:204-223)
+  orc::fft2(F.data(), ny, nx, +1);
+  for (size_t i = 0; i < N; ++i) out[i] = F[i].real();
+}
+
+// Host clipping of LoadImage (src/norm.cu:156-181): float sequential sums, int bounds.
+void clip_image(float* img, size_t N, int nx, int ny, int invert) {
+  float sum = 0, sum_s2 = 0;
+  for (size_t i = 0; i < N; ++i) {
+    const float cur = img[i];
+    sum += cur / nx / ny;
+    sum_s2 += (cur * cur / nx / ny);
+  }
+  const float avg = sum;
+  const float var = std::sqrt(sum_s2 - avg * avg);
+  const int up = (int)(avg + 6 * var);
+  const int low = (int)(avg - 6 * var);
+  if (sum_s2 > avg * avg)
+    for (size_t i = 0; i < N; ++i)
+      if (img[i] > up || img[i] < low) img[i] = avg;
+  if (invert)
+    for (size_t i = 0; i < N; ++i) img[i] = -img[i];
+}
+
+// Rotate + mask + standardise + forward FFT of one processed template (RotateTemplate,
+// src/norm.cu:479-521).
+void rotate_template(const cd* Tp, cd* out, int l, float theta, const P& p) {
+  const size_t L2 = (size_t)l * l;
+  rotate_plane(Tp, out, l, theta);
+  apply_mask(out, l, p.d_m, p.ph.edge_half_width);
+  double s = 0, s2 = 0;
+  for (size_t i = 0; i < L2; ++i) {
+    const double v = out[i].real();
+    s += v;
+    s2 += v * v;
+  }
+  const double mean = s / (double)L2;
+  const double var = s2 / (double)L2 - mean * mean;
+  for (size_t i = 0; i < L2; ++i) {
+    double v = out[i].real();
+    if (mean != 0) v -= mean;
+    if (var != 0) v /= std::sqrt(var);
+    out[i] = cd(v, out[i].imag());
+  }
+  orc::fft2(out, l, l, -1);
+  scale_all(out, L2, 1.0 / (double)L2);
+}
+
+// Block spectra (SplitImage, src/norm.cu:404-477): Fourier-space, spectrally normalised.
+void split_blocks(const double* img, int nx, int ny, int l, const std::vector<int>& ox,
+                  const std::vector<int>& oy, cd* out) {
+  const size_t L2 = (size_t)l * l;
+  const int bx = (int)ox.size(), by = (int)oy.size();
+#pragma omp parallel for schedule(dynamic)
+  for (int b = 0; b < bx * by; ++b) {
+    cd* B = out + (size_t)b * L2;
+    const int i = b % bx, j = b / bx;
+    for (int y = 0; y < l; ++y)
+      for (int x = 0; x < l; ++x) {
+        const int X = ox[i] + x, Y = oy[j] + y;
+        B[(size_t)y * l + x] = (X < nx && Y < ny) ? cd(img[(size_t)Y * nx + X], 0) : cd(0);
+      }
+    orc::fft2(B, l, l, -1);
+    scale_all(B, L2, 1.0 / (double)L2);
+    spectral_normalise(B, l);
+  }
+}
+
+void correlate(const cd* I, const cd* T, cd* C, int l) {
+  for (int y = 0; y < l; ++y)
+    for (int x = 0; x < l; ++x) {
+      const size_t k = (size_t)y * l + x;
+      C[k] = I[k] * std::conj(T[k]) * checker(x, y, l);
+    }
+  orc::fft2(C, l, l, +1);
+}
+
+bool pick_less(const orc_pick& a, const orc_pick& b) {  // NMS order: a before b
+  if (a.score != b.score) return a.score > b.score;
+  if (a.y != b.y) return a.y < b.y;
+  if (a.x != b.x) return a.x < b.x;
+  if (a.template_id != b.template_id) return a.template_id < b.template_id;
+  if (a.angle_index != b.angle_index) return a.angle_index < b.angle_index;
+  if (a.block != b.block) return a.block < b.block;
+  return a.pixel < b.pixel;
+}
+
+int nms(orc_pick* c, int n, float radius) {
+  std::stable_sort(c, c + n, pick_less);
+  if (radius <= 0) return n;
+  const double R2 = (double)radius * radius;
+  int kept = 0;
+  for (int i = 0; i < n; ++i) {
+    bool ok = true;
+    for (int k = 0; k < kept && ok; ++k) {
+      const double dx = (double)c[i].x - c[k].x, dy = (double)c[i].y - c[k].y;
+      if (dx * dx + dy * dy < R2) ok = false;
+    }
+    if (ok) c[kept++] = c[i];
+  }
+  return kept;
+}
+
+void check_config(const orc_config* c) {
+  if (!c) throw std::invalid_argument("null config");
+  if (c->window_size <= 0 || c->window_size % 2) throw std::invalid_argument("window_size must be even and > 0");
+  if (c->phistep <= 0) throw std::invalid_argument("phistep must be > 0");
+}
+
+}  // namespace
+
+extern "C" {
+
+int orc_fft1d(double* data, int n, int sign) {
+  try {
+    std::vector<cd> w(n);
+    orc::fft(reinterpret_cast<cd*>(data), w.data(), n, sign);
+    return 0;
+  } catch (...) { return -1; }
+}
+
+int orc_fft2d(double* data, int ny, int nx, int sign) {
+  try {
+    orc::fft2(reinterpret_cast<cd*>(data), ny, nx, sign);
+    return 0;
+  } catch (...) { return -1; }
+}
+
+double orc_ctf(int x, int y, int nx, int ny, double apix, double defocus, double dfdiff,
+               double dfang, double energy, double cs, double ampconst, int mode) {
+  orc_config c{};
+  c.angpix = (float)apix;
+  c.energy = (float)energy;
+  c.cs = (float)cs;
+  const double ctf3[3] = {defocus, dfdiff, dfang};
+  P p = make_params(&c, ctf3);
+  p.ph.ampconst = ampconst;
+  return ctf_ast(x, y, nx, ny, p, mode);
+}
+
+int orc_radial_bin(int dx, int dy) { return rbin(dx, dy); }
+
+int orc_block_offsets(int length, int l, int overlap, int* out, int max_out) {
+  auto o = block_offsets(length, l, overlap);
+  for (int i = 0; i < (int)o.size() && i < max_out; ++i) out[i] = o[i];
+  return (int)o.size();
+}
+
+// TileImages geometry (src/utils/tileimages.cpp:18-70): upper-left corners, x-major order.
+int orc_tile_geometry(int W, int H, int* ul_x, int* ul_y, int max_out) {
+  const int cols = (int)std::ceil((double)W / TILE), rows = (int)std::ceil((double)H / TILE);
+  const int x_step = W / cols, y_step = H / rows;
+  int k = 0;
+  for (int j = 0; j < cols; ++j) {
+    int cx = j == 0 ? j * TILE + TILE / 2 : (j == cols - 1 ? W - TILE / 2 : x_step * (2 * j + 1) / 2);
+    for (int i = 0; i < rows; ++i) {
+      int cy = i == 0 ? i * TILE + TILE / 2 : (i == rows - 1 ? H - TILE / 2 : y_step * (2 * i + 1) / 2);
+      if (k < max_out) {
+        ul_x[k] = std::max(cx - TILE / 2, 0);
+        ul_y[k] = std::max(cy - TILE / 2, 0);
+      }
+      ++k;
+    }
+  }
+  return k;
+}
+
+int orc_angles(float phistep, float* out, int max_out) {
+  auto a = angle_list(phistep);
+  for (int i = 0; i < (int)a.size() && i < max_out; ++i) out[i] = a[i];
+  return (int)a.size();
+}
+
+int orc_prep_templates(const orc_config* c, const float* tmpl, int n, int w, int h,
+                       const double* ctf3, double* out) {
+  try {
+    check_config(c);
+    P p = make_params(c, ctf3);
+    prep_templates(c, p, tmpl, n, w, h, reinterpret_cast<cd*>(out));
+    return 0;
+  } catch (...) { return -1; }
+}
+
+int orc_clip_image(float* img, int W, int H, int invert) {
+  clip_image(img, (size_t)W * H, W, H, invert);
+  return 0;
+}
+
+int orc_prep_image(const orc_config* c, const float* img, int W, int H, const double* ctf3,
+                   double* out) {
+  try {
+    check_config(c);
+    P p = make_params(c, ctf3);
+    prep_image(p, img, W, H, c->norm_type != 0, out);
+    return 0;
+  } catch (...) { return -1; }
+}
+
+int orc_rotate_templates(const orc_config* c, const double* prepped, int n, float theta,
+                         double* out) {
+  try {
+    check_config(c);
+    const double ctf3[3] = {1, 0, 0};
+    P p = make_params(c, ctf3);
+    const int l = c->window_size;
+    const size_t L2 = (size_t)l * l;
+    const cd* in = reinterpret_cast<const cd*>(prepped);
+    cd* o = reinterpret_cast<cd*>(out);
+#pragma omp parallel for schedule(dynamic)
+    for (int t = 0; t < n; ++t) rotate_template(in + t * L2, o + t * L2, l, theta, p);
+    return 0;
+  } catch (...) { return -1; }
+}
+
+int orc_search_norm(const orc_config* c, const float* image, int W, int H, const float* tmpl,
+                    int n, int w, int h, const double* ctf3, orc_pick* picks, int max_picks,
+                    double* mu_out, double* sigma_out, double* best_out, int32_t* arg_out,
+                    int n_angles_limit, int n_templates_limit) {
+  try {
+    check_config(c);
+    P p = make_params(c, ctf3);
+    const int l = c->window_size;
+    const size_t L2 = (size_t)l * l;
+    if (W < 1 || H < 1 || n < 1) throw std::invalid_argument("empty input");
+    // A7: clip + invert on a private copy (the reference mutates the caller's pixels)
+    std::vector<float> img(image, image + (size_t)W * H);
+    clip_image(img.data(), img.size(), W, H, c->invert);
+    std::vector<double> pre((size_t)W * H);
+    prep_image(p, img.data(), W, H, true, pre.data());
+    const auto ox = block_offsets(W, l, c->overlap), oy = block_offsets(H, l, c->overlap);
+    const int nb = (int)(ox.size() * oy.size());
+    std::vector<cd> blocks((size_t)nb * L2);
+    split_blocks(pre.data(), W, H, l, ox, oy, blocks.data());
+    std::vector<cd> T((size_t)n * L2);
+    prep_templates(c, p, tmpl, n, w, h, T.data());
+
+    auto angles = angle_list(c->phistep);
+    int na = (int)angles.size();
+    if (n_angles_limit > 0) na = std::min(na, n_angles_limit);
+    const int nt = n_templates_limit > 0 ? std::min(n, n_templates_limit) : n;
+    // running per-(block,pixel) moments and best correlation
+    std::vector<double> S((size_t)nb * L2, 0.0), SS((size_t)nb * L2, 0.0),
+        best((size_t)nb * L2, -INFINITY);
+    std::vector<int32_t> arg((size_t)nb * L2, -1);
+    std::vector<cd> R((size_t)nt * L2);
+    for (int a = 0; a < na; ++a) {
+#pragma omp parallel for schedule(dynamic)
+      for (int t = 0; t < nt; ++t) rotate_template(T.data() + t * L2, R.data() + t * L2, l, angles[a], p);
+#pragma omp parallel
+      {
+        std::vector<cd> C(L2);
+#pragma omp for schedule(dynamic)
+        for (int b = 0; b < nb; ++b) {
+          double* Sb = S.data() + b * L2;
+          double* SSb = SS.data() + b * L2;
+          double* Bb = best.data() + b * L2;
+          int32_t* Ab = arg.data() + b * L2;
+          for (int t = 0; t < nt; ++t) {
+            correlate(blocks.data() + b * L2, R.data() + t * L2, C.data(), l);
+            const int32_t id = a * n + t;
+            for (size_t k = 0; k < L2; ++k) {
+              const double s = C[k].real() / (double)L2;  // add_CCG_to_sum (k:691)
+              Sb[k] += s;
+              SSb[k] += s * s;
+              if (s > Bb[k]) { Bb[k] = s; Ab[k] = id; }
+            }
+          }
+        }
+      }
+    }
+    // set_CCG_mean (k:697-706): n = N_t * int(360/phistep) (src/norm.cu:549-550)
+    double total_n = (double)n * (int)(360 / c->phistep);
+    if (n_angles_limit > 0 || n_templates_limit > 0) total_n = (double)nt * na;
+    std::vector<orc_pick> cand;
+    const float dm = p.d_m;
+    for (int b = 0; b < nb; ++b) {
+      const int bi = b % (int)ox.size(), bj = b / (int)ox.size();
+      for (size_t k = 0; k < L2; ++k) {
+        const size_t q = b * L2 + k;
+        const double m = S[q] / total_n;
+        const double sd = std::sqrt(SS[q] / total_n - m * m);
+        const double z = sd > 0 ? (best[q] - m) / sd : 0.0;  // update_CCG (k:709-721)
+        if (mu_out) mu_out[q] = m;
+        if (sigma_out) sigma_out[q] = sd;
+        if (best_out) best_out[q] = best[q];
+        if (arg_out) arg_out[q] = arg[q];
+        if (z >= p.thresh) {  // src/norm.cu:585-590
+          const int cx = ox[bi] + (int)(k % l), cy = oy[bj] + (int)(k / l);
+          if (cx >= dm && cx < W - dm && cy >= dm && cy < H - dm) {
+            orc_pick pk{};
+            pk.x = (float)cx;
+            pk.y = (float)cy;
+            pk.template_id = arg[q] % n;
+            pk.angle_index = arg[q] / n;
+            float e = 360.0f - angles[pk.angle_index];  // src/norm.cu:654-655
+            if (e >= 360.0f) e -= 360.0f;
+            pk.psi = e;
+            pk.score = (float)z;
+            pk.block = b;
+            pk.pixel = (int)k;
+            pk.local_x = (float)(k % l);
+            pk.local_y = (float)(k / l);
+            cand.push_back(pk);
+          }
+        }
+      }
+    }
+    const int kept = nms(cand.data(), (int)cand.size(), c->nms_radius);
+    for (int i = 0; i < kept && i < max_picks; ++i) picks[i] = cand[i];
+    return kept;
+  } catch (...) { return -1; }
+}
+
+int orc_search_nonorm(const orc_config* c, const float* image, int W, int H, const float* tmpl,
+                      int n, int w, int h, const double* ctf3, orc_pick* recs, int max_recs,
+                      int* n_recs, orc_pick* picks, int max_picks) {
+  try {
+    check_config(c);
+    P p = make_params(c, ctf3);
+    const int l = c->window_size;
+    const size_t L2 = (size_t)l * l;
+    if (W < TILE || H < TILE) throw std::invalid_argument("norm_type=0 needs W,H >= 720");
+    if (l > TILE) throw std::invalid_argument("window_size must be <= 720");
+    // blocks inside a tile (src/nonorm.cu:71-85), uniform placement (k:605-624)
+    const int nblk = (int)block_offsets(TILE, l, c->overlap).size();
+    std::vector<cd> T((size_t)n * L2);
+    prep_templates(c, p, tmpl, n, w, h, T.data());
+    std::vector<int> ulx(64), uly(64);
+    int ntiles = orc_tile_geometry(W, H, ulx.data(), uly.data(), 0);
+    ulx.resize(ntiles);
+    uly.resize(ntiles);
+    orc_tile_geometry(W, H, ulx.data(), uly.data(), ntiles);
+    const auto angles = angle_list(c->phistep);
+    const float dm4 = p.d_m / 4;
+    std::vector<orc_pick> all;
+    for (int tile = 0; tile < ntiles; ++tile) {
+      std::vector<float> tdata((size_t)TILE * TILE);
+      for (int r = 0; r < TILE; ++r)
+        std::memcpy(&tdata[(size_t)r * TILE], image + (size_t)(uly[tile] + r) * W + ulx[tile],
+                    sizeof(float) * TILE);
+      if (c->invert)
+        for (auto& v : tdata) v = -v;  // src/nonorm.cu:157-165
+      std::vector<double> pre((size_t)TILE * TILE);
+      if (c->phase_flip == 1)
+        prep_image(p, tdata.data(), TILE, TILE, false, pre.data());
+      else
+        for (size_t i = 0; i < pre.size(); ++i) pre[i] = tdata[i];
+      // SplitImage (src/nonorm.cu:391-450)
+      std::vector<cd> blk((size_t)nblk * nblk * L2);
+      for (int b = 0; b < nblk * nblk; ++b) {
+        cd* B = blk.data() + b * L2;
+        const int bi = b % nblk, bj = b / nblk;
+        for (int y = 0; y < l; ++y)
+          for (int x = 0; x < l; ++x) {
+            const int X = bi * (l - c->overlap) + x, Y = bj * (l - c->overlap) + y;
+            B[(size_t)y * l + x] = (X < TILE && Y < TILE) ? cd(pre[(size_t)Y * TILE + X], 0) : cd(0);
+          }
+        if (c->phase_flip == 1) {
+          orc::fft2(B, l, l, -1);
+          scale_all(B, L2, 1.0 / (double)L2);
+          spectral_normalise(B, l);
+          orc::fft2(B, l, l, +1);
+        }
+      }
+      for (int a = 0; a < (int)angles.size(); ++a) {
+        const float e = angles[a];
+        // PickParticles (src/nonorm.cu:452-537)
+        std::vector<cd> rot(blk.size());
+        for (int b = 0; b < nblk * nblk; ++b) {
+          rotate_plane(blk.data() + b * L2, rot.data() + b * L2, l, e);
+          orc::fft2(rot.data() + b * L2, l, l, -1);
+          scale_all(rot.data() + b * L2, L2, 1.0 / (double)L2);
+        }
+        std::vector<double> sc((size_t)n, 0.0), cxs(n, 0.0), cys(n, 0.0);
+        std::vector<int> pk_pix(n, 0), pk_blk(n, 0);
+        const double ce = std::cos((double)e * PI / 180), se = std::sin((double)e * PI / 180);
+        for (int b = 0; b < nblk * nblk; ++b) {
+          const int bi = b % nblk, bj = b / nblk;
+          std::vector<double> peak(n), pos(n), sum(n), sum2(n);
+#pragma omp parallel
+          {
+            std::vector<cd> C(L2);
+#pragma omp for schedule(dynamic)
+            for (int t = 0; t < n; ++t) {
+              correlate(rot.data() + b * L2, T.data() + t * L2, C.data(), l);
+              double pk = 0, s = 0, s2 = 0;
+              long long ps = 0;
+              for (int y = 0; y < l; ++y)
+                for (int x = 0; x < l; ++x) {
+                  const double v = C[(size_t)y * l + x].real();
+                  s += v;
+                  s2 += v * v;
+                  const bool border = x < dm4 || x > l - dm4 || y < dm4 || y > l - dm4;
+                  if (!border && v > pk) {  // strictly greater: first maximum wins
+                    pk = v;
+                    ps = (long long)y * l + x;
+                  }
+                }
+              peak[t] = pk;
+              pos[t] = (double)ps;
+              sum[t] = s;
+              sum2[t] = s2;
+            }
+          }
+          for (int t = 0; t < n; ++t) {
+            const double ra = sum[t] - peak[t], rb = sum2[t] - peak[t] * peak[t];
+            const double rc = (double)L2 - 1;
+            const double sd = std::sqrt(rb / rc - (ra / rc) * (ra / rc));
+            const double score = sd == 0 ? 0.0 : peak[t] / sd;
+            const int cx = (int)pos[t] % l, cy = (int)pos[t] / l;
+            const float centerx = (float)(bi * (l - c->overlap) + (cx - l / 2) * ce + (cy - l / 2) * se + l / 2);
+            const float centery = (float)(bj * (l - c->overlap) + (cy - l / 2) * ce - (cx - l / 2) * se + l / 2);
+            if (sc[t] < score && centerx >= 0 && centerx < TILE && centery >= 0 && centery < TILE) {
+              sc[t] = score;
+              cxs[t] = centerx;
+              cys[t] = centery;
+              pk_pix[t] = (int)pos[t];
+              pk_blk[t] = b;
+            }
+          }
+        }
+        for (int t = 0; t < n; ++t)
+          if (sc[t] >= p.thresh) {  // OutputScore (src/nonorm.cu:539-557)
+            orc_pick r{};
+            r.local_x = (float)cxs[t];
+            r.local_y = (float)cys[t];
+            r.x = (float)(ulx[tile] + cxs[t]);
+            r.y = (float)(uly[tile] + cys[t]);
+            r.template_id = t;
+            r.angle_index = a;
+            r.psi = e;
+            r.score = (float)sc[t];
+            r.block = tile;
+            r.pixel = pk_blk[t] * (int)L2 + pk_pix[t];
+            all.push_back(r);
+          }
+      }
+    }
+    if (n_recs) *n_recs = (int)all.size();
+    for (int i = 0; i < (int)all.size() && i < max_recs; ++i) recs[i] = all[i];
+    const int kept = nms(all.data(), (int)all.size(), c->nms_radius);
+    for (int i = 0; i < kept && i < max_picks; ++i) picks[i] = all[i];
+    return kept;
+  } catch (...) { return -1; }
+}
+
+int orc_nms(orc_pick* cand, int n, float radius) { return nms(cand, n, radius); }
+
+int orc_num_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+void orc_set_num_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+#else
+  (void)n;
+#endif
+}
+
+}  // extern "C"

@@ -1,0 +1,95 @@
+/*
+ * FP64 CPU oracle for the GisSPA template-matching search.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library restates, in double precision, the
+ * scoring of the reference (src/kernels.cu, src/norm.cu, src/nonorm.cu) so that
+ * the HIP path can be checked against it.  Only tests/, the smoke check and the
+ * CPU-baseline leg of bench.py may load it; the product path never does.
+ *
+ * Conventions shared with the HIP library (include/gisspa.h):
+ *  - images are row-major float32, width W (x, fastest) by height H (y);
+ *  - templates are a stack of n row-major w x h float32 images;
+ *  - CTF parameters are (defocus um, dfdiff um, dfang deg) as in the LST file
+ *    (src/EMReader/DataReader2.cpp:80-101).
+ */
+#pragma once
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Search parameters: the reference's config keys (include/EMReader/DataReader2.h:17-44,73-83). */
+typedef struct {
+  float angpix, phistep, kk, energy, cs, highres, lowres, diameter, threshold;
+  int window_size, overlap, norm_type, invert, phase_flip;
+  float nms_radius; /* radius of the greedy non-maximum suppression, px (0 = off) */
+} orc_config;
+
+/* One pick: centre in micrograph pixels, template index, in-plane angle as printed by the
+ * reference (norm: 360-theta, src/norm.cu:654-655; nonorm: theta, src/nonorm.cu:584),
+ * score, plus bookkeeping used by the tests. */
+typedef struct {
+  float x, y;
+  int32_t template_id;
+  float psi;
+  float score;
+  int32_t angle_index;  /* position of theta in the angle loop */
+  int32_t block;        /* norm: block index; nonorm: tile index */
+  int32_t pixel;        /* norm: in-block pixel index; nonorm: in-block peak index */
+  float local_x, local_y; /* nonorm: tile-local centre as the reference prints it */
+} orc_pick;
+
+/* ---- FFT (mixed radix, unnormalised, sign -1 forward / +1 inverse) ---- */
+int orc_fft1d(double* data, int n, int sign);          /* interleaved complex, in place */
+int orc_fft2d(double* data, int ny, int nx, int sign); /* row-major ny x nx, in place */
+
+/* ---- scalar pieces ---- */
+double orc_ctf(int x, int y, int nx, int ny, double apix, double defocus, double dfdiff,
+               double dfang, double energy, double cs, double ampconst, int mode);
+int orc_radial_bin(int dx, int dy); /* floor(hypot(dx,dy)+0.5)-1, exact */
+int orc_block_offsets(int length, int l, int overlap, int* out, int max_out);
+int orc_tile_geometry(int W, int H, int* ul_x, int* ul_y, int max_out);
+int orc_angles(float phistep, float* out, int max_out); /* float loop of src/norm.cu:642 */
+
+/* ---- stages ---- */
+/* Template preprocessing.  norm (norm_type=1): real-space processed templates (n*l*l, re/im
+ * interleaved).  nonorm: Fourier-space processed templates (n*l*l complex). */
+int orc_prep_templates(const orc_config* c, const float* tmpl, int n, int w, int h,
+                       const double* ctf3, double* out);
+/* Host clipping of src/norm.cu:156-181 (norm only; modifies img in place). */
+int orc_clip_image(float* img, int W, int H, int invert);
+/* Whole-frame preprocessing of src/norm.cu:356-402 (norm) or src/nonorm.cu:346-389 (nonorm,
+ * on one tile).  img is the already clipped/inverted image; out is W*H real. */
+int orc_prep_image(const orc_config* c, const float* img, int W, int H, const double* ctf3,
+                   double* out);
+/* Rotated, masked, standardised template spectra for angle theta (norm): n*l*l complex. */
+int orc_rotate_templates(const orc_config* c, const double* prepped, int n, float theta,
+                         double* out);
+
+/* ---- full searches ---- */
+/* norm_type=1 search with per-(block,pixel) best records (single pass; equivalent to the
+ * reference's two passes because z is monotone in the correlation at fixed pixel).
+ * Optional outputs (may be NULL): mu/sigma/best/arg per block pixel (nb*l*l each) where
+ * arg = angle_index*n + template.  n_angles_limit/n_templates_limit (<=0 = all) restrict the
+ * work for timing samples; mu/sigma then use the restricted counts.  Returns #picks. */
+int orc_search_norm(const orc_config* c, const float* image, int W, int H, const float* tmpl,
+                    int n, int w, int h, const double* ctf3, orc_pick* picks, int max_picks,
+                    double* mu, double* sigma, double* best, int32_t* arg, int n_angles_limit,
+                    int n_templates_limit);
+/* norm_type=0 search: every (tile, theta, template) record above threshold (the reference's
+ * output lines) into recs, then NMS picks.  Returns #picks; *n_recs gets #records. */
+int orc_search_nonorm(const orc_config* c, const float* image, int W, int H, const float* tmpl,
+                      int n, int w, int h, const double* ctf3, orc_pick* recs, int max_recs,
+                      int* n_recs, orc_pick* picks, int max_picks);
+/* Greedy NMS: sort by score (desc; ties by y, x, template, angle), keep a candidate if no kept
+ * pick lies strictly closer than radius.  In place; returns the number kept. */
+int orc_nms(orc_pick* cand, int n, float radius);
+
+int orc_num_threads(void);
+void orc_set_num_threads(int n);
+
+#ifdef __cplusplus
+}
+#endif
