@@ -1,0 +1,169 @@
+"""Benchmark of the template-matching search on one MI355X.
+
+Workload (the production configuration of the headline metric): one 5760 x 4092 float32
+micrograph with 400 inserted particles at SNR 0.05, 1000 templates of 180 x 180 from a uniform
+projection-direction grid, in-plane step 5 deg (72 angles), norm_type=1 on the whole image,
+window 320 (234 blocks), defocus 2.0 um, astigmatism 50 A (dfdiff 0.005 um) at 30 deg.
+
+A step is one full search (template and image preprocessing, 72 x 1000 x 234 correlation planes,
+per-pixel statistics, records and NMS) with the image and templates already resident in HBM.
+value = template.angle.pixel correlations per second = 1000 * 72 * 5760 * 4092 / s.
+
+The roofline entry is for the dominant correlation kernel: its algorithmic bytes per launch
+(template half-spectrum read + inter-pass buffer write for the column pass; inter-pass buffer
+read for the row pass) over its average launch time, from HIP events recorded around every
+launch on the engine's stream during the timed steps.  The CPU baseline is the FP64 oracle on
+a bounded sample of the same workload (see cpu_baseline.sample).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "gisspa_v2.0_amd"))
+
+import numpy as np  # noqa: E402
+
+METRIC = "template·angle·pixel correlations/s + % HBM peak; refined peaks/s at tol"
+HBM_PEAK = 8.0e12
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--angles", type=int, default=0, help="limit angles (exploration only)")
+    ap.add_argument("--templates", type=int, default=1000)
+    ap.add_argument("--chunk", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-templates", type=int, default=2)
+    ap.add_argument("--cpu-angles", type=int, default=3)
+    return ap.parse_args()
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(cfg, ds, nt, na):
+    """FP64 oracle, marginal rate between 1 and 1+na angles on the first nt templates."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    oracle.load()
+    cores = oracle.num_threads()
+    tmpl = ds.templates[:nt]
+    times = []
+    for a in (1, 1 + na):
+        t0 = time.perf_counter()
+        oracle.search_norm(cfg, ds.image, tmpl, ds.ctf, n_angles=a)
+        times.append(time.perf_counter() - t0)
+        log(f"cpu oracle: {nt} templates x {a} angles: {times[-1]:.2f} s")
+    H, W = ds.image.shape
+    rate = nt * na * W * H / max(times[1] - times[0], 1e-9)
+    return {
+        "value": rate,
+        "unit": "correlations/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": (f"FP64 oracle on the same micrograph, first {nt} templates, marginal time of "
+                   f"{na} extra angles over all blocks (runs with 1 and {1 + na} angles: "
+                   f"{times[0]:.1f} s and {times[1]:.1f} s)"),
+    }
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported in this build "
+              "(the search shards by micrograph or block with no exchange; see DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+    import gisspa
+    import synth
+
+    t0 = time.perf_counter()
+    ds = synth.make_dataset(1001, 5760, 4092, n_templates=a.templates, tsize=180, n_particles=400,
+                            snr=0.05, defocus=2.0, dfdiff=0.005, dfang=30.0, grid=True)
+    log(f"dataset ready in {time.perf_counter() - t0:.1f} s")
+    cfg = gisspa.SearchConfig(angpix=1.0, phistep=5.0, kk=3.0, energy=300.0, cs=2.7, highres=8.0,
+                              lowres=100.0, diameter=128.0, threshold=7.0, window_size=320,
+                              overlap=0, norm_type=1, invert=0, phase_flip=0, nms_radius=64.0)
+    eng = gisspa.Engine(cfg, device=0)
+    eng.load_templates(ds.templates, ds.eulers[:, :2])
+    eng.set_image(ds.image, ds.ctf)
+    if a.angles:
+        eng.set_limits(a.angles, 0)
+    if a.chunk:
+        eng.set_chunk(a.chunk)
+    for i in range(a.warmup):
+        t1 = time.perf_counter()
+        eng.run()
+        log(f"warmup {i}: {time.perf_counter() - t1:.2f} s")
+    eng.set_profiling(True)
+    t1 = time.perf_counter()
+    for i in range(a.steps):
+        eng.run()
+        log(f"step {i} done")
+    dt = time.perf_counter() - t1
+    recs = eng.records()
+    col_ms, col_n = eng.kernel_timing(0)
+    row_ms, row_n = eng.kernel_timing(1)
+    total_ms, corr_ms = eng.timing()
+    H, W = ds.image.shape
+    n_ang = a.angles if a.angles else int(round(360 / cfg.phistep))
+    work = a.templates * n_ang * W * H
+    value = work * a.steps / dt
+    l = cfg.window_size
+    nb = 18 * 13
+    kx = l // 2 + 1
+    # planes per launch = templates per launch x blocks; launches per step = angles x chunks
+    launches_per_step = col_n / a.steps
+    planes_per_launch = a.templates * n_ang * nb / launches_per_step
+    col_bytes = planes_per_launch * (8 * l * kx + 8 * l * (l // 2))
+    row_bytes = planes_per_launch * (8 * l * (l // 2))
+    dominant = "column_pass" if col_ms >= row_ms else "row_pass"
+    dom_ms, dom_bytes = (col_ms, col_bytes) if col_ms >= row_ms else (row_ms, row_bytes)
+    achieved = dom_bytes / (dom_ms * 1e-3)
+    stage_bytes = planes_per_launch * 24 * l * kx
+    stage_bw = stage_bytes / ((col_ms + row_ms) * 1e-3)
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "correlations/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": dt * 1e3 / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (seeded blob-density projections, CTF, N(0,1) noise)",
+        "config": {
+            "workload": "plain search, production size: 5760x4092 fp32 micrograph, 1000 templates "
+                        "180x180, 72 in-plane angles, norm_type=1, window 320 (234 blocks)",
+            "templates": a.templates, "angles": n_ang, "image": [W, H], "window_size": l,
+            "blocks": nb, "picks": int(len(recs)),
+        },
+        "roofline": {
+            "bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK, "traffic": None, "kernel": dominant,
+            "avg_launch_ms": dom_ms, "algorithmic_bytes_per_launch": dom_bytes,
+            "column_pass_ms": col_ms, "row_pass_ms": row_ms,
+            "correlation_stage_frac": stage_bw / HBM_PEAK,
+        },
+        "pipeline_ms": {"device_total": total_ms, "angle_loop": corr_ms},
+    }
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(cfg, ds, a.cpu_templates, a.cpu_angles)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

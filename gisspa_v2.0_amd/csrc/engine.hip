@@ -153,10 +153,34 @@ static void nms(std::vector<gs_record>& v, float radius) {
   v.swap(kept);
 }
 
+// Event pairs around every launch of one kernel; read back once the run has synchronised, so
+// timing adds no host synchronisation inside the pipeline.
 struct KernelClock {
-  hipEvent_t a = nullptr, b = nullptr;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+  size_t used = 0;
   double total_ms = 0;
   long long launches = 0;
+  void reset() {
+    total_ms = 0;
+    launches = 0;
+    used = 0;
+  }
+  void harvest() {
+    for (size_t i = 0; i < used; ++i) {
+      float ms = 0;
+      GS_HIP(hipEventElapsedTime(&ms, ev[i].first, ev[i].second));
+      total_ms += ms;
+    }
+    launches += (long long)used;
+    used = 0;
+  }
+  void destroy() {
+    for (auto& e : ev) {
+      (void)hipEventDestroy(e.first);
+      (void)hipEventDestroy(e.second);
+    }
+    ev.clear();
+  }
 };
 
 class Engine {
@@ -178,10 +202,8 @@ class Engine {
     if (stream_) (void)hipStreamSynchronize(stream_);
     for (hipEvent_t e : {ev_start_, ev_corr0_, ev_corr1_, ev_end_})
       if (e) (void)hipEventDestroy(e);
-    for (KernelClock* k : {&clk_col_, &clk_row_}) {
-      if (k->a) (void)hipEventDestroy(k->a);
-      if (k->b) (void)hipEventDestroy(k->b);
-    }
+    clk_col_.destroy();
+    clk_row_.destroy();
     if (stream_) (void)hipStreamDestroy(stream_);
   }
 
@@ -243,6 +265,8 @@ class Engine {
     total_ms_ = ms;
     GS_HIP(hipEventElapsedTime(&ms, ev_corr0_, ev_corr1_));
     corr_ms_ = ms;
+    clk_col_.harvest();
+    clk_row_.harvest();
     if (P_.norm_type)
       collect_norm();
     else
@@ -294,7 +318,11 @@ class Engine {
     lim_angles_ = na;
     lim_templates_ = nt;
   }
-  void set_profiling(bool on) { profile_ = on; }
+  void set_profiling(bool on) {
+    profile_ = on;
+    clk_col_.reset();
+    clk_row_.reset();
+  }
   void timing(double* total, double* corr) const {
     if (total) *total = total_ms_;
     if (corr) *corr = corr_ms_;
@@ -437,20 +465,18 @@ class Engine {
 
   void clock_begin(KernelClock& k) {
     if (!profile_) return;
-    if (!k.a) {
-      GS_HIP(hipEventCreate(&k.a));
-      GS_HIP(hipEventCreate(&k.b));
+    if (k.used == k.ev.size()) {
+      std::pair<hipEvent_t, hipEvent_t> p{nullptr, nullptr};
+      GS_HIP(hipEventCreate(&p.first));
+      GS_HIP(hipEventCreate(&p.second));
+      k.ev.push_back(p);
     }
-    GS_HIP(hipEventRecord(k.a, stream_));
+    GS_HIP(hipEventRecord(k.ev[k.used].first, stream_));
   }
   void clock_end(KernelClock& k) {
     if (!profile_) return;
-    GS_HIP(hipEventRecord(k.b, stream_));
-    GS_HIP(hipEventSynchronize(k.b));
-    float ms = 0;
-    GS_HIP(hipEventElapsedTime(&ms, k.a, k.b));
-    k.total_ms += ms;
-    k.launches += 1;
+    GS_HIP(hipEventRecord(k.ev[k.used].second, stream_));
+    k.used += 1;
   }
 
   int chunk_for(int nb) const {

@@ -1,0 +1,159 @@
+// Host-side inputs of the search executable (see io.hpp).
+#include "io.hpp"
+
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+
+namespace gsio {
+
+Config::Config() {
+  using namespace std::string_literals;
+  value = {
+      {"input", ""s},       {"template", ""s},   {"template_dims", 2},  {"eulerfile", ""s},
+      {"angpix", -1.0f},    {"phistep", -1.0f},  {"kk", -1.0f},         {"energy", -1.0f},
+      {"cs", -1.0f},        {"Highres", -1.0f},  {"Lowres", 0.0f},      {"diameter", -1.0f},
+      {"threshold", 7.0f},  {"output", ""s},     {"first", 0},          {"last", 0},
+      {"GPU_ID", -1},       {"window_size", 320}, {"phase_flip", 0},    {"overlap", 0},
+      {"norm_type", 0},     {"invert", 0},
+      // new keys
+      {"nms_radius", 0.0f}, {"tile_local_centers", 0},
+  };
+}
+
+bool Config::parse(const std::string& path) {
+  std::ifstream f(path);
+  if (!f) {
+    errors.push_back("cannot open config file " + path);
+    return false;
+  }
+  std::string key, eq, val;
+  while (f >> key >> eq >> val) {
+    if (eq != "=") {
+      errors.push_back("malformed line near key " + key);
+      return false;
+    }
+    auto it = value.find(key);
+    if (it == value.end()) continue;
+    try {
+      if (std::holds_alternative<int>(it->second))
+        it->second = std::stoi(val);
+      else if (std::holds_alternative<float>(it->second))
+        it->second = std::stof(val);
+      else
+        it->second = val;
+    } catch (const std::exception&) {
+      errors.push_back("bad value for " + key + ": " + val);
+      return false;
+    }
+  }
+  return true;
+}
+
+std::vector<std::string> Config::missing_required() const {  // checkRequestPara
+  std::vector<std::string> m;
+  for (const char* k : {"input", "template", "eulerfile"})
+    if (gets(k).empty()) m.push_back(k);
+  for (const char* k : {"angpix", "phistep", "kk", "energy", "cs", "Highres", "Lowres", "diameter"})
+    if (getf(k) < 0) m.push_back(k);
+  return m;
+}
+
+std::vector<LstEntry> load_lst(const std::string& path) {
+  std::vector<LstEntry> out;
+  std::ifstream f(path);
+  if (!f) return out;
+  std::string line;
+  while (std::getline(f, line)) {
+    if (!line.empty() && line[0] == '#') continue;
+    LstEntry e;
+    char buf[1024] = {0};
+    std::sscanf(line.c_str(), "%d %1023s defocus=%lf dfdiff=%lf dfang=%lf", &e.unused, buf,
+                &e.defocus, &e.dfdiff, &e.dfang);
+    e.rpath = buf;
+    out.push_back(e);
+  }
+  return out;
+}
+
+Euler load_euler(const std::string& path) {
+  Euler e;
+  std::ifstream f(path);
+  if (!f) return e;
+  std::string line;
+  float alt, az, phi;
+  while (std::getline(f, line)) {
+    if (std::sscanf(line.c_str(), "%*f %f %f %f", &alt, &az, &phi) == 3 ||
+        std::sscanf(line.c_str(), "%f %f %f", &alt, &az, &phi) == 3 ||
+        std::sscanf(line.c_str(), "%*d %*s %f %f %f", &alt, &az, &phi) == 3) {
+      e.alt.push_back(alt);
+      e.az.push_back(az);
+      e.phi.push_back(phi);
+    }
+  }
+  return e;
+}
+
+bool read_mrc(const std::string& path, Mrc& out, std::string& err) {
+  std::ifstream f(path, std::ios::binary);
+  if (!f) {
+    err = "cannot open " + path;
+    return false;
+  }
+  int32_t hdr[256];
+  if (!f.read(reinterpret_cast<char*>(hdr), 1024)) {
+    err = "truncated MRC header: " + path;
+    return false;
+  }
+  bool swap = false;
+  auto bs = [](int32_t v) { return (int32_t)__builtin_bswap32((uint32_t)v); };
+  if (hdr[3] < 0 || hdr[3] > 16) {  // mode out of range: try the other byte order
+    swap = true;
+    for (int i = 0; i < 256; ++i) hdr[i] = bs(hdr[i]);
+  }
+  out.nx = hdr[0];
+  out.ny = hdr[1];
+  out.nz = hdr[2];
+  const int mode = hdr[3];
+  const int ext = hdr[23];
+  if (out.nx <= 0 || out.ny <= 0 || out.nz <= 0 || ext < 0) {
+    err = "bad MRC dimensions: " + path;
+    return false;
+  }
+  const size_t n = (size_t)out.nx * out.ny * out.nz;
+  int bytes = mode == 0 ? 1 : (mode == 1 || mode == 6) ? 2 : mode == 2 ? 4 : 0;
+  if (!bytes) {
+    err = "unsupported MRC mode " + std::to_string(mode) + ": " + path;
+    return false;
+  }
+  f.seekg(1024 + ext);
+  std::vector<char> raw(n * bytes);
+  if (!f.read(raw.data(), raw.size())) {
+    err = "truncated MRC data: " + path;
+    return false;
+  }
+  out.data.resize(n);
+  for (size_t i = 0; i < n; ++i) {
+    const char* p = raw.data() + i * bytes;
+    if (mode == 0) {
+      out.data[i] = (float)(int8_t)p[0];
+    } else if (mode == 1 || mode == 6) {
+      uint16_t u;
+      std::memcpy(&u, p, 2);
+      if (swap) u = __builtin_bswap16(u);
+      out.data[i] = mode == 1 ? (float)(int16_t)u : (float)u;
+    } else {
+      uint32_t u;
+      std::memcpy(&u, p, 4);
+      if (swap) u = __builtin_bswap32(u);
+      float v;
+      std::memcpy(&v, &u, 4);
+      out.data[i] = v;
+    }
+  }
+  return true;
+}
+
+}  // namespace gsio

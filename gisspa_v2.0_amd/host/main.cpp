@@ -1,0 +1,116 @@
+// gisspa_main <config>: the config-driven search executable (reference src/main.cu:14-98).
+//
+// Reads the config, the LST list of micrographs with CTF parameters, the Euler file and the
+// template stack, then searches micrographs [first, last) on one GPU and writes one record line
+// per pick in the reference's format (src/norm.cu:610-615).  Differences from the reference:
+// GPU_ID = -1 uses device 0 (one GPU per process; shard micrographs with first/last), errors
+// are reported with a non-zero exit status instead of being printed and ignored, and records
+// pass through NMS when nms_radius > 0.
+#include <chrono>
+#include <cstdio>
+#include <fstream>
+#include <string>
+#include <vector>
+
+#include "gisspa.h"
+#include "io.hpp"
+
+static int die(const std::string& msg) {
+  std::fprintf(stderr, "gisspa_main: %s\n", msg.c_str());
+  return 1;
+}
+
+int main(int argc, char** argv) {
+  if (argc < 2) return die("usage: gisspa_main <config>");
+  gsio::Config conf;
+  if (!conf.parse(argv[1])) return die(conf.errors.empty() ? "bad config" : conf.errors[0]);
+  auto missing = conf.missing_required();
+  if (!missing.empty()) {
+    std::string m = "missing required config keys:";
+    for (auto& k : missing) m += " " + k;
+    return die(m);
+  }
+  auto lst = gsio::load_lst(conf.gets("input"));
+  if (lst.empty()) return die("empty or unreadable LST file " + conf.gets("input"));
+  auto euler = gsio::load_euler(conf.gets("eulerfile"));
+  if (euler.size() == 0) return die("empty or unreadable Euler file " + conf.gets("eulerfile"));
+  // micrograph range, src/main.cu:25-26
+  const int first = std::max(0, std::min(conf.geti("first"), (int)lst.size() - 1));
+  const int last = std::min(conf.geti("last"), std::max(0, (int)lst.size()));
+  int device = conf.geti("GPU_ID");
+  if (device < 0) device = 0;
+
+  gsio::Mrc tm;
+  std::string err;
+  if (!gsio::read_mrc(conf.gets("template"), tm, err)) return die(err);
+  if ((size_t)tm.nz < euler.size())
+    return die("template stack has " + std::to_string(tm.nz) + " images, Euler file " +
+               std::to_string(euler.size()));
+  const int n = (int)euler.size();
+  std::vector<float> altaz(2 * n);
+  for (int i = 0; i < n; ++i) {
+    altaz[2 * i] = euler.alt[i];
+    altaz[2 * i + 1] = euler.az[i];
+  }
+
+  gs_params p{};
+  p.angpix = conf.getf("angpix");
+  p.phistep = conf.getf("phistep");
+  p.kk = conf.getf("kk");
+  p.energy = conf.getf("energy");
+  p.cs = conf.getf("cs");
+  p.highres = conf.getf("Highres");
+  p.lowres = conf.getf("Lowres");
+  p.diameter = conf.getf("diameter");
+  p.threshold = conf.getf("threshold");
+  p.window_size = conf.geti("window_size");
+  p.overlap = conf.geti("overlap");
+  p.norm_type = conf.geti("norm_type");
+  p.invert = conf.geti("invert");
+  p.phase_flip = conf.geti("phase_flip");
+  p.nms_radius = conf.getf("nms_radius");
+  const int tile_local = conf.geti("tile_local_centers");
+
+  gs_engine* eng = nullptr;
+  if (gs_create(&p, device, &eng) != GS_OK) return die(gs_last_error());
+  if (gs_load_templates(eng, tm.data.data(), n, tm.nx, tm.ny, altaz.data()) != GS_OK) {
+    std::string e = gs_last_error();
+    gs_destroy(eng);
+    return die(e);
+  }
+  std::ofstream out(conf.gets("output"), std::ios::out | std::ios::trunc);
+  if (!out) {
+    gs_destroy(eng);
+    return die("cannot write output " + conf.gets("output"));
+  }
+  int status = 0;
+  long long lines = 0;
+  for (int i = first; i < last; ++i) {
+    const auto& e = lst[i];
+    gsio::Mrc img;
+    if (!gsio::read_mrc(e.rpath, img, err)) {
+      status = die(err);
+      break;
+    }
+    gs_ctf ctf{e.defocus, e.dfdiff, e.dfang};
+    gs_record* recs = nullptr;
+    size_t nrec = 0;
+    auto t0 = std::chrono::steady_clock::now();
+    if (gs_search(eng, img.data.data(), img.nx, img.ny, &ctf, &recs, &nrec) != GS_OK) {
+      status = die(e.rpath + ": " + gs_last_error());
+      break;
+    }
+    char buf[2048];
+    for (size_t k = 0; k < nrec; ++k) {
+      gs_format_record(eng, &recs[k], e.unused, e.rpath.c_str(), &ctf, tile_local, buf, sizeof buf);
+      out << buf;
+      ++lines;
+    }
+    gs_free(recs);
+    auto ms = std::chrono::duration_cast<std::chrono::milliseconds>(std::chrono::steady_clock::now() - t0).count();
+    std::printf("%s: %zu records in %lld ms\n", e.rpath.c_str(), nrec, (long long)ms);
+  }
+  std::printf("output line count: %lld\n", lines);
+  gs_destroy(eng);
+  return status;
+}

@@ -821,6 +821,15 @@ int orc_search_nonorm(const orc_config* c, const float* image, int W, int H, con
 
 int orc_nms(orc_pick* cand, int n, float radius) { return nms(cand, n, radius); }
 
+int orc_correlate(const double* I, const double* T, int l, double* out) {
+  try {
+    std::vector<cd> C((size_t)l * l);
+    correlate(reinterpret_cast<const cd*>(I), reinterpret_cast<const cd*>(T), C.data(), l);
+    std::memcpy(out, C.data(), sizeof(cd) * C.size());
+    return 0;
+  } catch (...) { return -1; }
+}
+
 int orc_num_threads(void) {
 #ifdef _OPENMP
   return omp_get_max_threads();

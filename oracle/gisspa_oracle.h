@@ -86,6 +86,9 @@ int orc_search_nonorm(const orc_config* c, const float* image, int W, int H, con
 /* Greedy NMS: sort by score (desc; ties by y, x, template, angle), keep a candidate if no kept
  * pick lies strictly closer than radius.  In place; returns the number kept. */
 int orc_nms(orc_pick* cand, int n, float radius);
+/* compute_corner_CCG + inverse transform (k:645-678): out = IDFT_unnorm(I conj(T) checker),
+ * complex l x l planes. */
+int orc_correlate(const double* I, const double* T, int l, double* out);
 
 int orc_num_threads(void);
 void orc_set_num_threads(int n);

@@ -63,6 +63,7 @@ def load():
         "orc_search_nonorm": (C.c_int, [C.POINTER(Config), _P, C.c_int, C.c_int, _P, C.c_int, C.c_int,
                                         C.c_int, _P, _P, C.c_int, _P, _P, C.c_int]),
         "orc_nms": (C.c_int, [_P, C.c_int, C.c_float]),
+        "orc_correlate": (C.c_int, [_P, _P, C.c_int, _P]),
         "orc_num_threads": (C.c_int, []),
         "orc_set_num_threads": (None, [C.c_int]),
     }
@@ -197,6 +198,14 @@ def nms(cands: np.ndarray, radius: float) -> np.ndarray:
     a = np.ascontiguousarray(cands, dtype=PICK_DTYPE).copy()
     n = load().orc_nms(a.ctypes.data, a.size, radius)
     return a[:n].copy()
+
+
+def correlate(I: np.ndarray, T: np.ndarray) -> np.ndarray:
+    a = np.ascontiguousarray(I, dtype=np.complex128)
+    b = np.ascontiguousarray(T, dtype=np.complex128)
+    out = np.zeros_like(a)
+    _ok(load().orc_correlate(a.ctypes.data, b.ctypes.data, a.shape[0], out.ctypes.data))
+    return out
 
 
 def num_threads():
