@@ -78,21 +78,57 @@ __global__ void __launch_bounds__(CorrCfg<L>::CT)
   const int tid = threadIdx.x;
 
   for (int i = tid; i < L; i += NT) tw[i] = twL[i];
+  __syncthreads();
   const float2* Tp = that + (long long)(t0 + tl) * PLANE;
   const float2* Ip = ihat + (long long)b * PLANE;
   const int kx0 = cg * COLS;
-  for (int q = tid; q < COLS * L; q += NT) {
-    const int c = q / L, ky = q - c * L;
-    const long long off = (long long)kx0 * L + q;
-    float2 p = c_mulconj(Ip[off], Tp[off]);
-    if (kx0 == 0 && c == 0) {  // pack the Nyquist column as the imaginary part
-      const float2 pn = c_mulconj(Ip[(long long)H * L + ky], Tp[(long long)H * L + ky]);
-      p = make_float2(p.x - pn.y, p.y + pn.x);
+  // each wavefront loads and transforms its own CPW columns (contiguous in the spectra)
+  constexpr int CPW = 64 / TPC;
+  const int wave = tid / 64, lane = tid % 64;
+  const int cw0 = wave * CPW;
+  // issue every load of the wave's columns before using any, for memory-level parallelism
+  constexpr int NLD = (CPW * L + 63) / 64;
+  constexpr int NDC = (L + 63) / 64;  // loads covering column 0 (q < L)
+  const float2* Tw = Tp + (long long)(kx0 + cw0) * L;
+  const float2* Iw = Ip + (long long)(kx0 + cw0) * L;
+  float2 tv[NLD], iv[NLD];
+#pragma unroll
+  for (int i = 0; i < NLD; ++i) {
+    const int q = lane + 64 * i;
+    if ((CPW * L) % 64 == 0 || q < CPW * L) {
+      tv[i] = Tw[q];
+      iv[i] = Iw[q];
     }
-    buf[c * LP + ky] = p;
   }
-  __syncthreads();
-  ct::run_stockham<L, TPC, +1>(buf + (tid / TPC) * LP, tw, tid % TPC, typename C::ColPlan{});
+  const bool dc = (kx0 + cw0 == 0);  // this wave holds column 0: pack the Nyquist column
+  float2 tn[NDC], in_[NDC];
+  if (dc) {
+#pragma unroll
+    for (int i = 0; i < NDC; ++i) {
+      const int ky = lane + 64 * i;
+      if (ky < L) {
+        tn[i] = Tp[(long long)H * L + ky];
+        in_[i] = Ip[(long long)H * L + ky];
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < NLD; ++i) {
+    const int q = lane + 64 * i;
+    if ((CPW * L) % 64 == 0 || q < CPW * L) {
+      const int cl = q / L, ky = q - cl * L;
+      float2 p = c_mulconj(iv[i], tv[i]);
+      if (i < NDC && dc && cl == 0) {  // column 0 is q < L, i.e. load i < NDC with ky = q
+        const float2 pn = c_mulconj(in_[i < NDC ? i : 0], tn[i < NDC ? i : 0]);
+        p = make_float2(p.x - pn.y, p.y + pn.x);
+      }
+      buf[(cw0 + cl) * LP + ky] = p;
+    }
+  }
+  ct::lds_sync<ct::Sync::WAVE>();
+  ct::run_stockham<L, TPC, +1, ct::Sync::WAVE>(buf + (tid / TPC) * LP, tw, tid % TPC,
+                                                typename C::ColPlan{});
+  __syncthreads();  // rows of Z gather all COLS columns of the workgroup
 
   float2* Zp = Z + ((long long)tl * nb + b) * (long long)L * H;
   for (int q = tid; q < COLS * L; q += NT) {
@@ -261,6 +297,134 @@ __global__ void __launch_bounds__(CorrCfg<L>::RT)
   }
 }
 
+// Row pass, norm_type=1, wavefront-private rows: each wavefront owns RW rows of the block and
+// its own LDS slice, so the template loop needs no workgroup barrier.  Per iteration a wave
+// transforms RW rows x TPI templates (TPT lanes per length-H transform); each lane folds the
+// same PPL pixels of its rows into registers every iteration.  The real-to-complex packing
+// loads each Hermitian pair X(k), X(H-k) once and writes both packed values.  PF: load the next
+// iteration's pairs before transforming the current one (more registers, fewer stalls).
+template <int L, bool PF>
+__global__ void __launch_bounds__(256)
+    k_rowpass_pix_wave(const float2* __restrict__ Z, const float2* __restrict__ twL,
+                       const float2* __restrict__ twH, int tc, int nb, double* __restrict__ S,
+                       double* __restrict__ SS, float* __restrict__ best, int32_t* __restrict__ arg,
+                       int arg_base) {
+  using C = CorrCfg<L>;
+  constexpr int H = C::H, TPI = C::TPI, TPT = C::TPT;
+  constexpr int TPW = 64 / TPT;  // transforms per wave
+  constexpr int RW = TPW / TPI;  // rows per wave
+  constexpr int WAVES = 4;
+  constexpr int ROWS = RW * WAVES;
+  constexpr int PPL = RW * L / 64;
+  static_assert(TPW % TPI == 0 && RW * L % 64 == 0, "wave row mapping");
+  static_assert((H / 2) % TPT == 0, "paired packing");
+  __shared__ float2 buf[WAVES][TPW * H];
+  __shared__ float2 twh[H];
+  __shared__ float2 twc[H];
+
+  const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
+  for (int i = tid; i < H; i += 256) {
+    twh[i] = twH[i];
+    const float2 w = twL[i];
+    twc[i] = make_float2(w.x, -w.y);  // exp(+2 pi i k / L)
+  }
+  __syncthreads();
+  const int b = blockIdx.y;
+  const int y0 = blockIdx.x * ROWS + wave * RW;
+  float2* wb = buf[wave];
+  const int tr = lane / TPT, sub = lane % TPT;
+  const int rowl = tr / TPI, ti = tr % TPI;
+  float2* dst = wb + tr * H;
+
+  float s[PPL], ss[PPL], bs[PPL];
+  int ag[PPL];
+#pragma unroll
+  for (int i = 0; i < PPL; ++i) {
+    s[i] = 0.f;
+    ss[i] = 0.f;
+    bs[i] = -INFINITY;
+    ag[i] = -1;
+  }
+
+  // Z values of the next iteration are loaded into registers while the current one is
+  // transformed and folded (software pipelining of the only HBM stream of this kernel).
+  constexpr int NP = H / 2 / TPT;
+  float2 pa[NP], pb[NP], p0 = make_float2(0.f, 0.f);
+  auto prefetch = [&](int t) {
+    if (t + ti < tc) {
+      const float2* zr = Z + ((((long long)(t + ti) * nb + b) * L) + y0 + rowl) * H;
+      if (sub == 0) p0 = zr[0];
+#pragma unroll
+      for (int m = 0; m < NP; ++m) {
+        const int k = 1 + sub + TPT * m;  // 1 .. H/2
+        pa[m] = zr[k];
+        pb[m] = zr[H - k];
+      }
+    }
+  };
+  if constexpr (PF) prefetch(0);
+  for (int t = 0; t < tc; t += TPI) {
+    if constexpr (!PF) prefetch(t);
+    if (t + ti < tc) {
+      if (sub == 0) dst[0] = make_float2(p0.x + p0.y, p0.x - p0.y);  // X(0), X(H) real, slot 0
+#pragma unroll
+      for (int m = 0; m < NP; ++m) {
+        const int k = 1 + sub + TPT * m;
+        const float2 A = pa[m], Bv = pb[m];
+        {
+          const float2 Bc = c_conj(Bv);
+          const float2 sm = c_add(A, Bc), df = c_sub(A, Bc);
+          const float2 wd = c_mul(twc[k], df);
+          dst[k] = make_float2(sm.x - wd.y, sm.y + wd.x);
+        }
+        if (k != H - k) {
+          const float2 Ac = c_conj(A);
+          const float2 sm = c_add(Bv, Ac), df = c_sub(Bv, Ac);
+          const float2 wd = c_mul(twc[H - k], df);
+          dst[H - k] = make_float2(sm.x - wd.y, sm.y + wd.x);
+        }
+      }
+    }
+    if constexpr (PF) prefetch(t + TPI);
+    ct::lds_sync<ct::Sync::WAVE>();
+    ct::run_stockham<H, TPT, +1, ct::Sync::WAVE>(dst, twh, sub, typename C::RowPlan{});
+    const float* out = reinterpret_cast<const float*>(wb);
+#pragma unroll
+    for (int i = 0; i < PPL; ++i) {
+      const int q = lane + 64 * i;
+      const int r2 = q / L, x = q - r2 * L;
+#pragma unroll
+      for (int u = 0; u < TPI; ++u) {
+        if (t + u < tc) {
+          const float v = out[(r2 * TPI + u) * L + x];
+          s[i] += v;
+          ss[i] = fmaf(v, v, ss[i]);
+          if (v > bs[i]) {
+            bs[i] = v;
+            ag[i] = t + u;
+          }
+        }
+      }
+    }
+    ct::lds_sync<ct::Sync::WAVE>();
+  }
+
+#pragma unroll
+  for (int i = 0; i < PPL; ++i) {
+    const int q = lane + 64 * i;
+    const int r2 = q / L, x = q - r2 * L;
+    int px = x + L / 2;
+    if (px >= L) px -= L;
+    const long long gi = ((long long)b * L + (y0 + r2)) * L + px;
+    S[gi] += (double)s[i];
+    SS[gi] += (double)ss[i];
+    if (bs[i] > best[gi]) {
+      best[gi] = bs[i];
+      arg[gi] = arg_base + ag[i];
+    }
+  }
+}
+
 __global__ void k_init_acc(double* S, double* SS, float* best, int32_t* arg, long long count) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= count) return;
@@ -410,10 +574,16 @@ static void rowpass_impl(const float2* Z, int tc, int nb, double* S, double* SS,
 }
 
 bool launch_rowpass_pixel(int L, const float2* Z, int tc, int nb, double* S, double* SS,
-                          float* best, int32_t* arg, int arg_base, hipStream_t s) {
+                          float* best, int32_t* arg, int arg_base, int variant, hipStream_t s) {
   TwTables& T = tw_tables(current_device());
   if (L == 320) {
-    rowpass_impl<320, false>(Z, tc, nb, S, SS, best, arg, arg_base, nullptr, 0.f, T.t320, T.t160, s);
+    constexpr int ROWS = 4 * (64 / CorrCfg<320>::TPT) / CorrCfg<320>::TPI;
+    dim3 grid(320 / ROWS, nb);
+    if (variant == 1)
+      k_rowpass_pix_wave<320, true><<<grid, 256, 0, s>>>(Z, T.t320, T.t160, tc, nb, S, SS, best, arg, arg_base);
+    else
+      k_rowpass_pix_wave<320, false><<<grid, 256, 0, s>>>(Z, T.t320, T.t160, tc, nb, S, SS, best, arg, arg_base);
+    GS_CHECK_LAUNCH();
     return true;
   }
   if (L == 720) {

@@ -333,6 +333,10 @@ class Engine {
     if (count) *count = k.launches;
   }
   void set_chunk(int tc) { chunk_ = tc; }
+  void set_variant(int kernel, int variant) {
+    GS_REQUIRE(kernel == 1 && (variant == 0 || variant == 1), kInvalid, "unknown kernel variant");
+    row_variant_ = variant;
+  }
 
   const gs_params& params() const { return P_; }
   float alt(int j) const { return (j >= 0 && j < (int)alt_.size()) ? alt_[j] : 0.f; }
@@ -546,7 +550,8 @@ class Engine {
         GS_REQUIRE(launch_colpass(l, ca, stream_), kUnsupported, "window_size not supported");
         clock_end(clk_col_);
         clock_begin(clk_row_);
-        launch_rowpass_pixel(l, Z, tc, nb_, d_S_.p, d_SS_.p, d_best_.p, d_arg_.p, a * n + t0, stream_);
+        launch_rowpass_pixel(l, Z, tc, nb_, d_S_.p, d_SS_.p, d_best_.p, d_arg_.p, a * n + t0,
+                             row_variant_, stream_);
         clock_end(clk_row_);
       }
     }
@@ -729,7 +734,7 @@ class Engine {
   int nb_ = 0, bx_ = 0, nbk_ = 0, na_ = 0;
   std::vector<int> offx_, offy_, ulx_, uly_;
   std::vector<float> angles_;
-  int lim_angles_ = 0, lim_templates_ = 0, chunk_ = 0;
+  int lim_angles_ = 0, lim_templates_ = 0, chunk_ = 0, row_variant_ = 0;
   double total_ms_ = 0, corr_ms_ = 0;
   std::vector<gs_record> records_;
   std::vector<double> planes_host_;
@@ -921,6 +926,11 @@ gs_status gs_set_chunk(gs_engine* e, int templates_per_launch) {
   if (!e) return fail(GS_ERR_INVALID, "null engine");
   e->impl->set_chunk(templates_per_launch);
   return GS_OK;
+}
+
+gs_status gs_set_variant(gs_engine* e, int kernel, int variant) {
+  if (!e) return fail(GS_ERR_INVALID, "null engine");
+  return guarded([&] { e->impl->set_variant(kernel, variant); });
 }
 
 }  // extern "C"

@@ -236,9 +236,25 @@ __device__ __forceinline__ float2 tw_lookup(const float2* tw, int e) {
   return SIGN < 0 ? w : make_float2(w.x, -w.y);
 }
 
+// Ordering of LDS traffic between the read and write phases of a stage.  WAVE: all threads of
+// a transform belong to one wavefront, whose LDS operations execute in order, so only compiler
+// reordering must be prevented.  BLOCK: the transform spans wavefronts; use a workgroup barrier.
+enum class Sync { BLOCK, WAVE };
+
+template <Sync S>
+__device__ __forceinline__ void lds_sync() {
+  if constexpr (S == Sync::BLOCK) {
+    __syncthreads();
+  } else {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+}
+
 // One Stockham stage (radix R, NS = product of earlier radices) over a length-N transform in
 // buf, shared by T threads (tid in [0,T)); then recurse into the remaining radices.
-template <int N, int T, int SIGN, int NS, int R, int... REST>
+template <int N, int T, int SIGN, int NS, Sync S, int R, int... REST>
 __device__ __forceinline__ void stockham(float2* buf, const float2* tw, int tid) {
   constexpr int NB = N / R;
   constexpr int IT = (NB + T - 1) / T;
@@ -260,7 +276,7 @@ __device__ __forceinline__ void stockham(float2* buf, const float2* tw, int tid)
       Dft<R, SIGN>::run(v[it]);
     }
   }
-  __syncthreads();
+  lds_sync<S>();
 #pragma unroll
   for (int it = 0; it < IT; ++it) {
     const int j = tid + it * T;
@@ -271,8 +287,8 @@ __device__ __forceinline__ void stockham(float2* buf, const float2* tw, int tid)
       for (int r = 0; r < R; ++r) buf[idxD + r * NS] = v[it][r];
     }
   }
-  __syncthreads();
-  if constexpr (sizeof...(REST) > 0) stockham<N, T, SIGN, NS * R, REST...>(buf, tw, tid);
+  lds_sync<S>();
+  if constexpr (sizeof...(REST) > 0) stockham<N, T, SIGN, NS * R, S, REST...>(buf, tw, tid);
 }
 
 template <int... RS>
@@ -280,11 +296,12 @@ struct RadixList {
   static constexpr int product() { return (RS * ... * 1); }
 };
 
-template <int N, int T, int SIGN, int... RS>
+template <int N, int T, int SIGN, Sync S = Sync::BLOCK, int... RS>
 __device__ __forceinline__ void run_stockham(float2* buf, const float2* tw, int tid,
                                              RadixList<RS...>) {
   static_assert(RadixList<RS...>::product() == N, "radix plan must multiply to N");
-  stockham<N, T, SIGN, 1, RS...>(buf, tw, tid);
+  static_assert(S == Sync::BLOCK || (64 % T == 0), "wave-level sync needs transforms inside one wavefront");
+  stockham<N, T, SIGN, 1, S, RS...>(buf, tw, tid);
 }
 
 }  // namespace ct

@@ -75,8 +75,9 @@ struct CorrArgs {
 // Column pass: packed conj-multiply + inverse column FFTs into Z (returns false if L unsupported).
 bool launch_colpass(int L, const CorrArgs& a, hipStream_t s);
 // Row pass, norm_type=1: C2R rows + per-(block,pixel) running moments and best record.
+// variant (result-invariant): 0 plain, 1 software-prefetched Z loads.
 bool launch_rowpass_pixel(int L, const float2* Z, int tc, int nb, double* S, double* SS,
-                          float* best, int32_t* arg, int arg_base, hipStream_t s);
+                          float* best, int32_t* arg, int arg_base, int variant, hipStream_t s);
 // Row pass, norm_type=0: C2R rows + per-(template,block,row group) peak/sum partials.
 bool launch_rowpass_plane(int L, const float2* Z, int tc, int nb, float4* partial, float border,
                           hipStream_t s);

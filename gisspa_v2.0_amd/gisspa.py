@@ -70,6 +70,7 @@ SIGNATURES = {
     "gs_set_profiling": (C.c_int, [_P, C.c_int]),
     "gs_kernel_timing": (C.c_int, [_P, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]),
     "gs_set_chunk": (C.c_int, [_P, C.c_int]),
+    "gs_set_variant": (C.c_int, [_P, C.c_int, C.c_int]),
 }
 
 _lib = None
@@ -198,6 +199,9 @@ class Engine:
 
     def set_chunk(self, tc: int):
         _check(load().gs_set_chunk(self._h, tc))
+
+    def set_variant(self, kernel: int, variant: int):
+        _check(load().gs_set_variant(self._h, kernel, variant))
 
     def kernel_timing(self, which: int):
         avg = C.c_double()

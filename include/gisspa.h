@@ -141,6 +141,9 @@ gs_status gs_set_profiling(gs_engine* e, int on);
 gs_status gs_kernel_timing(const gs_engine* e, int which, double* avg_ms, long long* count);
 /* Templates per correlation launch (0 = automatic); a result-invariant tuning knob. */
 gs_status gs_set_chunk(gs_engine* e, int templates_per_launch);
+/* Selects a result-invariant code variant of a correlation kernel (kernel 1 = row pass:
+ * variant 0 plain, 1 software-prefetched loads). */
+gs_status gs_set_variant(gs_engine* e, int kernel, int variant);
 
 #ifdef __cplusplus
 }
