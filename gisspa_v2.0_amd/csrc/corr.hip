@@ -67,14 +67,25 @@ __global__ void __launch_bounds__(CorrCfg<L>::CT)
   __shared__ float2 tw[L];
 
   // Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a contiguous range of
-  // logical work so the template columns it streams stay in its own L2.
+  // logical work.  Logical order tiles the (block, template) product GEMM-style per column
+  // group: a tile of BT blocks x TT templates touches BT + TT column groups of spectra (about
+  // 1 MB), which its XCD's L2 keeps while the tile's BT*TT planes are produced.
+  constexpr int BT = 8, TT = 16;
   const unsigned G = gridDim.x, bid = blockIdx.x;
   const unsigned xcd = bid % 8u, idx = bid / 8u, per = G / 8u, rem = G % 8u;
-  const unsigned logical = (xcd < rem ? xcd * (per + 1u) : rem * (per + 1u) + (xcd - rem) * per) + idx;
-  const int cg = (int)(logical % NCG);
-  const unsigned rest = logical / NCG;
-  const int b = (int)(rest % (unsigned)nb);
-  const int tl = (int)(rest / (unsigned)nb);
+  unsigned logical = (xcd < rem ? xcd * (per + 1u) : rem * (per + 1u) + (xcd - rem) * per) + idx;
+  const unsigned ntb = (unsigned)(nb + BT - 1) / BT, ntt = (unsigned)(tc + TT - 1) / TT;
+  const int bi = (int)(logical % BT);
+  logical /= BT;
+  const int ti = (int)(logical % TT);
+  logical /= TT;
+  const int tt = (int)(logical % ntt);
+  logical /= ntt;
+  const int btile = (int)(logical % ntb);
+  const int cg = (int)(logical / ntb);
+  const int b = btile * BT + bi;
+  const int tl = tt * TT + ti;
+  if (b >= nb || tl >= tc) return;  // padding of the last tiles (uniform per workgroup)
   const int tid = threadIdx.x;
 
   for (int i = tid; i < L; i += NT) tw[i] = twL[i];
@@ -136,6 +147,94 @@ __global__ void __launch_bounds__(CorrCfg<L>::CT)
     int py = y + L / 2;
     if (py >= L) py -= L;
     Zp[(long long)py * H + kx0 + cc] = buf[cc * LP + y];
+  }
+}
+
+// Column pass, persistent over the templates of the launch: one workgroup owns one (block,
+// column group) and loops over the tc templates, so the twiddle table is staged once and the
+// image columns stay cache-resident.  PF: the next template's columns are loaded while the
+// current one is transformed.  Each wavefront transforms its own CPW columns (wave-level sync);
+// the workgroup meets once per template to write full 128-byte rows of Z.
+template <int L, bool PF>
+__global__ void __launch_bounds__(CorrCfg<L>::CT)
+    k_colpass_pers(const float2* __restrict__ that, const float2* __restrict__ ihat,
+                   const float2* __restrict__ twL, float2* __restrict__ Z, int t0, int tc, int nb) {
+  using C = CorrCfg<L>;
+  constexpr int H = C::H, COLS = C::COLS, NT = C::CT, TPC = C::TPC, LP = L + 1;
+  constexpr int NCG = H / COLS;
+  constexpr int CPW = 64 / TPC;
+  constexpr int NLD = (CPW * L + 63) / 64;
+  constexpr int NDC = (L + 63) / 64;
+  constexpr long long PLANE = (long long)(H + 1) * L;
+  static_assert(COLS * TPC == NT, "column pass thread mapping");
+  __shared__ float2 buf[COLS * LP];
+  __shared__ float2 tw[L];
+
+  const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
+  const int cg = (int)(blockIdx.x % NCG);
+  const int b = (int)(blockIdx.x / NCG);
+  for (int i = tid; i < L; i += NT) tw[i] = twL[i];
+  __syncthreads();
+  const int kx0 = cg * COLS, cw0 = wave * CPW;
+  const bool dc = (kx0 + cw0 == 0);
+  const float2* Iw = ihat + (long long)b * PLANE + (long long)(kx0 + cw0) * L;
+  const float2* In = ihat + (long long)b * PLANE + (long long)H * L;
+
+  float2 tv[NLD], iv[NLD], tn[NDC], in_[NDC];
+  auto load = [&](int tl) {
+    const float2* Tw = that + (long long)(t0 + tl) * PLANE + (long long)(kx0 + cw0) * L;
+#pragma unroll
+    for (int i = 0; i < NLD; ++i) {
+      const int q = lane + 64 * i;
+      if ((CPW * L) % 64 == 0 || q < CPW * L) {
+        tv[i] = Tw[q];
+        iv[i] = Iw[q];
+      }
+    }
+    if (dc) {
+      const float2* Tn = that + (long long)(t0 + tl) * PLANE + (long long)H * L;
+#pragma unroll
+      for (int i = 0; i < NDC; ++i) {
+        const int ky = lane + 64 * i;
+        if (ky < L) {
+          tn[i] = Tn[ky];
+          in_[i] = In[ky];
+        }
+      }
+    }
+  };
+
+  if constexpr (PF) load(0);
+  for (int tl = 0; tl < tc; ++tl) {
+    if constexpr (!PF) load(tl);
+#pragma unroll
+    for (int i = 0; i < NLD; ++i) {
+      const int q = lane + 64 * i;
+      if ((CPW * L) % 64 == 0 || q < CPW * L) {
+        const int cl = q / L, ky = q - cl * L;
+        float2 p = c_mulconj(iv[i], tv[i]);
+        if (i < NDC && dc && cl == 0) {  // pack the Nyquist column as the imaginary part
+          const float2 pn = c_mulconj(in_[i < NDC ? i : 0], tn[i < NDC ? i : 0]);
+          p = make_float2(p.x - pn.y, p.y + pn.x);
+        }
+        buf[(cw0 + cl) * LP + ky] = p;
+      }
+    }
+    if constexpr (PF) {
+      if (tl + 1 < tc) load(tl + 1);
+    }
+    ct::lds_sync<ct::Sync::WAVE>();
+    ct::run_stockham<L, TPC, +1, ct::Sync::WAVE>(buf + (tid / TPC) * LP, tw, tid % TPC,
+                                                  typename C::ColPlan{});
+    __syncthreads();
+    float2* Zp = Z + ((long long)tl * nb + b) * (long long)L * H;
+    for (int q = tid; q < COLS * L; q += NT) {
+      const int y = q / COLS, cc = q - y * COLS;
+      int py = y + L / 2;
+      if (py >= L) py -= L;
+      Zp[(long long)py * H + kx0 + cc] = buf[cc * LP + y];
+    }
+    __syncthreads();
   }
 }
 
@@ -541,22 +640,32 @@ int rowpass_groups(int L) {
 }
 
 template <int L>
-static void colpass_impl(const CorrArgs& a, const float2* twL, hipStream_t s) {
+static void colpass_impl(const CorrArgs& a, const float2* twL, int variant, hipStream_t s) {
   using C = CorrCfg<L>;
-  const long long G = (long long)(C::H / C::COLS) * a.tc * a.nb;
-  GS_REQUIRE(G > 0 && G < (1LL << 31), kInvalid, "column pass grid out of range");
-  k_colpass<L><<<(unsigned)G, C::CT, 0, s>>>(a.that, a.ihat, twL, a.Z, a.t0, a.tc, a.nb);
+  if (variant == 0) {
+    constexpr int BT = 8, TT = 16;  // tile shape of k_colpass's logical order
+    const long long G = (long long)(C::H / C::COLS) * ((a.tc + TT - 1) / TT * TT) *
+                        ((a.nb + BT - 1) / BT * BT);
+    GS_REQUIRE(G > 0 && G < (1LL << 31), kInvalid, "column pass grid out of range");
+    k_colpass<L><<<(unsigned)G, C::CT, 0, s>>>(a.that, a.ihat, twL, a.Z, a.t0, a.tc, a.nb);
+  } else {
+    const unsigned G = (unsigned)((C::H / C::COLS) * a.nb);
+    if (variant == 2)
+      k_colpass_pers<L, true><<<G, C::CT, 0, s>>>(a.that, a.ihat, twL, a.Z, a.t0, a.tc, a.nb);
+    else
+      k_colpass_pers<L, false><<<G, C::CT, 0, s>>>(a.that, a.ihat, twL, a.Z, a.t0, a.tc, a.nb);
+  }
   GS_CHECK_LAUNCH();
 }
 
-bool launch_colpass(int L, const CorrArgs& a, hipStream_t s) {
+bool launch_colpass(int L, const CorrArgs& a, int variant, hipStream_t s) {
   TwTables& T = tw_tables(current_device());
   if (L == 320) {
-    colpass_impl<320>(a, T.t320, s);
+    colpass_impl<320>(a, T.t320, variant, s);
     return true;
   }
   if (L == 720) {
-    colpass_impl<720>(a, T.t720, s);
+    colpass_impl<720>(a, T.t720, variant, s);
     return true;
   }
   return false;

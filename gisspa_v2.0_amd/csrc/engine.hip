@@ -204,6 +204,12 @@ class Engine {
       if (e) (void)hipEventDestroy(e);
     clk_col_.destroy();
     clk_row_.destroy();
+    if (stream2_) {
+      (void)hipStreamSynchronize(stream2_);
+      for (hipEvent_t e : {ev_zready_[0], ev_zready_[1], ev_zfree_[0], ev_zfree_[1], ev_sync_})
+        if (e) (void)hipEventDestroy(e);
+      (void)hipStreamDestroy(stream2_);
+    }
     if (stream_) (void)hipStreamDestroy(stream_);
   }
 
@@ -334,8 +340,15 @@ class Engine {
   }
   void set_chunk(int tc) { chunk_ = tc; }
   void set_variant(int kernel, int variant) {
-    GS_REQUIRE(kernel == 1 && (variant == 0 || variant == 1), kInvalid, "unknown kernel variant");
-    row_variant_ = variant;
+    GS_REQUIRE((kernel == 0 && variant >= 0 && variant <= 2) ||
+                   ((kernel == 1 || kernel == 2) && (variant == 0 || variant == 1)),
+               kInvalid, "unknown kernel variant");
+    if (kernel == 0)
+      col_variant_ = variant;
+    else if (kernel == 1)
+      row_variant_ = variant;
+    else
+      overlap_ = variant == 1;
   }
 
   const gs_params& params() const { return P_; }
@@ -467,7 +480,7 @@ class Engine {
     fft_rows(fft_, out, S, l, (long long)np * hx, -1, scale, stream_);
   }
 
-  void clock_begin(KernelClock& k) {
+  void clock_begin(KernelClock& k, hipStream_t s) {
     if (!profile_) return;
     if (k.used == k.ev.size()) {
       std::pair<hipEvent_t, hipEvent_t> p{nullptr, nullptr};
@@ -475,19 +488,30 @@ class Engine {
       GS_HIP(hipEventCreate(&p.second));
       k.ev.push_back(p);
     }
-    GS_HIP(hipEventRecord(k.ev[k.used].first, stream_));
+    GS_HIP(hipEventRecord(k.ev[k.used].first, s));
   }
-  void clock_end(KernelClock& k) {
+  void clock_end(KernelClock& k, hipStream_t s) {
     if (!profile_) return;
-    GS_HIP(hipEventRecord(k.ev[k.used].second, stream_));
+    GS_HIP(hipEventRecord(k.ev[k.used].second, s));
     k.used += 1;
+  }
+
+  // Second stream (default priority) for the row pass, and the events of the double-buffered
+  // inter-pass handoff; created on first use.
+  void ensure_pipeline() {
+    if (stream2_) return;
+    GS_HIP(hipStreamCreateWithFlags(&stream2_, hipStreamNonBlocking));
+    for (hipEvent_t* e : {&ev_zready_[0], &ev_zready_[1], &ev_zfree_[0], &ev_zfree_[1], &ev_sync_})
+      GS_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming));
   }
 
   int chunk_for(int nb) const {
     if (chunk_ > 0) return chunk_;
-    // keep the inter-pass buffer near 3 GB at l=320 (32 templates x 234 blocks)
+    // Inter-pass buffer near 6 GB (64 templates x 234 blocks at l=320): the row pass re-reads
+    // and re-writes the per-pixel records once per launch, so more templates per launch means
+    // less record traffic per plane.
     const double plane = (double)P_.window_size * (P_.window_size / 2) * sizeof(float2);
-    int tc = (int)std::max(4.0, std::min(64.0, 3.2e9 / (plane * std::max(nb, 1))));
+    int tc = (int)std::max(4.0, std::min(64.0, 6.4e9 / (plane * std::max(nb, 1))));
     return tc;
   }
 
@@ -527,13 +551,27 @@ class Engine {
     const int nt = lim_templates_ > 0 ? std::min(n, lim_templates_) : n;
     angles_ = angles;
     const int tc_max = chunk_for(nb_);
-    float2* Z = d_Z_.ensure((size_t)tc_max * nb_ * l * (l / 2));
+    const size_t zsz = (size_t)tc_max * nb_ * l * (l / 2);
+    // With overlap, the column pass of chunk c+1 (stream_) runs while the row pass of chunk c
+    // (stream2_) consumes the other half of a double-buffered inter-pass buffer.
+    const bool overlap = overlap_;
+    hipStream_t rs = stream_;
+    if (overlap) {
+      ensure_pipeline();
+      rs = stream2_;
+    }
+    float2* Zb = d_Z_.ensure(zsz * (overlap ? 2 : 1));
     float* rot = d_rot_.ensure((size_t)n * L2);
     double* part = d_part_.ensure((size_t)n * l * 2);
     float* stats = d_stats_.ensure((size_t)n * 2);
     float2* that = d_that_.ensure((size_t)n * hx * l);
     float2* work = d_work_.ensure((size_t)n * L2);
     GS_HIP(hipEventRecord(ev_corr0_, stream_));
+    if (overlap) {  // accumulators initialised on stream_ before the first row pass
+      GS_HIP(hipEventRecord(ev_sync_, stream_));
+      GS_HIP(hipStreamWaitEvent(stream2_, ev_sync_, 0));
+    }
+    long long chunk = 0;
     for (int a = 0; a < na; ++a) {
       // RotateTemplate (src/norm.cu:479-521)
       const float cose = (float)std::cos((double)angles[a] * kRefPi / 180);
@@ -543,17 +581,29 @@ class Engine {
       launch_standardise_to_complex(rot, nt, l, stats, work, stream_);
       half_spectra(work, nt, that, (float)(1.0 / (double)L2));
       // correlation + per-pixel statistics (ComputeCCGSum / PickParticles in one pass)
-      for (int t0 = 0; t0 < nt; t0 += tc_max) {
+      for (int t0 = 0; t0 < nt; t0 += tc_max, ++chunk) {
         const int tc = std::min(tc_max, nt - t0);
+        const int zb = overlap ? (int)(chunk % 2) : 0;
+        float2* Z = Zb + zb * zsz;
+        if (overlap && chunk >= 2) GS_HIP(hipStreamWaitEvent(stream_, ev_zfree_[zb], 0));
         CorrArgs ca{that, ihat, Z, t0, tc, nb_};
-        clock_begin(clk_col_);
-        GS_REQUIRE(launch_colpass(l, ca, stream_), kUnsupported, "window_size not supported");
-        clock_end(clk_col_);
-        clock_begin(clk_row_);
+        clock_begin(clk_col_, stream_);
+        GS_REQUIRE(launch_colpass(l, ca, col_variant_, stream_), kUnsupported, "window_size not supported");
+        clock_end(clk_col_, stream_);
+        if (overlap) {
+          GS_HIP(hipEventRecord(ev_zready_[zb], stream_));
+          GS_HIP(hipStreamWaitEvent(stream2_, ev_zready_[zb], 0));
+        }
+        clock_begin(clk_row_, rs);
         launch_rowpass_pixel(l, Z, tc, nb_, d_S_.p, d_SS_.p, d_best_.p, d_arg_.p, a * n + t0,
-                             row_variant_, stream_);
-        clock_end(clk_row_);
+                             row_variant_, rs);
+        clock_end(clk_row_, rs);
+        if (overlap) GS_HIP(hipEventRecord(ev_zfree_[zb], stream2_));
       }
+    }
+    if (overlap) {  // finalisation reads the accumulators written on stream2_
+      GS_HIP(hipEventRecord(ev_sync_, stream2_));
+      GS_HIP(hipStreamWaitEvent(stream_, ev_sync_, 0));
     }
     GS_HIP(hipEventRecord(ev_corr1_, stream_));
     // set_CCG_mean: n = N_t * int(360/phistep) (src/norm.cu:549-550)
@@ -651,12 +701,12 @@ class Engine {
         for (int t0 = 0; t0 < n; t0 += tc_max) {
           const int tc = std::min(tc_max, n - t0);
           CorrArgs ca{d_that_.p, ihat, Z, t0, tc, nbb};
-          clock_begin(clk_col_);
-          GS_REQUIRE(launch_colpass(l, ca, stream_), kUnsupported, "window_size not supported");
-          clock_end(clk_col_);
-          clock_begin(clk_row_);
+          clock_begin(clk_col_, stream_);
+          GS_REQUIRE(launch_colpass(l, ca, col_variant_, stream_), kUnsupported, "window_size not supported");
+          clock_end(clk_col_, stream_);
+          clock_begin(clk_row_, stream_);
           launch_rowpass_plane(l, Z, tc, nbb, part, border, stream_);
-          clock_end(clk_row_);
+          clock_end(clk_row_, stream_);
           // results for templates t0.. of this (tile, angle): layout [tile][angle][t][b][4]
           launch_finalize_plane(part, tc, nbb, groups,
                                 planes + ((size_t)tI * na + a) * per_angle + (size_t)t0 * nbb * 4, stream_);
@@ -719,7 +769,9 @@ class Engine {
 
   gs_params P_;
   int device_;
-  hipStream_t stream_ = nullptr;
+  hipStream_t stream_ = nullptr, stream2_ = nullptr;
+  hipEvent_t ev_zready_[2] = {nullptr, nullptr}, ev_zfree_[2] = {nullptr, nullptr}, ev_sync_ = nullptr;
+  bool overlap_ = false;
   hipEvent_t ev_start_ = nullptr, ev_corr0_ = nullptr, ev_corr1_ = nullptr, ev_end_ = nullptr;
   KernelClock clk_col_, clk_row_;
   bool profile_ = false;
@@ -734,7 +786,7 @@ class Engine {
   int nb_ = 0, bx_ = 0, nbk_ = 0, na_ = 0;
   std::vector<int> offx_, offy_, ulx_, uly_;
   std::vector<float> angles_;
-  int lim_angles_ = 0, lim_templates_ = 0, chunk_ = 0, row_variant_ = 0;
+  int lim_angles_ = 0, lim_templates_ = 0, chunk_ = 0, row_variant_ = 0, col_variant_ = 0;
   double total_ms_ = 0, corr_ms_ = 0;
   std::vector<gs_record> records_;
   std::vector<double> planes_host_;
@@ -900,6 +952,41 @@ gs_status gs_fft2_c2c(int device, float* data, int ny, int nx, int batch, int si
     GS_HIP(hipMemcpy(a.p, data, sizeof(float2) * N, hipMemcpyHostToDevice));
     gs::fft2_batched(cache, a.p, b.p, batch, ny, nx, sign, 1.f, nullptr);
     GS_HIP(hipMemcpy(data, a.p, sizeof(float2) * N, hipMemcpyDeviceToHost));
+  });
+}
+
+namespace {
+__global__ void k_stream_copy(const float4* __restrict__ a, float4* __restrict__ b, long long n) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    b[i] = a[i];
+}
+}  // namespace
+
+gs_status gs_copy_bandwidth(int device, size_t bytes, int reps, double* gbps) {
+  if (!gbps || bytes < 16 || reps < 1) return fail(GS_ERR_INVALID, "bad copy arguments");
+  return guarded([&] {
+    GS_HIP(hipSetDevice(device));
+    const long long n = (long long)(bytes / 16);
+    gs::DevBuf<float4> a, b;
+    a.ensure(n);
+    b.ensure(n);
+    GS_HIP(hipMemset(a.p, 0, n * 16));
+    hipEvent_t e0, e1;
+    GS_HIP(hipEventCreate(&e0));
+    GS_HIP(hipEventCreate(&e1));
+    k_stream_copy<<<4096, 256>>>(a.p, b.p, n);
+    GS_CHECK_LAUNCH();
+    GS_HIP(hipEventRecord(e0, nullptr));
+    for (int r = 0; r < reps; ++r) k_stream_copy<<<4096, 256>>>(a.p, b.p, n);
+    GS_CHECK_LAUNCH();
+    GS_HIP(hipEventRecord(e1, nullptr));
+    GS_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    GS_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *gbps = 2.0 * (double)n * 16 * reps / (ms * 1e-3) / 1e9;  // read + write
   });
 }
 

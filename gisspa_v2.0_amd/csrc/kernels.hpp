@@ -72,8 +72,10 @@ struct CorrArgs {
   float2* Z;           // [tc][nb][L][H] inter-pass buffer
   int t0, tc, nb;
 };
-// Column pass: packed conj-multiply + inverse column FFTs into Z (returns false if L unsupported).
-bool launch_colpass(int L, const CorrArgs& a, hipStream_t s);
+// Column pass: packed conj-multiply + inverse column FFTs into Z (returns false if L
+// unsupported).  variant (result-invariant): 0 one workgroup per plane and column group,
+// 1 persistent over templates, 2 persistent with prefetch.
+bool launch_colpass(int L, const CorrArgs& a, int variant, hipStream_t s);
 // Row pass, norm_type=1: C2R rows + per-(block,pixel) running moments and best record.
 // variant (result-invariant): 0 plain, 1 software-prefetched Z loads.
 bool launch_rowpass_pixel(int L, const float2* Z, int tc, int nb, double* S, double* SS,

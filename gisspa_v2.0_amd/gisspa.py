@@ -71,6 +71,7 @@ SIGNATURES = {
     "gs_kernel_timing": (C.c_int, [_P, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]),
     "gs_set_chunk": (C.c_int, [_P, C.c_int]),
     "gs_set_variant": (C.c_int, [_P, C.c_int, C.c_int]),
+    "gs_copy_bandwidth": (C.c_int, [C.c_int, C.c_size_t, C.c_int, C.POINTER(C.c_double)]),
 }
 
 _lib = None
@@ -228,6 +229,13 @@ class Engine:
         load().gs_format_record(self._h, C.byref(r), image_index, path.encode(), C.byref(self._ctf),
                                 1 if tile_local else 0, buf, 1024)
         return buf.value.decode()
+
+
+def copy_bandwidth(nbytes: int = 1 << 30, reps: int = 20, device: int = 0) -> float:
+    """Measured streaming-copy rate in GB/s (read + write)."""
+    g = C.c_double()
+    _check(load().gs_copy_bandwidth(device, nbytes, reps, C.byref(g)))
+    return g.value
 
 
 def fft_c2c(x: np.ndarray, sign: int, device: int = 0) -> np.ndarray:

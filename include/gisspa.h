@@ -133,6 +133,9 @@ gs_status gs_set_limits(gs_engine* e, int n_angles, int n_templates);
 gs_status gs_fft_c2c(int device, float* data, int n, int batch, int sign);
 /* Standalone 2-D transform of `batch` row-major ny x nx complex planes (in place). */
 gs_status gs_fft2_c2c(int device, float* data, int ny, int nx, int batch, int sign);
+/* Measured device-memory copy rate (read + write bytes per second, GB/s) of a plain streaming
+ * copy of `bytes` repeated `reps` times: the practical HBM ceiling reported next to the peak. */
+gs_status gs_copy_bandwidth(int device, size_t bytes, int reps, double* gbps);
 /* Wall time of the device pipeline of the last gs_run, and of its angle loop, in ms. */
 gs_status gs_last_timing(const gs_engine* e, double* total_ms, double* corr_ms);
 /* When on, every correlation launch is bracketed by HIP events on the engine's stream. */
@@ -141,8 +144,11 @@ gs_status gs_set_profiling(gs_engine* e, int on);
 gs_status gs_kernel_timing(const gs_engine* e, int which, double* avg_ms, long long* count);
 /* Templates per correlation launch (0 = automatic); a result-invariant tuning knob. */
 gs_status gs_set_chunk(gs_engine* e, int templates_per_launch);
-/* Selects a result-invariant code variant of a correlation kernel (kernel 1 = row pass:
- * variant 0 plain, 1 software-prefetched loads). */
+/* Selects a result-invariant code variant of a correlation kernel.  kernel 0 = column pass:
+ * 0 one workgroup per plane, 1 persistent over templates, 2 persistent + prefetch;
+ * kernel 1 = row pass: 0 plain, 1 software-prefetched loads;
+ * kernel 2 = pipeline: 0 both passes on one stream (default), 1 row pass on a second stream
+ * overlapping the next column pass through a double-buffered inter-pass buffer. */
 gs_status gs_set_variant(gs_engine* e, int kernel, int variant);
 
 #ifdef __cplusplus
