@@ -43,8 +43,8 @@ def parse():
     ap.add_argument("--overlap", type=int, default=0,
                     help="1: row pass on a second stream (blurs per-kernel timing); -1: tune")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-templates", type=int, default=2)
-    ap.add_argument("--cpu-angles", type=int, default=3)
+    ap.add_argument("--cpu-templates", type=int, default=4)
+    ap.add_argument("--cpu-angles", type=int, default=4)
     return ap.parse_args()
 
 

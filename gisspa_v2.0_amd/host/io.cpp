@@ -66,9 +66,9 @@ std::vector<LstEntry> load_lst(const std::string& path) {
   std::ifstream f(path);
   if (!f) return out;
   std::string line;
+  LstEntry e;  // reused across lines like the reference, so a blank line repeats the last CTF
   while (std::getline(f, line)) {
     if (!line.empty() && line[0] == '#') continue;
-    LstEntry e;
     char buf[1024] = {0};
     std::sscanf(line.c_str(), "%d %1023s defocus=%lf dfdiff=%lf dfang=%lf", &e.unused, buf,
                 &e.defocus, &e.dfdiff, &e.dfang);

@@ -20,8 +20,28 @@ static int die(const std::string& msg) {
   return 1;
 }
 
+// Canonical dump of the parsed inputs (no GPU use): config keys, LST entries, Euler angles.
+static int parse_dump(const char* path) {
+  gsio::Config conf;
+  if (!conf.parse(path)) return die(conf.errors.empty() ? "bad config" : conf.errors[0]);
+  for (const auto& kv : conf.value) {
+    if (std::holds_alternative<int>(kv.second))
+      std::printf("config %s %d\n", kv.first.c_str(), std::get<int>(kv.second));
+    else if (std::holds_alternative<float>(kv.second))
+      std::printf("config %s %.6g\n", kv.first.c_str(), std::get<float>(kv.second));
+    else
+      std::printf("config %s %s\n", kv.first.c_str(), std::get<std::string>(kv.second).c_str());
+  }
+  for (const auto& e : gsio::load_lst(conf.gets("input")))
+    std::printf("lst %d %s %.6f %.6f %.6f\n", e.unused, e.rpath.c_str(), e.defocus, e.dfdiff, e.dfang);
+  auto eu = gsio::load_euler(conf.gets("eulerfile"));
+  for (size_t i = 0; i < eu.size(); ++i) std::printf("euler %.6g %.6g %.6g\n", eu.alt[i], eu.az[i], eu.phi[i]);
+  return 0;
+}
+
 int main(int argc, char** argv) {
-  if (argc < 2) return die("usage: gisspa_main <config>");
+  if (argc < 2) return die("usage: gisspa_main <config> | gisspa_main --parse-dump <config>");
+  if (std::string(argv[1]) == "--parse-dump") return argc > 2 ? parse_dump(argv[2]) : die("missing config");
   gsio::Config conf;
   if (!conf.parse(argv[1])) return die(conf.errors.empty() ? "bad config" : conf.errors[0]);
   auto missing = conf.missing_required();

@@ -1,0 +1,93 @@
+"""CPU checks of the drop-in boundary: the C ABI library loads and exports every entry point of
+include/gisspa.h, rejects bad parameters before touching a device, and the executable parses
+config/LST/Euler files exactly like the reference's own parser (built from
+src/EMReader/DataReader2.cpp into oracle/_ref when the reference is present)."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+EXE = os.path.join(ROOT, "gisspa_v2.0_amd", "lib", "gisspa_main")
+REF_PARSE = os.path.join(ROOT, "oracle", "_ref", "ref_parse")
+
+
+def header_functions():
+    text = open(os.path.join(ROOT, "include", "gisspa.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(gs_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_every_header_symbol_is_exported(gs):
+    lib = gs.load()
+    names = header_functions()
+    assert "gs_create" in names and "gs_run" in names
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(names) <= set(gs.SIGNATURES), set(names) - set(gs.SIGNATURES)
+
+
+@pytest.mark.parametrize("field,value,status", [
+    ("window_size", 321, 1),   # odd
+    ("window_size", 256, 3),   # valid but not built
+    ("phistep", 0.0, 1),
+    ("angpix", -1.0, 1),
+    ("overlap", 400, 1),
+])
+def test_create_rejects_bad_parameters_without_a_device(gs, field, value, status):
+    cfg = gs.SearchConfig()
+    setattr(cfg, field, value)
+    p = cfg.to_c()
+    h = ctypes.c_void_p()
+    st = gs.load().gs_create(ctypes.byref(p), 0, ctypes.byref(h))
+    assert st == status
+    assert gs.load().gs_last_error().decode()
+
+
+def test_null_arguments(gs):
+    lib = gs.load()
+    assert lib.gs_create(None, 0, None) == 1
+    assert lib.gs_run(None) == 1
+    lib.gs_destroy(None)
+    lib.gs_free(None)
+
+
+def _dump(cmd):
+    out = subprocess.run(cmd, cwd=GOLDEN, capture_output=True, text=True, check=True).stdout
+    return [l for l in out.splitlines() if re.match(r"^(config|lst|euler) ", l)]
+
+
+@pytest.mark.skipif(not os.path.exists(EXE), reason="executable not built")
+def test_executable_parser_matches_reference_parser():
+    if not os.path.exists(REF_PARSE):
+        pytest.skip("oracle/_ref not built (needs the reference sources)")
+    ours = _dump([EXE, "--parse-dump", "parse_config.txt"])
+    ref = _dump([REF_PARSE, "parse_config.txt"])
+    ours_cfg = {l for l in ours if l.startswith("config")}
+    ref_cfg = {l for l in ref if l.startswith("config")}
+    assert ref_cfg <= ours_cfg  # ours adds nms_radius and tile_local_centers
+    assert [l for l in ours if not l.startswith("config")] == [l for l in ref if not l.startswith("config")]
+
+
+@pytest.mark.skipif(not os.path.exists(EXE), reason="executable not built")
+def test_executable_parse_dump_fixture():
+    lines = _dump([EXE, "--parse-dump", "parse_config.txt"])
+    assert "config angpix 1.06" in lines and "config window_size 720" in lines
+    assert [l for l in lines if l.startswith("lst")] == [
+        "lst 0 mic_000.mrc 1.500000 0.020000 30.000000",
+        "lst 1 mic_001.mrc 2.250000 0.000000 0.000000",
+        "lst 2 /abs/path/mic_002.mrc 0.800000 -0.015000 -45.500000",
+    ]
+    assert [l for l in lines if l.startswith("euler")] == [
+        "euler 12.5 33.25 0", "euler 90 180 10", "euler 45 270 5", "euler 60 120 0"]
+
+
+@pytest.mark.skipif(not os.path.exists(EXE), reason="executable not built")
+def test_executable_reports_missing_keys(tmp_path):
+    cfg = tmp_path / "c.txt"
+    cfg.write_text("input = a.lst\ntemplate = t.mrc\neulerfile = e.txt\n")
+    r = subprocess.run([EXE, str(cfg)], capture_output=True, text=True)
+    assert r.returncode != 0 and "angpix" in r.stderr

@@ -1,0 +1,60 @@
+"""End-to-end test of the config-driven executable on the GPU: MRC micrographs and template
+stack, LST and Euler files, and a reference-style config go in; the record lines that come out
+must be the oracle's picks in the reference's line format (src/norm.cu:610-615)."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "gisspa_v2.0_amd", "lib", "gisspa_main")
+LINE = re.compile(r"^(\d+)\t(\S+)\tdefocus=([-\d.]+)\tdfdiff=([-\d.]+)\tdfang=([-\d.]+)\t"
+                  r"euler=([-\d.]+),([-\d.]+),([-\d.]+)\tcenter=([-\d.]+),([-\d.]+)\tscore=([-\d.]+)$")
+
+
+def test_executable_matches_oracle(tmp_path, orc, gs):
+    import synth
+
+    cfg = gs.SearchConfig(angpix=1.0, phistep=10.0, kk=3.0, energy=300.0, cs=2.7, highres=8.0,
+                          lowres=100.0, diameter=128.0, threshold=7.0, window_size=320, norm_type=1,
+                          nms_radius=64.0)
+    mics = []
+    for k, seed in enumerate((21, 22)):
+        ds = synth.make_dataset(seed, 640, 640, n_templates=4, tsize=128, n_particles=3, snr=0.6,
+                                defocus=1.2 + 0.3 * k, dfdiff=0.01, dfang=20.0 * k)
+        path = tmp_path / f"mic_{k}.mrc"
+        synth.write_mrc(str(path), ds.image)
+        mics.append((path, ds))
+    templates = mics[0][1].templates  # same seed-independent density and template orientations?
+    eul = mics[0][1].eulers
+    synth.write_mrc(str(tmp_path / "templates.mrc"), templates)
+    (tmp_path / "euler.txt").write_text("".join(f"{j} {a:.4f} {b:.4f} 0.0\n" for j, (a, b, _) in enumerate(eul)))
+    (tmp_path / "list.lst").write_text("".join(
+        f"{k} {p} defocus={d.ctf[0]:.6f} dfdiff={d.ctf[1]:.6f} dfang={d.ctf[2]:.6f}\n"
+        for k, (p, d) in enumerate(mics)))
+    (tmp_path / "config.txt").write_text(
+        "input = list.lst\ntemplate = templates.mrc\neulerfile = euler.txt\nangpix = 1.0\n"
+        "phistep = 10\nkk = 3\nenergy = 300\ncs = 2.7\nHighres = 8\nLowres = 100\n"
+        "diameter = 128\nthreshold = 7.0\noutput = out.txt\nfirst = 0\nlast = 2\nGPU_ID = 0\n"
+        "window_size = 320\nnorm_type = 1\nnms_radius = 64\n")
+    r = subprocess.run([EXE, "config.txt"], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    lines = (tmp_path / "out.txt").read_text().splitlines()
+    got = [LINE.match(l).groups() for l in lines]
+    expect = []
+    for k, (p, d) in enumerate(mics):
+        picks, _ = orc.search_norm(cfg, d.image, templates, d.ctf)
+        for q in picks:
+            j = int(q["template_id"])
+            expect.append((k, str(p), j, float(q["psi"]), float(q["x"]), float(q["y"]), float(q["score"])))
+    assert len(got) == len(expect) and len(expect) > 0
+    for g, e in zip(got, expect):
+        k, path, j, psi, x, y, s = e
+        assert int(g[0]) == k and g[1] == path
+        assert abs(float(g[5]) - eul[j][0]) < 1e-3 and abs(float(g[6]) - eul[j][1]) < 1e-3
+        assert float(g[7]) == pytest.approx(psi, abs=1e-6)
+        assert float(g[8]) == x and float(g[9]) == y
+        assert abs(float(g[10]) - s) <= 1e-4 * abs(s) + 1e-6
