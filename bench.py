@@ -155,6 +155,17 @@ def main():
     dominant = "column_pass" if col_ms >= row_ms else "row_pass"
     dom_ms, dom_bytes = (col_ms, col_bytes) if col_ms >= row_ms else (row_ms, row_bytes)
     achieved = dom_bytes / (dom_ms * 1e-3)
+    # measured HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated), from
+    # the committed counter summary, scaled from its 64-template launches to this average launch
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "corr_c1_traffic.json")
+    if os.path.exists(tpath) and l == 320:
+        kname = {"column_pass": "k_colpass<320>", "row_pass": "k_rowpass_pix_wave<320, true>"}[dominant]
+        rec = json.load(open(tpath))["kernels"].get(kname)
+        if rec:
+            per_plane = (rec["fetch_bytes_per_launch"] + rec["write_bytes_per_launch"]) / (
+                rec["templates_per_launch"] * rec["blocks"])
+            traffic = per_plane * planes_per_launch
     # whole correlation stage: algorithmic bytes of every plane of the last step over the wall
     # time of its angle loop (the two passes may run concurrently)
     stage_bytes = a.templates * n_ang * nb * 24 * l * kx
@@ -181,7 +192,7 @@ def main():
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK, "traffic": None, "kernel": dominant,
+            "frac": achieved / HBM_PEAK, "traffic": traffic, "kernel": dominant,
             "avg_launch_ms": dom_ms, "algorithmic_bytes_per_launch": dom_bytes,
             "column_pass_ms": col_ms, "row_pass_ms": row_ms,
             "correlation_stage_frac": stage_bw / HBM_PEAK,

@@ -252,3 +252,12 @@ def fft2_c2c(x: np.ndarray, sign: int, device: int = 0) -> np.ndarray:
     ny, nx = a.shape[-2:]
     _check(load().gs_fft2_c2c(device, a.ctypes.data, ny, nx, a.size // (nx * ny), sign))
     return a
+
+
+if __name__ == "__main__":
+    import sys
+
+    if "--copy-probe" in sys.argv:
+        # one 1 GiB streaming copy (1 GiB read + 1 GiB write): the calibration run for the
+        # FETCH_SIZE / WRITE_SIZE traffic counters
+        print(f"copy {copy_bandwidth(1 << 30, 1):.1f} GB/s")
