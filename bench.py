@@ -105,7 +105,7 @@ def main():
         eng.set_chunk(a.chunk)
     # pick the faster result-invariant variant of each correlation kernel on a one-angle sample
     chosen = {0: a.col_variant, 1: a.row_variant, 2: a.overlap}
-    for kernel, options in ((2, (0, 1)), (0, (0, 1, 2)), (1, (0, 1))):
+    for kernel, options in ((2, (0, 1)), (0, (0, 3)), (1, (0, 1, 2, 3))):
         if chosen[kernel] >= 0:
             eng.set_variant(kernel, chosen[kernel])
             continue

@@ -53,7 +53,18 @@ struct CorrCfg<720> {
 
 namespace {
 
-template <int L>
+// Column-pass radix plans: Plan = 0 the configuration default, 1 the alternative with one LDS
+// round trip fewer where one exists (320 = 4*4*20: 16 lanes x 20 points every stage).
+template <int L, int Plan>
+struct ColPlanOf {
+  using type = typename CorrCfg<L>::ColPlan;
+};
+template <>
+struct ColPlanOf<320, 1> {
+  using type = ct::RadixList<4, 4, 20>;
+};
+
+template <int L, int Plan = 0>
 __global__ void __launch_bounds__(CorrCfg<L>::CT)
     k_colpass(const float2* __restrict__ that, const float2* __restrict__ ihat,
               const float2* __restrict__ twL, float2* __restrict__ Z, int t0, int tc, int nb) {
@@ -138,7 +149,7 @@ __global__ void __launch_bounds__(CorrCfg<L>::CT)
   }
   ct::lds_sync<ct::Sync::WAVE>();
   ct::run_stockham<L, TPC, +1, ct::Sync::WAVE>(buf + (tid / TPC) * LP, tw, tid % TPC,
-                                                typename C::ColPlan{});
+                                                typename ColPlanOf<L, Plan>::type{});
   __syncthreads();  // rows of Z gather all COLS columns of the workgroup
 
   float2* Zp = Z + ((long long)tl * nb + b) * (long long)L * H;
@@ -402,14 +413,15 @@ __global__ void __launch_bounds__(CorrCfg<L>::RT)
 // same PPL pixels of its rows into registers every iteration.  The real-to-complex packing
 // loads each Hermitian pair X(k), X(H-k) once and writes both packed values.  PF: load the next
 // iteration's pairs before transforming the current one (more registers, fewer stalls).
-template <int L, bool PF>
+// TPI_: templates per wave iteration (rows per wave = transforms per wave / TPI_).
+template <int L, bool PF, int TPI_ = CorrCfg<L>::TPI>
 __global__ void __launch_bounds__(256)
     k_rowpass_pix_wave(const float2* __restrict__ Z, const float2* __restrict__ twL,
                        const float2* __restrict__ twH, int tc, int nb, double* __restrict__ S,
                        double* __restrict__ SS, float* __restrict__ best, int32_t* __restrict__ arg,
                        int arg_base) {
   using C = CorrCfg<L>;
-  constexpr int H = C::H, TPI = C::TPI, TPT = C::TPT;
+  constexpr int H = C::H, TPI = TPI_, TPT = C::TPT;
   constexpr int TPW = 64 / TPT;  // transforms per wave
   constexpr int RW = TPW / TPI;  // rows per wave
   constexpr int WAVES = 4;
@@ -642,12 +654,15 @@ int rowpass_groups(int L) {
 template <int L>
 static void colpass_impl(const CorrArgs& a, const float2* twL, int variant, hipStream_t s) {
   using C = CorrCfg<L>;
-  if (variant == 0) {
+  if (variant == 0 || variant == 3) {
     constexpr int BT = 8, TT = 16;  // tile shape of k_colpass's logical order
     const long long G = (long long)(C::H / C::COLS) * ((a.tc + TT - 1) / TT * TT) *
                         ((a.nb + BT - 1) / BT * BT);
     GS_REQUIRE(G > 0 && G < (1LL << 31), kInvalid, "column pass grid out of range");
-    k_colpass<L><<<(unsigned)G, C::CT, 0, s>>>(a.that, a.ihat, twL, a.Z, a.t0, a.tc, a.nb);
+    if (variant == 3)
+      k_colpass<L, 1><<<(unsigned)G, C::CT, 0, s>>>(a.that, a.ihat, twL, a.Z, a.t0, a.tc, a.nb);
+    else
+      k_colpass<L, 0><<<(unsigned)G, C::CT, 0, s>>>(a.that, a.ihat, twL, a.Z, a.t0, a.tc, a.nb);
   } else {
     const unsigned G = (unsigned)((C::H / C::COLS) * a.nb);
     if (variant == 2)
@@ -686,10 +701,16 @@ bool launch_rowpass_pixel(int L, const float2* Z, int tc, int nb, double* S, dou
                           float* best, int32_t* arg, int arg_base, int variant, hipStream_t s) {
   TwTables& T = tw_tables(current_device());
   if (L == 320) {
-    constexpr int ROWS = 4 * (64 / CorrCfg<320>::TPT) / CorrCfg<320>::TPI;
-    dim3 grid(320 / ROWS, nb);
+    // variants 0/1: 2 rows x 4 templates per wave iteration; 2/3: 1 row x 8 templates
+    const int tpi = variant >= 2 ? 8 : CorrCfg<320>::TPI;
+    const int rows = 4 * (64 / CorrCfg<320>::TPT) / tpi;
+    dim3 grid(320 / rows, nb);
     if (variant == 1)
       k_rowpass_pix_wave<320, true><<<grid, 256, 0, s>>>(Z, T.t320, T.t160, tc, nb, S, SS, best, arg, arg_base);
+    else if (variant == 2)
+      k_rowpass_pix_wave<320, true, 8><<<grid, 256, 0, s>>>(Z, T.t320, T.t160, tc, nb, S, SS, best, arg, arg_base);
+    else if (variant == 3)
+      k_rowpass_pix_wave<320, false, 8><<<grid, 256, 0, s>>>(Z, T.t320, T.t160, tc, nb, S, SS, best, arg, arg_base);
     else
       k_rowpass_pix_wave<320, false><<<grid, 256, 0, s>>>(Z, T.t320, T.t160, tc, nb, S, SS, best, arg, arg_base);
     GS_CHECK_LAUNCH();

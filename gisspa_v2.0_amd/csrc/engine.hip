@@ -340,8 +340,8 @@ class Engine {
   }
   void set_chunk(int tc) { chunk_ = tc; }
   void set_variant(int kernel, int variant) {
-    GS_REQUIRE((kernel == 0 && variant >= 0 && variant <= 2) ||
-                   ((kernel == 1 || kernel == 2) && (variant == 0 || variant == 1)),
+    GS_REQUIRE((kernel == 0 && variant >= 0 && variant <= 3) || (kernel == 1 && variant >= 0 && variant <= 3) ||
+                   (kernel == 2 && (variant == 0 || variant == 1)),
                kInvalid, "unknown kernel variant");
     if (kernel == 0)
       col_variant_ = variant;
