@@ -41,7 +41,9 @@ def parse():
     ap.add_argument("--row-variant", type=int, default=-1, help="-1: tune on one angle")
     ap.add_argument("--col-variant", type=int, default=-1, help="-1: tune on one angle")
     ap.add_argument("--overlap", type=int, default=0,
-                    help="1: row pass on a second stream (blurs per-kernel timing); -1: tune")
+                    help="1: row pass on a second stream (blurs per-kernel timing)")
+    ap.add_argument("--batch", type=int, default=0,
+                    help="K > 0: uneven-micrograph batch workload instead (its own JSON line)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-templates", type=int, default=4)
     ap.add_argument("--cpu-angles", type=int, default=4)
@@ -79,6 +81,96 @@ def cpu_baseline(cfg, ds, nt, na):
     }
 
 
+def c1_config(gisspa):
+    return gisspa.SearchConfig(angpix=1.0, phistep=5.0, kk=3.0, energy=300.0, cs=2.7, highres=8.0,
+                               lowres=100.0, diameter=128.0, threshold=7.0, window_size=320,
+                               overlap=0, norm_type=1, invert=0, phase_flip=0, nms_radius=64.0)
+
+
+def batch_main(a):
+    """Uneven-micrograph batch: K micrographs with sizes drawn from {3710x3838, 4096x4096,
+    5760x4092}, defocus U(0.8, 3.0) um, N_t ~ U{200..1000} templates of a size drawn from
+    {128, 180, 256}, each micrograph with its own template set.  Three ways to run them:
+    `per_engine` = the reference's driver loop (an engine created, loaded, run and destroyed per
+    micrograph), `one_engine` = one engine and gs_search per micrograph, `batch` =
+    gs_search_batch (staging thread + graph replay).  value = correlations/s of `batch`."""
+    import gisspa
+    import synth
+
+    rng = np.random.default_rng(2000)
+    sizes = [(3710, 3838), (4096, 4096), (5760, 4092)]
+    items, work, desc = [], 0, []
+    t0 = time.perf_counter()
+    for i in range(a.batch):
+        W, H = sizes[int(rng.integers(3))]
+        nt = int(rng.integers(200, 1001))
+        ts = int(rng.choice([128, 180, 256]))
+        df = float(rng.uniform(0.8, 3.0))
+        ds = synth.make_dataset(1000 + i, W, H, n_templates=nt, tsize=ts, n_particles=100,
+                                snr=0.05, defocus=df)
+        items.append({"image": ds.image, "ctf": ds.ctf, "templates": ds.templates})
+        work += nt * 72 * W * H
+        desc.append(f"{W}x{H}/{nt}x{ts}")
+        log(f"micrograph {i}: {desc[-1]}, defocus {df:.2f}")
+    log(f"batch data ready in {time.perf_counter() - t0:.1f} s")
+    cfg = c1_config(gisspa)
+    times = {}
+    # warm the library (code objects, twiddle tables) on the first item
+    with gisspa.Engine(cfg) as e:
+        e.load_templates(items[0]["templates"])
+        e.set_limits(1, 0)
+        e.search(items[0]["image"], items[0]["ctf"])
+    recs = {}
+    t1 = time.perf_counter()
+    recs["per_engine"] = []
+    for it in items:
+        with gisspa.Engine(cfg) as e:
+            e.load_templates(it["templates"])
+            recs["per_engine"].append(e.search(it["image"], it["ctf"]))
+    times["per_engine"] = time.perf_counter() - t1
+    log(f"per-engine loop: {times['per_engine']:.2f} s")
+    t1 = time.perf_counter()
+    recs["one_engine"] = []
+    with gisspa.Engine(cfg) as e:
+        for it in items:
+            e.load_templates(it["templates"])
+            recs["one_engine"].append(e.search(it["image"], it["ctf"]))
+    times["one_engine"] = time.perf_counter() - t1
+    log(f"one-engine loop: {times['one_engine']:.2f} s")
+    with gisspa.Engine(cfg) as e:
+        e.set_graph(True)
+        t1 = time.perf_counter()
+        recs["batch"] = e.search_batch(items)
+        times["batch"] = time.perf_counter() - t1
+        graphs = e.graph_stats()
+    log(f"batch: {times['batch']:.2f} s, graph captures/launches {graphs}")
+    same = all(
+        len(x) == len(y) and all(np.array_equal(x[n], y[n]) for n in x.dtype.names)
+        for k in ("one_engine", "batch") for x, y in zip(recs["per_engine"], recs[k]))
+    out = {
+        "metric": "template·angle·pixel correlations/s over an uneven micrograph batch",
+        "value": work / times["batch"],
+        "unit": "correlations/s",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 1,
+        "ms_per_step": times["batch"] * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (seeded blob-density projections, CTF, N(0,1) noise)",
+        "config": {"workload": f"batch of {a.batch} uneven micrographs, norm_type=1, window 320, "
+                               "72 angles", "micrographs": desc},
+        "seconds": times,
+        "speedup_vs_per_engine": times["per_engine"] / times["batch"],
+        "speedup_vs_one_engine": times["one_engine"] / times["batch"],
+        "records_identical": bool(same),
+        "graph_captures_launches": list(graphs),
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -86,6 +178,9 @@ def main():
               "(the search shards by micrograph or block with no exchange; see DESIGN.md)",
               file=sys.stderr)
         sys.exit(2)
+    if a.batch:
+        batch_main(a)
+        return
     import gisspa
     import synth
 
@@ -93,39 +188,26 @@ def main():
     ds = synth.make_dataset(1001, 5760, 4092, n_templates=a.templates, tsize=180, n_particles=400,
                             snr=0.05, defocus=2.0, dfdiff=0.005, dfang=30.0, grid=True)
     log(f"dataset ready in {time.perf_counter() - t0:.1f} s")
-    cfg = gisspa.SearchConfig(angpix=1.0, phistep=5.0, kk=3.0, energy=300.0, cs=2.7, highres=8.0,
-                              lowres=100.0, diameter=128.0, threshold=7.0, window_size=320,
-                              overlap=0, norm_type=1, invert=0, phase_flip=0, nms_radius=64.0)
+    cfg = c1_config(gisspa)
     eng = gisspa.Engine(cfg, device=0)
     eng.load_templates(ds.templates, ds.eulers[:, :2])
     eng.set_image(ds.image, ds.ctf)
     if a.angles:
         eng.set_limits(a.angles, 0)
-    if a.chunk:
-        eng.set_chunk(a.chunk)
-    # pick the faster result-invariant variant of each correlation kernel on a one-angle sample
-    chosen = {0: a.col_variant, 1: a.row_variant, 2: a.overlap}
-    for kernel, options in ((2, (0, 1)), (0, (0, 3)), (1, (0, 1, 2, 3))):
-        if chosen[kernel] >= 0:
-            eng.set_variant(kernel, chosen[kernel])
-            continue
-        eng.set_limits(1, 0)
-        best = None
-        for v in options:
-            eng.set_variant(kernel, v)
-            eng.run()
-            eng.set_profiling(True)
-            eng.run()
-            # kernel 2 (pipeline overlap) is judged by the wall time of the angle loop
-            ms = eng.timing()[1] if kernel == 2 else eng.kernel_timing(kernel)[0]
-            log(f"kernel {kernel} variant {v}: {ms:.3f} ms")
-            if best is None or ms < best[0]:
-                best = (ms, v)
-        chosen[kernel] = best[1]
-        eng.set_variant(kernel, best[1])
-        eng.set_limits(a.angles, 0)
-        eng.set_profiling(False)
-    variant = chosen
+    # result-invariant kernel options: the library's autotuner on a one-angle sample, unless given
+    if a.col_variant < 0 and a.row_variant < 0 and not a.chunk:
+        t1 = time.perf_counter()
+        col, row, chunk = eng.autotune()
+        log(f"autotune: column variant {col}, row variant {row}, {chunk} templates per launch "
+            f"({time.perf_counter() - t1:.1f} s)")
+    else:
+        col, row, chunk = max(a.col_variant, 0), max(a.row_variant, 0), a.chunk
+        eng.set_variant(0, col)
+        eng.set_variant(1, row)
+        if chunk:
+            eng.set_chunk(chunk)
+    eng.set_variant(2, max(a.overlap, 0))
+    variant = {0: col, 1: row, 2: max(a.overlap, 0), "chunk": chunk}
     for i in range(a.warmup):
         t1 = time.perf_counter()
         eng.run()
@@ -189,6 +271,7 @@ def main():
             "templates": a.templates, "angles": n_ang, "image": [W, H], "window_size": l,
             "blocks": nb, "picks": int(len(recs)), "column_pass_variant": variant[0],
             "row_pass_variant": variant[1], "pass_overlap": variant[2],
+            "templates_per_launch": variant["chunk"] or None,
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",

@@ -645,6 +645,8 @@ static int current_device() {
 
 bool corr_supported(int L) { return L == 320 || L == 720; }
 
+void corr_prepare() { (void)tw_tables(current_device()); }
+
 int rowpass_groups(int L) {
   if (L == 320) return 320 / CorrCfg<320>::ROWS;
   if (L == 720) return 720 / CorrCfg<720>::ROWS;

@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <future>
 #include <map>
 #include <memory>
 #include <string>
@@ -183,6 +184,34 @@ struct KernelClock {
   }
 };
 
+// A captured angle loop and its executable instance.
+struct CapturedGraph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+// Page-locked host staging buffer (hipHostMalloc), grown on demand.
+struct PinnedBuf {
+  float* p = nullptr;
+  size_t n = 0;
+  PinnedBuf() = default;
+  PinnedBuf(const PinnedBuf&) = delete;
+  PinnedBuf& operator=(const PinnedBuf&) = delete;
+  ~PinnedBuf() { release(); }
+  void release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  float* ensure(size_t count) {
+    if (count <= n && p) return p;
+    release();
+    GS_HIP(hipHostMalloc((void**)&p, sizeof(float) * std::max<size_t>(count, 1), hipHostMallocDefault));
+    n = count;
+    return p;
+  }
+};
+
 class Engine {
  public:
   Engine(const gs_params& p, int device) : P_(p), device_(device) {
@@ -200,6 +229,7 @@ class Engine {
   ~Engine() {
     (void)hipSetDevice(device_);
     if (stream_) (void)hipStreamSynchronize(stream_);
+    drop_graphs();
     for (hipEvent_t e : {ev_start_, ev_corr0_, ev_corr1_, ev_end_})
       if (e) (void)hipEventDestroy(e);
     clk_col_.destroy();
@@ -233,7 +263,7 @@ class Engine {
     GS_HIP(hipStreamSynchronize(stream_));
   }
 
-  void set_image(const float* img, int W, int H, const gs_ctf& ctf) {
+  void check_image(const float* img, int W, int H) const {
     GS_REQUIRE(img && W > 0 && H > 0, kInvalid, "empty image");
     const int l = P_.window_size;
     if (P_.norm_type) {
@@ -241,16 +271,81 @@ class Engine {
     } else {
       GS_REQUIRE(W >= kTile && H >= kTile, kUnsupported, "norm_type=0 needs an image of at least 720 x 720");
     }
+  }
+
+  // Host part of LoadImage (src/norm.cu:156-181): copy into `dst` and, for norm_type=1, clip
+  // and invert there.  Makes no HIP call, so it may run on a helper thread.
+  void stage_image(const float* img, int W, int H, float* dst) const {
+    std::memcpy(dst, img, sizeof(float) * (size_t)W * H);
+    if (P_.norm_type) clip_image_host(dst, (size_t)W * H, W, H, P_.invert);
+  }
+
+  // Uploads a staged image (the output of stage_image).
+  void upload_image(const float* staged, int W, int H, const gs_ctf& ctf) {
     bind();
     W_ = W;
     H_ = H;
     ctf_ = ctf;
-    host_img_.assign(img, img + (size_t)W * H);
-    if (P_.norm_type) clip_image_host(host_img_.data(), host_img_.size(), W, H, P_.invert);
     d_img_.ensure((size_t)W * H);
-    GS_HIP(hipMemcpyAsync(d_img_.p, host_img_.data(), sizeof(float) * W * H, hipMemcpyHostToDevice, stream_));
+    GS_HIP(hipMemcpyAsync(d_img_.p, staged, sizeof(float) * W * H, hipMemcpyHostToDevice, stream_));
     GS_HIP(hipStreamSynchronize(stream_));
     have_image_ = true;
+  }
+
+  void set_image(const float* img, int W, int H, const gs_ctf& ctf) {
+    check_image(img, W, H);
+    bind();
+    float* staged = pinned_[0].ensure((size_t)W * H);
+    stage_image(img, W, H, staged);
+    upload_image(staged, W, H, ctf);
+  }
+
+  // Searches a sequence of micrographs (each with its own CTF and optionally its own template
+  // set) in order, as main's per-micrograph loop does (src/main.cu:37-49).  While micrograph i
+  // runs on the device, a helper thread copies and clips micrograph i+1 into the other of two
+  // page-locked staging buffers, and with graphs on the angle loop is replayed from a capture
+  // whenever the problem shape repeats.  Records equal those of one gs_search per micrograph.
+  void search_batch(const gs_batch_item* items, int n_items, std::vector<gs_record>& all,
+                    size_t* counts) {
+    GS_REQUIRE(items && n_items > 0 && counts, kInvalid, "empty batch");
+    for (int i = 0; i < n_items; ++i) {
+      check_image(items[i].image, items[i].width, items[i].height);
+      if (items[i].templates) {
+        GS_REQUIRE(items[i].n_templates > 0 && items[i].template_width > 0 && items[i].template_height > 0,
+                   kInvalid, "batch item with an empty template stack");
+        GS_REQUIRE(items[i].template_width <= P_.window_size && items[i].template_height <= P_.window_size,
+                   kInvalid, "templates larger than window_size");
+      } else {
+        GS_REQUIRE(i > 0 || n_ > 0, kState, "first batch item has no templates and none are loaded");
+      }
+    }
+    bind();
+    size_t biggest = 0;
+    for (int i = 0; i < n_items; ++i) biggest = std::max(biggest, (size_t)items[i].width * items[i].height);
+    pinned_[0].ensure(biggest);
+    pinned_[1].ensure(biggest);
+    auto stage = [this, items](int i) {
+      stage_image(items[i].image, items[i].width, items[i].height, pinned_[i % 2].p);
+    };
+    std::future<void> next = std::async(std::launch::async, stage, 0);
+    all.clear();
+    try {
+      for (int i = 0; i < n_items; ++i) {
+        next.get();  // micrograph i is staged (rethrows a staging failure)
+        const gs_batch_item& it = items[i];
+        if (it.templates)
+          load_templates(it.templates, it.n_templates, it.template_width, it.template_height, it.alt_az);
+        upload_image(pinned_[i % 2].p, it.width, it.height, it.ctf);
+        // the other buffer was last read by the upload of micrograph i-1, which has completed
+        if (i + 1 < n_items) next = std::async(std::launch::async, stage, i + 1);
+        run();
+        all.insert(all.end(), records_.begin(), records_.end());
+        counts[i] = records_.size();
+      }
+    } catch (...) {
+      if (next.valid()) next.wait();
+      throw;
+    }
   }
 
   void run() {
@@ -351,6 +446,82 @@ class Engine {
       overlap_ = variant == 1;
   }
 
+  void set_graph(bool on) {
+    use_graph_ = on;
+    if (!on) drop_graphs();
+  }
+  void graph_stats(long long* captures, long long* launches) const {
+    if (captures) *captures = graph_captures_;
+    if (launches) *launches = graph_launches_;
+  }
+
+  // Times each result-invariant option of the norm_type=1 correlation kernels on a one-angle
+  // sample of the current micrograph and templates and keeps the fastest: the column-pass
+  // variant, then the row-pass variant, then the templates per launch.  Overwrites the results
+  // of the last run.
+  void autotune(int* col_out, int* row_out, int* chunk_out) {
+    GS_REQUIRE(P_.norm_type == 1, kUnsupported, "autotune covers the norm_type=1 correlation kernels");
+    GS_REQUIRE(n_ > 0, kState, "no templates loaded");
+    GS_REQUIRE(have_image_, kState, "no image set");
+    const int la = lim_angles_, lt = lim_templates_;
+    const bool prof = profile_, graph = use_graph_;
+    lim_angles_ = 1;
+    lim_templates_ = 0;
+    use_graph_ = false;
+    // kernel time of one run: 0 column pass only, 1 row pass only, 2 both
+    auto measure = [&](int what) {
+      profile_ = false;
+      run();  // warm (allocations, caches)
+      set_profiling(true);
+      run();
+      const double c = clk_col_.total_ms, r = clk_row_.total_ms;
+      return what == 0 ? c : what == 1 ? r : c + r;
+    };
+    try {
+      double best = 0;
+      int pick = col_variant_;
+      for (int v : {0, 3}) {
+        col_variant_ = v;
+        const double ms = measure(0);
+        if (v == 0 || ms < best) best = ms, pick = v;
+      }
+      col_variant_ = pick;
+      pick = row_variant_;
+      for (int v : {0, 1, 2, 3}) {
+        row_variant_ = v;
+        const double ms = measure(1);
+        if (v == 0 || ms < best) best = ms, pick = v;
+      }
+      row_variant_ = pick;
+      size_t free_b = 0, total_b = 0;
+      GS_HIP(hipMemGetInfo(&free_b, &total_b));
+      const int keep = chunk_;
+      pick = 0;
+      const int nb = nb_ > 0 ? nb_ : 1;
+      const double plane = (double)P_.window_size * (P_.window_size / 2) * sizeof(float2);
+      bool first = true;
+      for (int c : {32, 64, 128}) {
+        if (c > n_ && c != 32) break;
+        // the inter-pass buffer of c templates must fit in half the free memory (plus what it holds now)
+        if (plane * nb * c > 0.5 * (double)free_b + (double)d_Z_.n * sizeof(float2)) break;
+        chunk_ = c;
+        const double ms = measure(2);
+        if (first || ms < best) best = ms, pick = c;
+        first = false;
+      }
+      chunk_ = pick ? pick : keep;
+    } catch (...) {
+      lim_angles_ = la, lim_templates_ = lt, use_graph_ = graph;
+      set_profiling(prof);
+      throw;
+    }
+    lim_angles_ = la, lim_templates_ = lt, use_graph_ = graph;
+    set_profiling(prof);
+    if (col_out) *col_out = col_variant_;
+    if (row_out) *row_out = row_variant_;
+    if (chunk_out) *chunk_out = chunk_;
+  }
+
   const gs_params& params() const { return P_; }
   float alt(int j) const { return (j >= 0 && j < (int)alt_.size()) ? alt_[j] : 0.f; }
   float az(int j) const { return (j >= 0 && j < (int)az_.size()) ? az_[j] : 0.f; }
@@ -366,6 +537,14 @@ class Engine {
   }
 
   void bind() { GS_HIP(hipSetDevice(device_)); }
+
+  void drop_graphs() {
+    for (auto& g : graphs_) {
+      if (g.second.exec) (void)hipGraphExecDestroy(g.second.exec);
+      if (g.second.graph) (void)hipGraphDestroy(g.second.graph);
+    }
+    graphs_.clear();
+  }
 
   void set_params() {  // SetParams, src/norm.cu:183-192
     DevParams& d = dp_;
@@ -571,35 +750,74 @@ class Engine {
       GS_HIP(hipEventRecord(ev_sync_, stream_));
       GS_HIP(hipStreamWaitEvent(stream2_, ev_sync_, 0));
     }
-    long long chunk = 0;
-    for (int a = 0; a < na; ++a) {
-      // RotateTemplate (src/norm.cu:479-521)
-      const float cose = (float)std::cos((double)angles[a] * kRefPi / 180);
-      const float sine = (float)std::sin((double)angles[a] * kRefPi / 180);
-      launch_rotate_mask(d_tprep_.p, nt, l, cose, sine, P_.diameter, 4.0f, rot, part, stream_);
-      launch_rotation_stats(part, nt, l, stats, stream_);
-      launch_standardise_to_complex(rot, nt, l, stats, work, stream_);
-      half_spectra(work, nt, that, (float)(1.0 / (double)L2));
-      // correlation + per-pixel statistics (ComputeCCGSum / PickParticles in one pass)
-      for (int t0 = 0; t0 < nt; t0 += tc_max, ++chunk) {
-        const int tc = std::min(tc_max, nt - t0);
-        const int zb = overlap ? (int)(chunk % 2) : 0;
-        float2* Z = Zb + zb * zsz;
-        if (overlap && chunk >= 2) GS_HIP(hipStreamWaitEvent(stream_, ev_zfree_[zb], 0));
-        CorrArgs ca{that, ihat, Z, t0, tc, nb_};
-        clock_begin(clk_col_, stream_);
-        GS_REQUIRE(launch_colpass(l, ca, col_variant_, stream_), kUnsupported, "window_size not supported");
-        clock_end(clk_col_, stream_);
-        if (overlap) {
-          GS_HIP(hipEventRecord(ev_zready_[zb], stream_));
-          GS_HIP(hipStreamWaitEvent(stream2_, ev_zready_[zb], 0));
+    auto enqueue_angles = [&]() {
+      long long chunk = 0;
+      for (int a = 0; a < na; ++a) {
+        // RotateTemplate (src/norm.cu:479-521)
+        const float cose = (float)std::cos((double)angles[a] * kRefPi / 180);
+        const float sine = (float)std::sin((double)angles[a] * kRefPi / 180);
+        launch_rotate_mask(d_tprep_.p, nt, l, cose, sine, P_.diameter, 4.0f, rot, part, stream_);
+        launch_rotation_stats(part, nt, l, stats, stream_);
+        launch_standardise_to_complex(rot, nt, l, stats, work, stream_);
+        half_spectra(work, nt, that, (float)(1.0 / (double)L2));
+        // correlation + per-pixel statistics (ComputeCCGSum / PickParticles in one pass)
+        for (int t0 = 0; t0 < nt; t0 += tc_max, ++chunk) {
+          const int tc = std::min(tc_max, nt - t0);
+          const int zb = overlap ? (int)(chunk % 2) : 0;
+          float2* Z = Zb + zb * zsz;
+          if (overlap && chunk >= 2) GS_HIP(hipStreamWaitEvent(stream_, ev_zfree_[zb], 0));
+          CorrArgs ca{that, ihat, Z, t0, tc, nb_};
+          clock_begin(clk_col_, stream_);
+          GS_REQUIRE(launch_colpass(l, ca, col_variant_, stream_), kUnsupported, "window_size not supported");
+          clock_end(clk_col_, stream_);
+          if (overlap) {
+            GS_HIP(hipEventRecord(ev_zready_[zb], stream_));
+            GS_HIP(hipStreamWaitEvent(stream2_, ev_zready_[zb], 0));
+          }
+          clock_begin(clk_row_, rs);
+          launch_rowpass_pixel(l, Z, tc, nb_, d_S_.p, d_SS_.p, d_best_.p, d_arg_.p, a * n + t0,
+                               row_variant_, rs);
+          clock_end(clk_row_, rs);
+          if (overlap) GS_HIP(hipEventRecord(ev_zfree_[zb], stream2_));
         }
-        clock_begin(clk_row_, rs);
-        launch_rowpass_pixel(l, Z, tc, nb_, d_S_.p, d_SS_.p, d_best_.p, d_arg_.p, a * n + t0,
-                             row_variant_, rs);
-        clock_end(clk_row_, rs);
-        if (overlap) GS_HIP(hipEventRecord(ev_zfree_[zb], stream2_));
       }
+    };
+    if (use_graph_ && !overlap && !profile_) {
+      // The angle loop has no micrograph-dependent arguments: capture it once per problem shape
+      // and buffer set, then replay it for every later micrograph of that shape.
+      auto ptr = [](const void* p) { return (long long)(uintptr_t)p; };
+      const std::vector<long long> key{
+          nb_, n, nt, na, tc_max, l, col_variant_, row_variant_, ptr(d_tprep_.p), ptr(rot),
+          ptr(part), ptr(stats), ptr(work), ptr(that), ptr(d_scratch_.p), ptr(ihat), ptr(Zb),
+          ptr(d_S_.p), ptr(d_SS_.p), ptr(d_best_.p), ptr(d_arg_.p)};
+      auto it = graphs_.find(key);
+      if (it == graphs_.end()) {
+        // nothing may allocate while the stream is captured: the scratch buffer of the template
+        // transforms already holds n planes, the transform plan for l exists (template
+        // preparation made it), and the correlation twiddle tables are built here
+        GS_REQUIRE(d_scratch_.n >= (size_t)nt * L2, kInvalid, "scratch buffer too small for capture");
+        (void)fft_.plan(l);
+        corr_prepare();
+        if (graphs_.size() >= kMaxGraphs) drop_graphs();
+        hipGraph_t g = nullptr;
+        GS_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+        try {
+          enqueue_angles();
+        } catch (...) {
+          (void)hipStreamEndCapture(stream_, &g);
+          if (g) (void)hipGraphDestroy(g);
+          throw;
+        }
+        GS_HIP(hipStreamEndCapture(stream_, &g));
+        hipGraphExec_t ex = nullptr;
+        GS_HIP(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
+        it = graphs_.emplace(key, CapturedGraph{g, ex}).first;
+        ++graph_captures_;
+      }
+      GS_HIP(hipGraphLaunch(it->second.exec, stream_));
+      ++graph_launches_;
+    } else {
+      enqueue_angles();
     }
     if (overlap) {  // finalisation reads the accumulators written on stream2_
       GS_HIP(hipEventRecord(ev_sync_, stream2_));
@@ -782,7 +1000,6 @@ class Engine {
   int W_ = 0, H_ = 0;
   gs_ctf ctf_{};
   bool have_image_ = false, ran_ = false;
-  std::vector<float> host_img_;
   int nb_ = 0, bx_ = 0, nbk_ = 0, na_ = 0;
   std::vector<int> offx_, offy_, ulx_, uly_;
   std::vector<float> angles_;
@@ -790,6 +1007,11 @@ class Engine {
   double total_ms_ = 0, corr_ms_ = 0;
   std::vector<gs_record> records_;
   std::vector<double> planes_host_;
+  static constexpr size_t kMaxGraphs = 8;
+  bool use_graph_ = false;
+  std::map<std::vector<long long>, CapturedGraph> graphs_;
+  long long graph_captures_ = 0, graph_launches_ = 0;
+  PinnedBuf pinned_[2];
   std::map<long long, std::unique_ptr<DevBuf<double>>> rb_cache_;
 
   DevBuf<float> d_raw_, d_img_, d_imgpre_, d_tprep_, d_rot_, d_stats_, d_best_, d_tile_;
@@ -1018,6 +1240,35 @@ gs_status gs_set_chunk(gs_engine* e, int templates_per_launch) {
 gs_status gs_set_variant(gs_engine* e, int kernel, int variant) {
   if (!e) return fail(GS_ERR_INVALID, "null engine");
   return guarded([&] { e->impl->set_variant(kernel, variant); });
+}
+
+gs_status gs_search_batch(gs_engine* e, const gs_batch_item* items, int n_items, gs_record** out,
+                          size_t* counts) {
+  if (!e || !items || !out || !counts || n_items <= 0) return fail(GS_ERR_INVALID, "null argument or empty batch");
+  return guarded([&] {
+    std::vector<gs_record> all;
+    e->impl->search_batch(items, n_items, all, counts);
+    gs_record* p = (gs_record*)std::malloc(sizeof(gs_record) * std::max<size_t>(all.size(), 1));
+    if (!p) throw std::bad_alloc();
+    if (!all.empty()) std::memcpy(p, all.data(), sizeof(gs_record) * all.size());
+    *out = p;
+  });
+}
+
+gs_status gs_set_graph(gs_engine* e, int on) {
+  if (!e) return fail(GS_ERR_INVALID, "null engine");
+  return guarded([&] { e->impl->set_graph(on != 0); });
+}
+
+gs_status gs_graph_stats(const gs_engine* e, long long* captures, long long* launches) {
+  if (!e) return fail(GS_ERR_INVALID, "null engine");
+  e->impl->graph_stats(captures, launches);
+  return GS_OK;
+}
+
+gs_status gs_autotune(gs_engine* e, int* col_variant, int* row_variant, int* templates_per_launch) {
+  if (!e) return fail(GS_ERR_INVALID, "null engine");
+  return guarded([&] { e->impl->autotune(col_variant, row_variant, templates_per_launch); });
 }
 
 }  // extern "C"

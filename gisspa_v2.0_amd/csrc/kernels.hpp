@@ -85,6 +85,9 @@ bool launch_rowpass_plane(int L, const float2* Z, int tc, int nb, float4* partia
                           hipStream_t s);
 int rowpass_groups(int L);
 bool corr_supported(int L);
+// Builds the correlation kernels' twiddle tables on the current device (they are otherwise
+// built on first use, which must not happen inside a stream capture).
+void corr_prepare();
 // Reduce plane partials to (peak, index, sum, sum2) per (template, block).
 void launch_finalize_plane(const float4* partial, int tc, int nb, int groups, double* out,
                            hipStream_t s);

@@ -42,6 +42,14 @@ class Record(C.Structure):
     ]
 
 
+class BatchItem(C.Structure):
+    _fields_ = [
+        ("image", C.c_void_p), ("width", C.c_int32), ("height", C.c_int32), ("ctf", Ctf),
+        ("templates", C.c_void_p), ("n_templates", C.c_int32), ("template_width", C.c_int32),
+        ("template_height", C.c_int32), ("alt_az", C.c_void_p),
+    ]
+
+
 RECORD_DTYPE = np.dtype([
     ("x", "<f4"), ("y", "<f4"), ("template_id", "<i4"), ("psi", "<f4"), ("score", "<f4"),
     ("angle_index", "<i4"), ("block", "<i4"), ("pixel", "<i4"), ("local_x", "<f4"),
@@ -72,6 +80,10 @@ SIGNATURES = {
     "gs_set_chunk": (C.c_int, [_P, C.c_int]),
     "gs_set_variant": (C.c_int, [_P, C.c_int, C.c_int]),
     "gs_copy_bandwidth": (C.c_int, [C.c_int, C.c_size_t, C.c_int, C.POINTER(C.c_double)]),
+    "gs_search_batch": (C.c_int, [_P, C.POINTER(BatchItem), C.c_int, C.POINTER(_P), C.POINTER(C.c_size_t)]),
+    "gs_set_graph": (C.c_int, [_P, C.c_int]),
+    "gs_graph_stats": (C.c_int, [_P, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
+    "gs_autotune": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
 }
 
 _lib = None
@@ -203,6 +215,60 @@ class Engine:
 
     def set_variant(self, kernel: int, variant: int):
         _check(load().gs_set_variant(self._h, kernel, variant))
+
+    def set_graph(self, on: bool):
+        _check(load().gs_set_graph(self._h, 1 if on else 0))
+
+    def graph_stats(self):
+        a, b = C.c_longlong(), C.c_longlong()
+        _check(load().gs_graph_stats(self._h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def autotune(self):
+        """Returns (column-pass variant, row-pass variant, templates per launch) chosen."""
+        c, r, k = C.c_int(), C.c_int(), C.c_int()
+        _check(load().gs_autotune(self._h, C.byref(c), C.byref(r), C.byref(k)))
+        return c.value, r.value, k.value
+
+    def search_batch(self, items) -> list:
+        """items: sequence of dicts with 'image', 'ctf' and optionally 'templates', 'alt_az'.
+        Returns one record array per item (gs_search_batch)."""
+        keep = []
+        arr = (BatchItem * len(items))()
+        for i, it in enumerate(items):
+            img = _f32(it["image"])
+            keep.append(img)
+            b = arr[i]
+            b.image = img.ctypes.data
+            b.height, b.width = img.shape
+            b.ctf = Ctf(*it.get("ctf", (1.5, 0.0, 0.0)))
+            t = it.get("templates")
+            if t is not None:
+                t = _f32(t)
+                keep.append(t)
+                b.templates = t.ctypes.data
+                b.n_templates, b.template_height, b.template_width = t.shape
+                aa = it.get("alt_az")
+                if aa is not None:
+                    aa = _f32(aa).reshape(-1, 2)
+                    keep.append(aa)
+                    b.alt_az = aa.ctypes.data
+        counts = (C.c_size_t * len(items))()
+        p = C.c_void_p()
+        _check(load().gs_search_batch(self._h, arr, len(items), C.byref(p), counts))
+        try:
+            total = sum(counts)
+            buf = (C.c_char * (total * RECORD_DTYPE.itemsize)).from_address(p.value) if total else b""
+            allrec = np.frombuffer(bytes(buf), dtype=RECORD_DTYPE).copy()
+        finally:
+            load().gs_free(p)
+        if items:
+            self._ctf = Ctf(*items[-1].get("ctf", (1.5, 0.0, 0.0)))
+        out, o = [], 0
+        for c in counts:
+            out.append(allrec[o:o + c])
+            o += c
+        return out
 
     def kernel_timing(self, which: int):
         avg = C.c_double()

@@ -153,6 +153,44 @@ gs_status gs_set_chunk(gs_engine* e, int templates_per_launch);
  * overlapping the next column pass through a double-buffered inter-pass buffer. */
 gs_status gs_set_variant(gs_engine* e, int kernel, int variant);
 
+/* ---- batching ---- */
+
+/* One micrograph of a batch.  image/width/height/ctf as in gs_search.  templates (n_templates x
+ * template_height x template_width float32) and alt_az (may be NULL) replace the engine's
+ * template set before this micrograph, as gs_load_templates; templates == NULL keeps the set of
+ * the previous item (or of the last gs_load_templates for the first item). */
+typedef struct {
+  const float* image;
+  int32_t width, height;
+  gs_ctf ctf;
+  const float* templates;
+  int32_t n_templates, template_width, template_height;
+  const float* alt_az;
+} gs_batch_item;
+
+/* Replaces main's per-micrograph loop (src/main.cu:37-49, one engine built and run per
+ * micrograph): searches n_items micrographs in order on one engine whose buffers persist, with
+ * the host-side clipping of micrograph i+1 staged on a helper thread while micrograph i runs on
+ * the device.  counts[i] receives the number of records of item i; *out receives all records
+ * concatenated in item order (free with gs_free).  The records of each item equal those of
+ * gs_search on that item with the same template set.  All items are validated before any
+ * work starts. */
+gs_status gs_search_batch(gs_engine* e, const gs_batch_item* items, int n_items, gs_record** out,
+                          size_t* counts);
+
+/* When on (default off), the norm_type=1 angle loop is captured as a HIP graph the first time a
+ * problem shape and buffer set occurs and replayed for later runs of the same shape; turning it
+ * off releases the captured graphs.  Profiling runs are never captured. */
+gs_status gs_set_graph(gs_engine* e, int on);
+/* Number of graph captures and graph launches so far. */
+gs_status gs_graph_stats(const gs_engine* e, long long* captures, long long* launches);
+
+/* Times the result-invariant options of the norm_type=1 correlation kernels (column-pass and
+ * row-pass variants, templates per launch) on a one-angle sample of the current image and
+ * templates, keeps the fastest and reports them (any output pointer may be NULL).  Overwrites
+ * the results of the last run. */
+gs_status gs_autotune(gs_engine* e, int* col_variant, int* row_variant, int* templates_per_launch);
+
 #ifdef __cplusplus
 }
 #endif

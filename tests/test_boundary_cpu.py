@@ -53,6 +53,37 @@ def test_null_arguments(gs):
     assert lib.gs_run(None) == 1
     lib.gs_destroy(None)
     lib.gs_free(None)
+    counts = (ctypes.c_size_t * 1)()
+    p = ctypes.c_void_p()
+    assert lib.gs_search_batch(None, None, 1, ctypes.byref(p), counts) == 1
+    assert lib.gs_set_graph(None, 1) == 1
+    assert lib.gs_autotune(None, None, None, None) == 1
+
+
+def test_struct_layouts_match_the_header(gs, tmp_path):
+    """The ctypes mirrors have the C layout (sizes and field offsets from the compiled header)."""
+    fields = {
+        "gs_params": (gs.Params, [f[0] for f in gs.Params._fields_]),
+        "gs_record": (gs.Record, [f[0] for f in gs.Record._fields_]),
+        "gs_batch_item": (gs.BatchItem, [f[0] for f in gs.BatchItem._fields_]),
+        "gs_ctf": (gs.Ctf, [f[0] for f in gs.Ctf._fields_]),
+    }
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "gisspa.h"', "int main(void) {"]
+    for cname, (_, names) in fields.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for n in names:
+            lines.append(f'printf("{cname} {n} %zu\\n", offsetof({cname}, {n}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l}
+    for cname, (cls, names) in fields.items():
+        assert got[(cname, "size")] == ctypes.sizeof(cls), cname
+        for n in names:
+            assert got[(cname, n)] == getattr(cls, n).offset, (cname, n)
 
 
 def _dump(cmd):
