@@ -81,6 +81,68 @@ def cpu_baseline(cfg, ds, nt, na):
     }
 
 
+REFINE_PEAKS = 2000
+REFINE_TOL = (0.05, 0.05, 0.1, 5.0)  # px, px, deg, Angstrom
+
+
+def refine_leg(eng, cfg, ds, no_cpu, sample=16):
+    """Adjoint refinement of the top REFINE_PEAKS block-pixel records of the search just run
+    (re-selected at z >= 5 without NMS), each perturbed by U(+-0.5 px) in x and y, U(+-2.5 deg)
+    in theta and U(+-300 A) in defocus, 20 ascent steps of 0.1 (scale 2 px, 2 px, 3 deg,
+    2000 A).  'at tol': a sample of the picks is refined from the same start by the FP64 oracle
+    with the same frozen mu/sigma; within_tol is the fraction whose four parameters agree
+    within (0.05 px, 0.05 px, 0.1 deg, 5 A)."""
+    recs = eng.select(5.0, 0.0)[:REFINE_PEAKS]
+    pk = eng.peaks_from_records(recs)
+    rng = np.random.default_rng(3000)
+    k = len(pk)
+    pk["dx"] = rng.uniform(-0.5, 0.5, k)
+    pk["dy"] = rng.uniform(-0.5, 0.5, k)
+    pk["theta"] += rng.uniform(-2.5, 2.5, k)
+    pk["ddf"] = rng.uniform(-300.0, 300.0, k)
+    eng.refine(pk[:16])  # warm (whitened template spectra, buffers)
+    t0 = time.perf_counter()
+    out = eng.refine(pk)
+    dt = time.perf_counter() - t0
+    dev_ms = eng.refine_timing()
+    log(f"refinement: {k} picks in {dt:.3f} s (device {dev_ms:.1f} ms)")
+    res = {
+        "picks": k, "iterations": 20, "peaks_per_s": k / dt, "wall_s": dt, "device_ms": dev_ms,
+        "mean_z_start": float(out["score0"].mean()), "mean_z_refined": float(out["score"].mean()),
+        "tolerance": list(REFINE_TOL),
+    }
+    if no_cpu:
+        return res
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    oracle.load()
+    idx = np.linspace(0, k - 1, sample).astype(int)
+    mu_all = eng.stage(4, np.float64)
+    sg_all = eng.stage(5, np.float64)
+    l = cfg.window_size
+    q = pk["block"][idx].astype(np.int64) * l * l + pk["pixel"][idx]
+    opk = np.zeros(sample, dtype=oracle.PEAK_DTYPE)
+    for n in pk.dtype.names:
+        opk[n] = pk[n][idx]
+    R = oracle.Refiner(cfg, ds.image, ds.templates, ds.ctf)
+    R.float_geometry(True)
+    t0 = time.perf_counter()
+    ref, zo, _, _ = R.run(opk, mu_all[q], sg_all[q], iters=20, step=0.1, scale=(2.0, 2.0, 3.0, 2000.0))
+    cpu_dt = time.perf_counter() - t0
+    R.close()
+    dev = np.stack([np.abs(out["peak"][n][idx] - ref[n]) for n in ("dx", "dy", "theta", "ddf")], 1)
+    ok = np.all(dev <= np.array(REFINE_TOL), axis=1)
+    res.update({
+        "oracle_sample": sample, "within_tol": float(ok.mean()),
+        "max_deviation": [float(v) for v in dev.max(0)],
+        "max_score_rel_dev": float(np.max(np.abs(out["score"][idx] - zo) / np.abs(zo))),
+        "cpu_peaks_per_s": sample / cpu_dt, "cpu_cores": oracle.num_threads(),
+    })
+    log(f"refinement oracle sample: {ok.mean() * 100:.0f}% within tolerance, {cpu_dt:.1f} s on CPU")
+    return res
+
+
 def c1_config(gisspa):
     return gisspa.SearchConfig(angpix=1.0, phistep=5.0, kk=3.0, energy=300.0, cs=2.7, highres=8.0,
                                lowres=100.0, diameter=128.0, threshold=7.0, window_size=320,
@@ -283,6 +345,8 @@ def main():
         "pipeline_ms": {"device_total": total_ms, "angle_loop": corr_ms},
         "copy_ceiling_GBps": gisspa.copy_bandwidth(1 << 30, 20),
     }
+    if not a.angles and a.templates == 1000:
+        out["refine"] = refine_leg(eng, cfg, ds, a.no_cpu)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cfg, ds, a.cpu_templates, a.cpu_angles)
     else:

@@ -251,6 +251,7 @@ class Engine {
     n_ = n;
     w_ = w;
     h_ = h;
+    white_ok_ = false;
     alt_.assign(n, 0.f);
     az_.assign(n, 0.f);
     if (alt_az)
@@ -522,6 +523,137 @@ class Engine {
     if (chunk_out) *chunk_out = chunk_;
   }
 
+  // Records of the last norm_type=1 run selected again at another threshold and NMS radius from
+  // the per-pixel records kept on the device (update_CCG + the host scan, src/kernels.cu:709-721
+  // and src/norm.cu:583-597, without a new search).  The run's own records are unchanged.
+  std::vector<gs_record> select(float threshold, float nms_radius) {
+    GS_REQUIRE(P_.norm_type == 1, kUnsupported, "select applies to norm_type=1 runs");
+    GS_REQUIRE(ran_, kState, "no completed run");
+    bind();
+    const int max_cand = 1 << 20;
+    GS_HIP(hipMemsetAsync(d_count_.p, 0, sizeof(int), stream_));
+    launch_finalize_norm(d_S_.p, d_SS_.p, d_best_.p, d_arg_.p, nb_, P_.window_size, total_n_, threshold,
+                         d_offx_.p, d_offy_.p, bx_, W_, H_, P_.diameter, d_mu_.p, d_sigma_.p, d_cand_.p,
+                         d_count_.p, max_cand, stream_);
+    GS_HIP(hipStreamSynchronize(stream_));
+    std::vector<gs_record> keep;
+    keep.swap(records_);
+    try {
+      collect_norm();
+    } catch (...) {
+      records_.swap(keep);
+      throw;
+    }
+    std::vector<gs_record> out;
+    out.swap(records_);
+    records_.swap(keep);
+    nms(out, nms_radius);
+    return out;
+  }
+
+  // A norm_type=1 record as a pick to refine: its block, pixel and template, zero shift and
+  // defocus offset, and theta = the angle-list value of its angle index (psi = 360 - theta).
+  gs_peak peak_from_record(const gs_record& r) const {
+    GS_REQUIRE(P_.norm_type == 1, kUnsupported, "picks are defined for norm_type=1 records");
+    const auto angles = angle_list(P_.phistep);
+    GS_REQUIRE(r.angle_index >= 0 && r.angle_index < (int)angles.size(), kInvalid, "angle index out of range");
+    gs_peak p{};
+    p.block = r.block;
+    p.pixel = r.pixel;
+    p.template_id = r.template_id;
+    p.theta = angles[r.angle_index];
+    return p;
+  }
+
+  // Adjoint refinement of k picks on the micrograph of the last norm_type=1 run: `iters` ascent
+  // steps p += step * scale^2 * dz/dp per parameter, then a final evaluation.  iters = 0 only
+  // evaluates z and its gradient.  mu and sigma of each pick's block pixel stay as the run
+  // computed them.  Outputs (any may be null): refined picks, final z, starting z, final
+  // gradient (k x 4: per px, px, deg, Angstrom).
+  void refine(const gs_peak* in, size_t k, int iters, double step, const double* scale, gs_peak* out_pk,
+              double* z, double* z0, double* grad) {
+    GS_REQUIRE(P_.norm_type == 1, kUnsupported, "refinement is defined for norm_type=1");
+    GS_REQUIRE(ran_ && have_image_ && n_ > 0 && nb_ > 0, kState, "refinement needs a completed norm_type=1 run");
+    GS_REQUIRE(in && k > 0, kInvalid, "no picks");
+    GS_REQUIRE(iters >= 0 && iters <= 100000, kInvalid, "iterations out of range");
+    GS_REQUIRE(std::isfinite(step) && scale && std::isfinite(scale[0]) && std::isfinite(scale[1]) &&
+                   std::isfinite(scale[2]) && std::isfinite(scale[3]),
+               kInvalid, "step and scales must be finite");
+    const int l = P_.window_size;
+    const size_t L2 = (size_t)l * l;
+    for (size_t i = 0; i < k; ++i) {
+      const gs_peak& p = in[i];
+      GS_REQUIRE(p.block >= 0 && p.block < nb_ && p.pixel >= 0 && (size_t)p.pixel < L2 && p.template_id >= 0 &&
+                     p.template_id < n_,
+                 kInvalid, "pick block, pixel or template out of range");
+      GS_REQUIRE(std::isfinite(p.dx) && std::isfinite(p.dy) && std::isfinite(p.theta) && std::isfinite(p.ddf),
+                 kInvalid, "pick parameters must be finite");
+    }
+    bind();
+    set_params();
+    GS_HIP(hipEventRecord(ev_start_, stream_));
+    if (!white_ok_) {
+      whiten_templates(d_twhite_.ensure((size_t)n_ * L2), d_scratch_.ensure((size_t)n_ * L2));
+      white_ok_ = true;
+    }
+    const double count = ref_region_count(l);
+    const int Kc = (int)std::min<size_t>(k, kRefineChunk);
+    float2* H = d_rH_.ensure((size_t)Kc * L2);
+    float2* AB = d_rAB_.ensure((size_t)2 * Kc * L2);
+    float2* S = d_rS_.ensure((size_t)2 * Kc * L2);  // also the rotation scratch (3 floats per pixel)
+    RefPeak* rp = d_rpk_.ensure(Kc);
+    double2* part = d_rpart_.ensure((size_t)Kc * 16);
+    double* part6 = d_rpart6_.ensure((size_t)Kc * l * 6);
+    double* stats = d_rstats_.ensure((size_t)Kc * 8);
+    double* cg = d_rcg_.ensure((size_t)Kc * 5);
+    double* dz = d_rz_.ensure(Kc);
+    double* dz0 = d_rz0_.ensure(Kc);
+    double* dg = d_rgrad_.ensure((size_t)Kc * 4);
+    std::vector<RefPeak> hp(Kc);
+    std::vector<double> hz(Kc), hz0(Kc), hg((size_t)Kc * 4);
+    for (size_t c0 = 0; c0 < k; c0 += Kc) {
+      const int K = (int)std::min<size_t>(Kc, k - c0);
+      for (int j = 0; j < K; ++j) {
+        const gs_peak& p = in[c0 + j];
+        hp[j] = RefPeak{p.block, p.pixel, p.template_id, 0, p.dx, p.dy, p.theta, p.ddf};
+      }
+      GS_HIP(hipMemcpyAsync(rp, hp.data(), sizeof(RefPeak) * K, hipMemcpyHostToDevice, stream_));
+      for (int it = 0; it <= iters; ++it) {
+        launch_ref_template(d_twhite_.p, rp, K, l, dp_, part, count, H, stream_);
+        fft2_batched(fft_, H, S, K, l, l, +1, 1.f, stream_);
+        launch_ref_rotate(H, rp, K, l, P_.diameter, 4.0f, reinterpret_cast<float*>(S), part6, stats, AB, stream_);
+        fft2_batched(fft_, AB, S, 2 * K, l, l, -1, (float)(1.0 / (double)L2), stream_);
+        GS_REQUIRE(launch_ref_dot(d_ihat_.p, AB, rp, K, l, cg, stream_), kUnsupported, "window_size not supported");
+        launch_ref_update(rp, K, cg, d_mu_.p, d_sigma_.p, l, step, scale, it == 0, it < iters, dz, dz0, dg, stream_);
+      }
+      GS_HIP(hipMemcpyAsync(hp.data(), rp, sizeof(RefPeak) * K, hipMemcpyDeviceToHost, stream_));
+      GS_HIP(hipMemcpyAsync(hz.data(), dz, sizeof(double) * K, hipMemcpyDeviceToHost, stream_));
+      GS_HIP(hipMemcpyAsync(hz0.data(), dz0, sizeof(double) * K, hipMemcpyDeviceToHost, stream_));
+      GS_HIP(hipMemcpyAsync(hg.data(), dg, sizeof(double) * 4 * K, hipMemcpyDeviceToHost, stream_));
+      GS_HIP(hipStreamSynchronize(stream_));
+      for (int j = 0; j < K; ++j) {
+        const size_t i = c0 + j;
+        if (out_pk) {
+          out_pk[i] = in[i];
+          out_pk[i].dx = hp[j].dx;
+          out_pk[i].dy = hp[j].dy;
+          out_pk[i].theta = hp[j].theta;
+          out_pk[i].ddf = hp[j].ddf;
+        }
+        if (z) z[i] = hz[j];
+        if (z0) z0[i] = hz0[j];
+        if (grad)
+          for (int q = 0; q < 4; ++q) grad[4 * i + q] = hg[4 * j + q];
+      }
+    }
+    GS_HIP(hipEventRecord(ev_end_, stream_));
+    GS_HIP(hipEventSynchronize(ev_end_));
+    float ms = 0;
+    GS_HIP(hipEventElapsedTime(&ms, ev_start_, ev_end_));
+    refine_ms_ = ms;
+  }
+  double refine_ms() const { return refine_ms_; }
+
   const gs_params& params() const { return P_; }
   float alt(int j) const { return (j >= 0 && j < (int)alt_.size()) ? alt_[j] : 0.f; }
   float az(int j) const { return (j >= 0 && j < (int)az_.size()) ? az_[j] : 0.f; }
@@ -617,11 +749,11 @@ class Engine {
 
   // Template preprocessing into full complex planes in work_ (src/norm.cu:194-354 or
   // src/nonorm.cu:178-344).  norm: ends in real space; nonorm: ends in Fourier space.
-  void prep_templates() {
+  // Padded, edge-standardised, transformed and whitened templates (src/norm.cu:132-318 up to
+  // the weighting) into T (n planes l x l); S is scratch of the same size.
+  void whiten_templates(float2* T, float2* S) {
     const int l = P_.window_size, n = n_;
     const size_t L2 = (size_t)l * l;
-    float2* T = d_work_.ensure((size_t)n * L2);
-    float2* S = d_scratch_.ensure((size_t)n * L2);
     launch_pad_templates(d_raw_.p, n, w_, h_, l, T, stream_);
     double* mom = d_mom_.ensure((size_t)n * 4);
     launch_template_moments(T, n, l, mom, stream_);
@@ -630,6 +762,14 @@ class Engine {
     double* ra = d_ra_.ensure((size_t)n * (l / 2));
     launch_radial_power_planes(T, n, l, ra, stream_);
     launch_whiten_planes(T, n, l, ra, radial_counts_dev(l, l), stream_);
+  }
+
+  void prep_templates() {
+    const int l = P_.window_size, n = n_;
+    const size_t L2 = (size_t)l * l;
+    float2* T = d_work_.ensure((size_t)n * L2);
+    float2* S = d_scratch_.ensure((size_t)n * L2);
+    whiten_templates(T, S);
     if (!P_.norm_type) {  // src/nonorm.cu:291-299
       fft2_batched(fft_, T, S, n, l, l, +1, 1.f, stream_);
       launch_mask_planes(T, n, l, P_.diameter, 4.0f, stream_);
@@ -827,6 +967,7 @@ class Engine {
     // set_CCG_mean: n = N_t * int(360/phistep) (src/norm.cu:549-550)
     double total_n = (double)n * (int)(360 / P_.phistep);
     if (lim_angles_ > 0 || lim_templates_ > 0) total_n = (double)nt * na;
+    total_n_ = total_n;
     const int max_cand = 1 << 20;
     Cand* cand = d_cand_.ensure(max_cand);
     int* count = d_count_.ensure(1);
@@ -1007,6 +1148,13 @@ class Engine {
   double total_ms_ = 0, corr_ms_ = 0;
   std::vector<gs_record> records_;
   std::vector<double> planes_host_;
+  double total_n_ = 0, refine_ms_ = 0;
+  static constexpr size_t kRefineChunk = 1024;  // picks per refinement batch (about 4 MB each at l=320)
+  bool white_ok_ = false;
+  DevBuf<float2> d_twhite_, d_rH_, d_rAB_, d_rS_;
+  DevBuf<RefPeak> d_rpk_;
+  DevBuf<double2> d_rpart_;
+  DevBuf<double> d_rpart6_, d_rstats_, d_rcg_, d_rz_, d_rz0_, d_rgrad_;
   static constexpr size_t kMaxGraphs = 8;
   bool use_graph_ = false;
   std::map<std::vector<long long>, CapturedGraph> graphs_;
@@ -1269,6 +1417,65 @@ gs_status gs_graph_stats(const gs_engine* e, long long* captures, long long* lau
 gs_status gs_autotune(gs_engine* e, int* col_variant, int* row_variant, int* templates_per_launch) {
   if (!e) return fail(GS_ERR_INVALID, "null engine");
   return guarded([&] { e->impl->autotune(col_variant, row_variant, templates_per_launch); });
+}
+
+gs_status gs_select(gs_engine* e, float threshold, float nms_radius, gs_record** out, size_t* n_out) {
+  if (!e || !out || !n_out) return fail(GS_ERR_INVALID, "null argument");
+  return guarded([&] {
+    const auto r = e->impl->select(threshold, nms_radius);
+    gs_record* p = (gs_record*)std::malloc(sizeof(gs_record) * std::max<size_t>(r.size(), 1));
+    if (!p) throw std::bad_alloc();
+    if (!r.empty()) std::memcpy(p, r.data(), sizeof(gs_record) * r.size());
+    *out = p;
+    *n_out = r.size();
+  });
+}
+
+gs_refine_opts gs_refine_defaults(void) {
+  gs_refine_opts o{};
+  o.iterations = 20;
+  o.step = 0.1;
+  o.scale[0] = 2.0;
+  o.scale[1] = 2.0;
+  o.scale[2] = 3.0;
+  o.scale[3] = 2000.0;
+  return o;
+}
+
+gs_status gs_peak_from_record(const gs_engine* e, const gs_record* r, gs_peak* out) {
+  if (!e || !r || !out) return fail(GS_ERR_INVALID, "null argument");
+  return guarded([&] { *out = e->impl->peak_from_record(*r); });
+}
+
+gs_status gs_score_and_grad(gs_engine* e, const gs_peak* peaks, size_t k, double* score, double* grad) {
+  if (!e || !peaks || !score || !grad) return fail(GS_ERR_INVALID, "null argument");
+  return guarded([&] {
+    const double unit[4] = {1, 1, 1, 1};
+    e->impl->refine(peaks, k, 0, 0.0, unit, nullptr, score, nullptr, grad);
+  });
+}
+
+gs_status gs_refine(gs_engine* e, const gs_peak* peaks, size_t k, const gs_refine_opts* opts,
+                    gs_refined* out) {
+  if (!e || !peaks || !out) return fail(GS_ERR_INVALID, "null argument");
+  return guarded([&] {
+    const gs_refine_opts o = opts ? *opts : gs_refine_defaults();
+    std::vector<gs_peak> pk(k);
+    std::vector<double> z(k), z0(k), g(4 * k);
+    e->impl->refine(peaks, k, o.iterations, o.step, o.scale, pk.data(), z.data(), z0.data(), g.data());
+    for (size_t i = 0; i < k; ++i) {
+      out[i].peak = pk[i];
+      out[i].score = z[i];
+      out[i].score0 = z0[i];
+      for (int q = 0; q < 4; ++q) out[i].grad[q] = g[4 * i + q];
+    }
+  });
+}
+
+gs_status gs_refine_timing(const gs_engine* e, double* ms) {
+  if (!e || !ms) return fail(GS_ERR_INVALID, "null argument");
+  *ms = e->impl->refine_ms();
+  return GS_OK;
 }
 
 }  // extern "C"

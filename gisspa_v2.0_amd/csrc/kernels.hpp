@@ -88,6 +88,32 @@ bool corr_supported(int L);
 // Builds the correlation kernels' twiddle tables on the current device (they are otherwise
 // built on first use, which must not happen inside a stream capture).
 void corr_prepare();
+
+// ---- adjoint refinement of norm_type=1 picks ----
+// Device record of one pick: block, in-block pixel, template, and the refined parameters
+// (sub-pixel shift px, rotation angle deg, defocus offset Angstrom).
+struct RefPeak {
+  int32_t block, pixel, t, pad;
+  double dx, dy, theta, ddf;
+};
+// Number of half-plane spectrum entries with radial bin < l/2 (the normalisation count).
+double ref_region_count(int l);
+// Weighted, normalised spectrum of each pick's template at its defocus, packed with its
+// defocus derivative: H = G + i dG (K planes of l*l).
+void launch_ref_template(const float2* white, const RefPeak* pk, int K, int l, const DevParams& p,
+                         double2* partial, double count, float2* H, hipStream_t s);
+// From the inverse transform t + i dt in H: rotation, mask, standardisation and their theta and
+// defocus derivatives, packed into AB (A = r + i dr/dtheta in planes 0..K-1, B = dr/d ddf in
+// planes K..2K-1).  f3 holds 3*K*l*l floats of scratch.
+void launch_ref_rotate(const float2* H, const RefPeak* pk, int K, int l, float d_m, float ehw,
+                       float* f3, double* partial6, double* stats, float2* AB, hipStream_t s);
+// c and dc/d(dx, dy, theta, ddf) per pick (out: K x 5) from the transformed AB.
+bool launch_ref_dot(const float2* ihat, const float2* AB, const RefPeak* pk, int K, int l, double* out,
+                    hipStream_t s);
+// z and gradient per pick; when update, one ascent step p += step * scale^2 * dz/dp.
+void launch_ref_update(RefPeak* pk, int K, const double* cg, const double* mu, const double* sigma, int l,
+                       double step, const double* scale4, bool first, bool update, double* z, double* z0,
+                       double* grad, hipStream_t s);
 // Reduce plane partials to (peak, index, sum, sum2) per (template, block).
 void launch_finalize_plane(const float4* partial, int tc, int nb, int groups, double* out,
                            hipStream_t s);

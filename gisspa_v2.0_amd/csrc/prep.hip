@@ -6,6 +6,7 @@
 #include <cmath>
 
 #include "common.hpp"
+#include "device_common.hpp"
 #include "kernels.hpp"
 
 namespace gs {
@@ -17,61 +18,11 @@ constexpr int BS = 256;
 
 inline unsigned nblk(long long n, int bs = BS) { return (unsigned)((n + bs - 1) / bs); }
 
-// floor(hypot(dx,dy)+0.5)-1 evaluated exactly on the integer squared radius.
-__device__ __forceinline__ int rbin(int dx, int dy) {
-  const int n = dx * dx + dy * dy;
-  int k = (int)sqrtf((float)n);
-  while (k * k > n) --k;
-  while ((k + 1) * (k + 1) <= n) ++k;
-  if (n > k * k + k) ++k;
-  return k - 1;
-}
-
-// CTF_AST (src/kernels.cu:290-317), mode 0 sign / 2 magnitude / else signed.
-__device__ double ctf_ast(int x1, int y1, int nx, int ny, const DevParams& p, int mode) {
-  y1 = x1 > nx / 2 ? ny - y1 : y1;
-  x1 = min(x1, nx - x1);
-  const double dx = min(x1, nx - x1);
-  const double dy = y1 - ny / 2;
-  const double ss = (dx * dx / (double)(nx * nx) + dy * dy / (double)(ny * ny)) /
-                    ((double)p.apix * (double)p.apix);
-  const double ag = atan2((double)(y1 - ny / 2), (double)x1);
-  const double df_ast = 0.5 * (p.dfu + p.dfv + 2 * p.dfdiff * cos(2 * (p.dfang * kRefPi / 180 - ag)));
-  const double lam = p.lambda;
-  const double gamma = -2 * kRefPi * (p.cs * 2.5e6 * lam * lam * lam * ss * ss + df_ast * 5000.0 * lam * ss);
-  const double v = sqrt(1.0 - p.ampconst * p.ampconst) * sin(gamma) + p.ampconst * cos(gamma);
-  if (mode == 0) return v > 0 ? 1.0 : -1.0;
-  if (mode == 2) return fabs(v);
-  return v;
-}
-
-__device__ __forceinline__ double band_template(int r, float hr, float lr) {  // k:263-275
-  if (r < hr && r >= 0) return 1.0;
-  if (r >= hr && r < hr + 8) return 0.5 * cos(kRefPi * (r - (double)hr) / 16) + 0.5;
-  if (r >= (lr - 8) && r < lr && r >= 0) return 0.5 * cos(kRefPi * ((double)lr - r) / 16) + 0.5;
-  return 0.0;
-}
-
-__device__ __forceinline__ double band_image(int r, float hr, float lr) {  // k:183-189
-  if (r < hr && r >= lr) return 1.0;
-  if (r >= hr && r < hr + 8) return 0.5 * cos(kRefPi * (r - (double)hr) / 16) + 0.5;
-  if (r >= (lr - 8) && r < lr && r >= 0) return 0.5 * cos(kRefPi * ((double)lr - r) / 16) + 0.5;
-  return 0.0;
-}
-
-// Block-wide sum of a double (blockDim.x a multiple of 64, at most 1024).
-__device__ double block_sum(double v) {
-  __shared__ double red[16];
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-  const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
-  __syncthreads();
-  if (lane == 0) red[w] = v;
-  __syncthreads();
-  double t = 0;
-  if (threadIdx.x == 0)
-    for (int i = 0; i < (int)(blockDim.x / 64); ++i) t += red[i];
-  return t;  // valid in thread 0
-}
+using dev::band_image;
+using dev::band_template;
+using dev::block_sum;
+using dev::ctf_ast;
+using dev::rbin;
 
 // ---------------------------------------------------------------- templates
 __global__ void k_pad_templates(const float* __restrict__ raw, int n, int w, int h, int l,
@@ -363,31 +314,8 @@ __global__ void k_split_blocks(const float* __restrict__ img, int nx, int ny, in
 }
 
 // ---------------------------------------------------------------- rotation
-// Bilinear sample geometry of rotate_subIMG (k:534-603), float arithmetic with explicit FMA.
-struct Bil {
-  int k0, k1, k2, k3;
-  float t, u;
-  bool ok;
-};
-__device__ __forceinline__ Bil bil_coords(int i, int j, int l, float cose, float sine) {
-  Bil o;
-  const float x = (float)(i - l / 2), y = (float)(j - l / 2);
-  const float half = (float)(l / 2);
-  const float x2 = __fmaf_rn(cose, x, sine * y) + half;
-  const float y2 = __fmaf_rn(-sine, x, cose * y) + half;
-  o.ok = !(x2 < 0 || x2 > l - 1.0f || y2 < 0 || y2 > l - 1.0f);
-  if (!o.ok) return o;
-  const int ii = (int)floorf(x2), jj = (int)floorf(y2);
-  o.k0 = ii + jj * l;
-  o.k1 = o.k0 + 1;
-  o.k2 = o.k0 + l + 1;
-  o.k3 = o.k0 + l;
-  if (ii == l - 1) { o.k1 -= 1; o.k2 -= 1; }
-  if (jj == l - 1) { o.k2 -= l; o.k3 -= l; }
-  o.t = x2 - (float)ii;
-  o.u = y2 - (float)jj;
-  return o;
-}
+using dev::Bil;
+using dev::bil_coords;
 
 // Rotate + mask one template row per block; per-row double partial sums for mean/var.
 __global__ void k_rotate_mask(const float* __restrict__ tp, int l, float cose, float sine, float rin,

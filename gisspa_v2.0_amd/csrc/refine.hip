@@ -1,0 +1,434 @@
+// Adjoint refinement of norm_type=1 picks: z(dx, dy, theta, ddf) and its exact gradient.
+//
+// No reference counterpart (the reference's picks stay on the pixel grid, the phistep angle
+// grid and the micrograph defocus).  The score is the search's correlation made continuous:
+//
+//   c = sum over the half spectrum of w(kx) Re[ Ihat_b(k) (-1)^(kx+ky) conj(T(k)) E(k, p + d) ]
+//
+// with Ihat_b the block spectrum as the correlation kernels use it (1/l^2 and the odd-l/2 sign
+// folded in), T the spectrum / l^2 of the template processed at defocus + ddf (whitened
+// spectrum of src/norm.cu:194-318, apply_weighting_function + CTF_AST k:243-317, spectral
+// normalisation k:319-382,445-458, inverse transform src/norm.cu:350), rotated by theta,
+// masked and standardised (RotateTemplate, src/norm.cu:479-521); E the band-limited shift
+// (signed frequency, cos(pi q) at Nyquist) and w = 1 on the kx = 0 and l/2 columns, else 2.
+// z = (c - mu_p) / sigma_p with mu_p, sigma_p frozen at the pick's block pixel.  At integer
+// shift, theta on the angle grid and ddf = 0 this is the search's z.
+//
+// One evaluation of K picks (batched, one plane per pick):
+//   k_ref_power / k_ref_pack  weighted spectrum G at the pick's defocus and dG/d ddf, spectrally
+//                             normalised, packed as G + i dG (both have real inverses)
+//   inverse 2-D transform    -> t + i dt (processed template and its defocus derivative)
+//   k_ref_rotate / k_ref_stats / k_ref_standardise
+//                             r, dr/dtheta (bilinear derivative), dr/d ddf through mask and
+//                             box standardisation, packed as A = r + i dr/dtheta, B = dr/d ddf
+//   forward 2-D transform    of A and B
+//   k_ref_dot                 c and its four derivatives (phase ramps for the shift)
+//   k_ref_update              z, gradient, ascent step
+#include <algorithm>
+#include <cmath>
+
+#include "common.hpp"
+#include "device_common.hpp"
+#include "kernels.hpp"
+
+namespace gs {
+
+namespace {
+
+using dev::band_template;
+using dev::bil_coords;
+using dev::Bil;
+using dev::block_sum;
+using dev::ctf_signed;
+using dev::kRefPi;
+using dev::rbin;
+
+constexpr int BS = 256;
+constexpr int kPowerChunks = 16;  // row chunks of the power reduction per pick
+
+// Template weight of apply_weighting_function (k:243-288) at (x, y) and its derivative with
+// respect to the defocus offset: f = band |v| g, g = (N + kk v^2)^-1/2, df = band sign(v) v' N g^3.
+__device__ __forceinline__ void weight_deriv(int x, int y, int l, const DevParams& p, double* f,
+                                             double* df) {
+  const int fx = min(x, l - x), fy = min(y, l - y);
+  const int rr = rbin(fx, fy);
+  const double bp = band_template(rr, p.hr_t, p.lr_t);
+  *f = bp;
+  *df = 0.0;
+  if (bp == 0.0) return;
+  if (rr < l / 2 && rr >= 0) {
+    const double r = hypot((double)fy, (double)fx);
+    const double ss = r * (double)p.ds;
+    double dv = 0;
+    const double v = ctf_signed(x, (y + l / 2) % l, l, l, p, &dv);
+    const double signal = exp(p.bf * ss * ss + p.bf2 * ss + p.bf3) / ((double)p.kk + 1);
+    const double N = exp(p.a * ss * ss + p.b * ss + p.b2) / signal;
+    const double g = sqrt(1 / (N + (double)p.kk * v * v));
+    *f = bp * fabs(v) * g;
+    *df = bp * (v > 0 ? 1.0 : (v < 0 ? -1.0 : 0.0)) * dv * N * g * g * g;
+  }
+}
+
+// The pick's micrograph parameters with its defocus offset applied.
+__device__ __forceinline__ DevParams at_offset(const DevParams& p, double ddf) {
+  DevParams q = p;
+  q.dfu -= ddf * 1e-4;
+  q.dfv -= ddf * 1e-4;
+  return q;
+}
+
+// Partial sums of |G|^2 and d|G|^2/d ddf over the normalisation region (x <= l/2, radial bin
+// < l/2) of compute_area_sum_ofSQR (k:319-382); grid (kPowerChunks, K).
+__global__ void k_ref_power(const float2* __restrict__ white, const RefPeak* __restrict__ pk, int l,
+                            DevParams p, double2* __restrict__ partial) {
+  const int j = blockIdx.y, ch = blockIdx.x;
+  const RefPeak P = pk[j];
+  const DevParams q = at_offset(p, P.ddf);
+  const float2* F = white + (long long)P.t * l * l;
+  const int hx = l / 2 + 1;
+  const int rows = (l + kPowerChunks - 1) / kPowerChunks;
+  const int y0 = ch * rows, y1 = min(l, y0 + rows);
+  double s = 0, ds = 0;
+  for (int e = threadIdx.x; e < (y1 - y0) * hx; e += blockDim.x) {
+    const int y = y0 + e / hx, x = e % hx;
+    const int r = rbin(min(x, l - x), min(y, l - y));
+    if (r >= l / 2 || r < 0) continue;
+    double f, df;
+    weight_deriv(x, y, l, q, &f, &df);
+    if (f == 0.0 && df == 0.0) continue;
+    const float2 v = F[(long long)y * l + x];
+    const double a2 = (double)v.x * v.x + (double)v.y * v.y;
+    s += f * f * a2;
+    ds += 2 * f * df * a2;
+  }
+  s = block_sum(s);
+  ds = block_sum(ds);
+  if (threadIdx.x == 0) partial[(long long)j * kPowerChunks + ch] = make_double2(s, ds);
+}
+
+// H = G/m + i (dG/m - G dm/m^2), m = sqrt(sum |G|^2) / count (normalize, k:445-458); the
+// region count is the number of half-plane entries with radial bin < l/2.
+__global__ void k_ref_pack(const float2* __restrict__ white, const RefPeak* __restrict__ pk, int l,
+                           DevParams p, const double2* __restrict__ partial, double count,
+                           float2* __restrict__ H) {
+  const int j = blockIdx.y;
+  const RefPeak P = pk[j];
+  __shared__ double sm[2];
+  if (threadIdx.x == 0) {
+    double s = 0, ds = 0;
+    for (int c = 0; c < kPowerChunks; ++c) {
+      s += partial[(long long)j * kPowerChunks + c].x;
+      ds += partial[(long long)j * kPowerChunks + c].y;
+    }
+    const double m = sqrt(s) / count;
+    sm[0] = m;
+    sm[1] = ds / (2 * sqrt(s) * count);
+  }
+  __syncthreads();
+  const double m = sm[0], dm = sm[1];
+  const DevParams q = at_offset(p, P.ddf);
+  const long long L2 = (long long)l * l;
+  const float2* F = white + (long long)P.t * L2;
+  float2* out = H + (long long)j * L2;
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < L2; e += gridDim.x * blockDim.x) {
+    const int y = e / l, x = e % l;
+    double f, df;
+    weight_deriv(x, y, l, q, &f, &df);
+    float2 h = make_float2(0.f, 0.f);
+    if (f != 0.0 || df != 0.0) {
+      const float2 v = F[e];
+      const double a = f / m, b = df / m - f * dm / (m * m);
+      // (a + i b) * v
+      h = make_float2((float)(a * v.x - b * v.y), (float)(a * v.y + b * v.x));
+    }
+    out[e] = h;
+  }
+}
+
+// Rotation (float geometry as the search), bilinear derivative, mask; one row of one pick per
+// block.  Writes r0, dr0/dtheta, dr0/d ddf (unstandardised) and per-row double sums.
+__global__ void k_ref_rotate(const float2* __restrict__ H, const RefPeak* __restrict__ pk, int l,
+                             float rin, float rout, float ehw, float* __restrict__ f3,
+                             double* __restrict__ partial6, int K) {
+  const int jrow = blockIdx.x, j = blockIdx.y;
+  const RefPeak P = pk[j];
+  const float cose = (float)cos(P.theta * kRefPi / 180);
+  const float sine = (float)sin(P.theta * kRefPi / 180);
+  const float deg = (float)(kRefPi / 180);
+  const long long L2 = (long long)l * l;
+  const float2* T = H + (long long)j * L2;
+  float* R0 = f3 + (long long)j * L2;
+  float* GT = f3 + ((long long)K + j) * L2;
+  float* GD = f3 + ((long long)2 * K + j) * L2;
+  double a[6] = {0, 0, 0, 0, 0, 0};
+  for (int i = threadIdx.x; i < l; i += blockDim.x) {
+    float v = 0.f, vth = 0.f, vdf = 0.f;
+    const Bil b = bil_coords(i, jrow, l, cose, sine);
+    if (b.ok) {
+      const float tt = 1.0f - b.t, uu = 1.0f - b.u;
+      const float2 h0 = T[b.k0], h1 = T[b.k1], h2 = T[b.k2], h3 = T[b.k3];
+      v = h0.x * tt * uu + h1.x * b.t * uu + h2.x * b.t * b.u + h3.x * tt * b.u;
+      vdf = h0.y * tt * uu + h1.y * b.t * uu + h2.y * b.t * b.u + h3.y * tt * b.u;
+      const float dvx = uu * (h1.x - h0.x) + b.u * (h2.x - h3.x);
+      const float dvy = tt * (h3.x - h0.x) + b.t * (h2.x - h1.x);
+      const float x = (float)(i - l / 2), y = (float)(jrow - l / 2);
+      vth = (dvx * (-sine * x + cose * y) + dvy * (-cose * x - sine * y)) * deg;
+    }
+    const float m = dev::mask_factor(i, jrow, l, rin, rout, ehw);
+    v *= m;
+    vth *= m;
+    vdf *= m;
+    const long long q = (long long)jrow * l + i;
+    R0[q] = v;
+    GT[q] = vth;
+    GD[q] = vdf;
+    a[0] += v;
+    a[1] += (double)v * v;
+    a[2] += vth;
+    a[3] += (double)v * vth;
+    a[4] += vdf;
+    a[5] += (double)v * vdf;
+  }
+  for (int k = 0; k < 6; ++k) {
+    const double t = block_sum(a[k]);
+    if (threadIdx.x == 0) partial6[((long long)j * l + jrow) * 6 + k] = t;
+  }
+}
+
+// Per pick: the search's float mean/variance (k_rotation_stats) and the derivatives of the
+// mean and standard deviation.  stats[j] = {meanf, varf, mean, sd, dmean_th, dsd_th, dmean_df, dsd_df}
+__global__ void k_ref_stats(const double* __restrict__ partial6, int l, int K, double* __restrict__ stats) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= K) return;
+  double s[6] = {0, 0, 0, 0, 0, 0};
+  for (int r = 0; r < l; ++r)
+    for (int k = 0; k < 6; ++k) s[k] += partial6[((long long)j * l + r) * 6 + k];
+  const double L2 = (double)l * l;
+  const double mean = s[0] / L2, var = s[1] / L2 - mean * mean, sd = sqrt(var);
+  const double dmt = s[2] / L2, dvt = 2 * (s[3] / L2 - mean * dmt);
+  const double dmd = s[4] / L2, dvd = 2 * (s[5] / L2 - mean * dmd);
+  double* o = stats + (long long)j * 8;
+  o[0] = (double)(float)mean;
+  o[1] = (double)(float)var;
+  o[2] = mean;
+  o[3] = sd;
+  o[4] = dmt;
+  o[5] = dvt / (2 * sd);
+  o[6] = dmd;
+  o[7] = dvd / (2 * sd);
+}
+
+// A = r + i dr/dtheta, B = dr/d ddf (+ 0i); r exactly as k_standardise_to_complex forms it.
+__global__ void k_ref_standardise(const float* __restrict__ f3, const double* __restrict__ stats, int l,
+                                  int K, float2* __restrict__ AB) {
+  const int j = blockIdx.y;
+  const long long L2 = (long long)l * l;
+  const double* st = stats + (long long)j * 8;
+  const float meanf = (float)st[0], varf = (float)st[1];
+  const float mean = (float)st[2], sd = (float)st[3];
+  const float dmt = (float)st[4], dst = (float)st[5], dmd = (float)st[6], dsd = (float)st[7];
+  const float inv = 1.0f / sd, inv2 = 1.0f / (sd * sd);
+  const float* R0 = f3 + (long long)j * L2;
+  const float* GT = f3 + ((long long)K + j) * L2;
+  const float* GD = f3 + ((long long)2 * K + j) * L2;
+  float2* A = AB + (long long)j * L2;
+  float2* B = AB + ((long long)K + j) * L2;
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < L2; e += (long long)gridDim.x * blockDim.x) {
+    const float r0 = R0[e];
+    float v = r0;
+    if (meanf != 0) v = v - meanf;
+    if (varf != 0) v = v / sqrtf(varf);
+    const float c = r0 - mean;
+    const float dth = (GT[e] - dmt) * inv - c * dst * inv2;
+    const float ddf = (GD[e] - dmd) * inv - c * dsd * inv2;
+    A[e] = make_float2(v, dth);
+    B[e] = make_float2(ddf, 0.f);
+  }
+}
+
+// Band-limited shift factor ph(k, q) = exp(2 pi i k_s q / l) (cos(pi q) at Nyquist) and d/dq.
+__device__ __forceinline__ void phase(int k, int l, int pint, double frac, float2& ph, float2& dph) {
+  if (2 * k == l) {
+    const double q = pint + frac;
+    double s, c;
+    sincospi(q, &s, &c);
+    ph = make_float2((float)c, 0.f);
+    dph = make_float2((float)(-M_PI * s), 0.f);
+    return;
+  }
+  const int ks = k < l / 2 ? k : k - l;
+  // angle / (2 pi) = (ks * pint mod l) / l + ks * frac / l, reduced exactly in integers first
+  long long m = ((long long)ks * pint) % l;
+  if (m < 0) m += l;
+  const double turns = (double)m / l + (double)ks * frac / l;
+  double s, c;
+  sincospi(2.0 * turns, &s, &c);
+  ph = make_float2((float)c, (float)s);
+  const double w = 2 * M_PI * ks / l;
+  dph = make_float2((float)(-w * s), (float)(w * c));
+}
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// c and its derivatives for one pick per block (256 threads; rows ky of the half spectrum).
+template <int L>
+__global__ __launch_bounds__(256) void k_ref_dot(const float2* __restrict__ ihat, const float2* __restrict__ AB,
+                                                 const RefPeak* __restrict__ pk, int K,
+                                                 double* __restrict__ out) {
+  constexpr int HX = L / 2 + 1;
+  __shared__ float2 phx[HX], dphx[HX], phy[L], dphy[L];
+  const int j = blockIdx.x;
+  const RefPeak P = pk[j];
+  const int px = P.pixel % L, py = P.pixel / L;
+  for (int k = threadIdx.x; k < HX; k += blockDim.x) phase(k, L, px, P.dx, phx[k], dphx[k]);
+  for (int k = threadIdx.x; k < L; k += blockDim.x) phase(k, L, py, P.dy, phy[k], dphy[k]);
+  __syncthreads();
+  const long long L2 = (long long)L * L;
+  const float2* I = ihat + (long long)P.block * HX * L;
+  const float2* A = AB + (long long)j * L2;
+  const float2* B = AB + ((long long)K + j) * L2;
+  double c = 0, gx = 0, gy = 0, gt = 0, gd = 0;
+  for (int ky = threadIdx.x; ky < L; ky += blockDim.x) {
+    const int my = (L - ky) % L;
+    const float2 ey = phy[ky], dey = dphy[ky];
+    float fc = 0, fgx = 0, fgy = 0, fgt = 0, fgd = 0;
+    for (int kx = 0; kx < HX; ++kx) {
+      const int mx = (L - kx) % L;
+      const float2 a = A[(long long)ky * L + kx], am = A[(long long)my * L + mx];
+      const float2 b = B[(long long)ky * L + kx], bm = B[(long long)my * L + mx];
+      // spectra of the three real fields
+      const float2 T = make_float2(0.5f * (a.x + am.x), 0.5f * (a.y - am.y));
+      const float2 Tt = make_float2(0.5f * (a.y + am.y), -0.5f * (a.x - am.x));
+      const float2 Td = make_float2(0.5f * (b.x + bm.x), 0.5f * (b.y - bm.y));
+      float2 X = I[(long long)kx * L + ky];
+      if ((kx + ky) & 1) X = make_float2(-X.x, -X.y);
+      const float w = (kx == 0 || 2 * kx == L) ? 1.f : 2.f;
+      const float2 E = cmul(phx[kx], ey);
+      const float2 Ex = cmul(dphx[kx], ey);
+      const float2 Ey = cmul(phx[kx], dey);
+      // X conj(T) and its partners
+      const float2 P0 = make_float2(X.x * T.x + X.y * T.y, X.y * T.x - X.x * T.y);
+      const float2 Pt = make_float2(X.x * Tt.x + X.y * Tt.y, X.y * Tt.x - X.x * Tt.y);
+      const float2 Pd = make_float2(X.x * Td.x + X.y * Td.y, X.y * Td.x - X.x * Td.y);
+      fc += w * (P0.x * E.x - P0.y * E.y);
+      fgx += w * (P0.x * Ex.x - P0.y * Ex.y);
+      fgy += w * (P0.x * Ey.x - P0.y * Ey.y);
+      fgt += w * (Pt.x * E.x - Pt.y * E.y);
+      fgd += w * (Pd.x * E.x - Pd.y * E.y);
+    }
+    c += fc;
+    gx += fgx;
+    gy += fgy;
+    gt += fgt;
+    gd += fgd;
+  }
+  c = block_sum(c);
+  gx = block_sum(gx);
+  gy = block_sum(gy);
+  gt = block_sum(gt);
+  gd = block_sum(gd);
+  if (threadIdx.x == 0) {
+    double* o = out + (long long)j * 5;
+    o[0] = c;
+    o[1] = gx;
+    o[2] = gy;
+    o[3] = gt;
+    o[4] = gd;
+  }
+}
+
+// z, gradient and (unless last) one ascent step p += step * scale^2 * dz/dp.
+__global__ void k_ref_update(RefPeak* __restrict__ pk, int K, const double* __restrict__ cg,
+                             const double* __restrict__ mu, const double* __restrict__ sigma, int l,
+                             double step, double4 scale, int first, int update,
+                             double* __restrict__ zout, double* __restrict__ z0out,
+                             double* __restrict__ gout) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= K) return;
+  RefPeak P = pk[j];
+  const long long q = (long long)P.block * l * l + P.pixel;
+  const double m = mu[q], s = sigma[q];
+  const double* o = cg + (long long)j * 5;
+  const double z = s > 0 ? (o[0] - m) / s : 0.0;
+  const double inv = s > 0 ? 1.0 / s : 0.0;
+  const double g0 = o[1] * inv, g1 = o[2] * inv, g2 = o[3] * inv, g3 = o[4] * inv;
+  zout[j] = z;
+  if (first) z0out[j] = z;
+  gout[4 * j + 0] = g0;
+  gout[4 * j + 1] = g1;
+  gout[4 * j + 2] = g2;
+  gout[4 * j + 3] = g3;
+  if (update) {
+    P.dx += step * scale.x * scale.x * g0;
+    P.dy += step * scale.y * scale.y * g1;
+    P.theta += step * scale.z * scale.z * g2;
+    P.ddf += step * scale.w * scale.w * g3;
+    pk[j] = P;
+  }
+}
+
+inline unsigned nblk(long long n, int bs = BS) { return (unsigned)((n + bs - 1) / bs); }
+
+}  // namespace
+
+double ref_region_count(int l) {
+  double cnt = 0;
+  for (int y = 0; y < l; ++y)
+    for (int x = 0; x <= l / 2; ++x) {
+      const long long dx = std::min(x, l - x), dy = std::min(y, l - y);
+      const long long n = dx * dx + dy * dy;
+      long long k = (long long)std::sqrt((double)n);
+      while (k * k > n) --k;
+      while ((k + 1) * (k + 1) <= n) ++k;
+      if (n > k * k + k) ++k;
+      const long long r = k - 1;
+      if (r >= 0 && r < l / 2) cnt += 1;
+    }
+  return cnt;
+}
+
+void launch_ref_template(const float2* white, const RefPeak* pk, int K, int l, const DevParams& p,
+                         double2* partial, double count, float2* H, hipStream_t s) {
+  k_ref_power<<<dim3(kPowerChunks, K), BS, 0, s>>>(white, pk, l, p, partial);
+  GS_CHECK_LAUNCH();
+  k_ref_pack<<<dim3(nblk((long long)l * l, BS), K), BS, 0, s>>>(white, pk, l, p, partial, count, H);
+  GS_CHECK_LAUNCH();
+}
+
+void launch_ref_rotate(const float2* H, const RefPeak* pk, int K, int l, float d_m, float ehw,
+                       float* f3, double* partial6, double* stats, float2* AB, hipStream_t s) {
+  const float dm = 1.5f * d_m;  // apply_mask: d_m = 1.5 * d_m (k:233)
+  const float rin = dm / 2, rout = dm / 2 + 2 * ehw;
+  k_ref_rotate<<<dim3(l, K), BS, 0, s>>>(H, pk, l, rin, rout, ehw, f3, partial6, K);
+  GS_CHECK_LAUNCH();
+  k_ref_stats<<<nblk(K, 64), 64, 0, s>>>(partial6, l, K, stats);
+  GS_CHECK_LAUNCH();
+  k_ref_standardise<<<dim3(nblk((long long)l * l, BS), K), BS, 0, s>>>(f3, stats, l, K, AB);
+  GS_CHECK_LAUNCH();
+}
+
+bool launch_ref_dot(const float2* ihat, const float2* AB, const RefPeak* pk, int K, int l, double* out,
+                    hipStream_t s) {
+  if (l == 320) {
+    k_ref_dot<320><<<K, 256, 0, s>>>(ihat, AB, pk, K, out);
+  } else if (l == 720) {
+    k_ref_dot<720><<<K, 256, 0, s>>>(ihat, AB, pk, K, out);
+  } else {
+    return false;
+  }
+  GS_CHECK_LAUNCH();
+  return true;
+}
+
+void launch_ref_update(RefPeak* pk, int K, const double* cg, const double* mu, const double* sigma, int l,
+                       double step, const double* scale4, bool first, bool update, double* z, double* z0,
+                       double* grad, hipStream_t s) {
+  const double4 sc = make_double4(scale4[0], scale4[1], scale4[2], scale4[3]);
+  k_ref_update<<<nblk(K, 64), 64, 0, s>>>(pk, K, cg, mu, sigma, l, step, sc, first ? 1 : 0, update ? 1 : 0,
+                                          z, z0, grad);
+  GS_CHECK_LAUNCH();
+}
+
+}  // namespace gs

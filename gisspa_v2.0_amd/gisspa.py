@@ -50,6 +50,37 @@ class BatchItem(C.Structure):
     ]
 
 
+class Peak(C.Structure):
+    _fields_ = [
+        ("block", C.c_int32), ("pixel", C.c_int32), ("template_id", C.c_int32),
+        ("dx", C.c_double), ("dy", C.c_double), ("theta", C.c_double), ("ddf", C.c_double),
+    ]
+
+
+class RefineOpts(C.Structure):
+    _fields_ = [("iterations", C.c_int32), ("step", C.c_double), ("scale", C.c_double * 4)]
+
+
+class Refined(C.Structure):
+    _fields_ = [("peak", Peak), ("score", C.c_double), ("score0", C.c_double),
+                ("grad", C.c_double * 4)]
+
+
+# gs_peak as a numpy record (same layout as the C struct)
+PEAK_DTYPE = np.dtype({
+    "names": ["block", "pixel", "template_id", "dx", "dy", "theta", "ddf"],
+    "formats": ["<i4", "<i4", "<i4", "<f8", "<f8", "<f8", "<f8"],
+    "offsets": [0, 4, 8, 16, 24, 32, 40],
+    "itemsize": 48,
+})
+REFINED_DTYPE = np.dtype({
+    "names": ["peak", "score", "score0", "grad"],
+    "formats": [PEAK_DTYPE, "<f8", "<f8", ("<f8", (4,))],
+    "offsets": [0, 48, 56, 64],
+    "itemsize": 96,
+})
+
+
 RECORD_DTYPE = np.dtype([
     ("x", "<f4"), ("y", "<f4"), ("template_id", "<i4"), ("psi", "<f4"), ("score", "<f4"),
     ("angle_index", "<i4"), ("block", "<i4"), ("pixel", "<i4"), ("local_x", "<f4"),
@@ -84,6 +115,12 @@ SIGNATURES = {
     "gs_set_graph": (C.c_int, [_P, C.c_int]),
     "gs_graph_stats": (C.c_int, [_P, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
     "gs_autotune": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "gs_select": (C.c_int, [_P, C.c_float, C.c_float, C.POINTER(_P), C.POINTER(C.c_size_t)]),
+    "gs_refine_defaults": (RefineOpts, []),
+    "gs_peak_from_record": (C.c_int, [_P, C.POINTER(Record), C.POINTER(Peak)]),
+    "gs_score_and_grad": (C.c_int, [_P, _P, C.c_size_t, _P, _P]),
+    "gs_refine": (C.c_int, [_P, _P, C.c_size_t, C.POINTER(RefineOpts), _P]),
+    "gs_refine_timing": (C.c_int, [_P, C.POINTER(C.c_double)]),
 }
 
 _lib = None
@@ -189,15 +226,63 @@ class Engine:
     def run(self):
         _check(load().gs_run(self._h))
 
-    def records(self) -> np.ndarray:
-        p = C.c_void_p()
-        n = C.c_size_t()
-        _check(load().gs_get_records(self._h, C.byref(p), C.byref(n)))
+    @staticmethod
+    def _take_records(p, n) -> np.ndarray:
         try:
             buf = (C.c_char * (n.value * RECORD_DTYPE.itemsize)).from_address(p.value) if n.value else b""
             return np.frombuffer(bytes(buf), dtype=RECORD_DTYPE).copy()
         finally:
             load().gs_free(p)
+
+    def records(self) -> np.ndarray:
+        p = C.c_void_p()
+        n = C.c_size_t()
+        _check(load().gs_get_records(self._h, C.byref(p), C.byref(n)))
+        return self._take_records(p, n)
+
+    def select(self, threshold: float, nms_radius: float) -> np.ndarray:
+        """Records of the last run at another threshold / NMS radius (gs_select)."""
+        p = C.c_void_p()
+        n = C.c_size_t()
+        _check(load().gs_select(self._h, threshold, nms_radius, C.byref(p), C.byref(n)))
+        return self._take_records(p, n)
+
+    def peaks_from_records(self, recs) -> np.ndarray:
+        out = np.zeros(len(recs), dtype=PEAK_DTYPE)
+        pk = Peak()
+        for i, rec in enumerate(recs):
+            r = Record(*[rec[name] for name in RECORD_DTYPE.names])
+            _check(load().gs_peak_from_record(self._h, C.byref(r), C.byref(pk)))
+            out[i] = (pk.block, pk.pixel, pk.template_id, pk.dx, pk.dy, pk.theta, pk.ddf)
+        return out
+
+    def score_and_grad(self, peaks):
+        """z (k,) and dz/d(dx, dy, theta, ddf) (k, 4) of picks (gs_score_and_grad)."""
+        p = np.ascontiguousarray(peaks, dtype=PEAK_DTYPE)
+        z = np.zeros(len(p))
+        g = np.zeros((len(p), 4))
+        _check(load().gs_score_and_grad(self._h, p.ctypes.data, len(p), z.ctypes.data, g.ctypes.data))
+        return z, g
+
+    def refine(self, peaks, iterations=None, step=None, scale=None) -> np.ndarray:
+        """Gradient ascent of picks (gs_refine); returns a REFINED_DTYPE array."""
+        p = np.ascontiguousarray(peaks, dtype=PEAK_DTYPE)
+        o = load().gs_refine_defaults()
+        if iterations is not None:
+            o.iterations = iterations
+        if step is not None:
+            o.step = step
+        if scale is not None:
+            for i in range(4):
+                o.scale[i] = scale[i]
+        out = np.zeros(len(p), dtype=REFINED_DTYPE)
+        _check(load().gs_refine(self._h, p.ctypes.data, len(p), C.byref(o), out.ctypes.data))
+        return out
+
+    def refine_timing(self) -> float:
+        ms = C.c_double()
+        _check(load().gs_refine_timing(self._h, C.byref(ms)))
+        return ms.value
 
     def search(self, image, ctf=(1.5, 0.0, 0.0)) -> np.ndarray:
         self.set_image(image, ctf)

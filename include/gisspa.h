@@ -191,6 +191,55 @@ gs_status gs_graph_stats(const gs_engine* e, long long* captures, long long* lau
  * the results of the last run. */
 gs_status gs_autotune(gs_engine* e, int* col_variant, int* row_variant, int* templates_per_launch);
 
+/* Records of the last norm_type=1 run selected again at another threshold and NMS radius from
+ * the per-pixel best records kept on the device (the scan of src/norm.cu:583-597 repeated
+ * without a new search); the run's own records are unchanged.  Free with gs_free. */
+gs_status gs_select(gs_engine* e, float threshold, float nms_radius, gs_record** out, size_t* n_out);
+
+/* ---- adjoint refinement (no reference counterpart: the reference's picks stay on the pixel
+ * grid, the phistep angle grid and the micrograph defocus) ----
+ * A pick on the micrograph of the last norm_type=1 run: its block, in-block pixel (y*l + x) and
+ * template (all from its gs_record), plus the continuous parameters refined: sub-pixel offset
+ * (dx, dy) px from the pixel, template rotation angle theta in degrees (the search's angle-list
+ * value; psi = 360 - theta) and a defocus offset ddf in Angstrom added to the micrograph
+ * defocus.  The score is z = (c - mu) / sigma, with c the search's correlation made continuous
+ * in all four parameters (band-limited shift, bilinear rotation, template preprocessing at the
+ * offset defocus; the micrograph side stays as searched) and mu, sigma frozen at the pick's
+ * block pixel.  At zero offsets and theta on the angle grid, z is the search's score. */
+typedef struct {
+  int32_t block, pixel, template_id;
+  double dx, dy, theta, ddf;
+} gs_peak;
+
+/* Ascent options: `iterations` steps p += step * scale^2 * dz/dp for each parameter p in
+ * (dx, dy, theta, ddf), i.e. gradient ascent with step `step` in the coordinates p / scale. */
+typedef struct {
+  int32_t iterations;
+  double step;
+  double scale[4];
+} gs_refine_opts;
+
+/* Defaults: 20 iterations, step 0.1, scale (2 px, 2 px, 3 deg, 2000 A). */
+gs_refine_opts gs_refine_defaults(void);
+
+typedef struct {
+  gs_peak peak;    /* refined parameters */
+  double score;    /* z at the refined parameters */
+  double score0;   /* z at the starting parameters */
+  double grad[4];  /* dz/d(dx, dy, theta, ddf) at the refined parameters (per px, px, deg, A) */
+} gs_refined;
+
+/* The pick of a norm_type=1 record: block, pixel, template, theta of its angle index, zero
+ * offsets. */
+gs_status gs_peak_from_record(const gs_engine* e, const gs_record* r, gs_peak* out);
+/* z and dz/d(dx, dy, theta, ddf) of k picks (score: k values, grad: k x 4). */
+gs_status gs_score_and_grad(gs_engine* e, const gs_peak* peaks, size_t k, double* score, double* grad);
+/* Gradient ascent of k picks (opts NULL = defaults); out receives k results. */
+gs_status gs_refine(gs_engine* e, const gs_peak* peaks, size_t k, const gs_refine_opts* opts,
+                    gs_refined* out);
+/* Device time (ms) of the last gs_refine or gs_score_and_grad call. */
+gs_status gs_refine_timing(const gs_engine* e, double* ms);
+
 #ifdef __cplusplus
 }
 #endif

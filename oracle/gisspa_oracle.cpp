@@ -19,28 +19,9 @@
 #endif
 
 #include "fft64.hpp"
+#include "oracle_impl.hpp"
 
-namespace {
-using orc::cd;
-
-constexpr double PI = 3.1415926535;  // include/utils/constants.h:3
-constexpr double EPS = 1e-8;         // include/utils/constants.h:4
-constexpr int TILE = 720;            // include/utils/constants.h:5
-
-// Physics constants of Parameters (include/EMReader/DataReader2.h:76-77), as the float values
-// the reference stores.
-struct Phys {
-  double edge_half_width = 4.0f, ampconst = 0.07f;
-  double a = -10.81f, b = 1.f, b2 = 0.32f, bf = -18.17f, bf2 = -15.22f, bf3 = 1.72f;
-};
-
-struct P {
-  // config (float in the reference)
-  float apix, kk, energy, cs, highres, lowres, d_m, thresh;
-  // per micrograph (SetParams, src/norm.cu:183-192)
-  double lambda, dfu, dfv, dfdiff, dfang, defocus;
-  Phys ph;
-};
+namespace orc_impl {
 
 P make_params(const orc_config* c, const double* ctf3) {
   P p{};
@@ -57,18 +38,10 @@ P make_params(const orc_config* c, const double* ctf3) {
   return p;
 }
 
-// floor(hypot(dx,dy) + 0.5) - 1 computed exactly (reference: k:118,136,165,264,329).
-inline int rbin(long long dx, long long dy) {
-  long long n = dx * dx + dy * dy;
-  long long k = (long long)std::sqrt((double)n);
-  while (k * k > n) --k;
-  while ((k + 1) * (k + 1) <= n) ++k;
-  if (n > k * k + k) ++k;  // sqrt(n) >= k + 1/2  <=>  n > k^2 + k
-  return (int)k - 1;
-}
-
-// CTF_AST (src/kernels.cu:290-317) in FP64.
-double ctf_ast(int x1, int y1, int nx, int ny, const P& p, int mode) {
+// CTF_AST (src/kernels.cu:290-317) in FP64: the signed CTF value and, when dv is given, its
+// derivative with respect to a defocus offset in Angstrom added to the micrograph defocus
+// (dfu and dfv both move by -offset*1e-4 um, so d gamma / d offset = PI * lambda * s^2).
+double ctf_signed(int x1, int y1, int nx, int ny, const P& p, double* dv) {
   y1 = x1 > nx / 2 ? ny - y1 : y1;
   x1 = std::min(x1, nx - x1);
   const double dx = std::min(x1, nx - x1);
@@ -81,17 +54,18 @@ double ctf_ast(int x1, int y1, int nx, int ny, const P& p, int mode) {
   const double lam = p.lambda;
   const double gamma =
       -2 * PI * ((double)p.cs * 2.5e6 * lam * lam * lam * ss * ss + df_ast * 5000.0 * lam * ss);
-  const double A = p.ph.ampconst;
-  const double v = std::sqrt(1.0 - A * A) * std::sin(gamma) + A * std::cos(gamma);
+  const double A = p.ph.ampconst, B = std::sqrt(1.0 - A * A);
+  if (dv) *dv = (B * std::cos(gamma) - A * std::sin(gamma)) * (PI * lam * ss);
+  return B * std::sin(gamma) + A * std::cos(gamma);
+}
+
+double ctf_ast(int x1, int y1, int nx, int ny, const P& p, int mode) {
+  const double v = ctf_signed(x1, y1, nx, ny, p, nullptr);
   if (mode == 0) return v > 0 ? 1.0 : -1.0;
   if (mode == 2) return std::fabs(v);
   return v;
 }
 
-// Band-pass radii are float expressions of config values in the reference (k:183-189,265-273).
-struct Band {
-  float hr, lr;
-};
 Band band(int l, const P& p) {
   Band b;
   b.hr = (float)l * p.apix / p.highres;
@@ -210,53 +184,26 @@ void spectral_normalise(cd* F, int l) {
     for (size_t i = 0; i < (size_t)l * l; ++i) F[i] /= m;
 }
 
-// apply_mask (k:225-241): soft disc on the real part.
-void apply_mask(cd* t, int l, double d_m_cfg, double ehw) {
+// apply_mask (k:225-241): soft disc on the real part; mask_factor is its value at (x, y).
+double mask_factor(int x, int y, int l, double d_m_cfg, double ehw) {
   const float d_m = 1.5f * (float)d_m_cfg;
   const double rout = (double)(d_m / 2 + 2 * (float)ehw), rin = (double)(d_m / 2);
+  const double r = std::hypot((double)(x - l / 2), (double)(y - l / 2));
+  if (r > rout) return 0.0;
+  if (r >= rin) return 0.5 * std::cos(PI * (r - rin) / (2 * ehw)) + 0.5;
+  return 1.0;
+}
+
+void apply_mask(cd* t, int l, double d_m_cfg, double ehw) {
   for (int y = 0; y < l; ++y)
     for (int x = 0; x < l; ++x) {
-      const double r = std::hypot((double)(x - l / 2), (double)(y - l / 2));
       cd& v = t[(size_t)y * l + x];
-      if (r > rout)
-        v = cd(0, v.imag());
-      else if (r >= rin)
-        v = cd(v.real() * (0.5 * std::cos(PI * (r - rin) / (2 * ehw)) + 0.5), v.imag());
+      v = cd(v.real() * mask_factor(x, y, l, d_m_cfg, ehw), v.imag());
     }
 }
 
 void scale_all(cd* x, size_t n, double s) {
   for (size_t i = 0; i < n; ++i) x[i] *= s;
-}
-
-// Rotation geometry of rotate_subIMG (k:534-603), evaluated in float exactly as the HIP path
-// does; returns false outside [0,l-1]^2.
-struct Bil {
-  int k0, k1, k2, k3;
-  float t, u;
-};
-inline bool bilinear_coords(int i, int j, int l, float cose, float sine, Bil& o) {
-  const float x = (float)(i - l / 2), y = (float)(j - l / 2);
-  const float half = (float)(l / 2);
-  const float x2 = std::fmaf(cose, x, sine * y) + half;
-  const float y2 = std::fmaf(-sine, x, cose * y) + half;
-  if (x2 < 0 || x2 > l - 1.0f || y2 < 0 || y2 > l - 1.0f) return false;
-  const int ii = (int)std::floor(x2), jj = (int)std::floor(y2);
-  o.k0 = ii + jj * l;
-  o.k1 = o.k0 + 1;
-  o.k2 = o.k0 + l + 1;
-  o.k3 = o.k0 + l;
-  if (ii == l - 1) { o.k1 -= 1; o.k2 -= 1; }
-  if (jj == l - 1) { o.k2 -= l; o.k3 -= l; }
-  o.t = x2 - (float)ii;
-  o.u = y2 - (float)jj;
-  return true;
-}
-
-// cos/sin of e*PI/180 rounded to float (k:538-539: double cos of a double argument).
-inline void rot_cs(float e, float& c, float& s) {
-  c = (float)std::cos((double)e * PI / 180);
-  s = (float)std::sin((double)e * PI / 180);
 }
 
 void rotate_plane(const cd* in, cd* out, int l, float theta) {
@@ -272,13 +219,6 @@ void rotate_plane(const cd* in, cd* out, int l, float theta) {
       }
       out[(size_t)j * l + i] = r;
     }
-}
-
-// Checkerboard of compute_corner_CCG (k:669-677): -1 where the reference negates.
-inline double checker(int x, int y, int l) {
-  const int of = (l / 2) % 2;
-  const int st = (of == y % 2) ? 1 : 0;
-  return ((x - st) % 2 == 0) ? -1.0 : 1.0;
 }
 
 std::vector<int> block_offsets(int length, int l, int overlap) {  // src/norm.cu:68-81
@@ -482,7 +422,9 @@ void check_config(const orc_config* c) {
   if (c->phistep <= 0) throw std::invalid_argument("phistep must be > 0");
 }
 
-}  // namespace
+}  // namespace orc_impl
+
+using namespace orc_impl;
 
 extern "C" {
 

@@ -90,6 +90,34 @@ int orc_nms(orc_pick* cand, int n, float radius);
  * complex l x l planes. */
 int orc_correlate(const double* I, const double* T, int l, double* out);
 
+/* ---- adjoint refinement of norm_type=1 picks (no reference counterpart) ----
+ * A pick is its block, in-block pixel (y*l + x) and template, plus the continuous parameters
+ * refined: sub-pixel offset (dx, dy) px, template rotation angle theta (deg; the search's
+ * angle-list value, psi = 360 - theta) and defocus offset ddf (Angstrom, added to the
+ * micrograph defocus).  z = (c - mu) / sigma with mu, sigma frozen at the pick's block pixel. */
+typedef struct {
+  int32_t block, pixel, template_id;
+  double dx, dy, theta, ddf;
+} orc_peak;
+
+/* Micrograph context: preprocessed block spectra and whitened template spectra.  NULL on
+ * failure; free with orc_refine_free. */
+void* orc_refine_create(const orc_config* c, const float* image, int W, int H, const float* tmpl,
+                        int n, int w, int h, const double* ctf3);
+void orc_refine_free(void* ctx);
+/* 1: rotation sample positions in float as the search computes them; 0 (default): double. */
+void orc_refine_geometry(void* ctx, int float_geometry);
+/* z, gradient dz/d(dx, dy, theta, ddf) (k x 4, per px, deg, Angstrom) and optionally c. */
+int orc_refine_eval(void* ctx, const orc_peak* peaks, int k, const double* mu, const double* sigma,
+                    double* z, double* grad, double* c);
+/* iters ascent steps p += step * scale^2 * dz/dp per parameter, then the final evaluation;
+ * peaks are updated in place; z0 (may be NULL) receives the starting z. */
+int orc_refine_run(void* ctx, orc_peak* peaks, int k, const double* mu, const double* sigma,
+                   int iters, double step, const double* scale4, double* z, double* grad, double* z0);
+/* Correlation plane Re(IDFT(I_b conj(T) checker))/l^2 of template t rotated by theta with its
+ * preprocessing at defocus offset ddf, through the search's own stage functions. */
+int orc_refine_plane(void* ctx, int block, int t, double theta, double ddf, double* out);
+
 int orc_num_threads(void);
 void orc_set_num_threads(int n);
 

@@ -66,6 +66,13 @@ def load():
         "orc_correlate": (C.c_int, [_P, _P, C.c_int, _P]),
         "orc_num_threads": (C.c_int, []),
         "orc_set_num_threads": (None, [C.c_int]),
+        "orc_refine_create": (_P, [C.POINTER(Config), _P, C.c_int, C.c_int, _P, C.c_int, C.c_int,
+                                   C.c_int, _P]),
+        "orc_refine_free": (None, [_P]),
+        "orc_refine_geometry": (None, [_P, C.c_int]),
+        "orc_refine_eval": (C.c_int, [_P, _P, C.c_int, _P, _P, _P, _P, _P]),
+        "orc_refine_run": (C.c_int, [_P, _P, C.c_int, _P, _P, C.c_int, C.c_double, _P, _P, _P, _P]),
+        "orc_refine_plane": (C.c_int, [_P, C.c_int, C.c_int, C.c_double, C.c_double, _P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -214,3 +221,67 @@ def num_threads():
 
 def set_num_threads(n):
     load().orc_set_num_threads(n)
+
+
+PEAK_DTYPE = np.dtype({
+    "names": ["block", "pixel", "template_id", "dx", "dy", "theta", "ddf"],
+    "formats": ["<i4", "<i4", "<i4", "<f8", "<f8", "<f8", "<f8"],
+    "offsets": [0, 4, 8, 16, 24, 32, 40],
+    "itemsize": 48,
+})
+
+
+class Refiner:
+    """FP64 adjoint refinement context of one micrograph (orc_refine_*)."""
+
+    def __init__(self, cfg, image, tmpl, ctf3):
+        img = np.ascontiguousarray(image, dtype=np.float32)
+        t = np.ascontiguousarray(tmpl, dtype=np.float32)
+        self._c = config(cfg)
+        d = np.ascontiguousarray(ctf3, dtype=np.float64)
+        self.l = cfg.window_size
+        self._h = load().orc_refine_create(C.byref(self._c), img.ctypes.data, img.shape[1], img.shape[0],
+                                           t.ctypes.data, t.shape[0], t.shape[2], t.shape[1], d.ctypes.data)
+        if not self._h:
+            raise RuntimeError("orc_refine_create failed")
+
+    def close(self):
+        if self._h:
+            load().orc_refine_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def float_geometry(self, on: bool):
+        load().orc_refine_geometry(self._h, 1 if on else 0)
+
+    def eval(self, peaks, mu, sigma):
+        """Returns z (k,), grad (k, 4) per (px, px, deg, Angstrom), c (k,)."""
+        p = np.ascontiguousarray(peaks, dtype=PEAK_DTYPE)
+        mu = np.ascontiguousarray(mu, dtype=np.float64)
+        sg = np.ascontiguousarray(sigma, dtype=np.float64)
+        z = np.zeros(len(p))
+        g = np.zeros((len(p), 4))
+        c = np.zeros(len(p))
+        _ok(load().orc_refine_eval(self._h, p.ctypes.data, len(p), mu.ctypes.data, sg.ctypes.data,
+                                   z.ctypes.data, g.ctypes.data, c.ctypes.data))
+        return z, g, c
+
+    def run(self, peaks, mu, sigma, iters=20, step=0.1, scale=(1.0, 1.0, 1.0, 1.0)):
+        """Returns refined peaks, final z, final grad, starting z."""
+        p = np.ascontiguousarray(peaks, dtype=PEAK_DTYPE).copy()
+        mu = np.ascontiguousarray(mu, dtype=np.float64)
+        sg = np.ascontiguousarray(sigma, dtype=np.float64)
+        sc = np.ascontiguousarray(scale, dtype=np.float64)
+        z = np.zeros(len(p))
+        z0 = np.zeros(len(p))
+        g = np.zeros((len(p), 4))
+        _ok(load().orc_refine_run(self._h, p.ctypes.data, len(p), mu.ctypes.data, sg.ctypes.data, iters,
+                                  step, sc.ctypes.data, z.ctypes.data, g.ctypes.data, z0.ctypes.data))
+        return p, z, g, z0
+
+    def plane(self, block, t, theta, ddf=0.0):
+        out = np.zeros(self.l * self.l)
+        _ok(load().orc_refine_plane(self._h, block, t, theta, ddf, out.ctypes.data))
+        return out.reshape(self.l, self.l)

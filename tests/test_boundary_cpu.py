@@ -67,6 +67,9 @@ def test_struct_layouts_match_the_header(gs, tmp_path):
         "gs_record": (gs.Record, [f[0] for f in gs.Record._fields_]),
         "gs_batch_item": (gs.BatchItem, [f[0] for f in gs.BatchItem._fields_]),
         "gs_ctf": (gs.Ctf, [f[0] for f in gs.Ctf._fields_]),
+        "gs_peak": (gs.Peak, [f[0] for f in gs.Peak._fields_]),
+        "gs_refine_opts": (gs.RefineOpts, [f[0] for f in gs.RefineOpts._fields_]),
+        "gs_refined": (gs.Refined, [f[0] for f in gs.Refined._fields_]),
     }
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "gisspa.h"', "int main(void) {"]
     for cname, (_, names) in fields.items():
@@ -84,6 +87,25 @@ def test_struct_layouts_match_the_header(gs, tmp_path):
         assert got[(cname, "size")] == ctypes.sizeof(cls), cname
         for n in names:
             assert got[(cname, n)] == getattr(cls, n).offset, (cname, n)
+    # the numpy mirrors of gs_peak / gs_refined used for arrays of picks
+    assert gs.PEAK_DTYPE.itemsize == got[("gs_peak", "size")]
+    assert [gs.PEAK_DTYPE.fields[n][1] for n in fields["gs_peak"][1]] == \
+        [got[("gs_peak", n)] for n in fields["gs_peak"][1]]
+    assert gs.REFINED_DTYPE.itemsize == got[("gs_refined", "size")]
+    assert [gs.REFINED_DTYPE.fields[n][1] for n in fields["gs_refined"][1]] == \
+        [got[("gs_refined", n)] for n in fields["gs_refined"][1]]
+
+
+def test_refine_defaults_and_argument_checks(gs):
+    lib = gs.load()
+    o = lib.gs_refine_defaults()
+    assert (o.iterations, o.step, list(o.scale)) == (20, 0.1, [2.0, 2.0, 3.0, 2000.0])
+    z = (ctypes.c_double * 1)()
+    assert lib.gs_score_and_grad(None, None, 1, z, z) == 1
+    assert lib.gs_refine(None, None, 1, None, None) == 1
+    p = ctypes.c_void_p()
+    n = ctypes.c_size_t()
+    assert lib.gs_select(None, 7.0, 64.0, ctypes.byref(p), ctypes.byref(n)) == 1
 
 
 def _dump(cmd):
