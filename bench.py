@@ -89,7 +89,7 @@ def refine_leg(eng, cfg, ds, no_cpu, sample=16):
     """Adjoint refinement of the top REFINE_PEAKS block-pixel records of the search just run
     (re-selected at z >= 5 without NMS), each perturbed by U(+-0.5 px) in x and y, U(+-2.5 deg)
     in theta and U(+-300 A) in defocus, 20 ascent steps of 0.1 (scale 2 px, 2 px, 3 deg,
-    2000 A).  'at tol': a sample of the picks is refined from the same start by the FP64 oracle
+    500 A).  'at tol': a sample of the picks is refined from the same start by the FP64 oracle
     with the same frozen mu/sigma; within_tol is the fraction whose four parameters agree
     within (0.05 px, 0.05 px, 0.1 deg, 5 A)."""
     recs = eng.select(5.0, 0.0)[:REFINE_PEAKS]
@@ -128,7 +128,7 @@ def refine_leg(eng, cfg, ds, no_cpu, sample=16):
     R = oracle.Refiner(cfg, ds.image, ds.templates, ds.ctf)
     R.float_geometry(True)
     t0 = time.perf_counter()
-    ref, zo, _, _ = R.run(opk, mu_all[q], sg_all[q], iters=20, step=0.1, scale=(2.0, 2.0, 3.0, 2000.0))
+    ref, zo, _, _ = R.run(opk, mu_all[q], sg_all[q], iters=20, step=0.1, scale=(2.0, 2.0, 3.0, 500.0))
     cpu_dt = time.perf_counter() - t0
     R.close()
     dev = np.stack([np.abs(out["peak"][n][idx] - ref[n]) for n in ("dx", "dy", "theta", "ddf")], 1)

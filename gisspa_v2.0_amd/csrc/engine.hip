@@ -1438,7 +1438,7 @@ gs_refine_opts gs_refine_defaults(void) {
   o.scale[0] = 2.0;
   o.scale[1] = 2.0;
   o.scale[2] = 3.0;
-  o.scale[3] = 2000.0;
+  o.scale[3] = 500.0;
   return o;
 }
 

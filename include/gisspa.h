@@ -219,7 +219,7 @@ typedef struct {
   double scale[4];
 } gs_refine_opts;
 
-/* Defaults: 20 iterations, step 0.1, scale (2 px, 2 px, 3 deg, 2000 A). */
+/* Defaults: 20 iterations, step 0.1, scale (2 px, 2 px, 3 deg, 500 A). */
 gs_refine_opts gs_refine_defaults(void);
 
 typedef struct {

@@ -99,7 +99,7 @@ def test_struct_layouts_match_the_header(gs, tmp_path):
 def test_refine_defaults_and_argument_checks(gs):
     lib = gs.load()
     o = lib.gs_refine_defaults()
-    assert (o.iterations, o.step, list(o.scale)) == (20, 0.1, [2.0, 2.0, 3.0, 2000.0])
+    assert (o.iterations, o.step, list(o.scale)) == (20, 0.1, [2.0, 2.0, 3.0, 500.0])
     z = (ctypes.c_double * 1)()
     assert lib.gs_score_and_grad(None, None, 1, z, z) == 1
     assert lib.gs_refine(None, None, 1, None, None) == 1

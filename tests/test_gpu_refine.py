@@ -80,10 +80,10 @@ def test_score_and_gradient_match_the_oracle(case, orc):
 def test_refinement_matches_the_oracle(case, orc):
     eng = case["eng"]
     pk = _perturbed(case, 8, seed=4)
-    out = eng.refine(pk)  # defaults: 20 steps of 0.1, scale (2 px, 2 px, 3 deg, 2000 A)
+    out = eng.refine(pk)  # defaults: 20 steps of 0.1, scale (2 px, 2 px, 3 deg, 500 A)
     mu, sg = _frozen(case, pk)
     ref, zo, go, z0o = case["R"].run(_as_oracle(orc, pk), mu, sg, iters=20, step=0.1,
-                                     scale=(2.0, 2.0, 3.0, 2000.0))
+                                     scale=(2.0, 2.0, 3.0, 500.0))
     got = out["peak"]
     for c, n in enumerate(("dx", "dy", "theta", "ddf")):
         assert np.max(np.abs(got[n] - ref[n])) <= TOL[c], n
@@ -117,7 +117,7 @@ def test_refinement_through_torch_autograd(case, gs):
 
     params = torch.tensor(np.stack([pk0["dx"], pk0["dy"], pk0["theta"], pk0["ddf"]], 1),
                           dtype=torch.float64, requires_grad=True)
-    s2 = torch.tensor([2.0, 2.0, 3.0, 2000.0], dtype=torch.float64) ** 2
+    s2 = torch.tensor([2.0, 2.0, 3.0, 500.0], dtype=torch.float64) ** 2
     for _ in range(20):
         z = ZScore.apply(params)
         params.grad = None

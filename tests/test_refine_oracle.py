@@ -104,7 +104,7 @@ def test_ascent_recovers_the_planted_pose(case):
     pk = _perturbed(case, seed=1)
     k = len(pk)
     ref, z, g, z0 = R.run(pk, case["mu"][:k], case["sigma"][:k], iters=20, step=0.1,
-                          scale=(2.0, 2.0, 3.0, 2000.0))
+                          scale=(2.0, 2.0, 3.0, 500.0))
     assert np.all(z >= z0 - 1e-9)
     checked = 0
     for i in range(k):
