@@ -14,6 +14,11 @@ The roofline entry is for the dominant correlation kernel: its algorithmic bytes
 read for the row pass) over its average launch time, from HIP events recorded around every
 launch on the engine's stream during the timed steps.  The CPU baseline is the FP64 oracle on
 a bounded sample of the same workload (see cpu_baseline.sample).
+
+The second half of the metric ("refined peaks/s at tol") is the `refine` object: the top 2000
+block-pixel records of the same search, perturbed, refined by 20 ascent steps on the GPU, with
+a sample refined by the FP64 oracle from the same starts to check the tolerance.
+`--batch K` measures a batch of K uneven micrographs instead (its own JSON line).
 """
 import argparse
 import json
