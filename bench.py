@@ -49,6 +49,9 @@ def parse():
                     help="1: row pass on a second stream (blurs per-kernel timing)")
     ap.add_argument("--batch", type=int, default=0,
                     help="K > 0: uneven-micrograph batch workload instead (its own JSON line)")
+    ap.add_argument("--refine-only", action="store_true",
+                    help="search restricted to --angles (default 8) then only the refinement leg "
+                         "(for profiling the refinement kernels)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-templates", type=int, default=4)
     ap.add_argument("--cpu-angles", type=int, default=4)
@@ -259,6 +262,12 @@ def main():
     eng = gisspa.Engine(cfg, device=0)
     eng.load_templates(ds.templates, ds.eulers[:, :2])
     eng.set_image(ds.image, ds.ctf)
+    if a.refine_only:
+        eng.set_limits(a.angles or 8, 0)
+        eng.run()
+        log(f"search restricted to {a.angles or 8} angles done")
+        print(json.dumps({"refine": refine_leg(eng, cfg, ds, True)}), flush=True)
+        return
     if a.angles:
         eng.set_limits(a.angles, 0)
     # result-invariant kernel options: the library's autotuner on a one-angle sample, unless given

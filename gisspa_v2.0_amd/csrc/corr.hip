@@ -647,6 +647,96 @@ bool corr_supported(int L) { return L == 320 || L == 720; }
 
 void corr_prepare() { (void)tw_tables(current_device()); }
 
+// ---- batched 2-D transforms of full L x L complex planes (row-major) ----
+// Two passes, each one read and one write of the planes: rows (RPB contiguous rows per
+// workgroup, TPC lanes per row transform, all transforms of a row set inside one wavefront) and
+// columns (256/TPC adjacent columns per workgroup, staged transposed in LDS with a padded
+// stride).  The column radix plan of the correlation is reused for both.
+namespace {
+
+template <int L, int SIGN>
+__global__ void __launch_bounds__(256)
+    k_plane_rows(float2* __restrict__ data, long long nrows, const float2* __restrict__ twL, float scale) {
+  using C = CorrCfg<L>;
+  constexpr int T = C::TPC, RPW = 64 / T, RPB = 4 * RPW;
+  __shared__ float2 buf[RPB * L];
+  __shared__ float2 tw[L];
+  const int tid = threadIdx.x;
+  for (int i = tid; i < L; i += 256) tw[i] = twL[i];
+  const long long r0 = (long long)blockIdx.x * RPB;
+  const int nr = (int)min((long long)RPB, nrows - r0);
+  float2* src = data + r0 * L;
+  for (int e = tid; e < RPB * L; e += 256) buf[e] = (e < nr * L) ? src[e] : make_float2(0.f, 0.f);
+  __syncthreads();
+  const int wave = tid / 64, lane = tid % 64;
+  ct::run_stockham<L, T, SIGN, ct::Sync::WAVE>(buf + (wave * RPW + lane / T) * L, tw, lane % T,
+                                               typename C::ColPlan{});
+  __syncthreads();
+  for (int e = tid; e < nr * L; e += 256) {
+    const float2 v = buf[e];
+    src[e] = make_float2(v.x * scale, v.y * scale);
+  }
+}
+
+template <int L, int SIGN>
+__global__ void __launch_bounds__(256)
+    k_plane_cols(float2* __restrict__ data, const float2* __restrict__ twL, float scale) {
+  using C = CorrCfg<L>;
+  constexpr int T = C::TPC, COLS = 256 / T, LP = L + 1;
+  static_assert(L % COLS == 0, "column groups");
+  __shared__ float2 buf[COLS * LP];
+  __shared__ float2 tw[L];
+  const int tid = threadIdx.x;
+  for (int i = tid; i < L; i += 256) tw[i] = twL[i];
+  const int c0 = blockIdx.x * COLS;
+  float2* P = data + (long long)blockIdx.y * L * L;
+  for (int e = tid; e < COLS * L; e += 256) {
+    const int y = e / COLS, c = e % COLS;
+    buf[c * LP + y] = P[(long long)y * L + c0 + c];
+  }
+  __syncthreads();
+  ct::run_stockham<L, T, SIGN, ct::Sync::WAVE>(buf + (tid / T) * LP, tw, tid % T, typename C::ColPlan{});
+  __syncthreads();
+  for (int e = tid; e < COLS * L; e += 256) {
+    const int y = e / COLS, c = e % COLS;
+    const float2 v = buf[c * LP + y];
+    P[(long long)y * L + c0 + c] = make_float2(v.x * scale, v.y * scale);
+  }
+}
+
+template <int L>
+void plane_fft2_impl(float2* data, int batch, int sign, float scale, const float2* tw, hipStream_t s) {
+  using C = CorrCfg<L>;
+  constexpr int RPB = 4 * (64 / C::TPC), COLS = 256 / C::TPC;
+  const long long nrows = (long long)batch * L;
+  const long long gr = (nrows + RPB - 1) / RPB;
+  GS_REQUIRE(batch > 0 && gr < (1LL << 31) && batch < 65536, kInvalid, "plane transform batch out of range");
+  const dim3 gc(L / COLS, (unsigned)batch);
+  if (sign < 0) {
+    k_plane_rows<L, -1><<<(unsigned)gr, 256, 0, s>>>(data, nrows, tw, 1.f);
+    k_plane_cols<L, -1><<<gc, 256, 0, s>>>(data, tw, scale);
+  } else {
+    k_plane_rows<L, +1><<<(unsigned)gr, 256, 0, s>>>(data, nrows, tw, 1.f);
+    k_plane_cols<L, +1><<<gc, 256, 0, s>>>(data, tw, scale);
+  }
+  GS_CHECK_LAUNCH();
+}
+
+}  // namespace
+
+bool launch_plane_fft2(float2* data, int batch, int L, int sign, float scale, hipStream_t s) {
+  TwTables& T = tw_tables(current_device());
+  if (L == 320) {
+    plane_fft2_impl<320>(data, batch, sign, scale, T.t320, s);
+    return true;
+  }
+  if (L == 720) {
+    plane_fft2_impl<720>(data, batch, sign, scale, T.t720, s);
+    return true;
+  }
+  return false;
+}
+
 int rowpass_groups(int L) {
   if (L == 320) return 320 / CorrCfg<320>::ROWS;
   if (L == 720) return 720 / CorrCfg<720>::ROWS;

@@ -22,16 +22,24 @@ __device__ __forceinline__ int rbin(int dx, int dy) {
 // CTF_AST (src/kernels.cu:290-317): the signed value and, when dv is not null, its derivative
 // with respect to a defocus offset in Angstrom added to the micrograph defocus (dfu and dfv
 // both move by -offset*1e-4 um, so d gamma / d offset = PI * lambda * s^2).
-__device__ __forceinline__ double ctf_signed(int x1, int y1, int nx, int ny, const DevParams& p,
-                                             double* dv) {
+// Frequency geometry of CTF_AST at (x1, y1): s^2 in 1/A^2 and the astigmatism term
+// cos(2 (dfang - angle)).
+__device__ __forceinline__ void ctf_geometry(int x1, int y1, int nx, int ny, const DevParams& p, double* ss,
+                                             double* cs_ast) {
   y1 = x1 > nx / 2 ? ny - y1 : y1;
   x1 = min(x1, nx - x1);
   const double dx = min(x1, nx - x1);
   const double dy = y1 - ny / 2;
-  const double ss = (dx * dx / (double)(nx * nx) + dy * dy / (double)(ny * ny)) /
-                    ((double)p.apix * (double)p.apix);
+  *ss = (dx * dx / (double)(nx * nx) + dy * dy / (double)(ny * ny)) / ((double)p.apix * (double)p.apix);
   const double ag = atan2((double)(y1 - ny / 2), (double)x1);
-  const double df_ast = 0.5 * (p.dfu + p.dfv + 2 * p.dfdiff * cos(2 * (p.dfang * kRefPi / 180 - ag)));
+  *cs_ast = cos(2 * (p.dfang * kRefPi / 180 - ag));
+}
+
+__device__ __forceinline__ double ctf_signed(int x1, int y1, int nx, int ny, const DevParams& p,
+                                             double* dv) {
+  double ss, cs_ast;
+  ctf_geometry(x1, y1, nx, ny, p, &ss, &cs_ast);
+  const double df_ast = 0.5 * (p.dfu + p.dfv + 2 * p.dfdiff * cs_ast);
   const double lam = p.lambda;
   const double gamma = -2 * kRefPi * (p.cs * 2.5e6 * lam * lam * lam * ss * ss + df_ast * 5000.0 * lam * ss);
   const double B = sqrt(1.0 - p.ampconst * p.ampconst);

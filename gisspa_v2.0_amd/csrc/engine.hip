@@ -597,6 +597,20 @@ class Engine {
       white_ok_ = true;
     }
     const double count = ref_region_count(l);
+    // weight table over the band-pass support and its slot map; block spectra as [b][ky][kx]
+    const int hx = l / 2 + 1;
+    const std::vector<int32_t> idx = ref_band_indices(l, dp_.hr_t, dp_.lr_t);
+    const int ntab = (int)idx.size();
+    std::vector<int32_t> slot(L2, -1);
+    for (int i = 0; i < ntab; ++i) slot[idx[i]] = i;
+    int32_t* didx = d_ridx_.ensure(std::max(ntab, 1));
+    int32_t* dslot = d_rslot_.ensure(L2);
+    RefWeight* tab = d_rtab_.ensure(std::max(ntab, 1));
+    if (ntab) GS_HIP(hipMemcpyAsync(didx, idx.data(), sizeof(int32_t) * ntab, hipMemcpyHostToDevice, stream_));
+    GS_HIP(hipMemcpyAsync(dslot, slot.data(), sizeof(int32_t) * L2, hipMemcpyHostToDevice, stream_));
+    launch_ref_table(didx, ntab, l, dp_, tab, stream_);
+    float2* ihat_t = d_rIT_.ensure((size_t)nb_ * hx * l);
+    transpose_batched(d_ihat_.p, ihat_t, nb_, hx, l, l, stream_);
     const int Kc = (int)std::min<size_t>(k, kRefineChunk);
     float2* H = d_rH_.ensure((size_t)Kc * L2);
     float2* AB = d_rAB_.ensure((size_t)2 * Kc * L2);
@@ -618,12 +632,17 @@ class Engine {
         hp[j] = RefPeak{p.block, p.pixel, p.template_id, 0, p.dx, p.dy, p.theta, p.ddf};
       }
       GS_HIP(hipMemcpyAsync(rp, hp.data(), sizeof(RefPeak) * K, hipMemcpyHostToDevice, stream_));
+      // LDS plane transforms where built (l = 320, 720), the runtime-planned FFT otherwise
+      auto fft2 = [&](float2* d, int batch, int sign, float scale) {
+        if (!launch_plane_fft2(d, batch, l, sign, scale, stream_))
+          fft2_batched(fft_, d, S, batch, l, l, sign, scale, stream_);
+      };
       for (int it = 0; it <= iters; ++it) {
-        launch_ref_template(d_twhite_.p, rp, K, l, dp_, part, count, H, stream_);
-        fft2_batched(fft_, H, S, K, l, l, +1, 1.f, stream_);
+        launch_ref_template(d_twhite_.p, tab, ntab, dslot, rp, K, l, dp_, part, count, H, stream_);
+        fft2(H, K, +1, 1.f);
         launch_ref_rotate(H, rp, K, l, P_.diameter, 4.0f, reinterpret_cast<float*>(S), part6, stats, AB, stream_);
-        fft2_batched(fft_, AB, S, 2 * K, l, l, -1, (float)(1.0 / (double)L2), stream_);
-        GS_REQUIRE(launch_ref_dot(d_ihat_.p, AB, rp, K, l, cg, stream_), kUnsupported, "window_size not supported");
+        fft2(AB, 2 * K, -1, (float)(1.0 / (double)L2));
+        GS_REQUIRE(launch_ref_dot(ihat_t, AB, rp, K, l, cg, stream_), kUnsupported, "window_size not supported");
         launch_ref_update(rp, K, cg, d_mu_.p, d_sigma_.p, l, step, scale, it == 0, it < iters, dz, dz0, dg, stream_);
       }
       GS_HIP(hipMemcpyAsync(hp.data(), rp, sizeof(RefPeak) * K, hipMemcpyDeviceToHost, stream_));
@@ -1144,14 +1163,17 @@ class Engine {
   int nb_ = 0, bx_ = 0, nbk_ = 0, na_ = 0;
   std::vector<int> offx_, offy_, ulx_, uly_;
   std::vector<float> angles_;
-  int lim_angles_ = 0, lim_templates_ = 0, chunk_ = 0, row_variant_ = 0, col_variant_ = 0;
+  // kernel variants default to the fastest measured on the production shape (l = 320)
+  int lim_angles_ = 0, lim_templates_ = 0, chunk_ = 0, row_variant_ = 1, col_variant_ = 3;
   double total_ms_ = 0, corr_ms_ = 0;
   std::vector<gs_record> records_;
   std::vector<double> planes_host_;
   double total_n_ = 0, refine_ms_ = 0;
   static constexpr size_t kRefineChunk = 1024;  // picks per refinement batch (about 4 MB each at l=320)
   bool white_ok_ = false;
-  DevBuf<float2> d_twhite_, d_rH_, d_rAB_, d_rS_;
+  DevBuf<float2> d_twhite_, d_rH_, d_rAB_, d_rS_, d_rIT_;
+  DevBuf<int32_t> d_ridx_, d_rslot_;
+  DevBuf<RefWeight> d_rtab_;
   DevBuf<RefPeak> d_rpk_;
   DevBuf<double2> d_rpart_;
   DevBuf<double> d_rpart6_, d_rstats_, d_rcg_, d_rz_, d_rz0_, d_rgrad_;
@@ -1320,7 +1342,9 @@ gs_status gs_fft2_c2c(int device, float* data, int ny, int nx, int batch, int si
     a.ensure(N);
     b.ensure(N);
     GS_HIP(hipMemcpy(a.p, data, sizeof(float2) * N, hipMemcpyHostToDevice));
-    gs::fft2_batched(cache, a.p, b.p, batch, ny, nx, sign, 1.f, nullptr);
+    // square planes of the built window sizes take the LDS plane transform
+    if (nx != ny || !gs::launch_plane_fft2(a.p, batch, nx, sign, 1.f, nullptr))
+      gs::fft2_batched(cache, a.p, b.p, batch, ny, nx, sign, 1.f, nullptr);
     GS_HIP(hipMemcpy(data, a.p, sizeof(float2) * N, hipMemcpyDeviceToHost));
   });
 }

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <vector>
 
 namespace gs {
 
@@ -88,6 +89,9 @@ bool corr_supported(int L);
 // Builds the correlation kernels' twiddle tables on the current device (they are otherwise
 // built on first use, which must not happen inside a stream capture).
 void corr_prepare();
+// In-place batched 2-D transform of `batch` full L x L complex planes (row-major), unnormalised
+// then multiplied by scale; LDS transforms for L = 320 and 720 (returns false otherwise).
+bool launch_plane_fft2(float2* data, int batch, int L, int sign, float scale, hipStream_t s);
 
 // ---- adjoint refinement of norm_type=1 picks ----
 // Device record of one pick: block, in-block pixel, template, and the refined parameters
@@ -96,18 +100,36 @@ struct RefPeak {
   int32_t block, pixel, t, pad;
   double dx, dy, theta, ddf;
 };
+// Defocus-independent factors of the template weight (apply_weighting_function + CTF_AST,
+// k:243-317) at one spectrum entry where the band-pass is not zero.
+struct RefWeight {
+  int32_t idx;    // y * l + x
+  int32_t flags;  // bit 0: in the normalisation region (x <= l/2, radial bin < l/2); bit 1: CTF applies
+  double band;    // band-pass factor
+  double a0;      // cs * 2.5e6 * lambda^3 * s^4 (spherical-aberration term of gamma)
+  double s2;      // s^2 of CTF_AST (1/A^2)
+  double cs_ast;  // cos(2 (dfang - angle))
+  double N;       // noise / signal curve N/S' of the Wiener-like weight
+};
 // Number of half-plane spectrum entries with radial bin < l/2 (the normalisation count).
 double ref_region_count(int l);
+// Spectrum indices y*l + x whose template band-pass factor can be non-zero (radial bin below
+// hr + 8 or inside [lr - 8, lr)), in increasing order.
+std::vector<int32_t> ref_band_indices(int l, float hr, float lr);
+// Weight table at the given indices for the micrograph parameters p.
+void launch_ref_table(const int32_t* idx, int n, int l, const DevParams& p, RefWeight* tab, hipStream_t s);
 // Weighted, normalised spectrum of each pick's template at its defocus, packed with its
-// defocus derivative: H = G + i dG (K planes of l*l).
-void launch_ref_template(const float2* white, const RefPeak* pk, int K, int l, const DevParams& p,
-                         double2* partial, double count, float2* H, hipStream_t s);
+// defocus derivative: H = G + i dG (K planes of l*l).  slot[e]: table entry of index e or -1.
+void launch_ref_template(const float2* white, const RefWeight* tab, int ntab, const int32_t* slot,
+                         const RefPeak* pk, int K, int l, const DevParams& p, double2* partial,
+                         double count, float2* H, hipStream_t s);
 // From the inverse transform t + i dt in H: rotation, mask, standardisation and their theta and
 // defocus derivatives, packed into AB (A = r + i dr/dtheta in planes 0..K-1, B = dr/d ddf in
 // planes K..2K-1).  f3 holds 3*K*l*l floats of scratch.
 void launch_ref_rotate(const float2* H, const RefPeak* pk, int K, int l, float d_m, float ehw,
                        float* f3, double* partial6, double* stats, float2* AB, hipStream_t s);
-// c and dc/d(dx, dy, theta, ddf) per pick (out: K x 5) from the transformed AB.
+// c and dc/d(dx, dy, theta, ddf) per pick (out: K x 5) from the transformed AB; ihat is the
+// block spectra transposed to [b][ky][kx 0..l/2].
 bool launch_ref_dot(const float2* ihat, const float2* AB, const RefPeak* pk, int K, int l, double* out,
                     hipStream_t s);
 // z and gradient per pick; when update, one ascent step p += step * scale^2 * dz/dp.

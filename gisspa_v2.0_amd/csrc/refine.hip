@@ -46,57 +46,67 @@ using dev::rbin;
 constexpr int BS = 256;
 constexpr int kPowerChunks = 16;  // row chunks of the power reduction per pick
 
-// Template weight of apply_weighting_function (k:243-288) at (x, y) and its derivative with
-// respect to the defocus offset: f = band |v| g, g = (N + kk v^2)^-1/2, df = band sign(v) v' N g^3.
-__device__ __forceinline__ void weight_deriv(int x, int y, int l, const DevParams& p, double* f,
-                                             double* df) {
+__global__ void k_ref_table(const int32_t* __restrict__ idx, int n, int l, DevParams p,
+                            RefWeight* __restrict__ tab) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int e = idx[i], y = e / l, x = e % l;
   const int fx = min(x, l - x), fy = min(y, l - y);
   const int rr = rbin(fx, fy);
-  const double bp = band_template(rr, p.hr_t, p.lr_t);
-  *f = bp;
-  *df = 0.0;
-  if (bp == 0.0) return;
-  if (rr < l / 2 && rr >= 0) {
+  RefWeight w{};
+  w.idx = e;
+  w.band = band_template(rr, p.hr_t, p.lr_t);
+  const bool ctf = rr < l / 2 && rr >= 0;
+  w.flags = ((x <= l / 2 && ctf) ? 1 : 0) | (ctf ? 2 : 0);
+  if (ctf) {
     const double r = hypot((double)fy, (double)fx);
     const double ss = r * (double)p.ds;
-    double dv = 0;
-    const double v = ctf_signed(x, (y + l / 2) % l, l, l, p, &dv);
     const double signal = exp(p.bf * ss * ss + p.bf2 * ss + p.bf3) / ((double)p.kk + 1);
-    const double N = exp(p.a * ss * ss + p.b * ss + p.b2) / signal;
-    const double g = sqrt(1 / (N + (double)p.kk * v * v));
-    *f = bp * fabs(v) * g;
-    *df = bp * (v > 0 ? 1.0 : (v < 0 ? -1.0 : 0.0)) * dv * N * g * g * g;
+    w.N = exp(p.a * ss * ss + p.b * ss + p.b2) / signal;
+    dev::ctf_geometry(x, (y + l / 2) % l, l, l, p, &w.s2, &w.cs_ast);
+    const double lam = p.lambda;
+    w.a0 = p.cs * 2.5e6 * lam * lam * lam * w.s2 * w.s2;
   }
+  tab[i] = w;
 }
 
-// The pick's micrograph parameters with its defocus offset applied.
-__device__ __forceinline__ DevParams at_offset(const DevParams& p, double ddf) {
-  DevParams q = p;
-  q.dfu -= ddf * 1e-4;
-  q.dfv -= ddf * 1e-4;
-  return q;
+// Weight f and df/d ddf of one entry at defocus offset ddf: f = band |v| g with
+// g = (N + kk v^2)^-1/2 and df = band sign(v) v' N g^3 (v = CTF, v' = dv/d ddf).
+__device__ __forceinline__ void entry_weight(const RefWeight& w, const DevParams& p, double ddf, double* f,
+                                             double* df) {
+  *f = w.band;
+  *df = 0.0;
+  if (!(w.flags & 2) || w.band == 0.0) return;
+  const double dfu = p.dfu - ddf * 1e-4, dfv = p.dfv - ddf * 1e-4;
+  const double df_ast = 0.5 * (dfu + dfv + 2 * p.dfdiff * w.cs_ast);
+  const double lam = p.lambda;
+  const double gamma = -2 * kRefPi * (w.a0 + df_ast * 5000.0 * lam * w.s2);
+  double sg, cg;
+  sincos(gamma, &sg, &cg);
+  const double B = sqrt(1.0 - p.ampconst * p.ampconst);
+  const double v = B * sg + p.ampconst * cg;
+  const double dv = (B * cg - p.ampconst * sg) * (kRefPi * lam * w.s2);
+  const double g = sqrt(1 / (w.N + (double)p.kk * v * v));
+  *f = w.band * fabs(v) * g;
+  *df = w.band * (v > 0 ? 1.0 : (v < 0 ? -1.0 : 0.0)) * dv * w.N * g * g * g;
 }
 
 // Partial sums of |G|^2 and d|G|^2/d ddf over the normalisation region (x <= l/2, radial bin
 // < l/2) of compute_area_sum_ofSQR (k:319-382); grid (kPowerChunks, K).
-__global__ void k_ref_power(const float2* __restrict__ white, const RefPeak* __restrict__ pk, int l,
-                            DevParams p, double2* __restrict__ partial) {
+__global__ void k_ref_power(const float2* __restrict__ white, const RefWeight* __restrict__ tab, int ntab,
+                            const RefPeak* __restrict__ pk, int l, DevParams p, double2* __restrict__ partial) {
   const int j = blockIdx.y, ch = blockIdx.x;
   const RefPeak P = pk[j];
-  const DevParams q = at_offset(p, P.ddf);
   const float2* F = white + (long long)P.t * l * l;
-  const int hx = l / 2 + 1;
-  const int rows = (l + kPowerChunks - 1) / kPowerChunks;
-  const int y0 = ch * rows, y1 = min(l, y0 + rows);
+  const int per = (ntab + kPowerChunks - 1) / kPowerChunks;
+  const int i0 = ch * per, i1 = min(ntab, i0 + per);
   double s = 0, ds = 0;
-  for (int e = threadIdx.x; e < (y1 - y0) * hx; e += blockDim.x) {
-    const int y = y0 + e / hx, x = e % hx;
-    const int r = rbin(min(x, l - x), min(y, l - y));
-    if (r >= l / 2 || r < 0) continue;
+  for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+    const RefWeight w = tab[i];
+    if (!(w.flags & 1)) continue;
     double f, df;
-    weight_deriv(x, y, l, q, &f, &df);
-    if (f == 0.0 && df == 0.0) continue;
-    const float2 v = F[(long long)y * l + x];
+    entry_weight(w, p, P.ddf, &f, &df);
+    const float2 v = F[w.idx];
     const double a2 = (double)v.x * v.x + (double)v.y * v.y;
     s += f * f * a2;
     ds += 2 * f * df * a2;
@@ -107,8 +117,10 @@ __global__ void k_ref_power(const float2* __restrict__ white, const RefPeak* __r
 }
 
 // H = G/m + i (dG/m - G dm/m^2), m = sqrt(sum |G|^2) / count (normalize, k:445-458); the
-// region count is the number of half-plane entries with radial bin < l/2.
-__global__ void k_ref_pack(const float2* __restrict__ white, const RefPeak* __restrict__ pk, int l,
+// region count is the number of half-plane entries with radial bin < l/2.  slot[e] is the
+// table entry of spectrum index e, or -1 where the weight is zero.
+__global__ void k_ref_pack(const float2* __restrict__ white, const RefWeight* __restrict__ tab,
+                           const int32_t* __restrict__ slot, const RefPeak* __restrict__ pk, int l,
                            DevParams p, const double2* __restrict__ partial, double count,
                            float2* __restrict__ H) {
   const int j = blockIdx.y;
@@ -126,20 +138,21 @@ __global__ void k_ref_pack(const float2* __restrict__ white, const RefPeak* __re
   }
   __syncthreads();
   const double m = sm[0], dm = sm[1];
-  const DevParams q = at_offset(p, P.ddf);
   const long long L2 = (long long)l * l;
   const float2* F = white + (long long)P.t * L2;
   float2* out = H + (long long)j * L2;
   for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < L2; e += gridDim.x * blockDim.x) {
-    const int y = e / l, x = e % l;
-    double f, df;
-    weight_deriv(x, y, l, q, &f, &df);
+    const int s = slot[e];
     float2 h = make_float2(0.f, 0.f);
-    if (f != 0.0 || df != 0.0) {
-      const float2 v = F[e];
-      const double a = f / m, b = df / m - f * dm / (m * m);
-      // (a + i b) * v
-      h = make_float2((float)(a * v.x - b * v.y), (float)(a * v.y + b * v.x));
+    if (s >= 0) {
+      double f, df;
+      entry_weight(tab[s], p, P.ddf, &f, &df);
+      if (f != 0.0 || df != 0.0) {
+        const float2 v = F[e];
+        const double a = f / m, b = df / m - f * dm / (m * m);
+        // (a + i b) * v
+        h = make_float2((float)(a * v.x - b * v.y), (float)(a * v.y + b * v.x));
+      }
     }
     out[e] = h;
   }
@@ -189,9 +202,19 @@ __global__ void k_ref_rotate(const float2* __restrict__ H, const RefPeak* __rest
     a[4] += vdf;
     a[5] += (double)v * vdf;
   }
-  for (int k = 0; k < 6; ++k) {
-    const double t = block_sum(a[k]);
-    if (threadIdx.x == 0) partial6[((long long)j * l + jrow) * 6 + k] = t;
+  // the six sums reduced together: wavefront shuffles, then one LDS exchange
+  __shared__ double red[4][6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k)
+    for (int o = 32; o > 0; o >>= 1) a[k] += __shfl_down(a[k], o, 64);
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < 6; ++k) red[wave][k] = a[k];
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    const int k = threadIdx.x;
+    partial6[((long long)j * l + jrow) * 6 + k] = red[0][k] + red[1][k] + red[2][k] + red[3][k];
   }
 }
 
@@ -272,9 +295,11 @@ __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
   return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
-// c and its derivatives for one pick per block (256 threads; rows ky of the half spectrum).
+// c and its derivatives for one pick per block.  The half spectrum (ky rows of HX entries) is
+// walked flat so that consecutive lanes read consecutive kx of A, B and of the block spectrum,
+// which is given transposed: ihat_t[b][ky][kx].
 template <int L>
-__global__ __launch_bounds__(256) void k_ref_dot(const float2* __restrict__ ihat, const float2* __restrict__ AB,
+__global__ __launch_bounds__(256) void k_ref_dot(const float2* __restrict__ ihat_t, const float2* __restrict__ AB,
                                                  const RefPeak* __restrict__ pk, int K,
                                                  double* __restrict__ out) {
   constexpr int HX = L / 2 + 1;
@@ -286,49 +311,50 @@ __global__ __launch_bounds__(256) void k_ref_dot(const float2* __restrict__ ihat
   for (int k = threadIdx.x; k < L; k += blockDim.x) phase(k, L, py, P.dy, phy[k], dphy[k]);
   __syncthreads();
   const long long L2 = (long long)L * L;
-  const float2* I = ihat + (long long)P.block * HX * L;
+  const float2* I = ihat_t + (long long)P.block * HX * L;
   const float2* A = AB + (long long)j * L2;
   const float2* B = AB + ((long long)K + j) * L2;
-  double c = 0, gx = 0, gy = 0, gt = 0, gd = 0;
-  for (int ky = threadIdx.x; ky < L; ky += blockDim.x) {
-    const int my = (L - ky) % L;
+  double acc[5] = {0, 0, 0, 0, 0};
+  for (int e = threadIdx.x; e < L * HX; e += blockDim.x) {
+    const int ky = e / HX, kx = e - ky * HX;
+    const int my = ky ? L - ky : 0, mx = kx ? L - kx : 0;
+    const float2 a = A[(long long)ky * L + kx], am = A[(long long)my * L + mx];
+    const float2 b = B[(long long)ky * L + kx], bm = B[(long long)my * L + mx];
+    // spectra of the three real fields
+    const float2 T = make_float2(0.5f * (a.x + am.x), 0.5f * (a.y - am.y));
+    const float2 Tt = make_float2(0.5f * (a.y + am.y), -0.5f * (a.x - am.x));
+    const float2 Td = make_float2(0.5f * (b.x + bm.x), 0.5f * (b.y - bm.y));
+    float2 X = I[e];
+    if ((kx + ky) & 1) X = make_float2(-X.x, -X.y);
+    const float w = (kx == 0 || 2 * kx == L) ? 1.f : 2.f;
     const float2 ey = phy[ky], dey = dphy[ky];
-    float fc = 0, fgx = 0, fgy = 0, fgt = 0, fgd = 0;
-    for (int kx = 0; kx < HX; ++kx) {
-      const int mx = (L - kx) % L;
-      const float2 a = A[(long long)ky * L + kx], am = A[(long long)my * L + mx];
-      const float2 b = B[(long long)ky * L + kx], bm = B[(long long)my * L + mx];
-      // spectra of the three real fields
-      const float2 T = make_float2(0.5f * (a.x + am.x), 0.5f * (a.y - am.y));
-      const float2 Tt = make_float2(0.5f * (a.y + am.y), -0.5f * (a.x - am.x));
-      const float2 Td = make_float2(0.5f * (b.x + bm.x), 0.5f * (b.y - bm.y));
-      float2 X = I[(long long)kx * L + ky];
-      if ((kx + ky) & 1) X = make_float2(-X.x, -X.y);
-      const float w = (kx == 0 || 2 * kx == L) ? 1.f : 2.f;
-      const float2 E = cmul(phx[kx], ey);
-      const float2 Ex = cmul(dphx[kx], ey);
-      const float2 Ey = cmul(phx[kx], dey);
-      // X conj(T) and its partners
-      const float2 P0 = make_float2(X.x * T.x + X.y * T.y, X.y * T.x - X.x * T.y);
-      const float2 Pt = make_float2(X.x * Tt.x + X.y * Tt.y, X.y * Tt.x - X.x * Tt.y);
-      const float2 Pd = make_float2(X.x * Td.x + X.y * Td.y, X.y * Td.x - X.x * Td.y);
-      fc += w * (P0.x * E.x - P0.y * E.y);
-      fgx += w * (P0.x * Ex.x - P0.y * Ex.y);
-      fgy += w * (P0.x * Ey.x - P0.y * Ey.y);
-      fgt += w * (Pt.x * E.x - Pt.y * E.y);
-      fgd += w * (Pd.x * E.x - Pd.y * E.y);
-    }
-    c += fc;
-    gx += fgx;
-    gy += fgy;
-    gt += fgt;
-    gd += fgd;
+    const float2 E = cmul(phx[kx], ey);
+    const float2 Ex = cmul(dphx[kx], ey);
+    const float2 Ey = cmul(phx[kx], dey);
+    // X conj(T) and its partners
+    const float2 P0 = make_float2(X.x * T.x + X.y * T.y, X.y * T.x - X.x * T.y);
+    const float2 Pt = make_float2(X.x * Tt.x + X.y * Tt.y, X.y * Tt.x - X.x * Tt.y);
+    const float2 Pd = make_float2(X.x * Td.x + X.y * Td.y, X.y * Td.x - X.x * Td.y);
+    acc[0] += w * (P0.x * E.x - P0.y * E.y);
+    acc[1] += w * (P0.x * Ex.x - P0.y * Ex.y);
+    acc[2] += w * (P0.x * Ey.x - P0.y * Ey.y);
+    acc[3] += w * (Pt.x * E.x - Pt.y * E.y);
+    acc[4] += w * (Pd.x * E.x - Pd.y * E.y);
   }
-  c = block_sum(c);
-  gx = block_sum(gx);
-  gy = block_sum(gy);
-  gt = block_sum(gt);
-  gd = block_sum(gd);
+  __shared__ double red[4][5];
+#pragma unroll
+  for (int k = 0; k < 5; ++k)
+    for (int o = 32; o > 0; o >>= 1) acc[k] += __shfl_down(acc[k], o, 64);
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < 5; ++k) red[wave][k] = acc[k];
+  __syncthreads();
+  const double c = red[0][0] + red[1][0] + red[2][0] + red[3][0];
+  const double gx = red[0][1] + red[1][1] + red[2][1] + red[3][1];
+  const double gy = red[0][2] + red[1][2] + red[2][2] + red[3][2];
+  const double gt = red[0][3] + red[1][3] + red[2][3] + red[3][3];
+  const double gd = red[0][4] + red[1][4] + red[2][4] + red[3][4];
   if (threadIdx.x == 0) {
     double* o = out + (long long)j * 5;
     o[0] = c;
@@ -373,27 +399,51 @@ inline unsigned nblk(long long n, int bs = BS) { return (unsigned)((n + bs - 1) 
 
 }  // namespace
 
+// floor(hypot(dx, dy) + 0.5) - 1 from the integer squared radius (the device rbin).
+static long long rbin_host(long long dx, long long dy) {
+  const long long n = dx * dx + dy * dy;
+  long long k = (long long)std::sqrt((double)n);
+  while (k * k > n) --k;
+  while ((k + 1) * (k + 1) <= n) ++k;
+  if (n > k * k + k) ++k;
+  return k - 1;
+}
+
 double ref_region_count(int l) {
   double cnt = 0;
   for (int y = 0; y < l; ++y)
     for (int x = 0; x <= l / 2; ++x) {
-      const long long dx = std::min(x, l - x), dy = std::min(y, l - y);
-      const long long n = dx * dx + dy * dy;
-      long long k = (long long)std::sqrt((double)n);
-      while (k * k > n) --k;
-      while ((k + 1) * (k + 1) <= n) ++k;
-      if (n > k * k + k) ++k;
-      const long long r = k - 1;
+      const long long r = rbin_host(std::min(x, l - x), std::min(y, l - y));
       if (r >= 0 && r < l / 2) cnt += 1;
     }
   return cnt;
 }
 
-void launch_ref_template(const float2* white, const RefPeak* pk, int K, int l, const DevParams& p,
-                         double2* partial, double count, float2* H, hipStream_t s) {
-  k_ref_power<<<dim3(kPowerChunks, K), BS, 0, s>>>(white, pk, l, p, partial);
+std::vector<int32_t> ref_band_indices(int l, float hr, float lr) {
+  // band_template is non-zero only for r < hr + 8 or lr - 8 <= r < lr (r >= 0); one bin of
+  // margin keeps every such entry (an extra entry gets a zero factor on the device)
+  std::vector<int32_t> idx;
+  for (int y = 0; y < l; ++y)
+    for (int x = 0; x < l; ++x) {
+      const long long r = rbin_host(std::min(x, l - x), std::min(y, l - y));
+      if (r < 0) continue;
+      if ((float)r < hr + 9.0f || ((float)r >= lr - 9.0f && (float)r < lr + 1.0f)) idx.push_back(y * l + x);
+    }
+  return idx;
+}
+
+void launch_ref_table(const int32_t* idx, int n, int l, const DevParams& p, RefWeight* tab, hipStream_t s) {
+  if (n <= 0) return;
+  k_ref_table<<<nblk(n, BS), BS, 0, s>>>(idx, n, l, p, tab);
   GS_CHECK_LAUNCH();
-  k_ref_pack<<<dim3(nblk((long long)l * l, BS), K), BS, 0, s>>>(white, pk, l, p, partial, count, H);
+}
+
+void launch_ref_template(const float2* white, const RefWeight* tab, int ntab, const int32_t* slot,
+                         const RefPeak* pk, int K, int l, const DevParams& p, double2* partial,
+                         double count, float2* H, hipStream_t s) {
+  k_ref_power<<<dim3(kPowerChunks, K), BS, 0, s>>>(white, tab, ntab, pk, l, p, partial);
+  GS_CHECK_LAUNCH();
+  k_ref_pack<<<dim3(nblk((long long)l * l, BS), K), BS, 0, s>>>(white, tab, slot, pk, l, p, partial, count, H);
   GS_CHECK_LAUNCH();
 }
 

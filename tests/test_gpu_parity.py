@@ -30,12 +30,15 @@ def test_fft_c2c_matches_numpy(gs, n):
         assert rel_l2(got, ref) < 2e-6, (n, sign, rel_l2(got, ref))
 
 
-@pytest.mark.parametrize("shape", [(48, 60), (320, 320), (100, 64)])
+@pytest.mark.parametrize("shape", [(48, 60), (320, 320), (100, 64), (720, 720)])
 def test_fft2_c2c_matches_numpy(gs, shape):
     rng = np.random.default_rng(7)
     x = (rng.standard_normal((2,) + shape) + 1j * rng.standard_normal((2,) + shape)).astype(np.complex64)
     got = gs.fft2_c2c(x, -1)
     ref = np.fft.fft2(x.astype(np.complex128))
+    assert rel_l2(got, ref) < 2e-6
+    got = gs.fft2_c2c(x, +1)
+    ref = np.fft.ifft2(x.astype(np.complex128)) * (shape[0] * shape[1])
     assert rel_l2(got, ref) < 2e-6
 
 
