@@ -318,7 +318,10 @@ def main():
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "corr_c1_traffic.json")
     if os.path.exists(tpath) and l == 320:
-        kname = {"column_pass": "k_colpass<320>", "row_pass": "k_rowpass_pix_wave<320, true>"}[dominant]
+        col_name = f"k_colpass<320, {1 if variant[0] == 3 else 0}>" if variant[0] in (0, 3) else ""
+        row_name = {0: "k_rowpass_pix_wave<320, false, 4>", 1: "k_rowpass_pix_wave<320, true, 4>",
+                    2: "k_rowpass_pix_wave<320, true, 8>", 3: "k_rowpass_pix_wave<320, false, 8>"}[variant[1]]
+        kname = col_name if dominant == "column_pass" else row_name
         rec = json.load(open(tpath))["kernels"].get(kname)
         if rec:
             per_plane = (rec["fetch_bytes_per_launch"] + rec["write_bytes_per_launch"]) / (
