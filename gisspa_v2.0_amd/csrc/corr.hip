@@ -573,16 +573,17 @@ __global__ void k_finalize_norm(const double* __restrict__ S, const double* __re
   }
 }
 
-// Per (template, block): merge row-group partials in row order (strict max, lowest index on
-// ties; sums in double).
+// Per (template, block): merge row-group partials (largest peak, lowest index among equal peaks
+// -- an order-independent rule; sums in double).  One wavefront per (template, block).
 __global__ void k_finalize_plane(const float4* __restrict__ partial, int tc, int nb, int groups,
-                                 double* __restrict__ out) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= tc * nb) return;
+                                 long long tstride, double* __restrict__ out) {
+  const int i = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+  const int lane = threadIdx.x % 64;
+  if (i >= tc * nb) return;  // uniform per wavefront
   float v = 0.f;
   int id = 0x7fffffff;
   double s = 0, s2 = 0;
-  for (int g = 0; g < groups; ++g) {
+  for (int g = lane; g < groups; g += 64) {
     const float4 p = partial[(long long)i * groups + g];
     const int pi = __float_as_int(p.y);
     if (p.x > v || (p.x == v && pi < id)) {
@@ -592,10 +593,23 @@ __global__ void k_finalize_plane(const float4* __restrict__ partial, int tc, int
     s += p.z;
     s2 += p.w;
   }
-  out[4LL * i + 0] = v;
-  out[4LL * i + 1] = (id == 0x7fffffff) ? 0.0 : (double)id;
-  out[4LL * i + 2] = s;
-  out[4LL * i + 3] = s2;
+  for (int o = 32; o > 0; o >>= 1) {
+    const float ov = __shfl_down(v, o, 64);
+    const int oid = __shfl_down(id, o, 64);
+    s += __shfl_down(s, o, 64);
+    s2 += __shfl_down(s2, o, 64);
+    if (ov > v || (ov == v && oid < id)) {
+      v = ov;
+      id = oid;
+    }
+  }
+  if (lane == 0) {
+    double* o = out + 4 * ((long long)(i / nb) * tstride + i % nb);
+    o[0] = v;
+    o[1] = (id == 0x7fffffff) ? 0.0 : (double)id;
+    o[2] = s;
+    o[3] = s2;
+  }
 }
 
 }  // namespace
@@ -829,9 +843,10 @@ bool launch_rowpass_plane(int L, const float2* Z, int tc, int nb, float4* partia
   return false;
 }
 
-void launch_finalize_plane(const float4* partial, int tc, int nb, int groups, double* out, hipStream_t s) {
-  const int n = tc * nb;
-  k_finalize_plane<<<(n + 255) / 256, 256, 0, s>>>(partial, tc, nb, groups, out);
+void launch_finalize_plane(const float4* partial, int tc, int nb, int groups, long long tstride, double* out,
+                           hipStream_t s) {
+  const int n = tc * nb;  // one wavefront each, four per workgroup
+  k_finalize_plane<<<(n + 3) / 4, 256, 0, s>>>(partial, tc, nb, groups, tstride, out);
   GS_CHECK_LAUNCH();
 }
 

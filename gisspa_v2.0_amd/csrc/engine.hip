@@ -306,9 +306,15 @@ class Engine {
   // runs on the device, a helper thread copies and clips micrograph i+1 into the other of two
   // page-locked staging buffers, and with graphs on the angle loop is replayed from a capture
   // whenever the problem shape repeats.  Records equal those of one gs_search per micrograph.
+  // With refine_top > 0 (norm_type=1), the first refine_top records of each micrograph (its
+  // best picks) are refined on that micrograph before the next one is searched; their results
+  // go to `refined` (item order) and their counts to refined_counts.
   void search_batch(const gs_batch_item* items, int n_items, std::vector<gs_record>& all,
-                    size_t* counts) {
+                    size_t* counts, int refine_top = 0, const gs_refine_opts* ropts = nullptr,
+                    std::vector<gs_refined>* refined = nullptr, size_t* refined_counts = nullptr) {
     GS_REQUIRE(items && n_items > 0 && counts, kInvalid, "empty batch");
+    GS_REQUIRE(refine_top <= 0 || (P_.norm_type == 1 && ropts && refined && refined_counts), kInvalid,
+               "refinement in a batch needs norm_type=1, options and outputs");
     for (int i = 0; i < n_items; ++i) {
       check_image(items[i].image, items[i].width, items[i].height);
       if (items[i].templates) {
@@ -342,6 +348,25 @@ class Engine {
         run();
         all.insert(all.end(), records_.begin(), records_.end());
         counts[i] = records_.size();
+        if (refine_top > 0) {
+          const size_t k = std::min<size_t>((size_t)refine_top, records_.size());
+          refined_counts[i] = k;
+          if (k) {
+            std::vector<gs_peak> pk(k), outp(k);
+            for (size_t q = 0; q < k; ++q) pk[q] = peak_from_record(records_[q]);
+            std::vector<double> z(k), z0(k), g(4 * k);
+            refine(pk.data(), k, ropts->iterations, ropts->step, ropts->scale, outp.data(), z.data(), z0.data(),
+                   g.data());
+            for (size_t q = 0; q < k; ++q) {
+              gs_refined r{};
+              r.peak = outp[q];
+              r.score = z[q];
+              r.score0 = z0[q];
+              for (int c = 0; c < 4; ++c) r.grad[c] = g[4 * q + c];
+              refined->push_back(r);
+            }
+          }
+        }
       }
     } catch (...) {
       if (next.valid()) next.wait();
@@ -777,7 +802,7 @@ class Engine {
     double* mom = d_mom_.ensure((size_t)n * 4);
     launch_template_moments(T, n, l, mom, stream_);
     launch_template_standardise(T, n, l, mom, stream_);
-    fft2_batched(fft_, T, S, n, l, l, -1, (float)(1.0 / (double)L2), stream_);
+    fft2_planes(T, S, n, -1, (float)(1.0 / (double)L2));
     double* ra = d_ra_.ensure((size_t)n * (l / 2));
     launch_radial_power_planes(T, n, l, ra, stream_);
     launch_whiten_planes(T, n, l, ra, radial_counts_dev(l, l), stream_);
@@ -790,17 +815,17 @@ class Engine {
     float2* S = d_scratch_.ensure((size_t)n * L2);
     whiten_templates(T, S);
     if (!P_.norm_type) {  // src/nonorm.cu:291-299
-      fft2_batched(fft_, T, S, n, l, l, +1, 1.f, stream_);
+      fft2_planes(T, S, n, +1, 1.f);
       launch_mask_planes(T, n, l, P_.diameter, 4.0f, stream_);
-      fft2_batched(fft_, T, S, n, l, l, -1, (float)(1.0 / (double)L2), stream_);
+      fft2_planes(T, S, n, -1, (float)(1.0 / (double)L2));
     }
     launch_weight_planes(T, n, l, dp_, stream_);
     double* m = d_m_.ensure(n);
-    launch_area_power_full(T, n, l, m, stream_);
+    launch_area_power_full(T, n, l, d_apart_.ensure((size_t)n * 16), m, stream_);
     const float sign = ((l / 2) % 2) ? -1.f : 1.f;  // checkerboard sign for odd l/2 (k:669-677)
     launch_scale_planes(T, n, (long long)L2, m, P_.norm_type ? 1.f : sign, stream_);
     if (P_.norm_type) {
-      fft2_batched(fft_, T, S, n, l, l, +1, 1.f, stream_);
+      fft2_planes(T, S, n, +1, 1.f);
       launch_extract_real(T, d_tprep_.ensure((size_t)n * L2), (long long)n * L2, stream_);
     } else {
       const int hx = l / 2 + 1;
@@ -813,9 +838,20 @@ class Engine {
   void half_spectra(float2* planes, int np, float2* out, float scale) {
     const int l = P_.window_size, hx = l / 2 + 1;
     float2* S = d_scratch_.ensure((size_t)np * l * l);
+    if (launch_plane_fft2(planes, np, l, -1, scale, stream_)) {  // LDS plane transform, then keep kx <= l/2
+      transpose_batched(planes, out, np, l, l, hx, stream_);
+      return;
+    }
     fft_rows(fft_, planes, S, l, (long long)np * l, -1, 1.f, stream_);
     transpose_batched(planes, out, np, l, l, hx, stream_);
     fft_rows(fft_, out, S, l, (long long)np * hx, -1, scale, stream_);
+  }
+
+  // In-place 2-D transforms of np l x l planes: the LDS plane transform where built, else the
+  // runtime-planned FFT with scratch S.
+  void fft2_planes(float2* T, float2* S, int np, int sign, float scale) {
+    const int l = P_.window_size;
+    if (!launch_plane_fft2(T, np, l, sign, scale, stream_)) fft2_batched(fft_, T, S, np, l, l, sign, scale, stream_);
   }
 
   void clock_begin(KernelClock& k, hipStream_t s) {
@@ -875,7 +911,7 @@ class Engine {
     float2* ihat = d_ihat_.ensure((size_t)nb_ * hx * l);
     half_spectra(blk, nb_, ihat, (float)(1.0 / (double)L2));
     double* m = d_m_.ensure(std::max(n, nb_));
-    launch_area_power_half(ihat, nb_, l, m, stream_);
+    launch_area_power_half(ihat, nb_, l, d_apart_.ensure((size_t)nb_ * 16), m, stream_);
     // fold the 1/l^2 of add_CCG_to_sum (k:691) and the odd-l/2 checkerboard sign into Ihat
     const float sign = ((l / 2) % 2) ? -1.f : 1.f;
     launch_scale_planes(ihat, nb_, (long long)hx * l, m, (float)(sign / (double)L2), stream_);
@@ -1043,17 +1079,28 @@ class Engine {
     angles_ = angles;
     na_ = na;
     const int groups = rowpass_groups(l);
-    const int tc_max = std::min(n, 64);
-    float2* Z = d_Z_.ensure((size_t)tc_max * nbb * l * (l / 2));
-    float4* part = reinterpret_cast<float4*>(d_plane_part_.ensure((size_t)tc_max * nbb * groups * 4));
-    const size_t per_angle = (size_t)n * nbb * 4;
-    double* planes = d_planes_.ensure((size_t)ntiles * na * per_angle);
+    // Angles are processed in groups: the rotated blocks of ag angles form the "block" axis of
+    // one correlation launch (ag * nbb planes per template), so small tiles still fill the GPU.
+    const int ag_max = std::max(1, std::min(na, 2048 / std::max(nbb, 1)));
+    const int nbp_max = ag_max * nbb;
+    const int tc_max = std::min(n, chunk_for(nbp_max));
+    float2* Z = d_Z_.ensure((size_t)tc_max * nbp_max * l * (l / 2));
+    float4* part = reinterpret_cast<float4*>(d_plane_part_.ensure((size_t)tc_max * nbp_max * groups * 4));
+    // plane records, per tile [t][angle][b][4]
+    const size_t per_tile = (size_t)n * na * nbb * 4;
+    double* planes = d_planes_.ensure((size_t)ntiles * per_tile);
     float* tile = d_tile_.ensure((size_t)kTile * kTile);
     float* tpre = d_imgpre_.ensure((size_t)kTile * kTile);
     float2* blk = d_blocks_.ensure((size_t)nbb * L2);
-    float2* rotb = d_work2_.ensure((size_t)nbb * L2);
-    float2* ihat = d_ihat_.ensure((size_t)nbb * hx * l);
+    float2* rotb = d_work2_.ensure((size_t)nbp_max * L2);
+    float2* ihat = d_ihat_.ensure((size_t)nbp_max * hx * l);
     double* m = d_m_.ensure(std::max(n, nbb));
+    std::vector<float2> cs(na);
+    for (int a = 0; a < na; ++a)
+      cs[a] = make_float2((float)std::cos((double)angles[a] * kRefPi / 180),
+                          (float)std::sin((double)angles[a] * kRefPi / 180));
+    float2* dcs = d_cs_.ensure(na);
+    GS_HIP(hipMemcpyAsync(dcs, cs.data(), sizeof(float2) * na, hipMemcpyHostToDevice, stream_));
     const float border = P_.diameter / 4;
     GS_HIP(hipEventRecord(ev_corr0_, stream_));
     for (int tI = 0; tI < ntiles; ++tI) {
@@ -1066,51 +1113,51 @@ class Engine {
       launch_split_blocks(src, kTile, kTile, l, doff, doff, nbk_, nbb, blk, stream_);
       if (P_.phase_flip == 1) {  // src/nonorm.cu:405-447
         float2* S = d_scratch_.ensure((size_t)nbb * L2);
-        fft2_batched(fft_, blk, S, nbb, l, l, -1, (float)(1.0 / (double)L2), stream_);
-        launch_area_power_full(blk, nbb, l, m, stream_);
+        fft2_planes(blk, S, nbb, -1, (float)(1.0 / (double)L2));
+        launch_area_power_full(blk, nbb, l, d_apart_.ensure((size_t)nbb * 16), m, stream_);
         launch_scale_planes(blk, nbb, (long long)L2, m, 1.f, stream_);
-        fft2_batched(fft_, blk, S, nbb, l, l, +1, 1.f, stream_);
+        fft2_planes(blk, S, nbb, +1, 1.f);
       }
-      for (int a = 0; a < na; ++a) {
-        const float cose = (float)std::cos((double)angles[a] * kRefPi / 180);
-        const float sine = (float)std::sin((double)angles[a] * kRefPi / 180);
-        launch_rotate_complex(blk, nbb, l, cose, sine, rotb, stream_);
-        half_spectra(rotb, nbb, ihat, (float)(1.0 / (double)L2));
+      for (int a0 = 0; a0 < na; a0 += ag_max) {
+        const int ag = std::min(ag_max, na - a0), nbp = ag * nbb;
+        // rotate_subIMG of every block for angles a0..a0+ag (k:534-603): planes [angle][block]
+        launch_rotate_complex_multi(blk, nbb, l, dcs + a0, ag, rotb, stream_);
+        half_spectra(rotb, nbp, ihat, (float)(1.0 / (double)L2));
         for (int t0 = 0; t0 < n; t0 += tc_max) {
           const int tc = std::min(tc_max, n - t0);
-          CorrArgs ca{d_that_.p, ihat, Z, t0, tc, nbb};
+          CorrArgs ca{d_that_.p, ihat, Z, t0, tc, nbp};
           clock_begin(clk_col_, stream_);
           GS_REQUIRE(launch_colpass(l, ca, col_variant_, stream_), kUnsupported, "window_size not supported");
           clock_end(clk_col_, stream_);
           clock_begin(clk_row_, stream_);
-          launch_rowpass_plane(l, Z, tc, nbb, part, border, stream_);
+          launch_rowpass_plane(l, Z, tc, nbp, part, border, stream_);
           clock_end(clk_row_, stream_);
-          // results for templates t0.. of this (tile, angle): layout [tile][angle][t][b][4]
-          launch_finalize_plane(part, tc, nbb, groups,
-                                planes + ((size_t)tI * na + a) * per_angle + (size_t)t0 * nbb * 4, stream_);
+          // record of (template t0+t, angle a0+q, block b) at tile base + ((t0+t)*na + a0)*nbb + q*nbb + b
+          launch_finalize_plane(part, tc, nbp, groups, (long long)na * nbb,
+                                planes + tI * per_tile + ((size_t)t0 * na + a0) * nbb * 4, stream_);
         }
       }
     }
     GS_HIP(hipEventRecord(ev_corr1_, stream_));
-    planes_host_.resize((size_t)ntiles * na * per_angle);
+    planes_host_.resize((size_t)ntiles * per_tile);
   }
 
   void collect_nonorm() {
     GS_HIP(hipMemcpy(planes_host_.data(), d_planes_.p, sizeof(double) * planes_host_.size(),
                      hipMemcpyDeviceToHost));
     const int l = P_.window_size, n = n_, nbb = nbk_ * nbk_;
-    const size_t per_angle = (size_t)n * nbb * 4;
+    const size_t per_tile = (size_t)n * na_ * nbb * 4;
     const double rc = (double)l * l - 1;
     for (int tI = 0; tI < (int)ulx_.size(); ++tI)
       for (int a = 0; a < na_; ++a) {
         const double e = angles_[a];
         const double ce = std::cos(e * kRefPi / 180), se = std::sin(e * kRefPi / 180);
-        const double* P = planes_host_.data() + ((size_t)tI * na_ + a) * per_angle;
+        const double* P = planes_host_.data() + tI * per_tile;
         for (int t = 0; t < n; ++t) {
           double sc = 0, cxs = 0, cys = 0;
           int pix = 0, blkid = 0;
           for (int b = 0; b < nbb; ++b) {  // PickParticles host scoring, src/nonorm.cu:509-534
-            const double* q = P + ((size_t)t * nbb + b) * 4;
+            const double* q = P + (((size_t)t * na_ + a) * nbb + b) * 4;
             const double peak = q[0], pos = q[1], sum = q[2], sum2 = q[3];
             const double ra = sum - peak, rb = sum2 - peak * peak;
             const double sd = std::sqrt(rb / rc - (ra / rc) * (ra / rc));
@@ -1175,7 +1222,7 @@ class Engine {
   DevBuf<int32_t> d_ridx_, d_rslot_;
   DevBuf<RefWeight> d_rtab_;
   DevBuf<RefPeak> d_rpk_;
-  DevBuf<double2> d_rpart_;
+  DevBuf<double2> d_rpart_, d_apart_;
   DevBuf<double> d_rpart6_, d_rstats_, d_rcg_, d_rz_, d_rz0_, d_rgrad_;
   static constexpr size_t kMaxGraphs = 8;
   bool use_graph_ = false;
@@ -1185,7 +1232,7 @@ class Engine {
   std::map<long long, std::unique_ptr<DevBuf<double>>> rb_cache_;
 
   DevBuf<float> d_raw_, d_img_, d_imgpre_, d_tprep_, d_rot_, d_stats_, d_best_, d_tile_;
-  DevBuf<float2> d_work_, d_work2_, d_scratch_, d_frame_, d_frame2_, d_blocks_, d_ihat_, d_that_, d_Z_;
+  DevBuf<float2> d_work_, d_work2_, d_scratch_, d_frame_, d_frame2_, d_blocks_, d_ihat_, d_that_, d_Z_, d_cs_;
   DevBuf<double> d_mom_, d_ra_, d_ra_img_, d_m_, d_part_, d_S_, d_SS_, d_mu_, d_sigma_, d_planes_;
   DevBuf<float> d_plane_part_;
   DevBuf<int32_t> d_arg_;
@@ -1493,6 +1540,31 @@ gs_status gs_refine(gs_engine* e, const gs_peak* peaks, size_t k, const gs_refin
       out[i].score0 = z0[i];
       for (int q = 0; q < 4; ++q) out[i].grad[q] = g[4 * i + q];
     }
+  });
+}
+
+gs_status gs_search_batch_refine(gs_engine* e, const gs_batch_item* items, int n_items, int top_k,
+                                 const gs_refine_opts* opts, gs_record** out, size_t* counts,
+                                 gs_refined** refined, size_t* refined_counts) {
+  if (!e || !items || !out || !counts || !refined || !refined_counts || n_items <= 0 || top_k < 0)
+    return fail(GS_ERR_INVALID, "null argument, empty batch or negative top_k");
+  return guarded([&] {
+    const gs_refine_opts o = opts ? *opts : gs_refine_defaults();
+    std::vector<gs_record> all;
+    std::vector<gs_refined> ref;
+    std::fill(refined_counts, refined_counts + n_items, (size_t)0);
+    e->impl->search_batch(items, n_items, all, counts, top_k, &o, &ref, refined_counts);
+    gs_record* p = (gs_record*)std::malloc(sizeof(gs_record) * std::max<size_t>(all.size(), 1));
+    gs_refined* q = (gs_refined*)std::malloc(sizeof(gs_refined) * std::max<size_t>(ref.size(), 1));
+    if (!p || !q) {
+      std::free(p);
+      std::free(q);
+      throw std::bad_alloc();
+    }
+    if (!all.empty()) std::memcpy(p, all.data(), sizeof(gs_record) * all.size());
+    if (!ref.empty()) std::memcpy(q, ref.data(), sizeof(gs_refined) * ref.size());
+    *out = p;
+    *refined = q;
   });
 }
 

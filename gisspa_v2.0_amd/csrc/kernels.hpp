@@ -37,8 +37,10 @@ void launch_radial_power_planes(const float2* F, int n, int l, double* ra, hipSt
 void launch_whiten_planes(float2* F, int n, int l, const double* ra, const double* rb, hipStream_t s);
 void launch_weight_planes(float2* F, int n, int l, const DevParams& p, hipStream_t s);
 void launch_mask_planes(float2* T, int n, int l, float d_m, float ehw, hipStream_t s);
-void launch_area_power_full(const float2* F, int n, int l, double* m, hipStream_t s);
-void launch_area_power_half(const float2* F, int n, int l, double* m, hipStream_t s);
+// Spectral-normalisation factor m per plane (compute_area_sum_ofSQR k:319-382 + the host mean of
+// src/norm.cu:334-342); part is scratch of 16 double2 per plane.
+void launch_area_power_full(const float2* F, int n, int l, double2* part, double* m, hipStream_t s);
+void launch_area_power_half(const float2* F, int n, int l, double2* part, double* m, hipStream_t s);
 void launch_scale_planes(float2* F, int n, long long plane, const double* m, float extra,
                          hipStream_t s);
 void launch_extract_real(const float2* F, float* out, long long count, hipStream_t s);
@@ -65,6 +67,9 @@ void launch_standardise_to_complex(const float* R, int n, int l, const float* st
                                    hipStream_t s);
 void launch_rotate_complex(const float2* in, int n, int l, float cose, float sine, float2* out,
                            hipStream_t s);
+// na angles at once: out[a][t] = rotation of in[t] by cs[a] = (cos, sin) (device array).
+void launch_rotate_complex_multi(const float2* in, int n, int l, const float2* cs, int na, float2* out,
+                                 hipStream_t s);
 
 // ---- correlation (north-star kernels b, c, d, e) ----
 struct CorrArgs {
@@ -136,8 +141,9 @@ bool launch_ref_dot(const float2* ihat, const float2* AB, const RefPeak* pk, int
 void launch_ref_update(RefPeak* pk, int K, const double* cg, const double* mu, const double* sigma, int l,
                        double step, const double* scale4, bool first, bool update, double* z, double* z0,
                        double* grad, hipStream_t s);
-// Reduce plane partials to (peak, index, sum, sum2) per (template, block).
-void launch_finalize_plane(const float4* partial, int tc, int nb, int groups, double* out,
+// Reduce plane partials to (peak, index, sum, sum2) per (template, block): the record of
+// (template t, block b) goes to out[(t * tstride + b) * 4].
+void launch_finalize_plane(const float4* partial, int tc, int nb, int groups, long long tstride, double* out,
                            hipStream_t s);
 void launch_init_accumulators(double* S, double* SS, float* best, int32_t* arg, long long count,
                               hipStream_t s);

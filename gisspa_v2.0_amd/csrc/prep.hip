@@ -161,12 +161,17 @@ __global__ void k_mask_planes(float2* __restrict__ T, int n, int l, float rin, f
 }
 
 // compute_area_sum_ofSQR (k:319-382) + host mean (src/norm.cu:334-342); full [y][x] planes.
-__global__ void k_area_power_full(const float2* __restrict__ F, int l, double* __restrict__ m) {
-  const int t = blockIdx.x;
+// Partial sums over a slice of the region; grid (kAreaChunks, n), then k_area_power_finish.
+constexpr int kAreaChunks = 16;
+
+__global__ void k_area_power_full(const float2* __restrict__ F, int l, double2* __restrict__ part) {
+  const int t = blockIdx.y, ch = blockIdx.x;
   const float2* P = F + (long long)t * l * l;
   const int hx = l / 2 + 1;
+  const int per = (l * hx + kAreaChunks - 1) / kAreaChunks;
+  const int p0 = ch * per, p1 = min(l * hx, p0 + per);
   double s = 0, c = 0;
-  for (int p = threadIdx.x; p < l * hx; p += blockDim.x) {
+  for (int p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
     const int y = p / hx, x = p % hx;
     const int r = rbin(min(x, l - x), min(y, l - y));
     if (r < l / 2 && r >= 0) {
@@ -177,16 +182,18 @@ __global__ void k_area_power_full(const float2* __restrict__ F, int l, double* _
   }
   s = block_sum(s);
   c = block_sum(c);
-  if (threadIdx.x == 0) m[t] = sqrt(s / (c * c));
+  if (threadIdx.x == 0) part[(long long)t * kAreaChunks + ch] = make_double2(s, c);
 }
 
 // Same over half spectra stored column-major [kx 0..l/2][ky].
-__global__ void k_area_power_half(const float2* __restrict__ F, int l, double* __restrict__ m) {
-  const int t = blockIdx.x;
+__global__ void k_area_power_half(const float2* __restrict__ F, int l, double2* __restrict__ part) {
+  const int t = blockIdx.y, ch = blockIdx.x;
   const int hx = l / 2 + 1;
   const float2* P = F + (long long)t * hx * l;
+  const int per = (l * hx + kAreaChunks - 1) / kAreaChunks;
+  const int p0 = ch * per, p1 = min(l * hx, p0 + per);
   double s = 0, c = 0;
-  for (int p = threadIdx.x; p < l * hx; p += blockDim.x) {
+  for (int p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
     const int x = p / l, y = p % l;
     const int r = rbin(min(x, l - x), min(y, l - y));
     if (r < l / 2 && r >= 0) {
@@ -197,7 +204,19 @@ __global__ void k_area_power_half(const float2* __restrict__ F, int l, double* _
   }
   s = block_sum(s);
   c = block_sum(c);
-  if (threadIdx.x == 0) m[t] = sqrt(s / (c * c));
+  if (threadIdx.x == 0) part[(long long)t * kAreaChunks + ch] = make_double2(s, c);
+}
+
+// m = sqrt(sum / count^2) per plane from the chunk partials (host mean of src/norm.cu:334-342).
+__global__ void k_area_power_finish(const double2* __restrict__ part, int n, double* __restrict__ m) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  double s = 0, c = 0;
+  for (int ch = 0; ch < kAreaChunks; ++ch) {
+    s += part[(long long)t * kAreaChunks + ch].x;
+    c += part[(long long)t * kAreaChunks + ch].y;
+  }
+  m[t] = sqrt(s / (c * c));
 }
 
 // normalize (k:445-458): F /= m (when m != 0), times an extra factor.
@@ -396,6 +415,28 @@ __global__ void k_rotate_complex(const float2* __restrict__ in, int n, int l, fl
   out[i] = res;
 }
 
+// The same for na angles at once: out[a][t] = rotate(in[t], cs[a]) with cs[a] = (cos, sin).
+__global__ void k_rotate_complex_multi(const float2* __restrict__ in, int n, int l,
+                                       const float2* __restrict__ cs, int na, float2* __restrict__ out) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long L2 = (long long)l * l;
+  if (i >= L2 * n * na) return;
+  const long long pl = i / L2;
+  const int a = (int)(pl / n), t = (int)(pl - (long long)a * n);
+  const int q = (int)(i - pl * L2);
+  const float2* P = in + (long long)t * L2;
+  const float2 c = cs[a];
+  const Bil b = bil_coords(q % l, q / l, l, c.x, c.y);
+  float2 res = make_float2(0.f, 0.f);
+  if (b.ok) {
+    const float tt = 1.0f - b.t, uu = 1.0f - b.u;
+    const float w0 = tt * uu, w1 = b.t * uu, w2 = b.t * b.u, w3 = tt * b.u;
+    res.x = P[b.k0].x * w0 + P[b.k1].x * w1 + P[b.k2].x * w2 + P[b.k3].x * w3;
+    res.y = P[b.k0].y * w0 + P[b.k1].y * w1 + P[b.k2].y * w2 + P[b.k3].y * w3;
+  }
+  out[i] = res;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------- launchers
@@ -429,12 +470,16 @@ void launch_mask_planes(float2* T, int n, int l, float d_m, float ehw, hipStream
   k_mask_planes<<<nblk((long long)n * l * l), BS, 0, s>>>(T, n, l, rin, rout, ehw);
   GS_CHECK_LAUNCH();
 }
-void launch_area_power_full(const float2* F, int n, int l, double* m, hipStream_t s) {
-  k_area_power_full<<<n, BS, 0, s>>>(F, l, m);
+void launch_area_power_full(const float2* F, int n, int l, double2* part, double* m, hipStream_t s) {
+  k_area_power_full<<<dim3(kAreaChunks, n), BS, 0, s>>>(F, l, part);
+  GS_CHECK_LAUNCH();
+  k_area_power_finish<<<nblk(n), BS, 0, s>>>(part, n, m);
   GS_CHECK_LAUNCH();
 }
-void launch_area_power_half(const float2* F, int n, int l, double* m, hipStream_t s) {
-  k_area_power_half<<<n, BS, 0, s>>>(F, l, m);
+void launch_area_power_half(const float2* F, int n, int l, double2* part, double* m, hipStream_t s) {
+  k_area_power_half<<<dim3(kAreaChunks, n), BS, 0, s>>>(F, l, part);
+  GS_CHECK_LAUNCH();
+  k_area_power_finish<<<nblk(n), BS, 0, s>>>(part, n, m);
   GS_CHECK_LAUNCH();
 }
 void launch_scale_planes(float2* F, int n, long long plane, const double* m, float extra, hipStream_t s) {
@@ -499,6 +544,11 @@ void launch_standardise_to_complex(const float* R, int n, int l, const float* st
 void launch_rotate_complex(const float2* in, int n, int l, float cose, float sine, float2* out,
                            hipStream_t s) {
   k_rotate_complex<<<nblk((long long)n * l * l), BS, 0, s>>>(in, n, l, cose, sine, out);
+  GS_CHECK_LAUNCH();
+}
+void launch_rotate_complex_multi(const float2* in, int n, int l, const float2* cs, int na, float2* out,
+                                 hipStream_t s) {
+  k_rotate_complex_multi<<<nblk((long long)n * l * l * na), BS, 0, s>>>(in, n, l, cs, na, out);
   GS_CHECK_LAUNCH();
 }
 

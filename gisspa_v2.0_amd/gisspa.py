@@ -121,6 +121,9 @@ SIGNATURES = {
     "gs_score_and_grad": (C.c_int, [_P, _P, C.c_size_t, _P, _P]),
     "gs_refine": (C.c_int, [_P, _P, C.c_size_t, C.POINTER(RefineOpts), _P]),
     "gs_refine_timing": (C.c_int, [_P, C.POINTER(C.c_double)]),
+    "gs_search_batch_refine": (C.c_int, [_P, C.POINTER(BatchItem), C.c_int, C.c_int, C.POINTER(RefineOpts),
+                                         C.POINTER(_P), C.POINTER(C.c_size_t), C.POINTER(_P),
+                                         C.POINTER(C.c_size_t)]),
 }
 
 _lib = None
@@ -315,9 +318,43 @@ class Engine:
         _check(load().gs_autotune(self._h, C.byref(c), C.byref(r), C.byref(k)))
         return c.value, r.value, k.value
 
-    def search_batch(self, items) -> list:
-        """items: sequence of dicts with 'image', 'ctf' and optionally 'templates', 'alt_az'.
-        Returns one record array per item (gs_search_batch)."""
+    def search_batch_refine(self, items, top_k: int, iterations=None, step=None, scale=None):
+        """gs_search_batch_refine: returns (record arrays per item, REFINED_DTYPE arrays per item)."""
+        arr, keep = self._batch_array(items)
+        o = load().gs_refine_defaults()
+        if iterations is not None:
+            o.iterations = iterations
+        if step is not None:
+            o.step = step
+        if scale is not None:
+            for i in range(4):
+                o.scale[i] = scale[i]
+        counts = (C.c_size_t * len(items))()
+        rcounts = (C.c_size_t * len(items))()
+        p, q = C.c_void_p(), C.c_void_p()
+        _check(load().gs_search_batch_refine(self._h, arr, len(items), top_k, C.byref(o), C.byref(p), counts,
+                                             C.byref(q), rcounts))
+        try:
+            total = sum(counts)
+            buf = (C.c_char * (total * RECORD_DTYPE.itemsize)).from_address(p.value) if total else b""
+            allrec = np.frombuffer(bytes(buf), dtype=RECORD_DTYPE).copy()
+            rtotal = sum(rcounts)
+            rbuf = (C.c_char * (rtotal * REFINED_DTYPE.itemsize)).from_address(q.value) if rtotal else b""
+            allref = np.frombuffer(bytes(rbuf), dtype=REFINED_DTYPE).copy()
+        finally:
+            load().gs_free(p)
+            load().gs_free(q)
+        self._ctf = Ctf(*items[-1].get("ctf", (1.5, 0.0, 0.0)))
+        recs, refs, o1, o2 = [], [], 0, 0
+        for c, rc in zip(counts, rcounts):
+            recs.append(allrec[o1:o1 + c])
+            refs.append(allref[o2:o2 + rc])
+            o1 += c
+            o2 += rc
+        return recs, refs
+
+    @staticmethod
+    def _batch_array(items):
         keep = []
         arr = (BatchItem * len(items))()
         for i, it in enumerate(items):
@@ -338,6 +375,12 @@ class Engine:
                     aa = _f32(aa).reshape(-1, 2)
                     keep.append(aa)
                     b.alt_az = aa.ctypes.data
+        return arr, keep
+
+    def search_batch(self, items) -> list:
+        """items: sequence of dicts with 'image', 'ctf' and optionally 'templates', 'alt_az'.
+        Returns one record array per item (gs_search_batch)."""
+        arr, keep = self._batch_array(items)
         counts = (C.c_size_t * len(items))()
         p = C.c_void_p()
         _check(load().gs_search_batch(self._h, arr, len(items), C.byref(p), counts))

@@ -118,7 +118,7 @@ typedef enum {
   GS_STAGE_SIGMA = 5,      /* norm: per block pixel sigma, nb*l*l float64 */
   GS_STAGE_BEST = 6,       /* norm: per block pixel best correlation, nb*l*l float32 */
   GS_STAGE_ARG = 7,        /* norm: per block pixel argmax angle_index*n+template, nb*l*l int32 */
-  GS_STAGE_PLANES = 8      /* nonorm: per (tile, angle, block, template) peak, index, sum, sum2 float64 */
+  GS_STAGE_PLANES = 8      /* nonorm: peak, index, sum, sum2 float64 per [tile][template][angle][block] */
 } gs_stage;
 
 /* Size in bytes of a stage buffer after the last run (0 if not available). */
@@ -239,6 +239,14 @@ gs_status gs_refine(gs_engine* e, const gs_peak* peaks, size_t k, const gs_refin
                     gs_refined* out);
 /* Device time (ms) of the last gs_refine or gs_score_and_grad call. */
 gs_status gs_refine_timing(const gs_engine* e, double* ms);
+
+/* gs_search_batch (norm_type=1) with refinement: after each micrograph is searched, its first
+ * top_k records (its best picks) are refined on it with opts (NULL = defaults) before the next
+ * micrograph.  *refined receives the refinement results in item order (free with gs_free) and
+ * refined_counts[i] their number for item i. */
+gs_status gs_search_batch_refine(gs_engine* e, const gs_batch_item* items, int n_items, int top_k,
+                                 const gs_refine_opts* opts, gs_record** out, size_t* counts,
+                                 gs_refined** refined, size_t* refined_counts);
 
 #ifdef __cplusplus
 }

@@ -65,6 +65,33 @@ def test_batch_equals_per_micrograph_search(gs):
         eb.close()
 
 
+def test_batch_refine_equals_search_then_refine(gs):
+    """gs_search_batch_refine = per micrograph: gs_search, then gs_refine of its top picks."""
+    a, b, _ = _micrographs()
+    cfg = _cfg(gs)
+    eng = gs.Engine(cfg)
+    eng.load_templates(a.templates)
+    plain, prefs = [], []
+    for ds in (a, b):
+        r = eng.search(ds.image, ds.ctf)
+        plain.append(r)
+        prefs.append(eng.refine(eng.peaks_from_records(r[:3]), iterations=5))
+    eng.close()
+    eb = gs.Engine(cfg)
+    recs, refs = eb.search_batch_refine([{"image": a.image, "ctf": a.ctf, "templates": a.templates},
+                                         {"image": b.image, "ctf": b.ctf}], 3, iterations=5)
+    eb.close()
+    assert len(recs) == 2 and len(refs) == 2
+    for x, y in zip(recs, plain):
+        _same(x, y)
+    for x, y in zip(refs, prefs):
+        assert len(x) == len(y) > 0
+        for n in ("score", "score0", "grad"):
+            assert np.array_equal(x[n], y[n]), n
+        for n in x["peak"].dtype.names:
+            assert np.array_equal(x["peak"][n], y["peak"][n]), n
+
+
 def test_graph_replays_match_eager_runs(gs):
     """The captured angle loop, replayed several times with other work in between (a search of
     another shape, a standalone transform, host synchronisation), gives the eager result every
