@@ -39,9 +39,25 @@ static int parse_dump(const char* path) {
   return 0;
 }
 
+// Dimensions and checksums of an MRC file as read (no GPU use).
+static int mrc_info(const char* path) {
+  gsio::Mrc m;
+  std::string err;
+  if (!gsio::read_mrc(path, m, err)) return die(err);
+  double s = 0, s2 = 0;
+  for (float v : m.data) {
+    s += v;
+    s2 += (double)v * v;
+  }
+  std::printf("mrc %d %d %d %.17g %.17g\n", m.nx, m.ny, m.nz, s, s2);
+  return 0;
+}
+
 int main(int argc, char** argv) {
-  if (argc < 2) return die("usage: gisspa_main <config> | gisspa_main --parse-dump <config>");
+  if (argc < 2)
+    return die("usage: gisspa_main <config> | --parse-dump <config> | --mrc-info <file.mrc>");
   if (std::string(argv[1]) == "--parse-dump") return argc > 2 ? parse_dump(argv[2]) : die("missing config");
+  if (std::string(argv[1]) == "--mrc-info") return argc > 2 ? mrc_info(argv[2]) : die("missing file");
   gsio::Config conf;
   if (!conf.parse(argv[1])) return die(conf.errors.empty() ? "bad config" : conf.errors[0]);
   auto missing = conf.missing_required();
