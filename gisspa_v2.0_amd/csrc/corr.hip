@@ -276,10 +276,10 @@ __global__ void __launch_bounds__(CorrCfg<L>::RT)
   constexpr int PPT = ROWS * L / NT;
   static_assert(PPT * NT == ROWS * L, "pixels per thread");
   static_assert(NTR * TPT == NT, "row pass thread mapping");
+  constexpr int NW = (NT + 63) / 64;  // wavefronts per workgroup
   __shared__ float2 buf[NTR * H];
   __shared__ float2 twh[H];
   __shared__ float2 twc[H];
-  __shared__ PlaneBest red[PLANE ? 512 : 1];
 
   const int rg = blockIdx.x, b = blockIdx.y, tid = threadIdx.x;
   const int groups = L / ROWS;
@@ -371,19 +371,21 @@ __global__ void __launch_bounds__(CorrCfg<L>::RT)
             }
           }
         }
-        red[tid] = pb;
-        for (int j = tid + NT; j < 512; j += NT) red[j] = PlaneBest{0.f, 0x7fffffff, 0.f, 0.f};
-        __syncthreads();
-        for (int h = 256; h > 0; h >>= 1) {
-          if (tid < h) red[tid] = plane_combine(red[tid], red[tid + h]);
-          __syncthreads();
+        // per-wavefront reduction; one partial per (row group, wavefront), merged by
+        // k_finalize_plane (the combine rule is order-independent)
+        const int wave = tid / 64, lane = tid % 64;
+        const int nact = min(64, NT - wave * 64);  // lanes present in this wavefront
+        for (int o = 32; o > 0; o >>= 1) {
+          PlaneBest q;
+          q.v = __shfl_down(pb.v, o, 64);
+          q.i = __shfl_down(pb.i, o, 64);
+          q.s = __shfl_down(pb.s, o, 64);
+          q.s2 = __shfl_down(pb.s2, o, 64);
+          if (lane + o < nact) pb = plane_combine(pb, q);
         }
-        if (tid == 0 && t + ti < tc) {
-          const PlaneBest r = red[0];
-          partial[((long long)(t + ti) * nb + b) * groups + rg] =
-              make_float4(r.v, __int_as_float(r.i), r.s, r.s2);
-        }
-        __syncthreads();
+        if (lane == 0 && t + ti < tc)
+          partial[(((long long)(t + ti) * nb + b) * groups + rg) * NW + wave] =
+              make_float4(pb.v, __int_as_float(pb.i), pb.s, pb.s2);
       }
     }
     __syncthreads();
@@ -751,9 +753,11 @@ bool launch_plane_fft2(float2* data, int batch, int L, int sign, float scale, hi
   return false;
 }
 
+// Partials per (template, block) written by the norm_type=0 row pass: one per row group and
+// wavefront.
 int rowpass_groups(int L) {
-  if (L == 320) return 320 / CorrCfg<320>::ROWS;
-  if (L == 720) return 720 / CorrCfg<720>::ROWS;
+  if (L == 320) return 320 / CorrCfg<320>::ROWS * ((CorrCfg<320>::RT + 63) / 64);
+  if (L == 720) return 720 / CorrCfg<720>::ROWS * ((CorrCfg<720>::RT + 63) / 64);
   return 0;
 }
 

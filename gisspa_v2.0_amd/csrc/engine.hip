@@ -778,7 +778,12 @@ class Engine {
     float2* F = d_frame_.ensure(N);
     float2* S = d_frame2_.ensure(N);
     launch_real_to_complex(in, F, (long long)N, 1.f, stream_);
-    fft2_batched(fft_, F, S, 1, ny, nx, -1, (float)(1.0 / (double)N), stream_);
+    // square frames of a built size (the 720 x 720 tiles of norm_type=0) take the LDS transform
+    auto fft2 = [&](int sign, float scale) {
+      if (nx != ny || !launch_plane_fft2(F, 1, nx, sign, scale, stream_))
+        fft2_batched(fft_, F, S, 1, ny, nx, sign, scale, stream_);
+    };
+    fft2(-1, (float)(1.0 / (double)N));
     set_image_band(nx, ny);
     launch_phase_flip(F, nx, ny, dp_, stream_);
     const int nbins = std::max(nx, ny) / 2;
@@ -787,7 +792,7 @@ class Engine {
     launch_radial_power_image(F, nx, ny, ra, stream_);
     launch_whiten_filter_image(F, nx, ny, ra, radial_counts_dev(nx, ny), dp_, stream_);
     if (dc_zero) launch_zero_dc(F, stream_);
-    fft2_batched(fft_, F, S, 1, ny, nx, +1, 1.f, stream_);
+    fft2(+1, 1.f);
     launch_extract_real(F, out, (long long)N, stream_);
   }
 
@@ -800,11 +805,12 @@ class Engine {
     const size_t L2 = (size_t)l * l;
     launch_pad_templates(d_raw_.p, n, w_, h_, l, T, stream_);
     double* mom = d_mom_.ensure((size_t)n * 4);
-    launch_template_moments(T, n, l, mom, stream_);
+    double* part = d_chpart_.ensure((size_t)n * 16 * std::max(4, l / 2));
+    launch_template_moments(T, n, l, part, mom, stream_);
     launch_template_standardise(T, n, l, mom, stream_);
     fft2_planes(T, S, n, -1, (float)(1.0 / (double)L2));
     double* ra = d_ra_.ensure((size_t)n * (l / 2));
-    launch_radial_power_planes(T, n, l, ra, stream_);
+    launch_radial_power_planes(T, n, l, part, ra, stream_);
     launch_whiten_planes(T, n, l, ra, radial_counts_dev(l, l), stream_);
   }
 
@@ -1233,7 +1239,7 @@ class Engine {
 
   DevBuf<float> d_raw_, d_img_, d_imgpre_, d_tprep_, d_rot_, d_stats_, d_best_, d_tile_;
   DevBuf<float2> d_work_, d_work2_, d_scratch_, d_frame_, d_frame2_, d_blocks_, d_ihat_, d_that_, d_Z_, d_cs_;
-  DevBuf<double> d_mom_, d_ra_, d_ra_img_, d_m_, d_part_, d_S_, d_SS_, d_mu_, d_sigma_, d_planes_;
+  DevBuf<double> d_mom_, d_ra_, d_ra_img_, d_m_, d_part_, d_S_, d_SS_, d_mu_, d_sigma_, d_planes_, d_chpart_;
   DevBuf<float> d_plane_part_;
   DevBuf<int32_t> d_arg_;
   DevBuf<int> d_offx_, d_offy_, d_count_;

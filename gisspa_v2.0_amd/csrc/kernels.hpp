@@ -31,9 +31,11 @@ struct Cand {
 
 // ---- template preparation (src/norm.cu:194-354, src/nonorm.cu:178-344) ----
 void launch_pad_templates(const float* raw, int n, int w, int h, int l, float2* out, hipStream_t s);
-void launch_template_moments(const float2* T, int n, int l, double* mom, hipStream_t s);
+// part: scratch of 64 doubles per template.
+void launch_template_moments(const float2* T, int n, int l, double* part, double* mom, hipStream_t s);
 void launch_template_standardise(float2* T, int n, int l, const double* mom, hipStream_t s);
-void launch_radial_power_planes(const float2* F, int n, int l, double* ra, hipStream_t s);
+// part: scratch of 16 * (l/2) doubles per plane.
+void launch_radial_power_planes(const float2* F, int n, int l, double* part, double* ra, hipStream_t s);
 void launch_whiten_planes(float2* F, int n, int l, const double* ra, const double* rb, hipStream_t s);
 void launch_weight_planes(float2* F, int n, int l, const DevParams& p, hipStream_t s);
 void launch_mask_planes(float2* T, int n, int l, float d_m, float ehw, hipStream_t s);

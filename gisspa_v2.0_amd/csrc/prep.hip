@@ -40,9 +40,14 @@ __global__ void k_pad_templates(const float* __restrict__ raw, int n, int w, int
   out[i] = make_float2(v, 0.f);
 }
 
-// Ring sum/count (generate_mask + multiCount_dot, k:28-80) and box sums (UpdateSigma, k:4-26).
-__global__ void k_template_moments(const float2* __restrict__ T, int l, double* __restrict__ mom) {
-  const int t = blockIdx.x;
+// Per-plane reductions are split over kChunks workgroups per plane (slices of the plane), whose
+// partials a finishing kernel adds in chunk order.
+constexpr int kChunks = 16;
+
+// Ring sum/count (generate_mask + multiCount_dot, k:28-80) and box sums (UpdateSigma, k:4-26);
+// grid (kChunks, n), partials [t][chunk][4].
+__global__ void k_template_moments(const float2* __restrict__ T, int l, double* __restrict__ part) {
+  const int t = blockIdx.y, ch = blockIdx.x;
   const float2* P = T + (long long)t * l * l;
   const float r = l / 2.f - 2.f;
   float up = 0.f, low = 0.f;
@@ -50,8 +55,10 @@ __global__ void k_template_moments(const float2* __restrict__ T, int l, double* 
     up = (r + 1) * (r + 1);
     low = (r - 1) * (r - 1);
   }
+  const int per = (l * l + kChunks - 1) / kChunks;
+  const int p0 = ch * per, p1 = min(l * l, p0 + per);
   double ring = 0, cnt = 0, s = 0, s2 = 0;
-  for (int p = threadIdx.x; p < l * l; p += blockDim.x) {
+  for (int p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
     const int x = p % l, y = p / l;
     const float rr = (float)((x - l / 2) * (x - l / 2) + (y - l / 2) * (y - l / 2));
     const double v = P[p].x;
@@ -67,11 +74,21 @@ __global__ void k_template_moments(const float2* __restrict__ T, int l, double* 
   s = block_sum(s);
   s2 = block_sum(s2);
   if (threadIdx.x == 0) {
-    mom[4 * t + 0] = ring;
-    mom[4 * t + 1] = cnt;
-    mom[4 * t + 2] = s;
-    mom[4 * t + 3] = s2;
+    double* o = part + ((long long)t * kChunks + ch) * 4;
+    o[0] = ring;
+    o[1] = cnt;
+    o[2] = s;
+    o[3] = s2;
   }
+}
+
+__global__ void k_template_moments_finish(const double* __restrict__ part, int n, double* __restrict__ mom) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  double a[4] = {0, 0, 0, 0};
+  for (int ch = 0; ch < kChunks; ++ch)
+    for (int k = 0; k < 4; ++k) a[k] += part[((long long)t * kChunks + ch) * 4 + k];
+  for (int k = 0; k < 4; ++k) mom[4 * t + k] = a[k];
 }
 
 // scale_each (k:82-90) with the host sigma of src/norm.cu:268-273.
@@ -89,15 +106,18 @@ __global__ void k_template_standardise(float2* __restrict__ T, int n, int l, con
   T[i].x = (float)(((double)T[i].x - em) / sigma);
 }
 
-// Radial power over the half plane x <= l/2 (SQRSum_by_circle, k:94-126); one block per plane.
-__global__ void k_radial_power_planes(const float2* __restrict__ F, int l, double* __restrict__ ra) {
+// Radial power over the half plane x <= l/2 (SQRSum_by_circle, k:94-126); grid (kChunks, n),
+// LDS histogram of one slice per workgroup, partials [t][chunk][bin].
+__global__ void k_radial_power_planes(const float2* __restrict__ F, int l, double* __restrict__ part) {
   extern __shared__ double hist[];
-  const int t = blockIdx.x, nbins = l / 2;
+  const int t = blockIdx.y, ch = blockIdx.x, nbins = l / 2;
   for (int i = threadIdx.x; i < nbins; i += blockDim.x) hist[i] = 0.0;
   __syncthreads();
   const float2* P = F + (long long)t * l * l;
   const int hx = l / 2 + 1;
-  for (int p = threadIdx.x; p < l * hx; p += blockDim.x) {
+  const int per = (l * hx + kChunks - 1) / kChunks;
+  const int p0 = ch * per, p1 = min(l * hx, p0 + per);
+  for (int p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
     const int y = p / hx, x = p % hx;
     const int r = rbin(min(x, l - x), min(y, l - y));
     if (r >= 0 && r < nbins) {
@@ -106,7 +126,16 @@ __global__ void k_radial_power_planes(const float2* __restrict__ F, int l, doubl
     }
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < nbins; i += blockDim.x) ra[(long long)t * nbins + i] = hist[i];
+  for (int i = threadIdx.x; i < nbins; i += blockDim.x) part[((long long)t * kChunks + ch) * nbins + i] = hist[i];
+}
+
+__global__ void k_radial_power_finish(const double* __restrict__ part, int n, int nbins, double* __restrict__ ra) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long long)n * nbins) return;
+  const long long t = i / nbins, b = i % nbins;
+  double s = 0;
+  for (int ch = 0; ch < kChunks; ++ch) s += part[(t * kChunks + ch) * nbins + b];
+  ra[i] = s;
 }
 
 // whiten_Tmp (k:128-149): |F| /= sqrt(ra[r]/rb[r]) for r in [0, l/2).
@@ -444,16 +473,20 @@ void launch_pad_templates(const float* raw, int n, int w, int h, int l, float2* 
   k_pad_templates<<<nblk((long long)n * l * l), BS, 0, s>>>(raw, n, w, h, l, out);
   GS_CHECK_LAUNCH();
 }
-void launch_template_moments(const float2* T, int n, int l, double* mom, hipStream_t s) {
-  k_template_moments<<<n, BS, 0, s>>>(T, l, mom);
+void launch_template_moments(const float2* T, int n, int l, double* part, double* mom, hipStream_t s) {
+  k_template_moments<<<dim3(kChunks, n), BS, 0, s>>>(T, l, part);
+  GS_CHECK_LAUNCH();
+  k_template_moments_finish<<<nblk(n), BS, 0, s>>>(part, n, mom);
   GS_CHECK_LAUNCH();
 }
 void launch_template_standardise(float2* T, int n, int l, const double* mom, hipStream_t s) {
   k_template_standardise<<<nblk((long long)n * l * l), BS, 0, s>>>(T, n, l, mom);
   GS_CHECK_LAUNCH();
 }
-void launch_radial_power_planes(const float2* F, int n, int l, double* ra, hipStream_t s) {
-  k_radial_power_planes<<<n, BS, sizeof(double) * (l / 2), s>>>(F, l, ra);
+void launch_radial_power_planes(const float2* F, int n, int l, double* part, double* ra, hipStream_t s) {
+  k_radial_power_planes<<<dim3(kChunks, n), BS, sizeof(double) * (l / 2), s>>>(F, l, part);
+  GS_CHECK_LAUNCH();
+  k_radial_power_finish<<<nblk((long long)n * (l / 2)), BS, 0, s>>>(part, n, l / 2, ra);
   GS_CHECK_LAUNCH();
 }
 void launch_whiten_planes(float2* F, int n, int l, const double* ra, const double* rb, hipStream_t s) {
