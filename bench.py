@@ -391,9 +391,12 @@ def main():
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "corr_c1_traffic.json")
     if os.path.exists(tpath) and l == 320:
-        col_name = f"k_colpass<320, {1 if variant[0] == 3 else 0}>" if variant[0] in (0, 3) else ""
+        col_name = {0: "k_colpass<320, 0>", 3: "k_colpass<320, 1>", 4: "k_colpass<320, 1, true>",
+                    5: "k_colpass<320, 0, true>"}.get(variant[0], "")
         row_name = {0: "k_rowpass_pix_wave<320, false, 4>", 1: "k_rowpass_pix_wave<320, true, 4>",
-                    2: "k_rowpass_pix_wave<320, true, 8>", 3: "k_rowpass_pix_wave<320, false, 8>"}[variant[1]]
+                    2: "k_rowpass_pix_wave<320, true, 8>", 3: "k_rowpass_pix_wave<320, false, 8>",
+                    4: "k_rowpass_pix_wave<320, true, 4, true>",
+                    5: "k_rowpass_pix_wave<320, false, 4, true>"}[variant[1]]
         kname = col_name if dominant == "column_pass" else row_name
         rec = json.load(open(tpath))["kernels"].get(kname)
         if rec:

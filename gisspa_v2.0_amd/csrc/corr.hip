@@ -64,18 +64,23 @@ struct ColPlanOf<320, 1> {
   using type = ct::RadixList<4, 4, 20>;
 };
 
-template <int L, int Plan = 0>
+// SW: columns stored unpadded with element y of column c at slot y ^ (c & 15) (an XOR swizzle
+// that keeps adjacent columns on different LDS banks) and twiddles read from the global table:
+// 40 KB of LDS at L = 320, so four workgroups fit a CU instead of three.
+template <int L, int Plan = 0, bool SW = false>
 __global__ void __launch_bounds__(CorrCfg<L>::CT)
     k_colpass(const float2* __restrict__ that, const float2* __restrict__ ihat,
               const float2* __restrict__ twL, float2* __restrict__ Z, int t0, int tc, int nb) {
   using C = CorrCfg<L>;
-  constexpr int H = C::H, COLS = C::COLS, NT = C::CT, TPC = C::TPC, LP = L + 1;
-  constexpr int NCG = H / COLS;
+  constexpr int H = C::H, COLS = C::COLS, NT = C::CT, TPC = C::TPC, LP = SW ? L : L + 1;
   static_assert(COLS * TPC == NT, "column pass thread mapping");
   static_assert(H % COLS == 0, "column groups");
+  static_assert(!SW || L % 16 == 0, "swizzle stays inside the column");
   constexpr long long PLANE = (long long)(H + 1) * L;
-  __shared__ float2 buf[COLS * LP];
-  __shared__ float2 tw[L];
+  __shared__ float2 smem[COLS * LP + (SW ? 0 : L)];
+  float2* const buf = smem;
+  float2* const tws = smem + COLS * LP;  // LDS twiddles (unless SW)
+  auto slot = [](int c, int y) { return SW ? (y ^ (c & 15)) : y; };
 
   // Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a contiguous range of
   // logical work.  Logical order tiles the (block, template) product GEMM-style per column
@@ -99,8 +104,11 @@ __global__ void __launch_bounds__(CorrCfg<L>::CT)
   if (b >= nb || tl >= tc) return;  // padding of the last tiles (uniform per workgroup)
   const int tid = threadIdx.x;
 
-  for (int i = tid; i < L; i += NT) tw[i] = twL[i];
-  __syncthreads();
+  if constexpr (!SW) {
+    for (int i = tid; i < L; i += NT) tws[i] = twL[i];
+    __syncthreads();
+  }
+  const float2* const tw = SW ? twL : tws;
   const float2* Tp = that + (long long)(t0 + tl) * PLANE;
   const float2* Ip = ihat + (long long)b * PLANE;
   const int kx0 = cg * COLS;
@@ -144,12 +152,18 @@ __global__ void __launch_bounds__(CorrCfg<L>::CT)
         const float2 pn = c_mulconj(in_[i < NDC ? i : 0], tn[i < NDC ? i : 0]);
         p = make_float2(p.x - pn.y, p.y + pn.x);
       }
-      buf[(cw0 + cl) * LP + ky] = p;
+      buf[(cw0 + cl) * LP + slot(cw0 + cl, ky)] = p;
     }
   }
   ct::lds_sync<ct::Sync::WAVE>();
-  ct::run_stockham<L, TPC, +1, ct::Sync::WAVE>(buf + (tid / TPC) * LP, tw, tid % TPC,
-                                                typename ColPlanOf<L, Plan>::type{});
+  const int col = tid / TPC;
+  if constexpr (SW) {
+    ct::run_stockham<L, TPC, +1, ct::Sync::WAVE>(buf + col * LP, tw, tid % TPC,
+                                                  typename ColPlanOf<L, Plan>::type{}, ct::XorMap{col & 15});
+  } else {
+    ct::run_stockham<L, TPC, +1, ct::Sync::WAVE>(buf + col * LP, tw, tid % TPC,
+                                                  typename ColPlanOf<L, Plan>::type{});
+  }
   __syncthreads();  // rows of Z gather all COLS columns of the workgroup
 
   float2* Zp = Z + ((long long)tl * nb + b) * (long long)L * H;
@@ -157,7 +171,7 @@ __global__ void __launch_bounds__(CorrCfg<L>::CT)
     const int y = q / COLS, cc = q - y * COLS;
     int py = y + L / 2;
     if (py >= L) py -= L;
-    Zp[(long long)py * H + kx0 + cc] = buf[cc * LP + y];
+    Zp[(long long)py * H + kx0 + cc] = buf[cc * LP + slot(cc, y)];
   }
 }
 
@@ -416,7 +430,9 @@ __global__ void __launch_bounds__(CorrCfg<L>::RT)
 // loads each Hermitian pair X(k), X(H-k) once and writes both packed values.  PF: load the next
 // iteration's pairs before transforming the current one (more registers, fewer stalls).
 // TPI_: templates per wave iteration (rows per wave = transforms per wave / TPI_).
-template <int L, bool PF, int TPI_ = CorrCfg<L>::TPI>
+// TWG: twiddles read from the global tables (L1-resident) instead of LDS copies, which keeps
+// the workgroup at exactly 40 KB of LDS so that four workgroups fit a CU.
+template <int L, bool PF, int TPI_ = CorrCfg<L>::TPI, bool TWG = false>
 __global__ void __launch_bounds__(256)
     k_rowpass_pix_wave(const float2* __restrict__ Z, const float2* __restrict__ twL,
                        const float2* __restrict__ twH, int tc, int nb, double* __restrict__ S,
@@ -431,20 +447,32 @@ __global__ void __launch_bounds__(256)
   constexpr int PPL = RW * L / 64;
   static_assert(TPW % TPI == 0 && RW * L % 64 == 0, "wave row mapping");
   static_assert((H / 2) % TPT == 0, "paired packing");
-  __shared__ float2 buf[WAVES][TPW * H];
-  __shared__ float2 twh[H];
-  __shared__ float2 twc[H];
+  // wave slices, then (unless TWG) the twiddles exp(-2 pi i k/H) and exp(+2 pi i k/L)
+  __shared__ float2 smem[WAVES * TPW * H + (TWG ? 0 : 2 * H)];
+  float2* const twh = smem + WAVES * TPW * H;
+  float2* const twc = twh + H;
 
   const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
-  for (int i = tid; i < H; i += 256) {
-    twh[i] = twH[i];
-    const float2 w = twL[i];
-    twc[i] = make_float2(w.x, -w.y);  // exp(+2 pi i k / L)
+  if constexpr (!TWG) {
+    for (int i = tid; i < H; i += 256) {
+      twh[i] = twH[i];
+      const float2 w = twL[i];
+      twc[i] = make_float2(w.x, -w.y);  // exp(+2 pi i k / L)
+    }
+    __syncthreads();
   }
-  __syncthreads();
+  auto wpack = [&](int k) {  // exp(+2 pi i k / L)
+    if constexpr (TWG) {
+      const float2 w = twL[k];
+      return make_float2(w.x, -w.y);
+    } else {
+      return twc[k];
+    }
+  };
+  const float2* const twf = TWG ? twH : twh;
   const int b = blockIdx.y;
   const int y0 = blockIdx.x * ROWS + wave * RW;
-  float2* wb = buf[wave];
+  float2* wb = smem + wave * TPW * H;
   const int tr = lane / TPT, sub = lane % TPT;
   const int rowl = tr / TPI, ti = tr % TPI;
   float2* dst = wb + tr * H;
@@ -487,20 +515,20 @@ __global__ void __launch_bounds__(256)
         {
           const float2 Bc = c_conj(Bv);
           const float2 sm = c_add(A, Bc), df = c_sub(A, Bc);
-          const float2 wd = c_mul(twc[k], df);
+          const float2 wd = c_mul(wpack(k), df);
           dst[k] = make_float2(sm.x - wd.y, sm.y + wd.x);
         }
         if (k != H - k) {
           const float2 Ac = c_conj(A);
           const float2 sm = c_add(Bv, Ac), df = c_sub(Bv, Ac);
-          const float2 wd = c_mul(twc[H - k], df);
+          const float2 wd = c_mul(wpack(H - k), df);
           dst[H - k] = make_float2(sm.x - wd.y, sm.y + wd.x);
         }
       }
     }
     if constexpr (PF) prefetch(t + TPI);
     ct::lds_sync<ct::Sync::WAVE>();
-    ct::run_stockham<H, TPT, +1, ct::Sync::WAVE>(dst, twh, sub, typename C::RowPlan{});
+    ct::run_stockham<H, TPT, +1, ct::Sync::WAVE>(dst, twf, sub, typename C::RowPlan{});
     const float* out = reinterpret_cast<const float*>(wb);
 #pragma unroll
     for (int i = 0; i < PPL; ++i) {
@@ -764,13 +792,17 @@ int rowpass_groups(int L) {
 template <int L>
 static void colpass_impl(const CorrArgs& a, const float2* twL, int variant, hipStream_t s) {
   using C = CorrCfg<L>;
-  if (variant == 0 || variant == 3) {
+  if (variant == 0 || variant >= 3) {
     constexpr int BT = 8, TT = 16;  // tile shape of k_colpass's logical order
     const long long G = (long long)(C::H / C::COLS) * ((a.tc + TT - 1) / TT * TT) *
                         ((a.nb + BT - 1) / BT * BT);
     GS_REQUIRE(G > 0 && G < (1LL << 31), kInvalid, "column pass grid out of range");
     if (variant == 3)
       k_colpass<L, 1><<<(unsigned)G, C::CT, 0, s>>>(a.that, a.ihat, twL, a.Z, a.t0, a.tc, a.nb);
+    else if (variant == 4)
+      k_colpass<L, 1, true><<<(unsigned)G, C::CT, 0, s>>>(a.that, a.ihat, twL, a.Z, a.t0, a.tc, a.nb);
+    else if (variant == 5)
+      k_colpass<L, 0, true><<<(unsigned)G, C::CT, 0, s>>>(a.that, a.ihat, twL, a.Z, a.t0, a.tc, a.nb);
     else
       k_colpass<L, 0><<<(unsigned)G, C::CT, 0, s>>>(a.that, a.ihat, twL, a.Z, a.t0, a.tc, a.nb);
   } else {
@@ -811,12 +843,17 @@ bool launch_rowpass_pixel(int L, const float2* Z, int tc, int nb, double* S, dou
                           float* best, int32_t* arg, int arg_base, int variant, hipStream_t s) {
   TwTables& T = tw_tables(current_device());
   if (L == 320) {
-    // variants 0/1: 2 rows x 4 templates per wave iteration; 2/3: 1 row x 8 templates
-    const int tpi = variant >= 2 ? 8 : CorrCfg<320>::TPI;
+    // variants 0/1: 2 rows x 4 templates per wave iteration; 2/3: 1 row x 8 templates;
+    // 4/5: as 1/0 with global twiddles (four workgroups per CU)
+    const int tpi = (variant == 2 || variant == 3) ? 8 : CorrCfg<320>::TPI;
     const int rows = 4 * (64 / CorrCfg<320>::TPT) / tpi;
     dim3 grid(320 / rows, nb);
     if (variant == 1)
       k_rowpass_pix_wave<320, true><<<grid, 256, 0, s>>>(Z, T.t320, T.t160, tc, nb, S, SS, best, arg, arg_base);
+    else if (variant == 4)
+      k_rowpass_pix_wave<320, true, 4, true><<<grid, 256, 0, s>>>(Z, T.t320, T.t160, tc, nb, S, SS, best, arg, arg_base);
+    else if (variant == 5)
+      k_rowpass_pix_wave<320, false, 4, true><<<grid, 256, 0, s>>>(Z, T.t320, T.t160, tc, nb, S, SS, best, arg, arg_base);
     else if (variant == 2)
       k_rowpass_pix_wave<320, true, 8><<<grid, 256, 0, s>>>(Z, T.t320, T.t160, tc, nb, S, SS, best, arg, arg_base);
     else if (variant == 3)

@@ -461,7 +461,7 @@ class Engine {
   }
   void set_chunk(int tc) { chunk_ = tc; }
   void set_variant(int kernel, int variant) {
-    GS_REQUIRE((kernel == 0 && variant >= 0 && variant <= 3) || (kernel == 1 && variant >= 0 && variant <= 3) ||
+    GS_REQUIRE((kernel == 0 && variant >= 0 && variant <= 5) || (kernel == 1 && variant >= 0 && variant <= 5) ||
                    (kernel == 2 && (variant == 0 || variant == 1)),
                kInvalid, "unknown kernel variant");
     if (kernel == 0)
@@ -504,6 +504,7 @@ class Engine {
       return what == 0 ? c : what == 1 ? r : c + r;
     };
     try {
+      // variants 4/5 (smaller LDS footprint) measured no faster on MI355X and are not tried
       double best = 0;
       int pick = col_variant_;
       for (int v : {0, 3}) {
@@ -639,7 +640,8 @@ class Engine {
     const int Kc = (int)std::min<size_t>(k, kRefineChunk);
     float2* H = d_rH_.ensure((size_t)Kc * L2);
     float2* AB = d_rAB_.ensure((size_t)2 * Kc * L2);
-    float2* S = d_rS_.ensure((size_t)2 * Kc * L2);  // also the rotation scratch (3 floats per pixel)
+    // scratch of the runtime-planned transform, needed only where no LDS plane transform exists
+    float2* S = corr_supported(l) ? nullptr : d_rS_.ensure((size_t)2 * Kc * L2);
     RefPeak* rp = d_rpk_.ensure(Kc);
     double2* part = d_rpart_.ensure((size_t)Kc * 16);
     double* part6 = d_rpart6_.ensure((size_t)Kc * l * 6);
@@ -665,9 +667,9 @@ class Engine {
       for (int it = 0; it <= iters; ++it) {
         launch_ref_template(d_twhite_.p, tab, ntab, dslot, rp, K, l, dp_, part, count, H, stream_);
         fft2(H, K, +1, 1.f);
-        launch_ref_rotate(H, rp, K, l, P_.diameter, 4.0f, reinterpret_cast<float*>(S), part6, stats, AB, stream_);
+        launch_ref_rotate(H, rp, K, l, P_.diameter, 4.0f, part6, stats, AB, stream_);
         fft2(AB, 2 * K, -1, (float)(1.0 / (double)L2));
-        GS_REQUIRE(launch_ref_dot(ihat_t, AB, rp, K, l, cg, stream_), kUnsupported, "window_size not supported");
+        GS_REQUIRE(launch_ref_dot(ihat_t, AB, rp, stats, K, l, cg, stream_), kUnsupported, "window_size not supported");
         launch_ref_update(rp, K, cg, d_mu_.p, d_sigma_.p, l, step, scale, it == 0, it < iters, dz, dz0, dg, stream_);
       }
       GS_HIP(hipMemcpyAsync(hp.data(), rp, sizeof(RefPeak) * K, hipMemcpyDeviceToHost, stream_));

@@ -252,10 +252,21 @@ __device__ __forceinline__ void lds_sync() {
   }
 }
 
+// Index map of a transform's LDS slots: identity, or an XOR swizzle of the low bits (used to
+// spread adjacent transforms over the LDS banks without padding; N must be a multiple of
+// the mask range).
+struct IdentityMap {
+  __device__ __forceinline__ int operator()(int i) const { return i; }
+};
+struct XorMap {
+  int mask;
+  __device__ __forceinline__ int operator()(int i) const { return i ^ mask; }
+};
+
 // One Stockham stage (radix R, NS = product of earlier radices) over a length-N transform in
 // buf, shared by T threads (tid in [0,T)); then recurse into the remaining radices.
-template <int N, int T, int SIGN, int NS, Sync S, int R, int... REST>
-__device__ __forceinline__ void stockham(float2* buf, const float2* tw, int tid) {
+template <int N, int T, int SIGN, int NS, Sync S, typename M, int R, int... REST>
+__device__ __forceinline__ void stockham(float2* buf, const float2* tw, int tid, M map) {
   constexpr int NB = N / R;
   constexpr int IT = (NB + T - 1) / T;
   constexpr bool even = (NB % T) == 0;
@@ -267,7 +278,7 @@ __device__ __forceinline__ void stockham(float2* buf, const float2* tw, int tid)
       const int jm = j % NS;
 #pragma unroll
       for (int r = 0; r < R; ++r) {
-        float2 a = buf[j + r * NB];
+        float2 a = buf[map(j + r * NB)];
         if constexpr (NS > 1) {
           if (r > 0) a = c_mul(a, tw_lookup<SIGN>(tw, r * jm * (N / (NS * R))));
         }
@@ -284,11 +295,11 @@ __device__ __forceinline__ void stockham(float2* buf, const float2* tw, int tid)
       const int jm = j % NS;
       const int idxD = (j / NS) * NS * R + jm;
 #pragma unroll
-      for (int r = 0; r < R; ++r) buf[idxD + r * NS] = v[it][r];
+      for (int r = 0; r < R; ++r) buf[map(idxD + r * NS)] = v[it][r];
     }
   }
   lds_sync<S>();
-  if constexpr (sizeof...(REST) > 0) stockham<N, T, SIGN, NS * R, S, REST...>(buf, tw, tid);
+  if constexpr (sizeof...(REST) > 0) stockham<N, T, SIGN, NS * R, S, M, REST...>(buf, tw, tid, map);
 }
 
 template <int... RS>
@@ -296,12 +307,12 @@ struct RadixList {
   static constexpr int product() { return (RS * ... * 1); }
 };
 
-template <int N, int T, int SIGN, Sync S = Sync::BLOCK, int... RS>
+template <int N, int T, int SIGN, Sync S = Sync::BLOCK, typename M = IdentityMap, int... RS>
 __device__ __forceinline__ void run_stockham(float2* buf, const float2* tw, int tid,
-                                             RadixList<RS...>) {
+                                             RadixList<RS...>, M map = M{}) {
   static_assert(RadixList<RS...>::product() == N, "radix plan must multiply to N");
   static_assert(S == Sync::BLOCK || (64 % T == 0), "wave-level sync needs transforms inside one wavefront");
-  stockham<N, T, SIGN, 1, S, RS...>(buf, tw, tid);
+  stockham<N, T, SIGN, 1, S, M, RS...>(buf, tw, tid, map);
 }
 
 }  // namespace ct

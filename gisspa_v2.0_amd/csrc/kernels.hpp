@@ -130,15 +130,16 @@ void launch_ref_table(const int32_t* idx, int n, int l, const DevParams& p, RefW
 void launch_ref_template(const float2* white, const RefWeight* tab, int ntab, const int32_t* slot,
                          const RefPeak* pk, int K, int l, const DevParams& p, double2* partial,
                          double count, float2* H, hipStream_t s);
-// From the inverse transform t + i dt in H: rotation, mask, standardisation and their theta and
-// defocus derivatives, packed into AB (A = r + i dr/dtheta in planes 0..K-1, B = dr/d ddf in
-// planes K..2K-1).  f3 holds 3*K*l*l floats of scratch.
+// From the inverse transform t + i dt in H: rotation and mask with their theta and defocus
+// derivatives, packed into AB (A = r0 + i dr0/dtheta in planes 0..K-1, B = dr0/d ddf in planes
+// K..2K-1), and per pick the box statistics (stats: 8 doubles) of the standardisation.
 void launch_ref_rotate(const float2* H, const RefPeak* pk, int K, int l, float d_m, float ehw,
-                       float* f3, double* partial6, double* stats, float2* AB, hipStream_t s);
-// c and dc/d(dx, dy, theta, ddf) per pick (out: K x 5) from the transformed AB; ihat is the
-// block spectra transposed to [b][ky][kx 0..l/2].
-bool launch_ref_dot(const float2* ihat, const float2* AB, const RefPeak* pk, int K, int l, double* out,
-                    hipStream_t s);
+                       double* partial6, double* stats, float2* AB, hipStream_t s);
+// c and dc/d(dx, dy, theta, ddf) per pick (out: K x 5) from the transformed AB and the
+// statistics (the standardisation is applied to the spectra); ihat is the block spectra
+// transposed to [b][ky][kx 0..l/2].
+bool launch_ref_dot(const float2* ihat, const float2* AB, const RefPeak* pk, const double* stats, int K, int l,
+                    double* out, hipStream_t s);
 // z and gradient per pick; when update, one ascent step p += step * scale^2 * dz/dp.
 void launch_ref_update(RefPeak* pk, int K, const double* cg, const double* mu, const double* sigma, int l,
                        double step, const double* scale4, bool first, bool update, double* z, double* z0,

@@ -159,9 +159,10 @@ __global__ void k_ref_pack(const float2* __restrict__ white, const RefWeight* __
 }
 
 // Rotation (float geometry as the search), bilinear derivative, mask; one row of one pick per
-// block.  Writes r0, dr0/dtheta, dr0/d ddf (unstandardised) and per-row double sums.
+// block.  Writes A = r0 + i dr0/dtheta and B = dr0/d ddf (before the box standardisation, which
+// is affine per pick and is applied to their spectra in k_ref_dot) and per-row double sums.
 __global__ void k_ref_rotate(const float2* __restrict__ H, const RefPeak* __restrict__ pk, int l,
-                             float rin, float rout, float ehw, float* __restrict__ f3,
+                             float rin, float rout, float ehw, float2* __restrict__ AB,
                              double* __restrict__ partial6, int K) {
   const int jrow = blockIdx.x, j = blockIdx.y;
   const RefPeak P = pk[j];
@@ -170,9 +171,8 @@ __global__ void k_ref_rotate(const float2* __restrict__ H, const RefPeak* __rest
   const float deg = (float)(kRefPi / 180);
   const long long L2 = (long long)l * l;
   const float2* T = H + (long long)j * L2;
-  float* R0 = f3 + (long long)j * L2;
-  float* GT = f3 + ((long long)K + j) * L2;
-  float* GD = f3 + ((long long)2 * K + j) * L2;
+  float2* A = AB + (long long)j * L2;        // r0 + i dr0/dtheta
+  float2* B = AB + ((long long)K + j) * L2;  // dr0/d ddf
   double a[6] = {0, 0, 0, 0, 0, 0};
   for (int i = threadIdx.x; i < l; i += blockDim.x) {
     float v = 0.f, vth = 0.f, vdf = 0.f;
@@ -192,9 +192,8 @@ __global__ void k_ref_rotate(const float2* __restrict__ H, const RefPeak* __rest
     vth *= m;
     vdf *= m;
     const long long q = (long long)jrow * l + i;
-    R0[q] = v;
-    GT[q] = vth;
-    GD[q] = vdf;
+    A[q] = make_float2(v, vth);
+    B[q] = make_float2(vdf, 0.f);
     a[0] += v;
     a[1] += (double)v * v;
     a[2] += vth;
@@ -241,34 +240,6 @@ __global__ void k_ref_stats(const double* __restrict__ partial6, int l, int K, d
   o[7] = dvd / (2 * sd);
 }
 
-// A = r + i dr/dtheta, B = dr/d ddf (+ 0i); r exactly as k_standardise_to_complex forms it.
-__global__ void k_ref_standardise(const float* __restrict__ f3, const double* __restrict__ stats, int l,
-                                  int K, float2* __restrict__ AB) {
-  const int j = blockIdx.y;
-  const long long L2 = (long long)l * l;
-  const double* st = stats + (long long)j * 8;
-  const float meanf = (float)st[0], varf = (float)st[1];
-  const float mean = (float)st[2], sd = (float)st[3];
-  const float dmt = (float)st[4], dst = (float)st[5], dmd = (float)st[6], dsd = (float)st[7];
-  const float inv = 1.0f / sd, inv2 = 1.0f / (sd * sd);
-  const float* R0 = f3 + (long long)j * L2;
-  const float* GT = f3 + ((long long)K + j) * L2;
-  const float* GD = f3 + ((long long)2 * K + j) * L2;
-  float2* A = AB + (long long)j * L2;
-  float2* B = AB + ((long long)K + j) * L2;
-  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < L2; e += (long long)gridDim.x * blockDim.x) {
-    const float r0 = R0[e];
-    float v = r0;
-    if (meanf != 0) v = v - meanf;
-    if (varf != 0) v = v / sqrtf(varf);
-    const float c = r0 - mean;
-    const float dth = (GT[e] - dmt) * inv - c * dst * inv2;
-    const float ddf = (GD[e] - dmd) * inv - c * dsd * inv2;
-    A[e] = make_float2(v, dth);
-    B[e] = make_float2(ddf, 0.f);
-  }
-}
-
 // Band-limited shift factor ph(k, q) = exp(2 pi i k_s q / l) (cos(pi q) at Nyquist) and d/dq.
 __device__ __forceinline__ void phase(int k, int l, int pint, double frac, float2& ph, float2& dph) {
   if (2 * k == l) {
@@ -300,8 +271,8 @@ __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
 // which is given transposed: ihat_t[b][ky][kx].
 template <int L>
 __global__ __launch_bounds__(256) void k_ref_dot(const float2* __restrict__ ihat_t, const float2* __restrict__ AB,
-                                                 const RefPeak* __restrict__ pk, int K,
-                                                 double* __restrict__ out) {
+                                                 const RefPeak* __restrict__ pk, const double* __restrict__ stats,
+                                                 int K, double* __restrict__ out) {
   constexpr int HX = L / 2 + 1;
   __shared__ float2 phx[HX], dphx[HX], phy[L], dphy[L];
   const int j = blockIdx.x;
@@ -309,6 +280,15 @@ __global__ __launch_bounds__(256) void k_ref_dot(const float2* __restrict__ ihat
   const int px = P.pixel % L, py = P.pixel / L;
   for (int k = threadIdx.x; k < HX; k += blockDim.x) phase(k, L, px, P.dx, phx[k], dphx[k]);
   for (int k = threadIdx.x; k < L; k += blockDim.x) phase(k, L, py, P.dy, phy[k], dphy[k]);
+  // box standardisation of the pick (substract_by_mean + divided_by_var, k:460-477, with the
+  // float mean and variance the search uses) and its theta and defocus derivatives, applied to
+  // the spectra: a constant c only moves the DC term (transforms are scaled by 1/l^2)
+  const double* st = stats + (long long)j * 8;
+  const float meanf = (float)st[0], varf = (float)st[1];
+  const float mean = (float)st[2], sd = (float)st[3];
+  const float dmt = (float)st[4], dst = (float)st[5], dmd = (float)st[6], dsd = (float)st[7];
+  const float inv_s = varf != 0 ? 1.0f / sqrtf(varf) : 1.0f;
+  const float inv = 1.0f / sd, inv2 = 1.0f / (sd * sd);
   __syncthreads();
   const long long L2 = (long long)L * L;
   const float2* I = ihat_t + (long long)P.block * HX * L;
@@ -320,10 +300,20 @@ __global__ __launch_bounds__(256) void k_ref_dot(const float2* __restrict__ ihat
     const int my = ky ? L - ky : 0, mx = kx ? L - kx : 0;
     const float2 a = A[(long long)ky * L + kx], am = A[(long long)my * L + mx];
     const float2 b = B[(long long)ky * L + kx], bm = B[(long long)my * L + mx];
-    // spectra of the three real fields
-    const float2 T = make_float2(0.5f * (a.x + am.x), 0.5f * (a.y - am.y));
-    const float2 Tt = make_float2(0.5f * (a.y + am.y), -0.5f * (a.x - am.x));
-    const float2 Td = make_float2(0.5f * (b.x + bm.x), 0.5f * (b.y - bm.y));
+    // spectra of the three raw fields r0, dr0/dtheta, dr0/d ddf
+    float2 R = make_float2(0.5f * (a.x + am.x), 0.5f * (a.y - am.y));
+    float2 Gt = make_float2(0.5f * (a.y + am.y), -0.5f * (a.x - am.x));
+    float2 Gd = make_float2(0.5f * (b.x + bm.x), 0.5f * (b.y - bm.y));
+    float2 Rc = R;  // r0 - mean
+    if (e == 0) {
+      R.x -= meanf;
+      Rc.x -= mean;
+      Gt.x -= dmt;
+      Gd.x -= dmd;
+    }
+    const float2 T = make_float2(R.x * inv_s, R.y * inv_s);
+    const float2 Tt = make_float2(Gt.x * inv - Rc.x * dst * inv2, Gt.y * inv - Rc.y * dst * inv2);
+    const float2 Td = make_float2(Gd.x * inv - Rc.x * dsd * inv2, Gd.y * inv - Rc.y * dsd * inv2);
     float2 X = I[e];
     if ((kx + ky) & 1) X = make_float2(-X.x, -X.y);
     const float w = (kx == 0 || 2 * kx == L) ? 1.f : 2.f;
@@ -448,23 +438,21 @@ void launch_ref_template(const float2* white, const RefWeight* tab, int ntab, co
 }
 
 void launch_ref_rotate(const float2* H, const RefPeak* pk, int K, int l, float d_m, float ehw,
-                       float* f3, double* partial6, double* stats, float2* AB, hipStream_t s) {
+                       double* partial6, double* stats, float2* AB, hipStream_t s) {
   const float dm = 1.5f * d_m;  // apply_mask: d_m = 1.5 * d_m (k:233)
   const float rin = dm / 2, rout = dm / 2 + 2 * ehw;
-  k_ref_rotate<<<dim3(l, K), BS, 0, s>>>(H, pk, l, rin, rout, ehw, f3, partial6, K);
+  k_ref_rotate<<<dim3(l, K), BS, 0, s>>>(H, pk, l, rin, rout, ehw, AB, partial6, K);
   GS_CHECK_LAUNCH();
   k_ref_stats<<<nblk(K, 64), 64, 0, s>>>(partial6, l, K, stats);
   GS_CHECK_LAUNCH();
-  k_ref_standardise<<<dim3(nblk((long long)l * l, BS), K), BS, 0, s>>>(f3, stats, l, K, AB);
-  GS_CHECK_LAUNCH();
 }
 
-bool launch_ref_dot(const float2* ihat, const float2* AB, const RefPeak* pk, int K, int l, double* out,
-                    hipStream_t s) {
+bool launch_ref_dot(const float2* ihat, const float2* AB, const RefPeak* pk, const double* stats, int K, int l,
+                    double* out, hipStream_t s) {
   if (l == 320) {
-    k_ref_dot<320><<<K, 256, 0, s>>>(ihat, AB, pk, K, out);
+    k_ref_dot<320><<<K, 256, 0, s>>>(ihat, AB, pk, stats, K, out);
   } else if (l == 720) {
-    k_ref_dot<720><<<K, 256, 0, s>>>(ihat, AB, pk, K, out);
+    k_ref_dot<720><<<K, 256, 0, s>>>(ihat, AB, pk, stats, K, out);
   } else {
     return false;
   }

@@ -31,6 +31,7 @@ def _key(a):
     (0, 1, 0, 0), (0, 2, 0, 0), (0, 3, 0, 0),
     (3, 1, 4, 0),   # two launches per angle, the second partial (4 + 2 templates)
     (0, 1, 2, 1),   # three launches per angle through the double-buffered two-stream pipeline
+    (4, 4, 0, 0), (5, 5, 0, 0), (4, 5, 4, 0),  # swizzled column buffer / global twiddles
 ])
 def test_variant_matches_oracle(gs, case, col, row, chunk, overlap):
     cfg, ds, picks, z_o = case
