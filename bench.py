@@ -350,7 +350,10 @@ def main():
         log(f"autotune: column variant {col}, row variant {row}, {chunk} templates per launch "
             f"({time.perf_counter() - t1:.1f} s)")
     else:
-        col, row, chunk = max(a.col_variant, 0), max(a.row_variant, 0), a.chunk
+        # an option not given keeps the library default (variant 4 of both passes)
+        col = a.col_variant if a.col_variant >= 0 else 4
+        row = a.row_variant if a.row_variant >= 0 else 4
+        chunk = a.chunk
         eng.set_variant(0, col)
         eng.set_variant(1, row)
         if chunk:
@@ -391,12 +394,11 @@ def main():
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "corr_c1_traffic.json")
     if os.path.exists(tpath) and l == 320:
-        col_name = {0: "k_colpass<320, 0>", 3: "k_colpass<320, 1>", 4: "k_colpass<320, 1, true>",
-                    5: "k_colpass<320, 0, true>"}.get(variant[0], "")
-        row_name = {0: "k_rowpass_pix_wave<320, false, 4>", 1: "k_rowpass_pix_wave<320, true, 4>",
-                    2: "k_rowpass_pix_wave<320, true, 8>", 3: "k_rowpass_pix_wave<320, false, 8>",
-                    4: "k_rowpass_pix_wave<320, true, 4, true>",
-                    5: "k_rowpass_pix_wave<320, false, 4, true>"}[variant[1]]
+        col_name = {0: "k_colpass<320, 0, false>", 3: "k_colpass<320, 1, false>",
+                    4: "k_colpass<320, 1, true>", 5: "k_colpass<320, 0, true>"}.get(variant[0], "")
+        row_name = "k_rowpass_pix_wave<320, %s>" % {
+            0: "false, 4, false", 1: "true, 4, false", 2: "true, 8, false", 3: "false, 8, false",
+            4: "true, 4, true", 5: "false, 4, true"}.get(variant[1], "")
         kname = col_name if dominant == "column_pass" else row_name
         rec = json.load(open(tpath))["kernels"].get(kname)
         if rec:

@@ -48,6 +48,18 @@ __device__ __forceinline__ float2 c_scale(float2 a, float s) { return make_float
 __device__ __forceinline__ float2 c_mulconj(float2 a, float2 b) {
   return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
 }
+// Streaming store of a value that no later instruction of the kernel reads back (non-temporal
+// hint: the line is not kept in L2 at the expense of lines the kernel is still reading).
+__device__ __forceinline__ void st_stream(float2* p, float2 v) {
+  typedef float f2v __attribute__((ext_vector_type(2)));
+  __builtin_nontemporal_store(f2v{v.x, v.y}, reinterpret_cast<f2v*>(p));
+}
+// Streaming load of a value read once (non-temporal hint).
+__device__ __forceinline__ float2 ld_stream(const float2* p) {
+  typedef float f2v __attribute__((ext_vector_type(2)));
+  const f2v v = __builtin_nontemporal_load(reinterpret_cast<const f2v*>(p));
+  return make_float2(v.x, v.y);
+}
 // i^k * a for the sign convention of the caller (s = +1 multiplies by +i, -1 by -i)
 __device__ __forceinline__ float2 c_muli(float2 a, int s) {
   return s > 0 ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);

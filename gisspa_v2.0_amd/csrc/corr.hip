@@ -64,23 +64,31 @@ struct ColPlanOf<320, 1> {
   using type = ct::RadixList<4, 4, 20>;
 };
 
-// SW: columns stored unpadded with element y of column c at slot y ^ (c & 15) (an XOR swizzle
-// that keeps adjacent columns on different LDS banks) and twiddles read from the global table:
-// 40 KB of LDS at L = 320, so four workgroups fit a CU instead of three.
-template <int L, int Plan = 0, bool SW = false>
+// Column LDS layouts:
+//  BK = false: column stride L + 1, slots in natural order, flat twiddle table.
+//  BK = true:  column stride L + 17, slots permuted by the radix-4 bank swizzle Swz16 and
+//              twiddles in the per-stage layout: with 16 lanes per column every Stockham read
+//              and write, every twiddle read and the row-wise gather of the write-out touch 16
+//              distinct bank pairs.
+// Z is written with streaming stores: it is read back only by the next kernel, and keeping it
+// out of L2 keeps the spectra columns that neighbouring workgroups are still reading there.
+template <int L, int Plan = 0, bool BK = false>
 __global__ void __launch_bounds__(CorrCfg<L>::CT)
     k_colpass(const float2* __restrict__ that, const float2* __restrict__ ihat,
               const float2* __restrict__ twL, float2* __restrict__ Z, int t0, int tc, int nb) {
   using C = CorrCfg<L>;
-  constexpr int H = C::H, COLS = C::COLS, NT = C::CT, TPC = C::TPC, LP = SW ? L : L + 1;
+  using P = typename ColPlanOf<L, Plan>::type;
+  constexpr int H = C::H, COLS = C::COLS, NT = C::CT, TPC = C::TPC;
+  constexpr int LP = BK ? L + 17 : L + 1;
   static_assert(COLS * TPC == NT, "column pass thread mapping");
   static_assert(H % COLS == 0, "column groups");
-  static_assert(!SW || L % 16 == 0, "swizzle stays inside the column");
+  static_assert(!BK || L % 16 == 0, "swizzle stays inside the column");
+  static_assert(!BK || ct::stage_twiddle_count(P{}) <= L, "per-stage twiddles fit the table slot");
   constexpr long long PLANE = (long long)(H + 1) * L;
-  __shared__ float2 smem[COLS * LP + (SW ? 0 : L)];
+  __shared__ float2 smem[COLS * LP + L];
   float2* const buf = smem;
-  float2* const tws = smem + COLS * LP;  // LDS twiddles (unless SW)
-  auto slot = [](int c, int y) { return SW ? (y ^ (c & 15)) : y; };
+  float2* const tw = smem + COLS * LP;  // LDS twiddles
+  auto slot = [](int y) { return BK ? ct::Swz16{}(y) : y; };
 
   // Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a contiguous range of
   // logical work.  Logical order tiles the (block, template) product GEMM-style per column
@@ -104,11 +112,11 @@ __global__ void __launch_bounds__(CorrCfg<L>::CT)
   if (b >= nb || tl >= tc) return;  // padding of the last tiles (uniform per workgroup)
   const int tid = threadIdx.x;
 
-  if constexpr (!SW) {
-    for (int i = tid; i < L; i += NT) tws[i] = twL[i];
-    __syncthreads();
-  }
-  const float2* const tw = SW ? twL : tws;
+  if constexpr (BK)
+    ct::fill_stage_twiddles<L>(tw, twL, tid, NT, P{});
+  else
+    for (int i = tid; i < L; i += NT) tw[i] = twL[i];
+  __syncthreads();
   const float2* Tp = that + (long long)(t0 + tl) * PLANE;
   const float2* Ip = ihat + (long long)b * PLANE;
   const int kx0 = cg * COLS;
@@ -152,18 +160,15 @@ __global__ void __launch_bounds__(CorrCfg<L>::CT)
         const float2 pn = c_mulconj(in_[i < NDC ? i : 0], tn[i < NDC ? i : 0]);
         p = make_float2(p.x - pn.y, p.y + pn.x);
       }
-      buf[(cw0 + cl) * LP + slot(cw0 + cl, ky)] = p;
+      buf[(cw0 + cl) * LP + slot(ky)] = p;
     }
   }
   ct::lds_sync<ct::Sync::WAVE>();
   const int col = tid / TPC;
-  if constexpr (SW) {
-    ct::run_stockham<L, TPC, +1, ct::Sync::WAVE>(buf + col * LP, tw, tid % TPC,
-                                                  typename ColPlanOf<L, Plan>::type{}, ct::XorMap{col & 15});
-  } else {
-    ct::run_stockham<L, TPC, +1, ct::Sync::WAVE>(buf + col * LP, tw, tid % TPC,
-                                                  typename ColPlanOf<L, Plan>::type{});
-  }
+  if constexpr (BK)
+    ct::run_stockham<L, TPC, +1, ct::Sync::WAVE, ct::Swz16, true>(buf + col * LP, tw, tid % TPC, P{});
+  else
+    ct::run_stockham<L, TPC, +1, ct::Sync::WAVE>(buf + col * LP, tw, tid % TPC, P{});
   __syncthreads();  // rows of Z gather all COLS columns of the workgroup
 
   float2* Zp = Z + ((long long)tl * nb + b) * (long long)L * H;
@@ -171,7 +176,7 @@ __global__ void __launch_bounds__(CorrCfg<L>::CT)
     const int y = q / COLS, cc = q - y * COLS;
     int py = y + L / 2;
     if (py >= L) py -= L;
-    Zp[(long long)py * H + kx0 + cc] = buf[cc * LP + slot(cc, y)];
+    st_stream(Zp + (long long)py * H + kx0 + cc, buf[cc * LP + slot(y)]);
   }
 }
 
@@ -257,7 +262,7 @@ __global__ void __launch_bounds__(CorrCfg<L>::CT)
       const int y = q / COLS, cc = q - y * COLS;
       int py = y + L / 2;
       if (py >= L) py -= L;
-      Zp[(long long)py * H + kx0 + cc] = buf[cc * LP + y];
+      st_stream(Zp + (long long)py * H + kx0 + cc, buf[cc * LP + y]);
     }
     __syncthreads();
   }
@@ -430,9 +435,11 @@ __global__ void __launch_bounds__(CorrCfg<L>::RT)
 // loads each Hermitian pair X(k), X(H-k) once and writes both packed values.  PF: load the next
 // iteration's pairs before transforming the current one (more registers, fewer stalls).
 // TPI_: templates per wave iteration (rows per wave = transforms per wave / TPI_).
-// TWG: twiddles read from the global tables (L1-resident) instead of LDS copies, which keeps
-// the workgroup at exactly 40 KB of LDS so that four workgroups fit a CU.
-template <int L, bool PF, int TPI_ = CorrCfg<L>::TPI, bool TWG = false>
+// BK: bank-conflict-free LDS layout: transforms H + 8 slots apart (adjacent transforms of a
+// 16-lane group fall on opposite halves of the banks), slots permuted by the radix-4 swizzle
+// Swz8 and the length-H twiddles in the per-stage layout; Z is read with streaming loads (each
+// value is read once, after the column pass wrote it with streaming stores).
+template <int L, bool PF, int TPI_ = CorrCfg<L>::TPI, bool BK = false>
 __global__ void __launch_bounds__(256)
     k_rowpass_pix_wave(const float2* __restrict__ Z, const float2* __restrict__ twL,
                        const float2* __restrict__ twH, int tc, int nb, double* __restrict__ S,
@@ -445,37 +452,31 @@ __global__ void __launch_bounds__(256)
   constexpr int WAVES = 4;
   constexpr int ROWS = RW * WAVES;
   constexpr int PPL = RW * L / 64;
+  constexpr int HS = BK ? H + 8 : H;  // LDS slots per transform
   static_assert(TPW % TPI == 0 && RW * L % 64 == 0, "wave row mapping");
   static_assert((H / 2) % TPT == 0, "paired packing");
-  // wave slices, then (unless TWG) the twiddles exp(-2 pi i k/H) and exp(+2 pi i k/L)
-  __shared__ float2 smem[WAVES * TPW * H + (TWG ? 0 : 2 * H)];
-  float2* const twh = smem + WAVES * TPW * H;
-  float2* const twc = twh + H;
+  static_assert(!BK || (TPT == 8 && H % 8 == 0 && ct::stage_twiddle_count(typename C::RowPlan{}) <= H),
+                "bank-conflict-free layout: 8 lanes per transform");
+  // wave slices, then the twiddles exp(-2 pi i k/H) (flat or per-stage) and exp(+2 pi i k/L)
+  __shared__ float2 smem[WAVES * TPW * HS + 2 * H];
+  float2* const twf = smem + WAVES * TPW * HS;
+  float2* const twc = twf + H;
+  auto map = [](int i) { return BK ? ct::Swz8{}(i) : i; };
 
   const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
-  if constexpr (!TWG) {
-    for (int i = tid; i < H; i += 256) {
-      twh[i] = twH[i];
-      const float2 w = twL[i];
-      twc[i] = make_float2(w.x, -w.y);  // exp(+2 pi i k / L)
-    }
-    __syncthreads();
+  if constexpr (BK) ct::fill_stage_twiddles<H>(twf, twH, tid, 256, typename C::RowPlan{});
+  for (int i = tid; i < H; i += 256) {
+    if constexpr (!BK) twf[i] = twH[i];
+    const float2 w = twL[i];
+    twc[i] = make_float2(w.x, -w.y);  // exp(+2 pi i k / L)
   }
-  auto wpack = [&](int k) {  // exp(+2 pi i k / L)
-    if constexpr (TWG) {
-      const float2 w = twL[k];
-      return make_float2(w.x, -w.y);
-    } else {
-      return twc[k];
-    }
-  };
-  const float2* const twf = TWG ? twH : twh;
+  __syncthreads();
   const int b = blockIdx.y;
   const int y0 = blockIdx.x * ROWS + wave * RW;
-  float2* wb = smem + wave * TPW * H;
+  float2* wb = smem + wave * TPW * HS;
   const int tr = lane / TPT, sub = lane % TPT;
   const int rowl = tr / TPI, ti = tr % TPI;
-  float2* dst = wb + tr * H;
+  float2* dst = wb + tr * HS;
 
   float s[PPL], ss[PPL], bs[PPL];
   int ag[PPL];
@@ -494,12 +495,12 @@ __global__ void __launch_bounds__(256)
   auto prefetch = [&](int t) {
     if (t + ti < tc) {
       const float2* zr = Z + ((((long long)(t + ti) * nb + b) * L) + y0 + rowl) * H;
-      if (sub == 0) p0 = zr[0];
+      if (sub == 0) p0 = BK ? ld_stream(zr) : zr[0];
 #pragma unroll
       for (int m = 0; m < NP; ++m) {
         const int k = 1 + sub + TPT * m;  // 1 .. H/2
-        pa[m] = zr[k];
-        pb[m] = zr[H - k];
+        pa[m] = BK ? ld_stream(zr + k) : zr[k];
+        pb[m] = BK ? ld_stream(zr + H - k) : zr[H - k];
       }
     }
   };
@@ -515,29 +516,33 @@ __global__ void __launch_bounds__(256)
         {
           const float2 Bc = c_conj(Bv);
           const float2 sm = c_add(A, Bc), df = c_sub(A, Bc);
-          const float2 wd = c_mul(wpack(k), df);
-          dst[k] = make_float2(sm.x - wd.y, sm.y + wd.x);
+          const float2 wd = c_mul(twc[k], df);
+          dst[map(k)] = make_float2(sm.x - wd.y, sm.y + wd.x);
         }
         if (k != H - k) {
           const float2 Ac = c_conj(A);
           const float2 sm = c_add(Bv, Ac), df = c_sub(Bv, Ac);
-          const float2 wd = c_mul(wpack(H - k), df);
-          dst[H - k] = make_float2(sm.x - wd.y, sm.y + wd.x);
+          const float2 wd = c_mul(twc[H - k], df);
+          dst[map(H - k)] = make_float2(sm.x - wd.y, sm.y + wd.x);
         }
       }
     }
     if constexpr (PF) prefetch(t + TPI);
     ct::lds_sync<ct::Sync::WAVE>();
-    ct::run_stockham<H, TPT, +1, ct::Sync::WAVE>(dst, twf, sub, typename C::RowPlan{});
+    if constexpr (BK)
+      ct::run_stockham<H, TPT, +1, ct::Sync::WAVE, ct::Swz8, true>(dst, twf, sub, typename C::RowPlan{});
+    else
+      ct::run_stockham<H, TPT, +1, ct::Sync::WAVE>(dst, twf, sub, typename C::RowPlan{});
     const float* out = reinterpret_cast<const float*>(wb);
 #pragma unroll
     for (int i = 0; i < PPL; ++i) {
       const int q = lane + 64 * i;
       const int r2 = q / L, x = q - r2 * L;
+      const int xo = 2 * map(x >> 1) + (x & 1);  // real output x sits in complex slot x / 2
 #pragma unroll
       for (int u = 0; u < TPI; ++u) {
         if (t + u < tc) {
-          const float v = out[(r2 * TPI + u) * L + x];
+          const float v = out[(r2 * TPI + u) * 2 * HS + xo];
           s[i] += v;
           ss[i] = fmaf(v, v, ss[i]);
           if (v > bs[i]) {
@@ -843,21 +848,21 @@ bool launch_rowpass_pixel(int L, const float2* Z, int tc, int nb, double* S, dou
                           float* best, int32_t* arg, int arg_base, int variant, hipStream_t s) {
   TwTables& T = tw_tables(current_device());
   if (L == 320) {
-    // variants 0/1: 2 rows x 4 templates per wave iteration; 2/3: 1 row x 8 templates;
-    // 4/5: as 1/0 with global twiddles (four workgroups per CU)
+    // variants 0/1: 2 rows x 4 templates per wave iteration without/with prefetch; 2/3: 1 row x
+    // 8 templates with/without prefetch; 4/5: as 1/0 with the bank-conflict-free LDS layout
     const int tpi = (variant == 2 || variant == 3) ? 8 : CorrCfg<320>::TPI;
     const int rows = 4 * (64 / CorrCfg<320>::TPT) / tpi;
     dim3 grid(320 / rows, nb);
     if (variant == 1)
       k_rowpass_pix_wave<320, true><<<grid, 256, 0, s>>>(Z, T.t320, T.t160, tc, nb, S, SS, best, arg, arg_base);
-    else if (variant == 4)
-      k_rowpass_pix_wave<320, true, 4, true><<<grid, 256, 0, s>>>(Z, T.t320, T.t160, tc, nb, S, SS, best, arg, arg_base);
-    else if (variant == 5)
-      k_rowpass_pix_wave<320, false, 4, true><<<grid, 256, 0, s>>>(Z, T.t320, T.t160, tc, nb, S, SS, best, arg, arg_base);
     else if (variant == 2)
       k_rowpass_pix_wave<320, true, 8><<<grid, 256, 0, s>>>(Z, T.t320, T.t160, tc, nb, S, SS, best, arg, arg_base);
     else if (variant == 3)
       k_rowpass_pix_wave<320, false, 8><<<grid, 256, 0, s>>>(Z, T.t320, T.t160, tc, nb, S, SS, best, arg, arg_base);
+    else if (variant == 4)
+      k_rowpass_pix_wave<320, true, 4, true><<<grid, 256, 0, s>>>(Z, T.t320, T.t160, tc, nb, S, SS, best, arg, arg_base);
+    else if (variant == 5)
+      k_rowpass_pix_wave<320, false, 4, true><<<grid, 256, 0, s>>>(Z, T.t320, T.t160, tc, nb, S, SS, best, arg, arg_base);
     else
       k_rowpass_pix_wave<320, false><<<grid, 256, 0, s>>>(Z, T.t320, T.t160, tc, nb, S, SS, best, arg, arg_base);
     GS_CHECK_LAUNCH();

@@ -504,20 +504,26 @@ class Engine {
       return what == 0 ? c : what == 1 ? r : c + r;
     };
     try {
-      // variants 4/5 (smaller LDS footprint) measured no faster on MI355X and are not tried
+      // candidates: the natural and the bank-conflict-free LDS layouts of the column pass (radix
+      // 4*4*20), then the row pass with and without that layout (with prefetch); the other
+      // variants measured slower on MI355X at every size tried
       double best = 0;
       int pick = col_variant_;
-      for (int v : {0, 3}) {
+      bool first = true;
+      for (int v : {4, 3}) {
         col_variant_ = v;
         const double ms = measure(0);
-        if (v == 0 || ms < best) best = ms, pick = v;
+        if (first || ms < best) best = ms, pick = v;
+        first = false;
       }
       col_variant_ = pick;
       pick = row_variant_;
-      for (int v : {0, 1, 2, 3}) {
+      first = true;
+      for (int v : {4, 1}) {
         row_variant_ = v;
         const double ms = measure(1);
-        if (v == 0 || ms < best) best = ms, pick = v;
+        if (first || ms < best) best = ms, pick = v;
+        first = false;
       }
       row_variant_ = pick;
       size_t free_b = 0, total_b = 0;
@@ -526,7 +532,7 @@ class Engine {
       pick = 0;
       const int nb = nb_ > 0 ? nb_ : 1;
       const double plane = (double)P_.window_size * (P_.window_size / 2) * sizeof(float2);
-      bool first = true;
+      first = true;
       for (int c : {32, 64, 128}) {
         if (c > n_ && c != 32) break;
         // the inter-pass buffer of c templates must fit in half the free memory (plus what it holds now)
@@ -1219,7 +1225,7 @@ class Engine {
   std::vector<int> offx_, offy_, ulx_, uly_;
   std::vector<float> angles_;
   // kernel variants default to the fastest measured on the production shape (l = 320)
-  int lim_angles_ = 0, lim_templates_ = 0, chunk_ = 0, row_variant_ = 1, col_variant_ = 3;
+  int lim_angles_ = 0, lim_templates_ = 0, chunk_ = 0, row_variant_ = 4, col_variant_ = 4;
   double total_ms_ = 0, corr_ms_ = 0;
   std::vector<gs_record> records_;
   std::vector<double> planes_host_;

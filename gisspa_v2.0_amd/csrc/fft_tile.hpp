@@ -263,9 +263,77 @@ struct XorMap {
   __device__ __forceinline__ int operator()(int i) const { return i ^ mask; }
 };
 
+template <int... RS>
+struct RadixList {
+  static constexpr int product() { return (RS * ... * 1); }
+};
+
+// Bank swizzles for the radix-4 prefixes of the Stockham plans (LDS slots of 8 bytes; 16 of
+// them span the 32 four-byte banks).  A radix-4 stage writes its outputs 4 (first stage) or
+// 16 (second stage) slots apart, so T consecutive lanes of one transform would share a few
+// banks.  XOR-ing bits of the slot's block index into its low bits spreads those writes over
+// distinct banks, while reads of T consecutive slots (one aligned block) stay a permutation of
+// that block.  Both maps keep every slot inside its aligned block of T, so N % T == 0 suffices.
+// Swz16: T = 16 lanes per transform, block t = bits 4-5, low bits ^= t | t << 2.
+struct Swz16 {
+  __device__ __forceinline__ int operator()(int i) const {
+    const int t = (i >> 4) & 3;
+    return i ^ (t | (t << 2));
+  }
+};
+// Swz8: T = 8 lanes per transform, block t = bits 3-4, low bits ^= t | (t & 2) << 1.
+struct Swz8 {
+  __device__ __forceinline__ int operator()(int i) const {
+    const int t = (i >> 3) & 3;
+    return i ^ (t | ((t & 2) << 1));
+  }
+};
+
+// Per-stage twiddle layout of a radix plan: stage s > 0 (NS = product of the earlier radices,
+// radix R) holds exp(-2 pi i r jm / (NS R)) for r in [1, R), jm in [0, NS) at offset
+// OFF_s + (r - 1) NS + jm, so the lanes of a stage read consecutive entries instead of the
+// stride-r entries r jm N / (NS R) of the flat table.
+template <int... RS>
+constexpr int stage_twiddle_count() {
+  int ns = 1, n = 0;
+  for (int r : {RS...}) {
+    if (ns > 1) n += (r - 1) * ns;
+    ns *= r;
+  }
+  return n;
+}
+template <int... RS>
+constexpr int stage_twiddle_count(RadixList<RS...>) {
+  return stage_twiddle_count<RS...>();
+}
+
+// Fills the per-stage layout from the flat table flat[e] = exp(-2 pi i e / N); threads tid of
+// nt cooperate (the caller synchronises before use).
+template <int N, int... RS>
+__device__ __forceinline__ void fill_stage_twiddles(float2* dst, const float2* flat, int tid, int nt,
+                                                    RadixList<RS...>) {
+  constexpr int rs[] = {RS...};
+  int off = 0, ns = 1;
+#pragma unroll
+  for (int s = 0; s < (int)sizeof...(RS); ++s) {
+    const int R = rs[s];
+    if (ns > 1) {
+      const int cnt = (R - 1) * ns;
+      for (int e = tid; e < cnt; e += nt) {
+        const int r = 1 + e / ns, jm = e - (r - 1) * ns;
+        dst[off + e] = flat[r * jm * (N / (ns * R))];
+      }
+      off += cnt;
+    }
+    ns *= R;
+  }
+}
+
 // One Stockham stage (radix R, NS = product of earlier radices) over a length-N transform in
-// buf, shared by T threads (tid in [0,T)); then recurse into the remaining radices.
-template <int N, int T, int SIGN, int NS, Sync S, typename M, int R, int... REST>
+// buf, shared by T threads (tid in [0,T)); then recurse into the remaining radices.  OFF < 0:
+// tw is the flat table; OFF >= 0: tw is the per-stage layout and this stage's entries start at
+// OFF.
+template <int N, int T, int SIGN, int NS, Sync S, typename M, int OFF, int R, int... REST>
 __device__ __forceinline__ void stockham(float2* buf, const float2* tw, int tid, M map) {
   constexpr int NB = N / R;
   constexpr int IT = (NB + T - 1) / T;
@@ -280,7 +348,12 @@ __device__ __forceinline__ void stockham(float2* buf, const float2* tw, int tid,
       for (int r = 0; r < R; ++r) {
         float2 a = buf[map(j + r * NB)];
         if constexpr (NS > 1) {
-          if (r > 0) a = c_mul(a, tw_lookup<SIGN>(tw, r * jm * (N / (NS * R))));
+          if (r > 0) {
+            if constexpr (OFF >= 0)
+              a = c_mul(a, tw_lookup<SIGN>(tw, OFF + (r - 1) * NS + jm));
+            else
+              a = c_mul(a, tw_lookup<SIGN>(tw, r * jm * (N / (NS * R))));
+          }
         }
         v[it][r] = a;
       }
@@ -299,20 +372,18 @@ __device__ __forceinline__ void stockham(float2* buf, const float2* tw, int tid,
     }
   }
   lds_sync<S>();
-  if constexpr (sizeof...(REST) > 0) stockham<N, T, SIGN, NS * R, S, M, REST...>(buf, tw, tid, map);
+  constexpr int OFF2 = OFF < 0 ? OFF : OFF + (NS > 1 ? (R - 1) * NS : 0);
+  if constexpr (sizeof...(REST) > 0) stockham<N, T, SIGN, NS * R, S, M, OFF2, REST...>(buf, tw, tid, map);
 }
 
-template <int... RS>
-struct RadixList {
-  static constexpr int product() { return (RS * ... * 1); }
-};
-
-template <int N, int T, int SIGN, Sync S = Sync::BLOCK, typename M = IdentityMap, int... RS>
+// ST: tw holds the per-stage layout (fill_stage_twiddles) instead of the flat table.
+template <int N, int T, int SIGN, Sync S = Sync::BLOCK, typename M = IdentityMap, bool ST = false,
+          int... RS>
 __device__ __forceinline__ void run_stockham(float2* buf, const float2* tw, int tid,
                                              RadixList<RS...>, M map = M{}) {
   static_assert(RadixList<RS...>::product() == N, "radix plan must multiply to N");
   static_assert(S == Sync::BLOCK || (64 % T == 0), "wave-level sync needs transforms inside one wavefront");
-  stockham<N, T, SIGN, 1, S, M, RS...>(buf, tw, tid, map);
+  stockham<N, T, SIGN, 1, S, M, ST ? 0 : -1, RS...>(buf, tw, tid, map);
 }
 
 }  // namespace ct

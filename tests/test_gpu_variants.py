@@ -1,5 +1,5 @@
 """Every result-invariant option of the norm_type=1 correlation kernels against the FP64 oracle:
-column-pass variants 0-3, row-pass variants 0-3, templates per launch (including a partial last
+column-pass and row-pass variants 0-5, templates per launch (including a partial last
 launch) and the two-stream pipeline.  Each must give the oracle's per-pixel z of the best
 record within 1e-4 and the oracle's pick list."""
 import numpy as np
@@ -27,11 +27,12 @@ def _key(a):
 
 
 @pytest.mark.parametrize("col,row,chunk,overlap", [
-    (0, 0, 0, 0), (1, 0, 0, 0), (2, 0, 0, 0), (3, 0, 0, 0),
-    (0, 1, 0, 0), (0, 2, 0, 0), (0, 3, 0, 0),
+    (0, 0, 0, 0), (1, 0, 0, 0), (2, 0, 0, 0), (3, 0, 0, 0), (4, 0, 0, 0), (5, 0, 0, 0),
+    (0, 1, 0, 0), (0, 2, 0, 0), (0, 3, 0, 0), (0, 4, 0, 0), (0, 5, 0, 0),
     (3, 1, 4, 0),   # two launches per angle, the second partial (4 + 2 templates)
+    (4, 4, 4, 0),   # the same with the bank-conflict-free layouts of both passes
     (0, 1, 2, 1),   # three launches per angle through the double-buffered two-stream pipeline
-    (4, 4, 0, 0), (5, 5, 0, 0), (4, 5, 4, 0),  # swizzled column buffer / global twiddles
+    (5, 5, 2, 1),
 ])
 def test_variant_matches_oracle(gs, case, col, row, chunk, overlap):
     cfg, ds, picks, z_o = case
