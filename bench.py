@@ -350,9 +350,9 @@ def main():
         log(f"autotune: column variant {col}, row variant {row}, {chunk} templates per launch "
             f"({time.perf_counter() - t1:.1f} s)")
     else:
-        # an option not given keeps the library default (variant 4 of both passes)
-        col = a.col_variant if a.col_variant >= 0 else 4
-        row = a.row_variant if a.row_variant >= 0 else 4
+        # an option not given keeps the library default (column variant 3, row variant 6)
+        col = a.col_variant if a.col_variant >= 0 else 3
+        row = a.row_variant if a.row_variant >= 0 else 6
         chunk = a.chunk
         eng.set_variant(0, col)
         eng.set_variant(1, row)
@@ -398,7 +398,7 @@ def main():
                     4: "k_colpass<320, 1, true>", 5: "k_colpass<320, 0, true>"}.get(variant[0], "")
         row_name = "k_rowpass_pix_wave<320, %s>" % {
             0: "false, 4, false", 1: "true, 4, false", 2: "true, 8, false", 3: "false, 8, false",
-            4: "true, 4, true", 5: "false, 4, true"}.get(variant[1], "")
+            4: "true, 4, true", 5: "false, 4, true", 6: "true, 8, true"}.get(variant[1], "")
         kname = col_name if dominant == "column_pass" else row_name
         rec = json.load(open(tpath))["kernels"].get(kname)
         if rec:

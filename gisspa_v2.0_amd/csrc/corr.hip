@@ -84,10 +84,14 @@ __global__ void __launch_bounds__(CorrCfg<L>::CT)
   static_assert(H % COLS == 0, "column groups");
   static_assert(!BK || L % 16 == 0, "swizzle stays inside the column");
   static_assert(!BK || ct::stage_twiddle_count(P{}) <= L, "per-stage twiddles fit the table slot");
+  // twiddles: each lane's own in registers where the plan allows it (BK), else a table in LDS
+  // (per-stage layout for BK, flat otherwise)
+  constexpr bool LT = BK && ct::lane_twiddles_fit<TPC>(P{});
+  constexpr ct::TwSrc TS = LT ? ct::TwSrc::LANE : (BK ? ct::TwSrc::STAGE : ct::TwSrc::FLAT);
   constexpr long long PLANE = (long long)(H + 1) * L;
-  __shared__ float2 smem[COLS * LP + L];
+  __shared__ float2 smem[COLS * LP + (LT ? 0 : L)];
   float2* const buf = smem;
-  float2* const tw = smem + COLS * LP;  // LDS twiddles
+  float2* const tws = smem + COLS * LP;  // LDS twiddles (unless LT)
   auto slot = [](int y) { return BK ? ct::Swz16{}(y) : y; };
 
   // Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a contiguous range of
@@ -112,11 +116,6 @@ __global__ void __launch_bounds__(CorrCfg<L>::CT)
   if (b >= nb || tl >= tc) return;  // padding of the last tiles (uniform per workgroup)
   const int tid = threadIdx.x;
 
-  if constexpr (BK)
-    ct::fill_stage_twiddles<L>(tw, twL, tid, NT, P{});
-  else
-    for (int i = tid; i < L; i += NT) tw[i] = twL[i];
-  __syncthreads();
   const float2* Tp = that + (long long)(t0 + tl) * PLANE;
   const float2* Ip = ihat + (long long)b * PLANE;
   const int kx0 = cg * COLS;
@@ -150,6 +149,18 @@ __global__ void __launch_bounds__(CorrCfg<L>::CT)
       }
     }
   }
+  // twiddles after the spectra loads, so that their latencies overlap
+  float2 wl[LT ? ct::lane_twiddle_count(P{}) : 1];
+  if constexpr (LT) {
+    ct::load_lane_twiddles<L, TPC>(wl, twL, tid % TPC, P{});
+  } else {
+    if constexpr (BK)
+      ct::fill_stage_twiddles<L>(tws, twL, tid, NT, P{});
+    else
+      for (int i = tid; i < L; i += NT) tws[i] = twL[i];
+    __syncthreads();
+  }
+  const float2* const tw = LT ? wl : tws;
 #pragma unroll
   for (int i = 0; i < NLD; ++i) {
     const int q = lane + 64 * i;
@@ -166,17 +177,25 @@ __global__ void __launch_bounds__(CorrCfg<L>::CT)
   ct::lds_sync<ct::Sync::WAVE>();
   const int col = tid / TPC;
   if constexpr (BK)
-    ct::run_stockham<L, TPC, +1, ct::Sync::WAVE, ct::Swz16, true>(buf + col * LP, tw, tid % TPC, P{});
+    ct::run_stockham<L, TPC, +1, ct::Sync::WAVE, ct::Swz16, TS>(buf + col * LP, tw, tid % TPC, P{});
   else
     ct::run_stockham<L, TPC, +1, ct::Sync::WAVE>(buf + col * LP, tw, tid % TPC, P{});
   __syncthreads();  // rows of Z gather all COLS columns of the workgroup
 
-  float2* Zp = Z + ((long long)tl * nb + b) * (long long)L * H;
-  for (int q = tid; q < COLS * L; q += NT) {
-    const int y = q / COLS, cc = q - y * COLS;
-    int py = y + L / 2;
-    if (py >= L) py -= L;
-    st_stream(Zp + (long long)py * H + kx0 + cc, buf[cc * LP + slot(y)]);
+  // write-out: lane (y0, cc) moves rows y0, y0 + RPI, ... of column cc; every LDS read is issued
+  // before the first store
+  static_assert(NT % COLS == 0, "write-out mapping");
+  constexpr int RPI = NT / COLS, NWO = (L + RPI - 1) / RPI;
+  const int cc = tid % COLS, y0 = tid / COLS;
+  float2* const Zp = Z + ((long long)tl * nb + b) * (long long)L * H + kx0 + cc;
+  float2 ov[NWO];
+#pragma unroll
+  for (int i = 0; i < NWO; ++i)
+    if (L % RPI == 0 || y0 + i * RPI < L) ov[i] = buf[cc * LP + slot(y0 + i * RPI)];
+#pragma unroll
+  for (int i = 0; i < NWO; ++i) {
+    const int y = y0 + i * RPI;
+    if (L % RPI == 0 || y < L) st_stream(Zp + (long long)(y < L / 2 ? y + L / 2 : y - L / 2) * H, ov[i]);
   }
 }
 
@@ -474,9 +493,7 @@ __global__ void __launch_bounds__(256)
   const int b = blockIdx.y;
   const int y0 = blockIdx.x * ROWS + wave * RW;
   float2* wb = smem + wave * TPW * HS;
-  const int tr = lane / TPT, sub = lane % TPT;
-  const int rowl = tr / TPI, ti = tr % TPI;
-  float2* dst = wb + tr * HS;
+  const int ti = (lane / TPT) % TPI;  // template of this lane's transform within an iteration
 
   float s[PPL], ss[PPL], bs[PPL];
   int ag[PPL];
@@ -492,7 +509,18 @@ __global__ void __launch_bounds__(256)
   // transformed and folded (software pipelining of the only HBM stream of this kernel).
   constexpr int NP = H / 2 / TPT;
   float2 pa[NP], pb[NP], p0 = make_float2(0.f, 0.f);
+  // Lane-dependent offsets are derived from an opaque copy of the lane index inside the loop:
+  // hoisted out of it, the swizzled LDS addresses and the Z offsets of every unrolled access
+  // would each stay live in a register across the loop (about 70 registers), halving the
+  // number of resident wavefronts.
+  auto opaque_lane = [&]() {
+    int ln = lane;
+    asm volatile("" : "+v"(ln));
+    return ln & 63;  // restores the known range (non-negative, below 64) for the index arithmetic
+  };
   auto prefetch = [&](int t) {
+    const int lnp = opaque_lane();
+    const int sub = lnp % TPT, ti = (lnp / TPT) % TPI, rowl = lnp / TPT / TPI;
     if (t + ti < tc) {
       const float2* zr = Z + ((((long long)(t + ti) * nb + b) * L) + y0 + rowl) * H;
       if (sub == 0) p0 = BK ? ld_stream(zr) : zr[0];
@@ -507,6 +535,9 @@ __global__ void __launch_bounds__(256)
   if constexpr (PF) prefetch(0);
   for (int t = 0; t < tc; t += TPI) {
     if constexpr (!PF) prefetch(t);
+    const int ln = opaque_lane();
+    const int sub = ln % TPT;
+    float2* const dst = wb + (ln / TPT) * HS;
     if (t + ti < tc) {
       if (sub == 0) dst[0] = make_float2(p0.x + p0.y, p0.x - p0.y);  // X(0), X(H) real, slot 0
 #pragma unroll
@@ -530,13 +561,13 @@ __global__ void __launch_bounds__(256)
     if constexpr (PF) prefetch(t + TPI);
     ct::lds_sync<ct::Sync::WAVE>();
     if constexpr (BK)
-      ct::run_stockham<H, TPT, +1, ct::Sync::WAVE, ct::Swz8, true>(dst, twf, sub, typename C::RowPlan{});
+      ct::run_stockham<H, TPT, +1, ct::Sync::WAVE, ct::Swz8, ct::TwSrc::STAGE>(dst, twf, sub, typename C::RowPlan{});
     else
       ct::run_stockham<H, TPT, +1, ct::Sync::WAVE>(dst, twf, sub, typename C::RowPlan{});
     const float* out = reinterpret_cast<const float*>(wb);
 #pragma unroll
     for (int i = 0; i < PPL; ++i) {
-      const int q = lane + 64 * i;
+      const int q = ln + 64 * i;
       const int r2 = q / L, x = q - r2 * L;
       const int xo = 2 * map(x >> 1) + (x & 1);  // real output x sits in complex slot x / 2
 #pragma unroll
@@ -849,8 +880,8 @@ bool launch_rowpass_pixel(int L, const float2* Z, int tc, int nb, double* S, dou
   TwTables& T = tw_tables(current_device());
   if (L == 320) {
     // variants 0/1: 2 rows x 4 templates per wave iteration without/with prefetch; 2/3: 1 row x
-    // 8 templates with/without prefetch; 4/5: as 1/0 with the bank-conflict-free LDS layout
-    const int tpi = (variant == 2 || variant == 3) ? 8 : CorrCfg<320>::TPI;
+    // 8 templates with/without prefetch; 4/5/6: as 1/0/2 with the bank-conflict-free LDS layout
+    const int tpi = (variant == 2 || variant == 3 || variant == 6) ? 8 : CorrCfg<320>::TPI;
     const int rows = 4 * (64 / CorrCfg<320>::TPT) / tpi;
     dim3 grid(320 / rows, nb);
     if (variant == 1)
@@ -863,6 +894,8 @@ bool launch_rowpass_pixel(int L, const float2* Z, int tc, int nb, double* S, dou
       k_rowpass_pix_wave<320, true, 4, true><<<grid, 256, 0, s>>>(Z, T.t320, T.t160, tc, nb, S, SS, best, arg, arg_base);
     else if (variant == 5)
       k_rowpass_pix_wave<320, false, 4, true><<<grid, 256, 0, s>>>(Z, T.t320, T.t160, tc, nb, S, SS, best, arg, arg_base);
+    else if (variant == 6)
+      k_rowpass_pix_wave<320, true, 8, true><<<grid, 256, 0, s>>>(Z, T.t320, T.t160, tc, nb, S, SS, best, arg, arg_base);
     else
       k_rowpass_pix_wave<320, false><<<grid, 256, 0, s>>>(Z, T.t320, T.t160, tc, nb, S, SS, best, arg, arg_base);
     GS_CHECK_LAUNCH();

@@ -461,7 +461,7 @@ class Engine {
   }
   void set_chunk(int tc) { chunk_ = tc; }
   void set_variant(int kernel, int variant) {
-    GS_REQUIRE((kernel == 0 && variant >= 0 && variant <= 5) || (kernel == 1 && variant >= 0 && variant <= 5) ||
+    GS_REQUIRE((kernel == 0 && variant >= 0 && variant <= 5) || (kernel == 1 && variant >= 0 && variant <= 6) ||
                    (kernel == 2 && (variant == 0 || variant == 1)),
                kInvalid, "unknown kernel variant");
     if (kernel == 0)
@@ -505,7 +505,8 @@ class Engine {
     };
     try {
       // candidates: the natural and the bank-conflict-free LDS layouts of the column pass (radix
-      // 4*4*20), then the row pass with and without that layout (with prefetch); the other
+      // 4*4*20), then the row pass with that layout (one row x eight templates per wave
+      // iteration) and without it (two rows x four templates), both prefetched; the other
       // variants measured slower on MI355X at every size tried.  A later candidate replaces the
       // current pick only when more than 3 % faster, so that timing noise between near-equal
       // options does not change the configuration from run to run.
@@ -513,7 +514,7 @@ class Engine {
       double best = 0;
       int pick = col_variant_;
       bool first = true;
-      for (int v : {4, 3}) {
+      for (int v : {3, 4}) {
         col_variant_ = v;
         const double ms = measure(0);
         if (first || ms < kMargin * best) best = ms, pick = v;
@@ -522,7 +523,7 @@ class Engine {
       col_variant_ = pick;
       pick = row_variant_;
       first = true;
-      for (int v : {4, 1}) {
+      for (int v : {6, 1}) {
         row_variant_ = v;
         const double ms = measure(1);
         if (first || ms < kMargin * best) best = ms, pick = v;
@@ -1228,7 +1229,7 @@ class Engine {
   std::vector<int> offx_, offy_, ulx_, uly_;
   std::vector<float> angles_;
   // kernel variants default to the fastest measured on the production shape (l = 320)
-  int lim_angles_ = 0, lim_templates_ = 0, chunk_ = 0, row_variant_ = 4, col_variant_ = 4;
+  int lim_angles_ = 0, lim_templates_ = 0, chunk_ = 0, row_variant_ = 6, col_variant_ = 3;
   double total_ms_ = 0, corr_ms_ = 0;
   std::vector<gs_record> records_;
   std::vector<double> planes_host_;

@@ -329,11 +329,54 @@ __device__ __forceinline__ void fill_stage_twiddles(float2* dst, const float2* f
   }
 }
 
+// Twiddle sources of a Stockham transform.
+//  FLAT:  tw[e] = exp(-2 pi i e / N), the flat table (global memory or an LDS copy).
+//  STAGE: the per-stage layout of fill_stage_twiddles.
+//  LANE:  the lane's own twiddles in registers (load_lane_twiddles): every stage with NS > 1
+//         needs T % NS == 0, so that a lane meets a single jm = tid % NS in that stage.
+enum class TwSrc { FLAT, STAGE, LANE };
+
+template <int T, int... RS>
+constexpr bool lane_twiddles_fit(RadixList<RS...>) {
+  int ns = 1;
+  for (int r : {RS...}) {
+    if (ns > 1 && T % ns != 0) return false;
+    ns *= r;
+  }
+  return true;
+}
+template <int... RS>
+constexpr int lane_twiddle_count(RadixList<RS...>) {
+  int ns = 1, n = 0;
+  for (int r : {RS...}) {
+    if (ns > 1) n += r - 1;
+    ns *= r;
+  }
+  return n;
+}
+
+template <int N, int T, int NS, int OFF, int R, int... REST>
+__device__ __forceinline__ void load_lane_twiddles_impl(float2* w, const float2* flat, int tid) {
+  if constexpr (NS > 1) {
+    static_assert(T % NS == 0, "lane twiddles need one jm per lane and stage");
+    const int jm = tid % NS;
+#pragma unroll
+    for (int r = 1; r < R; ++r) w[OFF + r - 1] = flat[r * jm * (N / (NS * R))];
+  }
+  if constexpr (sizeof...(REST) > 0)
+    load_lane_twiddles_impl<N, T, NS * R, OFF + (NS > 1 ? R - 1 : 0), REST...>(w, flat, tid);
+}
+// Loads the twiddles lane tid (of T per transform) multiplies by in every stage, in stage
+// order, into w[lane_twiddle_count(plan)] (a register array once inlined).
+template <int N, int T, int... RS>
+__device__ __forceinline__ void load_lane_twiddles(float2* w, const float2* flat, int tid, RadixList<RS...>) {
+  load_lane_twiddles_impl<N, T, 1, 0, RS...>(w, flat, tid);
+}
+
 // One Stockham stage (radix R, NS = product of earlier radices) over a length-N transform in
-// buf, shared by T threads (tid in [0,T)); then recurse into the remaining radices.  OFF < 0:
-// tw is the flat table; OFF >= 0: tw is the per-stage layout and this stage's entries start at
-// OFF.
-template <int N, int T, int SIGN, int NS, Sync S, typename M, int OFF, int R, int... REST>
+// buf, shared by T threads (tid in [0,T)); then recurse into the remaining radices.  OFF is the
+// offset of this stage's entries in the STAGE or LANE twiddle source.
+template <int N, int T, int SIGN, int NS, Sync S, typename M, TwSrc TS, int OFF, int R, int... REST>
 __device__ __forceinline__ void stockham(float2* buf, const float2* tw, int tid, M map) {
   constexpr int NB = N / R;
   constexpr int IT = (NB + T - 1) / T;
@@ -349,7 +392,9 @@ __device__ __forceinline__ void stockham(float2* buf, const float2* tw, int tid,
         float2 a = buf[map(j + r * NB)];
         if constexpr (NS > 1) {
           if (r > 0) {
-            if constexpr (OFF >= 0)
+            if constexpr (TS == TwSrc::LANE)
+              a = c_mul(a, tw_lookup<SIGN>(tw, OFF + r - 1));
+            else if constexpr (TS == TwSrc::STAGE)
               a = c_mul(a, tw_lookup<SIGN>(tw, OFF + (r - 1) * NS + jm));
             else
               a = c_mul(a, tw_lookup<SIGN>(tw, r * jm * (N / (NS * R))));
@@ -372,18 +417,19 @@ __device__ __forceinline__ void stockham(float2* buf, const float2* tw, int tid,
     }
   }
   lds_sync<S>();
-  constexpr int OFF2 = OFF < 0 ? OFF : OFF + (NS > 1 ? (R - 1) * NS : 0);
-  if constexpr (sizeof...(REST) > 0) stockham<N, T, SIGN, NS * R, S, M, OFF2, REST...>(buf, tw, tid, map);
+  constexpr int OFF2 = OFF + (NS > 1 ? (TS == TwSrc::LANE ? R - 1 : (R - 1) * NS) : 0);
+  if constexpr (sizeof...(REST) > 0)
+    stockham<N, T, SIGN, NS * R, S, M, TS, OFF2, REST...>(buf, tw, tid, map);
 }
 
-// ST: tw holds the per-stage layout (fill_stage_twiddles) instead of the flat table.
-template <int N, int T, int SIGN, Sync S = Sync::BLOCK, typename M = IdentityMap, bool ST = false,
-          int... RS>
+template <int N, int T, int SIGN, Sync S = Sync::BLOCK, typename M = IdentityMap,
+          TwSrc TS = TwSrc::FLAT, int... RS>
 __device__ __forceinline__ void run_stockham(float2* buf, const float2* tw, int tid,
                                              RadixList<RS...>, M map = M{}) {
   static_assert(RadixList<RS...>::product() == N, "radix plan must multiply to N");
   static_assert(S == Sync::BLOCK || (64 % T == 0), "wave-level sync needs transforms inside one wavefront");
-  stockham<N, T, SIGN, 1, S, M, ST ? 0 : -1, RS...>(buf, tw, tid, map);
+  static_assert(TS != TwSrc::LANE || lane_twiddles_fit<T>(RadixList<RS...>{}), "lane twiddles need T % NS == 0");
+  stockham<N, T, SIGN, 1, S, M, TS, 0, RS...>(buf, tw, tid, map);
 }
 
 }  // namespace ct
