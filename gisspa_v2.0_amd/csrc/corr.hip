@@ -64,35 +64,39 @@ struct ColPlanOf<320, 1> {
   using type = ct::RadixList<4, 4, 20>;
 };
 
-// Column LDS layouts:
-//  BK = false: column stride L + 1, slots in natural order, flat twiddle table.
-//  BK = true:  column stride L + 17, slots permuted by the radix-4 bank swizzle Swz16 and
-//              twiddles in the per-stage layout: with 16 lanes per column every Stockham read
-//              and write, every twiddle read and the row-wise gather of the write-out touch 16
-//              distinct bank pairs.
+// Column LDS layouts (LAY):
+//  0: column stride L + 1, slots in natural order, flat twiddle table in LDS.
+//  1: column stride L + 17, slots permuted by the radix-4 bank swizzle Swz16 and twiddles per
+//     lane in registers (or in the per-stage layout where the plan does not allow that): with 16
+//     lanes per column every Stockham read and write, every twiddle read and the row-wise gather
+//     of the write-out touch 16 distinct bank pairs.
+//  2: compact: columns unpadded, element y of column c at slot y ^ (c & 15) (adjacent columns on
+//     different banks for the write-out gather), twiddles read from the global table: exactly
+//     40 KB at L = 320, so that four workgroups fit a CU.
 // Z is written with streaming stores: it is read back only by the next kernel, and keeping it
 // out of L2 keeps the spectra columns that neighbouring workgroups are still reading there.
-template <int L, int Plan = 0, bool BK = false>
+template <int L, int Plan = 0, int LAY = 0>
 __global__ void __launch_bounds__(CorrCfg<L>::CT)
     k_colpass(const float2* __restrict__ that, const float2* __restrict__ ihat,
               const float2* __restrict__ twL, float2* __restrict__ Z, int t0, int tc, int nb) {
   using C = CorrCfg<L>;
   using P = typename ColPlanOf<L, Plan>::type;
+  constexpr bool BK = LAY == 1, CM = LAY == 2;
   constexpr int H = C::H, COLS = C::COLS, NT = C::CT, TPC = C::TPC;
-  constexpr int LP = BK ? L + 17 : L + 1;
+  constexpr int LP = BK ? L + 17 : (CM ? L : L + 1);
   static_assert(COLS * TPC == NT, "column pass thread mapping");
   static_assert(H % COLS == 0, "column groups");
-  static_assert(!BK || L % 16 == 0, "swizzle stays inside the column");
+  static_assert(LAY == 0 || L % 16 == 0, "swizzle stays inside the column");
   static_assert(!BK || ct::stage_twiddle_count(P{}) <= L, "per-stage twiddles fit the table slot");
   // twiddles: each lane's own in registers where the plan allows it (BK), else a table in LDS
-  // (per-stage layout for BK, flat otherwise)
+  // (per-stage layout for BK, flat otherwise), or the global table (CM)
   constexpr bool LT = BK && ct::lane_twiddles_fit<TPC>(P{});
   constexpr ct::TwSrc TS = LT ? ct::TwSrc::LANE : (BK ? ct::TwSrc::STAGE : ct::TwSrc::FLAT);
   constexpr long long PLANE = (long long)(H + 1) * L;
-  __shared__ float2 smem[COLS * LP + (LT ? 0 : L)];
+  __shared__ float2 smem[COLS * LP + (LT || CM ? 0 : L)];
   float2* const buf = smem;
-  float2* const tws = smem + COLS * LP;  // LDS twiddles (unless LT)
-  auto slot = [](int y) { return BK ? ct::Swz16{}(y) : y; };
+  float2* const tws = smem + COLS * LP;  // LDS twiddles (unless LT or CM)
+  auto slot = [](int c, int y) { return BK ? ct::Swz16{}(y) : (CM ? y ^ (c & 15) : y); };
 
   // Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a contiguous range of
   // logical work.  Logical order tiles the (block, template) product GEMM-style per column
@@ -153,14 +157,14 @@ __global__ void __launch_bounds__(CorrCfg<L>::CT)
   float2 wl[LT ? ct::lane_twiddle_count(P{}) : 1];
   if constexpr (LT) {
     ct::load_lane_twiddles<L, TPC>(wl, twL, tid % TPC, P{});
-  } else {
+  } else if constexpr (!CM) {
     if constexpr (BK)
       ct::fill_stage_twiddles<L>(tws, twL, tid, NT, P{});
     else
       for (int i = tid; i < L; i += NT) tws[i] = twL[i];
     __syncthreads();
   }
-  const float2* const tw = LT ? wl : tws;
+  const float2* const tw = LT ? wl : (CM ? twL : tws);
 #pragma unroll
   for (int i = 0; i < NLD; ++i) {
     const int q = lane + 64 * i;
@@ -171,13 +175,15 @@ __global__ void __launch_bounds__(CorrCfg<L>::CT)
         const float2 pn = c_mulconj(in_[i < NDC ? i : 0], tn[i < NDC ? i : 0]);
         p = make_float2(p.x - pn.y, p.y + pn.x);
       }
-      buf[(cw0 + cl) * LP + slot(ky)] = p;
+      buf[(cw0 + cl) * LP + slot(cw0 + cl, ky)] = p;
     }
   }
   ct::lds_sync<ct::Sync::WAVE>();
   const int col = tid / TPC;
   if constexpr (BK)
     ct::run_stockham<L, TPC, +1, ct::Sync::WAVE, ct::Swz16, TS>(buf + col * LP, tw, tid % TPC, P{});
+  else if constexpr (CM)
+    ct::run_stockham<L, TPC, +1, ct::Sync::WAVE>(buf + col * LP, tw, tid % TPC, P{}, ct::XorMap{col & 15});
   else
     ct::run_stockham<L, TPC, +1, ct::Sync::WAVE>(buf + col * LP, tw, tid % TPC, P{});
   __syncthreads();  // rows of Z gather all COLS columns of the workgroup
@@ -191,7 +197,7 @@ __global__ void __launch_bounds__(CorrCfg<L>::CT)
   float2 ov[NWO];
 #pragma unroll
   for (int i = 0; i < NWO; ++i)
-    if (L % RPI == 0 || y0 + i * RPI < L) ov[i] = buf[cc * LP + slot(y0 + i * RPI)];
+    if (L % RPI == 0 || y0 + i * RPI < L) ov[i] = buf[cc * LP + slot(cc, y0 + i * RPI)];
 #pragma unroll
   for (int i = 0; i < NWO; ++i) {
     const int y = y0 + i * RPI;
@@ -836,9 +842,11 @@ static void colpass_impl(const CorrArgs& a, const float2* twL, int variant, hipS
     if (variant == 3)
       k_colpass<L, 1><<<(unsigned)G, C::CT, 0, s>>>(a.that, a.ihat, twL, a.Z, a.t0, a.tc, a.nb);
     else if (variant == 4)
-      k_colpass<L, 1, true><<<(unsigned)G, C::CT, 0, s>>>(a.that, a.ihat, twL, a.Z, a.t0, a.tc, a.nb);
+      k_colpass<L, 1, 1><<<(unsigned)G, C::CT, 0, s>>>(a.that, a.ihat, twL, a.Z, a.t0, a.tc, a.nb);
     else if (variant == 5)
-      k_colpass<L, 0, true><<<(unsigned)G, C::CT, 0, s>>>(a.that, a.ihat, twL, a.Z, a.t0, a.tc, a.nb);
+      k_colpass<L, 0, 1><<<(unsigned)G, C::CT, 0, s>>>(a.that, a.ihat, twL, a.Z, a.t0, a.tc, a.nb);
+    else if (variant == 6)
+      k_colpass<L, 1, 2><<<(unsigned)G, C::CT, 0, s>>>(a.that, a.ihat, twL, a.Z, a.t0, a.tc, a.nb);
     else
       k_colpass<L, 0><<<(unsigned)G, C::CT, 0, s>>>(a.that, a.ihat, twL, a.Z, a.t0, a.tc, a.nb);
   } else {

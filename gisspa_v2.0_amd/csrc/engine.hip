@@ -461,7 +461,7 @@ class Engine {
   }
   void set_chunk(int tc) { chunk_ = tc; }
   void set_variant(int kernel, int variant) {
-    GS_REQUIRE((kernel == 0 && variant >= 0 && variant <= 5) || (kernel == 1 && variant >= 0 && variant <= 6) ||
+    GS_REQUIRE((kernel == 0 && variant >= 0 && variant <= 6) || (kernel == 1 && variant >= 0 && variant <= 6) ||
                    (kernel == 2 && (variant == 0 || variant == 1)),
                kInvalid, "unknown kernel variant");
     if (kernel == 0)
@@ -953,11 +953,12 @@ class Engine {
       rs = stream2_;
     }
     float2* Zb = d_Z_.ensure(zsz * (overlap ? 2 : 1));
-    float* rot = d_rot_.ensure((size_t)n * L2);
-    double* part = d_part_.ensure((size_t)n * l * 2);
+    double* part = d_part_.ensure((size_t)n * rotation_groups(l) * 2);
     float* stats = d_stats_.ensure((size_t)n * 2);
     float2* that = d_that_.ensure((size_t)n * hx * l);
-    float2* work = d_work_.ensure((size_t)n * L2);
+    // template pairs share one complex plane per angle (real and imaginary parts)
+    float2* work = d_work_.ensure((size_t)((n + 1) / 2) * L2);
+    float2* tscr = d_scratch_.ensure((size_t)((n + 1) / 2) * L2);
     GS_HIP(hipEventRecord(ev_corr0_, stream_));
     if (overlap) {  // accumulators initialised on stream_ before the first row pass
       GS_HIP(hipEventRecord(ev_sync_, stream_));
@@ -969,10 +970,12 @@ class Engine {
         // RotateTemplate (src/norm.cu:479-521)
         const float cose = (float)std::cos((double)angles[a] * kRefPi / 180);
         const float sine = (float)std::sin((double)angles[a] * kRefPi / 180);
-        launch_rotate_mask(d_tprep_.p, nt, l, cose, sine, P_.diameter, 4.0f, rot, part, stream_);
+        launch_rotate_mask_pairs(d_tprep_.p, nt, l, cose, sine, P_.diameter, 4.0f, work, part, stream_);
         launch_rotation_stats(part, nt, l, stats, stream_);
-        launch_standardise_to_complex(rot, nt, l, stats, work, stream_);
-        half_spectra(work, nt, that, (float)(1.0 / (double)L2));
+        // one forward transform per pair; the split, the standardisation (affine: applied to the
+        // spectra) and the 1/l^2 scale follow in one pass that writes the half spectra
+        fft2_planes(work, tscr, (nt + 1) / 2, -1, 1.f);
+        launch_unpack_pair_spectra(work, nt, l, stats, (float)(1.0 / (double)L2), that, stream_);
         // correlation + per-pixel statistics (ComputeCCGSum / PickParticles in one pass)
         for (int t0 = 0; t0 < nt; t0 += tc_max, ++chunk) {
           const int tc = std::min(tc_max, nt - t0);
@@ -1000,15 +1003,14 @@ class Engine {
       // and buffer set, then replay it for every later micrograph of that shape.
       auto ptr = [](const void* p) { return (long long)(uintptr_t)p; };
       const std::vector<long long> key{
-          nb_, n, nt, na, tc_max, l, col_variant_, row_variant_, ptr(d_tprep_.p), ptr(rot),
-          ptr(part), ptr(stats), ptr(work), ptr(that), ptr(d_scratch_.p), ptr(ihat), ptr(Zb),
+          nb_, n, nt, na, tc_max, l, col_variant_, row_variant_, ptr(d_tprep_.p), ptr(tscr),
+          ptr(part), ptr(stats), ptr(work), ptr(that), ptr(ihat), ptr(Zb),
           ptr(d_S_.p), ptr(d_SS_.p), ptr(d_best_.p), ptr(d_arg_.p)};
       auto it = graphs_.find(key);
       if (it == graphs_.end()) {
         // nothing may allocate while the stream is captured: the scratch buffer of the template
-        // transforms already holds n planes, the transform plan for l exists (template
-        // preparation made it), and the correlation twiddle tables are built here
-        GS_REQUIRE(d_scratch_.n >= (size_t)nt * L2, kInvalid, "scratch buffer too small for capture");
+        // transforms was sized above, the transform plan for l exists (template preparation made
+        // it), and the correlation twiddle tables are built here
         (void)fft_.plan(l);
         corr_prepare();
         if (graphs_.size() >= kMaxGraphs) drop_graphs();
@@ -1249,7 +1251,7 @@ class Engine {
   PinnedBuf pinned_[2];
   std::map<long long, std::unique_ptr<DevBuf<double>>> rb_cache_;
 
-  DevBuf<float> d_raw_, d_img_, d_imgpre_, d_tprep_, d_rot_, d_stats_, d_best_, d_tile_;
+  DevBuf<float> d_raw_, d_img_, d_imgpre_, d_tprep_, d_stats_, d_best_, d_tile_;
   DevBuf<float2> d_work_, d_work2_, d_scratch_, d_frame_, d_frame2_, d_blocks_, d_ihat_, d_that_, d_Z_, d_cs_;
   DevBuf<double> d_mom_, d_ra_, d_ra_img_, d_m_, d_part_, d_S_, d_SS_, d_mu_, d_sigma_, d_planes_, d_chpart_;
   DevBuf<float> d_plane_part_;

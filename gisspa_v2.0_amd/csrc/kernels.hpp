@@ -62,11 +62,15 @@ void launch_split_blocks(const float* img, int nx, int ny, int l, const int* off
                          int bx, int nb, float2* out, hipStream_t s);
 
 // ---- per-angle rotation (src/norm.cu:479-521, src/nonorm.cu:457) ----
-void launch_rotate_mask(const float* tp, int n, int l, float cose, float sine, float d_m, float ehw,
-                        float* out, double* partial, hipStream_t s);
+// Template pairs (2p, 2p+1) rotated and masked into the real and imaginary parts of out[p]
+// ((n + 1) / 2 complex l x l planes); partial holds n x rotation_groups(l) x 2 double sums.
+int rotation_groups(int l);
+void launch_rotate_mask_pairs(const float* tp, int n, int l, float cose, float sine, float d_m, float ehw,
+                              float2* out, double* partial, hipStream_t s);
 void launch_rotation_stats(const double* partial, int n, int l, float* stats, hipStream_t s);
-void launch_standardise_to_complex(const float* R, int n, int l, const float* stats, float2* out,
-                                   hipStream_t s);
+// Half spectra out[t][kx 0..l/2][ky] (standardised, times scale) of the pair transforms Z.
+void launch_unpack_pair_spectra(const float2* Z, int n, int l, const float* stats, float scale, float2* out,
+                                hipStream_t s);
 void launch_rotate_complex(const float2* in, int n, int l, float cose, float sine, float2* out,
                            hipStream_t s);
 // na angles at once: out[a][t] = rotation of in[t] by cs[a] = (cos, sin) (device array).

@@ -366,43 +366,71 @@ using dev::Bil;
 using dev::bil_coords;
 
 // Rotate + mask one template row per block; per-row double partial sums for mean/var.
-__global__ void k_rotate_mask(const float* __restrict__ tp, int l, float cose, float sine, float rin,
-                              float rout, float ehw, float* __restrict__ out, double* __restrict__ partial) {
-  const int j = blockIdx.x, t = blockIdx.y;
-  const float* P = tp + (long long)t * l * l;
-  double s = 0, s2 = 0;
-  for (int i = threadIdx.x; i < l; i += blockDim.x) {
-    float res = 0.f;
-    const Bil b = bil_coords(i, j, l, cose, sine);
-    if (b.ok) {
-      const float tt = 1.0f - b.t, uu = 1.0f - b.u;
-      res = P[b.k0] * tt * uu + P[b.k1] * b.t * uu + P[b.k2] * b.t * b.u + P[b.k3] * tt * b.u;
+constexpr int kRotRows = 8;  // rows per workgroup of the template rotation
+
+// RotateTemplate (src/norm.cu:479-521) for the template pair (2p, 2p+1): rotate_subIMG +
+// apply_mask (k:534-603, 225-241) of both templates into one complex plane (template 2p in the
+// real part, 2p+1 in the imaginary part; their joint forward transform gives both spectra),
+// plus compute_sum_sqr's double sums (k:384-443) per template and group of kRotRows rows.  The
+// sample geometry and the mask factor are computed once for both templates.
+__global__ void k_rotate_mask_pair(const float* __restrict__ tp, int n, int l, float cose, float sine,
+                                   float rin, float rout, float ehw, float2* __restrict__ out,
+                                   double* __restrict__ partial) {
+  const int g = blockIdx.x, p = blockIdx.y, groups = gridDim.x;
+  const int ta = 2 * p, tb = 2 * p + 1;
+  const bool hb = tb < n;
+  const float* A = tp + (long long)ta * l * l;
+  const float* B = tp + (long long)(hb ? tb : ta) * l * l;
+  const int j0 = g * kRotRows, cnt = (min(l, j0 + kRotRows) - j0) * l;
+  double sa = 0, sa2 = 0, sb = 0, sb2 = 0;
+  for (int e = threadIdx.x; e < cnt; e += blockDim.x) {
+    const int j = j0 + e / l, i = e % l;
+    float ra = 0.f, rb = 0.f;
+    const Bil c = bil_coords(i, j, l, cose, sine);
+    if (c.ok) {
+      const float tt = 1.0f - c.t, uu = 1.0f - c.u;
+      ra = A[c.k0] * tt * uu + A[c.k1] * c.t * uu + A[c.k2] * c.t * c.u + A[c.k3] * tt * c.u;
+      if (hb) rb = B[c.k0] * tt * uu + B[c.k1] * c.t * uu + B[c.k2] * c.t * c.u + B[c.k3] * tt * c.u;
     }
     const float r = hypotf((float)(i - l / 2), (float)(j - l / 2));
-    if (r > rout)
-      res = 0.f;
-    else if (r >= rin)
-      res *= 0.5f * cosf((float)kRefPi * (r - rin) / (2.0f * ehw)) + 0.5f;
-    out[((long long)t * l + j) * l + i] = res;
-    s += res;
-    s2 += (double)res * res;
+    if (r > rout) {
+      ra = 0.f;
+      rb = 0.f;
+    } else if (r >= rin) {
+      const float f = 0.5f * cosf((float)kRefPi * (r - rin) / (2.0f * ehw)) + 0.5f;
+      ra *= f;
+      rb *= f;
+    }
+    out[((long long)p * l + j) * l + i] = make_float2(ra, rb);
+    sa += ra;
+    sa2 += (double)ra * ra;
+    sb += rb;
+    sb2 += (double)rb * rb;
   }
-  s = block_sum(s);
-  s2 = block_sum(s2);
+  sa = block_sum(sa);
+  sa2 = block_sum(sa2);
+  sb = block_sum(sb);
+  sb2 = block_sum(sb2);
   if (threadIdx.x == 0) {
-    partial[((long long)t * l + j) * 2 + 0] = s;
-    partial[((long long)t * l + j) * 2 + 1] = s2;
+    partial[((long long)ta * groups + g) * 2 + 0] = sa;
+    partial[((long long)ta * groups + g) * 2 + 1] = sa2;
+    if (hb) {
+      partial[((long long)tb * groups + g) * 2 + 0] = sb;
+      partial[((long long)tb * groups + g) * 2 + 1] = sb2;
+    }
   }
 }
 
-// Host reduction of RotateTemplate (src/norm.cu:498-507): mean and E[x^2]-mean^2 per template.
-__global__ void k_rotation_stats(const double* __restrict__ partial, int n, int l, float* __restrict__ stats) {
+// Host reduction of RotateTemplate (src/norm.cu:498-507): mean and E[x^2]-mean^2 per template
+// from its row-group partials.
+__global__ void k_rotation_stats(const double* __restrict__ partial, int n, int groups, int l,
+                                 float* __restrict__ stats) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   double s = 0, s2 = 0;
-  for (int j = 0; j < l; ++j) {
-    s += partial[((long long)t * l + j) * 2];
-    s2 += partial[((long long)t * l + j) * 2 + 1];
+  for (int g = 0; g < groups; ++g) {
+    s += partial[((long long)t * groups + g) * 2];
+    s2 += partial[((long long)t * groups + g) * 2 + 1];
   }
   const double L2 = (double)l * l;
   const double mean = s / L2;
@@ -410,18 +438,57 @@ __global__ void k_rotation_stats(const double* __restrict__ partial, int n, int 
   stats[2 * t + 1] = (float)(s2 / L2 - mean * mean);
 }
 
-// substract_by_mean + divided_by_var (k:460-477) into a complex plane.
-__global__ void k_standardise_to_complex(const float* __restrict__ R, int n, int l, const float* __restrict__ stats,
-                                         float2* __restrict__ out) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long long L2 = (long long)l * l;
-  if (i >= L2 * n) return;
-  const int t = (int)(i / L2);
-  const float mean = stats[2 * t], var = stats[2 * t + 1];
-  float v = R[i];
-  if (mean != 0) v = v - mean;
-  if (var != 0) v = v / sqrtf(var);
-  out[i] = make_float2(v, 0.f);
+// Splits the joint forward transform Z of a template pair (2p in the real part, 2p+1 in the
+// imaginary part) into the two spectra, A = (Z(k) + conj Z(-k)) / 2 and B = (Z(k) - conj Z(-k))
+// / 2i, applies substract_by_mean + divided_by_var (k:460-477) in the frequency domain (the
+// standardisation is affine: the mean changes only the DC term by mean * l^2, the division
+// scales every term) and the scale of the reference's forward transform (src/norm.cu:516-519),
+// and writes the column-major half spectra out[t][kx 0..l/2][ky].  One workgroup per 32 x 32
+// tile of (ky, kx) and pair; both tiles (k and -k) are staged in LDS, read along kx and written
+// along ky.
+__global__ void __launch_bounds__(256)
+    k_unpack_pair_spectra(const float2* __restrict__ Z, int n, int l, const float* __restrict__ stats,
+                          float scale, float2* __restrict__ out) {
+  constexpr int TS = 32;
+  __shared__ float2 zk[TS][TS + 1], zm[TS][TS + 1];
+  const int hx = l / 2 + 1;
+  const int kx0 = blockIdx.x * TS, ky0 = blockIdx.y * TS, p = blockIdx.z;
+  const float2* P = Z + (long long)p * l * l;
+  const int c = threadIdx.x % TS;
+  for (int r = threadIdx.x / TS; r < TS; r += 256 / TS) {
+    const int ky = ky0 + r, kx = kx0 + c;
+    if (ky < l && kx < hx) {
+      zk[r][c] = P[(long long)ky * l + kx];
+      zm[r][c] = P[(long long)(ky ? l - ky : 0) * l + (kx ? l - kx : 0)];
+    }
+  }
+  __syncthreads();
+  const int b = threadIdx.x % TS, ky = ky0 + b;
+  float mean[2], sd[2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int t = min(2 * p + h, n - 1);
+    mean[h] = stats[2 * t];
+    const float var = stats[2 * t + 1];
+    sd[h] = var != 0 ? sqrtf(var) : 1.f;
+  }
+  for (int a = threadIdx.x / TS; a < TS; a += 256 / TS) {
+    const int kx = kx0 + a;
+    if (kx >= hx || ky >= l) continue;
+    const float2 u = zk[b][a], w = zm[b][a];
+    float2 v[2] = {make_float2(0.5f * (u.x + w.x), 0.5f * (u.y - w.y)),
+                   make_float2(0.5f * (u.y + w.y), 0.5f * (w.x - u.x))};
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int t = 2 * p + h;
+      if (t >= n) break;
+      float2 x = v[h];
+      if (kx == 0 && ky == 0 && mean[h] != 0) x.x -= mean[h] * (float)((long long)l * l);
+      x.x = x.x / sd[h] * scale;
+      x.y = x.y / sd[h] * scale;
+      out[((long long)t * hx + kx) * l + ky] = x;
+    }
+  }
 }
 
 // rotate_subIMG (k:534-603) on complex planes (both parts).
@@ -558,20 +625,24 @@ void launch_split_blocks(const float* img, int nx, int ny, int l, const int* off
   k_split_blocks<<<nblk((long long)nb * l * l), BS, 0, s>>>(img, nx, ny, l, offx, offy, bx, nb, out);
   GS_CHECK_LAUNCH();
 }
-void launch_rotate_mask(const float* tp, int n, int l, float cose, float sine, float d_m, float ehw,
-                        float* out, double* partial, hipStream_t s) {
+int rotation_groups(int l) { return (l + kRotRows - 1) / kRotRows; }
+void launch_rotate_mask_pairs(const float* tp, int n, int l, float cose, float sine, float d_m, float ehw,
+                              float2* out, double* partial, hipStream_t s) {
   const float dm = 1.5f * d_m;
   const float rin = dm / 2, rout = dm / 2 + 2 * ehw;
-  k_rotate_mask<<<dim3(l, n), BS, 0, s>>>(tp, l, cose, sine, rin, rout, ehw, out, partial);
+  k_rotate_mask_pair<<<dim3(rotation_groups(l), (n + 1) / 2), BS, 0, s>>>(tp, n, l, cose, sine, rin, rout,
+                                                                          ehw, out, partial);
   GS_CHECK_LAUNCH();
 }
 void launch_rotation_stats(const double* partial, int n, int l, float* stats, hipStream_t s) {
-  k_rotation_stats<<<nblk(n, 64), 64, 0, s>>>(partial, n, l, stats);
+  k_rotation_stats<<<nblk(n, 64), 64, 0, s>>>(partial, n, rotation_groups(l), l, stats);
   GS_CHECK_LAUNCH();
 }
-void launch_standardise_to_complex(const float* R, int n, int l, const float* stats, float2* out,
-                                   hipStream_t s) {
-  k_standardise_to_complex<<<nblk((long long)n * l * l), BS, 0, s>>>(R, n, l, stats, out);
+void launch_unpack_pair_spectra(const float2* Z, int n, int l, const float* stats, float scale, float2* out,
+                                hipStream_t s) {
+  const dim3 grid((unsigned)((l / 2 + 1 + 31) / 32), (unsigned)((l + 31) / 32), (unsigned)((n + 1) / 2));
+  GS_REQUIRE((n + 1) / 2 < 65536, kInvalid, "template count out of range");
+  k_unpack_pair_spectra<<<grid, 256, 0, s>>>(Z, n, l, stats, scale, out);
   GS_CHECK_LAUNCH();
 }
 void launch_rotate_complex(const float2* in, int n, int l, float cose, float sine, float2* out,
