@@ -394,8 +394,9 @@ def main():
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "corr_c1_traffic.json")
     if os.path.exists(tpath) and l == 320:
-        col_name = {0: "k_colpass<320, 0, 0>", 3: "k_colpass<320, 1, 0>", 4: "k_colpass<320, 1, 1>",
-                    5: "k_colpass<320, 0, 1>", 6: "k_colpass<320, 1, 2>"}.get(variant[0], "")
+        col_name = {0: "k_colpass<320, 0, 0, 8, 16>", 3: "k_colpass<320, 1, 0, 8, 16>",
+                    4: "k_colpass<320, 1, 1, 8, 16>", 5: "k_colpass<320, 0, 1, 8, 16>",
+                    6: "k_colpass<320, 1, 2, 8, 16>"}.get(variant[0], "")
         row_name = "k_rowpass_pix_wave<320, %s>" % {
             0: "false, 4, false", 1: "true, 4, false", 2: "true, 8, false", 3: "false, 8, false",
             4: "true, 4, true", 5: "false, 4, true", 6: "true, 8, true"}.get(variant[1], "")

@@ -75,7 +75,10 @@ struct ColPlanOf<320, 1> {
 //     40 KB at L = 320, so that four workgroups fit a CU.
 // Z is written with streaming stores: it is read back only by the next kernel, and keeping it
 // out of L2 keeps the spectra columns that neighbouring workgroups are still reading there.
-template <int L, int Plan = 0, int LAY = 0>
+// BT x TT: tile of the logical order (blocks x templates).  At 512 templates per launch, tiles
+// of 32 x 16 or 32 x 32 fetch 1.8 GB per launch from beyond L2 instead of 6.4 GB but ran 4 %
+// slower on MI355X; 16 x 16 and 16 x 32 ran as 8 x 16.
+template <int L, int Plan = 0, int LAY = 0, int BT = 8, int TT = 16>
 __global__ void __launch_bounds__(CorrCfg<L>::CT)
     k_colpass(const float2* __restrict__ that, const float2* __restrict__ ihat,
               const float2* __restrict__ twL, float2* __restrict__ Z, int t0, int tc, int nb) {
@@ -102,7 +105,6 @@ __global__ void __launch_bounds__(CorrCfg<L>::CT)
   // logical work.  Logical order tiles the (block, template) product GEMM-style per column
   // group: a tile of BT blocks x TT templates touches BT + TT column groups of spectra (about
   // 1 MB), which its XCD's L2 keeps while the tile's BT*TT planes are produced.
-  constexpr int BT = 8, TT = 16;
   const unsigned G = gridDim.x, bid = blockIdx.x;
   const unsigned xcd = bid % 8u, idx = bid / 8u, per = G / 8u, rem = G % 8u;
   unsigned logical = (xcd < rem ? xcd * (per + 1u) : rem * (per + 1u) + (xcd - rem) * per) + idx;
@@ -831,24 +833,25 @@ int rowpass_groups(int L) {
   return 0;
 }
 
+template <int L, int Plan, int LAY, int BT, int TT>
+static void colpass_tiled(const CorrArgs& a, const float2* twL, hipStream_t s) {
+  using C = CorrCfg<L>;
+  const long long G = (long long)(C::H / C::COLS) * ((a.tc + TT - 1) / TT * TT) * ((a.nb + BT - 1) / BT * BT);
+  GS_REQUIRE(G > 0 && G < (1LL << 31), kInvalid, "column pass grid out of range");
+  k_colpass<L, Plan, LAY, BT, TT><<<(unsigned)G, C::CT, 0, s>>>(a.that, a.ihat, twL, a.Z, a.t0, a.tc, a.nb);
+}
+
 template <int L>
 static void colpass_impl(const CorrArgs& a, const float2* twL, int variant, hipStream_t s) {
   using C = CorrCfg<L>;
   if (variant == 0 || variant >= 3) {
-    constexpr int BT = 8, TT = 16;  // tile shape of k_colpass's logical order
-    const long long G = (long long)(C::H / C::COLS) * ((a.tc + TT - 1) / TT * TT) *
-                        ((a.nb + BT - 1) / BT * BT);
-    GS_REQUIRE(G > 0 && G < (1LL << 31), kInvalid, "column pass grid out of range");
-    if (variant == 3)
-      k_colpass<L, 1><<<(unsigned)G, C::CT, 0, s>>>(a.that, a.ihat, twL, a.Z, a.t0, a.tc, a.nb);
-    else if (variant == 4)
-      k_colpass<L, 1, 1><<<(unsigned)G, C::CT, 0, s>>>(a.that, a.ihat, twL, a.Z, a.t0, a.tc, a.nb);
-    else if (variant == 5)
-      k_colpass<L, 0, 1><<<(unsigned)G, C::CT, 0, s>>>(a.that, a.ihat, twL, a.Z, a.t0, a.tc, a.nb);
-    else if (variant == 6)
-      k_colpass<L, 1, 2><<<(unsigned)G, C::CT, 0, s>>>(a.that, a.ihat, twL, a.Z, a.t0, a.tc, a.nb);
-    else
-      k_colpass<L, 0><<<(unsigned)G, C::CT, 0, s>>>(a.that, a.ihat, twL, a.Z, a.t0, a.tc, a.nb);
+    switch (variant) {
+      case 3: colpass_tiled<L, 1, 0, 8, 16>(a, twL, s); break;
+      case 4: colpass_tiled<L, 1, 1, 8, 16>(a, twL, s); break;
+      case 5: colpass_tiled<L, 0, 1, 8, 16>(a, twL, s); break;
+      case 6: colpass_tiled<L, 1, 2, 8, 16>(a, twL, s); break;
+      default: colpass_tiled<L, 0, 0, 8, 16>(a, twL, s); break;
+    }
   } else {
     const unsigned G = (unsigned)((C::H / C::COLS) * a.nb);
     if (variant == 2)

@@ -537,10 +537,12 @@ class Engine {
       const int nb = nb_ > 0 ? nb_ : 1;
       const double plane = (double)P_.window_size * (P_.window_size / 2) * sizeof(float2);
       first = true;
-      for (int c : {32, 64, 128}) {
-        if (c > n_ && c != 32) break;
+      // largest first: more templates per launch cost less record traffic, and a smaller count
+      // must be more than 3 % faster to replace a larger one
+      for (int c : {kMaxChunk, 256, 128, 64}) {
+        if (c > 64 && c / 2 >= n_) continue;  // a smaller count already takes every template at once
         // the inter-pass buffer of c templates must fit in half the free memory (plus what it holds now)
-        if (plane * nb * c > 0.5 * (double)free_b + (double)d_Z_.n * sizeof(float2)) break;
+        if (plane * nb * c > 0.5 * (double)free_b + (double)d_Z_.n * sizeof(float2)) continue;
         chunk_ = c;
         const double ms = measure(2);
         if (first || ms < kMargin * best) best = ms, pick = c;
@@ -897,14 +899,24 @@ class Engine {
       GS_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming));
   }
 
-  int chunk_for(int nb) const {
+  // Templates per launch when not set: as many as a memory budget holds, at most kMaxChunk.  The
+  // row pass reads and writes the per-pixel records once per launch, so more templates per
+  // launch means less record traffic per plane (on MI355X at l = 320 and 234 blocks, 64 -> 512
+  // templates per launch takes about 8 % off the row pass); at 1000 the column pass slowed
+  // down.  Budget: a quarter of the device memory or half of what was free when first asked,
+  // whichever is smaller, halved for the double-buffered two-stream pipeline (512 templates x
+  // 234 blocks at l = 320 is a 49 GB inter-pass buffer).
+  static constexpr int kMaxChunk = 512;
+  int chunk_for(int nb) {
     if (chunk_ > 0) return chunk_;
-    // Inter-pass buffer near 6 GB (64 templates x 234 blocks at l=320): the row pass re-reads
-    // and re-writes the per-pixel records once per launch, so more templates per launch means
-    // less record traffic per plane.
+    if (mem_budget_ <= 0) {
+      size_t fr = 0, tot = 0;
+      GS_HIP(hipMemGetInfo(&fr, &tot));
+      mem_budget_ = std::min(0.25 * (double)tot, 0.5 * (double)fr);
+    }
     const double plane = (double)P_.window_size * (P_.window_size / 2) * sizeof(float2);
-    int tc = (int)std::max(4.0, std::min(64.0, 6.4e9 / (plane * std::max(nb, 1))));
-    return tc;
+    const double per_t = plane * std::max(nb, 1) * (overlap_ ? 2 : 1);
+    return (int)std::max(4.0, std::min((double)kMaxChunk, std::floor(mem_budget_ / per_t)));
   }
 
   // norm_type = 1 (SearchNorm::work, src/norm.cu:620-660) with one correlation pass.
@@ -1232,6 +1244,7 @@ class Engine {
   std::vector<float> angles_;
   // kernel variants default to the fastest measured on the production shape (l = 320)
   int lim_angles_ = 0, lim_templates_ = 0, chunk_ = 0, row_variant_ = 6, col_variant_ = 3;
+  double mem_budget_ = 0;  // bytes the automatic templates-per-launch may give the inter-pass buffer
   double total_ms_ = 0, corr_ms_ = 0;
   std::vector<gs_record> records_;
   std::vector<double> planes_host_;
