@@ -366,25 +366,32 @@ using dev::Bil;
 using dev::bil_coords;
 
 // Rotate + mask one template row per block; per-row double partial sums for mean/var.
-constexpr int kRotRows = 8;  // rows per workgroup of the template rotation
+// Output tile of the template rotation: kRotTX x kRotTY pixels per workgroup of 256 threads,
+// two pixels per thread (rows y and y + 8).  A compact tile keeps the rotated source footprint
+// of a workgroup compact at every angle (a full-row strip fans out over many source rows near
+// 45 degrees).
+constexpr int kRotTX = 32, kRotTY = 16;
 
 // RotateTemplate (src/norm.cu:479-521) for the template pair (2p, 2p+1): rotate_subIMG +
 // apply_mask (k:534-603, 225-241) of both templates into one complex plane (template 2p in the
 // real part, 2p+1 in the imaginary part; their joint forward transform gives both spectra),
-// plus compute_sum_sqr's double sums (k:384-443) per template and group of kRotRows rows.  The
-// sample geometry and the mask factor are computed once for both templates.
-__global__ void k_rotate_mask_pair(const float* __restrict__ tp, int n, int l, float cose, float sine,
-                                   float rin, float rout, float ehw, float2* __restrict__ out,
-                                   double* __restrict__ partial) {
+// plus compute_sum_sqr's double sums (k:384-443) per template and output tile.  The sample
+// geometry and the mask factor are computed once for both templates.
+__global__ void __launch_bounds__(256)
+    k_rotate_mask_pair(const float* __restrict__ tp, int n, int l, float cose, float sine, float rin,
+                       float rout, float ehw, float2* __restrict__ out, double* __restrict__ partial) {
   const int g = blockIdx.x, p = blockIdx.y, groups = gridDim.x;
+  const int gx = (l + kRotTX - 1) / kRotTX;
   const int ta = 2 * p, tb = 2 * p + 1;
   const bool hb = tb < n;
   const float* A = tp + (long long)ta * l * l;
   const float* B = tp + (long long)(hb ? tb : ta) * l * l;
-  const int j0 = g * kRotRows, cnt = (min(l, j0 + kRotRows) - j0) * l;
+  const int i = (g % gx) * kRotTX + threadIdx.x % kRotTX;
   double sa = 0, sa2 = 0, sb = 0, sb2 = 0;
-  for (int e = threadIdx.x; e < cnt; e += blockDim.x) {
-    const int j = j0 + e / l, i = e % l;
+#pragma unroll
+  for (int h = 0; h < kRotTY / 8; ++h) {
+    const int j = (g / gx) * kRotTY + threadIdx.x / kRotTX + 8 * h;
+    if (i >= l || j >= l) continue;
     float ra = 0.f, rb = 0.f;
     const Bil c = bil_coords(i, j, l, cose, sine);
     if (c.ok) {
@@ -625,13 +632,14 @@ void launch_split_blocks(const float* img, int nx, int ny, int l, const int* off
   k_split_blocks<<<nblk((long long)nb * l * l), BS, 0, s>>>(img, nx, ny, l, offx, offy, bx, nb, out);
   GS_CHECK_LAUNCH();
 }
-int rotation_groups(int l) { return (l + kRotRows - 1) / kRotRows; }
+int rotation_groups(int l) { return ((l + kRotTX - 1) / kRotTX) * ((l + kRotTY - 1) / kRotTY); }
 void launch_rotate_mask_pairs(const float* tp, int n, int l, float cose, float sine, float d_m, float ehw,
                               float2* out, double* partial, hipStream_t s) {
   const float dm = 1.5f * d_m;
   const float rin = dm / 2, rout = dm / 2 + 2 * ehw;
-  k_rotate_mask_pair<<<dim3(rotation_groups(l), (n + 1) / 2), BS, 0, s>>>(tp, n, l, cose, sine, rin, rout,
-                                                                          ehw, out, partial);
+  GS_REQUIRE((n + 1) / 2 < 65536, kInvalid, "template count out of range");
+  k_rotate_mask_pair<<<dim3(rotation_groups(l), (n + 1) / 2), 256, 0, s>>>(tp, n, l, cose, sine, rin, rout,
+                                                                           ehw, out, partial);
   GS_CHECK_LAUNCH();
 }
 void launch_rotation_stats(const double* partial, int n, int l, float* stats, hipStream_t s) {
