@@ -83,6 +83,12 @@ __device__ __forceinline__ double block_sum(double v) {
   return t;
 }
 
+// Output tile of the template rotations: kRotTX x kRotTY pixels per workgroup of 256 threads, two
+// pixels per thread (rows y and y + 8).  A compact tile keeps the rotated source footprint of a
+// workgroup compact at every angle (a full-row strip fans out over many source rows near 45
+// degrees).
+constexpr int kRotTX = 32, kRotTY = 16;
+
 // Bilinear sample geometry of rotate_subIMG (k:534-603), float arithmetic with explicit FMA.
 struct Bil {
   int k0, k1, k2, k3;

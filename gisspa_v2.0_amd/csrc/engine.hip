@@ -651,12 +651,13 @@ class Engine {
     transpose_batched(d_ihat_.p, ihat_t, nb_, hx, l, l, stream_);
     const int Kc = (int)std::min<size_t>(k, kRefineChunk);
     float2* H = d_rH_.ensure((size_t)Kc * L2);
-    float2* AB = d_rAB_.ensure((size_t)2 * Kc * L2);
+    // K planes A (r + i dr/dtheta) and (K + 1) / 2 planes holding the B fields of two picks each
+    float2* AB = d_rAB_.ensure((size_t)(Kc + (Kc + 1) / 2) * L2);
     // scratch of the runtime-planned transform, needed only where no LDS plane transform exists
-    float2* S = corr_supported(l) ? nullptr : d_rS_.ensure((size_t)2 * Kc * L2);
+    float2* S = corr_supported(l) ? nullptr : d_rS_.ensure((size_t)(Kc + (Kc + 1) / 2) * L2);
     RefPeak* rp = d_rpk_.ensure(Kc);
     double2* part = d_rpart_.ensure((size_t)Kc * 16);
-    double* part6 = d_rpart6_.ensure((size_t)Kc * l * 6);
+    double* part6 = d_rpart6_.ensure((size_t)Kc * rotation_groups(l) * 6);
     double* stats = d_rstats_.ensure((size_t)Kc * 8);
     double* cg = d_rcg_.ensure((size_t)Kc * 5);
     double* dz = d_rz_.ensure(Kc);
@@ -680,7 +681,7 @@ class Engine {
         launch_ref_template(d_twhite_.p, tab, ntab, dslot, rp, K, l, dp_, part, count, H, stream_);
         fft2(H, K, +1, 1.f);
         launch_ref_rotate(H, rp, K, l, P_.diameter, 4.0f, part6, stats, AB, stream_);
-        fft2(AB, 2 * K, -1, (float)(1.0 / (double)L2));
+        fft2(AB, K + (K + 1) / 2, -1, (float)(1.0 / (double)L2));
         GS_REQUIRE(launch_ref_dot(ihat_t, AB, rp, stats, K, l, cg, stream_), kUnsupported, "window_size not supported");
         launch_ref_update(rp, K, cg, d_mu_.p, d_sigma_.p, l, step, scale, it == 0, it < iters, dz, dz0, dg, stream_);
       }

@@ -20,6 +20,8 @@ inline unsigned nblk(long long n, int bs = BS) { return (unsigned)((n + bs - 1) 
 
 using dev::band_image;
 using dev::band_template;
+using dev::kRotTX;
+using dev::kRotTY;
 using dev::block_sum;
 using dev::ctf_ast;
 using dev::rbin;
@@ -366,12 +368,6 @@ using dev::Bil;
 using dev::bil_coords;
 
 // Rotate + mask one template row per block; per-row double partial sums for mean/var.
-// Output tile of the template rotation: kRotTX x kRotTY pixels per workgroup of 256 threads,
-// two pixels per thread (rows y and y + 8).  A compact tile keeps the rotated source footprint
-// of a workgroup compact at every angle (a full-row strip fans out over many source rows near
-// 45 degrees).
-constexpr int kRotTX = 32, kRotTY = 16;
-
 // RotateTemplate (src/norm.cu:479-521) for the template pair (2p, 2p+1): rotate_subIMG +
 // apply_mask (k:534-603, 225-241) of both templates into one complex plane (template 2p in the
 // real part, 2p+1 in the imaginary part; their joint forward transform gives both spectra),

@@ -18,11 +18,13 @@
 //   k_ref_power / k_ref_pack  weighted spectrum G at the pick's defocus and dG/d ddf, spectrally
 //                             normalised, packed as G + i dG (both have real inverses)
 //   inverse 2-D transform    -> t + i dt (processed template and its defocus derivative)
-//   k_ref_rotate / k_ref_stats / k_ref_standardise
-//                             r, dr/dtheta (bilinear derivative), dr/d ddf through mask and
-//                             box standardisation, packed as A = r + i dr/dtheta, B = dr/d ddf
-//   forward 2-D transform    of A and B
-//   k_ref_dot                 c and its four derivatives (phase ramps for the shift)
+//   k_ref_rotate / k_ref_stats
+//                             r, dr/dtheta (bilinear derivative), dr/d ddf through the mask,
+//                             packed as A = r + i dr/dtheta per pick and B = dr/d ddf of two
+//                             picks per plane; box sums for the standardisation
+//   forward 2-D transform    of the K A planes and (K + 1) / 2 B planes
+//   k_ref_dot                 the standardisation applied to the spectra, c and its four
+//                             derivatives (phase ramps for the shift)
 //   k_ref_update              z, gradient, ascent step
 #include <algorithm>
 #include <cmath>
@@ -41,6 +43,8 @@ using dev::Bil;
 using dev::block_sum;
 using dev::ctf_signed;
 using dev::kRefPi;
+using dev::kRotTX;
+using dev::kRotTY;
 using dev::rbin;
 
 constexpr int BS = 256;
@@ -158,23 +162,32 @@ __global__ void k_ref_pack(const float2* __restrict__ white, const RefWeight* __
   }
 }
 
-// Rotation (float geometry as the search), bilinear derivative, mask; one row of one pick per
-// block.  Writes A = r0 + i dr0/dtheta and B = dr0/d ddf (before the box standardisation, which
-// is affine per pick and is applied to their spectra in k_ref_dot) and per-row double sums.
-__global__ void k_ref_rotate(const float2* __restrict__ H, const RefPeak* __restrict__ pk, int l,
-                             float rin, float rout, float ehw, float2* __restrict__ AB,
-                             double* __restrict__ partial6, int K) {
-  const int jrow = blockIdx.x, j = blockIdx.y;
+// Rotation (float geometry as the search), bilinear derivative, mask; one kRotTX x kRotTY output
+// tile of one pick per block.  Writes A = r0 + i dr0/dtheta and B = dr0/d ddf (before the box
+// standardisation, which is affine per pick and is applied to their spectra in k_ref_dot) and
+// the tile's six double sums.  The B fields of picks 2m and 2m+1 share plane K + m (real and
+// imaginary part: both are real fields, so one forward transform gives both spectra); the last
+// pick of an odd K writes a zero imaginary part.
+__global__ void __launch_bounds__(256)
+    k_ref_rotate(const float2* __restrict__ H, const RefPeak* __restrict__ pk, int l, float rin,
+                 float rout, float ehw, float2* __restrict__ AB, double* __restrict__ partial6, int K) {
+  const int g = blockIdx.x, j = blockIdx.y, groups = gridDim.x;
+  const int gx = (l + kRotTX - 1) / kRotTX;
   const RefPeak P = pk[j];
   const float cose = (float)cos(P.theta * kRefPi / 180);
   const float sine = (float)sin(P.theta * kRefPi / 180);
   const float deg = (float)(kRefPi / 180);
   const long long L2 = (long long)l * l;
   const float2* T = H + (long long)j * L2;
-  float2* A = AB + (long long)j * L2;        // r0 + i dr0/dtheta
-  float2* B = AB + ((long long)K + j) * L2;  // dr0/d ddf
+  float2* A = AB + (long long)j * L2;                                  // r0 + i dr0/dtheta
+  float* Bf = reinterpret_cast<float*>(AB + ((long long)K + j / 2) * L2);  // dr0/d ddf, part j & 1
+  const bool lone = (K & 1) && j == K - 1;
+  const int i = (g % gx) * kRotTX + threadIdx.x % kRotTX;
   double a[6] = {0, 0, 0, 0, 0, 0};
-  for (int i = threadIdx.x; i < l; i += blockDim.x) {
+#pragma unroll
+  for (int h = 0; h < kRotTY / 8; ++h) {
+    const int jrow = (g / gx) * kRotTY + threadIdx.x / kRotTX + 8 * h;
+    if (i >= l || jrow >= l) continue;
     float v = 0.f, vth = 0.f, vdf = 0.f;
     const Bil b = bil_coords(i, jrow, l, cose, sine);
     if (b.ok) {
@@ -193,7 +206,10 @@ __global__ void k_ref_rotate(const float2* __restrict__ H, const RefPeak* __rest
     vdf *= m;
     const long long q = (long long)jrow * l + i;
     A[q] = make_float2(v, vth);
-    B[q] = make_float2(vdf, 0.f);
+    if (lone)
+      reinterpret_cast<float2*>(Bf)[q] = make_float2(vdf, 0.f);
+    else
+      Bf[2 * q + (j & 1)] = vdf;
     a[0] += v;
     a[1] += (double)v * v;
     a[2] += vth;
@@ -213,18 +229,19 @@ __global__ void k_ref_rotate(const float2* __restrict__ H, const RefPeak* __rest
   __syncthreads();
   if (threadIdx.x < 6) {
     const int k = threadIdx.x;
-    partial6[((long long)j * l + jrow) * 6 + k] = red[0][k] + red[1][k] + red[2][k] + red[3][k];
+    partial6[((long long)j * groups + g) * 6 + k] = red[0][k] + red[1][k] + red[2][k] + red[3][k];
   }
 }
 
 // Per pick: the search's float mean/variance (k_rotation_stats) and the derivatives of the
 // mean and standard deviation.  stats[j] = {meanf, varf, mean, sd, dmean_th, dsd_th, dmean_df, dsd_df}
-__global__ void k_ref_stats(const double* __restrict__ partial6, int l, int K, double* __restrict__ stats) {
+__global__ void k_ref_stats(const double* __restrict__ partial6, int groups, int l, int K,
+                            double* __restrict__ stats) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= K) return;
   double s[6] = {0, 0, 0, 0, 0, 0};
-  for (int r = 0; r < l; ++r)
-    for (int k = 0; k < 6; ++k) s[k] += partial6[((long long)j * l + r) * 6 + k];
+  for (int g = 0; g < groups; ++g)
+    for (int k = 0; k < 6; ++k) s[k] += partial6[((long long)j * groups + g) * 6 + k];
   const double L2 = (double)l * l;
   const double mean = s[0] / L2, var = s[1] / L2 - mean * mean, sd = sqrt(var);
   const double dmt = s[2] / L2, dvt = 2 * (s[3] / L2 - mean * dmt);
@@ -293,7 +310,8 @@ __global__ __launch_bounds__(256) void k_ref_dot(const float2* __restrict__ ihat
   const long long L2 = (long long)L * L;
   const float2* I = ihat_t + (long long)P.block * HX * L;
   const float2* A = AB + (long long)j * L2;
-  const float2* B = AB + ((long long)K + j) * L2;
+  const float2* B = AB + ((long long)K + j / 2) * L2;  // shared with pick j ^ 1
+  const bool odd = j & 1;
   double acc[5] = {0, 0, 0, 0, 0};
   for (int e = threadIdx.x; e < L * HX; e += blockDim.x) {
     const int ky = e / HX, kx = e - ky * HX;
@@ -303,7 +321,8 @@ __global__ __launch_bounds__(256) void k_ref_dot(const float2* __restrict__ ihat
     // spectra of the three raw fields r0, dr0/dtheta, dr0/d ddf
     float2 R = make_float2(0.5f * (a.x + am.x), 0.5f * (a.y - am.y));
     float2 Gt = make_float2(0.5f * (a.y + am.y), -0.5f * (a.x - am.x));
-    float2 Gd = make_float2(0.5f * (b.x + bm.x), 0.5f * (b.y - bm.y));
+    float2 Gd = odd ? make_float2(0.5f * (b.y + bm.y), 0.5f * (bm.x - b.x))
+                    : make_float2(0.5f * (b.x + bm.x), 0.5f * (b.y - bm.y));
     float2 Rc = R;  // r0 - mean
     if (e == 0) {
       R.x -= meanf;
@@ -441,9 +460,9 @@ void launch_ref_rotate(const float2* H, const RefPeak* pk, int K, int l, float d
                        double* partial6, double* stats, float2* AB, hipStream_t s) {
   const float dm = 1.5f * d_m;  // apply_mask: d_m = 1.5 * d_m (k:233)
   const float rin = dm / 2, rout = dm / 2 + 2 * ehw;
-  k_ref_rotate<<<dim3(l, K), BS, 0, s>>>(H, pk, l, rin, rout, ehw, AB, partial6, K);
+  k_ref_rotate<<<dim3(rotation_groups(l), K), 256, 0, s>>>(H, pk, l, rin, rout, ehw, AB, partial6, K);
   GS_CHECK_LAUNCH();
-  k_ref_stats<<<nblk(K, 64), 64, 0, s>>>(partial6, l, K, stats);
+  k_ref_stats<<<nblk(K, 64), 64, 0, s>>>(partial6, rotation_groups(l), l, K, stats);
   GS_CHECK_LAUNCH();
 }
 

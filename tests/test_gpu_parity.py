@@ -108,6 +108,28 @@ def test_norm_picks_identical(norm_run):
     np.testing.assert_array_equal(g["psi"], o["psi"])
 
 
+@pytest.mark.parametrize("nt", [1, 5])
+def test_norm_odd_template_counts(gs, orc, nt):
+    """Templates are rotated and transformed in pairs (real and imaginary parts of one plane); with
+    an odd count the last one has no partner.  Per-pixel z and picks must still match the oracle."""
+    import synth
+
+    cfg = gs.SearchConfig(phistep=10.0, window_size=320, norm_type=1, nms_radius=64.0)
+    print(f"odd template count {nt}", flush=True)
+    ds, opicks, ost = _norm_case(orc, synth, cfg, seed=20 + nt, nt=nt)
+    eng = gs.Engine(cfg)
+    eng.load_templates(ds.templates)
+    g = eng.search(ds.image, ds.ctf)
+    best = eng.stage(6, np.float32).astype(np.float64)
+    z_g = (best - eng.stage(4, np.float64)) / eng.stage(5, np.float64)
+    z_o = (ost["best"] - ost["mu"]) / ost["sigma"]
+    assert np.max(np.abs(z_g - z_o) / np.maximum(1.0, np.abs(z_o))) < 1e-4
+    key = lambda a: [(int(p["x"]), int(p["y"]), int(p["template_id"]), int(p["angle_index"])) for p in a]
+    assert key(g) == key(opicks)
+    np.testing.assert_allclose(g["score"], opicks["score"], rtol=1e-4)
+    eng.close()
+
+
 def test_nonorm_small(gs, orc):
     import synth
 
