@@ -742,9 +742,11 @@ void corr_prepare() { (void)tw_tables(current_device()); }
 // stride).  The column radix plan of the correlation is reused for both.
 namespace {
 
+// Rows of `in` transformed into `out` (the same buffer for an in-place transform: every row a
+// workgroup reads is staged in LDS before it writes any, so the pointers carry no restrict).
 template <int L, int SIGN>
 __global__ void __launch_bounds__(256)
-    k_plane_rows(float2* __restrict__ data, long long nrows, const float2* __restrict__ twL, float scale) {
+    k_plane_rows(const float2* in, float2* out, long long nrows, const float2* __restrict__ twL, float scale) {
   using C = CorrCfg<L>;
   constexpr int T = C::TPC, RPW = 64 / T, RPB = 4 * RPW;
   __shared__ float2 buf[RPB * L];
@@ -753,7 +755,8 @@ __global__ void __launch_bounds__(256)
   for (int i = tid; i < L; i += 256) tw[i] = twL[i];
   const long long r0 = (long long)blockIdx.x * RPB;
   const int nr = (int)min((long long)RPB, nrows - r0);
-  float2* src = data + r0 * L;
+  const float2* src = in + r0 * L;
+  float2* dst = out + r0 * L;
   for (int e = tid; e < RPB * L; e += 256) buf[e] = (e < nr * L) ? src[e] : make_float2(0.f, 0.f);
   __syncthreads();
   const int wave = tid / 64, lane = tid % 64;
@@ -762,7 +765,7 @@ __global__ void __launch_bounds__(256)
   __syncthreads();
   for (int e = tid; e < nr * L; e += 256) {
     const float2 v = buf[e];
-    src[e] = make_float2(v.x * scale, v.y * scale);
+    dst[e] = make_float2(v.x * scale, v.y * scale);
   }
 }
 
@@ -793,7 +796,8 @@ __global__ void __launch_bounds__(256)
 }
 
 template <int L>
-void plane_fft2_impl(float2* data, int batch, int sign, float scale, const float2* tw, hipStream_t s) {
+void plane_fft2_impl(const float2* in, float2* data, int batch, int sign, float scale, const float2* tw,
+                     hipStream_t s) {
   using C = CorrCfg<L>;
   constexpr int RPB = 4 * (64 / C::TPC), COLS = 256 / C::TPC;
   const long long nrows = (long long)batch * L;
@@ -801,10 +805,10 @@ void plane_fft2_impl(float2* data, int batch, int sign, float scale, const float
   GS_REQUIRE(batch > 0 && gr < (1LL << 31) && batch < 65536, kInvalid, "plane transform batch out of range");
   const dim3 gc(L / COLS, (unsigned)batch);
   if (sign < 0) {
-    k_plane_rows<L, -1><<<(unsigned)gr, 256, 0, s>>>(data, nrows, tw, 1.f);
+    k_plane_rows<L, -1><<<(unsigned)gr, 256, 0, s>>>(in, data, nrows, tw, 1.f);
     k_plane_cols<L, -1><<<gc, 256, 0, s>>>(data, tw, scale);
   } else {
-    k_plane_rows<L, +1><<<(unsigned)gr, 256, 0, s>>>(data, nrows, tw, 1.f);
+    k_plane_rows<L, +1><<<(unsigned)gr, 256, 0, s>>>(in, data, nrows, tw, 1.f);
     k_plane_cols<L, +1><<<gc, 256, 0, s>>>(data, tw, scale);
   }
   GS_CHECK_LAUNCH();
@@ -812,17 +816,21 @@ void plane_fft2_impl(float2* data, int batch, int sign, float scale, const float
 
 }  // namespace
 
-bool launch_plane_fft2(float2* data, int batch, int L, int sign, float scale, hipStream_t s) {
+bool launch_plane_fft2_from(const float2* in, float2* out, int batch, int L, int sign, float scale, hipStream_t s) {
   TwTables& T = tw_tables(current_device());
   if (L == 320) {
-    plane_fft2_impl<320>(data, batch, sign, scale, T.t320, s);
+    plane_fft2_impl<320>(in, out, batch, sign, scale, T.t320, s);
     return true;
   }
   if (L == 720) {
-    plane_fft2_impl<720>(data, batch, sign, scale, T.t720, s);
+    plane_fft2_impl<720>(in, out, batch, sign, scale, T.t720, s);
     return true;
   }
   return false;
+}
+
+bool launch_plane_fft2(float2* data, int batch, int L, int sign, float scale, hipStream_t s) {
+  return launch_plane_fft2_from(data, data, batch, L, sign, scale, s);
 }
 
 // Partials per (template, block) written by the norm_type=0 row pass: one per row group and

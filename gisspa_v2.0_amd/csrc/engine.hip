@@ -639,18 +639,18 @@ class Engine {
     const int hx = l / 2 + 1;
     const std::vector<int32_t> idx = ref_band_indices(l, dp_.hr_t, dp_.lr_t);
     const int ntab = (int)idx.size();
-    std::vector<int32_t> slot(L2, -1);
-    for (int i = 0; i < ntab; ++i) slot[idx[i]] = i;
     int32_t* didx = d_ridx_.ensure(std::max(ntab, 1));
-    int32_t* dslot = d_rslot_.ensure(L2);
     RefWeight* tab = d_rtab_.ensure(std::max(ntab, 1));
     if (ntab) GS_HIP(hipMemcpyAsync(didx, idx.data(), sizeof(int32_t) * ntab, hipMemcpyHostToDevice, stream_));
-    GS_HIP(hipMemcpyAsync(dslot, slot.data(), sizeof(int32_t) * L2, hipMemcpyHostToDevice, stream_));
     launch_ref_table(didx, ntab, l, dp_, tab, stream_);
     float2* ihat_t = d_rIT_.ensure((size_t)nb_ * hx * l);
     transpose_batched(d_ihat_.p, ihat_t, nb_, hx, l, l, stream_);
     const int Kc = (int)std::min<size_t>(k, kRefineChunk);
+    // weighted spectra (written on the band-pass support only, zero elsewhere for the whole call)
+    // and their inverse transforms
     float2* H = d_rH_.ensure((size_t)Kc * L2);
+    launch_ref_clear(H, (long long)Kc * L2, stream_);
+    float2* Tt = d_rT_.ensure((size_t)Kc * L2);
     // K planes A (r + i dr/dtheta) and (K + 1) / 2 planes holding the B fields of two picks each
     float2* AB = d_rAB_.ensure((size_t)(Kc + (Kc + 1) / 2) * L2);
     // scratch of the runtime-planned transform, needed only where no LDS plane transform exists
@@ -678,9 +678,12 @@ class Engine {
           fft2_batched(fft_, d, S, batch, l, l, sign, scale, stream_);
       };
       for (int it = 0; it <= iters; ++it) {
-        launch_ref_template(d_twhite_.p, tab, ntab, dslot, rp, K, l, dp_, part, count, H, stream_);
-        fft2(H, K, +1, 1.f);
-        launch_ref_rotate(H, rp, K, l, P_.diameter, 4.0f, part6, stats, AB, stream_);
+        launch_ref_template(d_twhite_.p, tab, ntab, rp, K, l, dp_, part, count, H, stream_);
+        if (!launch_plane_fft2_from(H, Tt, K, l, +1, 1.f, stream_)) {
+          GS_HIP(hipMemcpyAsync(Tt, H, sizeof(float2) * (size_t)K * L2, hipMemcpyDeviceToDevice, stream_));
+          fft2_batched(fft_, Tt, S, K, l, l, +1, 1.f, stream_);
+        }
+        launch_ref_rotate(Tt, rp, K, l, P_.diameter, 4.0f, part6, stats, AB, stream_);
         fft2(AB, K + (K + 1) / 2, -1, (float)(1.0 / (double)L2));
         GS_REQUIRE(launch_ref_dot(ihat_t, AB, rp, stats, K, l, cg, stream_), kUnsupported, "window_size not supported");
         launch_ref_update(rp, K, cg, d_mu_.p, d_sigma_.p, l, step, scale, it == 0, it < iters, dz, dz0, dg, stream_);
@@ -1252,8 +1255,8 @@ class Engine {
   double total_n_ = 0, refine_ms_ = 0;
   static constexpr size_t kRefineChunk = 1024;  // picks per refinement batch (about 4 MB each at l=320)
   bool white_ok_ = false;
-  DevBuf<float2> d_twhite_, d_rH_, d_rAB_, d_rS_, d_rIT_;
-  DevBuf<int32_t> d_ridx_, d_rslot_;
+  DevBuf<float2> d_twhite_, d_rH_, d_rT_, d_rAB_, d_rS_, d_rIT_;
+  DevBuf<int32_t> d_ridx_;
   DevBuf<RefWeight> d_rtab_;
   DevBuf<RefPeak> d_rpk_;
   DevBuf<double2> d_rpart_, d_apart_;

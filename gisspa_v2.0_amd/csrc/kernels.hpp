@@ -103,6 +103,8 @@ void corr_prepare();
 // In-place batched 2-D transform of `batch` full L x L complex planes (row-major), unnormalised
 // then multiplied by scale; LDS transforms for L = 320 and 720 (returns false otherwise).
 bool launch_plane_fft2(float2* data, int batch, int L, int sign, float scale, hipStream_t s);
+// The same transform of `in` written to `out` (in is left unchanged).
+bool launch_plane_fft2_from(const float2* in, float2* out, int batch, int L, int sign, float scale, hipStream_t s);
 
 // ---- adjoint refinement of norm_type=1 picks ----
 // Device record of one pick: block, in-block pixel, template, and the refined parameters
@@ -130,13 +132,16 @@ std::vector<int32_t> ref_band_indices(int l, float hr, float lr);
 // Weight table at the given indices for the micrograph parameters p.
 void launch_ref_table(const int32_t* idx, int n, int l, const DevParams& p, RefWeight* tab, hipStream_t s);
 // Weighted, normalised spectrum of each pick's template at its defocus, packed with its
-// defocus derivative: H = G + i dG (K planes of l*l).  slot[e]: table entry of index e or -1.
-void launch_ref_template(const float2* white, const RefWeight* tab, int ntab, const int32_t* slot,
-                         const RefPeak* pk, int K, int l, const DevParams& p, double2* partial,
-                         double count, float2* H, hipStream_t s);
+// defocus derivative: H = G + i dG (K planes of l*l).  Only the table entries are written (the
+// weight is zero elsewhere): the rest of H must hold zeros (launch_ref_clear).
+void launch_ref_template(const float2* white, const RefWeight* tab, int ntab, const RefPeak* pk, int K,
+                         int l, const DevParams& p, double2* partial, double count, float2* H, hipStream_t s);
+// Zeros n complex values (a fill kernel, so that the call may be captured in a graph).
+void launch_ref_clear(float2* H, long long n, hipStream_t s);
 // From the inverse transform t + i dt in H: rotation and mask with their theta and defocus
-// derivatives, packed into AB (A = r0 + i dr0/dtheta in planes 0..K-1, B = dr0/d ddf in planes
-// K..2K-1), and per pick the box statistics (stats: 8 doubles) of the standardisation.
+// derivatives, packed into AB (A = r0 + i dr0/dtheta of pick j in plane j, B = dr0/d ddf of
+// picks 2m and 2m+1 in the real and imaginary parts of plane K + m), and per pick the box
+// statistics (stats: 8 doubles) of the standardisation.
 void launch_ref_rotate(const float2* H, const RefPeak* pk, int K, int l, float d_m, float ehw,
                        double* partial6, double* stats, float2* AB, hipStream_t s);
 // c and dc/d(dx, dy, theta, ddf) per pick (out: K x 5) from the transformed AB and the

@@ -121,12 +121,11 @@ __global__ void k_ref_power(const float2* __restrict__ white, const RefWeight* _
 }
 
 // H = G/m + i (dG/m - G dm/m^2), m = sqrt(sum |G|^2) / count (normalize, k:445-458); the
-// region count is the number of half-plane entries with radial bin < l/2.  slot[e] is the
-// table entry of spectrum index e, or -1 where the weight is zero.
-__global__ void k_ref_pack(const float2* __restrict__ white, const RefWeight* __restrict__ tab,
-                           const int32_t* __restrict__ slot, const RefPeak* __restrict__ pk, int l,
-                           DevParams p, const double2* __restrict__ partial, double count,
-                           float2* __restrict__ H) {
+// region count is the number of half-plane entries with radial bin < l/2.  Only the table
+// entries (the band-pass support) are written; grid (table chunks, K).
+__global__ void k_ref_pack(const float2* __restrict__ white, const RefWeight* __restrict__ tab, int ntab,
+                           const RefPeak* __restrict__ pk, int l, DevParams p,
+                           const double2* __restrict__ partial, double count, float2* __restrict__ H) {
   const int j = blockIdx.y;
   const RefPeak P = pk[j];
   __shared__ double sm[2];
@@ -145,21 +144,24 @@ __global__ void k_ref_pack(const float2* __restrict__ white, const RefWeight* __
   const long long L2 = (long long)l * l;
   const float2* F = white + (long long)P.t * L2;
   float2* out = H + (long long)j * L2;
-  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < L2; e += gridDim.x * blockDim.x) {
-    const int s = slot[e];
-    float2 h = make_float2(0.f, 0.f);
-    if (s >= 0) {
-      double f, df;
-      entry_weight(tab[s], p, P.ddf, &f, &df);
-      if (f != 0.0 || df != 0.0) {
-        const float2 v = F[e];
-        const double a = f / m, b = df / m - f * dm / (m * m);
-        // (a + i b) * v
-        h = make_float2((float)(a * v.x - b * v.y), (float)(a * v.y + b * v.x));
-      }
-    }
-    out[e] = h;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ntab) return;
+  const RefWeight w = tab[i];
+  float2 h = make_float2(0.f, 0.f);
+  double f, df;
+  entry_weight(w, p, P.ddf, &f, &df);
+  if (f != 0.0 || df != 0.0) {
+    const float2 v = F[w.idx];
+    const double a = f / m, b = df / m - f * dm / (m * m);
+    // (a + i b) * v
+    h = make_float2((float)(a * v.x - b * v.y), (float)(a * v.y + b * v.x));
   }
+  out[w.idx] = h;
+}
+
+__global__ void k_ref_clear(float2* __restrict__ H, long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    H[i] = make_float2(0.f, 0.f);
 }
 
 // Rotation (float geometry as the search), bilinear derivative, mask; one kRotTX x kRotTY output
@@ -447,12 +449,18 @@ void launch_ref_table(const int32_t* idx, int n, int l, const DevParams& p, RefW
   GS_CHECK_LAUNCH();
 }
 
-void launch_ref_template(const float2* white, const RefWeight* tab, int ntab, const int32_t* slot,
-                         const RefPeak* pk, int K, int l, const DevParams& p, double2* partial,
-                         double count, float2* H, hipStream_t s) {
+void launch_ref_template(const float2* white, const RefWeight* tab, int ntab, const RefPeak* pk, int K,
+                         int l, const DevParams& p, double2* partial, double count, float2* H, hipStream_t s) {
   k_ref_power<<<dim3(kPowerChunks, K), BS, 0, s>>>(white, tab, ntab, pk, l, p, partial);
   GS_CHECK_LAUNCH();
-  k_ref_pack<<<dim3(nblk((long long)l * l, BS), K), BS, 0, s>>>(white, tab, slot, pk, l, p, partial, count, H);
+  if (ntab > 0) {
+    k_ref_pack<<<dim3(nblk(ntab, BS), K), BS, 0, s>>>(white, tab, ntab, pk, l, p, partial, count, H);
+    GS_CHECK_LAUNCH();
+  }
+}
+
+void launch_ref_clear(float2* H, long long n, hipStream_t s) {
+  k_ref_clear<<<(unsigned)std::min<long long>(4096, (n + BS - 1) / BS), BS, 0, s>>>(H, n);
   GS_CHECK_LAUNCH();
 }
 
