@@ -22,6 +22,7 @@
 // per-pixel best record), or per-plane peak/sum partials for norm_type=0.
 #include <cmath>
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 #include "common.hpp"
@@ -573,24 +574,33 @@ __global__ void __launch_bounds__(256)
     else
       ct::run_stockham<H, TPT, +1, ct::Sync::WAVE>(dst, twf, sub, typename C::RowPlan{});
     const float* out = reinterpret_cast<const float*>(wb);
+    // Fold of the iteration's outputs into the lane's pixels.  Branch-free (the argmax update is a
+    // select): written with an `if`, the compiler made every (pixel, template) update a
+    // divergent branch with its LDS read waited on alone.  Only a launch's last iteration can
+    // hold fewer than TPI templates; it takes the checked path.
+    auto fold = [&](auto full) {
 #pragma unroll
-    for (int i = 0; i < PPL; ++i) {
-      const int q = ln + 64 * i;
-      const int r2 = q / L, x = q - r2 * L;
-      const int xo = 2 * map(x >> 1) + (x & 1);  // real output x sits in complex slot x / 2
+      for (int i = 0; i < PPL; ++i) {
+        const int q = ln + 64 * i;
+        const int r2 = q / L, x = q - r2 * L;
+        const int xo = 2 * map(x >> 1) + (x & 1);  // real output x sits in complex slot x / 2
 #pragma unroll
-      for (int u = 0; u < TPI; ++u) {
-        if (t + u < tc) {
-          const float v = out[(r2 * TPI + u) * 2 * HS + xo];
-          s[i] += v;
-          ss[i] = fmaf(v, v, ss[i]);
-          if (v > bs[i]) {
-            bs[i] = v;
-            ag[i] = t + u;
+        for (int u = 0; u < TPI; ++u) {
+          if (decltype(full)::value || t + u < tc) {
+            const float v = out[(r2 * TPI + u) * 2 * HS + xo];
+            s[i] += v;
+            ss[i] = fmaf(v, v, ss[i]);
+            const bool gt = v > bs[i];  // strict: the first maximum in template order is kept
+            bs[i] = gt ? v : bs[i];
+            ag[i] = gt ? t + u : ag[i];
           }
         }
       }
-    }
+    };
+    if (t + TPI <= tc)
+      fold(std::true_type{});
+    else
+      fold(std::false_type{});
     ct::lds_sync<ct::Sync::WAVE>();
   }
 
