@@ -74,6 +74,8 @@ struct ColPlanOf<320, 1> {
 //  2: compact: columns unpadded, element y of column c at slot y ^ (c & 15) (adjacent columns on
 //     different banks for the write-out gather), twiddles read from the global table: exactly
 //     40 KB at L = 320, so that four workgroups fit a CU.
+//  3: compact as 2, twiddles formed per lane by recurrence from one base per stage (registers;
+//     no table reads in the transform).
 // Z is written with streaming stores: it is read back only by the next kernel, and keeping it
 // out of L2 keeps the spectra columns that neighbouring workgroups are still reading there.
 // BT x TT: tile of the logical order (blocks x templates).  At 512 templates per launch, tiles
@@ -85,7 +87,7 @@ __global__ void __launch_bounds__(CorrCfg<L>::CT)
               const float2* __restrict__ twL, float2* __restrict__ Z, int t0, int tc, int nb) {
   using C = CorrCfg<L>;
   using P = typename ColPlanOf<L, Plan>::type;
-  constexpr bool BK = LAY == 1, CM = LAY == 2;
+  constexpr bool BK = LAY == 1, CM = LAY == 2 || LAY == 3, RC = LAY == 3;
   constexpr int H = C::H, COLS = C::COLS, NT = C::CT, TPC = C::TPC;
   constexpr int LP = BK ? L + 17 : (CM ? L : L + 1);
   static_assert(COLS * TPC == NT, "column pass thread mapping");
@@ -94,8 +96,9 @@ __global__ void __launch_bounds__(CorrCfg<L>::CT)
   static_assert(!BK || ct::stage_twiddle_count(P{}) <= L, "per-stage twiddles fit the table slot");
   // twiddles: each lane's own in registers where the plan allows it (BK), else a table in LDS
   // (per-stage layout for BK, flat otherwise), or the global table (CM)
-  constexpr bool LT = BK && ct::lane_twiddles_fit<TPC>(P{});
-  constexpr ct::TwSrc TS = LT ? ct::TwSrc::LANE : (BK ? ct::TwSrc::STAGE : ct::TwSrc::FLAT);
+  constexpr bool LT = (BK || RC) && ct::lane_twiddles_fit<TPC>(P{});
+  static_assert(!RC || LT, "recurrence twiddles need T % NS == 0");
+  constexpr ct::TwSrc TS = RC ? ct::TwSrc::RECUR : (LT ? ct::TwSrc::LANE : (BK ? ct::TwSrc::STAGE : ct::TwSrc::FLAT));
   constexpr long long PLANE = (long long)(H + 1) * L;
   __shared__ float2 smem[COLS * LP + (LT || CM ? 0 : L)];
   float2* const buf = smem;
@@ -157,8 +160,10 @@ __global__ void __launch_bounds__(CorrCfg<L>::CT)
     }
   }
   // twiddles after the spectra loads, so that their latencies overlap
-  float2 wl[LT ? ct::lane_twiddle_count(P{}) : 1];
-  if constexpr (LT) {
+  float2 wl[LT ? (RC ? ct::lane_base_count(P{}) : ct::lane_twiddle_count(P{})) : 1];
+  if constexpr (RC) {
+    ct::load_lane_bases<L, TPC>(wl, twL, tid % TPC, P{});
+  } else if constexpr (LT) {
     ct::load_lane_twiddles<L, TPC>(wl, twL, tid % TPC, P{});
   } else if constexpr (!CM) {
     if constexpr (BK)
@@ -185,6 +190,9 @@ __global__ void __launch_bounds__(CorrCfg<L>::CT)
   const int col = tid / TPC;
   if constexpr (BK)
     ct::run_stockham<L, TPC, +1, ct::Sync::WAVE, ct::Swz16, TS>(buf + col * LP, tw, tid % TPC, P{});
+  else if constexpr (RC)
+    ct::run_stockham<L, TPC, +1, ct::Sync::WAVE, ct::XorMap, ct::TwSrc::RECUR>(buf + col * LP, tw, tid % TPC, P{},
+                                                                              ct::XorMap{col & 15});
   else if constexpr (CM)
     ct::run_stockham<L, TPC, +1, ct::Sync::WAVE>(buf + col * LP, tw, tid % TPC, P{}, ct::XorMap{col & 15});
   else
@@ -868,6 +876,12 @@ static void colpass_impl(const CorrArgs& a, const float2* twL, int variant, hipS
       case 4: colpass_tiled<L, 1, 1, 8, 16>(a, twL, s); break;
       case 5: colpass_tiled<L, 0, 1, 8, 16>(a, twL, s); break;
       case 6: colpass_tiled<L, 1, 2, 8, 16>(a, twL, s); break;
+      case 7:  // recurrence twiddles need 16 lanes per column and a plan with T % NS == 0 (l = 320)
+        if constexpr (ct::lane_twiddles_fit<CorrCfg<L>::TPC>(typename ColPlanOf<L, 1>::type{}))
+          colpass_tiled<L, 1, 3, 8, 16>(a, twL, s);
+        else
+          colpass_tiled<L, 1, 0, 8, 16>(a, twL, s);
+        break;
       default: colpass_tiled<L, 0, 0, 8, 16>(a, twL, s); break;
     }
   } else {

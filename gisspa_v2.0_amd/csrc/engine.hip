@@ -461,7 +461,7 @@ class Engine {
   }
   void set_chunk(int tc) { chunk_ = tc; }
   void set_variant(int kernel, int variant) {
-    GS_REQUIRE((kernel == 0 && variant >= 0 && variant <= 6) || (kernel == 1 && variant >= 0 && variant <= 6) ||
+    GS_REQUIRE((kernel == 0 && variant >= 0 && variant <= 7) || (kernel == 1 && variant >= 0 && variant <= 6) ||
                    (kernel == 2 && (variant == 0 || variant == 1)),
                kInvalid, "unknown kernel variant");
     if (kernel == 0)
@@ -504,17 +504,18 @@ class Engine {
       return what == 0 ? c : what == 1 ? r : c + r;
     };
     try {
-      // candidates: the natural and the bank-conflict-free LDS layouts of the column pass (radix
-      // 4*4*20), then the row pass with that layout (one row x eight templates per wave
-      // iteration) and without it (two rows x four templates), both prefetched; the other
-      // variants measured slower on MI355X at every size tried.  A later candidate replaces the
+      // candidates: the column pass with the compact LDS layout and recurrence twiddles (four
+      // workgroups per CU) and with the natural layout (radix 4*4*20 both), then the row pass with
+      // the bank-conflict-free layout (one row x eight templates per wave iteration) and without
+      // it (two rows x four templates), both prefetched; the other variants measured slower on
+      // MI355X at every size tried.  A later candidate replaces the
       // current pick only when more than 3 % faster, so that timing noise between near-equal
       // options does not change the configuration from run to run.
       constexpr double kMargin = 0.97;
       double best = 0;
       int pick = col_variant_;
       bool first = true;
-      for (int v : {3, 4}) {
+      for (int v : {7, 3}) {
         col_variant_ = v;
         const double ms = measure(0);
         if (first || ms < kMargin * best) best = ms, pick = v;
@@ -1247,7 +1248,7 @@ class Engine {
   std::vector<int> offx_, offy_, ulx_, uly_;
   std::vector<float> angles_;
   // kernel variants default to the fastest measured on the production shape (l = 320)
-  int lim_angles_ = 0, lim_templates_ = 0, chunk_ = 0, row_variant_ = 6, col_variant_ = 3;
+  int lim_angles_ = 0, lim_templates_ = 0, chunk_ = 0, row_variant_ = 6, col_variant_ = 7;
   double mem_budget_ = 0;  // bytes the automatic templates-per-launch may give the inter-pass buffer
   double total_ms_ = 0, corr_ms_ = 0;
   std::vector<gs_record> records_;

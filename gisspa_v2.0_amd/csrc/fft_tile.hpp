@@ -334,7 +334,10 @@ __device__ __forceinline__ void fill_stage_twiddles(float2* dst, const float2* f
 //  STAGE: the per-stage layout of fill_stage_twiddles.
 //  LANE:  the lane's own twiddles in registers (load_lane_twiddles): every stage with NS > 1
 //         needs T % NS == 0, so that a lane meets a single jm = tid % NS in that stage.
-enum class TwSrc { FLAT, STAGE, LANE };
+//  RECUR: one base twiddle per lane and stage in registers (load_lane_bases), its powers formed
+//         by repeated complex multiplication within the stage (w_r = w_(r-1) w_1: about r
+//         roundings, under 2e-6 relative for r < 20); the same condition on T as LANE.
+enum class TwSrc { FLAT, STAGE, LANE, RECUR };
 
 template <int T, int... RS>
 constexpr bool lane_twiddles_fit(RadixList<RS...>) {
@@ -353,6 +356,32 @@ constexpr int lane_twiddle_count(RadixList<RS...>) {
     ns *= r;
   }
   return n;
+}
+
+template <int... RS>
+constexpr int lane_base_count(RadixList<RS...>) {
+  int ns = 1, n = 0;
+  for (int r : {RS...}) {
+    if (ns > 1) n += 1;
+    ns *= r;
+  }
+  return n;
+}
+
+template <int N, int T, int NS, int OFF, int R, int... REST>
+__device__ __forceinline__ void load_lane_bases_impl(float2* w, const float2* flat, int tid) {
+  if constexpr (NS > 1) {
+    static_assert(T % NS == 0, "lane twiddles need one jm per lane and stage");
+    w[OFF] = flat[(tid % NS) * (N / (NS * R))];
+  }
+  if constexpr (sizeof...(REST) > 0)
+    load_lane_bases_impl<N, T, NS * R, OFF + (NS > 1 ? 1 : 0), REST...>(w, flat, tid);
+}
+// Loads exp(-2 pi i jm / (NS R)) for lane tid's jm = tid % NS of every stage with NS > 1, in
+// stage order, into w[lane_base_count(plan)].
+template <int N, int T, int... RS>
+__device__ __forceinline__ void load_lane_bases(float2* w, const float2* flat, int tid, RadixList<RS...>) {
+  load_lane_bases_impl<N, T, 1, 0, RS...>(w, flat, tid);
 }
 
 template <int N, int T, int NS, int OFF, int R, int... REST>
@@ -387,12 +416,17 @@ __device__ __forceinline__ void stockham(float2* buf, const float2* tw, int tid,
     const int j = tid + it * T;
     if (even || j < NB) {
       const int jm = j % NS;
+      float2 wr = make_float2(1.f, 0.f);  // RECUR: w_1^r
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         float2 a = buf[map(j + r * NB)];
         if constexpr (NS > 1) {
           if (r > 0) {
-            if constexpr (TS == TwSrc::LANE)
+            if constexpr (TS == TwSrc::RECUR) {
+              const float2 w1 = tw_lookup<SIGN>(tw, OFF);
+              wr = r == 1 ? w1 : c_mul(wr, w1);
+              a = c_mul(a, wr);
+            } else if constexpr (TS == TwSrc::LANE)
               a = c_mul(a, tw_lookup<SIGN>(tw, OFF + r - 1));
             else if constexpr (TS == TwSrc::STAGE)
               a = c_mul(a, tw_lookup<SIGN>(tw, OFF + (r - 1) * NS + jm));
@@ -417,7 +451,8 @@ __device__ __forceinline__ void stockham(float2* buf, const float2* tw, int tid,
     }
   }
   lds_sync<S>();
-  constexpr int OFF2 = OFF + (NS > 1 ? (TS == TwSrc::LANE ? R - 1 : (R - 1) * NS) : 0);
+  constexpr int OFF2 =
+      OFF + (NS > 1 ? (TS == TwSrc::LANE ? R - 1 : (TS == TwSrc::RECUR ? 1 : (R - 1) * NS)) : 0);
   if constexpr (sizeof...(REST) > 0)
     stockham<N, T, SIGN, NS * R, S, M, TS, OFF2, REST...>(buf, tw, tid, map);
 }
@@ -428,7 +463,8 @@ __device__ __forceinline__ void run_stockham(float2* buf, const float2* tw, int 
                                              RadixList<RS...>, M map = M{}) {
   static_assert(RadixList<RS...>::product() == N, "radix plan must multiply to N");
   static_assert(S == Sync::BLOCK || (64 % T == 0), "wave-level sync needs transforms inside one wavefront");
-  static_assert(TS != TwSrc::LANE || lane_twiddles_fit<T>(RadixList<RS...>{}), "lane twiddles need T % NS == 0");
+  static_assert((TS != TwSrc::LANE && TS != TwSrc::RECUR) || lane_twiddles_fit<T>(RadixList<RS...>{}),
+                "lane twiddles need T % NS == 0");
   stockham<N, T, SIGN, 1, S, M, TS, 0, RS...>(buf, tw, tid, map);
 }
 

@@ -146,8 +146,10 @@ gs_status gs_kernel_timing(const gs_engine* e, int which, double* avg_ms, long l
 gs_status gs_set_chunk(gs_engine* e, int templates_per_launch);
 /* Selects a result-invariant code variant of a correlation kernel.  kernel 0 = column pass:
  * 0 one workgroup per plane (radix 4*4*4*5), 1 persistent over templates, 2 persistent +
- * prefetch, 3 (default) as 0 with radix 4*4*20 (window 720: as 0), 4/5 as 3/0 with the
- * bank-conflict-free LDS layout (swizzled slots; per-lane twiddles in registers for 4);
+ * prefetch, 3 as 0 with radix 4*4*20 (window 720: as 0), 4/5 as 3/0 with the
+ * bank-conflict-free LDS layout (swizzled slots; per-lane twiddles in registers for 4), 6 as 3
+ * with an unpadded column buffer and twiddles from global memory, 7 (default) as 6 with
+ * twiddles formed per lane by recurrence (window 720: as 3);
  * kernel 1 = row pass: 0 plain, 1 software-prefetched loads, 2/3 as 1/0 with one row and eight
  * templates per wave iteration, 4/5/6 (default 6) as 1/0/2 with the bank-conflict-free LDS
  * layout and streaming loads (window 720: one variant);

@@ -1,5 +1,5 @@
 """Every result-invariant option of the norm_type=1 correlation kernels against the FP64 oracle:
-column-pass and row-pass variants 0-6, templates per launch (including a partial last
+column-pass variants 0-7, row-pass variants 0-6, templates per launch (including a partial last
 launch) and the two-stream pipeline.  Each must give the oracle's per-pixel z of the best
 record within 1e-4 and the oracle's pick list."""
 import numpy as np
@@ -28,6 +28,7 @@ def _key(a):
 
 @pytest.mark.parametrize("col,row,chunk,overlap", [
     (0, 0, 0, 0), (1, 0, 0, 0), (2, 0, 0, 0), (3, 0, 0, 0), (4, 0, 0, 0), (5, 0, 0, 0), (6, 0, 0, 0),
+    (7, 6, 0, 0),
     (0, 1, 0, 0), (0, 2, 0, 0), (0, 3, 0, 0), (0, 4, 0, 0), (0, 5, 0, 0), (0, 6, 0, 0),
     (3, 6, 4, 0),   # eight templates per iteration with a partial last iteration (4 + 2 templates)
     (3, 1, 4, 0),   # two launches per angle, the second partial (4 + 2 templates)
