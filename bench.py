@@ -350,9 +350,11 @@ def main():
         log(f"autotune: column variant {col}, row variant {row}, {chunk} templates per launch "
             f"({time.perf_counter() - t1:.1f} s)")
     else:
-        # an option not given keeps the library default (column variant 7, row variant 6)
-        col = a.col_variant if a.col_variant >= 0 else 7
-        row = a.row_variant if a.row_variant >= 0 else 6
+        # an option not given keeps the library default (column variant 8 with row variant 7, the
+        # row-packed layout); row variant 7 reads only that layout, so another column variant
+        # defaults to row variant 6
+        col = a.col_variant if a.col_variant >= 0 else 8
+        row = a.row_variant if a.row_variant >= 0 else (7 if col == 8 else 6)
         chunk = a.chunk
         eng.set_variant(0, col)
         eng.set_variant(1, row)
@@ -396,10 +398,13 @@ def main():
     if os.path.exists(tpath) and l == 320:
         col_name = {0: "k_colpass<320, 0, 0, 8, 16>", 3: "k_colpass<320, 1, 0, 8, 16>",
                     4: "k_colpass<320, 1, 1, 8, 16>", 5: "k_colpass<320, 0, 1, 8, 16>",
-                    6: "k_colpass<320, 1, 2, 8, 16>", 7: "k_colpass<320, 1, 3, 8, 16>"}.get(variant[0], "")
+                    6: "k_colpass<320, 1, 2, 8, 16>", 7: "k_colpass<320, 1, 3, 8, 16>",
+                    8: "k_colpass<320, 1, 4, 8, 16>"}.get(variant[0], "")
         row_name = "k_rowpass_pix_wave<320, %s>" % {
-            0: "false, 4, false", 1: "true, 4, false", 2: "true, 8, false", 3: "false, 8, false",
-            4: "true, 4, true", 5: "false, 4, true", 6: "true, 8, true"}.get(variant[1], "")
+            0: "false, 4, false, false", 1: "true, 4, false, false", 2: "true, 8, false, false",
+            3: "false, 8, false, false",
+            4: "true, 4, true, false", 5: "false, 4, true, false", 6: "true, 8, true, false",
+            7: "true, 8, true, true"}.get(variant[1], "")
         kname = col_name if dominant == "column_pass" else row_name
         rec = json.load(open(tpath))["kernels"].get(kname)
         if rec:

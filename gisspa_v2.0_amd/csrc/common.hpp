@@ -54,11 +54,20 @@ __device__ __forceinline__ void st_stream(float2* p, float2 v) {
   typedef float f2v __attribute__((ext_vector_type(2)));
   __builtin_nontemporal_store(f2v{v.x, v.y}, reinterpret_cast<f2v*>(p));
 }
+__device__ __forceinline__ void st_stream(float4* p, float4 v) {
+  typedef float f4v __attribute__((ext_vector_type(4)));
+  __builtin_nontemporal_store(f4v{v.x, v.y, v.z, v.w}, reinterpret_cast<f4v*>(p));
+}
 // Streaming load of a value read once (non-temporal hint).
 __device__ __forceinline__ float2 ld_stream(const float2* p) {
   typedef float f2v __attribute__((ext_vector_type(2)));
   const f2v v = __builtin_nontemporal_load(reinterpret_cast<const f2v*>(p));
   return make_float2(v.x, v.y);
+}
+__device__ __forceinline__ float4 ld_stream(const float4* p) {
+  typedef float f4v __attribute__((ext_vector_type(4)));
+  const f4v v = __builtin_nontemporal_load(reinterpret_cast<const f4v*>(p));
+  return make_float4(v.x, v.y, v.z, v.w);
 }
 // i^k * a for the sign convention of the caller (s = +1 multiplies by +i, -1 by -i)
 __device__ __forceinline__ float2 c_muli(float2 a, int s) {

@@ -65,6 +65,17 @@ struct ColPlanOf<320, 1> {
   using type = ct::RadixList<4, 4, 20>;
 };
 
+// Real-to-complex packing of the row transform for one Hermitian pair of a row: from X(k) = a
+// and X(H - k) = b, Zc(k) = (a + conj b) + i w (a - conj b) with w = exp(+2 pi i k / L), whose
+// length-H inverse transform gives the row's real outputs two at a time (Zc(H - k) is
+// zpack(b, a, exp(+2 pi i (H - k) / L))).
+__device__ __forceinline__ float2 zpack(float2 a, float2 b, float2 w) {
+  const float2 bc = c_conj(b);
+  const float2 sm = c_add(a, bc), df = c_sub(a, bc);
+  const float2 wd = c_mul(w, df);
+  return make_float2(sm.x - wd.y, sm.y + wd.x);
+}
+
 // Column LDS layouts (LAY):
 //  0: column stride L + 1, slots in natural order, flat twiddle table in LDS.
 //  1: column stride L + 17, slots permuted by the radix-4 bank swizzle Swz16 and twiddles per
@@ -76,6 +87,12 @@ struct ColPlanOf<320, 1> {
 //     40 KB at L = 320, so that four workgroups fit a CU.
 //  3: compact as 2, twiddles formed per lane by recurrence from one base per stage (registers;
 //     no table reads in the transform).
+//  4: as 3, writing the row-packed layout Zp of the inter-pass buffer: the 16 columns of group g
+//     are 8 mirrored pairs (k, H - k) for k in [8g, 8g + 8) (k = 0 pairs the DC/Nyquist column
+//     with column H/2), and the write-out applies the real-to-complex packing of the row
+//     transform (zpack below) to each pair and stores the two packed values next to each other:
+//     Zp[..][y][2j + 0/1] = packed column j / H - j (H/2 for j = 0).  The row pass then starts
+//     from packed rows.
 // Z is written with streaming stores: it is read back only by the next kernel, and keeping it
 // out of L2 keeps the spectra columns that neighbouring workgroups are still reading there.
 // BT x TT: tile of the logical order (blocks x templates).  At 512 templates per launch, tiles
@@ -87,7 +104,7 @@ __global__ void __launch_bounds__(CorrCfg<L>::CT)
               const float2* __restrict__ twL, float2* __restrict__ Z, int t0, int tc, int nb) {
   using C = CorrCfg<L>;
   using P = typename ColPlanOf<L, Plan>::type;
-  constexpr bool BK = LAY == 1, CM = LAY == 2 || LAY == 3, RC = LAY == 3;
+  constexpr bool BK = LAY == 1, CM = LAY >= 2, RC = LAY >= 3, PK = LAY == 4;
   constexpr int H = C::H, COLS = C::COLS, NT = C::CT, TPC = C::TPC;
   constexpr int LP = BK ? L + 17 : (CM ? L : L + 1);
   static_assert(COLS * TPC == NT, "column pass thread mapping");
@@ -103,7 +120,17 @@ __global__ void __launch_bounds__(CorrCfg<L>::CT)
   __shared__ float2 smem[COLS * LP + (LT || CM ? 0 : L)];
   float2* const buf = smem;
   float2* const tws = smem + COLS * LP;  // LDS twiddles (unless LT or CM)
-  auto slot = [](int c, int y) { return BK ? ct::Swz16{}(y) : (CM ? y ^ (c & 15) : y); };
+  // PK gathers 8 pairs x 2 rows per 16 lanes in the write-out: XOR by 2 (c & 7) keeps those apart
+  auto slot = [](int c, int y) { return BK ? ct::Swz16{}(y) : (PK ? y ^ (2 * (c & 7)) : (CM ? y ^ (c & 15) : y)); };
+  // spectrum column held in LDS slot c (PK: the pair layout of group cg)
+  auto column = [](int cg, int c) {
+    if constexpr (PK) {
+      const int j = 8 * cg + (c & 7);
+      return c < 8 ? j : (j == 0 ? H / 2 : H - j);
+    } else {
+      return cg * COLS + c;
+    }
+  };
 
   // Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a contiguous range of
   // logical work.  Logical order tiles the (block, template) product GEMM-style per column
@@ -136,15 +163,15 @@ __global__ void __launch_bounds__(CorrCfg<L>::CT)
   // issue every load of the wave's columns before using any, for memory-level parallelism
   constexpr int NLD = (CPW * L + 63) / 64;
   constexpr int NDC = (L + 63) / 64;  // loads covering column 0 (q < L)
-  const float2* Tw = Tp + (long long)(kx0 + cw0) * L;
-  const float2* Iw = Ip + (long long)(kx0 + cw0) * L;
   float2 tv[NLD], iv[NLD];
 #pragma unroll
   for (int i = 0; i < NLD; ++i) {
     const int q = lane + 64 * i;
     if ((CPW * L) % 64 == 0 || q < CPW * L) {
-      tv[i] = Tw[q];
-      iv[i] = Iw[q];
+      const int cl = q / L, ky = q - cl * L;
+      const long long e = (long long)column(cg, cw0 + cl) * L + ky;
+      tv[i] = Tp[e];
+      iv[i] = Ip[e];
     }
   }
   const bool dc = (kx0 + cw0 == 0);  // this wave holds column 0: pack the Nyquist column
@@ -191,28 +218,53 @@ __global__ void __launch_bounds__(CorrCfg<L>::CT)
   if constexpr (BK)
     ct::run_stockham<L, TPC, +1, ct::Sync::WAVE, ct::Swz16, TS>(buf + col * LP, tw, tid % TPC, P{});
   else if constexpr (RC)
-    ct::run_stockham<L, TPC, +1, ct::Sync::WAVE, ct::XorMap, ct::TwSrc::RECUR>(buf + col * LP, tw, tid % TPC, P{},
-                                                                              ct::XorMap{col & 15});
+    ct::run_stockham<L, TPC, +1, ct::Sync::WAVE, ct::XorMap, ct::TwSrc::RECUR>(
+        buf + col * LP, tw, tid % TPC, P{}, ct::XorMap{PK ? 2 * (col & 7) : (col & 15)});
   else if constexpr (CM)
     ct::run_stockham<L, TPC, +1, ct::Sync::WAVE>(buf + col * LP, tw, tid % TPC, P{}, ct::XorMap{col & 15});
   else
     ct::run_stockham<L, TPC, +1, ct::Sync::WAVE>(buf + col * LP, tw, tid % TPC, P{});
   __syncthreads();  // rows of Z gather all COLS columns of the workgroup
 
-  // write-out: lane (y0, cc) moves rows y0, y0 + RPI, ... of column cc; every LDS read is issued
-  // before the first store
-  static_assert(NT % COLS == 0, "write-out mapping");
-  constexpr int RPI = NT / COLS, NWO = (L + RPI - 1) / RPI;
-  const int cc = tid % COLS, y0 = tid / COLS;
-  float2* const Zp = Z + ((long long)tl * nb + b) * (long long)L * H + kx0 + cc;
-  float2 ov[NWO];
+  if constexpr (!PK) {
+    // write-out: lane (y0, cc) moves rows y0, y0 + RPI, ... of column cc; every LDS read is
+    // issued before the first store
+    static_assert(NT % COLS == 0, "write-out mapping");
+    constexpr int RPI = NT / COLS, NWO = (L + RPI - 1) / RPI;
+    const int cc = tid % COLS, y0 = tid / COLS;
+    float2* const Zp = Z + ((long long)tl * nb + b) * (long long)L * H + kx0 + cc;
+    float2 ov[NWO];
 #pragma unroll
-  for (int i = 0; i < NWO; ++i)
-    if (L % RPI == 0 || y0 + i * RPI < L) ov[i] = buf[cc * LP + slot(cc, y0 + i * RPI)];
+    for (int i = 0; i < NWO; ++i)
+      if (L % RPI == 0 || y0 + i * RPI < L) ov[i] = buf[cc * LP + slot(cc, y0 + i * RPI)];
 #pragma unroll
-  for (int i = 0; i < NWO; ++i) {
-    const int y = y0 + i * RPI;
-    if (L % RPI == 0 || y < L) st_stream(Zp + (long long)(y < L / 2 ? y + L / 2 : y - L / 2) * H, ov[i]);
+    for (int i = 0; i < NWO; ++i) {
+      const int y = y0 + i * RPI;
+      if (L % RPI == 0 || y < L) st_stream(Zp + (long long)(y < L / 2 ? y + L / 2 : y - L / 2) * H, ov[i]);
+    }
+  } else {
+    // packed write-out: lane (y0, pair jl) packs rows y0, y0 + RPP, ... of the pair (j, H - j)
+    // and stores both values as one 16-byte word; 8 lanes cover the group's 128 bytes of a row
+    static_assert(NT % 8 == 0 && L % (NT / 8) == 0 && COLS == 16, "packed write-out mapping");
+    constexpr int RPP = NT / 8, NWP = L / RPP;
+    const int jl = tid % 8, y0 = tid / 8;
+    const int j = 8 * cg + jl, k2 = j == 0 ? H / 2 : H - j;
+    const float2 w1 = c_conj(twL[j]), w2 = c_conj(twL[k2]);  // exp(+2 pi i k / L)
+    float4* const Zq = reinterpret_cast<float4*>(Z + ((long long)tl * nb + b) * (long long)L * H + 16 * cg) + jl;
+    float2 av[NWP], bv[NWP];
+#pragma unroll
+    for (int i = 0; i < NWP; ++i) {
+      av[i] = buf[jl * LP + slot(jl, y0 + i * RPP)];
+      bv[i] = buf[(8 + jl) * LP + slot(8 + jl, y0 + i * RPP)];
+    }
+#pragma unroll
+    for (int i = 0; i < NWP; ++i) {
+      const int y = y0 + i * RPP;
+      // j = 0: the DC/Nyquist column's real pair (X(0), X(H)) and the self-paired column H/2
+      const float2 z1 = j == 0 ? make_float2(av[i].x + av[i].y, av[i].x - av[i].y) : zpack(av[i], bv[i], w1);
+      const float2 z2 = zpack(bv[i], j == 0 ? bv[i] : av[i], w2);
+      st_stream(Zq + (long long)(y < L / 2 ? y + L / 2 : y - L / 2) * (H / 2), make_float4(z1.x, z1.y, z2.x, z2.y));
+    }
   }
 }
 
@@ -475,7 +527,9 @@ __global__ void __launch_bounds__(CorrCfg<L>::RT)
 // 16-lane group fall on opposite halves of the banks), slots permuted by the radix-4 swizzle
 // Swz8 and the length-H twiddles in the per-stage layout; Z is read with streaming loads (each
 // value is read once, after the column pass wrote it with streaming stores).
-template <int L, bool PF, int TPI_ = CorrCfg<L>::TPI, bool BK = false>
+// PKR: Z holds row-packed pairs (column pass layout 4): each lane loads 16-byte pairs and writes
+// them to their two slots, with no packing arithmetic here.
+template <int L, bool PF, int TPI_ = CorrCfg<L>::TPI, bool BK = false, bool PKR = false>
 __global__ void __launch_bounds__(256)
     k_rowpass_pix_wave(const float2* __restrict__ Z, const float2* __restrict__ twL,
                        const float2* __restrict__ twH, int tc, int nb, double* __restrict__ S,
@@ -525,7 +579,8 @@ __global__ void __launch_bounds__(256)
   // Z values of the next iteration are loaded into registers while the current one is
   // transformed and folded (software pipelining of the only HBM stream of this kernel).
   constexpr int NP = H / 2 / TPT;
-  float2 pa[NP], pb[NP], p0 = make_float2(0.f, 0.f);
+  float2 pa[PKR ? 1 : NP], pb[PKR ? 1 : NP], p0 = make_float2(0.f, 0.f);
+  float4 pq[PKR ? NP : 1];
   // Lane-dependent offsets are derived from an opaque copy of the lane index inside the loop:
   // hoisted out of it, the swizzled LDS addresses and the Z offsets of every unrolled access
   // would each stay live in a register across the loop (about 70 registers), halving the
@@ -540,12 +595,17 @@ __global__ void __launch_bounds__(256)
     const int sub = lnp % TPT, ti = (lnp / TPT) % TPI, rowl = lnp / TPT / TPI;
     if (t + ti < tc) {
       const float2* zr = Z + ((((long long)(t + ti) * nb + b) * L) + y0 + rowl) * H;
-      if (sub == 0) p0 = BK ? ld_stream(zr) : zr[0];
+      if constexpr (PKR) {
 #pragma unroll
-      for (int m = 0; m < NP; ++m) {
-        const int k = 1 + sub + TPT * m;  // 1 .. H/2
-        pa[m] = BK ? ld_stream(zr + k) : zr[k];
-        pb[m] = BK ? ld_stream(zr + H - k) : zr[H - k];
+        for (int m = 0; m < NP; ++m) pq[m] = ld_stream(reinterpret_cast<const float4*>(zr) + sub + TPT * m);
+      } else {
+        if (sub == 0) p0 = BK ? ld_stream(zr) : zr[0];
+#pragma unroll
+        for (int m = 0; m < NP; ++m) {
+          const int k = 1 + sub + TPT * m;  // 1 .. H/2
+          pa[m] = BK ? ld_stream(zr + k) : zr[k];
+          pb[m] = BK ? ld_stream(zr + H - k) : zr[H - k];
+        }
       }
     }
   };
@@ -555,7 +615,14 @@ __global__ void __launch_bounds__(256)
     const int ln = opaque_lane();
     const int sub = ln % TPT;
     float2* const dst = wb + (ln / TPT) * HS;
-    if (t + ti < tc) {
+    if (PKR && t + ti < tc) {
+#pragma unroll
+      for (int m = 0; m < NP; ++m) {  // pair j = sub + TPT m: columns j and H - j (H/2 for j = 0)
+        const int j = sub + TPT * m;
+        dst[map(j)] = make_float2(pq[m].x, pq[m].y);
+        dst[map(j == 0 ? H / 2 : H - j)] = make_float2(pq[m].z, pq[m].w);
+      }
+    } else if (!PKR && t + ti < tc) {
       if (sub == 0) dst[0] = make_float2(p0.x + p0.y, p0.x - p0.y);  // X(0), X(H) real, slot 0
 #pragma unroll
       for (int m = 0; m < NP; ++m) {
@@ -877,10 +944,16 @@ static void colpass_impl(const CorrArgs& a, const float2* twL, int variant, hipS
       case 5: colpass_tiled<L, 0, 1, 8, 16>(a, twL, s); break;
       case 6: colpass_tiled<L, 1, 2, 8, 16>(a, twL, s); break;
       case 7:  // recurrence twiddles need 16 lanes per column and a plan with T % NS == 0 (l = 320)
-        if constexpr (ct::lane_twiddles_fit<CorrCfg<L>::TPC>(typename ColPlanOf<L, 1>::type{}))
-          colpass_tiled<L, 1, 3, 8, 16>(a, twL, s);
-        else
+      case 8:  // the same with the row-packed layout (the row pass must be variant 7)
+        if constexpr (ct::lane_twiddles_fit<CorrCfg<L>::TPC>(typename ColPlanOf<L, 1>::type{}) &&
+                      CorrCfg<L>::COLS == 16) {
+          if (variant == 8)
+            colpass_tiled<L, 1, 4, 8, 16>(a, twL, s);
+          else
+            colpass_tiled<L, 1, 3, 8, 16>(a, twL, s);
+        } else {
           colpass_tiled<L, 1, 0, 8, 16>(a, twL, s);
+        }
         break;
       default: colpass_tiled<L, 0, 0, 8, 16>(a, twL, s); break;
     }
@@ -923,8 +996,9 @@ bool launch_rowpass_pixel(int L, const float2* Z, int tc, int nb, double* S, dou
   TwTables& T = tw_tables(current_device());
   if (L == 320) {
     // variants 0/1: 2 rows x 4 templates per wave iteration without/with prefetch; 2/3: 1 row x
-    // 8 templates with/without prefetch; 4/5/6: as 1/0/2 with the bank-conflict-free LDS layout
-    const int tpi = (variant == 2 || variant == 3 || variant == 6) ? 8 : CorrCfg<320>::TPI;
+    // 8 templates with/without prefetch; 4/5/6: as 1/0/2 with the bank-conflict-free LDS layout;
+    // 7: as 6 reading the row-packed layout (column pass variant 8)
+    const int tpi = (variant == 2 || variant == 3 || variant == 6 || variant == 7) ? 8 : CorrCfg<320>::TPI;
     const int rows = 4 * (64 / CorrCfg<320>::TPT) / tpi;
     dim3 grid(320 / rows, nb);
     if (variant == 1)
@@ -939,6 +1013,8 @@ bool launch_rowpass_pixel(int L, const float2* Z, int tc, int nb, double* S, dou
       k_rowpass_pix_wave<320, false, 4, true><<<grid, 256, 0, s>>>(Z, T.t320, T.t160, tc, nb, S, SS, best, arg, arg_base);
     else if (variant == 6)
       k_rowpass_pix_wave<320, true, 8, true><<<grid, 256, 0, s>>>(Z, T.t320, T.t160, tc, nb, S, SS, best, arg, arg_base);
+    else if (variant == 7)
+      k_rowpass_pix_wave<320, true, 8, true, true><<<grid, 256, 0, s>>>(Z, T.t320, T.t160, tc, nb, S, SS, best, arg, arg_base);
     else
       k_rowpass_pix_wave<320, false><<<grid, 256, 0, s>>>(Z, T.t320, T.t160, tc, nb, S, SS, best, arg, arg_base);
     GS_CHECK_LAUNCH();

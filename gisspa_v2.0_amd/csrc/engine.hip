@@ -461,7 +461,7 @@ class Engine {
   }
   void set_chunk(int tc) { chunk_ = tc; }
   void set_variant(int kernel, int variant) {
-    GS_REQUIRE((kernel == 0 && variant >= 0 && variant <= 7) || (kernel == 1 && variant >= 0 && variant <= 6) ||
+    GS_REQUIRE((kernel == 0 && variant >= 0 && variant <= 8) || (kernel == 1 && variant >= 0 && variant <= 7) ||
                    (kernel == 2 && (variant == 0 || variant == 1)),
                kInvalid, "unknown kernel variant");
     if (kernel == 0)
@@ -504,37 +504,29 @@ class Engine {
       return what == 0 ? c : what == 1 ? r : c + r;
     };
     try {
-      // candidates: the column pass with the compact LDS layout and recurrence twiddles (four
-      // workgroups per CU) and with the natural layout (radix 4*4*20 both), then the row pass with
-      // the bank-conflict-free layout (one row x eight templates per wave iteration) and without
-      // it (two rows x four templates), both prefetched; the other variants measured slower on
-      // MI355X at every size tried.  A later candidate replaces the
-      // current pick only when more than 3 % faster, so that timing noise between near-equal
-      // options does not change the configuration from run to run.
+      // candidates, as (column pass, row pass) pairs timed together: the row-packed layout (column
+      // pass 8 packs the rows, row pass 7 reads them), the unpacked layout with the compact
+      // recurrence-twiddle column pass and the bank-conflict-free row pass (7, 6), and the natural
+      // layouts (3, 1); the other variants measured slower on MI355X at every size tried.  A
+      // later candidate replaces the current pick only when more than 3 % faster, so that timing
+      // noise between near-equal options does not change the configuration from run to run.
       constexpr double kMargin = 0.97;
       double best = 0;
-      int pick = col_variant_;
       bool first = true;
-      for (int v : {7, 3}) {
-        col_variant_ = v;
-        const double ms = measure(0);
-        if (first || ms < kMargin * best) best = ms, pick = v;
+      int pick_col = col_variant_, pick_row = row_variant_;
+      for (const auto& c : {std::pair<int, int>{8, 7}, std::pair<int, int>{7, 6}, std::pair<int, int>{3, 1}}) {
+        col_variant_ = c.first;
+        row_variant_ = c.second;
+        const double ms = measure(2);
+        if (first || ms < kMargin * best) best = ms, pick_col = c.first, pick_row = c.second;
         first = false;
       }
-      col_variant_ = pick;
-      pick = row_variant_;
-      first = true;
-      for (int v : {6, 1}) {
-        row_variant_ = v;
-        const double ms = measure(1);
-        if (first || ms < kMargin * best) best = ms, pick = v;
-        first = false;
-      }
-      row_variant_ = pick;
+      col_variant_ = pick_col;
+      row_variant_ = pick_row;
+      int pick = 0;
       size_t free_b = 0, total_b = 0;
       GS_HIP(hipMemGetInfo(&free_b, &total_b));
       const int keep = chunk_;
-      pick = 0;
       const int nb = nb_ > 0 ? nb_ : 1;
       const double plane = (double)P_.window_size * (P_.window_size / 2) * sizeof(float2);
       first = true;
@@ -969,6 +961,10 @@ class Engine {
       ensure_pipeline();
       rs = stream2_;
     }
+    // column variant 8 writes the row-packed layout that only row variant 7 reads (l = 320; other
+    // sizes use one unpacked layout)
+    GS_REQUIRE(l != 320 || (col_variant_ == 8) == (row_variant_ == 7), kInvalid,
+               "column variant 8 (row-packed inter-pass layout) and row variant 7 go together");
     float2* Zb = d_Z_.ensure(zsz * (overlap ? 2 : 1));
     double* part = d_part_.ensure((size_t)n * rotation_groups(l) * 2);
     float* stats = d_stats_.ensure((size_t)n * 2);
@@ -1164,7 +1160,9 @@ class Engine {
           const int tc = std::min(tc_max, n - t0);
           CorrArgs ca{d_that_.p, ihat, Z, t0, tc, nbp};
           clock_begin(clk_col_, stream_);
-          GS_REQUIRE(launch_colpass(l, ca, col_variant_, stream_), kUnsupported, "window_size not supported");
+          // the norm_type=0 row pass reads the unpacked layout: variant 8 runs as 7 here
+          GS_REQUIRE(launch_colpass(l, ca, col_variant_ == 8 ? 7 : col_variant_, stream_), kUnsupported,
+                     "window_size not supported");
           clock_end(clk_col_, stream_);
           clock_begin(clk_row_, stream_);
           launch_rowpass_plane(l, Z, tc, nbp, part, border, stream_);
@@ -1248,7 +1246,7 @@ class Engine {
   std::vector<int> offx_, offy_, ulx_, uly_;
   std::vector<float> angles_;
   // kernel variants default to the fastest measured on the production shape (l = 320)
-  int lim_angles_ = 0, lim_templates_ = 0, chunk_ = 0, row_variant_ = 6, col_variant_ = 7;
+  int lim_angles_ = 0, lim_templates_ = 0, chunk_ = 0, row_variant_ = 7, col_variant_ = 8;
   double mem_budget_ = 0;  // bytes the automatic templates-per-launch may give the inter-pass buffer
   double total_ms_ = 0, corr_ms_ = 0;
   std::vector<gs_record> records_;

@@ -1,5 +1,5 @@
 """Every result-invariant option of the norm_type=1 correlation kernels against the FP64 oracle:
-column-pass variants 0-7, row-pass variants 0-6, templates per launch (including a partial last
+column-pass variants 0-8, row-pass variants 0-7, templates per launch (including a partial last
 launch) and the two-stream pipeline.  Each must give the oracle's per-pixel z of the best
 record within 1e-4 and the oracle's pick list."""
 import numpy as np
@@ -29,6 +29,7 @@ def _key(a):
 @pytest.mark.parametrize("col,row,chunk,overlap", [
     (0, 0, 0, 0), (1, 0, 0, 0), (2, 0, 0, 0), (3, 0, 0, 0), (4, 0, 0, 0), (5, 0, 0, 0), (6, 0, 0, 0),
     (7, 6, 0, 0),
+    (8, 7, 0, 0), (8, 7, 4, 0),  # row-packed inter-pass layout (also with a partial last launch)
     (0, 1, 0, 0), (0, 2, 0, 0), (0, 3, 0, 0), (0, 4, 0, 0), (0, 5, 0, 0), (0, 6, 0, 0),
     (3, 6, 4, 0),   # eight templates per iteration with a partial last iteration (4 + 2 templates)
     (3, 1, 4, 0),   # two launches per angle, the second partial (4 + 2 templates)
@@ -53,4 +54,18 @@ def test_variant_matches_oracle(gs, case, col, row, chunk, overlap):
     assert len(picks) > 0
     assert _key(g) == _key(picks)
     np.testing.assert_allclose(g["score"], picks["score"], rtol=1e-4)
+    eng.close()
+
+
+def test_packed_layout_needs_its_row_pass(gs, case):
+    """Column variant 8 writes the row-packed layout that only row variant 7 reads; a mismatched
+    pair is rejected before any kernel runs."""
+    cfg, ds, _, _ = case
+    eng = gs.Engine(cfg)
+    eng.load_templates(ds.templates)
+    for col, row in ((8, 6), (7, 7)):
+        eng.set_variant(0, col)
+        eng.set_variant(1, row)
+        with pytest.raises(gs.GisspaError, match="go together"):
+            eng.search(ds.image, ds.ctf)
     eng.close()
