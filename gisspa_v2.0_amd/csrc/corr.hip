@@ -827,22 +827,35 @@ void corr_prepare() { (void)tw_tables(current_device()); }
 // stride).  The column radix plan of the correlation is reused for both.
 namespace {
 
+// Input rows (and, in the column pass, input elements) y with min(y, L - y) > band are known to
+// be zero: they are not read (band >= L/2: every row is read).
+__device__ __forceinline__ bool zero_row(int y, int L, int band) { return min(y, L - y) > band; }
+
 // Rows of `in` transformed into `out` (the same buffer for an in-place transform: every row a
-// workgroup reads is staged in LDS before it writes any, so the pointers carry no restrict).
+// workgroup reads is staged in LDS before it writes any, so the pointers carry no restrict).  A
+// workgroup whose rows are all zero by `band` only writes zeros.
 template <int L, int SIGN>
 __global__ void __launch_bounds__(256)
-    k_plane_rows(const float2* in, float2* out, long long nrows, const float2* __restrict__ twL, float scale) {
+    k_plane_rows(const float2* in, float2* out, long long nrows, const float2* __restrict__ twL, float scale,
+                 int band) {
   using C = CorrCfg<L>;
   constexpr int T = C::TPC, RPW = 64 / T, RPB = 4 * RPW;
+  static_assert(L % RPB == 0, "a workgroup's rows lie in one plane");
   __shared__ float2 buf[RPB * L];
   __shared__ float2 tw[L];
   const int tid = threadIdx.x;
-  for (int i = tid; i < L; i += 256) tw[i] = twL[i];
   const long long r0 = (long long)blockIdx.x * RPB;
   const int nr = (int)min((long long)RPB, nrows - r0);
   const float2* src = in + r0 * L;
   float2* dst = out + r0 * L;
-  for (int e = tid; e < RPB * L; e += 256) buf[e] = (e < nr * L) ? src[e] : make_float2(0.f, 0.f);
+  const int y0 = (int)(r0 % L);
+  if (zero_row(y0, L, band) && zero_row(y0 + RPB - 1, L, band) && y0 + RPB - 1 < L - band) {
+    for (int e = tid; e < nr * L; e += 256) dst[e] = make_float2(0.f, 0.f);
+    return;  // uniform per workgroup
+  }
+  for (int i = tid; i < L; i += 256) tw[i] = twL[i];
+  for (int e = tid; e < RPB * L; e += 256)
+    buf[e] = (e < nr * L && !zero_row(y0 + e / L, L, band)) ? src[e] : make_float2(0.f, 0.f);
   __syncthreads();
   const int wave = tid / 64, lane = tid % 64;
   ct::run_stockham<L, T, SIGN, ct::Sync::WAVE>(buf + (wave * RPW + lane / T) * L, tw, lane % T,
@@ -856,7 +869,7 @@ __global__ void __launch_bounds__(256)
 
 template <int L, int SIGN>
 __global__ void __launch_bounds__(256)
-    k_plane_cols(float2* __restrict__ data, const float2* __restrict__ twL, float scale) {
+    k_plane_cols(float2* __restrict__ data, const float2* __restrict__ twL, float scale, int band) {
   using C = CorrCfg<L>;
   constexpr int T = C::TPC, COLS = 256 / T, LP = L + 1;
   static_assert(L % COLS == 0, "column groups");
@@ -868,7 +881,7 @@ __global__ void __launch_bounds__(256)
   float2* P = data + (long long)blockIdx.y * L * L;
   for (int e = tid; e < COLS * L; e += 256) {
     const int y = e / COLS, c = e % COLS;
-    buf[c * LP + y] = P[(long long)y * L + c0 + c];
+    buf[c * LP + y] = zero_row(y, L, band) ? make_float2(0.f, 0.f) : P[(long long)y * L + c0 + c];
   }
   __syncthreads();
   ct::run_stockham<L, T, SIGN, ct::Sync::WAVE>(buf + (tid / T) * LP, tw, tid % T, typename C::ColPlan{});
@@ -882,7 +895,7 @@ __global__ void __launch_bounds__(256)
 
 template <int L>
 void plane_fft2_impl(const float2* in, float2* data, int batch, int sign, float scale, const float2* tw,
-                     hipStream_t s) {
+                     int band, hipStream_t s) {
   using C = CorrCfg<L>;
   constexpr int RPB = 4 * (64 / C::TPC), COLS = 256 / C::TPC;
   const long long nrows = (long long)batch * L;
@@ -890,25 +903,27 @@ void plane_fft2_impl(const float2* in, float2* data, int batch, int sign, float 
   GS_REQUIRE(batch > 0 && gr < (1LL << 31) && batch < 65536, kInvalid, "plane transform batch out of range");
   const dim3 gc(L / COLS, (unsigned)batch);
   if (sign < 0) {
-    k_plane_rows<L, -1><<<(unsigned)gr, 256, 0, s>>>(in, data, nrows, tw, 1.f);
-    k_plane_cols<L, -1><<<gc, 256, 0, s>>>(data, tw, scale);
+    k_plane_rows<L, -1><<<(unsigned)gr, 256, 0, s>>>(in, data, nrows, tw, 1.f, band);
+    k_plane_cols<L, -1><<<gc, 256, 0, s>>>(data, tw, scale, band);
   } else {
-    k_plane_rows<L, +1><<<(unsigned)gr, 256, 0, s>>>(in, data, nrows, tw, 1.f);
-    k_plane_cols<L, +1><<<gc, 256, 0, s>>>(data, tw, scale);
+    k_plane_rows<L, +1><<<(unsigned)gr, 256, 0, s>>>(in, data, nrows, tw, 1.f, band);
+    k_plane_cols<L, +1><<<gc, 256, 0, s>>>(data, tw, scale, band);
   }
   GS_CHECK_LAUNCH();
 }
 
 }  // namespace
 
-bool launch_plane_fft2_from(const float2* in, float2* out, int batch, int L, int sign, float scale, hipStream_t s) {
+bool launch_plane_fft2_from(const float2* in, float2* out, int batch, int L, int sign, float scale, hipStream_t s,
+                            int band) {
   TwTables& T = tw_tables(current_device());
+  if (band < 0 || band > L / 2) band = L / 2;
   if (L == 320) {
-    plane_fft2_impl<320>(in, out, batch, sign, scale, T.t320, s);
+    plane_fft2_impl<320>(in, out, batch, sign, scale, T.t320, band, s);
     return true;
   }
   if (L == 720) {
-    plane_fft2_impl<720>(in, out, batch, sign, scale, T.t720, s);
+    plane_fft2_impl<720>(in, out, batch, sign, scale, T.t720, band, s);
     return true;
   }
   return false;

@@ -632,6 +632,9 @@ class Engine {
     const int hx = l / 2 + 1;
     const std::vector<int32_t> idx = ref_band_indices(l, dp_.hr_t, dp_.lr_t);
     const int ntab = (int)idx.size();
+    // rows ky of the weighted spectra that can be non-zero: min(ky, l - ky) <= band
+    int band = 0;
+    for (int32_t e : idx) band = std::max(band, std::min(e / l, l - e / l));
     int32_t* didx = d_ridx_.ensure(std::max(ntab, 1));
     RefWeight* tab = d_rtab_.ensure(std::max(ntab, 1));
     if (ntab) GS_HIP(hipMemcpyAsync(didx, idx.data(), sizeof(int32_t) * ntab, hipMemcpyHostToDevice, stream_));
@@ -672,7 +675,7 @@ class Engine {
       };
       for (int it = 0; it <= iters; ++it) {
         launch_ref_template(d_twhite_.p, tab, ntab, rp, K, l, dp_, part, count, H, stream_);
-        if (!launch_plane_fft2_from(H, Tt, K, l, +1, 1.f, stream_)) {
+        if (!launch_plane_fft2_from(H, Tt, K, l, +1, 1.f, stream_, band)) {
           GS_HIP(hipMemcpyAsync(Tt, H, sizeof(float2) * (size_t)K * L2, hipMemcpyDeviceToDevice, stream_));
           fft2_batched(fft_, Tt, S, K, l, l, +1, 1.f, stream_);
         }

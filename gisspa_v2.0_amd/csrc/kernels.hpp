@@ -103,8 +103,10 @@ void corr_prepare();
 // In-place batched 2-D transform of `batch` full L x L complex planes (row-major), unnormalised
 // then multiplied by scale; LDS transforms for L = 320 and 720 (returns false otherwise).
 bool launch_plane_fft2(float2* data, int batch, int L, int sign, float scale, hipStream_t s);
-// The same transform of `in` written to `out` (in is left unchanged).
-bool launch_plane_fft2_from(const float2* in, float2* out, int batch, int L, int sign, float scale, hipStream_t s);
+// The same transform of `in` written to `out` (in is left unchanged).  Input rows y with
+// min(y, L - y) > band are taken as zero without being read (default: every row is read).
+bool launch_plane_fft2_from(const float2* in, float2* out, int batch, int L, int sign, float scale, hipStream_t s,
+                            int band = -1);
 
 // ---- adjoint refinement of norm_type=1 picks ----
 // Device record of one pick: block, in-block pixel, template, and the refined parameters
