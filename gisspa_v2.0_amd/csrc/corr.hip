@@ -120,8 +120,13 @@ __global__ void __launch_bounds__(CorrCfg<L>::CT)
   __shared__ float2 smem[COLS * LP + (LT || CM ? 0 : L)];
   float2* const buf = smem;
   float2* const tws = smem + COLS * LP;  // LDS twiddles (unless LT or CM)
-  // PK gathers 8 pairs x 2 rows per 16 lanes in the write-out: XOR by 2 (c & 7) keeps those apart
-  auto slot = [](int c, int y) { return BK ? ct::Swz16{}(y) : (PK ? y ^ (2 * (c & 7)) : (CM ? y ^ (c & 15) : y)); };
+  // Slot of element y of column c: as loaded and inside the transform (slot_in), and as the last
+  // stage leaves it for the write-out (slot_out).  The compact layouts swizzle only the output:
+  // XOR by c & 15 (PK: 2 (c & 7), as PK gathers 8 pairs x 2 rows per 16 lanes) keeps the columns
+  // a 16-lane group gathers on different banks.
+  auto cmask = [](int c) { return PK ? 2 * (c & 7) : (c & 15); };
+  auto slot_in = [](int c, int y) { return BK ? ct::Swz16{}(y) : y; };
+  auto slot_out = [&](int c, int y) { return BK ? ct::Swz16{}(y) : (CM ? ct::LowXor16{cmask(c)}(y) : y); };
   // spectrum column held in LDS slot c (PK: the pair layout of group cg)
   auto column = [](int cg, int c) {
     if constexpr (PK) {
@@ -210,7 +215,7 @@ __global__ void __launch_bounds__(CorrCfg<L>::CT)
         const float2 pn = c_mulconj(in_[i < NDC ? i : 0], tn[i < NDC ? i : 0]);
         p = make_float2(p.x - pn.y, p.y + pn.x);
       }
-      buf[(cw0 + cl) * LP + slot(cw0 + cl, ky)] = p;
+      buf[(cw0 + cl) * LP + slot_in(cw0 + cl, ky)] = p;
     }
   }
   ct::lds_sync<ct::Sync::WAVE>();
@@ -218,10 +223,11 @@ __global__ void __launch_bounds__(CorrCfg<L>::CT)
   if constexpr (BK)
     ct::run_stockham<L, TPC, +1, ct::Sync::WAVE, ct::Swz16, TS>(buf + col * LP, tw, tid % TPC, P{});
   else if constexpr (RC)
-    ct::run_stockham<L, TPC, +1, ct::Sync::WAVE, ct::XorMap, ct::TwSrc::RECUR>(
-        buf + col * LP, tw, tid % TPC, P{}, ct::XorMap{PK ? 2 * (col & 7) : (col & 15)});
+    ct::run_stockham_fm<L, TPC, +1, ct::Sync::WAVE, ct::IdentityMap, ct::LowXor16, ct::TwSrc::RECUR>(
+        buf + col * LP, tw, tid % TPC, P{}, ct::IdentityMap{}, ct::LowXor16{cmask(col)});
   else if constexpr (CM)
-    ct::run_stockham<L, TPC, +1, ct::Sync::WAVE>(buf + col * LP, tw, tid % TPC, P{}, ct::XorMap{col & 15});
+    ct::run_stockham_fm<L, TPC, +1, ct::Sync::WAVE, ct::IdentityMap, ct::LowXor16, ct::TwSrc::FLAT>(
+        buf + col * LP, tw, tid % TPC, P{}, ct::IdentityMap{}, ct::LowXor16{cmask(col)});
   else
     ct::run_stockham<L, TPC, +1, ct::Sync::WAVE>(buf + col * LP, tw, tid % TPC, P{});
   __syncthreads();  // rows of Z gather all COLS columns of the workgroup
@@ -236,7 +242,7 @@ __global__ void __launch_bounds__(CorrCfg<L>::CT)
     float2 ov[NWO];
 #pragma unroll
     for (int i = 0; i < NWO; ++i)
-      if (L % RPI == 0 || y0 + i * RPI < L) ov[i] = buf[cc * LP + slot(cc, y0 + i * RPI)];
+      if (L % RPI == 0 || y0 + i * RPI < L) ov[i] = buf[cc * LP + slot_out(cc, y0 + i * RPI)];
 #pragma unroll
     for (int i = 0; i < NWO; ++i) {
       const int y = y0 + i * RPI;
@@ -254,8 +260,8 @@ __global__ void __launch_bounds__(CorrCfg<L>::CT)
     float2 av[NWP], bv[NWP];
 #pragma unroll
     for (int i = 0; i < NWP; ++i) {
-      av[i] = buf[jl * LP + slot(jl, y0 + i * RPP)];
-      bv[i] = buf[(8 + jl) * LP + slot(8 + jl, y0 + i * RPP)];
+      av[i] = buf[jl * LP + slot_out(jl, y0 + i * RPP)];
+      bv[i] = buf[(8 + jl) * LP + slot_out(8 + jl, y0 + i * RPP)];
     }
 #pragma unroll
     for (int i = 0; i < NWP; ++i) {

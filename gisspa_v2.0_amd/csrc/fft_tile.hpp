@@ -262,6 +262,12 @@ struct XorMap {
   int mask;
   __device__ __forceinline__ int operator()(int i) const { return i ^ mask; }
 };
+// i ^ mask for mask < 16, written so that for i = lane part (< 16) + multiple of 16 the compiler
+// keeps the lane part's XOR as one value and folds the multiple into the access offset.
+struct LowXor16 {
+  int mask;
+  __device__ __forceinline__ int operator()(int i) const { return (i & ~15) | ((i & 15) ^ mask); }
+};
 
 template <int... RS>
 struct RadixList {
@@ -404,9 +410,10 @@ __device__ __forceinline__ void load_lane_twiddles(float2* w, const float2* flat
 
 // One Stockham stage (radix R, NS = product of earlier radices) over a length-N transform in
 // buf, shared by T threads (tid in [0,T)); then recurse into the remaining radices.  OFF is the
-// offset of this stage's entries in the STAGE or LANE twiddle source.
-template <int N, int T, int SIGN, int NS, Sync S, typename M, TwSrc TS, int OFF, int R, int... REST>
-__device__ __forceinline__ void stockham(float2* buf, const float2* tw, int tid, M map) {
+// offset of this stage's entries in the STAGE or LANE twiddle source.  Slots are read and written
+// through `map`, except that the last stage writes through `fmap` (the layout its consumer reads).
+template <int N, int T, int SIGN, int NS, Sync S, typename M, typename MF, TwSrc TS, int OFF, int R, int... REST>
+__device__ __forceinline__ void stockham(float2* buf, const float2* tw, int tid, M map, MF fmap) {
   constexpr int NB = N / R;
   constexpr int IT = (NB + T - 1) / T;
   constexpr bool even = (NB % T) == 0;
@@ -447,14 +454,19 @@ __device__ __forceinline__ void stockham(float2* buf, const float2* tw, int tid,
       const int jm = j % NS;
       const int idxD = (j / NS) * NS * R + jm;
 #pragma unroll
-      for (int r = 0; r < R; ++r) buf[map(idxD + r * NS)] = v[it][r];
+      for (int r = 0; r < R; ++r) {
+        if constexpr (sizeof...(REST) > 0)
+          buf[map(idxD + r * NS)] = v[it][r];
+        else
+          buf[fmap(idxD + r * NS)] = v[it][r];
+      }
     }
   }
   lds_sync<S>();
   constexpr int OFF2 =
       OFF + (NS > 1 ? (TS == TwSrc::LANE ? R - 1 : (TS == TwSrc::RECUR ? 1 : (R - 1) * NS)) : 0);
   if constexpr (sizeof...(REST) > 0)
-    stockham<N, T, SIGN, NS * R, S, M, TS, OFF2, REST...>(buf, tw, tid, map);
+    stockham<N, T, SIGN, NS * R, S, M, MF, TS, OFF2, REST...>(buf, tw, tid, map, fmap);
 }
 
 template <int N, int T, int SIGN, Sync S = Sync::BLOCK, typename M = IdentityMap,
@@ -465,7 +477,18 @@ __device__ __forceinline__ void run_stockham(float2* buf, const float2* tw, int 
   static_assert(S == Sync::BLOCK || (64 % T == 0), "wave-level sync needs transforms inside one wavefront");
   static_assert((TS != TwSrc::LANE && TS != TwSrc::RECUR) || lane_twiddles_fit<T>(RadixList<RS...>{}),
                 "lane twiddles need T % NS == 0");
-  stockham<N, T, SIGN, 1, S, M, TS, 0, RS...>(buf, tw, tid, map);
+  stockham<N, T, SIGN, 1, S, M, M, TS, 0, RS...>(buf, tw, tid, map, map);
+}
+
+// As run_stockham, with the last stage writing through fmap instead of map.
+template <int N, int T, int SIGN, Sync S, typename M, typename MF, TwSrc TS, int... RS>
+__device__ __forceinline__ void run_stockham_fm(float2* buf, const float2* tw, int tid, RadixList<RS...>, M map,
+                                                MF fmap) {
+  static_assert(RadixList<RS...>::product() == N, "radix plan must multiply to N");
+  static_assert(S == Sync::BLOCK || (64 % T == 0), "wave-level sync needs transforms inside one wavefront");
+  static_assert((TS != TwSrc::LANE && TS != TwSrc::RECUR) || lane_twiddles_fit<T>(RadixList<RS...>{}),
+                "lane twiddles need T % NS == 0");
+  stockham<N, T, SIGN, 1, S, M, MF, TS, 0, RS...>(buf, tw, tid, map, fmap);
 }
 
 }  // namespace ct
