@@ -280,18 +280,20 @@ struct RadixList {
 // banks.  XOR-ing bits of the slot's block index into its low bits spreads those writes over
 // distinct banks, while reads of T consecutive slots (one aligned block) stay a permutation of
 // that block.  Both maps keep every slot inside its aligned block of T, so N % T == 0 suffices.
-// Swz16: T = 16 lanes per transform, block t = bits 4-5, low bits ^= t | t << 2.
+// Swz16: T = 16 lanes per transform, block t = bits 4-5, low bits ^= t | t << 2 (written as Swz8).
 struct Swz16 {
   __device__ __forceinline__ int operator()(int i) const {
     const int t = (i >> 4) & 3;
-    return i ^ (t | (t << 2));
+    return (i & ~15) | ((i & 15) ^ (t | (t << 2)));
   }
 };
-// Swz8: T = 8 lanes per transform, block t = bits 3-4, low bits ^= t | (t & 2) << 1.
+// Swz8: T = 8 lanes per transform, block t = bits 3-4, low bits ^= t | (t & 2) << 1.  Written
+// as (i & ~7) | ((i & 7) ^ g): for i = lane part (< 8) + multiple of 8 the block index and the
+// XOR value are constants and the lane part's XOR is shared by all accesses with the same g.
 struct Swz8 {
   __device__ __forceinline__ int operator()(int i) const {
     const int t = (i >> 3) & 3;
-    return i ^ (t | ((t & 2) << 1));
+    return (i & ~7) | ((i & 7) ^ (t | ((t & 2) << 1)));
   }
 };
 
