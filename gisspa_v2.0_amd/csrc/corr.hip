@@ -75,6 +75,42 @@ __device__ __forceinline__ float2 zpack(float2 a, float2 b, float2 w) {
   const float2 wd = c_mul(w, df);
   return make_float2(sm.x - wd.y, sm.y + wd.x);
 }
+// Both packed values of a pair k != H - k: with sm = a + conj b, df = a - conj b, wd = w df,
+// Zc(k) = sm + i wd, and since exp(+2 pi i (H - k) / L) = -conj w, Zc(H - k) = conj sm + i conj wd.
+__device__ __forceinline__ void zpack2(float2 a, float2 b, float2 w, float2& zk, float2& zm) {
+  const float2 bc = c_conj(b);
+  const float2 sm = c_add(a, bc), df = c_sub(a, bc);
+  const float2 wd = c_mul(w, df);
+  zk = make_float2(sm.x - wd.y, sm.y + wd.x);
+  zm = make_float2(sm.x + wd.y, wd.x - sm.y);
+}
+
+// LDS layouts of the compact column buffer for the radix 4*4*20 plan with 16 lanes per column,
+// one per stage boundary (all XOR the slot's low four bits with bits of its 16-slot block, so
+// each stays a bijection of the column's 320 slots):
+//  0 (loads -> stage 1): natural order.  Stage 1 reads 16 consecutive slots per lane group.
+//  1 (stage 1 -> 2): i ^ ((i >> 4) & 3).  Stage 1 writes 4 t + r (+ 64 k) from lane t: bits 4-5
+//    of the slot are t >> 2, so the write lands on 4 (t & 3) + (r ^ (t >> 2)), sixteen distinct
+//    bank pairs; stage 2 reads t + 16 k, i.e. t ^ (k & 3) within the block.
+//  2 (stage 2 -> 3): i ^ (((i >> 4) & 3) << 2).  Stage 2 writes 16 (t >> 2) + (t & 3) + 4 r:
+//    (t & 3) + 4 (r ^ (t >> 2)) within the block, again sixteen distinct; stage 3 reads t + 16 k.
+//  3 (stage 3 -> write-out): LowXor16 by the column's mask (columns of a row gather on distinct
+//    banks).
+// Without layouts 1 and 2 the radix-4 writes of stages 1 and 2 fall four lanes to a bank pair.
+struct ColStageMaps {
+  int mask;
+  template <int S>
+  __device__ __forceinline__ int at(int i) const {
+    if constexpr (S == 0)
+      return i;
+    else if constexpr (S == 1)
+      return (i & ~15) | ((i & 15) ^ ((i >> 4) & 3));
+    else if constexpr (S == 2)
+      return (i & ~15) | ((i & 15) ^ (((i >> 4) & 3) << 2));
+    else
+      return (i & ~15) | ((i & 15) ^ mask);
+  }
+};
 
 // Column LDS layouts (LAY):
 //  0: column stride L + 1, slots in natural order, flat twiddle table in LDS.
@@ -86,7 +122,7 @@ __device__ __forceinline__ float2 zpack(float2 a, float2 b, float2 w) {
 //     different banks for the write-out gather), twiddles read from the global table: exactly
 //     40 KB at L = 320, so that four workgroups fit a CU.
 //  3: compact as 2, twiddles formed per lane by recurrence from one base per stage (registers;
-//     no table reads in the transform).
+//     no table reads in the transform), and the stage layouts of ColStageMaps.
 //  4: as 3, writing the row-packed layout Zp of the inter-pass buffer: the 16 columns of group g
 //     are 8 mirrored pairs (k, H - k) for k in [8g, 8g + 8) (k = 0 pairs the DC/Nyquist column
 //     with column H/2), and the write-out applies the real-to-complex packing of the row
@@ -222,9 +258,11 @@ __global__ void __launch_bounds__(CorrCfg<L>::CT)
   const int col = tid / TPC;
   if constexpr (BK)
     ct::run_stockham<L, TPC, +1, ct::Sync::WAVE, ct::Swz16, TS>(buf + col * LP, tw, tid % TPC, P{});
-  else if constexpr (RC)
-    ct::run_stockham_fm<L, TPC, +1, ct::Sync::WAVE, ct::IdentityMap, ct::LowXor16, ct::TwSrc::RECUR>(
-        buf + col * LP, tw, tid % TPC, P{}, ct::IdentityMap{}, ct::LowXor16{cmask(col)});
+  else if constexpr (RC) {
+    static_assert(std::is_same_v<P, ct::RadixList<4, 4, 20>> && TPC == 16, "stage layouts of the 4*4*20 plan");
+    ct::run_stockham_maps<L, TPC, +1, ct::Sync::WAVE, ct::TwSrc::RECUR>(buf + col * LP, tw, tid % TPC, P{},
+                                                                         ColStageMaps{cmask(col)});
+  }
   else if constexpr (CM)
     ct::run_stockham_fm<L, TPC, +1, ct::Sync::WAVE, ct::IdentityMap, ct::LowXor16, ct::TwSrc::FLAT>(
         buf + col * LP, tw, tid % TPC, P{}, ct::IdentityMap{}, ct::LowXor16{cmask(col)});
@@ -254,8 +292,8 @@ __global__ void __launch_bounds__(CorrCfg<L>::CT)
     static_assert(NT % 8 == 0 && L % (NT / 8) == 0 && COLS == 16, "packed write-out mapping");
     constexpr int RPP = NT / 8, NWP = L / RPP;
     const int jl = tid % 8, y0 = tid / 8;
-    const int j = 8 * cg + jl, k2 = j == 0 ? H / 2 : H - j;
-    const float2 w1 = c_conj(twL[j]), w2 = c_conj(twL[k2]);  // exp(+2 pi i k / L)
+    const int j = 8 * cg + jl;
+    const float2 w1 = c_conj(twL[j]);  // exp(+2 pi i j / L)
     float4* const Zq = reinterpret_cast<float4*>(Z + ((long long)tl * nb + b) * (long long)L * H + 16 * cg) + jl;
     float2 av[NWP], bv[NWP];
 #pragma unroll
@@ -263,14 +301,28 @@ __global__ void __launch_bounds__(CorrCfg<L>::CT)
       av[i] = buf[jl * LP + slot_out(jl, y0 + i * RPP)];
       bv[i] = buf[(8 + jl) * LP + slot_out(8 + jl, y0 + i * RPP)];
     }
+    // pair (j, H - j): both packed values from one sum, difference and product.  Group 0 (a
+    // uniform branch) also holds, in lane 0, the DC/Nyquist column's real pair (X(0), X(H)) and
+    // the self-paired column H/2, whose packing zpack(b, b, i) is exactly (2 b.x, -2 b.y).
+    auto pack_rows = [&](auto first) {
 #pragma unroll
-    for (int i = 0; i < NWP; ++i) {
-      const int y = y0 + i * RPP;
-      // j = 0: the DC/Nyquist column's real pair (X(0), X(H)) and the self-paired column H/2
-      const float2 z1 = j == 0 ? make_float2(av[i].x + av[i].y, av[i].x - av[i].y) : zpack(av[i], bv[i], w1);
-      const float2 z2 = zpack(bv[i], j == 0 ? bv[i] : av[i], w2);
-      st_stream(Zq + (long long)(y < L / 2 ? y + L / 2 : y - L / 2) * (H / 2), make_float4(z1.x, z1.y, z2.x, z2.y));
-    }
+      for (int i = 0; i < NWP; ++i) {
+        const int y = y0 + i * RPP;
+        float2 z1, z2;
+        zpack2(av[i], bv[i], w1, z1, z2);
+        if constexpr (decltype(first)::value) {
+          if (jl == 0) {
+            z1 = make_float2(av[i].x + av[i].y, av[i].x - av[i].y);
+            z2 = make_float2(2.f * bv[i].x, -2.f * bv[i].y);
+          }
+        }
+        st_stream(Zq + (long long)(y < L / 2 ? y + L / 2 : y - L / 2) * (H / 2), make_float4(z1.x, z1.y, z2.x, z2.y));
+      }
+    };
+    if (cg == 0)
+      pack_rows(std::true_type{});
+    else
+      pack_rows(std::false_type{});
   }
 }
 
@@ -549,7 +601,8 @@ __global__ void __launch_bounds__(256)
   constexpr int ROWS = RW * WAVES;
   constexpr int PPL = RW * L / 64;
   constexpr int HS = BK ? H + 8 : H;  // LDS slots per transform
-  static_assert(TPW % TPI == 0 && RW * L % 64 == 0, "wave row mapping");
+  static_assert(TPW % TPI == 0 && RW * L % 64 == 0 && L % 64 == 0, "wave row mapping");
+  static_assert((TPT & (TPT - 1)) == 0 && H % TPT == 0, "mirror slots: power-of-two lanes per transform");
   static_assert((H / 2) % TPT == 0, "paired packing");
   static_assert(!BK || (TPT == 8 && H % 8 == 0 && ct::stage_twiddle_count(typename C::RowPlan{}) <= H),
                 "bank-conflict-free layout: 8 lanes per transform");
@@ -587,64 +640,75 @@ __global__ void __launch_bounds__(256)
   constexpr int NP = H / 2 / TPT;
   float2 pa[PKR ? 1 : NP], pb[PKR ? 1 : NP], p0 = make_float2(0.f, 0.f);
   float4 pq[PKR ? NP : 1];
-  // Lane-dependent offsets are derived from an opaque copy of the lane index inside the loop:
-  // hoisted out of it, the swizzled LDS addresses and the Z offsets of every unrolled access
-  // would each stay live in a register across the loop (about 70 registers), halving the
-  // number of resident wavefronts.
-  auto opaque_lane = [&]() {
-    int ln = lane;
-    asm volatile("" : "+v"(ln));
-    return ln & 63;  // restores the known range (non-negative, below 64) for the index arithmetic
-  };
+  // Lane part of the Z offsets (template ti of the iteration, row zrow of the wave's rows, pair
+  // group zsub), computed once: per iteration only the uniform template base moves.
+  const int zsub = lane % TPT, zrow = lane / TPT / TPI;
+  const long long zlane = ((long long)ti * nb * L + zrow) * H;
   auto prefetch = [&](int t) {
-    const int lnp = opaque_lane();
-    const int sub = lnp % TPT, ti = (lnp / TPT) % TPI, rowl = lnp / TPT / TPI;
     if (t + ti < tc) {
-      const float2* zr = Z + ((((long long)(t + ti) * nb + b) * L) + y0 + rowl) * H;
+      const float2* zr = Z + (((long long)t * nb + b) * L + y0) * H + zlane;
       if constexpr (PKR) {
 #pragma unroll
-        for (int m = 0; m < NP; ++m) pq[m] = ld_stream(reinterpret_cast<const float4*>(zr) + sub + TPT * m);
+        for (int m = 0; m < NP; ++m) pq[m] = ld_stream(reinterpret_cast<const float4*>(zr) + zsub + TPT * m);
       } else {
-        if (sub == 0) p0 = BK ? ld_stream(zr) : zr[0];
+        if (zsub == 0) p0 = BK ? ld_stream(zr) : zr[0];
 #pragma unroll
         for (int m = 0; m < NP; ++m) {
-          const int k = 1 + sub + TPT * m;  // 1 .. H/2
+          const int k = 1 + zsub + TPT * m;  // 1 .. H/2
           pa[m] = BK ? ld_stream(zr + k) : zr[k];
           pb[m] = BK ? ld_stream(zr + H - k) : zr[H - k];
         }
       }
     }
   };
-  if constexpr (PF) prefetch(0);
-  for (int t = 0; t < tc; t += TPI) {
+  // LDS addresses are derived from an opaque copy of the lane index inside the loop: hoisted out
+  // of it, the swizzled address of every unrolled access would stay live in a register across
+  // the loop (about 70 registers), halving the number of resident wavefronts.
+  auto opaque_lane = [&]() {
+    int ln = lane;
+    asm volatile("" : "+v"(ln));
+    return ln & 63;  // restores the known range (non-negative, below 64) for the index arithmetic
+  };
+  // One wave iteration over templates t .. t + TPI - 1 (FULL: all of them below tc).  Every
+  // address below is a lane part shared by many accesses plus a compile-time offset.
+  auto iteration = [&](int t, auto full) {
+    constexpr bool FULL = decltype(full)::value;
     if constexpr (!PF) prefetch(t);
     const int ln = opaque_lane();
     const int sub = ln % TPT;
     float2* const dst = wb + (ln / TPT) * HS;
-    if (PKR && t + ti < tc) {
+    if (FULL || t + ti < tc) {
+      if constexpr (PKR) {
+        // pair j = sub + TPT m: columns j and H - j (H/2 for j = 0).  For sub != 0 the mirror
+        // H - j is TPT (H/TPT - 1 - m) + lp with lp = TPT - sub, a lane part below TPT on a
+        // compile-time block; for sub == 0 it is a constant.
+        const int lp = (TPT - sub) & (TPT - 1);
 #pragma unroll
-      for (int m = 0; m < NP; ++m) {  // pair j = sub + TPT m: columns j and H - j (H/2 for j = 0)
-        const int j = sub + TPT * m;
-        dst[map(j)] = make_float2(pq[m].x, pq[m].y);
-        dst[map(j == 0 ? H / 2 : H - j)] = make_float2(pq[m].z, pq[m].w);
-      }
-    } else if (!PKR && t + ti < tc) {
-      if (sub == 0) dst[0] = make_float2(p0.x + p0.y, p0.x - p0.y);  // X(0), X(H) real, slot 0
-#pragma unroll
-      for (int m = 0; m < NP; ++m) {
-        const int k = 1 + sub + TPT * m;
-        const float2 A = pa[m], Bv = pb[m];
-        {
-          const float2 Bc = c_conj(Bv);
-          const float2 sm = c_add(A, Bc), df = c_sub(A, Bc);
-          const float2 wd = c_mul(twc[k], df);
-          dst[map(k)] = make_float2(sm.x - wd.y, sm.y + wd.x);
+        for (int m = 0; m < NP; ++m) {
+          dst[map(sub + TPT * m)] = make_float2(pq[m].x, pq[m].y);
+          const int base = TPT * (H / TPT - 1 - m);
+          const int s1 = map(base + lp) - base;
+          const int s0 = map(m == 0 ? H / 2 : H - TPT * m) - base;
+          dst[base + (sub != 0 ? s1 : s0)] = make_float2(pq[m].z, pq[m].w);
         }
-        if (k != H - k) {
-          const float2 Ac = c_conj(A);
-          const float2 sm = c_add(Bv, Ac), df = c_sub(Bv, Ac);
-          const float2 wd = c_mul(twc[H - k], df);
-          dst[map(H - k)] = make_float2(sm.x - wd.y, sm.y + wd.x);
+      } else {
+        if (sub == 0) dst[0] = make_float2(p0.x + p0.y, p0.x - p0.y);  // X(0), X(H) real, slot 0
+#pragma unroll
+        for (int m = 0; m < NP; ++m) {
+          const int k = 1 + sub + TPT * m;
+          const float2 A = pa[m], Bv = pb[m];
+          {
+            const float2 Bc = c_conj(Bv);
+            const float2 sm = c_add(A, Bc), df = c_sub(A, Bc);
+            const float2 wd = c_mul(twc[k], df);
+            dst[map(k)] = make_float2(sm.x - wd.y, sm.y + wd.x);
+          }
+          if (k != H - k) {
+            const float2 Ac = c_conj(A);
+            const float2 sm = c_add(Bv, Ac), df = c_sub(Bv, Ac);
+            const float2 wd = c_mul(twc[H - k], df);
+            dst[map(H - k)] = make_float2(sm.x - wd.y, sm.y + wd.x);
+          }
         }
       }
     }
@@ -655,35 +719,34 @@ __global__ void __launch_bounds__(256)
     else
       ct::run_stockham<H, TPT, +1, ct::Sync::WAVE>(dst, twf, sub, typename C::RowPlan{});
     const float* out = reinterpret_cast<const float*>(wb);
-    // Fold of the iteration's outputs into the lane's pixels.  Branch-free (the argmax update is a
-    // select): written with an `if`, the compiler made every (pixel, template) update a
-    // divergent branch with its LDS read waited on alone.  Only a launch's last iteration can
-    // hold fewer than TPI templates; it takes the checked path.
-    auto fold = [&](auto full) {
+    // Fold of the iteration's outputs into the lane's pixels, branch-free (the argmax update is a
+    // select; written with an `if`, every (pixel, template) update became a divergent branch
+    // with its LDS read waited on alone).  Pixel i of the lane is x = ln + 64 (i % (L/64)) of
+    // row i / (L/64); its real output sits in complex slot x / 2 = ln / 2 + 32 (i % (L/64)), and
+    // map(a + 32 c) = map(a) + 32 c, so one lane part serves every pixel.
+    const int lx = 2 * map(ln >> 1) + (ln & 1);
 #pragma unroll
-      for (int i = 0; i < PPL; ++i) {
-        const int q = ln + 64 * i;
-        const int r2 = q / L, x = q - r2 * L;
-        const int xo = 2 * map(x >> 1) + (x & 1);  // real output x sits in complex slot x / 2
+    for (int i = 0; i < PPL; ++i) {
+      const int r2 = i / (L / 64), xo = 64 * (i % (L / 64)) + lx;
 #pragma unroll
-        for (int u = 0; u < TPI; ++u) {
-          if (decltype(full)::value || t + u < tc) {
-            const float v = out[(r2 * TPI + u) * 2 * HS + xo];
-            s[i] += v;
-            ss[i] = fmaf(v, v, ss[i]);
-            const bool gt = v > bs[i];  // strict: the first maximum in template order is kept
-            bs[i] = gt ? v : bs[i];
-            ag[i] = gt ? t + u : ag[i];
-          }
+      for (int u = 0; u < TPI; ++u) {
+        if (FULL || t + u < tc) {
+          const float v = out[(r2 * TPI + u) * 2 * HS + xo];
+          s[i] += v;
+          ss[i] = fmaf(v, v, ss[i]);
+          const bool gt = v > bs[i];  // strict: the first maximum in template order is kept
+          bs[i] = gt ? v : bs[i];
+          ag[i] = gt ? t + u : ag[i];
         }
       }
-    };
-    if (t + TPI <= tc)
-      fold(std::true_type{});
-    else
-      fold(std::false_type{});
+    }
     ct::lds_sync<ct::Sync::WAVE>();
-  }
+  };
+  if constexpr (PF) prefetch(0);
+  // full iterations, then the launch's partial last one (if any) on the checked path
+  const int tfull = tc - tc % TPI;
+  for (int t = 0; t < tfull; t += TPI) iteration(t, std::true_type{});
+  if (tfull < tc) iteration(tfull, std::false_type{});
 
 #pragma unroll
   for (int i = 0; i < PPL; ++i) {

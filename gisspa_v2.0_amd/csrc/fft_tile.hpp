@@ -1,10 +1,12 @@
 // Compile-time FFT building blocks for the l x l correlation transforms.
 //
-// * dft<R, SIGN>: length-R DFT held in registers (explicit 2/3/4/5, split composite radices
-//   R = A*B by an in-register Cooley-Tukey step, direct sums for other primes).  Twiddles are
-//   constexpr values, so every butterfly compiles to straight-line FMA code.
+// * Dft<R, SIGN>: length-R DFT held in registers (explicit 2/3/4/5; composite radices by the
+//   prime-factor algorithm where R splits into coprime factors, else by an in-register
+//   Cooley-Tukey step; direct sums for other primes).  Twiddles are constexpr values, so every
+//   butterfly compiles to straight-line FMA code.
 // * stockham<N, T, SIGN, R...>: self-sorting Stockham stages over an LDS-resident transform
-//   of length N shared by T threads (Govindaraju et al., SC'08).  Each stage reads all of its
+//   of length N shared by T threads (Govindaraju et al., SC'08), with an LDS slot map per stage
+//   boundary.  Each stage reads all of its
 //   inputs into registers, synchronises, writes its outputs in place and synchronises again, so
 //   one LDS buffer per transform suffices.  Stages whose butterfly count divides T compile
 //   without predication.
@@ -98,6 +100,19 @@ constexpr int smallest_factor(int n) {
     if (n % p == 0) return p;
   return n;
 }
+// Largest power of n's smallest prime that divides n (n itself for prime powers).
+constexpr int prime_power_part(int n) {
+  const int p = smallest_factor(n);
+  int q = 1;
+  while (n % (q * p) == 0) q *= p;
+  return q;
+}
+// Inverse of a modulo m (gcd(a, m) = 1, m > 1).
+constexpr int inv_mod(int a, int m) {
+  for (int x = 1; x < m; ++x)
+    if ((a * x) % m == 1) return x;
+  return 1;  // m == 1
+}
 
 template <int R, int SIGN>
 struct Dft;
@@ -164,11 +179,18 @@ struct Dft<5, SIGN> {
   }
 };
 
-// Composite R = A * B (A = smallest factor) or a prime handled by direct summation.
+// Composite R or a prime handled by direct summation.  R = P * Q with coprime P, Q > 1 (P the
+// prime-power part of the smallest prime) uses the prime-factor algorithm (Good-Thomas): input
+// n = (Q n1 + P n2) mod R, output k = (Q (Q^-1 mod P) k1 + P (P^-1 mod Q) k2) mod R, so that
+// X(k) = sum_n2 W_Q^(n2 k2) sum_n1 x(n) W_P^(n1 k1) with no twiddle factors between the two
+// sets of DFTs (the index maps are register renames).  Prime powers R = A * B (A = smallest
+// factor) take one Cooley-Tukey step with twiddles W_R^(a k2).
 template <int R, int SIGN>
 struct Dft {
   static constexpr int A = smallest_factor(R);
   static constexpr int B = R / A;
+  static constexpr int P = prime_power_part(R);
+  static constexpr int Q = R / P;
 
   template <int K, int N>
   __device__ __forceinline__ static void direct_acc(const float2* v, float2& acc) {
@@ -206,6 +228,24 @@ struct Dft {
       direct_all<0>(v, o);
 #pragma unroll
       for (int k = 0; k < R; ++k) v[k] = o[k];
+    } else if constexpr (Q > 1) {  // prime-factor algorithm
+      constexpr int t1 = inv_mod(Q % P, P), t2 = inv_mod(P % Q, Q);
+      float2 y[Q][P];
+#pragma unroll
+      for (int n2 = 0; n2 < Q; ++n2) {
+#pragma unroll
+        for (int n1 = 0; n1 < P; ++n1) y[n2][n1] = v[(Q * n1 + P * n2) % R];
+        Dft<P, SIGN>::run(y[n2]);
+      }
+#pragma unroll
+      for (int k1 = 0; k1 < P; ++k1) {
+        float2 z[Q];
+#pragma unroll
+        for (int n2 = 0; n2 < Q; ++n2) z[n2] = y[n2][k1];
+        Dft<Q, SIGN>::run(z);
+#pragma unroll
+        for (int k2 = 0; k2 < Q; ++k2) v[(Q * t1 * k1 + P * t2 * k2) % R] = z[k2];
+      }
     } else {
       // X[k2 + B k1] = sum_a W_A^{a k1} W_R^{a k2} sum_b v[a + A b] W_B^{b k2}
       float2 y[A][B];
@@ -410,12 +450,29 @@ __device__ __forceinline__ void load_lane_twiddles(float2* w, const float2* flat
   load_lane_twiddles_impl<N, T, 1, 0, RS...>(w, flat, tid);
 }
 
+// Slot maps of a transform's LDS buffer per layout: layout 0 is what the first stage reads,
+// layout s (0 < s < number of stages) what stage s - 1 writes and stage s reads, and the last
+// layout what the last stage writes (the layout its consumer reads).  Maps must be bijections
+// of the transform's slots.  InOutMaps: one map for every layout but the last.
+template <typename M, typename MF, int NST>
+struct InOutMaps {
+  M map;
+  MF fmap;
+  template <int S>
+  __device__ __forceinline__ int at(int i) const {
+    if constexpr (S == NST)
+      return fmap(i);
+    else
+      return map(i);
+  }
+};
+
 // One Stockham stage (radix R, NS = product of earlier radices) over a length-N transform in
 // buf, shared by T threads (tid in [0,T)); then recurse into the remaining radices.  OFF is the
-// offset of this stage's entries in the STAGE or LANE twiddle source.  Slots are read and written
-// through `map`, except that the last stage writes through `fmap` (the layout its consumer reads).
-template <int N, int T, int SIGN, int NS, Sync S, typename M, typename MF, TwSrc TS, int OFF, int R, int... REST>
-__device__ __forceinline__ void stockham(float2* buf, const float2* tw, int tid, M map, MF fmap) {
+// offset of this stage's entries in the STAGE or LANE twiddle source; SI is the stage's index:
+// it reads through maps.at<SI> and writes through maps.at<SI + 1>.
+template <int N, int T, int SIGN, int NS, Sync S, typename MS, TwSrc TS, int OFF, int SI, int R, int... REST>
+__device__ __forceinline__ void stockham(float2* buf, const float2* tw, int tid, const MS& maps) {
   constexpr int NB = N / R;
   constexpr int IT = (NB + T - 1) / T;
   constexpr bool even = (NB % T) == 0;
@@ -428,7 +485,7 @@ __device__ __forceinline__ void stockham(float2* buf, const float2* tw, int tid,
       float2 wr = make_float2(1.f, 0.f);  // RECUR: w_1^r
 #pragma unroll
       for (int r = 0; r < R; ++r) {
-        float2 a = buf[map(j + r * NB)];
+        float2 a = buf[maps.template at<SI>(j + r * NB)];
         if constexpr (NS > 1) {
           if (r > 0) {
             if constexpr (TS == TwSrc::RECUR) {
@@ -456,41 +513,38 @@ __device__ __forceinline__ void stockham(float2* buf, const float2* tw, int tid,
       const int jm = j % NS;
       const int idxD = (j / NS) * NS * R + jm;
 #pragma unroll
-      for (int r = 0; r < R; ++r) {
-        if constexpr (sizeof...(REST) > 0)
-          buf[map(idxD + r * NS)] = v[it][r];
-        else
-          buf[fmap(idxD + r * NS)] = v[it][r];
-      }
+      for (int r = 0; r < R; ++r) buf[maps.template at<SI + 1>(idxD + r * NS)] = v[it][r];
     }
   }
   lds_sync<S>();
   constexpr int OFF2 =
       OFF + (NS > 1 ? (TS == TwSrc::LANE ? R - 1 : (TS == TwSrc::RECUR ? 1 : (R - 1) * NS)) : 0);
   if constexpr (sizeof...(REST) > 0)
-    stockham<N, T, SIGN, NS * R, S, M, MF, TS, OFF2, REST...>(buf, tw, tid, map, fmap);
+    stockham<N, T, SIGN, NS * R, S, MS, TS, OFF2, SI + 1, REST...>(buf, tw, tid, maps);
+}
+
+template <int N, int T, int SIGN, Sync S, TwSrc TS, typename MS, int... RS>
+__device__ __forceinline__ void run_stockham_maps(float2* buf, const float2* tw, int tid, RadixList<RS...>,
+                                                  const MS& maps) {
+  static_assert(RadixList<RS...>::product() == N, "radix plan must multiply to N");
+  static_assert(S == Sync::BLOCK || (64 % T == 0), "wave-level sync needs transforms inside one wavefront");
+  static_assert((TS != TwSrc::LANE && TS != TwSrc::RECUR) || lane_twiddles_fit<T>(RadixList<RS...>{}),
+                "lane twiddles need T % NS == 0");
+  stockham<N, T, SIGN, 1, S, MS, TS, 0, 0, RS...>(buf, tw, tid, maps);
 }
 
 template <int N, int T, int SIGN, Sync S = Sync::BLOCK, typename M = IdentityMap,
           TwSrc TS = TwSrc::FLAT, int... RS>
 __device__ __forceinline__ void run_stockham(float2* buf, const float2* tw, int tid,
-                                             RadixList<RS...>, M map = M{}) {
-  static_assert(RadixList<RS...>::product() == N, "radix plan must multiply to N");
-  static_assert(S == Sync::BLOCK || (64 % T == 0), "wave-level sync needs transforms inside one wavefront");
-  static_assert((TS != TwSrc::LANE && TS != TwSrc::RECUR) || lane_twiddles_fit<T>(RadixList<RS...>{}),
-                "lane twiddles need T % NS == 0");
-  stockham<N, T, SIGN, 1, S, M, M, TS, 0, RS...>(buf, tw, tid, map, map);
+                                             RadixList<RS...> plan, M map = M{}) {
+  run_stockham_maps<N, T, SIGN, S, TS>(buf, tw, tid, plan, InOutMaps<M, M, sizeof...(RS)>{map, map});
 }
 
 // As run_stockham, with the last stage writing through fmap instead of map.
 template <int N, int T, int SIGN, Sync S, typename M, typename MF, TwSrc TS, int... RS>
-__device__ __forceinline__ void run_stockham_fm(float2* buf, const float2* tw, int tid, RadixList<RS...>, M map,
+__device__ __forceinline__ void run_stockham_fm(float2* buf, const float2* tw, int tid, RadixList<RS...> plan, M map,
                                                 MF fmap) {
-  static_assert(RadixList<RS...>::product() == N, "radix plan must multiply to N");
-  static_assert(S == Sync::BLOCK || (64 % T == 0), "wave-level sync needs transforms inside one wavefront");
-  static_assert((TS != TwSrc::LANE && TS != TwSrc::RECUR) || lane_twiddles_fit<T>(RadixList<RS...>{}),
-                "lane twiddles need T % NS == 0");
-  stockham<N, T, SIGN, 1, S, M, MF, TS, 0, RS...>(buf, tw, tid, map, fmap);
+  run_stockham_maps<N, T, SIGN, S, TS>(buf, tw, tid, plan, InOutMaps<M, MF, sizeof...(RS)>{map, fmap});
 }
 
 }  // namespace ct

@@ -108,10 +108,12 @@ def test_norm_picks_identical(norm_run):
     np.testing.assert_array_equal(g["psi"], o["psi"])
 
 
-@pytest.mark.parametrize("nt", [1, 5])
+@pytest.mark.parametrize("nt", [1, 5, 11])
 def test_norm_odd_template_counts(gs, orc, nt):
     """Templates are rotated and transformed in pairs (real and imaginary parts of one plane); with
-    an odd count the last one has no partner.  Per-pixel z and picks must still match the oracle."""
+    an odd count the last one has no partner.  Per-pixel z and picks must still match the oracle.
+    With 11 templates the row pass (eight templates per wave iteration) runs one full iteration
+    and a partial one in the same launch."""
     import synth
 
     cfg = gs.SearchConfig(phistep=10.0, window_size=320, norm_type=1, nms_radius=64.0)
