@@ -67,15 +67,8 @@ struct ColPlanOf<320, 1> {
 
 // Real-to-complex packing of the row transform for one Hermitian pair of a row: from X(k) = a
 // and X(H - k) = b, Zc(k) = (a + conj b) + i w (a - conj b) with w = exp(+2 pi i k / L), whose
-// length-H inverse transform gives the row's real outputs two at a time (Zc(H - k) is
-// zpack(b, a, exp(+2 pi i (H - k) / L))).
-__device__ __forceinline__ float2 zpack(float2 a, float2 b, float2 w) {
-  const float2 bc = c_conj(b);
-  const float2 sm = c_add(a, bc), df = c_sub(a, bc);
-  const float2 wd = c_mul(w, df);
-  return make_float2(sm.x - wd.y, sm.y + wd.x);
-}
-// Both packed values of a pair k != H - k: with sm = a + conj b, df = a - conj b, wd = w df,
+// length-H inverse transform gives the row's real outputs two at a time.  zpack2 forms both
+// packed values of a pair k != H - k: with sm = a + conj b, df = a - conj b, wd = w df,
 // Zc(k) = sm + i wd, and since exp(+2 pi i (H - k) / L) = -conj w, Zc(H - k) = conj sm + i conj wd.
 __device__ __forceinline__ void zpack2(float2 a, float2 b, float2 w, float2& zk, float2& zm) {
   const float2 bc = c_conj(b);
@@ -126,7 +119,7 @@ struct ColStageMaps {
 //  4: as 3, writing the row-packed layout Zp of the inter-pass buffer: the 16 columns of group g
 //     are 8 mirrored pairs (k, H - k) for k in [8g, 8g + 8) (k = 0 pairs the DC/Nyquist column
 //     with column H/2), and the write-out applies the real-to-complex packing of the row
-//     transform (zpack below) to each pair and stores the two packed values next to each other:
+//     transform (zpack2 above) to each pair and stores the two packed values next to each other:
 //     Zp[..][y][2j + 0/1] = packed column j / H - j (H/2 for j = 0).  The row pass then starts
 //     from packed rows.
 // Z is written with streaming stores: it is read back only by the next kernel, and keeping it
@@ -303,7 +296,8 @@ __global__ void __launch_bounds__(CorrCfg<L>::CT)
     }
     // pair (j, H - j): both packed values from one sum, difference and product.  Group 0 (a
     // uniform branch) also holds, in lane 0, the DC/Nyquist column's real pair (X(0), X(H)) and
-    // the self-paired column H/2, whose packing zpack(b, b, i) is exactly (2 b.x, -2 b.y).
+    // the self-paired column H/2, whose packed value (a = b, w = exp(+2 pi i (H/2) / L) = i) is
+    // exactly (2 b.x, -2 b.y).
     auto pack_rows = [&](auto first) {
 #pragma unroll
       for (int i = 0; i < NWP; ++i) {

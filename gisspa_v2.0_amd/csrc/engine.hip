@@ -489,6 +489,12 @@ class Engine {
     GS_REQUIRE(P_.norm_type == 1, kUnsupported, "autotune covers the norm_type=1 correlation kernels");
     GS_REQUIRE(n_ > 0, kState, "no templates loaded");
     GS_REQUIRE(have_image_, kState, "no image set");
+    if (!corr_supported(P_.window_size)) {  // one generic correlation: nothing to choose
+      if (col_out) *col_out = col_variant_;
+      if (row_out) *row_out = row_variant_;
+      if (chunk_out) *chunk_out = chunk_;
+      return;
+    }
     const int la = lim_angles_, lt = lim_templates_;
     const bool prof = profile_, graph = use_graph_;
     lim_angles_ = 1;
@@ -719,7 +725,6 @@ class Engine {
  private:
   void validate() {
     GS_REQUIRE(P_.window_size > 0 && P_.window_size % 2 == 0, kInvalid, "window_size must be even and positive");
-    GS_REQUIRE(corr_supported(P_.window_size), kUnsupported, "window_size must be 320 or 720 in this build");
     GS_REQUIRE(P_.phistep > 0, kInvalid, "phistep must be positive");
     GS_REQUIRE(P_.angpix > 0, kInvalid, "angpix must be positive");
     GS_REQUIRE(P_.overlap >= 0 && P_.overlap < P_.window_size, kInvalid, "overlap out of range");
@@ -906,6 +911,8 @@ class Engine {
   // down.  Budget: a quarter of the device memory or half of what was free when first asked,
   // whichever is smaller, halved for the double-buffered two-stream pipeline (512 templates x
   // 234 blocks at l = 320 is a 49 GB inter-pass buffer).
+  // The generic correlation (window sizes without tiled kernels) needs two buffers of (l/2 + 1)
+  // columns per plane instead of one of l/2, and never double-buffers.
   static constexpr int kMaxChunk = 512;
   int chunk_for(int nb) {
     if (chunk_ > 0) return chunk_;
@@ -914,8 +921,10 @@ class Engine {
       GS_HIP(hipMemGetInfo(&fr, &tot));
       mem_budget_ = std::min(0.25 * (double)tot, 0.5 * (double)fr);
     }
-    const double plane = (double)P_.window_size * (P_.window_size / 2) * sizeof(float2);
-    const double per_t = plane * std::max(nb, 1) * (overlap_ ? 2 : 1);
+    const int l = P_.window_size;
+    const double plane = corr_supported(l) ? (double)l * (l / 2) * sizeof(float2) * (overlap_ ? 2 : 1)
+                                           : 2.0 * (double)l * (l / 2 + 1) * sizeof(float2);
+    const double per_t = plane * std::max(nb, 1);
     return (int)std::max(4.0, std::min((double)kMaxChunk, std::floor(mem_budget_ / per_t)));
   }
 
@@ -955,10 +964,13 @@ class Engine {
     const int nt = lim_templates_ > 0 ? std::min(n, lim_templates_) : n;
     angles_ = angles;
     const int tc_max = chunk_for(nb_);
-    const size_t zsz = (size_t)tc_max * nb_ * l * (l / 2);
+    // window sizes without tiled kernels: the generic correlation with its two buffers
+    const bool gen = !corr_supported(l);
+    const size_t zsz = gen ? 2 * corr_generic_buffer(l, tc_max, nb_) : (size_t)tc_max * nb_ * l * (l / 2);
+    if (gen) corr_generic_prepare(fft_, l);
     // With overlap, the column pass of chunk c+1 (stream_) runs while the row pass of chunk c
     // (stream2_) consumes the other half of a double-buffered inter-pass buffer.
-    const bool overlap = overlap_;
+    const bool overlap = overlap_ && !gen;
     hipStream_t rs = stream_;
     if (overlap) {
       ensure_pipeline();
@@ -999,6 +1011,13 @@ class Engine {
           float2* Z = Zb + zb * zsz;
           if (overlap && chunk >= 2) GS_HIP(hipStreamWaitEvent(stream_, ev_zfree_[zb], 0));
           CorrArgs ca{that, ihat, Z, t0, tc, nb_};
+          if (gen) {
+            clock_begin(clk_col_, stream_);
+            launch_corr_generic_pixel(fft_, ca, l, Z, Z + zsz / 2, d_S_.p, d_SS_.p, d_best_.p, d_arg_.p, a * n + t0,
+                                      stream_);
+            clock_end(clk_col_, stream_);
+            continue;
+          }
           clock_begin(clk_col_, stream_);
           GS_REQUIRE(launch_colpass(l, ca, col_variant_, stream_), kUnsupported, "window_size not supported");
           clock_end(clk_col_, stream_);
@@ -1014,7 +1033,7 @@ class Engine {
         }
       }
     };
-    if (use_graph_ && !overlap && !profile_) {
+    if (use_graph_ && !overlap && !profile_ && !gen) {
       // The angle loop has no micrograph-dependent arguments: capture it once per problem shape
       // and buffer set, then replay it for every later micrograph of that shape.
       auto ptr = [](const void* p) { return (long long)(uintptr_t)p; };
@@ -1114,13 +1133,16 @@ class Engine {
     if (lim_angles_ > 0) na = std::min(na, lim_angles_);
     angles_ = angles;
     na_ = na;
-    const int groups = rowpass_groups(l);
+    const bool gen = !corr_supported(l);  // generic correlation: one partial per plane
+    const int groups = gen ? 1 : rowpass_groups(l);
+    if (gen) corr_generic_prepare(fft_, l);
     // Angles are processed in groups: the rotated blocks of ag angles form the "block" axis of
     // one correlation launch (ag * nbb planes per template), so small tiles still fill the GPU.
     const int ag_max = std::max(1, std::min(na, 2048 / std::max(nbb, 1)));
     const int nbp_max = ag_max * nbb;
     const int tc_max = std::min(n, chunk_for(nbp_max));
-    float2* Z = d_Z_.ensure((size_t)tc_max * nbp_max * l * (l / 2));
+    const size_t zsz = gen ? 2 * corr_generic_buffer(l, tc_max, nbp_max) : (size_t)tc_max * nbp_max * l * (l / 2);
+    float2* Z = d_Z_.ensure(zsz);
     float4* part = reinterpret_cast<float4*>(d_plane_part_.ensure((size_t)tc_max * nbp_max * groups * 4));
     // plane records, per tile [t][angle][b][4]
     const size_t per_tile = (size_t)n * na * nbb * 4;
@@ -1163,13 +1185,18 @@ class Engine {
           const int tc = std::min(tc_max, n - t0);
           CorrArgs ca{d_that_.p, ihat, Z, t0, tc, nbp};
           clock_begin(clk_col_, stream_);
-          // the norm_type=0 row pass reads the unpacked layout: variant 8 runs as 7 here
-          GS_REQUIRE(launch_colpass(l, ca, col_variant_ == 8 ? 7 : col_variant_, stream_), kUnsupported,
-                     "window_size not supported");
-          clock_end(clk_col_, stream_);
-          clock_begin(clk_row_, stream_);
-          launch_rowpass_plane(l, Z, tc, nbp, part, border, stream_);
-          clock_end(clk_row_, stream_);
+          if (gen) {
+            launch_corr_generic_plane(fft_, ca, l, Z, Z + zsz / 2, part, border, stream_);
+            clock_end(clk_col_, stream_);
+          } else {
+            // the norm_type=0 row pass reads the unpacked layout: variant 8 runs as 7 here
+            GS_REQUIRE(launch_colpass(l, ca, col_variant_ == 8 ? 7 : col_variant_, stream_), kUnsupported,
+                       "window_size not supported");
+            clock_end(clk_col_, stream_);
+            clock_begin(clk_row_, stream_);
+            launch_rowpass_plane(l, Z, tc, nbp, part, border, stream_);
+            clock_end(clk_row_, stream_);
+          }
           // record of (template t0+t, angle a0+q, block b) at tile base + ((t0+t)*na + a0)*nbb + q*nbb + b
           launch_finalize_plane(part, tc, nbp, groups, (long long)na * nbb,
                                 planes + tI * per_tile + ((size_t)t0 * na + a0) * nbb * 4, stream_);

@@ -7,6 +7,8 @@
 
 namespace gs {
 
+class FftCache;
+
 // Per-micrograph parameters (reference Parameters, include/EMReader/DataReader2.h:73-83, with
 // SetParams, src/norm.cu:183-192).  CTF quantities are kept in double: the CTF phase reaches
 // tens of radians on whole frames, and these kernels run once per micrograph.
@@ -96,7 +98,20 @@ bool launch_rowpass_pixel(int L, const float2* Z, int tc, int nb, double* S, dou
 bool launch_rowpass_plane(int L, const float2* Z, int tc, int nb, float4* partial, float border,
                           hipStream_t s);
 int rowpass_groups(int L);
+// True where the tiled correlation kernels exist (L = 320 and 720); any other even L runs the
+// generic path below.
 bool corr_supported(int L);
+// Generic correlation for any even L (corr_generic.hip): P and W are two caller buffers of
+// corr_generic_buffer(L, tc, nb) elements each; the pixel form updates the same per-pixel
+// records as launch_rowpass_pixel, the plane form writes one partial per (template, block)
+// (finalise with groups = 1).  corr_generic_prepare builds its plans and tables (nothing is
+// then allocated on later calls).
+size_t corr_generic_buffer(int L, int tc, int nb);
+void corr_generic_prepare(FftCache& cache, int L);
+void launch_corr_generic_pixel(FftCache& cache, const CorrArgs& a, int L, float2* P, float2* W, double* S,
+                               double* SS, float* best, int32_t* arg, int arg_base, hipStream_t s);
+void launch_corr_generic_plane(FftCache& cache, const CorrArgs& a, int L, float2* P, float2* W, float4* partial,
+                               float border, hipStream_t s);
 // Builds the correlation kernels' twiddle tables on the current device (they are otherwise
 // built on first use, which must not happen inside a stream capture).
 void corr_prepare();

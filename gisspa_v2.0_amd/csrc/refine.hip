@@ -288,12 +288,20 @@ __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
 // c and its derivatives for one pick per block.  The half spectrum (ky rows of HX entries) is
 // walked flat so that consecutive lanes read consecutive kx of A, B and of the block spectrum,
 // which is given transposed: ihat_t[b][ky][kx].
-template <int L>
+// LT = 0: any even l = lrt, with the phase tables in dynamic LDS (2 (l/2 + 1) + 2 l entries).
+template <int LT>
 __global__ __launch_bounds__(256) void k_ref_dot(const float2* __restrict__ ihat_t, const float2* __restrict__ AB,
                                                  const RefPeak* __restrict__ pk, const double* __restrict__ stats,
-                                                 int K, double* __restrict__ out) {
-  constexpr int HX = L / 2 + 1;
-  __shared__ float2 phx[HX], dphx[HX], phy[L], dphy[L];
+                                                 int K, double* __restrict__ out, int lrt) {
+  const int L = LT ? LT : lrt;
+  const int HX = L / 2 + 1;
+  constexpr int SX = LT ? LT / 2 + 1 : 1, SY = LT ? LT : 1;
+  __shared__ float2 sphx[SX], sdphx[SX], sphy[SY], sdphy[SY];
+  extern __shared__ float2 dyn[];
+  float2* const phx = LT ? sphx : dyn;
+  float2* const dphx = LT ? sdphx : dyn + HX;
+  float2* const phy = LT ? sphy : dyn + 2 * HX;
+  float2* const dphy = LT ? sdphy : dyn + 2 * HX + L;
   const int j = blockIdx.x;
   const RefPeak P = pk[j];
   const int px = P.pixel % L, py = P.pixel / L;
@@ -477,11 +485,14 @@ void launch_ref_rotate(const float2* H, const RefPeak* pk, int K, int l, float d
 bool launch_ref_dot(const float2* ihat, const float2* AB, const RefPeak* pk, const double* stats, int K, int l,
                     double* out, hipStream_t s) {
   if (l == 320) {
-    k_ref_dot<320><<<K, 256, 0, s>>>(ihat, AB, pk, stats, K, out);
+    k_ref_dot<320><<<K, 256, 0, s>>>(ihat, AB, pk, stats, K, out, l);
   } else if (l == 720) {
-    k_ref_dot<720><<<K, 256, 0, s>>>(ihat, AB, pk, stats, K, out);
+    k_ref_dot<720><<<K, 256, 0, s>>>(ihat, AB, pk, stats, K, out, l);
   } else {
-    return false;
+    if (l <= 0 || l % 2) return false;
+    const size_t lds = sizeof(float2) * (size_t)(2 * (l / 2 + 1) + 2 * l);
+    GS_REQUIRE(lds <= 64 * 1024, kUnsupported, "window_size too large for the refinement");
+    k_ref_dot<0><<<K, 256, lds, s>>>(ihat, AB, pk, stats, K, out, l);
   }
   GS_CHECK_LAUNCH();
   return true;

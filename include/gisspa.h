@@ -38,7 +38,11 @@ typedef enum {
  *   angpix, phistep, kk, energy, cs, Highres, Lowres, diameter, threshold,
  *   window_size, overlap, norm_type, invert, phase_flip.
  * nms_radius is new: radius (px) of the greedy non-maximum suppression applied to the
- * records before they are returned (0 returns every record above threshold). */
+ * records before they are returned (0 returns every record above threshold).
+ * window_size: any even size, as in the reference (norm_type=0: at most 720).  320 and 720 run
+ * the tiled correlation kernels; other sizes run a generic correlation on runtime-planned
+ * transforms (same results to float rounding, slower), for which the kernel variants,
+ * gs_autotune and the two-stream pipeline have no effect. */
 typedef struct {
   float angpix, phistep, kk, energy, cs, highres, lowres, diameter, threshold;
   int32_t window_size, overlap, norm_type, invert, phase_flip;
