@@ -96,7 +96,11 @@ Euler load_euler(const std::string& path) {
   return e;
 }
 
-bool read_mrc(const std::string& path, Mrc& out, std::string& err) {
+// Sections [first, first + count) of an MRC file (count < 0: all of them).  Modes as the
+// reference's readMRC (src/EMReader/emdata.cpp:412-677) converts them: 0 unsigned bytes
+// v / 100 - 1.28 (emdata.cpp:540-547), 1 int16, 2 float32, and 6 uint16 (which the reference
+// rejects); either byte order.
+static bool read_mrc_sections(const std::string& path, int first, int count, Mrc& out, std::string& err) {
   std::ifstream f(path, std::ios::binary);
   if (!f) {
     err = "cannot open " + path;
@@ -113,32 +117,38 @@ bool read_mrc(const std::string& path, Mrc& out, std::string& err) {
     swap = true;
     for (int i = 0; i < 256; ++i) hdr[i] = bs(hdr[i]);
   }
-  out.nx = hdr[0];
-  out.ny = hdr[1];
-  out.nz = hdr[2];
+  const int nx = hdr[0], ny = hdr[1], nz = hdr[2];
   const int mode = hdr[3];
   const int ext = hdr[23];
-  if (out.nx <= 0 || out.ny <= 0 || out.nz <= 0 || ext < 0) {
+  if (nx <= 0 || ny <= 0 || nz <= 0 || ext < 0) {
     err = "bad MRC dimensions: " + path;
     return false;
   }
-  const size_t n = (size_t)out.nx * out.ny * out.nz;
+  if (count < 0) first = 0, count = nz;
+  if (first < 0 || first + count > nz) {
+    err = path + " has " + std::to_string(nz) + " sections, " + std::to_string(first + count) + " needed";
+    return false;
+  }
   int bytes = mode == 0 ? 1 : (mode == 1 || mode == 6) ? 2 : mode == 2 ? 4 : 0;
   if (!bytes) {
     err = "unsupported MRC mode " + std::to_string(mode) + ": " + path;
     return false;
   }
-  f.seekg(1024 + ext);
+  const size_t sec = (size_t)nx * ny, n = sec * count;
+  f.seekg(1024 + (std::streamoff)ext + (std::streamoff)(sec * first * bytes));
   std::vector<char> raw(n * bytes);
   if (!f.read(raw.data(), raw.size())) {
     err = "truncated MRC data: " + path;
     return false;
   }
+  out.nx = nx;
+  out.ny = ny;
+  out.nz = count;
   out.data.resize(n);
   for (size_t i = 0; i < n; ++i) {
     const char* p = raw.data() + i * bytes;
     if (mode == 0) {
-      out.data[i] = (float)(int8_t)p[0];
+      out.data[i] = (float)((double)(uint8_t)p[0] / 100.0 - 1.28);
     } else if (mode == 1 || mode == 6) {
       uint16_t u;
       std::memcpy(&u, p, 2);
@@ -154,6 +164,27 @@ bool read_mrc(const std::string& path, Mrc& out, std::string& err) {
     }
   }
   return true;
+}
+
+bool read_mrc(const std::string& path, Mrc& out, std::string& err) { return read_mrc_sections(path, 0, -1, out, err); }
+
+static int stack_size(const std::string& path) {
+  std::ifstream f(path, std::ios::binary);
+  int32_t hdr[4] = {0, 0, 0, 0};
+  if (!f.read(reinterpret_cast<char*>(hdr), 16)) return 0;
+  if (hdr[3] < 0 || hdr[3] > 16) hdr[2] = (int32_t)__builtin_bswap32((uint32_t)hdr[2]);
+  return hdr[2];
+}
+
+bool read_image(const std::string& path, int index, Mrc& out, std::string& err) {
+  if (is_hdf5(path)) return read_hdf5_images(path, index < 0 ? 0 : index, 1, out, err);
+  const int nz = stack_size(path);
+  return read_mrc_sections(path, (index >= 0 && index < nz) ? index : 0, 1, out, err);
+}
+
+bool read_stack(const std::string& path, int count, Mrc& out, std::string& err) {
+  if (is_hdf5(path)) return read_hdf5_images(path, 0, count, out, err);
+  return read_mrc_sections(path, 0, count, out, err);
 }
 
 }  // namespace gsio

@@ -1,13 +1,14 @@
 // gisspa_main <config>: the config-driven search executable (reference src/main.cu:14-98).
 //
 // Reads the config, the LST list of micrographs with CTF parameters, the Euler file and the
-// template stack, then searches micrographs [first, last) on one GPU and writes one record line
-// per pick in the reference's format (src/norm.cu:610-615).  Differences from the reference:
+// template stack (MRC or EMAN HDF5), then searches micrographs [first, last) on one GPU and
+// writes one record line per pick in the reference's format (src/norm.cu:610-615).  Differences from the reference:
 // GPU_ID = -1 uses device 0 (one GPU per process; shard micrographs with first/last), errors
 // are reported with a non-zero exit status instead of being printed and ignored, and records
 // pass through NMS when nms_radius > 0.
 #include <chrono>
 #include <cstdio>
+#include <cstdlib>
 #include <fstream>
 #include <string>
 #include <vector>
@@ -53,11 +54,45 @@ static int mrc_info(const char* path) {
   return 0;
 }
 
+// Dimensions and checksums of one image as the search reads a micrograph (MRC section or HDF5
+// image `index`; no GPU use).
+static int image_info(const char* path, int index) {
+  gsio::Mrc m;
+  std::string err;
+  if (!gsio::read_image(path, index, m, err)) return die(err);
+  double s = 0, s2 = 0;
+  for (float v : m.data) {
+    s += v;
+    s2 += (double)v * v;
+  }
+  std::printf("image %d %d %.17g %.17g\n", m.nx, m.ny, s, s2);
+  return 0;
+}
+
+// The same for images 0 .. count - 1 of a stack as the search reads its templates.
+static int stack_info(const char* path, int count) {
+  gsio::Mrc m;
+  std::string err;
+  if (!gsio::read_stack(path, count, m, err)) return die(err);
+  double s = 0, s2 = 0;
+  for (float v : m.data) {
+    s += v;
+    s2 += (double)v * v;
+  }
+  std::printf("stack %d %d %d %.17g %.17g\n", m.nx, m.ny, m.nz, s, s2);
+  return 0;
+}
+
 int main(int argc, char** argv) {
   if (argc < 2)
-    return die("usage: gisspa_main <config> | --parse-dump <config> | --mrc-info <file.mrc>");
+    return die("usage: gisspa_main <config> | --parse-dump <config> | --mrc-info <file.mrc> | "
+               "--image-info <file> [index] | --stack-info <file> <count>");
   if (std::string(argv[1]) == "--parse-dump") return argc > 2 ? parse_dump(argv[2]) : die("missing config");
   if (std::string(argv[1]) == "--mrc-info") return argc > 2 ? mrc_info(argv[2]) : die("missing file");
+  if (std::string(argv[1]) == "--image-info")
+    return argc > 2 ? image_info(argv[2], argc > 3 ? std::atoi(argv[3]) : 0) : die("missing file");
+  if (std::string(argv[1]) == "--stack-info")
+    return argc > 3 ? stack_info(argv[2], std::atoi(argv[3])) : die("missing file or count");
   gsio::Config conf;
   if (!conf.parse(argv[1])) return die(conf.errors.empty() ? "bad config" : conf.errors[0]);
   auto missing = conf.missing_required();
@@ -76,13 +111,11 @@ int main(int argc, char** argv) {
   int device = conf.geti("GPU_ID");
   if (device < 0) device = 0;
 
+  // one template per Euler line (Templates(path, euler.size()), src/main.cu:32)
   gsio::Mrc tm;
   std::string err;
-  if (!gsio::read_mrc(conf.gets("template"), tm, err)) return die(err);
-  if ((size_t)tm.nz < euler.size())
-    return die("template stack has " + std::to_string(tm.nz) + " images, Euler file " +
-               std::to_string(euler.size()));
   const int n = (int)euler.size();
+  if (!gsio::read_stack(conf.gets("template"), n, tm, err)) return die(err);
   std::vector<float> altaz(2 * n);
   for (int i = 0; i < n; ++i) {
     altaz[2 * i] = euler.alt[i];
@@ -124,7 +157,7 @@ int main(int argc, char** argv) {
   for (int i = first; i < last; ++i) {
     const auto& e = lst[i];
     gsio::Mrc img;
-    if (!gsio::read_mrc(e.rpath, img, err)) {
+    if (!gsio::read_image(e.rpath, e.unused, img, err)) {  // image number = LST index
       status = die(err);
       break;
     }

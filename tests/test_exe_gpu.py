@@ -58,3 +58,42 @@ def test_executable_matches_oracle(tmp_path, orc, gs):
         assert float(g[7]) == pytest.approx(psi, abs=1e-6)
         assert float(g[8]) == x and float(g[9]) == y
         assert abs(float(g[10]) - s) <= 1e-4 * abs(s) + 1e-6
+
+
+def test_executable_reads_hdf5_templates_and_stack_sections(tmp_path, gs):
+    """The same search from other file layouts: templates in an EMAN2 HDF5 stack and both
+    micrographs in one MRC stack, each read at its LST index (src/utils/image.cpp:9).  The
+    record lines must equal those of one-file-per-micrograph MRC inputs."""
+    import hdf5_fixture
+    import synth
+
+    ds = synth.make_dataset(31, 640, 640, n_templates=3, tsize=128, n_particles=3, snr=0.6,
+                            defocus=1.3, dfdiff=0.01, dfang=10.0)
+    templates, eul = ds.templates, ds.eulers
+    # two micrographs of the same particles: the image and its 180-degree rotation
+    images = [ds.image, np.ascontiguousarray(np.rot90(ds.image, 2))]
+    (tmp_path / "euler.txt").write_text("".join(f"{j} {a:.4f} {b:.4f} 0.0\n" for j, (a, b, _) in enumerate(eul)))
+    common = ("eulerfile = euler.txt\nangpix = 1.0\nphistep = 10\nkk = 3\nenergy = 300\ncs = 2.7\n"
+              "Highres = 8\nLowres = 100\ndiameter = 128\nthreshold = 7.0\nfirst = 0\nlast = 2\n"
+              "GPU_ID = 0\nwindow_size = 320\nnorm_type = 1\nnms_radius = 64\n")
+    ctf = "defocus=1.300000 dfdiff=0.010000 dfang=10.000000"
+    # reference layout: one MRC per micrograph, MRC template stack
+    synth.write_mrc(str(tmp_path / "templates.mrc"), templates)
+    for k, im in enumerate(images):
+        synth.write_mrc(str(tmp_path / f"m{k}.mrc"), im)
+    (tmp_path / "a.lst").write_text("".join(f"0 m{k}.mrc {ctf}\n" for k in range(2)))
+    (tmp_path / "a.txt").write_text("input = a.lst\ntemplate = templates.mrc\noutput = a.out\n" + common)
+    # other layout: EMAN2 HDF5 templates, one two-section MRC read at LST indices 0 and 1
+    hdf5_fixture.write(tmp_path / "templates.hdf", templates, eman1=False)
+    synth.write_mrc(str(tmp_path / "stack.mrc"), np.stack(images))
+    (tmp_path / "b.lst").write_text("".join(f"{k} stack.mrc {ctf}\n" for k in range(2)))
+    (tmp_path / "b.txt").write_text("input = b.lst\ntemplate = templates.hdf\noutput = b.out\n" + common)
+    for c in ("a.txt", "b.txt"):
+        r = subprocess.run([EXE, c], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, r.stderr
+    a = [LINE.match(l).groups() for l in (tmp_path / "a.out").read_text().splitlines()]
+    b = [LINE.match(l).groups() for l in (tmp_path / "b.out").read_text().splitlines()]
+    assert len(a) > 0 and len(a) == len(b)
+    for ga, gb in zip(a, b):  # identical except the micrograph index and path columns
+        assert ga[2:] == gb[2:]
+    assert [g[0] for g in b] == sorted(g[0] for g in b) and {g[0] for g in b} == {"0", "1"}

@@ -1,7 +1,11 @@
 """CPU checks of the data formats either side of the search and of the synthetic generator.
 
 * MRC reading in the executable (modes 0, 1, 2, 6, either byte order; emdata.cpp:412-677 is the
-  reference's reader): dimensions and checksums of files written here with numpy.
+  reference's reader, whose mode 0 maps unsigned bytes to v / 100 - 1.28): dimensions and
+  checksums of files written here with numpy; a micrograph is the section of its LST index
+  (image.cpp:9, emdata.cpp:516-524).
+* EMAN HDF5 stacks (emdata.cpp:192-220: EMAN2 /MDF/images/i/image, EMAN1 root datasets "i" under
+  a "num_dataset" attribute), written here through the HDF5 C library (skipped without it).
 * Template generation: the closed-form projector of isotropic Gaussian blobs against numerical
   integration along the beam, rotation invariance of a centred blob, and the ZXZ Euler
   convention; the harness CTF against the oracle's CTF_AST.
@@ -11,6 +15,8 @@ import subprocess
 
 import numpy as np
 import pytest
+
+import hdf5_fixture
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 EXE = os.path.join(ROOT, "gisspa_v2.0_amd", "lib", "gisspa_main")
@@ -48,9 +54,44 @@ def test_mrc_reader_modes(tmp_path, mode, big):
     _write_mrc(p, a, mode, big)
     out = subprocess.run([EXE, "--mrc-info", str(p)], capture_output=True, text=True, check=True).stdout.split()
     assert out[0] == "mrc" and [int(v) for v in out[1:4]] == [29, 17, 3]
-    v = a.astype(np.float32).astype(np.float64)
+    if mode == 0:  # the reference's conversion of (unsigned) bytes
+        v = ((a.astype(np.int64) & 0xFF) / 100.0 - 1.28).astype(np.float32).astype(np.float64)
+    else:
+        v = a.astype(np.float32).astype(np.float64)
     assert float(out[4]) == pytest.approx(v.sum(), rel=1e-12, abs=1e-9)
     assert float(out[5]) == pytest.approx((v * v).sum(), rel=1e-12)
+    # one section as a micrograph is read: index in range selects it, any other index section 0
+    for idx, sec in ((0, 0), (2, 2), (3, 0), (-1, 0)):
+        out = subprocess.run([EXE, "--image-info", str(p), str(idx)], capture_output=True, text=True,
+                             check=True).stdout.split()
+        assert out[0] == "image" and [int(x) for x in out[1:3]] == [29, 17]
+        assert float(out[3]) == pytest.approx(v[sec].sum(), rel=1e-12, abs=1e-9)
+        assert float(out[4]) == pytest.approx((v[sec] * v[sec]).sum(), rel=1e-12)
+
+
+@pytest.mark.parametrize("eman1", [False, True])
+def test_hdf5_stacks(tmp_path, eman1):
+    if not os.path.exists(EXE):
+        pytest.skip("executable not built")
+    stack = np.random.default_rng(5 + eman1).standard_normal((3, 17, 29)).astype(np.float32)
+    p = tmp_path / ("t1.hdf" if eman1 else "t2.hdf")
+    hdf5_fixture.write(p, stack, eman1)
+    # EMAN2 dims are [ny][nx]; EMAN1 as the reference reads them: nx = dims[0], ny = dims[1]
+    nx, ny = (17, 29) if eman1 else (29, 17)
+    v = stack.astype(np.float64)
+    for i in range(3):
+        out = subprocess.run([EXE, "--image-info", str(p), str(i)], capture_output=True, text=True,
+                             check=True).stdout.split()
+        assert out[0] == "image" and [int(x) for x in out[1:3]] == [nx, ny]
+        assert float(out[3]) == pytest.approx(v[i].sum(), rel=1e-12, abs=1e-9)
+        assert float(out[4]) == pytest.approx((v[i] * v[i]).sum(), rel=1e-12)
+    out = subprocess.run([EXE, "--stack-info", str(p), "3"], capture_output=True, text=True,
+                         check=True).stdout.split()
+    assert out[0] == "stack" and [int(x) for x in out[1:4]] == [nx, ny, 3]
+    assert float(out[4]) == pytest.approx(v.sum(), rel=1e-12, abs=1e-9)
+    # a stack shorter than asked for is an error, not a silent short read
+    r = subprocess.run([EXE, "--stack-info", str(p), "4"], capture_output=True, text=True)
+    assert r.returncode != 0 and "no image 3" in r.stderr
 
 
 def test_synth_mrc_round_trip(tmp_path):
