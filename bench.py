@@ -167,7 +167,9 @@ def batch_main(a):
     {128, 180, 256}, each micrograph with its own template set.  Three ways to run them:
     `per_engine` = the reference's driver loop (an engine created, loaded, run and destroyed per
     micrograph), `one_engine` = one engine and gs_search per micrograph, `batch` =
-    gs_search_batch (staging thread + graph replay).  value = correlations/s of `batch`."""
+    gs_search_batch (staging thread + graph replay).  value = correlations/s of `batch`.  The
+    batch + refinement form (gs_search_batch_refine, top REFINE_TOP picks per micrograph) is
+    compared with the same work run per micrograph (engine, gs_search, gs_refine)."""
     import gisspa
     import synth
 
@@ -226,6 +228,21 @@ def batch_main(a):
         times["batch_refine"] = time.perf_counter() - t1
     n_ref = int(sum(len(r) for r in refs_c4))
     log(f"batch + refinement of {n_ref} picks: {times['batch_refine']:.2f} s")
+    # the same work run sequentially per micrograph, as the plain search followed by the plain
+    # refinement would be: an engine per micrograph, gs_search, then gs_refine of its top picks
+    t1 = time.perf_counter()
+    refs_seq = []
+    for it in items:
+        with gisspa.Engine(cfg) as e:
+            e.load_templates(it["templates"])
+            r = e.search(it["image"], it["ctf"])[:REFINE_TOP]
+            refs_seq.append(e.refine(e.peaks_from_records(r)) if len(r) else None)
+    times["per_engine_refine"] = time.perf_counter() - t1
+    log(f"per-engine search + refinement: {times['per_engine_refine']:.2f} s")
+    seq_same = all((x is None and len(y) == 0) or
+                   (x is not None and np.array_equal(x["score"], y["score"]) and
+                    np.array_equal(x["peak"], y["peak"]))
+                   for x, y in zip(refs_seq, refs_c4))
     same = all(
         len(x) == len(y) and all(np.array_equal(x[n], y[n]) for n in x.dtype.names)
         for k in ("one_engine", "batch") for x, y in zip(recs["per_engine"], recs[k]))
@@ -257,6 +274,10 @@ def batch_main(a):
                 for x, y in zip(recs["batch"], recs_c4))),
             "mean_z_start": float(np.mean(np.concatenate([r["score0"] for r in refs_c4]))) if n_ref else None,
             "mean_z_refined": float(np.mean(np.concatenate([r["score"] for r in refs_c4]))) if n_ref else None,
+            "micrographs_per_s": a.batch / times["batch_refine"],
+            "micrographs_per_s_sequential": a.batch / times["per_engine_refine"],
+            "speedup_vs_sequential": times["per_engine_refine"] / times["batch_refine"],
+            "refined_identical_to_sequential": bool(seq_same),
         },
     }
     print(json.dumps(out), flush=True)
