@@ -1,0 +1,71 @@
+"""Search options the other GPU tests leave at their defaults, against the FP64 oracle through
+the C ABI: block overlap (norm_type=1 and 0), inverted contrast, phase flipping under
+norm_type=1, an odd template size, and overlap at a window size without tiled kernels.
+
+Bar: per-pixel z of the best record within 1e-4 and identical picks with scores within 1e-4
+(norm_type=1); identical records (norm_type=0).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _key(a):
+    return [(int(p["x"]), int(p["y"]), int(p["template_id"]), int(p["angle_index"])) for p in a]
+
+
+@pytest.mark.parametrize("opts,tsize", [
+    (dict(overlap=32), 128),
+    (dict(invert=1), 128),
+    (dict(phase_flip=1), 128),
+    (dict(), 127),
+    (dict(overlap=48, window_size=256), 128),
+])
+def test_norm_options(gs, orc, opts, tsize):
+    import synth
+
+    opts = dict(opts)
+    l = opts.pop("window_size", 320)
+    print(f"norm_type=1 options {opts} window {l} template {tsize}", flush=True)
+    cfg = gs.SearchConfig(phistep=10.0, window_size=l, norm_type=1, nms_radius=64.0, **opts)
+    ds = synth.make_dataset(77, 640, 608, n_templates=3, tsize=tsize, n_particles=3, snr=0.5,
+                            defocus=1.5, dfdiff=0.02, dfang=30.0)
+    image = -ds.image if cfg.invert else ds.image  # inverted contrast, undone by invert = 1
+    opicks, st = orc.search_norm(cfg, image, ds.templates, ds.ctf, want_stats=True)
+    eng = gs.Engine(cfg)
+    eng.load_templates(ds.templates)
+    g = eng.search(image, ds.ctf)
+    best = eng.stage(6, np.float32).astype(np.float64)
+    z_g = (best - eng.stage(4, np.float64)) / eng.stage(5, np.float64)
+    z_o = (st["best"] - st["mu"]) / st["sigma"]
+    assert z_g.shape == z_o.shape
+    assert np.max(np.abs(z_g - z_o) / np.maximum(1.0, np.abs(z_o))) < 1e-4
+    assert len(opicks) > 0
+    assert _key(g) == _key(opicks)
+    np.testing.assert_allclose(g["score"], opicks["score"], rtol=1e-4)
+    eng.close()
+
+
+def test_nonorm_overlap(gs, orc):
+    import synth
+
+    print("norm_type=0 window 360 overlap 40", flush=True)
+    cfg = gs.SearchConfig(phistep=30.0, window_size=360, overlap=40, norm_type=0, phase_flip=1,
+                          threshold=3.0, nms_radius=0.0)
+    ds = synth.make_dataset(8, 720, 720, n_templates=2, tsize=128, n_particles=2, snr=0.5, defocus=1.5)
+    _, orecs = orc.search_nonorm(cfg, ds.image, ds.templates, ds.ctf)
+    eng = gs.Engine(cfg)
+    eng.load_templates(ds.templates)
+    g = eng.search(ds.image, ds.ctf)
+    assert len(orecs) > 0
+    ko = sorted((int(p["angle_index"]), int(p["template_id"])) for p in orecs)
+    kg = sorted((int(p["angle_index"]), int(p["template_id"])) for p in g)
+    assert kg == ko
+    oi = {(int(p["angle_index"]), int(p["template_id"])): p for p in orecs}
+    for p in g:
+        q = oi[(int(p["angle_index"]), int(p["template_id"]))]
+        assert int(p["pixel"]) == int(q["pixel"])
+        assert abs(p["score"] - q["score"]) <= 1e-4 * abs(q["score"])
+        assert abs(p["local_x"] - q["local_x"]) < 1e-3 and abs(p["local_y"] - q["local_y"]) < 1e-3
+    eng.close()
