@@ -1518,6 +1518,7 @@ gs_status gs_kernel_timing(const gs_engine* e, int which, double* avg_ms, long l
 
 gs_status gs_set_chunk(gs_engine* e, int templates_per_launch) {
   if (!e) return fail(GS_ERR_INVALID, "null engine");
+  if (templates_per_launch < 0) return fail(GS_ERR_INVALID, "templates per launch must be 0 (automatic) or positive");
   e->impl->set_chunk(templates_per_launch);
   return GS_OK;
 }

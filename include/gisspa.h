@@ -146,7 +146,8 @@ gs_status gs_last_timing(const gs_engine* e, double* total_ms, double* corr_ms);
 gs_status gs_set_profiling(gs_engine* e, int on);
 /* Average duration (ms) and count of the timed launches: which = 0 column pass, 1 row pass. */
 gs_status gs_kernel_timing(const gs_engine* e, int which, double* avg_ms, long long* count);
-/* Templates per correlation launch (0 = automatic); a result-invariant tuning knob. */
+/* Templates per correlation launch (0 = automatic, negative: GS_ERR_INVALID); a result-invariant
+ * tuning knob. */
 gs_status gs_set_chunk(gs_engine* e, int templates_per_launch);
 /* Selects a result-invariant code variant of a correlation kernel.  kernel 0 = column pass:
  * 0 one workgroup per plane (radix 4*4*4*5), 1 persistent over templates, 2 persistent +
