@@ -996,6 +996,35 @@ bool launch_plane_fft2(float2* data, int batch, int L, int sign, float scale, hi
   return launch_plane_fft2_from(data, data, batch, L, sign, scale, s);
 }
 
+bool launch_plane_cols(float2* data, int batch, int L, int sign, float scale, hipStream_t s) {
+  TwTables& T = tw_tables(current_device());
+  auto go = [&](auto lc, const float2* tw) {
+    constexpr int LL = decltype(lc)::value;
+    constexpr int COLS = 256 / CorrCfg<LL>::TPC;
+    GS_REQUIRE(batch > 0 && batch < 65536, kInvalid, "plane transform batch out of range");
+    const dim3 gc(LL / COLS, (unsigned)batch);
+    if (sign < 0)
+      k_plane_cols<LL, -1><<<gc, 256, 0, s>>>(data, tw, scale, LL / 2);
+    else
+      k_plane_cols<LL, +1><<<gc, 256, 0, s>>>(data, tw, scale, LL / 2);
+    GS_CHECK_LAUNCH();
+  };
+  if (L == 320) {
+    go(std::integral_constant<int, 320>{}, T.t320);
+    return true;
+  }
+  if (L == 720) {
+    go(std::integral_constant<int, 720>{}, T.t720);
+    return true;
+  }
+  return false;
+}
+
+const float2* plane_twiddles(int L) {
+  TwTables& T = tw_tables(current_device());
+  return L == 320 ? T.t320 : (L == 720 ? T.t720 : nullptr);
+}
+
 // Partials per (template, block) written by the norm_type=0 row pass: one per row group and
 // wavefront.
 int rowpass_groups(int L) {

@@ -685,8 +685,15 @@ class Engine {
           GS_HIP(hipMemcpyAsync(Tt, H, sizeof(float2) * (size_t)K * L2, hipMemcpyDeviceToDevice, stream_));
           fft2_batched(fft_, Tt, S, K, l, l, +1, 1.f, stream_);
         }
-        launch_ref_rotate(Tt, rp, K, l, P_.diameter, 4.0f, part6, stats, AB, stream_);
-        fft2(AB, K + (K + 1) / 2, -1, (float)(1.0 / (double)L2));
+        // l = 320, 720: the rotation writes rows already transformed, so only the column pass
+        // of the forward transform remains
+        if (launch_ref_rotate_rows(Tt, rp, K, l, P_.diameter, 4.0f, part6, stats, AB, stream_)) {
+          GS_REQUIRE(launch_plane_cols(AB, K + (K + 1) / 2, l, -1, (float)(1.0 / (double)L2), stream_),
+                     kUnsupported, "window_size not supported");
+        } else {
+          launch_ref_rotate(Tt, rp, K, l, P_.diameter, 4.0f, part6, stats, AB, stream_);
+          fft2(AB, K + (K + 1) / 2, -1, (float)(1.0 / (double)L2));
+        }
         GS_REQUIRE(launch_ref_dot(ihat_t, AB, rp, stats, K, l, cg, stream_), kUnsupported, "window_size not supported");
         launch_ref_update(rp, K, cg, d_mu_.p, d_sigma_.p, l, step, scale, it == 0, it < iters, dz, dz0, dg, stream_);
       }

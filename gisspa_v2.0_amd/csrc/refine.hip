@@ -31,6 +31,7 @@
 
 #include "common.hpp"
 #include "device_common.hpp"
+#include "fft_tile.hpp"
 #include "kernels.hpp"
 
 namespace gs {
@@ -233,6 +234,118 @@ __global__ void __launch_bounds__(256)
     const int k = threadIdx.x;
     partial6[((long long)j * groups + g) * 6 + k] = red[0][k] + red[1][k] + red[2][k] + red[3][k];
   }
+}
+
+// Row transforms of the plane passes (the same lanes per row and radix plan as corr.hip's
+// k_plane_rows, so the rows below are bit-identical to its output).
+template <int L>
+struct RefRowCfg;
+template <>
+struct RefRowCfg<320> {
+  static constexpr int T = 16;
+  using Plan = ct::RadixList<4, 4, 4, 5>;
+};
+template <>
+struct RefRowCfg<720> {
+  static constexpr int T = 32;
+  using Plan = ct::RadixList<4, 4, 5, 9>;
+};
+
+// k_ref_rotate and the row pass of the forward transform in one kernel: one workgroup = RPB rows
+// of one output plane.  For an A plane (pick p) it samples r and dr/dtheta of its rows, for a B
+// plane m = p - K the defocus-derivative fields of picks 2m and 2m + 1 (the same float
+// expressions as k_ref_rotate), transforms the rows in LDS and writes them; the planes between
+// the rotation and the row pass are never stored.  The six box sums go out per (pick, row group):
+// sums 0-3 from the A plane's workgroup, sums 4-5 from the B plane's.
+template <int L>
+__global__ void __launch_bounds__(256)
+    k_ref_rotate_rows(const float2* __restrict__ H, const RefPeak* __restrict__ pk, float rin, float rout,
+                      float ehw, const float2* __restrict__ twL, float2* __restrict__ AB,
+                      double* __restrict__ partial6, int K) {
+  constexpr int T = RefRowCfg<L>::T, RPW = 64 / T, RPB = 4 * RPW, NG = L / RPB;
+  static_assert(L % RPB == 0, "row groups");
+  __shared__ float2 buf[RPB * L];
+  __shared__ float2 tw[L];
+  __shared__ double red[4][4];
+  const int tid = threadIdx.x, g = blockIdx.x, p = blockIdx.y;
+  const int y0 = g * RPB;
+  const long long L2 = (long long)L * L;
+  const float deg = (float)(kRefPi / 180);
+  for (int i = tid; i < L; i += 256) tw[i] = twL[i];
+  const bool aplane = p < K;
+  const int j0 = aplane ? p : 2 * (p - K);
+  const bool two = !aplane && j0 + 1 < K;
+  // rotation of the plane's pick(s)
+  float cs[2], sn[2];
+  const float2* Tp[2];
+  for (int q = 0; q < 2; ++q) {
+    const int j = min(j0 + q, K - 1);
+    const RefPeak P = pk[j];
+    cs[q] = (float)cos(P.theta * kRefPi / 180);
+    sn[q] = (float)sin(P.theta * kRefPi / 180);
+    Tp[q] = H + (long long)j * L2;
+  }
+  double a[4] = {0, 0, 0, 0};
+  for (int e = tid; e < RPB * L; e += 256) {
+    const int r = e / L, i = e - r * L, jrow = y0 + r;
+    const float m = dev::mask_factor(i, jrow, L, rin, rout, ehw);
+    float2 out = make_float2(0.f, 0.f);
+    for (int q = 0; q < (aplane ? 1 : (two ? 2 : 1)); ++q) {
+      float v = 0.f, vth = 0.f, vdf = 0.f;
+      const Bil b = bil_coords(i, jrow, L, cs[q], sn[q]);
+      if (b.ok) {
+        const float tt = 1.0f - b.t, uu = 1.0f - b.u;
+        const float2* Tq = Tp[q];
+        const float2 h0 = Tq[b.k0], h1 = Tq[b.k1], h2 = Tq[b.k2], h3 = Tq[b.k3];
+        v = h0.x * tt * uu + h1.x * b.t * uu + h2.x * b.t * b.u + h3.x * tt * b.u;
+        vdf = h0.y * tt * uu + h1.y * b.t * uu + h2.y * b.t * b.u + h3.y * tt * b.u;
+        if (aplane) {
+          const float dvx = uu * (h1.x - h0.x) + b.u * (h2.x - h3.x);
+          const float dvy = tt * (h3.x - h0.x) + b.t * (h2.x - h1.x);
+          const float x = (float)(i - L / 2), y = (float)(jrow - L / 2);
+          vth = (dvx * (-sn[q] * x + cs[q] * y) + dvy * (-cs[q] * x - sn[q] * y)) * deg;
+        }
+      }
+      v *= m;
+      vth *= m;
+      vdf *= m;
+      if (aplane) {
+        out = make_float2(v, vth);
+        a[0] += v;
+        a[1] += (double)v * v;
+        a[2] += vth;
+        a[3] += (double)v * vth;
+      } else {
+        if (q == 0)
+          out.x = vdf;
+        else
+          out.y = vdf;
+        a[2 * q] += vdf;
+        a[2 * q + 1] += (double)v * vdf;
+      }
+    }
+    buf[e] = out;
+  }
+  const int wave = tid / 64, lane = tid % 64;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    double v = a[k];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    if (lane == 0) red[wave][k] = v;
+  }
+  __syncthreads();
+  if (tid < 4) {
+    const double v = red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid];
+    if (aplane)
+      partial6[((long long)j0 * NG + g) * 6 + tid] = v;
+    else if (tid < 2 || two)
+      partial6[((long long)(j0 + tid / 2) * NG + g) * 6 + 4 + tid % 2] = v;
+  }
+  ct::run_stockham<L, T, -1, ct::Sync::WAVE>(buf + (wave * RPW + lane / T) * L, tw, lane % T,
+                                             typename RefRowCfg<L>::Plan{});
+  __syncthreads();
+  float2* dst = AB + (long long)p * L2 + (long long)y0 * L;
+  for (int e = tid; e < RPB * L; e += 256) dst[e] = buf[e];
 }
 
 // Per pick: the search's float mean/variance (k_rotation_stats) and the derivatives of the
@@ -480,6 +593,32 @@ void launch_ref_rotate(const float2* H, const RefPeak* pk, int K, int l, float d
   GS_CHECK_LAUNCH();
   k_ref_stats<<<nblk(K, 64), 64, 0, s>>>(partial6, rotation_groups(l), l, K, stats);
   GS_CHECK_LAUNCH();
+}
+
+bool launch_ref_rotate_rows(const float2* H, const RefPeak* pk, int K, int l, float d_m, float ehw,
+                            double* partial6, double* stats, float2* AB, hipStream_t s) {
+  const float2* tw = plane_twiddles(l);
+  if (!tw || K <= 0) return false;
+  const float dm = 1.5f * d_m;  // apply_mask: d_m = 1.5 * d_m (k:233)
+  const float rin = dm / 2, rout = dm / 2 + 2 * ehw;
+  const int planes = K + (K + 1) / 2;
+  GS_REQUIRE(planes < 65536, kInvalid, "refinement batch too large");
+  int groups = 0;
+  if (l == 320) {
+    constexpr int RPB = 4 * (64 / RefRowCfg<320>::T);
+    groups = 320 / RPB;
+    k_ref_rotate_rows<320><<<dim3(groups, planes), 256, 0, s>>>(H, pk, rin, rout, ehw, tw, AB, partial6, K);
+  } else if (l == 720) {
+    constexpr int RPB = 4 * (64 / RefRowCfg<720>::T);
+    groups = 720 / RPB;
+    k_ref_rotate_rows<720><<<dim3(groups, planes), 256, 0, s>>>(H, pk, rin, rout, ehw, tw, AB, partial6, K);
+  } else {
+    return false;
+  }
+  GS_CHECK_LAUNCH();
+  k_ref_stats<<<nblk(K, 64), 64, 0, s>>>(partial6, groups, l, K, stats);
+  GS_CHECK_LAUNCH();
+  return true;
 }
 
 bool launch_ref_dot(const float2* ihat, const float2* AB, const RefPeak* pk, const double* stats, int K, int l,
