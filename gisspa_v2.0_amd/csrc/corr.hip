@@ -896,7 +896,8 @@ __device__ __forceinline__ bool zero_row(int y, int L, int band) { return min(y,
 
 // Rows of `in` transformed into `out` (the same buffer for an in-place transform: every row a
 // workgroup reads is staged in LDS before it writes any, so the pointers carry no restrict).  A
-// workgroup whose rows are all zero by `band` only writes zeros.
+// workgroup whose rows are all zero by `band` writes nothing: those rows of `out` are left as
+// they were and must only be read through the column pass with the same band.
 template <int L, int SIGN>
 __global__ void __launch_bounds__(256)
     k_plane_rows(const float2* in, float2* out, long long nrows, const float2* __restrict__ twL, float scale,
@@ -912,10 +913,9 @@ __global__ void __launch_bounds__(256)
   const float2* src = in + r0 * L;
   float2* dst = out + r0 * L;
   const int y0 = (int)(r0 % L);
-  if (zero_row(y0, L, band) && zero_row(y0 + RPB - 1, L, band) && y0 + RPB - 1 < L - band) {
-    for (int e = tid; e < nr * L; e += 256) dst[e] = make_float2(0.f, 0.f);
-    return;  // uniform per workgroup
-  }
+  if (zero_row(y0, L, band) && zero_row(y0 + RPB - 1, L, band) && y0 + RPB - 1 < L - band)
+    return;  // uniform per workgroup: zero rows transform to zero rows, which the column pass
+             // (same band) takes as zero without reading them
   for (int i = tid; i < L; i += 256) tw[i] = twL[i];
   for (int e = tid; e < RPB * L; e += 256)
     buf[e] = (e < nr * L && !zero_row(y0 + e / L, L, band)) ? src[e] : make_float2(0.f, 0.f);

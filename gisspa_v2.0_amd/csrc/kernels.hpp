@@ -119,7 +119,8 @@ void corr_prepare();
 // then multiplied by scale; LDS transforms for L = 320 and 720 (returns false otherwise).
 bool launch_plane_fft2(float2* data, int batch, int L, int sign, float scale, hipStream_t s);
 // The same transform of `in` written to `out` (in is left unchanged).  Input rows y with
-// min(y, L - y) > band are taken as zero without being read (default: every row is read).
+// min(y, L - y) > band are taken as zero without being read (default: every row is read); the
+// result in `out` is complete either way.
 bool launch_plane_fft2_from(const float2* in, float2* out, int batch, int L, int sign, float scale, hipStream_t s,
                             int band = -1);
 // The second (column) pass of launch_plane_fft2 alone, in place and scaled, for planes whose rows
