@@ -930,9 +930,12 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// Columns of planes whose rows are already transformed; input rows y that are zero by `band`, or
+// outside [ylo, yhi], are taken as zero without being read.
 template <int L, int SIGN>
 __global__ void __launch_bounds__(256)
-    k_plane_cols(float2* __restrict__ data, const float2* __restrict__ twL, float scale, int band) {
+    k_plane_cols(float2* __restrict__ data, const float2* __restrict__ twL, float scale, int band, int ylo,
+                 int yhi) {
   using C = CorrCfg<L>;
   constexpr int T = C::TPC, COLS = 256 / T, LP = L + 1;
   static_assert(L % COLS == 0, "column groups");
@@ -944,7 +947,8 @@ __global__ void __launch_bounds__(256)
   float2* P = data + (long long)blockIdx.y * L * L;
   for (int e = tid; e < COLS * L; e += 256) {
     const int y = e / COLS, c = e % COLS;
-    buf[c * LP + y] = zero_row(y, L, band) ? make_float2(0.f, 0.f) : P[(long long)y * L + c0 + c];
+    buf[c * LP + y] =
+        (zero_row(y, L, band) || y < ylo || y > yhi) ? make_float2(0.f, 0.f) : P[(long long)y * L + c0 + c];
   }
   __syncthreads();
   ct::run_stockham<L, T, SIGN, ct::Sync::WAVE>(buf + (tid / T) * LP, tw, tid % T, typename C::ColPlan{});
@@ -967,10 +971,10 @@ void plane_fft2_impl(const float2* in, float2* data, int batch, int sign, float 
   const dim3 gc(L / COLS, (unsigned)batch);
   if (sign < 0) {
     k_plane_rows<L, -1><<<(unsigned)gr, 256, 0, s>>>(in, data, nrows, tw, 1.f, band);
-    k_plane_cols<L, -1><<<gc, 256, 0, s>>>(data, tw, scale, band);
+    k_plane_cols<L, -1><<<gc, 256, 0, s>>>(data, tw, scale, band, 0, L - 1);
   } else {
     k_plane_rows<L, +1><<<(unsigned)gr, 256, 0, s>>>(in, data, nrows, tw, 1.f, band);
-    k_plane_cols<L, +1><<<gc, 256, 0, s>>>(data, tw, scale, band);
+    k_plane_cols<L, +1><<<gc, 256, 0, s>>>(data, tw, scale, band, 0, L - 1);
   }
   GS_CHECK_LAUNCH();
 }
@@ -996,17 +1000,18 @@ bool launch_plane_fft2(float2* data, int batch, int L, int sign, float scale, hi
   return launch_plane_fft2_from(data, data, batch, L, sign, scale, s);
 }
 
-bool launch_plane_cols(float2* data, int batch, int L, int sign, float scale, hipStream_t s) {
+bool launch_plane_cols(float2* data, int batch, int L, int sign, float scale, hipStream_t s, int ylo, int yhi) {
   TwTables& T = tw_tables(current_device());
+  if (yhi < 0) yhi = L - 1;
   auto go = [&](auto lc, const float2* tw) {
     constexpr int LL = decltype(lc)::value;
     constexpr int COLS = 256 / CorrCfg<LL>::TPC;
     GS_REQUIRE(batch > 0 && batch < 65536, kInvalid, "plane transform batch out of range");
     const dim3 gc(LL / COLS, (unsigned)batch);
     if (sign < 0)
-      k_plane_cols<LL, -1><<<gc, 256, 0, s>>>(data, tw, scale, LL / 2);
+      k_plane_cols<LL, -1><<<gc, 256, 0, s>>>(data, tw, scale, LL / 2, ylo, yhi);
     else
-      k_plane_cols<LL, +1><<<gc, 256, 0, s>>>(data, tw, scale, LL / 2);
+      k_plane_cols<LL, +1><<<gc, 256, 0, s>>>(data, tw, scale, LL / 2, ylo, yhi);
     GS_CHECK_LAUNCH();
   };
   if (L == 320) {

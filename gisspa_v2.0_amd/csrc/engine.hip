@@ -687,8 +687,9 @@ class Engine {
         }
         // l = 320, 720: the rotation writes rows already transformed, so only the column pass
         // of the forward transform remains
-        if (launch_ref_rotate_rows(Tt, rp, K, l, P_.diameter, 4.0f, part6, stats, AB, stream_)) {
-          GS_REQUIRE(launch_plane_cols(AB, K + (K + 1) / 2, l, -1, (float)(1.0 / (double)L2), stream_),
+        int ylo = 0, yhi = l - 1;
+        if (launch_ref_rotate_rows(Tt, rp, K, l, P_.diameter, 4.0f, part6, stats, AB, &ylo, &yhi, stream_)) {
+          GS_REQUIRE(launch_plane_cols(AB, K + (K + 1) / 2, l, -1, (float)(1.0 / (double)L2), stream_, ylo, yhi),
                      kUnsupported, "window_size not supported");
         } else {
           launch_ref_rotate(Tt, rp, K, l, P_.diameter, 4.0f, part6, stats, AB, stream_);

@@ -124,8 +124,10 @@ bool launch_plane_fft2(float2* data, int batch, int L, int sign, float scale, hi
 bool launch_plane_fft2_from(const float2* in, float2* out, int batch, int L, int sign, float scale, hipStream_t s,
                             int band = -1);
 // The second (column) pass of launch_plane_fft2 alone, in place and scaled, for planes whose rows
-// are already transformed (L = 320, 720; returns false otherwise).
-bool launch_plane_cols(float2* data, int batch, int L, int sign, float scale, hipStream_t s);
+// are already transformed (L = 320, 720; returns false otherwise).  Rows outside [ylo, yhi]
+// (default: none) are zero and not read; the result is complete.
+bool launch_plane_cols(float2* data, int batch, int L, int sign, float scale, hipStream_t s, int ylo = 0,
+                       int yhi = -1);
 // The flat twiddle table exp(-2 pi i e / L) of the plane transforms on the current device
 // (L = 320, 720; nullptr otherwise).
 const float2* plane_twiddles(int L);
@@ -167,10 +169,12 @@ void launch_ref_clear(float2* H, long long n, hipStream_t s);
 // picks 2m and 2m+1 in the real and imaginary parts of plane K + m), and per pick the box
 // statistics (stats: 8 doubles) of the standardisation.
 // launch_ref_rotate fused with the row pass of the forward transform (l = 320, 720; returns
-// false otherwise): AB receives the A and B planes with their rows already transformed (finish
-// with launch_plane_cols); partial6 needs K * l / 8 * 6 doubles, stats as launch_ref_rotate's.
+// false otherwise): AB receives the A and B planes with their rows already transformed, except
+// rows outside [*ylo, *yhi], which the mask makes zero and which are left unwritten: finish with
+// launch_plane_cols over that row support.  partial6 needs K * l / 8 * 6 doubles, stats as
+// launch_ref_rotate's.
 bool launch_ref_rotate_rows(const float2* H, const RefPeak* pk, int K, int l, float d_m, float ehw,
-                            double* partial6, double* stats, float2* AB, hipStream_t s);
+                            double* partial6, double* stats, float2* AB, int* ylo, int* yhi, hipStream_t s);
 void launch_ref_rotate(const float2* H, const RefPeak* pk, int K, int l, float d_m, float ehw,
                        double* partial6, double* stats, float2* AB, hipStream_t s);
 // c and dc/d(dx, dy, theta, ddf) per pick (out: K x 5) from the transformed AB and the

@@ -261,7 +261,7 @@ template <int L>
 __global__ void __launch_bounds__(256)
     k_ref_rotate_rows(const float2* __restrict__ H, const RefPeak* __restrict__ pk, float rin, float rout,
                       float ehw, const float2* __restrict__ twL, float2* __restrict__ AB,
-                      double* __restrict__ partial6, int K) {
+                      double* __restrict__ partial6, int K, int ylo, int yhi) {
   constexpr int T = RefRowCfg<L>::T, RPW = 64 / T, RPB = 4 * RPW, NG = L / RPB;
   static_assert(L % RPB == 0, "row groups");
   __shared__ float2 buf[RPB * L];
@@ -275,6 +275,15 @@ __global__ void __launch_bounds__(256)
   const bool aplane = p < K;
   const int j0 = aplane ? p : 2 * (p - K);
   const bool two = !aplane && j0 + 1 < K;
+  if (y0 > yhi || y0 + RPB - 1 < ylo) {  // rows outside the mask's support: zero, left unwritten
+    if (tid < 4) {
+      if (aplane)
+        partial6[((long long)j0 * NG + g) * 6 + tid] = 0.0;
+      else if (tid < 2 || two)
+        partial6[((long long)(j0 + tid / 2) * NG + g) * 6 + 4 + tid % 2] = 0.0;
+    }
+    return;  // uniform per workgroup
+  }
   // rotation of the plane's pick(s)
   float cs[2], sn[2];
   const float2* Tp[2];
@@ -290,7 +299,8 @@ __global__ void __launch_bounds__(256)
     const int r = e / L, i = e - r * L, jrow = y0 + r;
     const float m = dev::mask_factor(i, jrow, L, rin, rout, ehw);
     float2 out = make_float2(0.f, 0.f);
-    for (int q = 0; q < (aplane ? 1 : (two ? 2 : 1)); ++q) {
+    // where the mask is zero every field is zero: no samples (the sums gain nothing either)
+    for (int q = 0; q < (m == 0.f ? 0 : (aplane ? 1 : (two ? 2 : 1))); ++q) {
       float v = 0.f, vth = 0.f, vdf = 0.f;
       const Bil b = bil_coords(i, jrow, L, cs[q], sn[q]);
       if (b.ok) {
@@ -596,22 +606,27 @@ void launch_ref_rotate(const float2* H, const RefPeak* pk, int K, int l, float d
 }
 
 bool launch_ref_rotate_rows(const float2* H, const RefPeak* pk, int K, int l, float d_m, float ehw,
-                            double* partial6, double* stats, float2* AB, hipStream_t s) {
+                            double* partial6, double* stats, float2* AB, int* ylo, int* yhi, hipStream_t s) {
   const float2* tw = plane_twiddles(l);
   if (!tw || K <= 0) return false;
   const float dm = 1.5f * d_m;  // apply_mask: d_m = 1.5 * d_m (k:233)
   const float rin = dm / 2, rout = dm / 2 + 2 * ehw;
+  // rows y with |y - l/2| > rout hold only masked (zero) pixels
+  *ylo = std::max(0, (int)std::floor(l / 2 - (double)rout));
+  *yhi = std::min(l - 1, (int)std::ceil(l / 2 + (double)rout));
   const int planes = K + (K + 1) / 2;
   GS_REQUIRE(planes < 65536, kInvalid, "refinement batch too large");
   int groups = 0;
   if (l == 320) {
     constexpr int RPB = 4 * (64 / RefRowCfg<320>::T);
     groups = 320 / RPB;
-    k_ref_rotate_rows<320><<<dim3(groups, planes), 256, 0, s>>>(H, pk, rin, rout, ehw, tw, AB, partial6, K);
+    k_ref_rotate_rows<320><<<dim3(groups, planes), 256, 0, s>>>(H, pk, rin, rout, ehw, tw, AB, partial6, K, *ylo,
+                                                                 *yhi);
   } else if (l == 720) {
     constexpr int RPB = 4 * (64 / RefRowCfg<720>::T);
     groups = 720 / RPB;
-    k_ref_rotate_rows<720><<<dim3(groups, planes), 256, 0, s>>>(H, pk, rin, rout, ehw, tw, AB, partial6, K);
+    k_ref_rotate_rows<720><<<dim3(groups, planes), 256, 0, s>>>(H, pk, rin, rout, ehw, tw, AB, partial6, K, *ylo,
+                                                                 *yhi);
   } else {
     return false;
   }
