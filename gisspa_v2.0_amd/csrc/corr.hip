@@ -26,6 +26,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "device_common.hpp"
 #include "fft_tile.hpp"
 #include "kernels.hpp"
 
@@ -916,9 +917,13 @@ __global__ void __launch_bounds__(256)
   if (zero_row(y0, L, band) && zero_row(y0 + RPB - 1, L, band) && y0 + RPB - 1 < L - band)
     return;  // uniform per workgroup: zero rows transform to zero rows, which the column pass
              // (same band) takes as zero without reading them
-  for (int i = tid; i < L; i += 256) tw[i] = twL[i];
-  for (int e = tid; e < RPB * L; e += 256)
-    buf[e] = (e < nr * L && !zero_row(y0 + e / L, L, band)) ? src[e] : make_float2(0.f, 0.f);
+  dev::RegStage<L> twr;
+  twr.load(twL, tid);
+  dev::burst_copy<RPB * L, 24>(
+      tid,
+      [&](int e) { return (e < nr * L && !zero_row(y0 + e / L, L, band)) ? src[e] : make_float2(0.f, 0.f); },
+      [&](int e, float2 v) { buf[e] = v; });
+  twr.store(tw, tid);
   __syncthreads();
   const int wave = tid / 64, lane = tid % 64;
   ct::run_stockham<L, T, SIGN, ct::Sync::WAVE>(buf + (wave * RPW + lane / T) * L, tw, lane % T,
@@ -942,14 +947,18 @@ __global__ void __launch_bounds__(256)
   __shared__ float2 buf[COLS * LP];
   __shared__ float2 tw[L];
   const int tid = threadIdx.x;
-  for (int i = tid; i < L; i += 256) tw[i] = twL[i];
+  dev::RegStage<L> twr;
+  twr.load(twL, tid);
   const int c0 = blockIdx.x * COLS;
   float2* P = data + (long long)blockIdx.y * L * L;
-  for (int e = tid; e < COLS * L; e += 256) {
-    const int y = e / COLS, c = e % COLS;
-    buf[c * LP + y] =
-        (zero_row(y, L, band) || y < ylo || y > yhi) ? make_float2(0.f, 0.f) : P[(long long)y * L + c0 + c];
-  }
+  dev::burst_copy<COLS * L, 24>(
+      tid,
+      [&](int e) {
+        const int y = e / COLS, c = e % COLS;
+        return (zero_row(y, L, band) || y < ylo || y > yhi) ? make_float2(0.f, 0.f) : P[(long long)y * L + c0 + c];
+      },
+      [&](int e, float2 v) { buf[(e % COLS) * LP + e / COLS] = v; });
+  twr.store(tw, tid);
   __syncthreads();
   ct::run_stockham<L, T, SIGN, ct::Sync::WAVE>(buf + (tid / T) * LP, tw, tid % T, typename C::ColPlan{});
   __syncthreads();

@@ -115,6 +115,46 @@ __device__ __forceinline__ Bil bil_coords(int i, int j, int l, float cose, float
   return o;
 }
 
+// Copies the N elements e = tid + 256 k of a 256-thread workgroup's tile in bursts of up to CH
+// loads per thread, all issued before the first is waited for (a plain strided loop waits for
+// each load in turn, which leaves the copy bound by memory latency).  src(e) returns element e,
+// dst(e, v) stores it.
+template <int N, int CH, typename Src, typename Dst>
+__device__ __forceinline__ void burst_copy(int tid, Src&& src, Dst&& dst) {
+  constexpr int NE = (N + 255) / 256;
+#pragma unroll
+  for (int k0 = 0; k0 < NE; k0 += CH) {
+    float2 v[CH];
+#pragma unroll
+    for (int k = 0; k < CH; ++k) {
+      const int e = tid + 256 * (k0 + k);
+      if (k0 + k < NE && (N % 256 == 0 || e < N)) v[k] = src(e);
+    }
+#pragma unroll
+    for (int k = 0; k < CH; ++k) {
+      const int e = tid + 256 * (k0 + k);
+      if (k0 + k < NE && (N % 256 == 0 || e < N)) dst(e, v[k]);
+    }
+  }
+}
+
+// N elements of a table held in registers of a 256-thread workgroup between a load and a store,
+// so that the load's latency overlaps the other loads issued in between.
+template <int N>
+struct RegStage {
+  float2 v[(N + 255) / 256];
+  __device__ __forceinline__ void load(const float2* __restrict__ src, int tid) {
+#pragma unroll
+    for (int k = 0; k < (N + 255) / 256; ++k)
+      if (N % 256 == 0 || tid + 256 * k < N) v[k] = src[tid + 256 * k];
+  }
+  __device__ __forceinline__ void store(float2* dst, int tid) const {
+#pragma unroll
+    for (int k = 0; k < (N + 255) / 256; ++k)
+      if (N % 256 == 0 || tid + 256 * k < N) dst[tid + 256 * k] = v[k];
+  }
+};
+
 // Soft disc of apply_mask (k:225-241) at (i, j), in float as the rotation kernel applies it.
 __device__ __forceinline__ float mask_factor(int i, int j, int l, float rin, float rout, float ehw) {
   const float r = hypotf((float)(i - l / 2), (float)(j - l / 2));

@@ -271,7 +271,6 @@ __global__ void __launch_bounds__(256)
   const int y0 = g * RPB;
   const long long L2 = (long long)L * L;
   const float deg = (float)(kRefPi / 180);
-  for (int i = tid; i < L; i += 256) tw[i] = twL[i];
   const bool aplane = p < K;
   const int j0 = aplane ? p : 2 * (p - K);
   const bool two = !aplane && j0 + 1 < K;
@@ -294,47 +293,81 @@ __global__ void __launch_bounds__(256)
     sn[q] = (float)sin(P.theta * kRefPi / 180);
     Tp[q] = H + (long long)j * L2;
   }
+  dev::RegStage<L> twr;
+  twr.load(twL, tid);
   double a[4] = {0, 0, 0, 0};
-  for (int e = tid; e < RPB * L; e += 256) {
-    const int r = e / L, i = e - r * L, jrow = y0 + r;
-    const float m = dev::mask_factor(i, jrow, L, rin, rout, ehw);
-    float2 out = make_float2(0.f, 0.f);
-    // where the mask is zero every field is zero: no samples (the sums gain nothing either)
-    for (int q = 0; q < (m == 0.f ? 0 : (aplane ? 1 : (two ? 2 : 1))); ++q) {
-      float v = 0.f, vth = 0.f, vdf = 0.f;
-      const Bil b = bil_coords(i, jrow, L, cs[q], sn[q]);
-      if (b.ok) {
-        const float tt = 1.0f - b.t, uu = 1.0f - b.u;
-        const float2* Tq = Tp[q];
-        const float2 h0 = Tq[b.k0], h1 = Tq[b.k1], h2 = Tq[b.k2], h3 = Tq[b.k3];
-        v = h0.x * tt * uu + h1.x * b.t * uu + h2.x * b.t * b.u + h3.x * tt * b.u;
-        vdf = h0.y * tt * uu + h1.y * b.t * uu + h2.y * b.t * b.u + h3.y * tt * b.u;
-        if (aplane) {
-          const float dvx = uu * (h1.x - h0.x) + b.u * (h2.x - h3.x);
-          const float dvy = tt * (h3.x - h0.x) + b.t * (h2.x - h1.x);
-          const float x = (float)(i - L / 2), y = (float)(jrow - L / 2);
-          vth = (dvx * (-sn[q] * x + cs[q] * y) + dvy * (-cs[q] * x - sn[q] * y)) * deg;
+  // pixels e = tid + 256 k in batches of BT, all gathers of a batch in flight together
+  constexpr int N = RPB * L, NE = (N + 255) / 256, BT = 4;
+  for (int k0 = 0; k0 < NE; k0 += BT) {
+    float2 h[BT][2][4];
+    float bt[BT][2], bu[BT][2], mk[BT];
+    bool act[BT][2];
+#pragma unroll
+    for (int b = 0; b < BT; ++b) {
+      const int e = tid + 256 * (k0 + b);
+      const int r = e / L, i = e - r * L, jrow = y0 + r;
+      // where the mask is zero every field is zero: no samples (the sums gain nothing either)
+      mk[b] = (k0 + b < NE && e < N) ? dev::mask_factor(i, jrow, L, rin, rout, ehw) : 0.f;
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const Bil c = bil_coords(i, jrow, L, cs[q], sn[q]);
+        act[b][q] = mk[b] != 0.f && (q == 0 || two) && c.ok;
+        bt[b][q] = c.t;
+        bu[b][q] = c.u;
+        if (act[b][q]) {
+          const float2* Tq = Tp[q];
+          h[b][q][0] = Tq[c.k0];
+          h[b][q][1] = Tq[c.k1];
+          h[b][q][2] = Tq[c.k2];
+          h[b][q][3] = Tq[c.k3];
         }
       }
-      v *= m;
-      vth *= m;
-      vdf *= m;
-      if (aplane) {
-        out = make_float2(v, vth);
-        a[0] += v;
-        a[1] += (double)v * v;
-        a[2] += vth;
-        a[3] += (double)v * vth;
-      } else {
-        if (q == 0)
-          out.x = vdf;
-        else
-          out.y = vdf;
-        a[2 * q] += vdf;
-        a[2 * q + 1] += (double)v * vdf;
-      }
     }
-    buf[e] = out;
+#pragma unroll
+    for (int b = 0; b < BT; ++b) {
+      const int e = tid + 256 * (k0 + b);
+      if (!(k0 + b < NE && e < N)) continue;
+      const int r = e / L, i = e - r * L, jrow = y0 + r;
+      const float m = mk[b];
+      float2 out = make_float2(0.f, 0.f);
+      const int nq = m == 0.f ? 0 : (aplane ? 1 : (two ? 2 : 1));
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        if (q >= nq) continue;
+        float v = 0.f, vth = 0.f, vdf = 0.f;
+        if (act[b][q]) {
+          const float t = bt[b][q], u = bu[b][q];
+          const float tt = 1.0f - t, uu = 1.0f - u;
+          const float2 h0 = h[b][q][0], h1 = h[b][q][1], h2 = h[b][q][2], h3 = h[b][q][3];
+          v = h0.x * tt * uu + h1.x * t * uu + h2.x * t * u + h3.x * tt * u;
+          vdf = h0.y * tt * uu + h1.y * t * uu + h2.y * t * u + h3.y * tt * u;
+          if (aplane) {
+            const float dvx = uu * (h1.x - h0.x) + u * (h2.x - h3.x);
+            const float dvy = tt * (h3.x - h0.x) + t * (h2.x - h1.x);
+            const float x = (float)(i - L / 2), y = (float)(jrow - L / 2);
+            vth = (dvx * (-sn[q] * x + cs[q] * y) + dvy * (-cs[q] * x - sn[q] * y)) * deg;
+          }
+        }
+        v *= m;
+        vth *= m;
+        vdf *= m;
+        if (aplane) {
+          out = make_float2(v, vth);
+          a[0] += v;
+          a[1] += (double)v * v;
+          a[2] += vth;
+          a[3] += (double)v * vth;
+        } else {
+          if (q == 0)
+            out.x = vdf;
+          else
+            out.y = vdf;
+          a[2 * q] += vdf;
+          a[2 * q + 1] += (double)v * vdf;
+        }
+      }
+      buf[e] = out;
+    }
   }
   const int wave = tid / 64, lane = tid % 64;
 #pragma unroll
@@ -343,6 +376,7 @@ __global__ void __launch_bounds__(256)
     for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
     if (lane == 0) red[wave][k] = v;
   }
+  twr.store(tw, tid);
   __syncthreads();
   if (tid < 4) {
     const double v = red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid];
