@@ -936,7 +936,7 @@ __global__ void __launch_bounds__(256)
 }
 
 // Columns of planes whose rows are already transformed; input rows y that are zero by `band`, or
-// outside [ylo, yhi], are taken as zero without being read.
+// outside [ylo, yhi] (all rows for the plane transforms), are taken as zero without being read.
 template <int L, int SIGN>
 __global__ void __launch_bounds__(256)
     k_plane_cols(float2* __restrict__ data, const float2* __restrict__ twL, float scale, int band, int ylo,
@@ -1007,31 +1007,6 @@ bool launch_plane_fft2_from(const float2* in, float2* out, int batch, int L, int
 
 bool launch_plane_fft2(float2* data, int batch, int L, int sign, float scale, hipStream_t s) {
   return launch_plane_fft2_from(data, data, batch, L, sign, scale, s);
-}
-
-bool launch_plane_cols(float2* data, int batch, int L, int sign, float scale, hipStream_t s, int ylo, int yhi) {
-  TwTables& T = tw_tables(current_device());
-  if (yhi < 0) yhi = L - 1;
-  auto go = [&](auto lc, const float2* tw) {
-    constexpr int LL = decltype(lc)::value;
-    constexpr int COLS = 256 / CorrCfg<LL>::TPC;
-    GS_REQUIRE(batch > 0 && batch < 65536, kInvalid, "plane transform batch out of range");
-    const dim3 gc(LL / COLS, (unsigned)batch);
-    if (sign < 0)
-      k_plane_cols<LL, -1><<<gc, 256, 0, s>>>(data, tw, scale, LL / 2, ylo, yhi);
-    else
-      k_plane_cols<LL, +1><<<gc, 256, 0, s>>>(data, tw, scale, LL / 2, ylo, yhi);
-    GS_CHECK_LAUNCH();
-  };
-  if (L == 320) {
-    go(std::integral_constant<int, 320>{}, T.t320);
-    return true;
-  }
-  if (L == 720) {
-    go(std::integral_constant<int, 720>{}, T.t720);
-    return true;
-  }
-  return false;
 }
 
 const float2* plane_twiddles(int L) {

@@ -661,10 +661,14 @@ class Engine {
     double2* part = d_rpart_.ensure((size_t)Kc * 16);
     double* part6 = d_rpart6_.ensure((size_t)Kc * rotation_groups(l) * 6);
     double* stats = d_rstats_.ensure((size_t)Kc * 8);
-    double* cg = d_rcg_.ensure((size_t)Kc * 5);
+    double* cg = d_rcg_.ensure((size_t)Kc * std::max(kRefDotSplit, ref_cols_dot_parts(l)) * 5);
     double* dz = d_rz_.ensure(Kc);
     double* dz0 = d_rz0_.ensure(Kc);
     double* dg = d_rgrad_.ensure((size_t)Kc * 4);
+    // the rotation's mask table and the rows it leaves non-zero (the fused rotation kernels)
+    int ylo = 0, yhi = l - 1;
+    float* mtab = d_rmask_.ensure((size_t)L2);
+    launch_ref_mask(l, P_.diameter, 4.0f, mtab, &ylo, &yhi, stream_);
     std::vector<RefPeak> hp(Kc);
     std::vector<double> hz(Kc), hz0(Kc), hg((size_t)Kc * 4);
     for (size_t c0 = 0; c0 < k; c0 += Kc) {
@@ -685,18 +689,22 @@ class Engine {
           GS_HIP(hipMemcpyAsync(Tt, H, sizeof(float2) * (size_t)K * L2, hipMemcpyDeviceToDevice, stream_));
           fft2_batched(fft_, Tt, S, K, l, l, +1, 1.f, stream_);
         }
-        // l = 320, 720: the rotation writes rows already transformed, so only the column pass
-        // of the forward transform remains
-        int ylo = 0, yhi = l - 1;
-        if (launch_ref_rotate_rows(Tt, rp, K, l, P_.diameter, 4.0f, part6, stats, AB, &ylo, &yhi, stream_)) {
-          GS_REQUIRE(launch_plane_cols(AB, K + (K + 1) / 2, l, -1, (float)(1.0 / (double)L2), stream_, ylo, yhi),
+        // l = 320, 720: the rotation writes rows already transformed and the column pass of the
+        // forward transform is fused with the dot, so the spectra of AB are never stored
+        int parts = kRefDotSplit;
+        if (launch_ref_rotate_rows(Tt, rp, K, l, mtab, ylo, yhi, part6, stats, AB, stream_)) {
+          GS_REQUIRE(launch_ref_cols_dot(d_ihat_.p, AB, rp, stats, K, l, ylo, yhi, (float)(1.0 / (double)L2), cg,
+                                         stream_),
                      kUnsupported, "window_size not supported");
+          parts = ref_cols_dot_parts(l);
         } else {
           launch_ref_rotate(Tt, rp, K, l, P_.diameter, 4.0f, part6, stats, AB, stream_);
           fft2(AB, K + (K + 1) / 2, -1, (float)(1.0 / (double)L2));
+          GS_REQUIRE(launch_ref_dot(ihat_t, AB, rp, stats, K, l, cg, stream_), kUnsupported,
+                     "window_size not supported");
         }
-        GS_REQUIRE(launch_ref_dot(ihat_t, AB, rp, stats, K, l, cg, stream_), kUnsupported, "window_size not supported");
-        launch_ref_update(rp, K, cg, d_mu_.p, d_sigma_.p, l, step, scale, it == 0, it < iters, dz, dz0, dg, stream_);
+        launch_ref_update(rp, K, cg, parts, d_mu_.p, d_sigma_.p, l, step, scale, it == 0, it < iters, dz, dz0, dg,
+                          stream_);
       }
       GS_HIP(hipMemcpyAsync(hp.data(), rp, sizeof(RefPeak) * K, hipMemcpyDeviceToHost, stream_));
       GS_HIP(hipMemcpyAsync(hz.data(), dz, sizeof(double) * K, hipMemcpyDeviceToHost, stream_));
@@ -1305,6 +1313,7 @@ class Engine {
   PinnedBuf pinned_[2];
   std::map<long long, std::unique_ptr<DevBuf<double>>> rb_cache_;
 
+  DevBuf<float> d_rmask_;
   DevBuf<float> d_raw_, d_img_, d_imgpre_, d_tprep_, d_stats_, d_best_, d_tile_;
   DevBuf<float2> d_work_, d_work2_, d_scratch_, d_frame_, d_frame2_, d_blocks_, d_ihat_, d_that_, d_Z_, d_cs_;
   DevBuf<double> d_mom_, d_ra_, d_ra_img_, d_m_, d_part_, d_S_, d_SS_, d_mu_, d_sigma_, d_planes_, d_chpart_;

@@ -123,11 +123,6 @@ bool launch_plane_fft2(float2* data, int batch, int L, int sign, float scale, hi
 // result in `out` is complete either way.
 bool launch_plane_fft2_from(const float2* in, float2* out, int batch, int L, int sign, float scale, hipStream_t s,
                             int band = -1);
-// The second (column) pass of launch_plane_fft2 alone, in place and scaled, for planes whose rows
-// are already transformed (L = 320, 720; returns false otherwise).  Rows outside [ylo, yhi]
-// (default: none) are zero and not read; the result is complete.
-bool launch_plane_cols(float2* data, int batch, int L, int sign, float scale, hipStream_t s, int ylo = 0,
-                       int yhi = -1);
 // The flat twiddle table exp(-2 pi i e / L) of the plane transforms on the current device
 // (L = 320, 720; nullptr otherwise).
 const float2* plane_twiddles(int L);
@@ -168,22 +163,36 @@ void launch_ref_clear(float2* H, long long n, hipStream_t s);
 // derivatives, packed into AB (A = r0 + i dr0/dtheta of pick j in plane j, B = dr0/d ddf of
 // picks 2m and 2m+1 in the real and imaginary parts of plane K + m), and per pick the box
 // statistics (stats: 8 doubles) of the standardisation.
+// The rotation's soft mask (diameter d_m, edge half-width ehw) as an l x l float table, and the
+// rows [*ylo, *yhi] outside which it is zero.
+void launch_ref_mask(int l, float d_m, float ehw, float* mask, int* ylo, int* yhi, hipStream_t s);
+
 // launch_ref_rotate fused with the row pass of the forward transform (l = 320, 720; returns
-// false otherwise): AB receives the A and B planes with their rows already transformed, except
-// rows outside [*ylo, *yhi], which the mask makes zero and which are left unwritten: finish with
-// launch_plane_cols over that row support.  partial6 needs K * l / 8 * 6 doubles, stats as
-// launch_ref_rotate's.
-bool launch_ref_rotate_rows(const float2* H, const RefPeak* pk, int K, int l, float d_m, float ehw,
-                            double* partial6, double* stats, float2* AB, int* ylo, int* yhi, hipStream_t s);
+// false otherwise), with the mask table of launch_ref_mask: AB receives the A and B planes with
+// their rows already transformed, except rows outside [ylo, yhi], which the mask makes zero and
+// which are left unwritten: finish with launch_ref_cols_dot over that row support.  partial6
+// needs K * l / 8 * 6 doubles, stats as launch_ref_rotate's.
+bool launch_ref_rotate_rows(const float2* H, const RefPeak* pk, int K, int l, const float* mask, int ylo,
+                            int yhi, double* partial6, double* stats, float2* AB, hipStream_t s);
 void launch_ref_rotate(const float2* H, const RefPeak* pk, int K, int l, float d_m, float ehw,
                        double* partial6, double* stats, float2* AB, hipStream_t s);
-// c and dc/d(dx, dy, theta, ddf) per pick (out: K x 5) from the transformed AB and the
-// statistics (the standardisation is applied to the spectra); ihat is the block spectra
-// transposed to [b][ky][kx 0..l/2].
+// c and dc/d(dx, dy, theta, ddf) per pick from the transformed AB and the statistics (the
+// standardisation is applied to the spectra); ihat is the block spectra transposed to
+// [b][ky][kx 0..l/2].  out: K x kRefDotSplit x 5 partial sums over consecutive shares of the
+// half spectrum, summed by launch_ref_update.
+constexpr int kRefDotSplit = 4;
 bool launch_ref_dot(const float2* ihat, const float2* AB, const RefPeak* pk, const double* stats, int K, int l,
                     double* out, hipStream_t s);
-// z and gradient per pick; when update, one ascent step p += step * scale^2 * dz/dp.
-void launch_ref_update(RefPeak* pk, int K, const double* cg, const double* mu, const double* sigma, int l,
+// The column pass of the AB planes (left by launch_ref_rotate_rows, rows outside [ylo, yhi]
+// unwritten) fused with launch_ref_dot, the spectra scaled by `scale` (l = 320, 720; returns false
+// otherwise); AB is only read.  ihat: the block half spectra [b][kx 0..l/2][ky].  out: K x
+// ref_cols_dot_parts(l) x 5 partial sums.
+int ref_cols_dot_parts(int l);
+bool launch_ref_cols_dot(const float2* ihat, const float2* AB, const RefPeak* pk, const double* stats, int K, int l,
+                         int ylo, int yhi, float scale, double* out, hipStream_t s);
+// z and gradient per pick from cg's `parts` partial sums per pick; when update, one ascent step
+// p += step * scale^2 * dz/dp.
+void launch_ref_update(RefPeak* pk, int K, const double* cg, int parts, const double* mu, const double* sigma, int l,
                        double step, const double* scale4, bool first, bool update, double* z, double* z0,
                        double* grad, hipStream_t s);
 // Reduce plane partials to (peak, index, sum, sum2) per (template, block): the record of

@@ -26,6 +26,9 @@
 //   k_ref_dot                 the standardisation applied to the spectra, c and its four
 //                             derivatives (phase ramps for the shift)
 //   k_ref_update              z, gradient, ascent step
+// For l = 320 and 720 the rotation (k_ref_mask_table once, then k_ref_rotate_rows) also does the
+// row pass of the forward transform, writing only the rows the mask leaves non-zero, and the
+// column pass is fused with the dot (k_ref_cols_dot): the spectra of A and B are never stored.
 #include <algorithm>
 #include <cmath>
 
@@ -259,8 +262,8 @@ struct RefRowCfg<720> {
 // sums 0-3 from the A plane's workgroup, sums 4-5 from the B plane's.
 template <int L>
 __global__ void __launch_bounds__(256)
-    k_ref_rotate_rows(const float2* __restrict__ H, const RefPeak* __restrict__ pk, float rin, float rout,
-                      float ehw, const float2* __restrict__ twL, float2* __restrict__ AB,
+    k_ref_rotate_rows(const float2* __restrict__ H, const RefPeak* __restrict__ pk,
+                      const float* __restrict__ mask, const float2* __restrict__ twL, float2* __restrict__ AB,
                       double* __restrict__ partial6, int K, int ylo, int yhi) {
   constexpr int T = RefRowCfg<L>::T, RPW = 64 / T, RPB = 4 * RPW, NG = L / RPB;
   static_assert(L % RPB == 0, "row groups");
@@ -296,25 +299,36 @@ __global__ void __launch_bounds__(256)
   dev::RegStage<L> twr;
   twr.load(twL, tid);
   double a[4] = {0, 0, 0, 0};
-  // pixels e = tid + 256 k in batches of BT, all gathers of a batch in flight together
+  // pixels e = tid + 256 k in batches of BT, all gathers of a batch in flight together; the mask
+  // factors of the next batch are loaded while the current one is processed
   constexpr int N = RPB * L, NE = (N + 255) / 256, BT = 4;
+  const float* Mrows = mask + (long long)y0 * L;
+  float mnext[BT];
+#pragma unroll
+  for (int b = 0; b < BT; ++b) {
+    const int e = tid + 256 * b;
+    mnext[b] = (b < NE && e < N) ? Mrows[e] : 0.f;
+  }
   for (int k0 = 0; k0 < NE; k0 += BT) {
     float2 h[BT][2][4];
     float bt[BT][2], bu[BT][2], mk[BT];
     bool act[BT][2];
 #pragma unroll
+    for (int b = 0; b < BT; ++b) mk[b] = mnext[b];
+#pragma unroll
     for (int b = 0; b < BT; ++b) {
       const int e = tid + 256 * (k0 + b);
       const int r = e / L, i = e - r * L, jrow = y0 + r;
-      // where the mask is zero every field is zero: no samples (the sums gain nothing either)
-      mk[b] = (k0 + b < NE && e < N) ? dev::mask_factor(i, jrow, L, rin, rout, ehw) : 0.f;
 #pragma unroll
       for (int q = 0; q < 2; ++q) {
+        act[b][q] = false;
+        // where the mask is zero every field is zero: no samples (the sums gain nothing either)
+        if ((q == 1 && !two) || mk[b] == 0.f) continue;
         const Bil c = bil_coords(i, jrow, L, cs[q], sn[q]);
-        act[b][q] = mk[b] != 0.f && (q == 0 || two) && c.ok;
+        act[b][q] = c.ok;
         bt[b][q] = c.t;
         bu[b][q] = c.u;
-        if (act[b][q]) {
+        if (c.ok) {
           const float2* Tq = Tp[q];
           h[b][q][0] = Tq[c.k0];
           h[b][q][1] = Tq[c.k1];
@@ -322,6 +336,11 @@ __global__ void __launch_bounds__(256)
           h[b][q][3] = Tq[c.k3];
         }
       }
+    }
+#pragma unroll
+    for (int b = 0; b < BT; ++b) {
+      const int e = tid + 256 * (k0 + BT + b);
+      mnext[b] = (k0 + BT + b < NE && e < N) ? Mrows[e] : 0.f;
     }
 #pragma unroll
     for (int b = 0; b < BT; ++b) {
@@ -392,6 +411,14 @@ __global__ void __launch_bounds__(256)
   for (int e = tid; e < RPB * L; e += 256) dst[e] = buf[e];
 }
 
+// The soft mask of the refinement's rotation (apply_mask, k:225-241, with the search's float
+// evaluation) as an l x l table.
+__global__ void k_ref_mask_table(int l, float rin, float rout, float ehw, float* __restrict__ M) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= l * l) return;
+  M[e] = dev::mask_factor(e % l, e / l, l, rin, rout, ehw);
+}
+
 // Per pick: the search's float mean/variance (k_rotation_stats) and the derivatives of the
 // mean and standard deviation.  stats[j] = {meanf, varf, mean, sd, dmean_th, dsd_th, dmean_df, dsd_df}
 __global__ void k_ref_stats(const double* __restrict__ partial6, int groups, int l, int K,
@@ -459,7 +486,7 @@ __global__ __launch_bounds__(256) void k_ref_dot(const float2* __restrict__ ihat
   float2* const dphx = LT ? sdphx : dyn + HX;
   float2* const phy = LT ? sphy : dyn + 2 * HX;
   float2* const dphy = LT ? sdphy : dyn + 2 * HX + L;
-  const int j = blockIdx.x;
+  const int j = blockIdx.x, part = blockIdx.y;
   const RefPeak P = pk[j];
   const int px = P.pixel % L, py = P.pixel / L;
   for (int k = threadIdx.x; k < HX; k += blockDim.x) phase(k, L, px, P.dx, phx[k], dphx[k]);
@@ -480,42 +507,64 @@ __global__ __launch_bounds__(256) void k_ref_dot(const float2* __restrict__ ihat
   const float2* B = AB + ((long long)K + j / 2) * L2;  // shared with pick j ^ 1
   const bool odd = j & 1;
   double acc[5] = {0, 0, 0, 0, 0};
-  for (int e = threadIdx.x; e < L * HX; e += blockDim.x) {
-    const int ky = e / HX, kx = e - ky * HX;
-    const int my = ky ? L - ky : 0, mx = kx ? L - kx : 0;
-    const float2 a = A[(long long)ky * L + kx], am = A[(long long)my * L + mx];
-    const float2 b = B[(long long)ky * L + kx], bm = B[(long long)my * L + mx];
-    // spectra of the three raw fields r0, dr0/dtheta, dr0/d ddf
-    float2 R = make_float2(0.5f * (a.x + am.x), 0.5f * (a.y - am.y));
-    float2 Gt = make_float2(0.5f * (a.y + am.y), -0.5f * (a.x - am.x));
-    float2 Gd = odd ? make_float2(0.5f * (b.y + bm.y), 0.5f * (bm.x - b.x))
-                    : make_float2(0.5f * (b.x + bm.x), 0.5f * (b.y - bm.y));
-    float2 Rc = R;  // r0 - mean
-    if (e == 0) {
-      R.x -= meanf;
-      Rc.x -= mean;
-      Gt.x -= dmt;
-      Gd.x -= dmd;
+  // this block's share [elo, ehi) of the half spectrum; elements e = elo + threadIdx.x + 256 k in
+  // batches of BT, the loads of a batch in flight together
+  constexpr int BT = 4;
+  const int NT = (int)(((long long)L * HX * (part + 1)) / kRefDotSplit);
+  const int elo = (int)(((long long)L * HX * part) / kRefDotSplit);
+  for (int e0 = elo + threadIdx.x; e0 < NT; e0 += BT * 256) {
+    float2 va[BT], vam[BT], vb[BT], vbm[BT], vx[BT];
+#pragma unroll
+    for (int q = 0; q < BT; ++q) {
+      const int e = e0 + 256 * q;
+      if (e < NT) {
+        const int ky = e / HX, kx = e - ky * HX;
+        const int my = ky ? L - ky : 0, mx = kx ? L - kx : 0;
+        va[q] = A[(long long)ky * L + kx];
+        vam[q] = A[(long long)my * L + mx];
+        vb[q] = B[(long long)ky * L + kx];
+        vbm[q] = B[(long long)my * L + mx];
+        vx[q] = I[e];
+      }
     }
-    const float2 T = make_float2(R.x * inv_s, R.y * inv_s);
-    const float2 Tt = make_float2(Gt.x * inv - Rc.x * dst * inv2, Gt.y * inv - Rc.y * dst * inv2);
-    const float2 Td = make_float2(Gd.x * inv - Rc.x * dsd * inv2, Gd.y * inv - Rc.y * dsd * inv2);
-    float2 X = I[e];
-    if ((kx + ky) & 1) X = make_float2(-X.x, -X.y);
-    const float w = (kx == 0 || 2 * kx == L) ? 1.f : 2.f;
-    const float2 ey = phy[ky], dey = dphy[ky];
-    const float2 E = cmul(phx[kx], ey);
-    const float2 Ex = cmul(dphx[kx], ey);
-    const float2 Ey = cmul(phx[kx], dey);
-    // X conj(T) and its partners
-    const float2 P0 = make_float2(X.x * T.x + X.y * T.y, X.y * T.x - X.x * T.y);
-    const float2 Pt = make_float2(X.x * Tt.x + X.y * Tt.y, X.y * Tt.x - X.x * Tt.y);
-    const float2 Pd = make_float2(X.x * Td.x + X.y * Td.y, X.y * Td.x - X.x * Td.y);
-    acc[0] += w * (P0.x * E.x - P0.y * E.y);
-    acc[1] += w * (P0.x * Ex.x - P0.y * Ex.y);
-    acc[2] += w * (P0.x * Ey.x - P0.y * Ey.y);
-    acc[3] += w * (Pt.x * E.x - Pt.y * E.y);
-    acc[4] += w * (Pd.x * E.x - Pd.y * E.y);
+#pragma unroll
+    for (int q = 0; q < BT; ++q) {
+      const int e = e0 + 256 * q;
+      if (e >= NT) break;
+      const int ky = e / HX, kx = e - ky * HX;
+      const float2 a = va[q], am = vam[q], b = vb[q], bm = vbm[q];
+      // spectra of the three raw fields r0, dr0/dtheta, dr0/d ddf
+      float2 R = make_float2(0.5f * (a.x + am.x), 0.5f * (a.y - am.y));
+      float2 Gt = make_float2(0.5f * (a.y + am.y), -0.5f * (a.x - am.x));
+      float2 Gd = odd ? make_float2(0.5f * (b.y + bm.y), 0.5f * (bm.x - b.x))
+                      : make_float2(0.5f * (b.x + bm.x), 0.5f * (b.y - bm.y));
+      float2 Rc = R;  // r0 - mean
+      if (e == 0) {
+        R.x -= meanf;
+        Rc.x -= mean;
+        Gt.x -= dmt;
+        Gd.x -= dmd;
+      }
+      const float2 T = make_float2(R.x * inv_s, R.y * inv_s);
+      const float2 Tt = make_float2(Gt.x * inv - Rc.x * dst * inv2, Gt.y * inv - Rc.y * dst * inv2);
+      const float2 Td = make_float2(Gd.x * inv - Rc.x * dsd * inv2, Gd.y * inv - Rc.y * dsd * inv2);
+      float2 X = vx[q];
+      if ((kx + ky) & 1) X = make_float2(-X.x, -X.y);
+      const float w = (kx == 0 || 2 * kx == L) ? 1.f : 2.f;
+      const float2 ey = phy[ky], dey = dphy[ky];
+      const float2 E = cmul(phx[kx], ey);
+      const float2 Ex = cmul(dphx[kx], ey);
+      const float2 Ey = cmul(phx[kx], dey);
+      // X conj(T) and its partners
+      const float2 P0 = make_float2(X.x * T.x + X.y * T.y, X.y * T.x - X.x * T.y);
+      const float2 Pt = make_float2(X.x * Tt.x + X.y * Tt.y, X.y * Tt.x - X.x * Tt.y);
+      const float2 Pd = make_float2(X.x * Td.x + X.y * Td.y, X.y * Td.x - X.x * Td.y);
+      acc[0] += w * (P0.x * E.x - P0.y * E.y);
+      acc[1] += w * (P0.x * Ex.x - P0.y * Ex.y);
+      acc[2] += w * (P0.x * Ey.x - P0.y * Ey.y);
+      acc[3] += w * (Pt.x * E.x - Pt.y * E.y);
+      acc[4] += w * (Pd.x * E.x - Pd.y * E.y);
+    }
   }
   __shared__ double red[4][5];
 #pragma unroll
@@ -532,7 +581,7 @@ __global__ __launch_bounds__(256) void k_ref_dot(const float2* __restrict__ ihat
   const double gt = red[0][3] + red[1][3] + red[2][3] + red[3][3];
   const double gd = red[0][4] + red[1][4] + red[2][4] + red[3][4];
   if (threadIdx.x == 0) {
-    double* o = out + (long long)j * 5;
+    double* o = out + ((long long)j * kRefDotSplit + part) * 5;
     o[0] = c;
     o[1] = gx;
     o[2] = gy;
@@ -541,8 +590,221 @@ __global__ __launch_bounds__(256) void k_ref_dot(const float2* __restrict__ ihat
   }
 }
 
+// The forward column pass of the A and B planes fused with k_ref_dot (l = 320, 720): one
+// workgroup = NU mirrored column pairs (u, l - u; pair 0 = columns 0 and l/2) of the three planes
+// of a pick pair (B_m, A_2m, A_2m+1), transformed in LDS one plane at a time, each plane's
+// spectrum consumed where it is made: the transformed planes never reach memory.  The defocus
+// term of the dot is linear in the B and the A spectrum separately,
+//   Re[X conj(Gd inv - Rc dsd inv2) E] = inv Re[X conj(Gd) E] - dsd inv2 Re[X conj(Rc) E],
+// so the B pass accumulates the first part for both picks and each A pass the second.  Rows
+// outside [ylo, yhi] of AB are zero and not read; ihat is the block half spectra [b][kx][ky].
+// out[(j * G + g) * 5 + k]: the five partial sums of pick j over workgroup g's columns.
+template <int L>
+__global__ void __launch_bounds__(256)
+    k_ref_cols_dot(const float2* __restrict__ ihat, const float2* __restrict__ AB, const RefPeak* __restrict__ pk,
+                   const double* __restrict__ stats, const float2* __restrict__ twL, int K, int ylo, int yhi,
+                   float scale, double* __restrict__ out) {
+  constexpr int T = RefRowCfg<L>::T, COLS = 256 / T, NU = COLS / 2, LP = L + 1, HX = L / 2 + 1;
+  constexpr int G = (L / 2) / NU;
+  static_assert((L / 2) % NU == 0 && COLS % 2 == 0, "column pairs");
+  constexpr int NO = (NU * L + L + 255) / 256;  // owned half-spectrum elements per thread, at most
+  __shared__ float2 buf[COLS * LP];
+  __shared__ float2 tw[L];
+  __shared__ float2 phy[2][L], dphy[L];
+  __shared__ float2 phx[2][NU + 1], dphx[2][NU + 1];
+  __shared__ double red[4][5];
+  const int tid = threadIdx.x, g = blockIdx.x, m = blockIdx.y;
+  const int j0 = 2 * m;
+  const int nq = j0 + 1 < K ? 2 : 1;
+  const long long L2 = (long long)L * L;
+  // slot cc < NU: column u = g NU + cc; slot NU + v: the mirror column of u = g NU + v
+  auto col_of = [&](int cc) {
+    const int u = g * NU + cc % NU;
+    return cc < NU ? u : (u == 0 ? L / 2 : L - u);
+  };
+  // owned elements o = tid + 256 k: slot o / L (the extra slot NU, column l/2, in workgroup 0), ky = o % L
+  const int nown = NU * L + (g == 0 ? L : 0);
+  const float2* const Ib0 = ihat + (long long)pk[j0].block * HX * L;
+  const float2* const Ib1 = ihat + (long long)pk[min(j0 + 1, K - 1)].block * HX * L;
+  // block spectrum X of pick q at owned element o (clamped into the owned range, so that the
+  // loads of a batch need no branch)
+  auto xat = [&](int q, int o) __attribute__((always_inline)) {
+    o = min(o, nown - 1);
+    return (q == 0 ? Ib0 : Ib1)[col_of(o / L) * L + o % L];
+  };
+  dev::RegStage<L> twr;
+  twr.load(twL, tid);
+  // phase factors of both picks (shift derivative along y: pick j0 first)
+  for (int q = 0; q < nq; ++q) {
+    const RefPeak P = pk[j0 + q];
+    const int px = P.pixel % L, py = P.pixel / L;
+    for (int k = tid; k < L; k += 256) {
+      float2 a, d;
+      phase(k, L, py, P.dy, a, d);
+      phy[q][k] = a;
+      if (q == 0) dphy[k] = d;
+    }
+    for (int sl = tid; sl <= NU; sl += 256) phase(sl < NU ? g * NU + sl : L / 2, L, px, P.dx, phx[q][sl], dphx[q][sl]);
+  }
+  // Each pass works from its own copy of the thread index, opaque to the compiler: otherwise the
+  // per-thread LDS and global addresses of the transform and of the element loops, identical in
+  // every pass, are computed once and held in registers for the whole kernel (some seventy more).
+  auto pass_tid = [&]() __attribute__((always_inline)) {
+    int t = tid;
+    asm volatile("" : "+v"(t));
+    return t;
+  };
+  // a thread's elements of a plane lie in one column (256 is a multiple of COLS), rows YS apart
+  constexpr int NE = (COLS * L + 255) / 256, YS = 256 / COLS;
+  auto load_plane = [&](const float2* P, int t) __attribute__((always_inline)) {
+    const int lcol = t % COLS, ybase = t / COLS;
+    const float2* src = P + col_of(lcol);
+    float2 v[NE];
+#pragma unroll
+    for (int k = 0; k < NE; ++k) {
+      const int y = ybase + YS * k;
+      v[k] = (y < L && y >= ylo && y <= yhi) ? src[y * L] : make_float2(0.f, 0.f);
+    }
+#pragma unroll
+    for (int k = 0; k < NE; ++k) {
+      const int y = ybase + YS * k;
+      if (y < L) buf[lcol * LP + y] = v[k];
+    }
+  };
+  auto transform = [&](int t) __attribute__((always_inline)) {
+    __syncthreads();
+    ct::run_stockham<L, T, -1, ct::Sync::WAVE>(buf + (t / T) * LP, tw, t % T, typename RefRowCfg<L>::Plan{});
+    __syncthreads();
+  };
+  // element o: its value and its mirror's (-kx, -ky), scaled as the plane transform scales
+  auto pair_at = [&](int o, float2& v, float2& vm) {
+    const int cc = o / L, ky = o % L, kx = col_of(cc);
+    const int mc = (cc < NU && kx != 0) ? cc + NU : cc, my = ky ? L - ky : 0;
+    v = buf[cc * LP + ky];
+    vm = buf[mc * LP + my];
+    v = make_float2(v.x * scale, v.y * scale);
+    vm = make_float2(vm.x * scale, vm.y * scale);
+  };
+  auto reduce_store = [&](double (&acc)[5], int j) {
+#pragma unroll
+    for (int k = 0; k < 5; ++k)
+      for (int o = 32; o > 0; o >>= 1) acc[k] += __shfl_down(acc[k], o, 64);
+    const int wave = tid / 64, lane = tid % 64;
+    if (lane == 0)
+#pragma unroll
+      for (int k = 0; k < 5; ++k) red[wave][k] = acc[k];
+    __syncthreads();
+    if (tid < 5) out[((long long)j * G + g) * 5 + tid] = red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid];
+  };
+
+  // B pass: Re[X conj(Gd) E] of both picks, owned elements in batches of EB with the
+  // block-spectrum loads of a batch in flight together
+  const int tb = pass_tid();
+  load_plane(AB + ((long long)K + m) * L2, tb);
+  twr.store(tw, tid);
+  transform(tb);
+  double sg[2] = {0, 0};
+  constexpr int EB = 4;
+#pragma unroll 1
+  for (int k0 = 0; k0 < NO; k0 += EB) {
+    float2 xb[2][EB];
+#pragma unroll
+    for (int k = 0; k < EB; ++k) {
+      xb[0][k] = xat(0, tb + 256 * (k0 + k));
+      xb[1][k] = xat(1, tb + 256 * (k0 + k));
+    }
+#pragma unroll
+    for (int k = 0; k < EB; ++k) {
+      const int o = tb + 256 * (k0 + k);
+      if (o >= nown) break;
+      const int cc = o / L, ky = o % L, kx = col_of(cc);
+      float2 b, bm;
+      pair_at(o, b, bm);
+      const float w = (kx == 0 || 2 * kx == L) ? 1.f : 2.f;
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        if (q >= nq) break;
+        float2 Gd = q == 0 ? make_float2(0.5f * (b.x + bm.x), 0.5f * (b.y - bm.y))
+                           : make_float2(0.5f * (b.y + bm.y), 0.5f * (bm.x - b.x));
+        if (o == 0 && g == 0) Gd.x -= (float)stats[(long long)(j0 + q) * 8 + 6];
+        float2 X = xb[q][k];
+        if ((kx + ky) & 1) X = make_float2(-X.x, -X.y);
+        const float2 E = cmul(phx[q][cc], phy[q][ky]);
+        const float2 Pg = make_float2(X.x * Gd.x + X.y * Gd.y, X.y * Gd.x - X.x * Gd.y);
+        sg[q] += w * (Pg.x * E.x - Pg.y * E.y);
+      }
+    }
+  }
+  // A passes, one per pick
+  for (int q = 0; q < nq; ++q) {
+    const int j = j0 + q;
+    __syncthreads();  // the previous pass has read buf (and dphy)
+    if (q == 1) {
+      const RefPeak P = pk[j];
+      const int py = P.pixel / L;
+      for (int k = tid; k < L; k += 256) {
+        float2 a;
+        phase(k, L, py, P.dy, a, dphy[k]);
+      }
+    }
+    const int ta = pass_tid();
+    load_plane(AB + (long long)j * L2, ta);
+    transform(ta);
+    const double* st = stats + (long long)j * 8;
+    const float meanf = (float)st[0], varf = (float)st[1];
+    const float mean = (float)st[2], sd = (float)st[3];
+    const float dmt = (float)st[4], dst = (float)st[5], dsd = (float)st[7];
+    const float inv_s = varf != 0 ? 1.0f / sqrtf(varf) : 1.0f;
+    const float inv = 1.0f / sd, inv2 = 1.0f / (sd * sd);
+    double acc[5] = {0, 0, 0, 0, 0};
+    double sr = 0;
+#pragma unroll 1
+    for (int k0 = 0; k0 < NO; k0 += EB) {
+      float2 xa[EB];
+#pragma unroll
+      for (int k = 0; k < EB; ++k) xa[k] = xat(q, ta + 256 * (k0 + k));
+#pragma unroll
+      for (int k = 0; k < EB; ++k) {
+        const int o = ta + 256 * (k0 + k);
+        if (o >= nown) break;
+        const int cc = o / L, ky = o % L, kx = col_of(cc);
+        float2 a, am;
+        pair_at(o, a, am);
+        // spectra of r0 and dr0/dtheta
+        float2 R = make_float2(0.5f * (a.x + am.x), 0.5f * (a.y - am.y));
+        float2 Gt = make_float2(0.5f * (a.y + am.y), -0.5f * (a.x - am.x));
+        float2 Rc = R;  // r0 - mean
+        if (o == 0 && g == 0) {
+          R.x -= meanf;
+          Rc.x -= mean;
+          Gt.x -= dmt;
+        }
+        const float2 Tn = make_float2(R.x * inv_s, R.y * inv_s);
+        const float2 Tt = make_float2(Gt.x * inv - Rc.x * dst * inv2, Gt.y * inv - Rc.y * dst * inv2);
+        float2 X = xa[k];
+        if ((kx + ky) & 1) X = make_float2(-X.x, -X.y);
+        const float w = (kx == 0 || 2 * kx == L) ? 1.f : 2.f;
+        const float2 ey = phy[q][ky], dey = dphy[ky];
+        const float2 E = cmul(phx[q][cc], ey);
+        const float2 Ex = cmul(dphx[q][cc], ey);
+        const float2 Ey = cmul(phx[q][cc], dey);
+        const float2 P0 = make_float2(X.x * Tn.x + X.y * Tn.y, X.y * Tn.x - X.x * Tn.y);
+        const float2 Pt = make_float2(X.x * Tt.x + X.y * Tt.y, X.y * Tt.x - X.x * Tt.y);
+        const float2 Pr = make_float2(X.x * Rc.x + X.y * Rc.y, X.y * Rc.x - X.x * Rc.y);
+        acc[0] += w * (P0.x * E.x - P0.y * E.y);
+        acc[1] += w * (P0.x * Ex.x - P0.y * Ex.y);
+        acc[2] += w * (P0.x * Ey.x - P0.y * Ey.y);
+        acc[3] += w * (Pt.x * E.x - Pt.y * E.y);
+        sr += w * (Pr.x * E.x - Pr.y * E.y);
+      }
+    }
+    acc[4] = (double)inv * sg[q] - (double)dsd * (double)inv2 * sr;
+    reduce_store(acc, j);
+  }
+}
+
 // z, gradient and (unless last) one ascent step p += step * scale^2 * dz/dp.
-__global__ void k_ref_update(RefPeak* __restrict__ pk, int K, const double* __restrict__ cg,
+__global__ void k_ref_update(RefPeak* __restrict__ pk, int K, const double* __restrict__ cg, int parts,
                              const double* __restrict__ mu, const double* __restrict__ sigma, int l,
                              double step, double4 scale, int first, int update,
                              double* __restrict__ zout, double* __restrict__ z0out,
@@ -552,7 +814,9 @@ __global__ void k_ref_update(RefPeak* __restrict__ pk, int K, const double* __re
   RefPeak P = pk[j];
   const long long q = (long long)P.block * l * l + P.pixel;
   const double m = mu[q], s = sigma[q];
-  const double* o = cg + (long long)j * 5;
+  double o[5] = {0, 0, 0, 0, 0};
+  for (int part = 0; part < parts; ++part)
+    for (int k = 0; k < 5; ++k) o[k] += cg[((long long)j * parts + part) * 5 + k];
   const double z = s > 0 ? (o[0] - m) / s : 0.0;
   const double inv = s > 0 ? 1.0 / s : 0.0;
   const double g0 = o[1] * inv, g1 = o[2] * inv, g2 = o[3] * inv, g3 = o[4] * inv;
@@ -639,28 +903,31 @@ void launch_ref_rotate(const float2* H, const RefPeak* pk, int K, int l, float d
   GS_CHECK_LAUNCH();
 }
 
-bool launch_ref_rotate_rows(const float2* H, const RefPeak* pk, int K, int l, float d_m, float ehw,
-                            double* partial6, double* stats, float2* AB, int* ylo, int* yhi, hipStream_t s) {
-  const float2* tw = plane_twiddles(l);
-  if (!tw || K <= 0) return false;
+void launch_ref_mask(int l, float d_m, float ehw, float* mask, int* ylo, int* yhi, hipStream_t s) {
   const float dm = 1.5f * d_m;  // apply_mask: d_m = 1.5 * d_m (k:233)
   const float rin = dm / 2, rout = dm / 2 + 2 * ehw;
   // rows y with |y - l/2| > rout hold only masked (zero) pixels
   *ylo = std::max(0, (int)std::floor(l / 2 - (double)rout));
   *yhi = std::min(l - 1, (int)std::ceil(l / 2 + (double)rout));
+  k_ref_mask_table<<<nblk((long long)l * l, BS), BS, 0, s>>>(l, rin, rout, ehw, mask);
+  GS_CHECK_LAUNCH();
+}
+
+bool launch_ref_rotate_rows(const float2* H, const RefPeak* pk, int K, int l, const float* mask, int ylo,
+                            int yhi, double* partial6, double* stats, float2* AB, hipStream_t s) {
+  const float2* tw = plane_twiddles(l);
+  if (!tw || K <= 0) return false;
   const int planes = K + (K + 1) / 2;
   GS_REQUIRE(planes < 65536, kInvalid, "refinement batch too large");
   int groups = 0;
   if (l == 320) {
     constexpr int RPB = 4 * (64 / RefRowCfg<320>::T);
     groups = 320 / RPB;
-    k_ref_rotate_rows<320><<<dim3(groups, planes), 256, 0, s>>>(H, pk, rin, rout, ehw, tw, AB, partial6, K, *ylo,
-                                                                 *yhi);
+    k_ref_rotate_rows<320><<<dim3(groups, planes), 256, 0, s>>>(H, pk, mask, tw, AB, partial6, K, ylo, yhi);
   } else if (l == 720) {
     constexpr int RPB = 4 * (64 / RefRowCfg<720>::T);
     groups = 720 / RPB;
-    k_ref_rotate_rows<720><<<dim3(groups, planes), 256, 0, s>>>(H, pk, rin, rout, ehw, tw, AB, partial6, K, *ylo,
-                                                                 *yhi);
+    k_ref_rotate_rows<720><<<dim3(groups, planes), 256, 0, s>>>(H, pk, mask, tw, AB, partial6, K, ylo, yhi);
   } else {
     return false;
   }
@@ -673,24 +940,44 @@ bool launch_ref_rotate_rows(const float2* H, const RefPeak* pk, int K, int l, fl
 bool launch_ref_dot(const float2* ihat, const float2* AB, const RefPeak* pk, const double* stats, int K, int l,
                     double* out, hipStream_t s) {
   if (l == 320) {
-    k_ref_dot<320><<<K, 256, 0, s>>>(ihat, AB, pk, stats, K, out, l);
+    k_ref_dot<320><<<dim3(K, kRefDotSplit), 256, 0, s>>>(ihat, AB, pk, stats, K, out, l);
   } else if (l == 720) {
-    k_ref_dot<720><<<K, 256, 0, s>>>(ihat, AB, pk, stats, K, out, l);
+    k_ref_dot<720><<<dim3(K, kRefDotSplit), 256, 0, s>>>(ihat, AB, pk, stats, K, out, l);
   } else {
     if (l <= 0 || l % 2) return false;
     const size_t lds = sizeof(float2) * (size_t)(2 * (l / 2 + 1) + 2 * l);
     GS_REQUIRE(lds <= 64 * 1024, kUnsupported, "window_size too large for the refinement");
-    k_ref_dot<0><<<K, 256, lds, s>>>(ihat, AB, pk, stats, K, out, l);
+    k_ref_dot<0><<<dim3(K, kRefDotSplit), 256, lds, s>>>(ihat, AB, pk, stats, K, out, l);
   }
   GS_CHECK_LAUNCH();
   return true;
 }
 
-void launch_ref_update(RefPeak* pk, int K, const double* cg, const double* mu, const double* sigma, int l,
+int ref_cols_dot_parts(int l) {
+  if (l == 320) return 320 / (256 / RefRowCfg<320>::T);
+  if (l == 720) return 720 / (256 / RefRowCfg<720>::T);
+  return 0;
+}
+
+bool launch_ref_cols_dot(const float2* ihat, const float2* AB, const RefPeak* pk, const double* stats, int K, int l,
+                         int ylo, int yhi, float scale, double* out, hipStream_t s) {
+  const float2* tw = plane_twiddles(l);
+  const int G = ref_cols_dot_parts(l);
+  if (!tw || !G || K <= 0) return false;
+  const dim3 grid(G, (K + 1) / 2);
+  if (l == 320)
+    k_ref_cols_dot<320><<<grid, 256, 0, s>>>(ihat, AB, pk, stats, tw, K, ylo, yhi, scale, out);
+  else
+    k_ref_cols_dot<720><<<grid, 256, 0, s>>>(ihat, AB, pk, stats, tw, K, ylo, yhi, scale, out);
+  GS_CHECK_LAUNCH();
+  return true;
+}
+
+void launch_ref_update(RefPeak* pk, int K, const double* cg, int parts, const double* mu, const double* sigma, int l,
                        double step, const double* scale4, bool first, bool update, double* z, double* z0,
                        double* grad, hipStream_t s) {
   const double4 sc = make_double4(scale4[0], scale4[1], scale4[2], scale4[3]);
-  k_ref_update<<<nblk(K, 64), 64, 0, s>>>(pk, K, cg, mu, sigma, l, step, sc, first ? 1 : 0, update ? 1 : 0,
+  k_ref_update<<<nblk(K, 64), 64, 0, s>>>(pk, K, cg, parts, mu, sigma, l, step, sc, first ? 1 : 0, update ? 1 : 0,
                                           z, z0, grad);
   GS_CHECK_LAUNCH();
 }
