@@ -389,13 +389,13 @@ __global__ void __launch_bounds__(256)
     const int j = (g / gx) * kRotTY + threadIdx.x / kRotTX + 8 * h;
     if (i >= l || j >= l) continue;
     float ra = 0.f, rb = 0.f;
+    const float r = hypotf((float)(i - l / 2), (float)(j - l / 2));
     const Bil c = bil_coords(i, j, l, cose, sine);
-    if (c.ok) {
+    if (c.ok && !(r > rout)) {  // outside the mask the sample is not needed: it is zeroed
       const float tt = 1.0f - c.t, uu = 1.0f - c.u;
       ra = A[c.k0] * tt * uu + A[c.k1] * c.t * uu + A[c.k2] * c.t * c.u + A[c.k3] * tt * c.u;
       if (hb) rb = B[c.k0] * tt * uu + B[c.k1] * c.t * uu + B[c.k2] * c.t * c.u + B[c.k3] * tt * c.u;
     }
-    const float r = hypotf((float)(i - l / 2), (float)(j - l / 2));
     if (r > rout) {
       ra = 0.f;
       rb = 0.f;
