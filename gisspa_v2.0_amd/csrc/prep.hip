@@ -458,11 +458,23 @@ __global__ void __launch_bounds__(256)
   const int kx0 = blockIdx.x * TS, ky0 = blockIdx.y * TS, p = blockIdx.z;
   const float2* P = Z + (long long)p * l * l;
   const int c = threadIdx.x % TS;
-  for (int r = threadIdx.x / TS; r < TS; r += 256 / TS) {
-    const int ky = ky0 + r, kx = kx0 + c;
+  // all loads of a thread issued before the first is used
+  constexpr int NR = TS / (256 / TS);
+  float2 vk[NR], vm[NR];
+#pragma unroll
+  for (int q = 0; q < NR; ++q) {
+    const int r = threadIdx.x / TS + q * (256 / TS), ky = ky0 + r, kx = kx0 + c;
     if (ky < l && kx < hx) {
-      zk[r][c] = P[(long long)ky * l + kx];
-      zm[r][c] = P[(long long)(ky ? l - ky : 0) * l + (kx ? l - kx : 0)];
+      vk[q] = P[(long long)ky * l + kx];
+      vm[q] = P[(long long)(ky ? l - ky : 0) * l + (kx ? l - kx : 0)];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < NR; ++q) {
+    const int r = threadIdx.x / TS + q * (256 / TS), ky = ky0 + r, kx = kx0 + c;
+    if (ky < l && kx < hx) {
+      zk[r][c] = vk[q];
+      zm[r][c] = vm[q];
     }
   }
   __syncthreads();
