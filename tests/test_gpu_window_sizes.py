@@ -1,5 +1,6 @@
 """Window sizes other than 320 and 720 (the generic correlation path: runtime-planned transforms
-instead of the tiled kernels) against the FP64 oracle, through the C ABI.
+instead of the tiled kernels), and the refinement at all three kinds of window, against the FP64
+oracle, through the C ABI.
 
 The reference plans its transforms for any window_size (src/norm.cu:93-116); so does this
 build.  Same bar as the l = 320 parity tests: per-pixel z of the best record within 1e-4,
@@ -72,11 +73,14 @@ def test_nonorm_window_size(gs, orc):
     eng.close()
 
 
-def test_refinement_window_size(gs, orc):
+@pytest.mark.parametrize("l,npk", [(256, 4), (320, 5), (720, 5)])
+def test_refinement_window_size(gs, orc, l, npk):
+    """Refinement at a window without LDS plane transforms (256: runtime FFT, k_ref_rotate,
+    k_ref_dot) and at both fused sizes (320, 720: k_ref_rotate_rows, k_ref_cols_dot) with an odd
+    pick count, so that the last B plane holds one pick."""
     import synth
 
-    l = 256
-    print(f"refinement window {l}", flush=True)
+    print(f"refinement window {l}, {npk} picks", flush=True)
     cfg = gs.SearchConfig(angpix=1.0, phistep=10.0, kk=3.0, energy=300.0, cs=2.7, highres=8.0,
                           lowres=100.0, diameter=128.0, threshold=5.0, window_size=l, overlap=0,
                           norm_type=1, invert=0, phase_flip=0, nms_radius=64.0)
@@ -90,8 +94,12 @@ def test_refinement_window_size(gs, orc):
     pk = eng.peaks_from_records(recs)
     z, _ = eng.score_and_grad(pk)
     np.testing.assert_allclose(z, recs["score"], rtol=2e-5)
+    k = min(npk, len(recs))
+    if npk % 2 and k % 2 == 0:
+        k -= 1  # keep the pick count odd
+    assert k >= 3
     rng = np.random.default_rng(9)
-    pk = eng.peaks_from_records(recs[:4])
+    pk = eng.peaks_from_records(recs[:k])
     pk["dx"] = rng.uniform(-0.5, 0.5, len(pk))
     pk["dy"] = rng.uniform(-0.5, 0.5, len(pk))
     pk["theta"] += rng.uniform(-2.5, 2.5, len(pk))
