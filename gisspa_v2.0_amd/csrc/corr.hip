@@ -937,19 +937,22 @@ __global__ void __launch_bounds__(256)
 
 // Columns of planes whose rows are already transformed; input rows y that are zero by `band`, or
 // outside [ylo, yhi] (all rows for the plane transforms), are taken as zero without being read.
+// Workgroups whose columns all lie outside [clo, chi] do nothing (those columns are left as the
+// row pass wrote them).
 template <int L, int SIGN>
 __global__ void __launch_bounds__(256)
     k_plane_cols(float2* __restrict__ data, const float2* __restrict__ twL, float scale, int band, int ylo,
-                 int yhi) {
+                 int yhi, int clo, int chi) {
   using C = CorrCfg<L>;
   constexpr int T = C::TPC, COLS = 256 / T, LP = L + 1;
   static_assert(L % COLS == 0, "column groups");
   __shared__ float2 buf[COLS * LP];
   __shared__ float2 tw[L];
   const int tid = threadIdx.x;
+  const int c0 = blockIdx.x * COLS;
+  if (c0 > chi || c0 + COLS - 1 < clo) return;  // no column the caller needs (uniform per workgroup)
   dev::RegStage<L> twr;
   twr.load(twL, tid);
-  const int c0 = blockIdx.x * COLS;
   float2* P = data + (long long)blockIdx.y * L * L;
   dev::burst_copy<COLS * L, 24>(
       tid,
@@ -971,7 +974,7 @@ __global__ void __launch_bounds__(256)
 
 template <int L>
 void plane_fft2_impl(const float2* in, float2* data, int batch, int sign, float scale, const float2* tw,
-                     int band, hipStream_t s) {
+                     int band, int clo, int chi, hipStream_t s) {
   using C = CorrCfg<L>;
   constexpr int RPB = 4 * (64 / C::TPC), COLS = 256 / C::TPC;
   const long long nrows = (long long)batch * L;
@@ -980,10 +983,10 @@ void plane_fft2_impl(const float2* in, float2* data, int batch, int sign, float 
   const dim3 gc(L / COLS, (unsigned)batch);
   if (sign < 0) {
     k_plane_rows<L, -1><<<(unsigned)gr, 256, 0, s>>>(in, data, nrows, tw, 1.f, band);
-    k_plane_cols<L, -1><<<gc, 256, 0, s>>>(data, tw, scale, band, 0, L - 1);
+    k_plane_cols<L, -1><<<gc, 256, 0, s>>>(data, tw, scale, band, 0, L - 1, clo, chi);
   } else {
     k_plane_rows<L, +1><<<(unsigned)gr, 256, 0, s>>>(in, data, nrows, tw, 1.f, band);
-    k_plane_cols<L, +1><<<gc, 256, 0, s>>>(data, tw, scale, band, 0, L - 1);
+    k_plane_cols<L, +1><<<gc, 256, 0, s>>>(data, tw, scale, band, 0, L - 1, clo, chi);
   }
   GS_CHECK_LAUNCH();
 }
@@ -991,15 +994,15 @@ void plane_fft2_impl(const float2* in, float2* data, int batch, int sign, float 
 }  // namespace
 
 bool launch_plane_fft2_from(const float2* in, float2* out, int batch, int L, int sign, float scale, hipStream_t s,
-                            int band) {
+                            int band, int clo, int chi) {
   TwTables& T = tw_tables(current_device());
   if (band < 0 || band > L / 2) band = L / 2;
   if (L == 320) {
-    plane_fft2_impl<320>(in, out, batch, sign, scale, T.t320, band, s);
+    plane_fft2_impl<320>(in, out, batch, sign, scale, T.t320, band, clo, chi < 0 ? L - 1 : chi, s);
     return true;
   }
   if (L == 720) {
-    plane_fft2_impl<720>(in, out, batch, sign, scale, T.t720, band, s);
+    plane_fft2_impl<720>(in, out, batch, sign, scale, T.t720, band, clo, chi < 0 ? L - 1 : chi, s);
     return true;
   }
   return false;

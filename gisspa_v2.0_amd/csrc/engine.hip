@@ -685,7 +685,9 @@ class Engine {
       };
       for (int it = 0; it <= iters; ++it) {
         launch_ref_template(d_twhite_.p, tab, ntab, rp, K, l, dp_, part, count, H, stream_);
-        if (!launch_plane_fft2_from(H, Tt, K, l, +1, 1.f, stream_, band)) {
+        // the rotation's bilinear taps read only columns (and rows) [ylo - 1, yhi + 1] of the
+        // processed templates: rotation keeps the distance to the centre, which the mask bounds
+        if (!launch_plane_fft2_from(H, Tt, K, l, +1, 1.f, stream_, band, ylo - 2, yhi + 2)) {
           GS_HIP(hipMemcpyAsync(Tt, H, sizeof(float2) * (size_t)K * L2, hipMemcpyDeviceToDevice, stream_));
           fft2_batched(fft_, Tt, S, K, l, l, +1, 1.f, stream_);
         }

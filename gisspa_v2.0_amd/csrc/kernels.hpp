@@ -119,10 +119,11 @@ void corr_prepare();
 // then multiplied by scale; LDS transforms for L = 320 and 720 (returns false otherwise).
 bool launch_plane_fft2(float2* data, int batch, int L, int sign, float scale, hipStream_t s);
 // The same transform of `in` written to `out` (in is left unchanged).  Input rows y with
-// min(y, L - y) > band are taken as zero without being read (default: every row is read); the
-// result in `out` is complete either way.
+// min(y, L - y) > band are taken as zero without being read (default: every row is read).  The
+// result in `out` is complete in the columns [clo, chi] (default: all; the second pass skips
+// the groups of columns outside).
 bool launch_plane_fft2_from(const float2* in, float2* out, int batch, int L, int sign, float scale, hipStream_t s,
-                            int band = -1);
+                            int band = -1, int clo = 0, int chi = -1);
 // The flat twiddle table exp(-2 pi i e / L) of the plane transforms on the current device
 // (L = 320, 720; nullptr otherwise).
 const float2* plane_twiddles(int L);
