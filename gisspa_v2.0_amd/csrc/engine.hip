@@ -662,6 +662,7 @@ class Engine {
     double* part6 = d_rpart6_.ensure((size_t)Kc * rotation_groups(l) * 6);
     double* stats = d_rstats_.ensure((size_t)Kc * 8);
     double* cg = d_rcg_.ensure((size_t)Kc * std::max(kRefDotSplit, ref_cols_dot_parts(l)) * 5);
+    float2* phs = d_rph_.ensure((size_t)Kc * (2 * l + 2 * hx));
     double* dz = d_rz_.ensure(Kc);
     double* dz0 = d_rz0_.ensure(Kc);
     double* dg = d_rgrad_.ensure((size_t)Kc * 4);
@@ -695,8 +696,8 @@ class Engine {
         // forward transform is fused with the dot, so the spectra of AB are never stored
         int parts = kRefDotSplit;
         if (launch_ref_rotate_rows(Tt, rp, K, l, mtab, ylo, yhi, part6, stats, AB, stream_)) {
-          GS_REQUIRE(launch_ref_cols_dot(d_ihat_.p, AB, rp, stats, K, l, ylo, yhi, (float)(1.0 / (double)L2), cg,
-                                         stream_),
+          GS_REQUIRE(launch_ref_cols_dot(d_ihat_.p, AB, rp, stats, K, l, ylo, yhi, (float)(1.0 / (double)L2), phs,
+                                         cg, stream_),
                      kUnsupported, "window_size not supported");
           parts = ref_cols_dot_parts(l);
         } else {
@@ -1316,6 +1317,7 @@ class Engine {
   std::map<long long, std::unique_ptr<DevBuf<double>>> rb_cache_;
 
   DevBuf<float> d_rmask_;
+  DevBuf<float2> d_rph_;
   DevBuf<float> d_raw_, d_img_, d_imgpre_, d_tprep_, d_stats_, d_best_, d_tile_;
   DevBuf<float2> d_work_, d_work2_, d_scratch_, d_frame_, d_frame2_, d_blocks_, d_ihat_, d_that_, d_Z_, d_cs_;
   DevBuf<double> d_mom_, d_ra_, d_ra_img_, d_m_, d_part_, d_S_, d_SS_, d_mu_, d_sigma_, d_planes_, d_chpart_;

@@ -186,11 +186,12 @@ bool launch_ref_dot(const float2* ihat, const float2* AB, const RefPeak* pk, con
                     double* out, hipStream_t s);
 // The column pass of the AB planes (left by launch_ref_rotate_rows, rows outside [ylo, yhi]
 // unwritten) fused with launch_ref_dot, the spectra scaled by `scale` (l = 320, 720; returns false
-// otherwise); AB is only read.  ihat: the block half spectra [b][kx 0..l/2][ky].  out: K x
-// ref_cols_dot_parts(l) x 5 partial sums.
+// otherwise); AB is only read.  ihat: the block half spectra [b][kx 0..l/2][ky].  ph: scratch for
+// the picks' shift phase tables, K x (2 l + 2 (l/2 + 1)) entries.  out: K x ref_cols_dot_parts(l)
+// x 5 partial sums.
 int ref_cols_dot_parts(int l);
 bool launch_ref_cols_dot(const float2* ihat, const float2* AB, const RefPeak* pk, const double* stats, int K, int l,
-                         int ylo, int yhi, float scale, double* out, hipStream_t s);
+                         int ylo, int yhi, float scale, float2* ph, double* out, hipStream_t s);
 // z and gradient per pick from cg's `parts` partial sums per pick; when update, one ascent step
 // p += step * scale^2 * dz/dp.
 void launch_ref_update(RefPeak* pk, int K, const double* cg, int parts, const double* mu, const double* sigma, int l,

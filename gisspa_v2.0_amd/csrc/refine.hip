@@ -590,6 +590,20 @@ __global__ __launch_bounds__(256) void k_ref_dot(const float2* __restrict__ ihat
   }
 }
 
+// Per pick the shift phase factors of k_ref_dot: ph[j] = {phase(ky) for ky < l, its derivative
+// for ky < l, phase(kx) for kx <= l/2, its derivative}, 2 l + 2 (l/2 + 1) entries; grid (., K).
+__global__ void k_ref_phases(const RefPeak* __restrict__ pk, int l, float2* __restrict__ ph) {
+  const int j = blockIdx.y, HX = l / 2 + 1, n = 2 * l + 2 * HX;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= l + HX) return;
+  const RefPeak P = pk[j];
+  float2* o = ph + (long long)j * n;
+  if (e < l)
+    phase(e, l, P.pixel / l, P.dy, o[e], o[l + e]);
+  else
+    phase(e - l, l, P.pixel % l, P.dx, o[2 * l + e - l], o[2 * l + HX + e - l]);
+}
+
 // The forward column pass of the A and B planes fused with k_ref_dot (l = 320, 720): one
 // workgroup = NU mirrored column pairs (u, l - u; pair 0 = columns 0 and l/2) of the three planes
 // of a pick pair (B_m, A_2m, A_2m+1), transformed in LDS one plane at a time, each plane's
@@ -602,8 +616,8 @@ __global__ __launch_bounds__(256) void k_ref_dot(const float2* __restrict__ ihat
 template <int L>
 __global__ void __launch_bounds__(256)
     k_ref_cols_dot(const float2* __restrict__ ihat, const float2* __restrict__ AB, const RefPeak* __restrict__ pk,
-                   const double* __restrict__ stats, const float2* __restrict__ twL, int K, int ylo, int yhi,
-                   float scale, double* __restrict__ out) {
+                   const double* __restrict__ stats, const float2* __restrict__ ph, const float2* __restrict__ twL,
+                   int K, int ylo, int yhi, float scale, double* __restrict__ out) {
   constexpr int T = RefRowCfg<L>::T, COLS = 256 / T, NU = COLS / 2, LP = L + 1, HX = L / 2 + 1;
   constexpr int G = (L / 2) / NU;
   static_assert((L / 2) % NU == 0 && COLS % 2 == 0, "column pairs");
@@ -634,17 +648,19 @@ __global__ void __launch_bounds__(256)
   };
   dev::RegStage<L> twr;
   twr.load(twL, tid);
-  // phase factors of both picks (shift derivative along y: pick j0 first)
+  // phase factors of both picks from the tables of k_ref_phases (shift derivative along y: pick
+  // j0 first)
   for (int q = 0; q < nq; ++q) {
-    const RefPeak P = pk[j0 + q];
-    const int px = P.pixel % L, py = P.pixel / L;
+    const float2* P = ph + (long long)(j0 + q) * (2 * L + 2 * HX);
     for (int k = tid; k < L; k += 256) {
-      float2 a, d;
-      phase(k, L, py, P.dy, a, d);
-      phy[q][k] = a;
-      if (q == 0) dphy[k] = d;
+      phy[q][k] = P[k];
+      if (q == 0) dphy[k] = P[L + k];
     }
-    for (int sl = tid; sl <= NU; sl += 256) phase(sl < NU ? g * NU + sl : L / 2, L, px, P.dx, phx[q][sl], dphx[q][sl]);
+    for (int sl = tid; sl <= NU; sl += 256) {
+      const int kx = sl < NU ? g * NU + sl : L / 2;
+      phx[q][sl] = P[2 * L + kx];
+      dphx[q][sl] = P[2 * L + HX + kx];
+    }
   }
   // Each pass works from its own copy of the thread index, opaque to the compiler: otherwise the
   // per-thread LDS and global addresses of the transform and of the element loops, identical in
@@ -740,12 +756,8 @@ __global__ void __launch_bounds__(256)
     const int j = j0 + q;
     __syncthreads();  // the previous pass has read buf (and dphy)
     if (q == 1) {
-      const RefPeak P = pk[j];
-      const int py = P.pixel / L;
-      for (int k = tid; k < L; k += 256) {
-        float2 a;
-        phase(k, L, py, P.dy, a, dphy[k]);
-      }
+      const float2* P = ph + (long long)j * (2 * L + 2 * HX);
+      for (int k = tid; k < L; k += 256) dphy[k] = P[L + k];
     }
     const int ta = pass_tid();
     load_plane(AB + (long long)j * L2, ta);
@@ -960,15 +972,17 @@ int ref_cols_dot_parts(int l) {
 }
 
 bool launch_ref_cols_dot(const float2* ihat, const float2* AB, const RefPeak* pk, const double* stats, int K, int l,
-                         int ylo, int yhi, float scale, double* out, hipStream_t s) {
+                         int ylo, int yhi, float scale, float2* ph, double* out, hipStream_t s) {
   const float2* tw = plane_twiddles(l);
   const int G = ref_cols_dot_parts(l);
   if (!tw || !G || K <= 0) return false;
+  k_ref_phases<<<dim3(nblk(l + l / 2 + 1, BS), K), BS, 0, s>>>(pk, l, ph);
+  GS_CHECK_LAUNCH();
   const dim3 grid(G, (K + 1) / 2);
   if (l == 320)
-    k_ref_cols_dot<320><<<grid, 256, 0, s>>>(ihat, AB, pk, stats, tw, K, ylo, yhi, scale, out);
+    k_ref_cols_dot<320><<<grid, 256, 0, s>>>(ihat, AB, pk, stats, ph, tw, K, ylo, yhi, scale, out);
   else
-    k_ref_cols_dot<720><<<grid, 256, 0, s>>>(ihat, AB, pk, stats, tw, K, ylo, yhi, scale, out);
+    k_ref_cols_dot<720><<<grid, 256, 0, s>>>(ihat, AB, pk, stats, ph, tw, K, ylo, yhi, scale, out);
   GS_CHECK_LAUNCH();
   return true;
 }
