@@ -31,6 +31,7 @@
 // column pass is fused with the dot (k_ref_cols_dot): the spectra of A and B are never stored.
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 
 #include "common.hpp"
 #include "device_common.hpp"
@@ -267,9 +268,14 @@ __global__ void __launch_bounds__(256)
                       double* __restrict__ partial6, int K, int ylo, int yhi) {
   constexpr int T = RefRowCfg<L>::T, RPW = 64 / T, RPB = 4 * RPW, NG = L / RPB;
   static_assert(L % RPB == 0, "row groups");
+  // l = 320: the row transform as 4*4*20 with twiddles formed per lane by recurrence from two
+  // base twiddles in registers, and the box sums reduced through buf after the rows are written,
+  // so that the workgroup's LDS is the 40 KB of buf alone (four workgroups per CU)
+  constexpr bool RT = L == 320;
+  using RowPlan = std::conditional_t<RT, ct::RadixList<4, 4, 20>, typename RefRowCfg<L>::Plan>;
   __shared__ float2 buf[RPB * L];
-  __shared__ float2 tw[L];
-  __shared__ double red[4][4];
+  __shared__ float2 tw[RT ? 1 : L];
+  __shared__ double red[RT ? 1 : 4][4];
   const int tid = threadIdx.x, g = blockIdx.x, p = blockIdx.y;
   const int y0 = g * RPB;
   const long long L2 = (long long)L * L;
@@ -296,12 +302,16 @@ __global__ void __launch_bounds__(256)
     sn[q] = (float)sin(P.theta * kRefPi / 180);
     Tp[q] = H + (long long)j * L2;
   }
-  dev::RegStage<L> twr;
-  twr.load(twL, tid);
+  dev::RegStage<RT ? 1 : L> twr;
+  float2 wl[ct::lane_base_count(RowPlan{})];
+  if constexpr (RT)
+    ct::load_lane_bases<L, T>(wl, twL, (tid % 64) % T, RowPlan{});
+  else
+    twr.load(twL, tid);
   double a[4] = {0, 0, 0, 0};
   // pixels e = tid + 256 k in batches of BT, all gathers of a batch in flight together; the mask
   // factors of the next batch are loaded while the current one is processed
-  constexpr int N = RPB * L, NE = (N + 255) / 256, BT = 4;
+  constexpr int N = RPB * L, NE = (N + 255) / 256, BT = RT ? 3 : 4;
   const float* Mrows = mask + (long long)y0 * L;
   float mnext[BT];
 #pragma unroll
@@ -390,25 +400,38 @@ __global__ void __launch_bounds__(256)
   }
   const int wave = tid / 64, lane = tid % 64;
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    double v = a[k];
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-    if (lane == 0) red[wave][k] = v;
+  for (int k = 0; k < 4; ++k)
+    for (int o = 32; o > 0; o >>= 1) a[k] += __shfl_down(a[k], o, 64);
+  // the box sums of the workgroup (wave partials in lane 0 of each wave)
+  auto store_sums = [&](double (*rd)[4]) {
+    if (lane == 0)
+#pragma unroll
+      for (int k = 0; k < 4; ++k) rd[wave][k] = a[k];
+    __syncthreads();
+    if (tid < 4) {
+      const double v = rd[0][tid] + rd[1][tid] + rd[2][tid] + rd[3][tid];
+      if (aplane)
+        partial6[((long long)j0 * NG + g) * 6 + tid] = v;
+      else if (tid < 2 || two)
+        partial6[((long long)(j0 + tid / 2) * NG + g) * 6 + 4 + tid % 2] = v;
+    }
+  };
+  if constexpr (RT) {
+    __syncthreads();
+    ct::run_stockham<L, T, -1, ct::Sync::WAVE, ct::IdentityMap, ct::TwSrc::RECUR>(
+        buf + (wave * RPW + lane / T) * L, wl, lane % T, RowPlan{});
+  } else {
+    twr.store(tw, tid);
+    store_sums(red);
+    ct::run_stockham<L, T, -1, ct::Sync::WAVE>(buf + (wave * RPW + lane / T) * L, tw, lane % T, RowPlan{});
   }
-  twr.store(tw, tid);
-  __syncthreads();
-  if (tid < 4) {
-    const double v = red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid];
-    if (aplane)
-      partial6[((long long)j0 * NG + g) * 6 + tid] = v;
-    else if (tid < 2 || two)
-      partial6[((long long)(j0 + tid / 2) * NG + g) * 6 + 4 + tid % 2] = v;
-  }
-  ct::run_stockham<L, T, -1, ct::Sync::WAVE>(buf + (wave * RPW + lane / T) * L, tw, lane % T,
-                                             typename RefRowCfg<L>::Plan{});
   __syncthreads();
   float2* dst = AB + (long long)p * L2 + (long long)y0 * L;
   for (int e = tid; e < RPB * L; e += 256) dst[e] = buf[e];
+  if constexpr (RT) {
+    __syncthreads();  // buf's rows are written: its first bytes hold the reduction
+    store_sums(reinterpret_cast<double(*)[4]>(buf));
+  }
 }
 
 // The soft mask of the refinement's rotation (apply_mask, k:225-241, with the search's float
