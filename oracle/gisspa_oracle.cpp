@@ -263,20 +263,16 @@ void prep_templates(const orc_config* c, const P& p, const float* tmpl, int n, i
   }
 }
 
-// Whole-frame preprocessing: src/norm.cu:356-402 (dc_zero=1) / src/nonorm.cu:346-389.
-void prep_image(const P& p, const float* img, int nx, int ny, bool dc_zero, double* out) {
-  const size_t N = (size_t)nx * ny;
-  std::vector<cd> F(N);
-  for (size_t i = 0; i < N; ++i) F[i] = cd(img[i], 0);
-  orc::fft2(F.data(), ny, nx, -1);
-  scale_all(F.data(), N, 1.0 / (double)N);
+// do_phase_flip (k:828-839) + SQRSum_by_circle mode 1 (k:94-126) + whiten_filter_weight_Img
+// (k:151-202) on the scaled whole-frame spectrum F (ny x nx, row-major), in place.
+void filter_image_spectrum(const P& p, cd* F, int nx, int ny) {
   const int l = std::max(nx, ny);
   // do_phase_flip (k:828-839)
 #pragma omp parallel for schedule(static)
   for (int y = 0; y < ny; ++y)
     for (int x = 0; x < nx; ++x)
       F[(size_t)y * nx + x] *= ctf_ast(x, (y + ny / 2) % ny, nx, ny, p, 0);
-  // SQRSum_by_circle mode 1 (k:94-126)
+  // SQRSum_by_circle mode 1 (k:94-126): radial power over the half plane x <= nx/2
   std::vector<double> ra(l / 2, 0.0), rb(l / 2, 0.0);
   for (int y = 0; y < ny; ++y)
     for (int x = 0; x <= nx / 2; ++x) {
@@ -313,6 +309,16 @@ void prep_image(const P& p, const float* img, int nx, int ny, bool dc_zero, doub
       }
       F[(size_t)y * nx + x] *= f;
     }
+}
+
+// Whole-frame preprocessing: src/norm.cu:356-402 (dc_zero=1) / src/nonorm.cu:346-389.
+void prep_image(const P& p, const float* img, int nx, int ny, bool dc_zero, double* out) {
+  const size_t N = (size_t)nx * ny;
+  std::vector<cd> F(N);
+  for (size_t i = 0; i < N; ++i) F[i] = cd(img[i], 0);
+  orc::fft2(F.data(), ny, nx, -1);
+  scale_all(F.data(), N, 1.0 / (double)N);
+  filter_image_spectrum(p, F.data(), nx, ny);
   if (dc_zero) F[0] = 0;  // set_0Hz_to_0_at_RI (k:204-223)
   orc::fft2(F.data(), ny, nx, +1);
   for (size_t i = 0; i < N; ++i) out[i] = F[i].real();
@@ -390,6 +396,44 @@ void correlate(const cd* I, const cd* T, cd* C, int l) {
   orc::fft2(C, l, l, +1);
 }
 
+// get_peak_and_SUM (k:725-763) over one correlation plane plus the host merge of its chunk
+// results (src/nonorm.cu:497-509): the peak is the largest value off the border of width d_m/4,
+// which the reference sets to 0 before the max (so the peak is at least 0; first maximum in
+// raster order here), and the sums run over the whole plane, border included.
+PlaneStats nonorm_plane(const cd* C, int l, float dm4) {
+  PlaneStats o{0.0, 0.0, 0.0, 0};
+  for (int y = 0; y < l; ++y)
+    for (int x = 0; x < l; ++x) {
+      const double v = C[(size_t)y * l + x].real();
+      o.sum += v;
+      o.sum2 += v * v;
+      const bool border = x < dm4 || x > l - dm4 || y < dm4 || y > l - dm4;
+      if (!border && v > o.peak) {  // strictly greater: first maximum wins
+        o.peak = v;
+        o.pos = (long long)y * l + x;
+      }
+    }
+  return o;
+}
+
+// Score of src/nonorm.cu:511-519: the peak over the standard deviation of the other l^2 - 1
+// values (ra, rb: their sum and sum of squares; rc: their count); 0 when that deviation is 0.
+double nonorm_score(const PlaneStats& s, int l) {
+  const double ra = s.sum - s.peak, rb = s.sum2 - s.peak * s.peak;
+  const double rc = (double)l * l - 1;
+  const double sd = std::sqrt(rb / rc - (ra / rc) * (ra / rc));
+  return sd == 0 ? 0.0 : s.peak / sd;
+}
+
+// Centre of src/nonorm.cu:521-524: the peak position of the correlation of the block rotated by
+// e, rotated back about (l/2, l/2), plus the block's offset in the tile.
+void nonorm_center(long long pos, int bi, int bj, int l, int overlap, float e, float& cx, float& cy) {
+  const double ce = std::cos((double)e * PI / 180), se = std::sin((double)e * PI / 180);
+  const int px = (int)(pos % l), py = (int)(pos / l);
+  cx = (float)(bi * (l - overlap) + (px - l / 2) * ce + (py - l / 2) * se + l / 2);
+  cy = (float)(bj * (l - overlap) + (py - l / 2) * ce - (px - l / 2) * se + l / 2);
+}
+
 bool pick_less(const orc_pick& a, const orc_pick& b) {  // NMS order: a before b
   if (a.score != b.score) return a.score > b.score;
   if (a.y != b.y) return a.y < b.y;
@@ -420,6 +464,225 @@ void check_config(const orc_config* c) {
   if (!c) throw std::invalid_argument("null config");
   if (c->window_size <= 0 || c->window_size % 2) throw std::invalid_argument("window_size must be even and > 0");
   if (c->phistep <= 0) throw std::invalid_argument("phistep must be > 0");
+}
+
+
+// ---- norm_type=1 search (src/norm.cu:479-660) over a subset of the blocks ----
+// mu_p and sigma_p are per block pixel over all (template, angle) hypotheses (k:681-721), so a
+// block's statistics, best records and reference records depend on that block alone and any
+// subset of blocks gives exactly the values the full search gives for them.
+struct NormCtx {
+  const orc_config* c = nullptr;
+  P p{};
+  int W = 0, H = 0, n = 0, l = 0, nt = 0, na = 0;
+  std::vector<int> ox, oy, blist;
+  std::vector<cd> blocks;  // spectra of the listed blocks, in list order
+  std::vector<cd> T;       // processed templates (real space)
+  std::vector<float> angles;
+  double total_n = 0;
+};
+
+void norm_prepare(NormCtx& X, const orc_config* c, const float* image, int W, int H, const float* tmpl,
+                  int n, int w, int h, const double* ctf3, const int* blocks, int nlist, int na_lim,
+                  int nt_lim) {
+  check_config(c);
+  X.c = c;
+  X.p = make_params(c, ctf3);
+  X.W = W;
+  X.H = H;
+  X.n = n;
+  X.l = c->window_size;
+  const int l = X.l;
+  const size_t L2 = (size_t)l * l;
+  if (W < 1 || H < 1 || n < 1) throw std::invalid_argument("empty input");
+  // A7: clip + invert on a private copy (the reference mutates the caller's pixels)
+  std::vector<float> img(image, image + (size_t)W * H);
+  clip_image(img.data(), img.size(), W, H, c->invert);
+  std::vector<double> pre((size_t)W * H);
+  prep_image(X.p, img.data(), W, H, true, pre.data());
+  X.ox = block_offsets(W, l, c->overlap);
+  X.oy = block_offsets(H, l, c->overlap);
+  const int nb = (int)(X.ox.size() * X.oy.size());
+  if (blocks && nlist > 0) {
+    X.blist.assign(blocks, blocks + nlist);
+    for (int b : X.blist)
+      if (b < 0 || b >= nb) throw std::invalid_argument("block index out of range");
+  } else {
+    X.blist.resize(nb);
+    std::iota(X.blist.begin(), X.blist.end(), 0);
+  }
+  std::vector<cd> all((size_t)nb * L2);
+  split_blocks(pre.data(), W, H, l, X.ox, X.oy, all.data());
+  X.blocks.resize(X.blist.size() * L2);
+  for (size_t i = 0; i < X.blist.size(); ++i)
+    std::copy(all.begin() + (size_t)X.blist[i] * L2, all.begin() + (size_t)(X.blist[i] + 1) * L2,
+              X.blocks.begin() + i * L2);
+  X.T.resize((size_t)n * L2);
+  prep_templates(c, X.p, tmpl, n, w, h, X.T.data());
+  X.angles = angle_list(c->phistep);
+  X.na = (int)X.angles.size();
+  if (na_lim > 0) X.na = std::min(X.na, na_lim);
+  X.nt = nt_lim > 0 ? std::min(n, nt_lim) : n;
+  // set_CCG_mean (k:697-706): n = N_t * int(360/phistep) (src/norm.cu:549-550)
+  X.total_n = (double)n * (int)(360 / c->phistep);
+  if (na_lim > 0 || nt_lim > 0) X.total_n = (double)X.nt * X.na;
+}
+
+// Runs fn(a, first listed block, number of blocks, planes) for every angle and group of listed
+// blocks, with planes[(i * nt + t) * L2 + k] = c/l^2 (the real correlation of add_CCG_to_sum,
+// k:691) of template t against listed block first + i.  Groups bound the plane memory.
+template <typename Fn>
+void norm_correlations(const NormCtx& X, Fn&& fn) {
+  const int l = X.l, nt = X.nt;
+  const size_t L2 = (size_t)l * l;
+  const int nlist = (int)X.blist.size();
+  const size_t budget = (size_t)2 << 30;
+  const int group = (int)std::max<size_t>(1, std::min<size_t>(nlist, budget / (sizeof(double) * nt * L2)));
+  std::vector<cd> R((size_t)nt * L2);
+  std::vector<double> planes((size_t)group * nt * L2);
+  for (int a = 0; a < X.na; ++a) {
+#pragma omp parallel for schedule(dynamic)
+    for (int t = 0; t < nt; ++t) rotate_template(X.T.data() + t * L2, R.data() + t * L2, l, X.angles[a], X.p);
+    for (int g0 = 0; g0 < nlist; g0 += group) {
+      const int ng = std::min(group, nlist - g0);
+#pragma omp parallel
+      {
+        std::vector<cd> C(L2);
+#pragma omp for schedule(dynamic)
+        for (int q = 0; q < ng * nt; ++q) {
+          const int i = q / nt, t = q % nt;
+          correlate(X.blocks.data() + (size_t)(g0 + i) * L2, R.data() + t * L2, C.data(), l);
+          double* o = planes.data() + (size_t)q * L2;
+          for (size_t k = 0; k < L2; ++k) o[k] = C[k].real() / (double)L2;
+        }
+      }
+      fn(a, g0, ng, planes.data());
+    }
+  }
+}
+
+// Pass 1 (ComputeCCGSum + ComputeCCGMean, src/norm.cu:523-552): per listed block pixel the sum,
+// sum of squares, the best and second-best correlation and their hypotheses (angle * n +
+// template, first strict maximum in (angle, template) order).
+struct NormStats {
+  std::vector<double> S, SS, best, second;
+  std::vector<int32_t> arg, arg2;
+};
+
+NormStats norm_pass1(const NormCtx& X) {
+  const size_t L2 = (size_t)X.l * X.l, nl = X.blist.size();
+  NormStats st;
+  st.S.assign(nl * L2, 0.0);
+  st.SS.assign(nl * L2, 0.0);
+  st.best.assign(nl * L2, -INFINITY);
+  st.second.assign(nl * L2, -INFINITY);
+  st.arg.assign(nl * L2, -1);
+  st.arg2.assign(nl * L2, -1);
+  const int nt = X.nt, n = X.n;
+  norm_correlations(X, [&](int a, int g0, int ng, const double* planes) {
+#pragma omp parallel for schedule(static)
+    for (long long e = 0; e < (long long)ng * (long long)L2; ++e) {
+      const int i = (int)(e / (long long)L2);
+      const size_t k = (size_t)(e % (long long)L2);
+      const size_t q = (size_t)(g0 + i) * L2 + k;
+      double s = st.S[q], s2 = st.SS[q], b1 = st.best[q], b2 = st.second[q];
+      int32_t a1 = st.arg[q], a2 = st.arg2[q];
+      for (int t = 0; t < nt; ++t) {
+        const double v = planes[((size_t)i * nt + t) * L2 + k];
+        s += v;
+        s2 += v * v;
+        const int32_t id = a * n + t;
+        if (v > b1) {
+          b2 = b1;
+          a2 = a1;
+          b1 = v;
+          a1 = id;
+        } else if (v > b2) {
+          b2 = v;
+          a2 = id;
+        }
+      }
+      st.S[q] = s;
+      st.SS[q] = s2;
+      st.best[q] = b1;
+      st.second[q] = b2;
+      st.arg[q] = a1;
+      st.arg2[q] = a2;
+    }
+  });
+  return st;
+}
+
+double norm_mu(const NormCtx& X, const NormStats& st, size_t q) { return st.S[q] / X.total_n; }
+double norm_sigma(const NormCtx& X, const NormStats& st, size_t q) {
+  const double m = st.S[q] / X.total_n;
+  return std::sqrt(st.SS[q] / X.total_n - m * m);
+}
+
+// Pass 2 (PickParticles, src/norm.cu:560-600, with update_CCG and get_peak_pos, k:709-793): the
+// reference's own records.  Per (angle, block, template) plane, z = (c/l^2 - mu)/sigma (0 when
+// sigma is not positive) and, per 256-element chunk of the plane in index order, the maximum of
+// get_peak_pos's tree (at stride s = 128, 64, ..., 1 element i keeps its value unless element
+// i + s is strictly greater); a record when that maximum reaches the threshold and its centre
+// (block offset + index % l, block offset + index / l) lies in [d_m, W - d_m) x [d_m, H - d_m).
+// margin: the chunk maximum minus the largest other value of the chunk.
+std::vector<orc_refrec> norm_records(const NormCtx& X, const NormStats& st) {
+  const int l = X.l, nt = X.nt;
+  const size_t L2 = (size_t)l * l;
+  if (L2 % 256) throw std::invalid_argument("reference records need l*l divisible by 256");
+  const int chunks = (int)(L2 / 256);
+  const float dm = X.p.d_m;
+  const int bx = (int)X.ox.size();
+  std::vector<orc_refrec> out;
+  norm_correlations(X, [&](int a, int g0, int ng, const double* planes) {
+    std::vector<std::vector<orc_refrec>> per((size_t)ng * nt);
+#pragma omp parallel for schedule(dynamic)
+    for (int q = 0; q < ng * nt; ++q) {
+      const int i = q / nt, t = q % nt;
+      const int b = X.blist[g0 + i];
+      const size_t sb = (size_t)(g0 + i) * L2;
+      const double* C = planes + (size_t)q * L2;
+      double z[256];
+      int pos[256];
+      for (int ch = 0; ch < chunks; ++ch) {
+        for (int j = 0; j < 256; ++j) {
+          const size_t k = (size_t)ch * 256 + j;
+          const double m = norm_mu(X, st, sb + k), sd = norm_sigma(X, st, sb + k);
+          z[j] = sd > 0 ? (C[k] - m) / sd : 0.0;
+          pos[j] = (int)k;
+        }
+        double zz[256];
+        std::copy(z, z + 256, zz);
+        for (int s = 128; s > 0; s >>= 1)
+          for (int j = 0; j < s; ++j)
+            if (zz[j + s] > zz[j]) {
+              zz[j] = zz[j + s];
+              pos[j] = pos[j + s];
+            }
+        const double zmax = zz[0];
+        if (!(zmax >= X.p.thresh)) continue;
+        const int k = pos[0];
+        const int cx = X.ox[b % bx] + k % l, cy = X.oy[b / bx] + k / l;
+        if (!(cx >= dm && cx < X.W - dm && cy >= dm && cy < X.H - dm)) continue;
+        double other = -INFINITY;
+        for (int j = 0; j < 256; ++j)
+          if (ch * 256 + j != k) other = std::max(other, z[j]);
+        orc_refrec r{};
+        r.score = (float)zmax;
+        r.x = (float)cx;
+        r.y = (float)cy;
+        r.template_id = t;
+        r.angle_index = a;
+        r.block = b;
+        r.chunk = ch;
+        r.pixel = k;
+        r.margin = zmax - other;
+        per[q].push_back(r);
+      }
+    }
+    for (auto& v : per) out.insert(out.end(), v.begin(), v.end());
+  });
+  return out;
 }
 
 }  // namespace orc_impl
@@ -534,82 +797,34 @@ int orc_search_norm(const orc_config* c, const float* image, int W, int H, const
                     double* mu_out, double* sigma_out, double* best_out, int32_t* arg_out,
                     int n_angles_limit, int n_templates_limit) {
   try {
-    check_config(c);
-    P p = make_params(c, ctf3);
-    const int l = c->window_size;
+    NormCtx X;
+    norm_prepare(X, c, image, W, H, tmpl, n, w, h, ctf3, nullptr, 0, n_angles_limit, n_templates_limit);
+    const NormStats st = norm_pass1(X);
+    const int l = X.l;
     const size_t L2 = (size_t)l * l;
-    if (W < 1 || H < 1 || n < 1) throw std::invalid_argument("empty input");
-    // A7: clip + invert on a private copy (the reference mutates the caller's pixels)
-    std::vector<float> img(image, image + (size_t)W * H);
-    clip_image(img.data(), img.size(), W, H, c->invert);
-    std::vector<double> pre((size_t)W * H);
-    prep_image(p, img.data(), W, H, true, pre.data());
-    const auto ox = block_offsets(W, l, c->overlap), oy = block_offsets(H, l, c->overlap);
-    const int nb = (int)(ox.size() * oy.size());
-    std::vector<cd> blocks((size_t)nb * L2);
-    split_blocks(pre.data(), W, H, l, ox, oy, blocks.data());
-    std::vector<cd> T((size_t)n * L2);
-    prep_templates(c, p, tmpl, n, w, h, T.data());
-
-    auto angles = angle_list(c->phistep);
-    int na = (int)angles.size();
-    if (n_angles_limit > 0) na = std::min(na, n_angles_limit);
-    const int nt = n_templates_limit > 0 ? std::min(n, n_templates_limit) : n;
-    // running per-(block,pixel) moments and best correlation
-    std::vector<double> S((size_t)nb * L2, 0.0), SS((size_t)nb * L2, 0.0),
-        best((size_t)nb * L2, -INFINITY);
-    std::vector<int32_t> arg((size_t)nb * L2, -1);
-    std::vector<cd> R((size_t)nt * L2);
-    for (int a = 0; a < na; ++a) {
-#pragma omp parallel for schedule(dynamic)
-      for (int t = 0; t < nt; ++t) rotate_template(T.data() + t * L2, R.data() + t * L2, l, angles[a], p);
-#pragma omp parallel
-      {
-        std::vector<cd> C(L2);
-#pragma omp for schedule(dynamic)
-        for (int b = 0; b < nb; ++b) {
-          double* Sb = S.data() + b * L2;
-          double* SSb = SS.data() + b * L2;
-          double* Bb = best.data() + b * L2;
-          int32_t* Ab = arg.data() + b * L2;
-          for (int t = 0; t < nt; ++t) {
-            correlate(blocks.data() + b * L2, R.data() + t * L2, C.data(), l);
-            const int32_t id = a * n + t;
-            for (size_t k = 0; k < L2; ++k) {
-              const double s = C[k].real() / (double)L2;  // add_CCG_to_sum (k:691)
-              Sb[k] += s;
-              SSb[k] += s * s;
-              if (s > Bb[k]) { Bb[k] = s; Ab[k] = id; }
-            }
-          }
-        }
-      }
-    }
-    // set_CCG_mean (k:697-706): n = N_t * int(360/phistep) (src/norm.cu:549-550)
-    double total_n = (double)n * (int)(360 / c->phistep);
-    if (n_angles_limit > 0 || n_templates_limit > 0) total_n = (double)nt * na;
+    const int nb = (int)X.blist.size();
     std::vector<orc_pick> cand;
-    const float dm = p.d_m;
+    const float dm = X.p.d_m;
     for (int b = 0; b < nb; ++b) {
-      const int bi = b % (int)ox.size(), bj = b / (int)ox.size();
+      const int bi = b % (int)X.ox.size(), bj = b / (int)X.ox.size();
       for (size_t k = 0; k < L2; ++k) {
         const size_t q = b * L2 + k;
-        const double m = S[q] / total_n;
-        const double sd = std::sqrt(SS[q] / total_n - m * m);
-        const double z = sd > 0 ? (best[q] - m) / sd : 0.0;  // update_CCG (k:709-721)
+        const double m = norm_mu(X, st, q);
+        const double sd = norm_sigma(X, st, q);
+        const double z = sd > 0 ? (st.best[q] - m) / sd : 0.0;  // update_CCG (k:709-721)
         if (mu_out) mu_out[q] = m;
         if (sigma_out) sigma_out[q] = sd;
-        if (best_out) best_out[q] = best[q];
-        if (arg_out) arg_out[q] = arg[q];
-        if (z >= p.thresh) {  // src/norm.cu:585-590
-          const int cx = ox[bi] + (int)(k % l), cy = oy[bj] + (int)(k / l);
+        if (best_out) best_out[q] = st.best[q];
+        if (arg_out) arg_out[q] = st.arg[q];
+        if (z >= X.p.thresh) {  // src/norm.cu:585-590
+          const int cx = X.ox[bi] + (int)(k % l), cy = X.oy[bj] + (int)(k / l);
           if (cx >= dm && cx < W - dm && cy >= dm && cy < H - dm) {
             orc_pick pk{};
             pk.x = (float)cx;
             pk.y = (float)cy;
-            pk.template_id = arg[q] % n;
-            pk.angle_index = arg[q] / n;
-            float e = 360.0f - angles[pk.angle_index];  // src/norm.cu:654-655
+            pk.template_id = st.arg[q] % n;
+            pk.angle_index = st.arg[q] / n;
+            float e = 360.0f - X.angles[pk.angle_index];  // src/norm.cu:654-655
             if (e >= 360.0f) e -= 360.0f;
             pk.psi = e;
             pk.score = (float)z;
@@ -625,6 +840,41 @@ int orc_search_norm(const orc_config* c, const float* image, int W, int H, const
     const int kept = nms(cand.data(), (int)cand.size(), c->nms_radius);
     for (int i = 0; i < kept && i < max_picks; ++i) picks[i] = cand[i];
     return kept;
+  } catch (...) { return -1; }
+}
+
+int orc_search_norm_blocks(const orc_config* c, const float* image, int W, int H, const float* tmpl,
+                           int n, int w, int h, const double* ctf3, const int* blocks, int nlist,
+                           int n_angles_limit, int n_templates_limit, double* mu, double* sigma,
+                           double* best, double* second, int32_t* arg, int32_t* arg2) {
+  try {
+    if (!blocks || nlist < 1) throw std::invalid_argument("empty block list");
+    NormCtx X;
+    norm_prepare(X, c, image, W, H, tmpl, n, w, h, ctf3, blocks, nlist, n_angles_limit, n_templates_limit);
+    const NormStats st = norm_pass1(X);
+    const size_t N = X.blist.size() * (size_t)X.l * X.l;
+    for (size_t q = 0; q < N; ++q) {
+      mu[q] = norm_mu(X, st, q);
+      sigma[q] = norm_sigma(X, st, q);
+      best[q] = st.best[q];
+      second[q] = st.second[q];
+      arg[q] = st.arg[q];
+      arg2[q] = st.arg2[q];
+    }
+    return 0;
+  } catch (...) { return -1; }
+}
+
+int orc_search_norm_records(const orc_config* c, const float* image, int W, int H, const float* tmpl,
+                            int n, int w, int h, const double* ctf3, const int* blocks, int nlist,
+                            int n_angles_limit, int n_templates_limit, orc_refrec* recs, int max_recs) {
+  try {
+    NormCtx X;
+    norm_prepare(X, c, image, W, H, tmpl, n, w, h, ctf3, blocks, nlist, n_angles_limit, n_templates_limit);
+    const NormStats st = norm_pass1(X);
+    const auto r = norm_records(X, st);
+    for (int i = 0; i < (int)r.size() && i < max_recs; ++i) recs[i] = r[i];
+    return (int)r.size();
   } catch (...) { return -1; }
 }
 
@@ -690,48 +940,27 @@ int orc_search_nonorm(const orc_config* c, const float* image, int W, int H, con
         }
         std::vector<double> sc((size_t)n, 0.0), cxs(n, 0.0), cys(n, 0.0);
         std::vector<int> pk_pix(n, 0), pk_blk(n, 0);
-        const double ce = std::cos((double)e * PI / 180), se = std::sin((double)e * PI / 180);
         for (int b = 0; b < nblk * nblk; ++b) {
           const int bi = b % nblk, bj = b / nblk;
-          std::vector<double> peak(n), pos(n), sum(n), sum2(n);
+          std::vector<PlaneStats> st(n);
 #pragma omp parallel
           {
             std::vector<cd> C(L2);
 #pragma omp for schedule(dynamic)
             for (int t = 0; t < n; ++t) {
               correlate(rot.data() + b * L2, T.data() + t * L2, C.data(), l);
-              double pk = 0, s = 0, s2 = 0;
-              long long ps = 0;
-              for (int y = 0; y < l; ++y)
-                for (int x = 0; x < l; ++x) {
-                  const double v = C[(size_t)y * l + x].real();
-                  s += v;
-                  s2 += v * v;
-                  const bool border = x < dm4 || x > l - dm4 || y < dm4 || y > l - dm4;
-                  if (!border && v > pk) {  // strictly greater: first maximum wins
-                    pk = v;
-                    ps = (long long)y * l + x;
-                  }
-                }
-              peak[t] = pk;
-              pos[t] = (double)ps;
-              sum[t] = s;
-              sum2[t] = s2;
+              st[t] = nonorm_plane(C.data(), l, dm4);
             }
           }
           for (int t = 0; t < n; ++t) {
-            const double ra = sum[t] - peak[t], rb = sum2[t] - peak[t] * peak[t];
-            const double rc = (double)L2 - 1;
-            const double sd = std::sqrt(rb / rc - (ra / rc) * (ra / rc));
-            const double score = sd == 0 ? 0.0 : peak[t] / sd;
-            const int cx = (int)pos[t] % l, cy = (int)pos[t] / l;
-            const float centerx = (float)(bi * (l - c->overlap) + (cx - l / 2) * ce + (cy - l / 2) * se + l / 2);
-            const float centery = (float)(bj * (l - c->overlap) + (cy - l / 2) * ce - (cx - l / 2) * se + l / 2);
+            const double score = nonorm_score(st[t], l);
+            float centerx, centery;
+            nonorm_center(st[t].pos, bi, bj, l, c->overlap, e, centerx, centery);
             if (sc[t] < score && centerx >= 0 && centerx < TILE && centery >= 0 && centery < TILE) {
               sc[t] = score;
               cxs[t] = centerx;
               cys[t] = centery;
-              pk_pix[t] = (int)pos[t];
+              pk_pix[t] = (int)st[t].pos;
               pk_blk[t] = b;
             }
           }
@@ -759,6 +988,60 @@ int orc_search_nonorm(const orc_config* c, const float* image, int W, int H, con
     for (int i = 0; i < kept && i < max_picks; ++i) picks[i] = all[i];
     return kept;
   } catch (...) { return -1; }
+}
+
+// ---- stage pieces, exposed so that each can be pinned on its own ----
+int orc_whiten_template(double* F, int l) {
+  try {
+    if (l <= 0 || l % 2) throw std::invalid_argument("l");
+    whiten_template(reinterpret_cast<cd*>(F), l);
+    return 0;
+  } catch (...) { return -1; }
+}
+
+int orc_weight_template(const orc_config* c, const double* ctf3, double* F, int l) {
+  try {
+    check_config(c);
+    P p = make_params(c, ctf3);
+    weight_template(reinterpret_cast<cd*>(F), l, p);
+    return 0;
+  } catch (...) { return -1; }
+}
+
+int orc_filter_image_spectrum(const orc_config* c, const double* ctf3, double* F, int nx, int ny) {
+  try {
+    check_config(c);
+    P p = make_params(c, ctf3);
+    filter_image_spectrum(p, reinterpret_cast<cd*>(F), nx, ny);
+    return 0;
+  } catch (...) { return -1; }
+}
+
+int orc_rotate_plane(const double* in, double* out, int l, float theta) {
+  try {
+    rotate_plane(reinterpret_cast<const cd*>(in), reinterpret_cast<cd*>(out), l, theta);
+    return 0;
+  } catch (...) { return -1; }
+}
+
+int orc_nonorm_plane(const double* C, int l, float d_m, double* stats4) {
+  try {
+    const PlaneStats s = nonorm_plane(reinterpret_cast<const cd*>(C), l, d_m / 4);
+    stats4[0] = s.peak;
+    stats4[1] = (double)s.pos;
+    stats4[2] = s.sum;
+    stats4[3] = s.sum2;
+    return 0;
+  } catch (...) { return -1; }
+}
+
+double orc_nonorm_score(const double* stats4, int l) {
+  PlaneStats s{stats4[0], stats4[2], stats4[3], (long long)stats4[1]};
+  return nonorm_score(s, l);
+}
+
+void orc_nonorm_center(long long pos, int bi, int bj, int l, int overlap, float theta, float* cx, float* cy) {
+  nonorm_center(pos, bi, bj, l, overlap, theta, *cx, *cy);
 }
 
 int orc_nms(orc_pick* cand, int n, float radius) { return nms(cand, n, radius); }

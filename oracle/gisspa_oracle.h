@@ -83,6 +83,50 @@ int orc_search_norm(const orc_config* c, const float* image, int W, int H, const
 int orc_search_nonorm(const orc_config* c, const float* image, int W, int H, const float* tmpl,
                       int n, int w, int h, const double* ctf3, orc_pick* recs, int max_recs,
                       int* n_recs, orc_pick* picks, int max_picks);
+/* Reference records of the norm_type=1 search (the second pass of src/norm.cu:560-600): one
+ * per (angle, block, template, 256-element chunk) whose chunk maximum of z reaches the
+ * threshold with its centre inside [d_m, W - d_m) x [d_m, H - d_m).  margin: the chunk maximum
+ * minus the largest other z of the chunk (how far the chosen pixel is from a tie). */
+typedef struct {
+  float x, y;
+  int32_t template_id, angle_index, block, chunk, pixel;
+  float score;
+  double margin;
+} orc_refrec;
+
+/* norm_type=1 statistics of the listed blocks only (each block's mu/sigma/best depend on that
+ * block alone): per listed block pixel ([list index][pixel]) mu, sigma, the best and second-best
+ * correlation c/l^2 and their hypotheses (angle_index * n + template).  Limits as in
+ * orc_search_norm. */
+int orc_search_norm_blocks(const orc_config* c, const float* image, int W, int H, const float* tmpl,
+                           int n, int w, int h, const double* ctf3, const int* blocks, int nlist,
+                           int n_angles_limit, int n_templates_limit, double* mu, double* sigma,
+                           double* best, double* second, int32_t* arg, int32_t* arg2);
+/* Reference records over the listed blocks (blocks NULL or nlist 0: all), in the reference's
+ * order: angle, block, template, chunk.  Returns the number of records (l*l must be divisible
+ * by 256). */
+int orc_search_norm_records(const orc_config* c, const float* image, int W, int H, const float* tmpl,
+                            int n, int w, int h, const double* ctf3, const int* blocks, int nlist,
+                            int n_angles_limit, int n_templates_limit, orc_refrec* recs, int max_recs);
+
+/* ---- stage pieces (pinned one by one in the tests) ---- */
+/* SQRSum_by_circle + whiten_Tmp (k:94-149) on one l x l complex spectrum, in place. */
+int orc_whiten_template(double* F, int l);
+/* apply_weighting_function (k:243-288) on one l x l complex spectrum, in place. */
+int orc_weight_template(const orc_config* c, const double* ctf3, double* F, int l);
+/* do_phase_flip + SQRSum_by_circle (mode 1) + whiten_filter_weight_Img (k:94-202, 828-839) on a
+ * scaled whole-frame spectrum (ny x nx complex), in place. */
+int orc_filter_image_spectrum(const orc_config* c, const double* ctf3, double* F, int nx, int ny);
+/* rotate_subIMG (k:534-603) of one l x l complex plane. */
+int orc_rotate_plane(const double* in, double* out, int l, float theta);
+/* norm_type=0 plane reduction (get_peak_and_SUM, k:725-763): stats4 = peak, peak index, sum,
+ * sum of squares over an l x l complex plane's real parts (border d_m/4). */
+int orc_nonorm_plane(const double* C, int l, float d_m, double* stats4);
+/* norm_type=0 score peak / sd of the other l*l - 1 values (src/nonorm.cu:511-519). */
+double orc_nonorm_score(const double* stats4, int l);
+/* norm_type=0 centre of a peak index in block (bi, bj) at angle theta (src/nonorm.cu:521-524). */
+void orc_nonorm_center(long long pos, int bi, int bj, int l, int overlap, float theta, float* cx, float* cy);
+
 /* Greedy NMS: sort by score (desc; ties by y, x, template, angle), keep a candidate if no kept
  * pick lies strictly closer than radius.  In place; returns the number kept. */
 int orc_nms(orc_pick* cand, int n, float radius);

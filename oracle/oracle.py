@@ -12,7 +12,7 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "lib", "libgisspa_oracle.so")
+LIB_PATH = os.environ.get("GISSPA_ORACLE_LIB") or os.path.join(HERE, "lib", "libgisspa_oracle.so")
 
 
 class Config(C.Structure):
@@ -62,6 +62,18 @@ def load():
                                       C.c_int, _P, _P, C.c_int, _P, _P, _P, _P, C.c_int, C.c_int]),
         "orc_search_nonorm": (C.c_int, [C.POINTER(Config), _P, C.c_int, C.c_int, _P, C.c_int, C.c_int,
                                         C.c_int, _P, _P, C.c_int, _P, _P, C.c_int]),
+        "orc_search_norm_blocks": (C.c_int, [C.POINTER(Config), _P, C.c_int, C.c_int, _P, C.c_int, C.c_int,
+                                             C.c_int, _P, _P, C.c_int, C.c_int, C.c_int, _P, _P, _P, _P,
+                                             _P, _P]),
+        "orc_search_norm_records": (C.c_int, [C.POINTER(Config), _P, C.c_int, C.c_int, _P, C.c_int, C.c_int,
+                                              C.c_int, _P, _P, C.c_int, C.c_int, C.c_int, _P, C.c_int]),
+        "orc_whiten_template": (C.c_int, [_P, C.c_int]),
+        "orc_weight_template": (C.c_int, [C.POINTER(Config), _P, _P, C.c_int]),
+        "orc_filter_image_spectrum": (C.c_int, [C.POINTER(Config), _P, _P, C.c_int, C.c_int]),
+        "orc_rotate_plane": (C.c_int, [_P, _P, C.c_int, C.c_float]),
+        "orc_nonorm_plane": (C.c_int, [_P, C.c_int, C.c_float, _P]),
+        "orc_nonorm_score": (C.c_double, [_P, C.c_int]),
+        "orc_nonorm_center": (None, [C.c_longlong, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, _P, _P]),
         "orc_nms": (C.c_int, [_P, C.c_int, C.c_float]),
         "orc_correlate": (C.c_int, [_P, _P, C.c_int, _P]),
         "orc_num_threads": (C.c_int, []),
@@ -185,6 +197,95 @@ def search_norm(cfg, image, tmpl, ctf3, want_stats=False, n_angles=0, n_template
                                    t.shape[0], t.shape[2], t.shape[1], d.ctypes.data, picks.ctypes.data,
                                    max_picks, *ptrs, n_angles, n_templates))
     return picks[:min(n, max_picks)].copy(), stats
+
+
+REFREC_DTYPE = np.dtype([
+    ("x", "<f4"), ("y", "<f4"), ("template_id", "<i4"), ("angle_index", "<i4"), ("block", "<i4"),
+    ("chunk", "<i4"), ("pixel", "<i4"), ("score", "<f4"), ("margin", "<f8"),
+])
+
+
+def search_norm_blocks(cfg, image, tmpl, ctf3, blocks, n_angles=0, n_templates=0):
+    """norm_type=1 statistics of the listed blocks: dict of mu, sigma, best, second (float64)
+    and arg, arg2 (int32), each [len(blocks)][l*l]."""
+    img = np.ascontiguousarray(image, dtype=np.float32)
+    t = np.ascontiguousarray(tmpl, dtype=np.float32)
+    c = config(cfg)
+    d = np.ascontiguousarray(ctf3, dtype=np.float64)
+    bl = np.ascontiguousarray(blocks, dtype=np.int32)
+    L2 = cfg.window_size ** 2
+    out = {k: np.zeros((bl.size, L2)) for k in ("mu", "sigma", "best", "second")}
+    out.update({k: np.zeros((bl.size, L2), dtype=np.int32) for k in ("arg", "arg2")})
+    _ok(load().orc_search_norm_blocks(C.byref(c), img.ctypes.data, img.shape[1], img.shape[0], t.ctypes.data,
+                                      t.shape[0], t.shape[2], t.shape[1], d.ctypes.data, bl.ctypes.data, bl.size,
+                                      n_angles, n_templates,
+                                      *[out[k].ctypes.data for k in ("mu", "sigma", "best", "second", "arg",
+                                                                     "arg2")]))
+    return out
+
+
+def search_norm_records(cfg, image, tmpl, ctf3, blocks=None, n_angles=0, n_templates=0, max_out=1 << 21):
+    """The reference's own norm_type=1 records (REFREC_DTYPE), in its order."""
+    img = np.ascontiguousarray(image, dtype=np.float32)
+    t = np.ascontiguousarray(tmpl, dtype=np.float32)
+    c = config(cfg)
+    d = np.ascontiguousarray(ctf3, dtype=np.float64)
+    bl = np.ascontiguousarray([] if blocks is None else blocks, dtype=np.int32)
+    recs = np.zeros(max_out, dtype=REFREC_DTYPE)
+    n = _ok(load().orc_search_norm_records(C.byref(c), img.ctypes.data, img.shape[1], img.shape[0],
+                                           t.ctypes.data, t.shape[0], t.shape[2], t.shape[1], d.ctypes.data,
+                                           bl.ctypes.data if bl.size else None, bl.size, n_angles, n_templates,
+                                           recs.ctypes.data, max_out))
+    if n > max_out:
+        raise RuntimeError(f"{n} records exceed max_out={max_out}")
+    return recs[:n].copy()
+
+
+def whiten_template(F):
+    a = np.ascontiguousarray(F, dtype=np.complex128).copy()
+    _ok(load().orc_whiten_template(a.ctypes.data, a.shape[0]))
+    return a
+
+
+def weight_template(cfg, F, ctf3):
+    a = np.ascontiguousarray(F, dtype=np.complex128).copy()
+    c = config(cfg)
+    d = np.ascontiguousarray(ctf3, dtype=np.float64)
+    _ok(load().orc_weight_template(C.byref(c), d.ctypes.data, a.ctypes.data, a.shape[0]))
+    return a
+
+
+def filter_image_spectrum(cfg, F, ctf3):
+    a = np.ascontiguousarray(F, dtype=np.complex128).copy()
+    c = config(cfg)
+    d = np.ascontiguousarray(ctf3, dtype=np.float64)
+    _ok(load().orc_filter_image_spectrum(C.byref(c), d.ctypes.data, a.ctypes.data, a.shape[1], a.shape[0]))
+    return a
+
+
+def rotate_plane(P, theta):
+    a = np.ascontiguousarray(P, dtype=np.complex128)
+    out = np.zeros_like(a)
+    _ok(load().orc_rotate_plane(a.ctypes.data, out.ctypes.data, a.shape[0], theta))
+    return out
+
+
+def nonorm_plane(Cplane, d_m):
+    a = np.ascontiguousarray(Cplane, dtype=np.complex128)
+    st = np.zeros(4)
+    _ok(load().orc_nonorm_plane(a.ctypes.data, a.shape[0], d_m, st.ctypes.data))
+    return st
+
+
+def nonorm_score(stats4, l):
+    st = np.ascontiguousarray(stats4, dtype=np.float64)
+    return load().orc_nonorm_score(st.ctypes.data, l)
+
+
+def nonorm_center(pos, bi, bj, l, overlap, theta):
+    cx, cy = C.c_float(), C.c_float()
+    load().orc_nonorm_center(int(pos), bi, bj, l, overlap, theta, C.byref(cx), C.byref(cy))
+    return cx.value, cy.value
 
 
 def search_nonorm(cfg, image, tmpl, ctf3, max_out=1 << 18):

@@ -102,12 +102,24 @@ void apply_mask(cd* t, int l, double d_m_cfg, double ehw);
 void scale_all(cd* x, size_t n, double s);
 std::vector<int> block_offsets(int length, int l, int overlap);
 std::vector<float> angle_list(float phistep);
+void filter_image_spectrum(const P& p, cd* F, int nx, int ny);
 void prep_image(const P& p, const float* img, int nx, int ny, bool dc_zero, double* out);
+double bandpass_image(int r, const Band& b);
+void rotate_plane(const cd* in, cd* out, int l, float theta);
 void clip_image(float* img, size_t N, int nx, int ny, int invert);
 void rotate_template(const cd* Tp, cd* out, int l, float theta, const P& p);
 void split_blocks(const double* img, int nx, int ny, int l, const std::vector<int>& ox,
                   const std::vector<int>& oy, cd* out);
 void correlate(const cd* I, const cd* T, cd* C, int l);
+
+// Per-plane reduction of the norm_type=0 scoring.
+struct PlaneStats {
+  double peak, sum, sum2;
+  long long pos;
+};
+PlaneStats nonorm_plane(const cd* C, int l, float dm4);
+double nonorm_score(const PlaneStats& s, int l);
+void nonorm_center(long long pos, int bi, int bj, int l, int overlap, float e, float& cx, float& cy);
 void check_config(const orc_config* c);
 
 }  // namespace orc_impl
