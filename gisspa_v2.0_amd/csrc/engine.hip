@@ -826,8 +826,8 @@ class Engine {
     launch_phase_flip(F, nx, ny, dp_, stream_);
     const int nbins = std::max(nx, ny) / 2;
     double* ra = d_ra_img_.ensure(nbins);
-    GS_HIP(hipMemsetAsync(ra, 0, sizeof(double) * nbins, stream_));
-    launch_radial_power_image(F, nx, ny, ra, stream_);
+    double* part = d_chpart_.ensure((size_t)radial_image_groups() * nbins);
+    launch_radial_power_image(F, nx, ny, part, ra, stream_);
     launch_whiten_filter_image(F, nx, ny, ra, radial_counts_dev(nx, ny), dp_, stream_);
     if (dc_zero) launch_zero_dc(F, stream_);
     fft2(+1, 1.f);

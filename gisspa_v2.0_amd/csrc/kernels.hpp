@@ -52,7 +52,9 @@ void launch_extract_real(const float2* F, float* out, long long count, hipStream
 // ---- whole-frame preparation (src/norm.cu:356-402, src/nonorm.cu:346-389) ----
 void launch_real_to_complex(const float* in, float2* out, long long count, float sign, hipStream_t s);
 void launch_phase_flip(float2* F, int nx, int ny, const DevParams& p, hipStream_t s);
-void launch_radial_power_image(const float2* F, int nx, int ny, double* ra, hipStream_t s);
+// part: radial_image_groups() * (max(nx, ny) / 2) doubles of scratch.
+int radial_image_groups();
+void launch_radial_power_image(const float2* F, int nx, int ny, double* part, double* ra, hipStream_t s);
 void launch_whiten_filter_image(float2* F, int nx, int ny, const double* ra, const double* rb,
                                 const DevParams& p, hipStream_t s);
 void launch_zero_dc(float2* F, hipStream_t s);

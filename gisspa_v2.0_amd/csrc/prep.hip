@@ -1,8 +1,8 @@
 // Preprocessing kernels: templates, whole frame, blocks and per-angle rotation.
 //
 // These stages run once per micrograph (or once per angle on the template side) and are small
-// next to the correlation.  Reductions are deterministic (fixed-order double accumulation)
-// except the radial power spectra, which use double-precision LDS atomics.
+// next to the correlation.  Every reduction is deterministic: double accumulation in an order
+// fixed by the data layout and the launch shape, never by the scheduling of the hardware.
 #include <cmath>
 
 #include "common.hpp"
@@ -45,6 +45,8 @@ __global__ void k_pad_templates(const float* __restrict__ raw, int n, int w, int
 // Per-plane reductions are split over kChunks workgroups per plane (slices of the plane), whose
 // partials a finishing kernel adds in chunk order.
 constexpr int kChunks = 16;
+// Workgroups (row slices) of the whole-frame radial power.
+constexpr int kImageGroups = 256;
 
 // Ring sum/count (generate_mask + multiCount_dot, k:28-80) and box sums (UpdateSigma, k:4-26);
 // grid (kChunks, n), partials [t][chunk][4].
@@ -108,35 +110,73 @@ __global__ void k_template_standardise(float2* __restrict__ T, int n, int l, con
   T[i].x = (float)(((double)T[i].x - em) / sigma);
 }
 
-// Radial power over the half plane x <= l/2 (SQRSum_by_circle, k:94-126); grid (kChunks, n),
-// LDS histogram of one slice per workgroup, partials [t][chunk][bin].
+// Radial power of one row of a half plane (x = 0 .. hx - 1, folded row index fy) added to a
+// wave-private histogram.  Along such a row the bin floor(hypot(x, fy) + 1/2) - 1 is
+// non-decreasing in x, so the pixels of one bin among 64 consecutive x are one run of lanes: a
+// segmented inclusive scan over the wave (keyed by bin) leaves each run's sum in its last lane,
+// which adds it to the histogram.  Runs have distinct bins, so no two lanes touch one entry, and
+// every addition happens in an order fixed by the data layout (the reference's float atomics,
+// k:123-124, sum in whatever order the hardware schedules them).
+__device__ __forceinline__ void radial_row(const float2* __restrict__ row, int hx, int fy, int nbins,
+                                           double* __restrict__ hist, int lane) {
+  for (int x0 = 0; x0 < hx; x0 += 64) {
+    const int x = x0 + lane;
+    int key = -1 - lane;  // a lane outside the row or the bins: a run of its own, never added
+    double v = 0.0;
+    if (x < hx) {
+      const int r = rbin(x, fy);
+      if (r >= 0 && r < nbins) {
+        const float2 f = row[x];
+        v = (double)f.x * f.x + (double)f.y * f.y;
+        key = r;
+      }
+    }
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const double pv = __shfl_up(v, d, 64);
+      const int pk = __shfl_up(key, d, 64);
+      if (lane >= d && pk == key) v += pv;
+    }
+    const int nk = __shfl_down(key, 1, 64);
+    if (key >= 0 && (lane == 63 || nk != key)) hist[key] += v;
+  }
+}
+
+// Sums the wave-private histograms in wave order into out[0 .. nbins).
+__device__ __forceinline__ void radial_merge(const double* __restrict__ hist, int waves, int nbins,
+                                             double* __restrict__ out) {
+  for (int i = threadIdx.x; i < nbins; i += blockDim.x) {
+    double s = 0.0;
+    for (int w = 0; w < waves; ++w) s += hist[w * nbins + i];
+    out[i] = s;
+  }
+}
+
+// Radial power over the half plane x <= l/2 of each template spectrum (SQRSum_by_circle,
+// k:94-126); grid (kChunks, n), each workgroup a slice of rows (wave w takes rows w, w + 4, ...
+// of it), partials [t][chunk][bin].
 __global__ void k_radial_power_planes(const float2* __restrict__ F, int l, double* __restrict__ part) {
   extern __shared__ double hist[];
   const int t = blockIdx.y, ch = blockIdx.x, nbins = l / 2;
-  for (int i = threadIdx.x; i < nbins; i += blockDim.x) hist[i] = 0.0;
+  const int waves = blockDim.x / 64, wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  for (int i = threadIdx.x; i < waves * nbins; i += blockDim.x) hist[i] = 0.0;
   __syncthreads();
   const float2* P = F + (long long)t * l * l;
-  const int hx = l / 2 + 1;
-  const int per = (l * hx + kChunks - 1) / kChunks;
-  const int p0 = ch * per, p1 = min(l * hx, p0 + per);
-  for (int p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
-    const int y = p / hx, x = p % hx;
-    const int r = rbin(min(x, l - x), min(y, l - y));
-    if (r >= 0 && r < nbins) {
-      const float2 v = P[(long long)y * l + x];
-      atomicAdd(&hist[r], (double)v.x * v.x + (double)v.y * v.y);
-    }
-  }
+  const int per = (l + kChunks - 1) / kChunks;
+  const int y0 = ch * per, y1 = min(l, y0 + per);
+  for (int y = y0 + wave; y < y1; y += waves)
+    radial_row(P + (long long)y * l, l / 2 + 1, min(y, l - y), nbins, hist + wave * nbins, lane);
   __syncthreads();
-  for (int i = threadIdx.x; i < nbins; i += blockDim.x) part[((long long)t * kChunks + ch) * nbins + i] = hist[i];
+  radial_merge(hist, waves, nbins, part + ((long long)t * kChunks + ch) * nbins);
 }
 
-__global__ void k_radial_power_finish(const double* __restrict__ part, int n, int nbins, double* __restrict__ ra) {
+__global__ void k_radial_power_finish(const double* __restrict__ part, int n, int nbins, int chunks,
+                                      double* __restrict__ ra) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= (long long)n * nbins) return;
   const long long t = i / nbins, b = i % nbins;
   double s = 0;
-  for (int ch = 0; ch < kChunks; ++ch) s += part[(t * kChunks + ch) * nbins + b];
+  for (int ch = 0; ch < chunks; ++ch) s += part[(t * chunks + ch) * nbins + b];
   ra[i] = s;
 }
 
@@ -282,27 +322,20 @@ __global__ void k_phase_flip(float2* __restrict__ F, int nx, int ny, DevParams p
   F[i] = make_float2(F[i].x * v, F[i].y * v);
 }
 
-// SQRSum_by_circle mode 1 (k:94-126) over the whole frame: per-block LDS histogram, then global.
-__global__ void k_radial_power_image(const float2* __restrict__ F, int nx, int ny, double* __restrict__ ra) {
+// SQRSum_by_circle mode 1 (k:94-126) over the whole frame: workgroup g takes a slice of rows
+// (its waves interleaved), partials [g][bin]; k_radial_power_finish sums them in order.
+__global__ void k_radial_power_image(const float2* __restrict__ F, int nx, int ny, double* __restrict__ part) {
   extern __shared__ double hist[];
   const int nbins = max(nx, ny) / 2;
-  for (int i = threadIdx.x; i < nbins; i += blockDim.x) hist[i] = 0.0;
+  const int waves = blockDim.x / 64, wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  for (int i = threadIdx.x; i < waves * nbins; i += blockDim.x) hist[i] = 0.0;
   __syncthreads();
-  const int hx = nx / 2 + 1;
-  const long long total = (long long)ny * hx;
-  const long long per = (total + gridDim.x - 1) / gridDim.x;
-  const long long beg = (long long)blockIdx.x * per, end = min(total, beg + per);
-  for (long long p = beg + threadIdx.x; p < end; p += blockDim.x) {
-    const int y = (int)(p / hx), x = (int)(p % hx);
-    const int r = rbin(min(x, nx - x), min(y, ny - y));
-    if (r < nbins && r >= 0) {
-      const float2 v = F[(long long)y * nx + x];
-      atomicAdd(&hist[r], (double)v.x * v.x + (double)v.y * v.y);
-    }
-  }
+  const int per = (ny + gridDim.x - 1) / gridDim.x;
+  const int y0 = blockIdx.x * per, y1 = min(ny, y0 + per);
+  for (int y = y0 + wave; y < y1; y += waves)
+    radial_row(F + (long long)y * nx, nx / 2 + 1, min(y, ny - y), nbins, hist + wave * nbins, lane);
   __syncthreads();
-  for (int i = threadIdx.x; i < nbins; i += blockDim.x)
-    if (hist[i] != 0.0) atomicAdd(&ra[i], hist[i]);
+  radial_merge(hist, waves, nbins, part + (long long)blockIdx.x * nbins);
 }
 
 // whiten_filter_weight_Img (k:151-202)
@@ -566,9 +599,11 @@ void launch_template_standardise(float2* T, int n, int l, const double* mom, hip
   GS_CHECK_LAUNCH();
 }
 void launch_radial_power_planes(const float2* F, int n, int l, double* part, double* ra, hipStream_t s) {
-  k_radial_power_planes<<<dim3(kChunks, n), BS, sizeof(double) * (l / 2), s>>>(F, l, part);
+  const int nbins = l / 2;
+  GS_REQUIRE(4 * nbins * sizeof(double) <= 64 * 1024, kUnsupported, "window too large for radial histogram");
+  k_radial_power_planes<<<dim3(kChunks, n), BS, 4 * sizeof(double) * nbins, s>>>(F, l, part);
   GS_CHECK_LAUNCH();
-  k_radial_power_finish<<<nblk((long long)n * (l / 2)), BS, 0, s>>>(part, n, l / 2, ra);
+  k_radial_power_finish<<<nblk((long long)n * nbins), BS, 0, s>>>(part, n, nbins, kChunks, ra);
   GS_CHECK_LAUNCH();
 }
 void launch_whiten_planes(float2* F, int n, int l, const double* ra, const double* rb, hipStream_t s) {
@@ -613,12 +648,15 @@ void launch_phase_flip(float2* F, int nx, int ny, const DevParams& p, hipStream_
   k_phase_flip<<<nblk((long long)nx * ny), BS, 0, s>>>(F, nx, ny, p);
   GS_CHECK_LAUNCH();
 }
-void launch_radial_power_image(const float2* F, int nx, int ny, double* ra, hipStream_t s) {
+int radial_image_groups() { return kImageGroups; }
+void launch_radial_power_image(const float2* F, int nx, int ny, double* part, double* ra, hipStream_t s) {
   const int nbins = std::max(nx, ny) / 2;
-  GS_REQUIRE(nbins * sizeof(double) <= 48 * 1024, kUnsupported, "frame too large for radial histogram");
-  const long long work = (long long)ny * (nx / 2 + 1);
-  const unsigned grid = (unsigned)std::min<long long>(1024, (work + 4095) / 4096);
-  k_radial_power_image<<<grid, BS, sizeof(double) * nbins, s>>>(F, nx, ny, ra);
+  // wave-private histograms within 64 KB of LDS per workgroup
+  const int waves = (int)std::min<size_t>(4, (64 * 1024) / (sizeof(double) * std::max(1, nbins)));
+  GS_REQUIRE(waves >= 1, kUnsupported, "frame too large for radial histogram");
+  k_radial_power_image<<<kImageGroups, 64 * waves, sizeof(double) * nbins * waves, s>>>(F, nx, ny, part);
+  GS_CHECK_LAUNCH();
+  k_radial_power_finish<<<nblk(nbins), BS, 0, s>>>(part, 1, nbins, kImageGroups, ra);
   GS_CHECK_LAUNCH();
 }
 void launch_whiten_filter_image(float2* F, int nx, int ny, const double* ra, const double* rb,
