@@ -17,6 +17,9 @@ struct FftPlan {
   int n = 0;
   std::vector<int> radices;
   std::vector<cd> wfwd;  // exp(-2 pi i e / n), e in [0, n)
+  // per Stockham stage s (radix R, Ns = product of the earlier radices): w^(r jm) at
+  // [jm (R - 1) + r - 1], w = exp(-2 pi i / (Ns R)), forward sign
+  std::vector<std::vector<cd>> stage_tw;
 };
 
 const FftPlan& fft_plan(int n);

@@ -367,6 +367,48 @@ void rotate_template(const cd* Tp, cd* out, int l, float theta, const P& p) {
   scale_all(out, L2, 1.0 / (double)L2);
 }
 
+// rotate_template for two templates with one forward transform: the rotated, masked and
+// standardised planes are real (the imaginary rounding residue of the processed templates is
+// dropped), so the transform of a + i b gives both spectra through Hermitian symmetry:
+// A(k) = (Z(k) + conj Z(-k)) / 2, B(k) = (Z(k) - conj Z(-k)) / 2i.
+void rotate_template_pair(const cd* Ta, const cd* Tb, cd* oa, cd* ob, int l, float theta, const P& p) {
+  const size_t L2 = (size_t)l * l;
+  std::vector<cd> Z(L2);
+  double* planes[2];
+  std::vector<double> ra(L2), rb(L2);
+  planes[0] = ra.data();
+  planes[1] = rb.data();
+  const cd* src[2] = {Ta, Tb};
+  for (int u = 0; u < 2; ++u) {
+    rotate_plane(src[u], oa, l, theta);  // oa as scratch
+    apply_mask(oa, l, p.d_m, p.ph.edge_half_width);
+    double sm = 0, s2 = 0;
+    for (size_t i = 0; i < L2; ++i) {
+      const double v = oa[i].real();
+      sm += v;
+      s2 += v * v;
+    }
+    const double mean = sm / (double)L2;
+    const double var = s2 / (double)L2 - mean * mean;
+    for (size_t i = 0; i < L2; ++i) {
+      double v = oa[i].real();
+      if (mean != 0) v -= mean;
+      if (var != 0) v /= std::sqrt(var);
+      planes[u][i] = v;
+    }
+  }
+  for (size_t i = 0; i < L2; ++i) Z[i] = cd(ra[i], rb[i]);
+  orc::fft2(Z.data(), l, l, -1);
+  const double sc = 0.5 / (double)L2;
+  for (int y = 0; y < l; ++y)
+    for (int x = 0; x < l; ++x) {
+      const size_t k = (size_t)y * l + x, km = (size_t)((l - y) % l) * l + (l - x) % l;
+      const cd z = Z[k], zc = std::conj(Z[km]);
+      oa[k] = (z + zc) * sc;
+      ob[k] = (z - zc) * cd(0, -1) * sc;
+    }
+}
+
 // Block spectra (SplitImage, src/norm.cu:404-477): Fourier-space, spectrally normalised.
 void split_blocks(const double* img, int nx, int ny, int l, const std::vector<int>& ox,
                   const std::vector<int>& oy, cd* out) {
@@ -542,18 +584,46 @@ void norm_correlations(const NormCtx& X, Fn&& fn) {
   std::vector<double> planes((size_t)group * nt * L2);
   for (int a = 0; a < X.na; ++a) {
 #pragma omp parallel for schedule(dynamic)
-    for (int t = 0; t < nt; ++t) rotate_template(X.T.data() + t * L2, R.data() + t * L2, l, X.angles[a], X.p);
+    for (int t = 0; t < nt; t += 2) {
+      if (t + 1 < nt)
+        rotate_template_pair(X.T.data() + t * L2, X.T.data() + (t + 1) * L2, R.data() + t * L2,
+                             R.data() + (t + 1) * L2, l, X.angles[a], X.p);
+      else
+        rotate_template(X.T.data() + t * L2, R.data() + t * L2, l, X.angles[a], X.p);
+    }
     for (int g0 = 0; g0 < nlist; g0 += group) {
       const int ng = std::min(group, nlist - g0);
+      // Templates t and t + 1 share one inverse transform: both products are Hermitian (real
+      // block and template planes; the checkerboard is even), so the transform of
+      // P_t + i P_(t+1) is c_t + i c_(t+1), equal to two separate transforms up to FP64 rounding.
+      const int tp = (nt + 1) / 2;
 #pragma omp parallel
       {
         std::vector<cd> C(L2);
 #pragma omp for schedule(dynamic)
-        for (int q = 0; q < ng * nt; ++q) {
-          const int i = q / nt, t = q % nt;
-          correlate(X.blocks.data() + (size_t)(g0 + i) * L2, R.data() + t * L2, C.data(), l);
-          double* o = planes.data() + (size_t)q * L2;
-          for (size_t k = 0; k < L2; ++k) o[k] = C[k].real() / (double)L2;
+        for (int q = 0; q < ng * tp; ++q) {
+          const int i = q / tp, t = 2 * (q % tp);
+          const cd* I = X.blocks.data() + (size_t)(g0 + i) * L2;
+          double* o0 = planes.data() + ((size_t)i * nt + t) * L2;
+          if (t + 1 < nt) {
+            const cd* T0 = R.data() + t * L2;
+            const cd* T1 = R.data() + (t + 1) * L2;
+            for (int y = 0; y < l; ++y)
+              for (int x = 0; x < l; ++x) {
+                const size_t k = (size_t)y * l + x;
+                const double ch = checker(x, y, l);
+                C[k] = (I[k] * std::conj(T0[k]) + cd(0, 1) * (I[k] * std::conj(T1[k]))) * ch;
+              }
+            orc::fft2(C.data(), l, l, +1);
+            double* o1 = o0 + L2;
+            for (size_t k = 0; k < L2; ++k) {
+              o0[k] = C[k].real() / (double)L2;
+              o1[k] = C[k].imag() / (double)L2;
+            }
+          } else {
+            correlate(I, R.data() + t * L2, C.data(), l);
+            for (size_t k = 0; k < L2; ++k) o0[k] = C[k].real() / (double)L2;
+          }
         }
       }
       fn(a, g0, ng, planes.data());
