@@ -56,6 +56,8 @@ def parse():
                     help="search restricted to --angles (default 8) then only the refinement leg "
                          "(for profiling the refinement kernels)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch-refine", type=int, default=4,
+                    help="K uneven micrographs for the batched + refined leg of the default line (0: skip)")
     ap.add_argument("--cpu-templates", type=int, default=4)
     ap.add_argument("--cpu-angles", type=int, default=4)
     return ap.parse_args()
@@ -93,18 +95,20 @@ def cpu_baseline(cfg, ds, nt, na):
 
 
 REFINE_PEAKS = 2000
+# the top REFINE_PEAKS picks are NMS-separated peaks (64 px, the search's own radius) at z >= 4.5
+REFINE_SELECT = (4.5, 64.0)
 REFINE_TOP = 300  # picks refined per micrograph in the batch workload
 REFINE_TOL = (0.05, 0.05, 0.1, 5.0)  # px, px, deg, Angstrom
 
 
 def refine_leg(eng, cfg, ds, no_cpu, sample=16):
-    """Adjoint refinement of the top REFINE_PEAKS block-pixel records of the search just run
-    (re-selected at z >= 5 without NMS), each perturbed by U(+-0.5 px) in x and y, U(+-2.5 deg)
+    """Adjoint refinement of the top REFINE_PEAKS picks of the search just run (re-selected at
+    z >= 4.5 with the search's 64 px NMS, so that every pick is a distinct peak), each perturbed by U(+-0.5 px) in x and y, U(+-2.5 deg)
     in theta and U(+-300 A) in defocus, 20 ascent steps of 0.1 (scale 2 px, 2 px, 3 deg,
     500 A).  'at tol': a sample of the picks is refined from the same start by the FP64 oracle
     with the same frozen mu/sigma; within_tol is the fraction whose four parameters agree
     within (0.05 px, 0.05 px, 0.1 deg, 5 A)."""
-    recs = eng.select(5.0, 0.0)[:REFINE_PEAKS]
+    recs = eng.select(REFINE_SELECT[0], REFINE_SELECT[1])[:REFINE_PEAKS]
     pk = eng.peaks_from_records(recs)
     rng = np.random.default_rng(3000)
     k = len(pk)
@@ -119,7 +123,8 @@ def refine_leg(eng, cfg, ds, no_cpu, sample=16):
     dev_ms = eng.refine_timing()
     log(f"refinement: {k} picks in {dt:.3f} s (device {dev_ms:.1f} ms)")
     res = {
-        "picks": k, "iterations": 20, "peaks_per_s": k / dt, "wall_s": dt, "device_ms": dev_ms,
+        "picks": k, "selection": f"z >= {REFINE_SELECT[0]}, NMS radius {REFINE_SELECT[1]} px",
+        "min_z_selected": float(recs["score"].min()) if k else None, "iterations": 20, "peaks_per_s": k / dt, "wall_s": dt, "device_ms": dev_ms,
         "mean_z_start": float(out["score0"].mean()), "mean_z_refined": float(out["score"].mean()),
         "tolerance": list(REFINE_TOL),
     }
@@ -152,6 +157,103 @@ def refine_leg(eng, cfg, ds, no_cpu, sample=16):
         "cpu_peaks_per_s": sample / cpu_dt, "cpu_cores": oracle.num_threads(),
     })
     log(f"refinement oracle sample: {ok.mean() * 100:.0f}% within tolerance, {cpu_dt:.1f} s on CPU")
+    return res
+
+
+def batch_items(k, seed=2000):
+    """K uneven micrographs: sizes drawn from {3710x3838, 4096x4096, 5760x4092}, defocus
+    U(0.8, 3.0) um, N_t ~ U{200..1000} templates of a size drawn from {128, 180, 256}, each with
+    its own template set and 100 particles at SNR 0.05."""
+    import synth
+
+    rng = np.random.default_rng(seed)
+    sizes = [(3710, 3838), (4096, 4096), (5760, 4092)]
+    items, work, desc, dss = [], 0, [], []
+    for i in range(k):
+        W, H = sizes[int(rng.integers(3))]
+        nt = int(rng.integers(200, 1001))
+        ts = int(rng.choice([128, 180, 256]))
+        df = float(rng.uniform(0.8, 3.0))
+        ds = synth.make_dataset(1000 + i, W, H, n_templates=nt, tsize=ts, n_particles=100,
+                                snr=0.05, defocus=df)
+        items.append({"image": ds.image, "ctf": ds.ctf, "templates": ds.templates})
+        dss.append(ds)
+        work += nt * 72 * W * H
+        desc.append(f"{W}x{H}/{nt}x{ts}/df{df:.2f}")
+        log(f"micrograph {i}: {desc[-1]}")
+    return items, work, desc, dss
+
+
+def batch_refine_leg(k, no_cpu, sample=8):
+    """Batched + refined form against the plain per-micrograph form on the same GPU: K uneven
+    micrographs searched by gs_search_batch_refine (one engine, staging thread, graph replay,
+    top REFINE_TOP picks of each micrograph refined right after its search) against the
+    reference's driver loop followed by refinement (an engine per micrograph, gs_search, then
+    gs_refine of its top REFINE_TOP picks).  A sample of refined picks of the last micrograph is
+    checked against the FP64 oracle refiner from the same start."""
+    import gisspa
+
+    t0 = time.perf_counter()
+    items, work, desc, dss = batch_items(k)
+    log(f"batch data ready in {time.perf_counter() - t0:.1f} s")
+    cfg = c1_config(gisspa)
+    with gisspa.Engine(cfg) as e:  # warm the library on the first item
+        e.load_templates(items[0]["templates"])
+        e.set_limits(1, 0)
+        e.search(items[0]["image"], items[0]["ctf"])
+    with gisspa.Engine(cfg) as e:
+        e.set_graph(True)
+        t1 = time.perf_counter()
+        recs_b, refs_b = e.search_batch_refine(items, REFINE_TOP)
+        t_batch = time.perf_counter() - t1
+    log(f"batch + refinement: {t_batch:.2f} s")
+    t1 = time.perf_counter()
+    refs_s, last = [], None
+    for it in items:
+        with gisspa.Engine(cfg) as e:
+            e.load_templates(it["templates"])
+            r = e.search(it["image"], it["ctf"])[:REFINE_TOP]
+            pk = e.peaks_from_records(r) if len(r) else None
+            refs_s.append(e.refine(pk) if len(r) else None)
+            if it is items[-1] and len(r):
+                last = (pk, e.stage(4, np.float64), e.stage(5, np.float64))
+    t_seq = time.perf_counter() - t1
+    log(f"per-micrograph search + refinement: {t_seq:.2f} s")
+    same = all((x is None and len(y) == 0) or
+               (x is not None and np.array_equal(x["score"], y["score"]) and np.array_equal(x["peak"], y["peak"]))
+               for x, y in zip(refs_s, refs_b))
+    n_ref = int(sum(len(r) for r in refs_b))
+    res = {
+        "micrographs": desc, "top_k": REFINE_TOP, "refined_picks": n_ref,
+        "seconds_batched_refined": t_batch, "seconds_per_micrograph": t_seq,
+        "micrographs_per_s": k / t_batch, "micrographs_per_s_per_micrograph_form": k / t_seq,
+        "gain": t_seq / t_batch, "correlations_per_s_incl_refinement": work / t_batch,
+        "refined_identical_to_per_micrograph_form": bool(same),
+    }
+    if not no_cpu and last is not None:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+
+        oracle.load()
+        pk, mu_all, sg_all = last
+        idx = np.linspace(0, len(pk) - 1, min(sample, len(pk))).astype(int)
+        l = cfg.window_size
+        q = pk["block"][idx].astype(np.int64) * l * l + pk["pixel"][idx]
+        opk = np.zeros(len(idx), dtype=oracle.PEAK_DTYPE)
+        for n in pk.dtype.names:
+            opk[n] = pk[n][idx]
+        ds = dss[-1]
+        R = oracle.Refiner(cfg, ds.image, ds.templates, ds.ctf)
+        R.float_geometry(True)
+        ref, zo, _, _ = R.run(opk, mu_all[q], sg_all[q], iters=20, step=0.1, scale=(2.0, 2.0, 3.0, 500.0))
+        R.close()
+        got = refs_b[-1]
+        dev = np.stack([np.abs(got["peak"][n][idx] - ref[n]) for n in ("dx", "dy", "theta", "ddf")], 1)
+        ok = np.all(dev <= np.array(REFINE_TOL), axis=1)
+        res.update({"oracle_sample": int(len(idx)), "within_tol": float(ok.mean()),
+                    "max_deviation": [float(v) for v in dev.max(0)],
+                    "max_score_rel_dev": float(np.max(np.abs(got["score"][idx] - zo) / np.abs(zo)))})
+        log(f"batch refinement oracle sample: {ok.mean() * 100:.0f}% within tolerance")
     return res
 
 
@@ -407,15 +509,20 @@ def main():
     # planes per launch = templates per launch x blocks; launches per step = angles x chunks
     launches_per_step = col_n / a.steps
     planes_per_launch = a.templates * n_ang * nb / launches_per_step
-    col_bytes = planes_per_launch * (8 * l * kx + 8 * l * (l // 2))
-    row_bytes = planes_per_launch * (8 * l * (l // 2))
-    dominant = "column_pass" if col_ms >= row_ms else "row_pass"
-    dom_ms, dom_bytes = (col_ms, col_bytes) if col_ms >= row_ms else (row_ms, row_bytes)
-    achieved = dom_bytes / (dom_ms * 1e-3)
+    tc = planes_per_launch / nb
+    # Algorithmic HBM bytes per launch, i.e. what the two-pass transform must move at least:
+    # column pass = inter-pass buffer written once per plane + each template spectrum and each
+    # block spectrum read once per launch (all blocks share a template spectrum, so its reads
+    # per plane are cache traffic, not HBM); row pass = inter-pass buffer read once per plane +
+    # one read-modify-write of the per-pixel records (sum, sum of squares: double; best: float;
+    # argmax: int32) per launch.
+    zbytes = 8 * l * (l // 2)
+    col_bytes = planes_per_launch * zbytes + (tc + nb) * 8 * l * kx
+    row_bytes = planes_per_launch * zbytes + nb * l * l * 24 * 2
     # measured HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated), from
-    # the committed counter summary, scaled from its 64-template launches to this average launch
-    traffic = None
+    # the committed counter summary of the same kernel variant, scaled to this launch's planes
     tpath = os.path.join(ROOT, "profiles", "corr_c1_traffic.json")
+    traffic = {"column_pass": None, "row_pass": None}
     if os.path.exists(tpath) and l == 320:
         col_name = {0: "k_colpass<320, 0, 0, 8, 16>", 3: "k_colpass<320, 1, 0, 8, 16>",
                     4: "k_colpass<320, 1, 1, 8, 16>", 5: "k_colpass<320, 0, 1, 8, 16>",
@@ -426,16 +533,32 @@ def main():
             3: "false, 8, false, false",
             4: "true, 4, true, false", 5: "false, 4, true, false", 6: "true, 8, true, false",
             7: "true, 8, true, true"}.get(variant[1], "")
-        kname = col_name if dominant == "column_pass" else row_name
-        rec = json.load(open(tpath))["kernels"].get(kname)
-        if rec:
-            per_plane = (rec["fetch_bytes_per_launch"] + rec["write_bytes_per_launch"]) / (
-                rec["templates_per_launch"] * rec["blocks"])
-            traffic = per_plane * planes_per_launch
-    # whole correlation stage: algorithmic bytes of every plane of the last step over the wall
-    # time of its angle loop (the two passes may run concurrently)
+        recs_t = json.load(open(tpath))["kernels"]
+        for key, kname in (("column_pass", col_name), ("row_pass", row_name)):
+            rec = recs_t.get(kname)
+            if rec:
+                per_plane = (rec["fetch_bytes_per_launch"] + rec["write_bytes_per_launch"]) / (
+                    rec["templates_per_launch"] * rec["blocks"])
+                traffic[key] = per_plane * planes_per_launch
+    kernels = {}
+    for key, ms, nbytes in (("column_pass", col_ms, col_bytes), ("row_pass", row_ms, row_bytes)):
+        bw = nbytes / (ms * 1e-3)
+        kernels[key] = {
+            "avg_launch_ms": ms, "algorithmic_bytes_per_launch": nbytes, "achieved_GBps": bw / 1e9,
+            "frac": bw / HBM_PEAK, "traffic": traffic[key],
+            "hbm_frac_counters": None if traffic[key] is None else traffic[key] / (ms * 1e-3) / HBM_PEAK,
+        }
+    dominant = "column_pass" if col_ms >= row_ms else "row_pass"
+    dom = kernels[dominant]
+    # work ratio of the whole correlation stage: 24 l (l/2 + 1) bytes per plane (template
+    # spectrum read + inter-pass write + read) over the angle loop's wall time; the template
+    # spectrum reads it counts per plane are mostly served by L2, so this is not an HBM fraction
     stage_bytes = a.templates * n_ang * nb * 24 * l * kx
     stage_bw = stage_bytes / (corr_ms * 1e-3)
+    valu = None
+    vpath = os.path.join(ROOT, "profiles", "corr_c1_valu.json")
+    if os.path.exists(vpath):
+        valu = json.load(open(vpath)).get("kernels")
     out = {
         "metric": METRIC,
         "value": value,
@@ -458,17 +581,25 @@ def main():
             "templates_per_launch": variant["chunk"] or None,
         },
         "roofline": {
-            "bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK, "traffic": traffic, "kernel": dominant,
-            "avg_launch_ms": dom_ms, "algorithmic_bytes_per_launch": dom_bytes,
-            "column_pass_ms": col_ms, "row_pass_ms": row_ms,
-            "correlation_stage_frac": stage_bw / HBM_PEAK,
+            "bound": "hbm", "achieved": dom["achieved_GBps"], "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+            "frac": dom["frac"], "traffic": dom["traffic"],
+            "traffic_source": "profiles/corr_c1_traffic.json: rocprofv3 FETCH_SIZE and WRITE_SIZE of the "
+                              "same kernel variant, copy-calibrated, scaled to this launch's planes",
+            "kernel": dominant, "avg_launch_ms": dom["avg_launch_ms"],
+            "algorithmic_bytes_per_launch": dom["algorithmic_bytes_per_launch"],
+            "hbm_frac_counters": dom["hbm_frac_counters"],
+            "kernels": kernels,
+            "limiter": valu,
+            "stage_work_ratio": stage_bw / HBM_PEAK,
         },
         "pipeline_ms": {"device_total": total_ms, "angle_loop": corr_ms},
         "copy_ceiling_GBps": gisspa.copy_bandwidth(1 << 30, 20),
     }
     if not a.angles and a.templates == 1000:
         out["refine"] = refine_leg(eng, cfg, ds, a.no_cpu)
+    eng.close()
+    if not a.angles and a.templates == 1000 and a.batch_refine > 0:
+        out["batch_refine"] = batch_refine_leg(a.batch_refine, a.no_cpu)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cfg, ds, a.cpu_templates, a.cpu_angles)
     else:
