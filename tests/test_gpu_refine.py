@@ -144,3 +144,29 @@ def test_select_reproduces_and_extends_the_records(case):
     # the run's own records are unchanged
     again = eng.records()
     assert np.array_equal(again["score"], recs["score"])
+
+
+def test_refinement_beyond_one_device_batch(case, orc):
+    """More picks than one device batch of the refinement (1024): the second batch reuses the
+    buffers of the first with a smaller count, and its picks must come out exactly as a
+    separate call on those picks alone, and agree with the oracle on a sample."""
+    eng = case["eng"]
+    base = _perturbed(case, 6, seed=11)
+    k = 1100
+    pk = np.concatenate([base] * (k // len(base) + 1))[:k].copy()
+    rng = np.random.default_rng(12)
+    pk["dx"] += rng.uniform(-0.2, 0.2, k)
+    pk["theta"] += rng.uniform(-0.5, 0.5, k)
+    out = eng.refine(pk, iterations=3)
+    tail = eng.refine(pk[1024:].copy(), iterations=3)
+    assert np.array_equal(out["score"][1024:], tail["score"])
+    assert np.array_equal(out["peak"][1024:], tail["peak"])
+    head = eng.refine(pk[:8].copy(), iterations=3)
+    np.testing.assert_allclose(out["score"][:8], head["score"], rtol=1e-6)
+    idx = np.array([0, 1023, 1024, 1099])
+    mu, sg = _frozen(case, pk[idx])
+    o, zo, _, _ = case["R"].run(_as_oracle(orc, pk[idx]), mu, sg, iters=3, step=0.1,
+                                scale=(2.0, 2.0, 3.0, 500.0))
+    np.testing.assert_allclose(out["score"][idx], zo, rtol=1e-4)
+    for n, tol in zip(("dx", "dy", "theta", "ddf"), TOL):
+        assert np.max(np.abs(out["peak"][n][idx] - o[n])) <= tol
