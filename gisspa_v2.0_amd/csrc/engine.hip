@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <functional>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -1359,6 +1360,11 @@ gs_status guarded(F&& f) {
   }
 }
 }  // namespace
+
+namespace gs {
+// The status plumbing above for the C entry points of other source files.
+gs_status c_abi_guarded(const std::function<void()>& f) { return guarded(f); }
+}  // namespace gs
 
 extern "C" {
 

@@ -1,4 +1,5 @@
 // Runtime-planned mixed-radix FFT (see fft_generic.hpp).
+#include <algorithm>
 #include <cmath>
 
 #include "common.hpp"
@@ -11,23 +12,14 @@ FftPlanGpu::~FftPlanGpu() {
   if (tw) (void)hipFree(tw);
 }
 
-static std::vector<int> factor_radices(int n) {
-  std::vector<int> f;
-  while (n % 4 == 0) { f.push_back(4); n /= 4; }
-  while (n % 2 == 0) { f.push_back(2); n /= 2; }
-  for (int p = 3; p * p <= n; p += 2)
-    while (n % p == 0) { f.push_back(p); n /= p; }
-  if (n > 1) f.push_back(n);
-  return f;
-}
-
 const FftPlanGpu& FftCache::plan(int n) {
   auto it = plans_.find(n);
   if (it != plans_.end()) return *it->second;
   GS_REQUIRE(n >= 1, kInvalid, "FFT length must be positive");
   auto p = std::make_unique<FftPlanGpu>();
   p->n = n;
-  p->radices = factor_radices(n);
+  int rad[FftTable::kMax];
+  p->radices.assign(rad, rad + factor_radices(n, rad, FftTable::kMax));
   std::vector<float2> h(n);
   for (int e = 0; e < n; ++e) {
     const double a = 2.0 * M_PI * (double)e / (double)n;
@@ -116,10 +108,23 @@ __global__ void k_copy_scale(const float2* __restrict__ src, float2* __restrict_
 
 static inline unsigned nblk(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
-void fft_rows(FftCache& cache, float2* data, float2* scratch, int n, long long nrows, int sign,
-              float scale, hipStream_t s) {
+int factor_radices(int n, int* out, int max_out) {
+  int k = 0;
+  auto push = [&](int r) {
+    GS_REQUIRE(k < max_out, kInvalid, "FFT length has too many factors");
+    out[k++] = r;
+  };
+  while (n % 4 == 0) { push(4); n /= 4; }
+  while (n % 2 == 0) { push(2); n /= 2; }
+  for (int p = 3; p * p <= n; p += 2)
+    while (n % p == 0) { push(p); n /= p; }
+  if (n > 1) push(n);
+  return k;
+}
+
+void fft_rows_table(const int* radices, int nrad, const float2* tw, float2* data, float2* scratch, int n,
+                    long long nrows, int sign, float scale, hipStream_t s) {
   if (nrows <= 0) return;
-  const FftPlanGpu& P = cache.plan(n);
   if (n == 1) {
     if (scale != 1.f) {
       k_copy_scale<<<nblk(nrows, 256), 256, 0, s>>>(data, data, nrows, scale);
@@ -130,15 +135,14 @@ void fft_rows(FftCache& cache, float2* data, float2* scratch, int n, long long n
   float2* src = data;
   float2* dst = scratch;
   int Ns = 1;
-  const int nst = (int)P.radices.size();
-  for (int st = 0; st < nst; ++st) {
-    const int R = P.radices[st];
-    const float sc = (st == nst - 1) ? scale : 1.f;
+  for (int st = 0; st < nrad; ++st) {
+    const int R = radices[st];
+    const float sc = (st == nrad - 1) ? scale : 1.f;
     const long long work = nrows * (n / R);
     if (sign < 0)
-      k_fft_stage<-1><<<nblk(work, 256), 256, 0, s>>>(src, dst, P.tw, n, R, Ns, nrows, sc);
+      k_fft_stage<-1><<<nblk(work, 256), 256, 0, s>>>(src, dst, tw, n, R, Ns, nrows, sc);
     else
-      k_fft_stage<+1><<<nblk(work, 256), 256, 0, s>>>(src, dst, P.tw, n, R, Ns, nrows, sc);
+      k_fft_stage<+1><<<nblk(work, 256), 256, 0, s>>>(src, dst, tw, n, R, Ns, nrows, sc);
     GS_CHECK_LAUNCH();
     std::swap(src, dst);
     Ns *= R;
@@ -147,6 +151,27 @@ void fft_rows(FftCache& cache, float2* data, float2* scratch, int n, long long n
     k_copy_scale<<<nblk((long long)n * nrows, 256), 256, 0, s>>>(src, data, (long long)n * nrows, 1.f);
     GS_CHECK_LAUNCH();
   }
+}
+
+void fft_rows(FftCache& cache, float2* data, float2* scratch, int n, long long nrows, int sign,
+              float scale, hipStream_t s) {
+  if (nrows <= 0) return;
+  const FftPlanGpu& P = cache.plan(n);
+  fft_rows_table(P.radices.data(), (int)P.radices.size(), P.tw, data, scratch, n, nrows, sign, scale, s);
+}
+
+// tw[e] = exp(-2 pi i e / n), evaluated in double precision and rounded to float (the values the
+// host builds for the plan cache).
+__global__ void k_fill_twiddles(float2* __restrict__ tw, int n) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const double a = 2.0 * M_PI * (double)e / (double)n;
+  tw[e] = make_float2((float)cos(a), (float)-sin(a));
+}
+
+void launch_fill_twiddles(float2* tw, int n, hipStream_t s) {
+  k_fill_twiddles<<<nblk(n, 256), 256, 0, s>>>(tw, n);
+  GS_CHECK_LAUNCH();
 }
 
 // 32x32 tiles through LDS; grid (ceil(cols/32), ceil(rows/32), batch).
@@ -178,13 +203,28 @@ void transpose_batched(const float2* in, float2* out, int batch, int rows, int c
   GS_CHECK_LAUNCH();
 }
 
+void fft2_table(const FftTable& rx, const FftTable& ry, float2* data, float2* scratch, int batch, int ny, int nx,
+                int sign, float scale, hipStream_t s) {
+  // rows (length nx), transpose to [nx][ny], rows (length ny), transpose back
+  fft_rows_table(rx.radices, rx.nrad, rx.tw, data, scratch, nx, (long long)batch * ny, sign, 1.f, s);
+  transpose_batched(data, scratch, batch, ny, nx, nx, s);
+  fft_rows_table(ry.radices, ry.nrad, ry.tw, scratch, data, ny, (long long)batch * nx, sign, scale, s);
+  transpose_batched(scratch, data, batch, nx, ny, ny, s);
+}
+
 void fft2_batched(FftCache& cache, float2* data, float2* scratch, int batch, int ny, int nx,
                   int sign, float scale, hipStream_t s) {
-  // rows (length nx), transpose to [nx][ny], rows (length ny), transpose back
-  fft_rows(cache, data, scratch, nx, (long long)batch * ny, sign, 1.f, s);
-  transpose_batched(data, scratch, batch, ny, nx, nx, s);
-  fft_rows(cache, scratch, data, ny, (long long)batch * nx, sign, scale, s);
-  transpose_batched(scratch, data, batch, nx, ny, ny, s);
+  const FftPlanGpu& px = cache.plan(nx);
+  const FftPlanGpu& py = cache.plan(ny);
+  FftTable rx{}, ry{};
+  rx.nrad = (int)px.radices.size();
+  ry.nrad = (int)py.radices.size();
+  GS_REQUIRE(rx.nrad <= FftTable::kMax && ry.nrad <= FftTable::kMax, kInvalid, "FFT length has too many factors");
+  std::copy(px.radices.begin(), px.radices.end(), rx.radices);
+  std::copy(py.radices.begin(), py.radices.end(), ry.radices);
+  rx.tw = px.tw;
+  ry.tw = py.tw;
+  fft2_table(rx, ry, data, scratch, batch, ny, nx, sign, scale, s);
 }
 
 }  // namespace gs

@@ -32,6 +32,25 @@ class FftCache {
   std::map<int, std::unique_ptr<FftPlanGpu>> plans_;
 };
 
+// A factorisation and twiddle table exp(-2 pi i e / n) (device, n entries) of one length,
+// held in a fixed-size struct so that a transform can be set up without any allocation.
+struct FftTable {
+  static constexpr int kMax = 32;
+  int radices[kMax];
+  int nrad;
+  const float2* tw;
+};
+// Radices of n (4s first, then 2, then odd primes) into out; returns their count.
+int factor_radices(int n, int* out, int max_out);
+// tw[e] = exp(-2 pi i e / n) for e < n, computed on the device.
+void launch_fill_twiddles(float2* tw, int n, hipStream_t s);
+// fft_rows with an explicit factorisation and twiddle table (no plan cache, nothing allocated).
+void fft_rows_table(const int* radices, int nrad, const float2* tw, float2* data, float2* scratch, int n,
+                    long long nrows, int sign, float scale, hipStream_t s);
+// fft2_batched with explicit tables for the row (nx) and column (ny) lengths.
+void fft2_table(const FftTable& rx, const FftTable& ry, float2* data, float2* scratch, int batch, int ny, int nx,
+                int sign, float scale, hipStream_t s);
+
 // In-place batched 1-D transform of nrows contiguous rows of length n, unnormalised, then
 // multiplied by scale.  scratch must hold n*nrows elements.
 void fft_rows(FftCache& cache, float2* data, float2* scratch, int n, long long nrows, int sign,

@@ -105,6 +105,7 @@ SIGNATURES = {
     "gs_set_limits": (C.c_int, [_P, C.c_int, C.c_int]),
     "gs_fft_c2c": (C.c_int, [C.c_int, _P, C.c_int, C.c_int, C.c_int]),
     "gs_fft2_c2c": (C.c_int, [C.c_int, _P, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "gs_fft2_c2c_noalloc": (C.c_int, [C.c_int, _P, C.c_int, C.c_int, C.c_int, C.c_int, _P]),
     "gs_last_timing": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "gs_set_profiling": (C.c_int, [_P, C.c_int]),
     "gs_kernel_timing": (C.c_int, [_P, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]),

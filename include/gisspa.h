@@ -137,6 +137,18 @@ gs_status gs_set_limits(gs_engine* e, int n_angles, int n_templates);
 gs_status gs_fft_c2c(int device, float* data, int n, int batch, int sign);
 /* Standalone 2-D transform of `batch` row-major ny x nx complex planes (in place). */
 gs_status gs_fft2_c2c(int device, float* data, int ny, int nx, int batch, int sign);
+/* gs_fft2_c2c without any allocation, for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  No counterpart in the
+ * reference.  d_data: `batch` row-major ny x nx complex planes (interleaved re/im float32) in
+ * device memory of `device`, transformed in place; the work is queued on `stream` (a hipStream_t,
+ * NULL = the default stream) and the call returns without waiting for it.  The transform's work
+ * buffer (batch * ny * nx + nx + ny complex elements) is a static 64 MiB device array of the
+ * library, one per device, held until the process ends: larger requests fail with
+ * GS_ERR_INVALID before any work is queued.  Nothing in it needs clearing between calls (every
+ * element read is written first by the same call); calls that overlap on one device share that
+ * array and race. */
+gs_status gs_fft2_c2c_noalloc(int device, float* d_data, int ny, int nx, int batch, int sign,
+                              void* stream);
 /* Measured device-memory copy rate (read + write bytes per second, GB/s) of a plain streaming
  * copy of `bytes` repeated `reps` times: the practical HBM ceiling reported next to the peak. */
 gs_status gs_copy_bandwidth(int device, size_t bytes, int reps, double* gbps);

@@ -61,6 +61,14 @@ def test_null_arguments(gs):
     assert lib.gs_autotune(None, None, None, None) == 1
 
 
+def test_noalloc_transform_checks_arguments_before_any_device_work(gs):
+    lib = gs.load()
+    assert lib.gs_fft2_c2c_noalloc(0, None, 8, 8, 1, -1, None) == 1
+    # beyond the static work buffer: rejected by the size check alone
+    assert lib.gs_fft2_c2c_noalloc(0, ctypes.c_void_p(256), 4096, 4096, 1, -1, None) == 1
+    assert b"exceed the work buffer" in lib.gs_last_error()
+
+
 def test_struct_layouts_match_the_header(gs, tmp_path):
     """The ctypes mirrors have the C layout (sizes and field offsets from the compiled header)."""
     fields = {
