@@ -168,6 +168,60 @@ __global__ void __launch_bounds__(BS)
   }
 }
 
+// Reference records: one wavefront per 256-element chunk of a (template, block) plane in the
+// correlation index order p = y*l + x (the reference's CCG after compute_corner_CCG: p reads
+// output ((y + l/2) mod l, (x + l/2) mod l)).  Lane j holds chunk elements j, j + 64, j + 128,
+// j + 192; get_peak_pos's tree (k:766-793: at stride s = 128, 64, ..., 1 element i keeps its
+// value unless element i + s is strictly greater) is followed exactly: the strides 128 and 64
+// inside the lane, the strides 32 .. 1 across lanes.
+__global__ void __launch_bounds__(BS)
+    k_gen_chunk_records(const float* __restrict__ out, int tc, int nb, int l, const double* __restrict__ mu,
+                        const double* __restrict__ sigma, int t0, int angle, float thresh,
+                        const int* __restrict__ offx, const int* __restrict__ offy, int bx, int Wimg, int Himg,
+                        float d_m, RefRec* __restrict__ rec, int* __restrict__ count, int max_out) {
+  const long long L2 = (long long)l * l;
+  const int chunks = (int)(L2 / 256);
+  const long long wid = (long long)blockIdx.x * (BS / 64) + threadIdx.x / 64;
+  const int lane = threadIdx.x % 64;
+  if (wid >= (long long)tc * nb * chunks) return;  // uniform per wavefront
+  const int c = (int)(wid % chunks);
+  const long long q = wid / chunks;  // plane index t * nb + b
+  const int b = (int)(q % nb), t = (int)(q / nb);
+  const float* o = out + q * L2;
+  float v[4];
+  int pos[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int p = c * 256 + lane + 64 * k;
+    const int py = p / l, px = p - py * l;
+    const int y = (py + l / 2) % l, x = (px + l / 2) % l;
+    const double m = mu[(long long)b * L2 + p], sd = sigma[(long long)b * L2 + p];
+    v[k] = sd > 0 ? (float)(((double)o[(long long)y * l + x] - m) / sd) : 0.f;  // update_CCG (k:709-721)
+    pos[k] = p;
+  }
+  // stride 128: element i vs i + 128 (lane values k = 0 vs 2, 1 vs 3); stride 64: k = 0 vs 1
+  if (v[2] > v[0]) { v[0] = v[2]; pos[0] = pos[2]; }
+  if (v[3] > v[1]) { v[1] = v[3]; pos[1] = pos[3]; }
+  if (v[1] > v[0]) { v[0] = v[1]; pos[0] = pos[1]; }
+  float z = v[0];
+  int zp = pos[0];
+  for (int st = 32; st > 0; st >>= 1) {
+    const float oz = __shfl_down(z, st, 64);
+    const int op = __shfl_down(zp, st, 64);
+    if (lane < st && oz > z) {
+      z = oz;
+      zp = op;
+    }
+  }
+  if (lane == 0 && z >= thresh) {  // src/norm.cu:583-590
+    const int cx = offx[b % bx] + zp % l, cy = offy[b / bx] + zp / l;
+    if (cx >= d_m && cx < Wimg - d_m && cy >= d_m && cy < Himg - d_m) {
+      const int slot = atomicAdd(count, 1);
+      if (slot < max_out) rec[slot] = RefRec{angle, b, t0 + t, c, zp, z};
+    }
+  }
+}
+
 // exp(-2 pi i e / l), e < l, per (device, l), built once (host double precision).
 const float2* twiddles(int l) {
   static std::mutex mu;
@@ -225,6 +279,21 @@ void launch_corr_generic_pixel(FftCache& cache, const CorrArgs& a, int l, float2
   const float* out = corr_generic_transform(cache, a, l, P, W, s);
   const long long n = (long long)a.nb * l * l;
   k_gen_fold_pixel<<<(unsigned)((n + BS - 1) / BS), BS, 0, s>>>(out, a.tc, a.nb, l, S, SS, best, arg, arg_base);
+  GS_CHECK_LAUNCH();
+}
+
+void launch_corr_generic_records(FftCache& cache, const CorrArgs& a, int l, float2* P, float2* W,
+                                 const double* mu, const double* sigma, int angle, float thresh,
+                                 const int* offx, const int* offy, int bx, int Wimg, int Himg, float d_m,
+                                 RefRec* rec, int* count, int max_out, hipStream_t s) {
+  GS_REQUIRE(l > 0 && l % 2 == 0 && ((long long)l * l) % 256 == 0, kInvalid,
+             "reference records need window_size * window_size divisible by 256");
+  const float* out = corr_generic_transform(cache, a, l, P, W, s);
+  const long long waves = (long long)a.tc * a.nb * ((long long)l * l / 256);
+  const long long blocks = (waves + BS / 64 - 1) / (BS / 64);
+  GS_REQUIRE(blocks < (1LL << 31), kInvalid, "correlation launch too large");
+  k_gen_chunk_records<<<(unsigned)blocks, BS, 0, s>>>(out, a.tc, a.nb, l, mu, sigma, a.t0, angle, thresh, offx,
+                                                     offy, bx, Wimg, Himg, d_m, rec, count, max_out);
   GS_CHECK_LAUNCH();
 }
 

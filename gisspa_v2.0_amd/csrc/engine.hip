@@ -395,11 +395,15 @@ class Engine {
     corr_ms_ = ms;
     clk_col_.harvest();
     clk_row_.harvest();
-    if (P_.norm_type)
-      collect_norm();
-    else
-      collect_nonorm();
-    nms(records_, P_.nms_radius);
+    if (P_.norm_type && P_.reference_records) {
+      collect_reference();  // the reference's own records, unsuppressed, in its order
+    } else {
+      if (P_.norm_type)
+        collect_norm();
+      else
+        collect_nonorm();
+      nms(records_, P_.nms_radius);
+    }
     ran_ = true;
   }
 
@@ -749,6 +753,11 @@ class Engine {
     GS_REQUIRE(P_.angpix > 0, kInvalid, "angpix must be positive");
     GS_REQUIRE(P_.overlap >= 0 && P_.overlap < P_.window_size, kInvalid, "overlap out of range");
     GS_REQUIRE(P_.norm_type || P_.window_size <= kTile, kInvalid, "window_size must be <= 720 for norm_type=0");
+    GS_REQUIRE(P_.reference_records == 0 || P_.reference_records == 1, kInvalid, "reference_records must be 0 or 1");
+    GS_REQUIRE(!P_.reference_records || P_.norm_type == 1, kUnsupported,
+               "reference_records applies to norm_type=1 (norm_type=0 records are the reference's already)");
+    GS_REQUIRE(!P_.reference_records || ((long long)P_.window_size * P_.window_size) % 256 == 0, kUnsupported,
+               "reference_records needs window_size * window_size divisible by 256 (the reference's chunks)");
   }
 
   void bind() { GS_HIP(hipSetDevice(device_)); }
@@ -1105,6 +1114,67 @@ class Engine {
     launch_finalize_norm(d_S_.p, d_SS_.p, d_best_.p, d_arg_.p, nb_, l, total_n, P_.threshold, doffx,
                          doffy, bx_, W_, H_, P_.diameter, d_mu_.ensure(npix), d_sigma_.ensure(npix),
                          cand, count, max_cand, stream_);
+    if (!P_.reference_records) return;
+    // Second pass of the reference (PickParticles, src/norm.cu:560-600): every correlation again,
+    // now with mu and sigma known, reduced to the reference's per-256-chunk records.  It runs on
+    // the generic correlation, whose two buffers take the inter-pass buffer's memory.
+    corr_generic_prepare(fft_, l);
+    const size_t per_t = 2 * corr_generic_buffer(l, 1, nb_);
+    const int tc2 = (int)std::max<size_t>(1, std::min<size_t>((size_t)nt, zsz * (overlap ? 2 : 1) / per_t));
+    float2* P2 = d_Z_.ensure(per_t * tc2);
+    float2* W2 = P2 + per_t / 2 * tc2;
+    RefRec* rec = d_refrec_.ensure(kMaxRefRecords);
+    int* rcount = d_refcount_.ensure(1);
+    GS_HIP(hipMemsetAsync(rcount, 0, sizeof(int), stream_));
+    for (int a = 0; a < na; ++a) {
+      const float cose = (float)std::cos((double)angles[a] * kRefPi / 180);
+      const float sine = (float)std::sin((double)angles[a] * kRefPi / 180);
+      launch_rotate_mask_pairs(d_tprep_.p, nt, l, cose, sine, P_.diameter, 4.0f, work, part, stream_);
+      launch_rotation_stats(part, nt, l, stats, stream_);
+      fft2_planes(work, tscr, (nt + 1) / 2, -1, 1.f);
+      launch_unpack_pair_spectra(work, nt, l, stats, (float)(1.0 / (double)L2), that, stream_);
+      for (int t0 = 0; t0 < nt; t0 += tc2) {
+        const CorrArgs ca{that, ihat, P2, t0, std::min(tc2, nt - t0), nb_};
+        launch_corr_generic_records(fft_, ca, l, P2, W2, d_mu_.p, d_sigma_.p, a, P_.threshold, doffx, doffy, bx_,
+                                    W_, H_, P_.diameter, rec, rcount, kMaxRefRecords, stream_);
+      }
+    }
+  }
+
+  // The reference records of the last run, in the reference's output order: angle, block
+  // (block_id = i + j * block_x), template, chunk (src/norm.cu:560-600).  No NMS: the reference
+  // prints every record.
+  void collect_reference() {
+    int count = 0;
+    GS_HIP(hipMemcpy(&count, d_refcount_.p, sizeof(int), hipMemcpyDeviceToHost));
+    GS_REQUIRE(count <= kMaxRefRecords, kUnsupported,
+               "more than " + std::to_string(kMaxRefRecords) + " reference records above threshold");
+    std::vector<RefRec> c(count);
+    if (count) GS_HIP(hipMemcpy(c.data(), d_refrec_.p, sizeof(RefRec) * count, hipMemcpyDeviceToHost));
+    std::sort(c.begin(), c.end(), [](const RefRec& a, const RefRec& b) {
+      if (a.angle != b.angle) return a.angle < b.angle;
+      if (a.block != b.block) return a.block < b.block;
+      if (a.templ != b.templ) return a.templ < b.templ;
+      return a.chunk < b.chunk;
+    });
+    const int l = P_.window_size;
+    records_.reserve(count);
+    for (const RefRec& k : c) {
+      gs_record r{};
+      r.x = (float)(offx_[k.block % bx_] + k.pixel % l);
+      r.y = (float)(offy_[k.block / bx_] + k.pixel / l);
+      r.score = k.score;
+      r.template_id = k.templ;
+      r.angle_index = k.angle;
+      float e = 360.0f - angles_[k.angle];  // src/norm.cu:654-655
+      if (e >= 360.0f) e -= 360.0f;
+      r.psi = e;
+      r.block = k.block;
+      r.pixel = k.pixel;
+      r.local_x = (float)(k.pixel % l);
+      r.local_y = (float)(k.pixel / l);
+      records_.push_back(r);
+    }
   }
 
   void collect_norm() {
@@ -1326,6 +1396,9 @@ class Engine {
   DevBuf<int32_t> d_arg_;
   DevBuf<int> d_offx_, d_offy_, d_count_;
   DevBuf<Cand> d_cand_;
+  static constexpr int kMaxRefRecords = 1 << 22;
+  DevBuf<RefRec> d_refrec_;
+  DevBuf<int> d_refcount_;
 };
 
 }  // namespace gs

@@ -114,6 +114,18 @@ void launch_corr_generic_pixel(FftCache& cache, const CorrArgs& a, int L, float2
                                double* SS, float* best, int32_t* arg, int arg_base, hipStream_t s);
 void launch_corr_generic_plane(FftCache& cache, const CorrArgs& a, int L, float2* P, float2* W, float4* partial,
                                float border, hipStream_t s);
+// Reference records (the second pass of src/norm.cu:560-600): one per (template, block,
+// 256-element chunk of the correlation index y*L + x) whose maximum of z = (c - mu)/sigma, taken
+// by get_peak_pos's tree (k:766-793), reaches thresh with its centre in [d_m, W - d_m) x
+// [d_m, H - d_m); appended in any order (count is an atomic counter).  L*L % 256 == 0.
+struct RefRec {
+  int32_t angle, block, templ, chunk, pixel;
+  float score;
+};
+void launch_corr_generic_records(FftCache& cache, const CorrArgs& a, int L, float2* P, float2* W,
+                                 const double* mu, const double* sigma, int angle, float thresh,
+                                 const int* offx, const int* offy, int bx, int Wimg, int Himg, float d_m,
+                                 RefRec* out, int* count, int max_out, hipStream_t s);
 // Builds the correlation kernels' twiddle tables on the current device (they are otherwise
 // built on first use, which must not happen inside a stream capture).
 void corr_prepare();

@@ -26,7 +26,7 @@ class Params(C.Structure):
         ("cs", C.c_float), ("highres", C.c_float), ("lowres", C.c_float), ("diameter", C.c_float),
         ("threshold", C.c_float), ("window_size", C.c_int32), ("overlap", C.c_int32),
         ("norm_type", C.c_int32), ("invert", C.c_int32), ("phase_flip", C.c_int32),
-        ("nms_radius", C.c_float),
+        ("nms_radius", C.c_float), ("reference_records", C.c_int32),
     ]
 
 
@@ -177,6 +177,7 @@ class SearchConfig:
     invert: int = 0
     phase_flip: int = 0
     nms_radius: float = 64.0
+    reference_records: int = 0
 
     def to_c(self) -> Params:
         return Params(**{f.name: getattr(self, f.name) for f in fields(self)})

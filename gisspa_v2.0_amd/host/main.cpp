@@ -5,7 +5,8 @@
 // writes one record line per pick in the reference's format (src/norm.cu:610-615).  Differences from the reference:
 // GPU_ID = -1 uses device 0 (one GPU per process; shard micrographs with first/last), errors
 // are reported with a non-zero exit status instead of being printed and ignored, and records
-// pass through NMS when nms_radius > 0.
+// pass through NMS when nms_radius > 0.  With reference_records = 1 (norm_type=1) the records are
+// the reference's own per-chunk records, in its order and without NMS.
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -138,6 +139,7 @@ int main(int argc, char** argv) {
   p.invert = conf.geti("invert");
   p.phase_flip = conf.geti("phase_flip");
   p.nms_radius = conf.getf("nms_radius");
+  p.reference_records = conf.geti("reference_records");
   const int tile_local = conf.geti("tile_local_centers");
 
   gs_engine* eng = nullptr;

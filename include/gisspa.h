@@ -42,11 +42,18 @@ typedef enum {
  * window_size: any even size, as in the reference (norm_type=0: at most 720).  320 and 720 run
  * the tiled correlation kernels; other sizes run a generic correlation on runtime-planned
  * transforms (same results to float rounding, slower), for which the kernel variants,
- * gs_autotune and the two-stream pipeline have no effect. */
+ * gs_autotune and the two-stream pipeline have no effect.
+ * reference_records is new (0 default): with norm_type=1, 1 makes the records of a run the
+ * reference's own output (src/norm.cu:560-600): one per (angle, block, template, 256-element
+ * chunk of the block's correlation index y*l + x) whose largest z reaches the threshold, the
+ * chunk maximum chosen by get_peak_pos's tree (src/kernels.cu:766-793), centre in
+ * [d_m, W - d_m) x [d_m, H - d_m), in the reference's order and without NMS.  It costs a second
+ * correlation pass (on the generic correlation) and needs window_size^2 divisible by 256. */
 typedef struct {
   float angpix, phistep, kk, energy, cs, highres, lowres, diameter, threshold;
   int32_t window_size, overlap, norm_type, invert, phase_flip;
   float nms_radius;
+  int32_t reference_records;
 } gs_params;
 
 /* Per-micrograph CTF of an LST line (src/EMReader/DataReader2.cpp:80-101):

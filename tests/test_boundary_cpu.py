@@ -153,3 +153,16 @@ def test_executable_reports_missing_keys(tmp_path):
     cfg.write_text("input = a.lst\ntemplate = t.mrc\neulerfile = e.txt\n")
     r = subprocess.run([EXE, str(cfg)], capture_output=True, text=True)
     assert r.returncode != 0 and "angpix" in r.stderr
+
+
+@pytest.mark.parametrize("changes", [dict(norm_type=0, window_size=720), dict(window_size=330)])
+def test_reference_records_limits(gs, changes):
+    """reference_records needs norm_type=1 and window_size^2 divisible by 256 (the reference's
+    256-element chunks); both are rejected at gs_create, before any device work."""
+    cfg = gs.SearchConfig(reference_records=1)
+    for k, v in changes.items():
+        setattr(cfg, k, v)
+    p = cfg.to_c()
+    h = ctypes.c_void_p()
+    assert gs.load().gs_create(ctypes.byref(p), 0, ctypes.byref(h)) == 3
+    assert b"reference_records" in gs.load().gs_last_error()
