@@ -95,15 +95,16 @@ def cpu_baseline(cfg, ds, nt, na):
 
 
 REFINE_PEAKS = 2000
-# the top REFINE_PEAKS picks are NMS-separated peaks (64 px, the search's own radius) at z >= 4.5
-REFINE_SELECT = (4.5, 64.0)
+# the top REFINE_PEAKS picks are NMS-separated peaks (64 px, the search's own radius) at z >= 5
+REFINE_SELECT = (5.0, 64.0)
 REFINE_TOP = 300  # picks refined per micrograph in the batch workload
 REFINE_TOL = (0.05, 0.05, 0.1, 5.0)  # px, px, deg, Angstrom
 
 
 def refine_leg(eng, cfg, ds, no_cpu, sample=16):
     """Adjoint refinement of the top REFINE_PEAKS picks of the search just run (re-selected at
-    z >= 4.5 with the search's 64 px NMS, so that every pick is a distinct peak), each perturbed by U(+-0.5 px) in x and y, U(+-2.5 deg)
+    z >= 5 with the search's 64 px NMS, so that every pick is a distinct peak), each perturbed
+    by U(+-0.5 px) in x and y, U(+-2.5 deg)
     in theta and U(+-300 A) in defocus, 20 ascent steps of 0.1 (scale 2 px, 2 px, 3 deg,
     500 A).  'at tol': a sample of the picks is refined from the same start by the FP64 oracle
     with the same frozen mu/sigma; within_tol is the fraction whose four parameters agree

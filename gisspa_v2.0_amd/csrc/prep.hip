@@ -170,14 +170,18 @@ __global__ void k_radial_power_planes(const float2* __restrict__ F, int l, doubl
   radial_merge(hist, waves, nbins, part + ((long long)t * kChunks + ch) * nbins);
 }
 
+// ra[t][bin] = sum over chunks of part[t][chunk][bin], one wavefront per (t, bin): lane j adds
+// chunks j, j + 64, ... in order, then a fixed butterfly over the lanes (deterministic).
 __global__ void k_radial_power_finish(const double* __restrict__ part, int n, int nbins, int chunks,
                                       double* __restrict__ ra) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (long long)n * nbins) return;
-  const long long t = i / nbins, b = i % nbins;
+  const long long w = (long long)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+  const int lane = threadIdx.x % 64;
+  if (w >= (long long)n * nbins) return;  // uniform per wavefront
+  const long long t = w / nbins, b = w % nbins;
   double s = 0;
-  for (int ch = 0; ch < chunks; ++ch) s += part[(t * chunks + ch) * nbins + b];
-  ra[i] = s;
+  for (int ch = lane; ch < chunks; ch += 64) s += part[(t * chunks + ch) * nbins + b];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if (lane == 0) ra[w] = s;
 }
 
 // whiten_Tmp (k:128-149): |F| /= sqrt(ra[r]/rb[r]) for r in [0, l/2).
@@ -603,7 +607,7 @@ void launch_radial_power_planes(const float2* F, int n, int l, double* part, dou
   GS_REQUIRE(4 * nbins * sizeof(double) <= 64 * 1024, kUnsupported, "window too large for radial histogram");
   k_radial_power_planes<<<dim3(kChunks, n), BS, 4 * sizeof(double) * nbins, s>>>(F, l, part);
   GS_CHECK_LAUNCH();
-  k_radial_power_finish<<<nblk((long long)n * nbins), BS, 0, s>>>(part, n, nbins, kChunks, ra);
+  k_radial_power_finish<<<nblk((long long)n * nbins * 64), BS, 0, s>>>(part, n, nbins, kChunks, ra);
   GS_CHECK_LAUNCH();
 }
 void launch_whiten_planes(float2* F, int n, int l, const double* ra, const double* rb, hipStream_t s) {
@@ -656,7 +660,7 @@ void launch_radial_power_image(const float2* F, int nx, int ny, double* part, do
   GS_REQUIRE(waves >= 1, kUnsupported, "frame too large for radial histogram");
   k_radial_power_image<<<kImageGroups, 64 * waves, sizeof(double) * nbins * waves, s>>>(F, nx, ny, part);
   GS_CHECK_LAUNCH();
-  k_radial_power_finish<<<nblk(nbins), BS, 0, s>>>(part, 1, nbins, kImageGroups, ra);
+  k_radial_power_finish<<<nblk((long long)nbins * 64), BS, 0, s>>>(part, 1, nbins, kImageGroups, ra);
   GS_CHECK_LAUNCH();
 }
 void launch_whiten_filter_image(float2* F, int nx, int ny, const double* ra, const double* rb,
