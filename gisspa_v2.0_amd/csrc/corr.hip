@@ -569,6 +569,236 @@ __global__ void __launch_bounds__(CorrCfg<L>::RT)
   }
 }
 
+// Column pass of the norm_type=0 correlation at l = 720, one column per wavefront: workgroup
+// (template, block, group of 8 columns) with 8 waves; each wave forms the product of its
+// spectrum column (column 0 packs the Nyquist column as its imaginary part), transforms it in
+// its own LDS slice (64 lanes, radix 4 * 4 * 5 * 9) and the workgroup writes the group's 8
+// columns row-shifted by l/2 into the column-grouped inter-pass layout
+// Zg[t][b][group][y][8], one contiguous 46 KB region per workgroup.  Against 8 columns over 4
+// waves of 32 lanes (46 KB of LDS per 4 waves, 64-byte row segments scattered over the plane)
+// this holds twice the wavefronts per CU and writes whole lines.
+template <int L>
+struct ColGroupCfg;
+template <>
+struct ColGroupCfg<720> {
+  static constexpr int H = 360, COLS = 8;
+  using Plan = ct::RadixList<4, 4, 5, 9>;
+};
+
+template <int L>
+__global__ void __launch_bounds__(64 * ColGroupCfg<L>::COLS)
+    k_colpass_grouped(const float2* __restrict__ that, const float2* __restrict__ ihat,
+                      const float2* __restrict__ twL, float2* __restrict__ Z, int t0, int tc, int nb) {
+  using C = ColGroupCfg<L>;
+  constexpr int H = C::H, COLS = C::COLS, NT = 64 * COLS;
+  constexpr int NCG = H / COLS;
+  constexpr long long PLANE = (long long)(H + 1) * L;
+  constexpr int NLD = (L + 63) / 64;
+  static_assert(H % COLS == 0 && COLS % 2 == 0, "column groups");
+  __shared__ float2 buf[COLS * L + L];
+  float2* const tw = buf + COLS * L;
+  // XCD-aware order: each XCD takes a contiguous range of (group, block, template), templates
+  // fastest, so an XCD's L2 holds the group's spectrum columns of the launch
+  const unsigned G = gridDim.x, bid = blockIdx.x;
+  const unsigned xcd = bid % 8u, idx = bid / 8u, per = G / 8u, rem = G % 8u;
+  const unsigned lg = (xcd < rem ? xcd * (per + 1u) : rem * (per + 1u) + (xcd - rem) * per) + idx;
+  const int tl = (int)(lg % (unsigned)tc);
+  const int b = (int)((lg / (unsigned)tc) % (unsigned)nb);
+  const int cg = (int)(lg / ((unsigned)tc * (unsigned)nb));
+  const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
+  const int kx = cg * COLS + wave;
+  const float2* Tp = that + (long long)(t0 + tl) * PLANE;
+  const float2* Ip = ihat + (long long)b * PLANE;
+  float2 tv[NLD], iv[NLD], tn[NLD], in_[NLD];
+#pragma unroll
+  for (int i = 0; i < NLD; ++i) {
+    const int ky = lane + 64 * i;
+    if (L % 64 == 0 || ky < L) {
+      tv[i] = Tp[(long long)kx * L + ky];
+      iv[i] = Ip[(long long)kx * L + ky];
+    }
+  }
+  const bool dc = kx == 0;  // column 0 carries the Nyquist column as its imaginary part
+  if (dc) {
+#pragma unroll
+    for (int i = 0; i < NLD; ++i) {
+      const int ky = lane + 64 * i;
+      if (L % 64 == 0 || ky < L) {
+        tn[i] = Tp[(long long)H * L + ky];
+        in_[i] = Ip[(long long)H * L + ky];
+      }
+    }
+  }
+  for (int i = tid; i < L; i += NT) tw[i] = twL[i];
+  float2* const col = buf + wave * L;
+#pragma unroll
+  for (int i = 0; i < NLD; ++i) {
+    const int ky = lane + 64 * i;
+    if (L % 64 == 0 || ky < L) {
+      float2 p = c_mulconj(iv[i], tv[i]);
+      if (dc) {
+        const float2 pn = c_mulconj(in_[i], tn[i]);
+        p = make_float2(p.x - pn.y, p.y + pn.x);
+      }
+      col[ky] = p;
+    }
+  }
+  __syncthreads();  // the twiddle table
+  ct::run_stockham<L, 64, +1, ct::Sync::WAVE>(col, tw, lane, typename C::Plan{});
+  __syncthreads();  // rows of the group gather all its columns
+  float4* const Zg = reinterpret_cast<float4*>(Z + (((long long)tl * nb + b) * NCG + cg) * (long long)L * COLS);
+  constexpr int NQ = L * COLS / 2;  // 16-byte words of the group's region
+#pragma unroll
+  for (int e0 = 0; e0 < NQ; e0 += NT) {
+    const int e = e0 + tid;
+    if (NQ % NT == 0 || e < NQ) {
+      const int yd = e / (COLS / 2), cp = e % (COLS / 2);
+      const int y = yd < L / 2 ? yd + L / 2 : yd - L / 2;  // output row yd holds transform row y
+      const float2 v0 = buf[(2 * cp) * L + y], v1 = buf[(2 * cp + 1) * L + y];
+      st_stream(Zg + e, make_float4(v0.x, v0.y, v1.x, v1.y));
+    }
+  }
+}
+
+// Row pass, norm_type=0, wavefront-private (l = 720): one wavefront per (template, block, chunk of
+// RCH rows) of the correlation planes, read from the column-grouped layout of k_colpass_grouped.  Each wave iteration transforms TPW rows (one length-H
+// inverse transform of the packed row per TPT lanes, radix 6 * 6 * 10), with the next
+// iteration's Hermitian pairs already loaded, and folds the rows' outputs into per-lane
+// records: the largest value off the border of width d_m/4 and above 0 (lowest plane index among
+// equal values), the sum and the sum of squares (float per row, double across rows).  One
+// partial per wave, reduced over the lanes with the order-independent rule of plane_combine.
+// With one transform per wave (TPT = 64) a wave needs 3 KB of LDS and few registers, so that
+// many waves per SIMD hide the latency of each transform's LDS round trips (five transforms of
+// 12 lanes per wave ran at 2 waves per SIMD and half the speed).
+// Replaces get_peak_and_SUM (k:725-763) over the planes of src/nonorm.cu:478-509.
+template <int L>
+struct PlaneWaveCfg;
+template <>
+struct PlaneWaveCfg<720> {
+  static constexpr int H = 360, TPT = 64, TPW = 1, RCH = 20, WAVES = 4;
+  using Plan = ct::RadixList<6, 6, 10>;
+};
+
+template <int L>
+__global__ void __launch_bounds__(64 * PlaneWaveCfg<L>::WAVES)
+    k_rowpass_plane_wave(const float2* __restrict__ Z, const float2* __restrict__ twL,
+                         const float2* __restrict__ twH, int tc, int nb, float border,
+                         float4* __restrict__ partial) {
+  using C = PlaneWaveCfg<L>;
+  constexpr int H = C::H, TPT = C::TPT, TPW = C::TPW, RCH = C::RCH, WAVES = C::WAVES;
+  constexpr int NCH = L / RCH;                  // chunks (partials) per plane
+  constexpr int ITER = RCH / TPW;               // wave iterations per chunk
+  constexpr int NP = (H / 2 + TPT - 1) / TPT;   // Hermitian pairs per lane and row (last guarded)
+  constexpr int NO = (H + TPT - 1) / TPT;       // output slots per lane and row (last guarded)
+  constexpr bool DUMMY = TPW * TPT < 64;        // lanes past the last transform run a spare slice
+  // transforms HS slots apart (with 12 lanes per transform, 2 * (H + 4) = 728 dwords puts the
+  // wave's transforms 24 banks apart)
+  constexpr int HS = H + 4;
+  constexpr int SLICE = (TPW + (DUMMY ? 1 : 0)) * HS;
+  static_assert(L % RCH == 0 && RCH % TPW == 0 && TPW * TPT <= 64, "row mapping");
+  static_assert(!DUMMY || TPW * TPT + TPT > 64, "the spare slice covers the remaining lanes");
+  __shared__ float2 smem[WAVES * SLICE + H + H / 2 + 1];
+  float2* const twh = smem + WAVES * SLICE;  // exp(-2 pi i e / H)
+  float2* const twc = twh + H;               // exp(+2 pi i k / L), k <= H/2
+  const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
+  for (int i = tid; i < H; i += 64 * WAVES) twh[i] = twH[i];
+  for (int i = tid; i <= H / 2; i += 64 * WAVES) {
+    const float2 w = twL[i];
+    twc[i] = make_float2(w.x, -w.y);
+  }
+  __syncthreads();
+  const long long wid = (long long)blockIdx.x * WAVES + wave;
+  if (wid >= (long long)tc * nb * NCH) return;  // whole wavefronts; no barrier follows
+  const int ch = (int)(wid % NCH);
+  const long long q = wid / NCH;  // plane t * nb + b
+  const float2* const Zq = Z + q * (long long)L * H;
+  const int y0 = ch * RCH;
+  const int tr = lane / TPT, sub = lane % TPT;
+  const bool act = tr < TPW;  // lanes of the spare slice load nothing and fold nothing
+  float2* const dst = smem + wave * SLICE + tr * HS;
+
+  float2 pa[NP], pb[NP], p0 = make_float2(0.f, 0.f);
+#pragma unroll
+  for (int m = 0; m < NP; ++m) pa[m] = pb[m] = make_float2(0.f, 0.f);
+  // element k of row y in the column-grouped layout Zg[..][k / 8][y][k % 8] of k_colpass_grouped
+  constexpr int GC = ColGroupCfg<L>::COLS;
+  auto at = [](const float2* zq, int y, int k) { return zq[((long long)(k / GC) * L + y) * GC + k % GC]; };
+  auto prefetch = [&](int it) {
+    if (act) {
+      const int y = y0 + it * TPW + tr;
+      if (sub == 0) p0 = at(Zq, y, 0);
+#pragma unroll
+      for (int m = 0; m < NP; ++m) {
+        const int k = 1 + sub + TPT * m;  // 1 .. H/2
+        if ((H / 2) % TPT == 0 || k <= H / 2) {
+          pa[m] = at(Zq, y, k);
+          pb[m] = at(Zq, y, H - k);
+        }
+      }
+    }
+  };
+  float vbest = 0.f;
+  int ibest = 0x7fffffff;
+  double ds = 0.0, ds2 = 0.0;
+  prefetch(0);
+  for (int it = 0; it < ITER; ++it) {
+    // real-to-complex packing of the row's inverse transform (X(0) and X(H) share slot 0)
+    if (sub == 0) dst[0] = make_float2(p0.x + p0.y, p0.x - p0.y);
+#pragma unroll
+    for (int m = 0; m < NP; ++m) {
+      const int k = 1 + sub + TPT * m;
+      if ((H / 2) % TPT == 0 || k <= H / 2) {
+        float2 zk, zm;
+        zpack2(pa[m], pb[m], twc[k], zk, zm);
+        dst[k] = zk;
+        if (k != H - k) dst[H - k] = zm;
+      }
+    }
+    if (it + 1 < ITER) prefetch(it + 1);
+    ct::lds_sync<ct::Sync::WAVE>();
+    ct::run_stockham<H, TPT, +1, ct::Sync::WAVE>(dst, twh, sub, typename C::Plan{});
+    if (act) {
+      const int py = y0 + it * TPW + tr;
+      const bool yedge = py < border || py > L - border;
+      float rs = 0.f, rs2 = 0.f;
+#pragma unroll
+      for (int i = 0; i < NO; ++i) {
+        const int sl = sub + TPT * i;
+        if (H % TPT == 0 || sl < H) {
+          const float2 v2 = dst[sl];
+#pragma unroll
+          for (int h = 0; h < 2; ++h) {
+            const float v = h ? v2.y : v2.x;
+            const int x = 2 * sl + h;
+            const int px = x < L / 2 ? x + L / 2 : x - L / 2;
+            rs += v;
+            rs2 = fmaf(v, v, rs2);
+            const int id = py * L + px;
+            const bool ok = !yedge && !(px < border || px > L - border) && v > 0.f &&
+                            (v > vbest || (v == vbest && id < ibest));
+            vbest = ok ? v : vbest;
+            ibest = ok ? id : ibest;
+          }
+        }
+      }
+      ds += (double)rs;
+      ds2 += (double)rs2;
+    }
+    ct::lds_sync<ct::Sync::WAVE>();  // the next packing overwrites the slice
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    const float ov = __shfl_down(vbest, o, 64);
+    const int oi = __shfl_down(ibest, o, 64);
+    ds += __shfl_down(ds, o, 64);
+    ds2 += __shfl_down(ds2, o, 64);
+    if (lane + o < 64 && (ov > vbest || (ov == vbest && oi < ibest))) {
+      vbest = ov;
+      ibest = oi;
+    }
+  }
+  if (lane == 0) partial[q * NCH + ch] = make_float4(vbest, __int_as_float(ibest), (float)ds, (float)ds2);
+}
+
 // Row pass, norm_type=1, wavefront-private rows: each wavefront owns RW rows of the block and
 // its own LDS slice, so the template loop needs no workgroup barrier.  Per iteration a wave
 // transforms RW rows x TPI templates (TPT lanes per length-H transform); each lane folds the
@@ -1021,7 +1251,7 @@ const float2* plane_twiddles(int L) {
 // wavefront.
 int rowpass_groups(int L) {
   if (L == 320) return 320 / CorrCfg<320>::ROWS * ((CorrCfg<320>::RT + 63) / 64);
-  if (L == 720) return 720 / CorrCfg<720>::ROWS * ((CorrCfg<720>::RT + 63) / 64);
+  if (L == 720) return 720 / PlaneWaveCfg<720>::RCH;
   return 0;
 }
 
@@ -1064,6 +1294,17 @@ static void colpass_impl(const CorrArgs& a, const float2* twL, int variant, hipS
       k_colpass_pers<L, false><<<G, C::CT, 0, s>>>(a.that, a.ihat, twL, a.Z, a.t0, a.tc, a.nb);
   }
   GS_CHECK_LAUNCH();
+}
+
+bool launch_colpass_plane(int L, const CorrArgs& a, int variant, hipStream_t s) {
+  if (L != 720) return launch_colpass(L, a, variant, s);
+  TwTables& T = tw_tables(current_device());
+  using C = ColGroupCfg<720>;
+  const long long G = (long long)(C::H / C::COLS) * a.tc * a.nb;
+  GS_REQUIRE(G > 0 && G < (1LL << 31), kInvalid, "column pass grid out of range");
+  k_colpass_grouped<720><<<(unsigned)G, 64 * C::COLS, 0, s>>>(a.that, a.ihat, T.t720, a.Z, a.t0, a.tc, a.nb);
+  GS_CHECK_LAUNCH();
+  return true;
 }
 
 bool launch_colpass(int L, const CorrArgs& a, int variant, hipStream_t s) {
@@ -1134,7 +1375,12 @@ bool launch_rowpass_plane(int L, const float2* Z, int tc, int nb, float4* partia
     return true;
   }
   if (L == 720) {
-    rowpass_impl<720, true>(Z, tc, nb, nullptr, nullptr, nullptr, nullptr, 0, partial, border, T.t720, T.t360, s);
+    using C = PlaneWaveCfg<720>;
+    const long long waves = (long long)tc * nb * (720 / C::RCH);
+    const long long blocks = (waves + C::WAVES - 1) / C::WAVES;
+    GS_REQUIRE(blocks > 0 && blocks < (1LL << 31), kInvalid, "row pass grid out of range");
+    k_rowpass_plane_wave<720><<<(unsigned)blocks, 64 * C::WAVES, 0, s>>>(Z, T.t720, T.t360, tc, nb, border, partial);
+    GS_CHECK_LAUNCH();
     return true;
   }
   return false;

@@ -853,7 +853,7 @@ class Engine {
     const size_t L2 = (size_t)l * l;
     launch_pad_templates(d_raw_.p, n, w_, h_, l, T, stream_);
     double* mom = d_mom_.ensure((size_t)n * 4);
-    double* part = d_chpart_.ensure((size_t)n * 16 * std::max(4, l / 2));
+    double* part = d_chpart_.ensure((size_t)n * plane_reduction_chunks(n) * std::max(4, l / 2));
     launch_template_moments(T, n, l, part, mom, stream_);
     launch_template_standardise(T, n, l, mom, stream_);
     fft2_planes(T, S, n, -1, (float)(1.0 / (double)L2));
@@ -875,7 +875,7 @@ class Engine {
     }
     launch_weight_planes(T, n, l, dp_, stream_);
     double* m = d_m_.ensure(n);
-    launch_area_power_full(T, n, l, d_apart_.ensure((size_t)n * 16), m, stream_);
+    launch_area_power_full(T, n, l, d_apart_.ensure((size_t)n * plane_reduction_chunks(n)), m, stream_);
     const float sign = ((l / 2) % 2) ? -1.f : 1.f;  // checkerboard sign for odd l/2 (k:669-677)
     launch_scale_planes(T, n, (long long)L2, m, P_.norm_type ? 1.f : sign, stream_);
     if (P_.norm_type) {
@@ -979,7 +979,7 @@ class Engine {
     float2* ihat = d_ihat_.ensure((size_t)nb_ * hx * l);
     half_spectra(blk, nb_, ihat, (float)(1.0 / (double)L2));
     double* m = d_m_.ensure(std::max(n, nb_));
-    launch_area_power_half(ihat, nb_, l, d_apart_.ensure((size_t)nb_ * 16), m, stream_);
+    launch_area_power_half(ihat, nb_, l, d_apart_.ensure((size_t)nb_ * plane_reduction_chunks(nb_)), m, stream_);
     // fold the 1/l^2 of add_CCG_to_sum (k:691) and the odd-l/2 checkerboard sign into Ihat
     const float sign = ((l / 2) % 2) ? -1.f : 1.f;
     launch_scale_planes(ihat, nb_, (long long)hx * l, m, (float)(sign / (double)L2), stream_);
@@ -1262,7 +1262,7 @@ class Engine {
       if (P_.phase_flip == 1) {  // src/nonorm.cu:405-447
         float2* S = d_scratch_.ensure((size_t)nbb * L2);
         fft2_planes(blk, S, nbb, -1, (float)(1.0 / (double)L2));
-        launch_area_power_full(blk, nbb, l, d_apart_.ensure((size_t)nbb * 16), m, stream_);
+        launch_area_power_full(blk, nbb, l, d_apart_.ensure((size_t)nbb * plane_reduction_chunks(nbb)), m, stream_);
         launch_scale_planes(blk, nbb, (long long)L2, m, 1.f, stream_);
         fft2_planes(blk, S, nbb, +1, 1.f);
       }
@@ -1280,7 +1280,7 @@ class Engine {
             clock_end(clk_col_, stream_);
           } else {
             // the norm_type=0 row pass reads the unpacked layout: variant 8 runs as 7 here
-            GS_REQUIRE(launch_colpass(l, ca, col_variant_ == 8 ? 7 : col_variant_, stream_), kUnsupported,
+            GS_REQUIRE(launch_colpass_plane(l, ca, col_variant_ == 8 ? 7 : col_variant_, stream_), kUnsupported,
                        "window_size not supported");
             clock_end(clk_col_, stream_);
             clock_begin(clk_row_, stream_);

@@ -527,7 +527,9 @@ template <int N, int T, int SIGN, Sync S, TwSrc TS, typename MS, int... RS>
 __device__ __forceinline__ void run_stockham_maps(float2* buf, const float2* tw, int tid, RadixList<RS...>,
                                                   const MS& maps) {
   static_assert(RadixList<RS...>::product() == N, "radix plan must multiply to N");
-  static_assert(S == Sync::BLOCK || (64 % T == 0), "wave-level sync needs transforms inside one wavefront");
+  // WAVE: the caller places every transform inside one wavefront (with 64 % T != 0 it maps the
+  // floor(64 / T) transforms of a wave to lanes [T j, T j + T) itself)
+  static_assert(S == Sync::BLOCK || T <= 64, "wave-level sync needs transforms inside one wavefront");
   static_assert((TS != TwSrc::LANE && TS != TwSrc::RECUR) || lane_twiddles_fit<T>(RadixList<RS...>{}),
                 "lane twiddles need T % NS == 0");
   stockham<N, T, SIGN, 1, S, MS, TS, 0, 0, RS...>(buf, tw, tid, maps);

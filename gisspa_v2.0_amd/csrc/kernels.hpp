@@ -37,6 +37,10 @@ void launch_pad_templates(const float* raw, int n, int w, int h, int l, float2* 
 void launch_template_moments(const float2* T, int n, int l, double* part, double* mom, hipStream_t s);
 void launch_template_standardise(float2* T, int n, int l, const double* mom, hipStream_t s);
 // part: scratch of 16 * (l/2) doubles per plane.
+// Workgroups per plane of the chunked per-plane reductions (template moments, radial power, area
+// power): 16 for many planes, more for few so that a small batch still fills the GPU.  Their
+// partial buffers hold n * plane_reduction_chunks(n) entries per reduced quantity.
+int plane_reduction_chunks(int n);
 void launch_radial_power_planes(const float2* F, int n, int l, double* part, double* ra, hipStream_t s);
 void launch_whiten_planes(float2* F, int n, int l, const double* ra, const double* rb, hipStream_t s);
 void launch_weight_planes(float2* F, int n, int l, const DevParams& p, hipStream_t s);
@@ -96,6 +100,10 @@ bool launch_colpass(int L, const CorrArgs& a, int variant, hipStream_t s);
 // variant (result-invariant): 0 plain, 1 software-prefetched Z loads.
 bool launch_rowpass_pixel(int L, const float2* Z, int tc, int nb, double* S, double* SS,
                           float* best, int32_t* arg, int arg_base, int variant, hipStream_t s);
+// Column pass of the norm_type=0 correlation: as launch_colpass, except at L = 720, where one
+// column per wavefront writes the column-grouped inter-pass layout that the L = 720 row pass of
+// launch_rowpass_plane reads (variant ignored there).
+bool launch_colpass_plane(int L, const CorrArgs& a, int variant, hipStream_t s);
 // Row pass, norm_type=0: C2R rows + per-(template,block,row group) peak/sum partials.
 bool launch_rowpass_plane(int L, const float2* Z, int tc, int nb, float4* partial, float border,
                           hipStream_t s);

@@ -42,14 +42,14 @@ __global__ void k_pad_templates(const float* __restrict__ raw, int n, int w, int
   out[i] = make_float2(v, 0.f);
 }
 
-// Per-plane reductions are split over kChunks workgroups per plane (slices of the plane), whose
-// partials a finishing kernel adds in chunk order.
-constexpr int kChunks = 16;
+// Per-plane reductions are split over plane_reduction_chunks(n) workgroups per plane (slices of
+// the plane, gridDim.x of the reduction kernels), whose partials a finishing kernel adds in
+// chunk order.
 // Workgroups (row slices) of the whole-frame radial power.
 constexpr int kImageGroups = 256;
 
 // Ring sum/count (generate_mask + multiCount_dot, k:28-80) and box sums (UpdateSigma, k:4-26);
-// grid (kChunks, n), partials [t][chunk][4].
+// grid (chunks, n), partials [t][chunk][4].
 __global__ void k_template_moments(const float2* __restrict__ T, int l, double* __restrict__ part) {
   const int t = blockIdx.y, ch = blockIdx.x;
   const float2* P = T + (long long)t * l * l;
@@ -59,7 +59,8 @@ __global__ void k_template_moments(const float2* __restrict__ T, int l, double* 
     up = (r + 1) * (r + 1);
     low = (r - 1) * (r - 1);
   }
-  const int per = (l * l + kChunks - 1) / kChunks;
+  const int chunks = gridDim.x;
+  const int per = (l * l + chunks - 1) / chunks;
   const int p0 = ch * per, p1 = min(l * l, p0 + per);
   double ring = 0, cnt = 0, s = 0, s2 = 0;
   for (int p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
@@ -78,7 +79,7 @@ __global__ void k_template_moments(const float2* __restrict__ T, int l, double* 
   s = block_sum(s);
   s2 = block_sum(s2);
   if (threadIdx.x == 0) {
-    double* o = part + ((long long)t * kChunks + ch) * 4;
+    double* o = part + ((long long)t * chunks + ch) * 4;
     o[0] = ring;
     o[1] = cnt;
     o[2] = s;
@@ -86,12 +87,13 @@ __global__ void k_template_moments(const float2* __restrict__ T, int l, double* 
   }
 }
 
-__global__ void k_template_moments_finish(const double* __restrict__ part, int n, double* __restrict__ mom) {
+__global__ void k_template_moments_finish(const double* __restrict__ part, int n, int chunks,
+                                          double* __restrict__ mom) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   double a[4] = {0, 0, 0, 0};
-  for (int ch = 0; ch < kChunks; ++ch)
-    for (int k = 0; k < 4; ++k) a[k] += part[((long long)t * kChunks + ch) * 4 + k];
+  for (int ch = 0; ch < chunks; ++ch)
+    for (int k = 0; k < 4; ++k) a[k] += part[((long long)t * chunks + ch) * 4 + k];
   for (int k = 0; k < 4; ++k) mom[4 * t + k] = a[k];
 }
 
@@ -153,7 +155,7 @@ __device__ __forceinline__ void radial_merge(const double* __restrict__ hist, in
 }
 
 // Radial power over the half plane x <= l/2 of each template spectrum (SQRSum_by_circle,
-// k:94-126); grid (kChunks, n), each workgroup a slice of rows (wave w takes rows w, w + 4, ...
+// k:94-126); grid (chunks, n), each workgroup a slice of rows (wave w takes rows w, w + 4, ...
 // of it), partials [t][chunk][bin].
 __global__ void k_radial_power_planes(const float2* __restrict__ F, int l, double* __restrict__ part) {
   extern __shared__ double hist[];
@@ -162,12 +164,12 @@ __global__ void k_radial_power_planes(const float2* __restrict__ F, int l, doubl
   for (int i = threadIdx.x; i < waves * nbins; i += blockDim.x) hist[i] = 0.0;
   __syncthreads();
   const float2* P = F + (long long)t * l * l;
-  const int per = (l + kChunks - 1) / kChunks;
+  const int per = (l + (int)gridDim.x - 1) / (int)gridDim.x;
   const int y0 = ch * per, y1 = min(l, y0 + per);
   for (int y = y0 + wave; y < y1; y += waves)
     radial_row(P + (long long)y * l, l / 2 + 1, min(y, l - y), nbins, hist + wave * nbins, lane);
   __syncthreads();
-  radial_merge(hist, waves, nbins, part + ((long long)t * kChunks + ch) * nbins);
+  radial_merge(hist, waves, nbins, part + ((long long)t * gridDim.x + ch) * nbins);
 }
 
 // ra[t][bin] = sum over chunks of part[t][chunk][bin], one wavefront per (t, bin): lane j adds
@@ -236,14 +238,13 @@ __global__ void k_mask_planes(float2* __restrict__ T, int n, int l, float rin, f
 }
 
 // compute_area_sum_ofSQR (k:319-382) + host mean (src/norm.cu:334-342); full [y][x] planes.
-// Partial sums over a slice of the region; grid (kAreaChunks, n), then k_area_power_finish.
-constexpr int kAreaChunks = 16;
+// Partial sums over a slice of the region; grid (chunks, n), then k_area_power_finish.
 
 __global__ void k_area_power_full(const float2* __restrict__ F, int l, double2* __restrict__ part) {
   const int t = blockIdx.y, ch = blockIdx.x;
   const float2* P = F + (long long)t * l * l;
   const int hx = l / 2 + 1;
-  const int per = (l * hx + kAreaChunks - 1) / kAreaChunks;
+  const int per = (l * hx + (int)gridDim.x - 1) / (int)gridDim.x;
   const int p0 = ch * per, p1 = min(l * hx, p0 + per);
   double s = 0, c = 0;
   for (int p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
@@ -257,7 +258,7 @@ __global__ void k_area_power_full(const float2* __restrict__ F, int l, double2* 
   }
   s = block_sum(s);
   c = block_sum(c);
-  if (threadIdx.x == 0) part[(long long)t * kAreaChunks + ch] = make_double2(s, c);
+  if (threadIdx.x == 0) part[(long long)t * gridDim.x + ch] = make_double2(s, c);
 }
 
 // Same over half spectra stored column-major [kx 0..l/2][ky].
@@ -265,7 +266,7 @@ __global__ void k_area_power_half(const float2* __restrict__ F, int l, double2* 
   const int t = blockIdx.y, ch = blockIdx.x;
   const int hx = l / 2 + 1;
   const float2* P = F + (long long)t * hx * l;
-  const int per = (l * hx + kAreaChunks - 1) / kAreaChunks;
+  const int per = (l * hx + (int)gridDim.x - 1) / (int)gridDim.x;
   const int p0 = ch * per, p1 = min(l * hx, p0 + per);
   double s = 0, c = 0;
   for (int p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
@@ -279,17 +280,17 @@ __global__ void k_area_power_half(const float2* __restrict__ F, int l, double2* 
   }
   s = block_sum(s);
   c = block_sum(c);
-  if (threadIdx.x == 0) part[(long long)t * kAreaChunks + ch] = make_double2(s, c);
+  if (threadIdx.x == 0) part[(long long)t * gridDim.x + ch] = make_double2(s, c);
 }
 
 // m = sqrt(sum / count^2) per plane from the chunk partials (host mean of src/norm.cu:334-342).
-__global__ void k_area_power_finish(const double2* __restrict__ part, int n, double* __restrict__ m) {
+__global__ void k_area_power_finish(const double2* __restrict__ part, int n, int chunks, double* __restrict__ m) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   double s = 0, c = 0;
-  for (int ch = 0; ch < kAreaChunks; ++ch) {
-    s += part[(long long)t * kAreaChunks + ch].x;
-    c += part[(long long)t * kAreaChunks + ch].y;
+  for (int ch = 0; ch < chunks; ++ch) {
+    s += part[(long long)t * chunks + ch].x;
+    c += part[(long long)t * chunks + ch].y;
   }
   m[t] = sqrt(s / (c * c));
 }
@@ -582,20 +583,23 @@ __global__ void k_rotate_complex_multi(const float2* __restrict__ in, int n, int
     res.x = P[b.k0].x * w0 + P[b.k1].x * w1 + P[b.k2].x * w2 + P[b.k3].x * w3;
     res.y = P[b.k0].y * w0 + P[b.k1].y * w1 + P[b.k2].y * w2 + P[b.k3].y * w3;
   }
-  out[i] = res;
+  st_stream(out + i, res);  // read once, by the next transform; keeps the source block in L2
 }
 
 }  // namespace
 
 // ---------------------------------------------------------------- launchers
+int plane_reduction_chunks(int n) { return n >= 64 ? 16 : std::min(128, std::max(16, 1024 / std::max(n, 1))); }
+
 void launch_pad_templates(const float* raw, int n, int w, int h, int l, float2* out, hipStream_t s) {
   k_pad_templates<<<nblk((long long)n * l * l), BS, 0, s>>>(raw, n, w, h, l, out);
   GS_CHECK_LAUNCH();
 }
 void launch_template_moments(const float2* T, int n, int l, double* part, double* mom, hipStream_t s) {
-  k_template_moments<<<dim3(kChunks, n), BS, 0, s>>>(T, l, part);
+  const int ch = plane_reduction_chunks(n);
+  k_template_moments<<<dim3(ch, n), BS, 0, s>>>(T, l, part);
   GS_CHECK_LAUNCH();
-  k_template_moments_finish<<<nblk(n), BS, 0, s>>>(part, n, mom);
+  k_template_moments_finish<<<nblk(n), BS, 0, s>>>(part, n, ch, mom);
   GS_CHECK_LAUNCH();
 }
 void launch_template_standardise(float2* T, int n, int l, const double* mom, hipStream_t s) {
@@ -605,9 +609,10 @@ void launch_template_standardise(float2* T, int n, int l, const double* mom, hip
 void launch_radial_power_planes(const float2* F, int n, int l, double* part, double* ra, hipStream_t s) {
   const int nbins = l / 2;
   GS_REQUIRE(4 * nbins * sizeof(double) <= 64 * 1024, kUnsupported, "window too large for radial histogram");
-  k_radial_power_planes<<<dim3(kChunks, n), BS, 4 * sizeof(double) * nbins, s>>>(F, l, part);
+  const int ch = plane_reduction_chunks(n);
+  k_radial_power_planes<<<dim3(ch, n), BS, 4 * sizeof(double) * nbins, s>>>(F, l, part);
   GS_CHECK_LAUNCH();
-  k_radial_power_finish<<<nblk((long long)n * nbins * 64), BS, 0, s>>>(part, n, nbins, kChunks, ra);
+  k_radial_power_finish<<<nblk((long long)n * nbins * 64), BS, 0, s>>>(part, n, nbins, ch, ra);
   GS_CHECK_LAUNCH();
 }
 void launch_whiten_planes(float2* F, int n, int l, const double* ra, const double* rb, hipStream_t s) {
@@ -625,15 +630,17 @@ void launch_mask_planes(float2* T, int n, int l, float d_m, float ehw, hipStream
   GS_CHECK_LAUNCH();
 }
 void launch_area_power_full(const float2* F, int n, int l, double2* part, double* m, hipStream_t s) {
-  k_area_power_full<<<dim3(kAreaChunks, n), BS, 0, s>>>(F, l, part);
+  const int ch = plane_reduction_chunks(n);
+  k_area_power_full<<<dim3(ch, n), BS, 0, s>>>(F, l, part);
   GS_CHECK_LAUNCH();
-  k_area_power_finish<<<nblk(n), BS, 0, s>>>(part, n, m);
+  k_area_power_finish<<<nblk(n), BS, 0, s>>>(part, n, ch, m);
   GS_CHECK_LAUNCH();
 }
 void launch_area_power_half(const float2* F, int n, int l, double2* part, double* m, hipStream_t s) {
-  k_area_power_half<<<dim3(kAreaChunks, n), BS, 0, s>>>(F, l, part);
+  const int ch = plane_reduction_chunks(n);
+  k_area_power_half<<<dim3(ch, n), BS, 0, s>>>(F, l, part);
   GS_CHECK_LAUNCH();
-  k_area_power_finish<<<nblk(n), BS, 0, s>>>(part, n, m);
+  k_area_power_finish<<<nblk(n), BS, 0, s>>>(part, n, ch, m);
   GS_CHECK_LAUNCH();
 }
 void launch_scale_planes(float2* F, int n, long long plane, const double* m, float extra, hipStream_t s) {
