@@ -321,6 +321,124 @@ __global__ void __launch_bounds__(CorrCfg<L>::CT)
   }
 }
 
+// Column pass looping over TL templates per workgroup (l = 320, the row-packed layout of variant
+// 8): workgroup (block, column group, chunk of TL templates).  Every plane's transform uses the
+// same 22 lane twiddles of the 4 * 4 * 20 plan (they depend only on the lane's place in its
+// column), so they are loaded once per workgroup into registers instead of being formed by
+// recurrence for every plane (about a quarter of the transform's VALU instructions); the block's
+// spectrum columns are read through L2 again for each template.  Arithmetic otherwise as
+// k_colpass<320, 1, 4>: product with the Nyquist column packed into column 0, radix 4 * 4 * 20
+// over the ColStageMaps layouts, real-to-complex pair packing in the write-out, streaming stores.
+// Logical order: tiles of BT blocks x TT template chunks per column group, XCD-contiguous.
+template <int L, int TL, int BT, int TT>
+__global__ void __launch_bounds__(CorrCfg<L>::CT)
+    k_colpass_tl(const float2* __restrict__ that, const float2* __restrict__ ihat,
+                 const float2* __restrict__ twL, float2* __restrict__ Z, int t0, int tc, int nb) {
+  using C = CorrCfg<L>;
+  using P = ct::RadixList<4, 4, 20>;
+  constexpr int H = C::H, COLS = C::COLS, NT = C::CT, TPC = C::TPC, LP = L;
+  static_assert(L == 320 && COLS == 16 && TPC == 16 && COLS * TPC == NT, "the 4*4*20 plan over 16 lanes");
+  constexpr int NLT = ct::lane_twiddle_count(P{});
+  constexpr long long PLANE = (long long)(H + 1) * L;
+  __shared__ float2 buf[COLS * LP];
+  auto cmask = [](int c) { return 2 * (c & 7); };
+  auto column = [](int cg, int c) {
+    const int j = 8 * cg + (c & 7);
+    return c < 8 ? j : (j == 0 ? H / 2 : H - j);
+  };
+  const unsigned G = gridDim.x, bid = blockIdx.x;
+  const unsigned xcd = bid % 8u, idx = bid / 8u, per = G / 8u, rem = G % 8u;
+  unsigned logical = (xcd < rem ? xcd * (per + 1u) : rem * (per + 1u) + (xcd - rem) * per) + idx;
+  const int ntc = (tc + TL - 1) / TL;  // template chunks
+  const unsigned ntb = (unsigned)(nb + BT - 1) / BT, ntt = (unsigned)(ntc + TT - 1) / TT;
+  const int bi = (int)(logical % BT);
+  logical /= BT;
+  const int ti = (int)(logical % TT);
+  logical /= TT;
+  const int tt = (int)(logical % ntt);
+  logical /= ntt;
+  const int btile = (int)(logical % ntb);
+  const int cg = (int)(logical / ntb);
+  const int b = btile * BT + bi;
+  const int tch = tt * TT + ti;
+  if (b >= nb || tch >= ntc) return;  // padding of the last tiles (uniform per workgroup)
+  const int tid = threadIdx.x;
+  constexpr int CPW = 64 / TPC;
+  const int wave = tid / 64, lane = tid % 64;
+  const int cw0 = wave * CPW;
+  constexpr int NLD = (CPW * L + 63) / 64;
+  constexpr int NDC = (L + 63) / 64;
+  static_assert((CPW * L) % 64 == 0, "whole loads");
+  const float2* Ip = ihat + (long long)b * PLANE;
+  const bool dc = (cg == 0 && cw0 == 0);
+  float2 wl[NLT];
+  ct::load_lane_twiddles<L, TPC>(wl, twL, tid % TPC, P{});
+  const int col = tid / TPC;
+  // packed write-out mapping (as k_colpass with LAY 4)
+  constexpr int RPP = NT / 8, NWP = L / RPP;
+  const int jl = tid % 8, y0 = tid / 8;
+  const int j = 8 * cg + jl;
+  const float2 w1 = c_conj(twL[j]);  // exp(+2 pi i j / L)
+  const int tl1 = min(tc, (tch + 1) * TL);
+  for (int tl = tch * TL; tl < tl1; ++tl) {
+    const float2* Tp = that + (long long)(t0 + tl) * PLANE;
+    float2 tv[NLD], iv[NLD];
+#pragma unroll
+    for (int i = 0; i < NLD; ++i) {
+      const int q = lane + 64 * i;
+      const int cl = q / L, ky = q - cl * L;
+      const long long e = (long long)column(cg, cw0 + cl) * L + ky;
+      tv[i] = Tp[e];
+      iv[i] = Ip[e];
+    }
+    float2 tn[NDC], in_[NDC];
+    if (dc) {
+#pragma unroll
+      for (int i = 0; i < NDC; ++i) {
+        const int ky = lane + 64 * i;
+        if (ky < L) {
+          tn[i] = Tp[(long long)H * L + ky];
+          in_[i] = Ip[(long long)H * L + ky];
+        }
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < NLD; ++i) {
+      const int q = lane + 64 * i;
+      const int cl = q / L, ky = q - cl * L;
+      float2 p = c_mulconj(iv[i], tv[i]);
+      if (i < NDC && dc && cl == 0) {
+        const float2 pn = c_mulconj(in_[i < NDC ? i : 0], tn[i < NDC ? i : 0]);
+        p = make_float2(p.x - pn.y, p.y + pn.x);
+      }
+      buf[(cw0 + cl) * LP + ky] = p;
+    }
+    ct::lds_sync<ct::Sync::WAVE>();
+    ct::run_stockham_maps<L, TPC, +1, ct::Sync::WAVE, ct::TwSrc::LANE>(buf + col * LP, wl, tid % TPC, P{},
+                                                                        ColStageMaps{cmask(col)});
+    __syncthreads();  // rows of Z gather all COLS columns of the workgroup
+    float4* const Zq = reinterpret_cast<float4*>(Z + ((long long)tl * nb + b) * (long long)L * H + 16 * cg) + jl;
+    float2 av[NWP], bv[NWP];
+#pragma unroll
+    for (int i = 0; i < NWP; ++i) {
+      av[i] = buf[jl * LP + ColStageMaps{cmask(jl)}.template at<3>(y0 + i * RPP)];
+      bv[i] = buf[(8 + jl) * LP + ColStageMaps{cmask(8 + jl)}.template at<3>(y0 + i * RPP)];
+    }
+#pragma unroll
+    for (int i = 0; i < NWP; ++i) {
+      const int y = y0 + i * RPP;
+      float2 z1, z2;
+      zpack2(av[i], bv[i], w1, z1, z2);
+      if (cg == 0 && jl == 0) {
+        z1 = make_float2(av[i].x + av[i].y, av[i].x - av[i].y);
+        z2 = make_float2(2.f * bv[i].x, -2.f * bv[i].y);
+      }
+      st_stream(Zq + (long long)(y < L / 2 ? y + L / 2 : y - L / 2) * (H / 2), make_float4(z1.x, z1.y, z2.x, z2.y));
+    }
+    __syncthreads();  // the next template's columns overwrite the buffer
+  }
+}
+
 // Column pass, persistent over the templates of the launch: one workgroup owns one (block,
 // column group) and loops over the tc templates, so the twiddle table is staged once and the
 // image columns stay cache-resident.  PF: the next template's columns are loaded while the
@@ -1309,6 +1427,17 @@ bool launch_colpass_plane(int L, const CorrArgs& a, int variant, hipStream_t s) 
 
 bool launch_colpass(int L, const CorrArgs& a, int variant, hipStream_t s) {
   TwTables& T = tw_tables(current_device());
+  if (L == 320 && variant == 9) {  // 8 templates per workgroup, lane twiddles in registers
+    constexpr int TL = 8, BT = 8, TT = 8;
+    const long long ntc = (a.tc + TL - 1) / TL;
+    const long long G = (long long)(CorrCfg<320>::H / CorrCfg<320>::COLS) * ((ntc + TT - 1) / TT * TT) *
+                        ((a.nb + BT - 1) / BT * BT);
+    GS_REQUIRE(G > 0 && G < (1LL << 31), kInvalid, "column pass grid out of range");
+    k_colpass_tl<320, TL, BT, TT><<<(unsigned)G, CorrCfg<320>::CT, 0, s>>>(a.that, a.ihat, T.t320, a.Z, a.t0, a.tc,
+                                                                         a.nb);
+    GS_CHECK_LAUNCH();
+    return true;
+  }
   if (L == 320) {
     colpass_impl<320>(a, T.t320, variant, s);
     return true;

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <functional>
 #include <cmath>
 #include <cstdio>
@@ -335,12 +336,32 @@ class Engine {
     auto stage = [this, items](int i) {
       stage_image(items[i].image, items[i].width, items[i].height, pinned_[i % 2].p);
     };
+    // Capturing the angle loop as a graph costs more than one eager run of it (about 0.14 s at
+    // these sizes) and pays off only when the shape comes again: capture only shapes (micrograph
+    // size, template count and size) that occur at least twice in the batch.
+    std::vector<int> shape_count(n_items, 0);
+    {
+      std::map<std::array<long long, 4>, int> seen;
+      std::vector<std::array<long long, 4>> keys(n_items);
+      long long tn = n_, tw = w_, th = h_;
+      for (int i = 0; i < n_items; ++i) {
+        if (items[i].templates) tn = items[i].n_templates, tw = items[i].template_width, th = items[i].template_height;
+        keys[i] = {(long long)items[i].width * 100000 + items[i].height, tn, tw, th};
+        ++seen[keys[i]];
+      }
+      for (int i = 0; i < n_items; ++i) shape_count[i] = seen[keys[i]];
+    }
+    struct GraphGate {
+      bool& flag;
+      ~GraphGate() { flag = true; }
+    } gate{graph_allowed_};
     std::future<void> next = std::async(std::launch::async, stage, 0);
     all.clear();
     try {
       for (int i = 0; i < n_items; ++i) {
         next.get();  // micrograph i is staged (rethrows a staging failure)
         const gs_batch_item& it = items[i];
+        graph_allowed_ = shape_count[i] >= 2;
         if (it.templates)
           load_templates(it.templates, it.n_templates, it.template_width, it.template_height, it.alt_az);
         upload_image(pinned_[i % 2].p, it.width, it.height, it.ctf);
@@ -466,7 +487,7 @@ class Engine {
   }
   void set_chunk(int tc) { chunk_ = tc; }
   void set_variant(int kernel, int variant) {
-    GS_REQUIRE((kernel == 0 && variant >= 0 && variant <= 8) || (kernel == 1 && variant >= 0 && variant <= 7) ||
+    GS_REQUIRE((kernel == 0 && variant >= 0 && variant <= 9) || (kernel == 1 && variant >= 0 && variant <= 7) ||
                    (kernel == 2 && (variant == 0 || variant == 1)),
                kInvalid, "unknown kernel variant");
     if (kernel == 0)
@@ -1007,7 +1028,7 @@ class Engine {
     }
     // column variant 8 writes the row-packed layout that only row variant 7 reads (l = 320; other
     // sizes use one unpacked layout)
-    GS_REQUIRE(l != 320 || (col_variant_ == 8) == (row_variant_ == 7), kInvalid,
+    GS_REQUIRE(l != 320 || (col_variant_ == 8 || col_variant_ == 9) == (row_variant_ == 7), kInvalid,
                "column variant 8 (row-packed inter-pass layout) and row variant 7 go together");
     float2* Zb = d_Z_.ensure(zsz * (overlap ? 2 : 1));
     double* part = d_part_.ensure((size_t)n * rotation_groups(l) * 2);
@@ -1062,7 +1083,7 @@ class Engine {
         }
       }
     };
-    if (use_graph_ && !overlap && !profile_ && !gen) {
+    if (use_graph_ && graph_allowed_ && !overlap && !profile_ && !gen) {
       // The angle loop has no micrograph-dependent arguments: capture it once per problem shape
       // and buffer set, then replay it for every later micrograph of that shape.
       auto ptr = [](const void* p) { return (long long)(uintptr_t)p; };
@@ -1280,7 +1301,7 @@ class Engine {
             clock_end(clk_col_, stream_);
           } else {
             // the norm_type=0 row pass reads the unpacked layout: variant 8 runs as 7 here
-            GS_REQUIRE(launch_colpass_plane(l, ca, col_variant_ == 8 ? 7 : col_variant_, stream_), kUnsupported,
+            GS_REQUIRE(launch_colpass_plane(l, ca, col_variant_ >= 8 ? 7 : col_variant_, stream_), kUnsupported,
                        "window_size not supported");
             clock_end(clk_col_, stream_);
             clock_begin(clk_row_, stream_);
@@ -1367,6 +1388,7 @@ class Engine {
   std::vector<float> angles_;
   // kernel variants default to the fastest measured on the production shape (l = 320)
   int lim_angles_ = 0, lim_templates_ = 0, chunk_ = 0, row_variant_ = 7, col_variant_ = 8;
+  bool graph_allowed_ = true;  // false while a batch runs an item whose shape occurs only once
   double mem_budget_ = 0;  // bytes the automatic templates-per-launch may give the inter-pass buffer
   double total_ms_ = 0, corr_ms_ = 0;
   std::vector<gs_record> records_;

@@ -30,6 +30,7 @@ def _key(a):
     (0, 0, 0, 0), (1, 0, 0, 0), (2, 0, 0, 0), (3, 0, 0, 0), (4, 0, 0, 0), (5, 0, 0, 0), (6, 0, 0, 0),
     (7, 6, 0, 0),
     (8, 7, 0, 0), (8, 7, 4, 0),  # row-packed inter-pass layout (also with a partial last launch)
+    (9, 7, 0, 0), (9, 7, 4, 0),  # the same with 8 templates per workgroup (one partial chunk)
     (0, 1, 0, 0), (0, 2, 0, 0), (0, 3, 0, 0), (0, 4, 0, 0), (0, 5, 0, 0), (0, 6, 0, 0),
     (3, 6, 4, 0),   # eight templates per iteration with a partial last iteration (4 + 2 templates)
     (3, 1, 4, 0),   # two launches per angle, the second partial (4 + 2 templates)
@@ -58,12 +59,12 @@ def test_variant_matches_oracle(gs, case, col, row, chunk, overlap):
 
 
 def test_packed_layout_needs_its_row_pass(gs, case):
-    """Column variant 8 writes the row-packed layout that only row variant 7 reads; a mismatched
+    """Column variants 8 and 9 write the row-packed layout that only row variant 7 reads; a mismatched
     pair is rejected before any kernel runs."""
     cfg, ds, _, _ = case
     eng = gs.Engine(cfg)
     eng.load_templates(ds.templates)
-    for col, row in ((8, 6), (7, 7)):
+    for col, row in ((8, 6), (9, 6), (7, 7)):
         eng.set_variant(0, col)
         eng.set_variant(1, row)
         with pytest.raises(gs.GisspaError, match="go together"):
