@@ -842,26 +842,34 @@ class Engine {
   }
 
   // Whole-frame (or tile) preprocessing: src/norm.cu:356-402 / src/nonorm.cu:346-389.
-  void prep_frame(const float* in, int nx, int ny, bool dc_zero, float* out) {
+  // Whole-frame preparation of nf complex frames of nx x ny in place (src/norm.cu:356-402 with
+  // dc_zero, src/nonorm.cu:346-389 per tile): forward transform, phase flip, radial power,
+  // whitening and filter, inverse transform.  dc_zero needs nf = 1.
+  void prep_frames(float2* F, int nf, int nx, int ny, bool dc_zero) {
     const size_t N = (size_t)nx * ny;
-    float2* F = d_frame_.ensure(N);
-    float2* S = d_frame2_.ensure(N);
-    launch_real_to_complex(in, F, (long long)N, 1.f, stream_);
+    GS_REQUIRE(!dc_zero || nf == 1, kInvalid, "dc_zero applies to one frame");
     // square frames of a built size (the 720 x 720 tiles of norm_type=0) take the LDS transform
     auto fft2 = [&](int sign, float scale) {
-      if (nx != ny || !launch_plane_fft2(F, 1, nx, sign, scale, stream_))
-        fft2_batched(fft_, F, S, 1, ny, nx, sign, scale, stream_);
+      if (nx != ny || !launch_plane_fft2(F, nf, nx, sign, scale, stream_))
+        fft2_batched(fft_, F, d_frame2_.ensure(N * nf), nf, ny, nx, sign, scale, stream_);
     };
     fft2(-1, (float)(1.0 / (double)N));
     set_image_band(nx, ny);
-    launch_phase_flip(F, nx, ny, dp_, stream_);
+    launch_phase_flip(F, nx, ny, nf, dp_, stream_);
     const int nbins = std::max(nx, ny) / 2;
-    double* ra = d_ra_img_.ensure(nbins);
-    double* part = d_chpart_.ensure((size_t)radial_image_groups() * nbins);
-    launch_radial_power_image(F, nx, ny, part, ra, stream_);
-    launch_whiten_filter_image(F, nx, ny, ra, radial_counts_dev(nx, ny), dp_, stream_);
+    double* ra = d_ra_img_.ensure((size_t)nf * nbins);
+    double* part = d_chpart_.ensure((size_t)nf * radial_image_groups(nf) * nbins);
+    launch_radial_power_image(F, nx, ny, nf, part, ra, stream_);
+    launch_whiten_filter_image(F, nx, ny, nf, ra, radial_counts_dev(nx, ny), dp_, stream_);
     if (dc_zero) launch_zero_dc(F, stream_);
     fft2(+1, 1.f);
+  }
+
+  void prep_frame(const float* in, int nx, int ny, bool dc_zero, float* out) {
+    const size_t N = (size_t)nx * ny;
+    float2* F = d_frame_.ensure(N);
+    launch_real_to_complex(in, F, (long long)N, 1.f, stream_);
+    prep_frames(F, 1, nx, ny, dc_zero);
     launch_extract_real(F, out, (long long)N, stream_);
   }
 
@@ -1053,7 +1061,7 @@ class Engine {
         // one forward transform per pair; the split, the standardisation (affine: applied to the
         // spectra) and the 1/l^2 scale follow in one pass that writes the half spectra
         fft2_planes(work, tscr, (nt + 1) / 2, -1, 1.f);
-        launch_unpack_pair_spectra(work, nt, l, stats, (float)(1.0 / (double)L2), that, stream_);
+        launch_unpack_pair_spectra(work, nt, 1, l, stats, (float)(1.0 / (double)L2), that, stream_);
         // correlation + per-pixel statistics (ComputeCCGSum / PickParticles in one pass)
         for (int t0 = 0; t0 < nt; t0 += tc_max, ++chunk) {
           const int tc = std::min(tc_max, nt - t0);
@@ -1153,7 +1161,7 @@ class Engine {
       launch_rotate_mask_pairs(d_tprep_.p, nt, l, cose, sine, P_.diameter, 4.0f, work, part, stream_);
       launch_rotation_stats(part, nt, l, stats, stream_);
       fft2_planes(work, tscr, (nt + 1) / 2, -1, 1.f);
-      launch_unpack_pair_spectra(work, nt, l, stats, (float)(1.0 / (double)L2), that, stream_);
+      launch_unpack_pair_spectra(work, nt, 1, l, stats, (float)(1.0 / (double)L2), that, stream_);
       for (int t0 = 0; t0 < nt; t0 += tc2) {
         const CorrArgs ca{that, ihat, P2, t0, std::min(tc2, nt - t0), nb_};
         launch_corr_generic_records(fft_, ca, l, P2, W2, d_mu_.p, d_sigma_.p, a, P_.threshold, doffx, doffy, bx_,
@@ -1227,7 +1235,7 @@ class Engine {
   // norm_type = 0 (SearchNoNorm::work, src/nonorm.cu:559-589).
   void run_nonorm() {
     const int l = P_.window_size, hx = l / 2 + 1, n = n_;
-    const size_t L2 = (size_t)l * l;
+    const size_t L2 = (size_t)l * l, T2 = (size_t)kTile * kTile;
     prep_templates();  // -> d_that_ [n][hx][l]
     ulx_.clear();
     uly_.clear();
@@ -1235,10 +1243,18 @@ class Engine {
     const int ntiles = (int)ulx_.size();
     nbk_ = (int)block_offsets(kTile, l, P_.overlap).size();
     const int nbb = nbk_ * nbk_;
-    std::vector<int> off(nbk_);
-    for (int i = 0; i < nbk_; ++i) off[i] = i * (l - P_.overlap);  // split_IMG (k:615-620)
-    int* doff = d_offx_.ensure(nbk_);
-    GS_HIP(hipMemcpyAsync(doff, off.data(), sizeof(int) * nbk_, hipMemcpyHostToDevice, stream_));
+    // Every tile is searched at once: the blocks of all tiles (g = tile * nbb + block) are one
+    // batch for the block normalisation, the rotation and the correlation launches.
+    const int B = ntiles * nbb;
+    std::vector<int> geo(nbk_ + 2 * ntiles);
+    for (int i = 0; i < nbk_; ++i) geo[i] = i * (l - P_.overlap);  // split_IMG (k:615-620)
+    for (int k = 0; k < ntiles; ++k) {
+      geo[nbk_ + 2 * k] = ulx_[k];
+      geo[nbk_ + 2 * k + 1] = uly_[k];
+    }
+    int* doff = d_tilegeo_.ensure(geo.size());
+    const int* dul = doff + nbk_;
+    GS_HIP(hipMemcpyAsync(doff, geo.data(), sizeof(int) * geo.size(), hipMemcpyHostToDevice, stream_));
     auto angles = angle_list(P_.phistep);
     int na = (int)angles.size();
     if (lim_angles_ > 0) na = std::min(na, lim_angles_);
@@ -1248,22 +1264,25 @@ class Engine {
     const int groups = gen ? 1 : rowpass_groups(l);
     if (gen) corr_generic_prepare(fft_, l);
     // Angles are processed in groups: the rotated blocks of ag angles form the "block" axis of
-    // one correlation launch (ag * nbb planes per template), so small tiles still fill the GPU.
-    const int ag_max = std::max(1, std::min(na, 2048 / std::max(nbb, 1)));
-    const int nbp_max = ag_max * nbb;
+    // one correlation launch (ag * B planes per template), so small micrographs still fill the
+    // GPU.  Groups hold an even number of angles so that the rotation's angle pairs never
+    // straddle two groups.
+    int ag_max = std::max(1, std::min(na, 2048 / std::max(B, 1)));
+    if (ag_max < na && ag_max > 1) ag_max &= ~1;
+    const int nbp_max = ag_max * B;
     const int tc_max = std::min(n, chunk_for(nbp_max));
     const size_t zsz = gen ? 2 * corr_generic_buffer(l, tc_max, nbp_max) : (size_t)tc_max * nbp_max * l * (l / 2);
     float2* Z = d_Z_.ensure(zsz);
     float4* part = reinterpret_cast<float4*>(d_plane_part_.ensure((size_t)tc_max * nbp_max * groups * 4));
-    // plane records, per tile [t][angle][b][4]
-    const size_t per_tile = (size_t)n * na * nbb * 4;
-    double* planes = d_planes_.ensure((size_t)ntiles * per_tile);
-    float* tile = d_tile_.ensure((size_t)kTile * kTile);
-    float* tpre = d_imgpre_.ensure((size_t)kTile * kTile);
-    float2* blk = d_blocks_.ensure((size_t)nbb * L2);
-    float2* rotb = d_work2_.ensure((size_t)nbp_max * L2);
+    // plane records [template][angle][tile * nbb + block][4]
+    double* planes = d_planes_.ensure((size_t)n * na * B * 4);
+    float2* F = d_frame_.ensure((size_t)ntiles * T2);
+    float2* blk = d_blocks_.ensure((size_t)B * L2);
+    const size_t npair_max = (size_t)(ag_max + 1) / 2 * B;
+    float2* rotb = d_work2_.ensure(npair_max * L2);
     float2* ihat = d_ihat_.ensure((size_t)nbp_max * hx * l);
-    double* m = d_m_.ensure(std::max(n, nbb));
+    float2* S = d_scratch_.ensure(std::max((size_t)B, npair_max) * L2);
+    double* m = d_m_.ensure(std::max(n, B));
     std::vector<float2> cs(na);
     for (int a = 0; a < na; ++a)
       cs[a] = make_float2((float)std::cos((double)angles[a] * kRefPi / 180),
@@ -1272,68 +1291,64 @@ class Engine {
     GS_HIP(hipMemcpyAsync(dcs, cs.data(), sizeof(float2) * na, hipMemcpyHostToDevice, stream_));
     const float border = P_.diameter / 4;
     GS_HIP(hipEventRecord(ev_corr0_, stream_));
-    for (int tI = 0; tI < ntiles; ++tI) {
-      launch_extract_tile(d_img_.p, W_, ulx_[tI], uly_[tI], kTile, P_.invert ? -1.f : 1.f, tile, stream_);
-      const float* src = tile;
-      if (P_.phase_flip == 1) {
-        prep_frame(tile, kTile, kTile, false, tpre);
-        src = tpre;
-      }
-      launch_split_blocks(src, kTile, kTile, l, doff, doff, nbk_, nbb, blk, stream_);
-      if (P_.phase_flip == 1) {  // src/nonorm.cu:405-447
-        float2* S = d_scratch_.ensure((size_t)nbb * L2);
-        fft2_planes(blk, S, nbb, -1, (float)(1.0 / (double)L2));
-        launch_area_power_full(blk, nbb, l, d_apart_.ensure((size_t)nbb * plane_reduction_chunks(nbb)), m, stream_);
-        launch_scale_planes(blk, nbb, (long long)L2, m, 1.f, stream_);
-        fft2_planes(blk, S, nbb, +1, 1.f);
-      }
-      for (int a0 = 0; a0 < na; a0 += ag_max) {
-        const int ag = std::min(ag_max, na - a0), nbp = ag * nbb;
-        // rotate_subIMG of every block for angles a0..a0+ag (k:534-603): planes [angle][block]
-        launch_rotate_complex_multi(blk, nbb, l, dcs + a0, ag, rotb, stream_);
-        half_spectra(rotb, nbp, ihat, (float)(1.0 / (double)L2));
-        for (int t0 = 0; t0 < n; t0 += tc_max) {
-          const int tc = std::min(tc_max, n - t0);
-          CorrArgs ca{d_that_.p, ihat, Z, t0, tc, nbp};
-          clock_begin(clk_col_, stream_);
-          if (gen) {
-            launch_corr_generic_plane(fft_, ca, l, Z, Z + zsz / 2, part, border, stream_);
-            clock_end(clk_col_, stream_);
-          } else {
-            // the norm_type=0 row pass reads the unpacked layout: variant 8 runs as 7 here
-            GS_REQUIRE(launch_colpass_plane(l, ca, col_variant_ >= 8 ? 7 : col_variant_, stream_), kUnsupported,
-                       "window_size not supported");
-            clock_end(clk_col_, stream_);
-            clock_begin(clk_row_, stream_);
-            launch_rowpass_plane(l, Z, tc, nbp, part, border, stream_);
-            clock_end(clk_row_, stream_);
-          }
-          // record of (template t0+t, angle a0+q, block b) at tile base + ((t0+t)*na + a0)*nbb + q*nbb + b
-          launch_finalize_plane(part, tc, nbp, groups, (long long)na * nbb,
-                                planes + tI * per_tile + ((size_t)t0 * na + a0) * nbb * 4, stream_);
+    // TileImages (src/utils/tileimages.cpp:66-71), the per-tile preparation (src/nonorm.cu:346-389)
+    // and SplitImage (src/nonorm.cu:391-450) of every tile
+    launch_extract_tiles(d_img_.p, W_, dul, kTile, ntiles, P_.invert ? -1.f : 1.f, F, stream_);
+    if (P_.phase_flip == 1) prep_frames(F, ntiles, kTile, kTile, false);
+    launch_split_tile_blocks(F, kTile, l, doff, nbk_, ntiles, blk, stream_);
+    if (P_.phase_flip == 1) {  // src/nonorm.cu:405-447
+      fft2_planes(blk, S, B, -1, (float)(1.0 / (double)L2));
+      launch_area_power_full(blk, B, l, d_apart_.ensure((size_t)B * plane_reduction_chunks(B)), m, stream_);
+      launch_scale_planes(blk, B, (long long)L2, m, 1.f, stream_);
+      fft2_planes(blk, S, B, +1, 1.f);
+    }
+    for (int a0 = 0; a0 < na; a0 += ag_max) {
+      const int ag = std::min(ag_max, na - a0), nbp = ag * B, np2 = (ag + 1) / 2 * B;
+      // rotate_subIMG of every block for angles a0..a0+ag (k:534-603), two angles per plane, and
+      // the forward transforms: half spectra [angle][block] scaled by 1/l^2 (src/nonorm.cu:457-470)
+      launch_rotate_real_pairs(blk, B, l, dcs + a0, ag, rotb, stream_);
+      fft2_planes(rotb, S, np2, -1, 1.f);
+      launch_unpack_pair_spectra(rotb, ag, B, l, nullptr, (float)(1.0 / (double)L2), ihat, stream_);
+      for (int t0 = 0; t0 < n; t0 += tc_max) {
+        const int tc = std::min(tc_max, n - t0);
+        CorrArgs ca{d_that_.p, ihat, Z, t0, tc, nbp};
+        clock_begin(clk_col_, stream_);
+        if (gen) {
+          launch_corr_generic_plane(fft_, ca, l, Z, Z + zsz / 2, part, border, stream_);
+          clock_end(clk_col_, stream_);
+        } else {
+          // the norm_type=0 row pass reads the unpacked layout: variant 8 runs as 7 here
+          GS_REQUIRE(launch_colpass_plane(l, ca, col_variant_ >= 8 ? 7 : col_variant_, stream_), kUnsupported,
+                     "window_size not supported");
+          clock_end(clk_col_, stream_);
+          clock_begin(clk_row_, stream_);
+          launch_rowpass_plane(l, Z, tc, nbp, part, border, stream_);
+          clock_end(clk_row_, stream_);
         }
+        // record of (template t0+t, angle a0+q, block g) at ((t0+t)*na + a0 + q)*B + g
+        launch_finalize_plane(part, tc, nbp, groups, (long long)na * B,
+                              planes + ((size_t)t0 * na + a0) * B * 4, stream_);
       }
     }
     GS_HIP(hipEventRecord(ev_corr1_, stream_));
-    planes_host_.resize((size_t)ntiles * per_tile);
+    planes_host_.resize((size_t)n * na * B * 4);
   }
 
   void collect_nonorm() {
     GS_HIP(hipMemcpy(planes_host_.data(), d_planes_.p, sizeof(double) * planes_host_.size(),
                      hipMemcpyDeviceToHost));
-    const int l = P_.window_size, n = n_, nbb = nbk_ * nbk_;
-    const size_t per_tile = (size_t)n * na_ * nbb * 4;
+    const int l = P_.window_size, n = n_, nbb = nbk_ * nbk_, B = (int)ulx_.size() * nbb;
     const double rc = (double)l * l - 1;
     for (int tI = 0; tI < (int)ulx_.size(); ++tI)
       for (int a = 0; a < na_; ++a) {
         const double e = angles_[a];
         const double ce = std::cos(e * kRefPi / 180), se = std::sin(e * kRefPi / 180);
-        const double* P = planes_host_.data() + tI * per_tile;
+        const double* P = planes_host_.data();
         for (int t = 0; t < n; ++t) {
           double sc = 0, cxs = 0, cys = 0;
           int pix = 0, blkid = 0;
           for (int b = 0; b < nbb; ++b) {  // PickParticles host scoring, src/nonorm.cu:509-534
-            const double* q = P + (((size_t)t * na_ + a) * nbb + b) * 4;
+            const double* q = P + (((size_t)t * na_ + a) * B + tI * nbb + b) * 4;
             const double peak = q[0], pos = q[1], sum = q[2], sum2 = q[3];
             const double ra = sum - peak, rb = sum2 - peak * peak;
             const double sd = std::sqrt(rb / rc - (ra / rc) * (ra / rc));
@@ -1411,12 +1426,12 @@ class Engine {
 
   DevBuf<float> d_rmask_;
   DevBuf<float2> d_rph_;
-  DevBuf<float> d_raw_, d_img_, d_imgpre_, d_tprep_, d_stats_, d_best_, d_tile_;
+  DevBuf<float> d_raw_, d_img_, d_imgpre_, d_tprep_, d_stats_, d_best_;
   DevBuf<float2> d_work_, d_work2_, d_scratch_, d_frame_, d_frame2_, d_blocks_, d_ihat_, d_that_, d_Z_, d_cs_;
   DevBuf<double> d_mom_, d_ra_, d_ra_img_, d_m_, d_part_, d_S_, d_SS_, d_mu_, d_sigma_, d_planes_, d_chpart_;
   DevBuf<float> d_plane_part_;
   DevBuf<int32_t> d_arg_;
-  DevBuf<int> d_offx_, d_offy_, d_count_;
+  DevBuf<int> d_offx_, d_offy_, d_count_, d_tilegeo_;
   DevBuf<Cand> d_cand_;
   static constexpr int kMaxRefRecords = 1 << 22;
   DevBuf<RefRec> d_refrec_;

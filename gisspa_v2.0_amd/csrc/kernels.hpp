@@ -55,19 +55,26 @@ void launch_extract_real(const float2* F, float* out, long long count, hipStream
 
 // ---- whole-frame preparation (src/norm.cu:356-402, src/nonorm.cu:346-389) ----
 void launch_real_to_complex(const float* in, float2* out, long long count, float sign, hipStream_t s);
-void launch_phase_flip(float2* F, int nx, int ny, const DevParams& p, hipStream_t s);
-// part: radial_image_groups() * (max(nx, ny) / 2) doubles of scratch.
-int radial_image_groups();
-void launch_radial_power_image(const float2* F, int nx, int ny, double* part, double* ra, hipStream_t s);
-void launch_whiten_filter_image(float2* F, int nx, int ny, const double* ra, const double* rb,
+// The frame functions take nf frames of nx x ny, contiguous; ra holds max(nx, ny) / 2 bins per
+// frame and part is scratch of nf * radial_image_groups(nf) * (max(nx, ny) / 2) doubles.
+void launch_phase_flip(float2* F, int nx, int ny, int nf, const DevParams& p, hipStream_t s);
+int radial_image_groups(int nf);
+void launch_radial_power_image(const float2* F, int nx, int ny, int nf, double* part, double* ra, hipStream_t s);
+void launch_whiten_filter_image(float2* F, int nx, int ny, int nf, const double* ra, const double* rb,
                                 const DevParams& p, hipStream_t s);
 void launch_zero_dc(float2* F, hipStream_t s);
-void launch_extract_tile(const float* img, int W, int ulx, int uly, int tile, float sign,
-                         float* out, hipStream_t s);
+// ntiles tiles of tile x tile (upper-left corners ul[2k], ul[2k + 1], device array) as the real
+// parts of complex frames, times sign.
+void launch_extract_tiles(const float* img, int W, const int* ul, int tile, int ntiles, float sign, float2* out,
+                          hipStream_t s);
 
 // ---- blocks (src/norm.cu:404-477, src/nonorm.cu:391-450) ----
 void launch_split_blocks(const float* img, int nx, int ny, int l, const int* offx, const int* offy,
                          int bx, int nb, float2* out, hipStream_t s);
+// Blocks of every tile from the real parts of complex tiles: out[k * nbk^2 + b], block b at
+// (off[b % nbk], off[b / nbk]) of tile k.
+void launch_split_tile_blocks(const float2* F, int tile, int l, const int* off, int nbk, int ntiles, float2* out,
+                              hipStream_t s);
 
 // ---- per-angle rotation (src/norm.cu:479-521, src/nonorm.cu:457) ----
 // Template pairs (2p, 2p+1) rotated and masked into the real and imaginary parts of out[p]
@@ -76,14 +83,14 @@ int rotation_groups(int l);
 void launch_rotate_mask_pairs(const float* tp, int n, int l, float cose, float sine, float d_m, float ehw,
                               float2* out, double* partial, hipStream_t s);
 void launch_rotation_stats(const double* partial, int n, int l, float* stats, hipStream_t s);
-// Half spectra out[t][kx 0..l/2][ky] (standardised, times scale) of the pair transforms Z.
-void launch_unpack_pair_spectra(const float2* Z, int n, int l, const float* stats, float scale, float2* out,
-                                hipStream_t s);
-void launch_rotate_complex(const float2* in, int n, int l, float cose, float sine, float2* out,
-                           hipStream_t s);
-// na angles at once: out[a][t] = rotation of in[t] by cs[a] = (cos, sin) (device array).
-void launch_rotate_complex_multi(const float2* in, int n, int l, const float2* cs, int na, float2* out,
-                                 hipStream_t s);
+// Half spectra out[i * stride + g][kx 0..l/2][ky] (times scale; standardised when stats is not
+// null) of items i < n from the pair transforms Z[j * stride + g] holding items 2j and 2j + 1.
+void launch_unpack_pair_spectra(const float2* Z, int n, int stride, int l, const float* stats, float scale,
+                                float2* out, hipStream_t s);
+// Rotated copies of nb real blocks (real parts of in) for na angles cs[a] = (cos, sin) (device
+// array), two angles per complex plane: out[j * nb + g] = rot(in[g], cs[2j]) + i rot(in[g], cs[2j+1]).
+void launch_rotate_real_pairs(const float2* in, int nb, int l, const float2* cs, int na, float2* out,
+                              hipStream_t s);
 
 // ---- correlation (north-star kernels b, c, d, e) ----
 struct CorrArgs {

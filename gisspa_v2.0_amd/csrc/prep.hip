@@ -318,37 +318,43 @@ __global__ void k_real_to_complex(const float* __restrict__ in, float2* __restri
   if (i < count) out[i] = make_float2(sign * in[i], 0.f);
 }
 
-// do_phase_flip (k:828-839)
-__global__ void k_phase_flip(float2* __restrict__ F, int nx, int ny, DevParams p) {
+// do_phase_flip (k:828-839) on nf frames of nx x ny.
+__global__ void k_phase_flip(float2* __restrict__ F, int nx, int ny, int nf, DevParams p) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (long long)nx * ny) return;
-  const int x = (int)(i % nx), y = (int)(i / nx);
+  const long long N = (long long)nx * ny;
+  if (i >= N * nf) return;
+  const int q = (int)(i % N);
+  const int x = q % nx, y = q / nx;
   const float v = (float)ctf_ast(x, (y + ny / 2) % ny, nx, ny, p, 0);
   F[i] = make_float2(F[i].x * v, F[i].y * v);
 }
 
-// SQRSum_by_circle mode 1 (k:94-126) over the whole frame: workgroup g takes a slice of rows
-// (its waves interleaved), partials [g][bin]; k_radial_power_finish sums them in order.
+// SQRSum_by_circle mode 1 (k:94-126) over whole frames: grid (groups, frames), workgroup g of
+// frame f takes a slice of its rows (its waves interleaved), partials [f][g][bin];
+// k_radial_power_finish sums them in order.
 __global__ void k_radial_power_image(const float2* __restrict__ F, int nx, int ny, double* __restrict__ part) {
   extern __shared__ double hist[];
   const int nbins = max(nx, ny) / 2;
   const int waves = blockDim.x / 64, wave = threadIdx.x / 64, lane = threadIdx.x % 64;
   for (int i = threadIdx.x; i < waves * nbins; i += blockDim.x) hist[i] = 0.0;
   __syncthreads();
+  const float2* P = F + (long long)blockIdx.y * nx * ny;
   const int per = (ny + gridDim.x - 1) / gridDim.x;
   const int y0 = blockIdx.x * per, y1 = min(ny, y0 + per);
   for (int y = y0 + wave; y < y1; y += waves)
-    radial_row(F + (long long)y * nx, nx / 2 + 1, min(y, ny - y), nbins, hist + wave * nbins, lane);
+    radial_row(P + (long long)y * nx, nx / 2 + 1, min(y, ny - y), nbins, hist + wave * nbins, lane);
   __syncthreads();
-  radial_merge(hist, waves, nbins, part + (long long)blockIdx.x * nbins);
+  radial_merge(hist, waves, nbins, part + ((long long)blockIdx.y * gridDim.x + blockIdx.x) * nbins);
 }
 
-// whiten_filter_weight_Img (k:151-202)
-__global__ void k_whiten_filter_image(float2* __restrict__ F, int nx, int ny, const double* __restrict__ ra,
+// whiten_filter_weight_Img (k:151-202) on nf frames; ra holds max(nx, ny) / 2 bins per frame.
+__global__ void k_whiten_filter_image(float2* __restrict__ F, int nx, int ny, int nf, const double* __restrict__ ra,
                                       const double* __restrict__ rb, DevParams p) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (long long)nx * ny) return;
-  const int x = (int)(i % nx), y = (int)(i / nx);
+  const long long N = (long long)nx * ny;
+  if (i >= N * nf) return;
+  const int fr = (int)(i / N), q = (int)(i - (long long)fr * N);
+  const int x = q % nx, y = q / nx;
   const int dxi = min(x, nx - x), dyi = min(y, ny - y);
   const double dx = dxi, dy = dyi;
   const int r = rbin(dxi, dyi);
@@ -360,7 +366,7 @@ __global__ void k_whiten_filter_image(float2* __restrict__ F, int nx, int ny, co
     v = ctf_ast(x, (y + ny / 2) % ny, nx, ny, p, 2);
     signal = exp(p.bf * ss * ss + p.bf2 * ss + p.bf3);
     Ncurve = exp(p.a * ss * ss + p.b * ss + p.b2);
-    f *= sqrt((signal * v * v + Ncurve) / signal) / sqrt(ra[r] / rb[r]);
+    f *= sqrt((signal * v * v + Ncurve) / signal) / sqrt(ra[(long long)fr * (l / 2) + r] / rb[r]);
     if (r > p.hi_thr) f *= exp(-100 * ss * ss);
   }
   f *= band_image(r, p.hr_i, p.lr_i);
@@ -376,13 +382,16 @@ __global__ void k_zero_dc(float2* F) {
   if (threadIdx.x == 0 && blockIdx.x == 0) F[0] = make_float2(0.f, 0.f);
 }
 
-// Tile copy of TileImages (src/utils/tileimages.cpp:66-71) with optional inversion.
-__global__ void k_extract_tile(const float* __restrict__ img, int W, int ulx, int uly, int tile,
-                               float sign, float* __restrict__ out) {
+// Tile copies of TileImages (src/utils/tileimages.cpp:66-71) with optional inversion, as the
+// real parts of complex tiles out[k] (k < ntiles) whose upper-left corners are ul[2k], ul[2k+1].
+__global__ void k_extract_tiles(const float* __restrict__ img, int W, const int* __restrict__ ul, int tile,
+                                int ntiles, float sign, float2* __restrict__ out) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (long long)tile * tile) return;
-  const int r = (int)(i / tile), c = (int)(i % tile);
-  out[i] = sign * img[(long long)(uly + r) * W + ulx + c];
+  const long long T2 = (long long)tile * tile;
+  if (i >= T2 * ntiles) return;
+  const int k = (int)(i / T2), q = (int)(i - (long long)k * T2);
+  const int r = q / tile, c = q % tile;
+  out[i] = make_float2(sign * img[(long long)(ul[2 * k + 1] + r) * W + ul[2 * k] + c], 0.f);
 }
 
 // ---------------------------------------------------------------- blocks
@@ -398,6 +407,22 @@ __global__ void k_split_blocks(const float* __restrict__ img, int nx, int ny, in
   const int x = p % l, y = p / l;
   const int X = offx[b % bx] + x, Y = offy[b / bx] + y;
   const float v = (X < nx && Y < ny) ? img[(long long)Y * nx + X] : 0.f;
+  out[i] = make_float2(v, 0.f);
+}
+
+// split_IMG of every tile: the real parts of ntiles complex tile x tile frames into
+// out[k * nbk^2 + b] (block b = (i, j), i = b % nbk, at (off[i], off[j]); zero outside).
+__global__ void k_split_tile_blocks(const float2* __restrict__ F, int tile, int l, const int* __restrict__ off,
+                                    int nbk, int ntiles, float2* __restrict__ out) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long L2 = (long long)l * l;
+  const int nbb = nbk * nbk;
+  if (i >= L2 * nbb * ntiles) return;
+  const int g = (int)(i / L2);
+  const int p = (int)(i - (long long)g * L2);
+  const int k = g / nbb, b = g - k * nbb;
+  const int X = off[b % nbk] + p % l, Y = off[b / nbk] + p / l;
+  const float v = (X < tile && Y < tile) ? F[((long long)k * tile + Y) * tile + X].x : 0.f;
   out[i] = make_float2(v, 0.f);
 }
 
@@ -479,21 +504,24 @@ __global__ void k_rotation_stats(const double* __restrict__ partial, int n, int 
   stats[2 * t + 1] = (float)(s2 / L2 - mean * mean);
 }
 
-// Splits the joint forward transform Z of a template pair (2p in the real part, 2p+1 in the
-// imaginary part) into the two spectra, A = (Z(k) + conj Z(-k)) / 2 and B = (Z(k) - conj Z(-k))
-// / 2i, applies substract_by_mean + divided_by_var (k:460-477) in the frequency domain (the
-// standardisation is affine: the mean changes only the DC term by mean * l^2, the division
-// scales every term) and the scale of the reference's forward transform (src/norm.cu:516-519),
-// and writes the column-major half spectra out[t][kx 0..l/2][ky].  One workgroup per 32 x 32
+// Splits the joint forward transform Z of a pair of real planes (item 2j in the real part, 2j+1
+// in the imaginary part) into the two spectra, A = (Z(k) + conj Z(-k)) / 2 and B = (Z(k) -
+// conj Z(-k)) / 2i, and writes the column-major half spectra out[o][kx 0..l/2][ky] times scale.
+// Pairs are laid out [j][g] for g < stride, and item i of group g goes to out plane o = i *
+// stride + g (stride 1: template pairs, o = t).  With stats, the template standardisation
+// substract_by_mean + divided_by_var (k:460-477) is applied in the frequency domain (it is
+// affine: the mean changes only the DC term by mean * l^2, the division scales every term), with
+// the scale of the reference's forward transform (src/norm.cu:516-519).  One workgroup per 32 x 32
 // tile of (ky, kx) and pair; both tiles (k and -k) are staged in LDS, read along kx and written
 // along ky.
 __global__ void __launch_bounds__(256)
-    k_unpack_pair_spectra(const float2* __restrict__ Z, int n, int l, const float* __restrict__ stats,
+    k_unpack_pair_spectra(const float2* __restrict__ Z, int n, int stride, int l, const float* __restrict__ stats,
                           float scale, float2* __restrict__ out) {
   constexpr int TS = 32;
   __shared__ float2 zk[TS][TS + 1], zm[TS][TS + 1];
   const int hx = l / 2 + 1;
   const int kx0 = blockIdx.x * TS, ky0 = blockIdx.y * TS, p = blockIdx.z;
+  const int j = p / stride, g = p - j * stride;
   const float2* P = Z + (long long)p * l * l;
   const int c = threadIdx.x % TS;
   // all loads of a thread issued before the first is used
@@ -517,13 +545,15 @@ __global__ void __launch_bounds__(256)
   }
   __syncthreads();
   const int b = threadIdx.x % TS, ky = ky0 + b;
-  float mean[2], sd[2];
+  float mean[2] = {0.f, 0.f}, sd[2] = {1.f, 1.f};
+  if (stats) {
 #pragma unroll
-  for (int h = 0; h < 2; ++h) {
-    const int t = min(2 * p + h, n - 1);
-    mean[h] = stats[2 * t];
-    const float var = stats[2 * t + 1];
-    sd[h] = var != 0 ? sqrtf(var) : 1.f;
+    for (int h = 0; h < 2; ++h) {
+      const int o = min(2 * j + h, n - 1) * stride + g;
+      mean[h] = stats[2 * o];
+      const float var = stats[2 * o + 1];
+      sd[h] = var != 0 ? sqrtf(var) : 1.f;
+    }
   }
   for (int a = threadIdx.x / TS; a < TS; a += 256 / TS) {
     const int kx = kx0 + a;
@@ -533,57 +563,46 @@ __global__ void __launch_bounds__(256)
                    make_float2(0.5f * (u.y + w.y), 0.5f * (w.x - u.x))};
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
-      const int t = 2 * p + h;
-      if (t >= n) break;
+      const int i = 2 * j + h;
+      if (i >= n) break;
       float2 x = v[h];
       if (kx == 0 && ky == 0 && mean[h] != 0) x.x -= mean[h] * (float)((long long)l * l);
       x.x = x.x / sd[h] * scale;
       x.y = x.y / sd[h] * scale;
-      out[((long long)t * hx + kx) * l + ky] = x;
+      out[((long long)(i * stride + g) * hx + kx) * l + ky] = x;
     }
   }
 }
 
-// rotate_subIMG (k:534-603) on complex planes (both parts).
-__global__ void k_rotate_complex(const float2* __restrict__ in, int n, int l, float cose, float sine,
-                                 float2* __restrict__ out) {
+// rotate_subIMG (k:534-603) of real planes, two angles per complex output plane: for the
+// pair plane p = j * nb + g, out[p] = rotate(in[g].x, cs[2j]) + i rotate(in[g].x, cs[2j + 1])
+// (the imaginary part zero when 2j + 1 = na).  The blocks are real images (split_IMG of a real
+// tile; the phase-flip normalisation keeps them real up to rounding), so one forward transform
+// of the pair plane gives the spectra of both rotated copies (k_unpack_pair_spectra).
+__global__ void k_rotate_real_pairs(const float2* __restrict__ in, int nb, int l, const float2* __restrict__ cs,
+                                    int na, float2* __restrict__ out) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long L2 = (long long)l * l;
-  if (i >= L2 * n) return;
-  const int t = (int)(i / L2);
-  const int q = (int)(i - (long long)t * L2);
-  const float2* P = in + (long long)t * L2;
-  const Bil b = bil_coords(q % l, q / l, l, cose, sine);
-  float2 res = make_float2(0.f, 0.f);
-  if (b.ok) {
-    const float tt = 1.0f - b.t, uu = 1.0f - b.u;
-    const float w0 = tt * uu, w1 = b.t * uu, w2 = b.t * b.u, w3 = tt * b.u;
-    res.x = P[b.k0].x * w0 + P[b.k1].x * w1 + P[b.k2].x * w2 + P[b.k3].x * w3;
-    res.y = P[b.k0].y * w0 + P[b.k1].y * w1 + P[b.k2].y * w2 + P[b.k3].y * w3;
-  }
-  out[i] = res;
-}
-
-// The same for na angles at once: out[a][t] = rotate(in[t], cs[a]) with cs[a] = (cos, sin).
-__global__ void k_rotate_complex_multi(const float2* __restrict__ in, int n, int l,
-                                       const float2* __restrict__ cs, int na, float2* __restrict__ out) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long long L2 = (long long)l * l;
-  if (i >= L2 * n * na) return;
+  const int npairs = (na + 1) / 2;
+  if (i >= L2 * nb * npairs) return;
   const long long pl = i / L2;
-  const int a = (int)(pl / n), t = (int)(pl - (long long)a * n);
+  const int j = (int)(pl / nb), g = (int)(pl - (long long)j * nb);
   const int q = (int)(i - pl * L2);
-  const float2* P = in + (long long)t * L2;
-  const float2 c = cs[a];
-  const Bil b = bil_coords(q % l, q / l, l, c.x, c.y);
-  float2 res = make_float2(0.f, 0.f);
-  if (b.ok) {
-    const float tt = 1.0f - b.t, uu = 1.0f - b.u;
-    const float w0 = tt * uu, w1 = b.t * uu, w2 = b.t * b.u, w3 = tt * b.u;
-    res.x = P[b.k0].x * w0 + P[b.k1].x * w1 + P[b.k2].x * w2 + P[b.k3].x * w3;
-    res.y = P[b.k0].y * w0 + P[b.k1].y * w1 + P[b.k2].y * w2 + P[b.k3].y * w3;
+  const float2* P = in + (long long)g * L2;
+  float res[2] = {0.f, 0.f};
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int a = 2 * j + h;
+    if (a >= na) break;
+    const float2 c = cs[a];
+    const Bil b = bil_coords(q % l, q / l, l, c.x, c.y);
+    if (b.ok) {
+      const float tt = 1.0f - b.t, uu = 1.0f - b.u;
+      const float w0 = tt * uu, w1 = b.t * uu, w2 = b.t * b.u, w3 = tt * b.u;
+      res[h] = P[b.k0].x * w0 + P[b.k1].x * w1 + P[b.k2].x * w2 + P[b.k3].x * w3;
+    }
   }
-  st_stream(out + i, res);  // read once, by the next transform; keeps the source block in L2
+  st_stream(out + i, make_float2(res[0], res[1]));  // read once, by the next transform
 }
 
 }  // namespace
@@ -655,38 +674,44 @@ void launch_real_to_complex(const float* in, float2* out, long long count, float
   k_real_to_complex<<<nblk(count), BS, 0, s>>>(in, out, count, sign);
   GS_CHECK_LAUNCH();
 }
-void launch_phase_flip(float2* F, int nx, int ny, const DevParams& p, hipStream_t s) {
-  k_phase_flip<<<nblk((long long)nx * ny), BS, 0, s>>>(F, nx, ny, p);
+void launch_phase_flip(float2* F, int nx, int ny, int nf, const DevParams& p, hipStream_t s) {
+  k_phase_flip<<<nblk((long long)nx * ny * nf), BS, 0, s>>>(F, nx, ny, nf, p);
   GS_CHECK_LAUNCH();
 }
-int radial_image_groups() { return kImageGroups; }
-void launch_radial_power_image(const float2* F, int nx, int ny, double* part, double* ra, hipStream_t s) {
-  const int nbins = std::max(nx, ny) / 2;
+int radial_image_groups(int nf) { return nf <= 1 ? kImageGroups : std::max(16, kImageGroups / nf); }
+void launch_radial_power_image(const float2* F, int nx, int ny, int nf, double* part, double* ra, hipStream_t s) {
+  const int nbins = std::max(nx, ny) / 2, groups = radial_image_groups(nf);
   // wave-private histograms within 64 KB of LDS per workgroup
   const int waves = (int)std::min<size_t>(4, (64 * 1024) / (sizeof(double) * std::max(1, nbins)));
   GS_REQUIRE(waves >= 1, kUnsupported, "frame too large for radial histogram");
-  k_radial_power_image<<<kImageGroups, 64 * waves, sizeof(double) * nbins * waves, s>>>(F, nx, ny, part);
+  GS_REQUIRE(nf >= 1 && nf < 65536, kInvalid, "frame count out of range");
+  k_radial_power_image<<<dim3(groups, nf), 64 * waves, sizeof(double) * nbins * waves, s>>>(F, nx, ny, part);
   GS_CHECK_LAUNCH();
-  k_radial_power_finish<<<nblk((long long)nbins * 64), BS, 0, s>>>(part, 1, nbins, kImageGroups, ra);
+  k_radial_power_finish<<<nblk((long long)nf * nbins * 64), BS, 0, s>>>(part, nf, nbins, groups, ra);
   GS_CHECK_LAUNCH();
 }
-void launch_whiten_filter_image(float2* F, int nx, int ny, const double* ra, const double* rb,
+void launch_whiten_filter_image(float2* F, int nx, int ny, int nf, const double* ra, const double* rb,
                                 const DevParams& p, hipStream_t s) {
-  k_whiten_filter_image<<<nblk((long long)nx * ny), BS, 0, s>>>(F, nx, ny, ra, rb, p);
+  k_whiten_filter_image<<<nblk((long long)nx * ny * nf), BS, 0, s>>>(F, nx, ny, nf, ra, rb, p);
   GS_CHECK_LAUNCH();
 }
 void launch_zero_dc(float2* F, hipStream_t s) {
   k_zero_dc<<<1, 64, 0, s>>>(F);
   GS_CHECK_LAUNCH();
 }
-void launch_extract_tile(const float* img, int W, int ulx, int uly, int tile, float sign, float* out,
-                         hipStream_t s) {
-  k_extract_tile<<<nblk((long long)tile * tile), BS, 0, s>>>(img, W, ulx, uly, tile, sign, out);
+void launch_extract_tiles(const float* img, int W, const int* ul, int tile, int ntiles, float sign, float2* out,
+                          hipStream_t s) {
+  k_extract_tiles<<<nblk((long long)tile * tile * ntiles), BS, 0, s>>>(img, W, ul, tile, ntiles, sign, out);
   GS_CHECK_LAUNCH();
 }
 void launch_split_blocks(const float* img, int nx, int ny, int l, const int* offx, const int* offy,
                          int bx, int nb, float2* out, hipStream_t s) {
   k_split_blocks<<<nblk((long long)nb * l * l), BS, 0, s>>>(img, nx, ny, l, offx, offy, bx, nb, out);
+  GS_CHECK_LAUNCH();
+}
+void launch_split_tile_blocks(const float2* F, int tile, int l, const int* off, int nbk, int ntiles, float2* out,
+                              hipStream_t s) {
+  k_split_tile_blocks<<<nblk((long long)nbk * nbk * ntiles * l * l), BS, 0, s>>>(F, tile, l, off, nbk, ntiles, out);
   GS_CHECK_LAUNCH();
 }
 int rotation_groups(int l) { return ((l + kRotTX - 1) / kRotTX) * ((l + kRotTY - 1) / kRotTY); }
@@ -703,21 +728,17 @@ void launch_rotation_stats(const double* partial, int n, int l, float* stats, hi
   k_rotation_stats<<<nblk(n, 64), 64, 0, s>>>(partial, n, rotation_groups(l), l, stats);
   GS_CHECK_LAUNCH();
 }
-void launch_unpack_pair_spectra(const float2* Z, int n, int l, const float* stats, float scale, float2* out,
-                                hipStream_t s) {
-  const dim3 grid((unsigned)((l / 2 + 1 + 31) / 32), (unsigned)((l + 31) / 32), (unsigned)((n + 1) / 2));
-  GS_REQUIRE((n + 1) / 2 < 65536, kInvalid, "template count out of range");
-  k_unpack_pair_spectra<<<grid, 256, 0, s>>>(Z, n, l, stats, scale, out);
+void launch_unpack_pair_spectra(const float2* Z, int n, int stride, int l, const float* stats, float scale,
+                                float2* out, hipStream_t s) {
+  const long long pairs = (long long)((n + 1) / 2) * stride;
+  GS_REQUIRE(stride >= 1 && pairs < 65536, kInvalid, "plane count out of range");
+  const dim3 grid((unsigned)((l / 2 + 1 + 31) / 32), (unsigned)((l + 31) / 32), (unsigned)pairs);
+  k_unpack_pair_spectra<<<grid, 256, 0, s>>>(Z, n, stride, l, stats, scale, out);
   GS_CHECK_LAUNCH();
 }
-void launch_rotate_complex(const float2* in, int n, int l, float cose, float sine, float2* out,
-                           hipStream_t s) {
-  k_rotate_complex<<<nblk((long long)n * l * l), BS, 0, s>>>(in, n, l, cose, sine, out);
-  GS_CHECK_LAUNCH();
-}
-void launch_rotate_complex_multi(const float2* in, int n, int l, const float2* cs, int na, float2* out,
-                                 hipStream_t s) {
-  k_rotate_complex_multi<<<nblk((long long)n * l * l * na), BS, 0, s>>>(in, n, l, cs, na, out);
+void launch_rotate_real_pairs(const float2* in, int nb, int l, const float2* cs, int na, float2* out,
+                              hipStream_t s) {
+  k_rotate_real_pairs<<<nblk((long long)l * l * nb * ((na + 1) / 2)), BS, 0, s>>>(in, nb, l, cs, na, out);
   GS_CHECK_LAUNCH();
 }
 

@@ -129,7 +129,7 @@ typedef enum {
   GS_STAGE_SIGMA = 5,      /* norm: per block pixel sigma, nb*l*l float64 */
   GS_STAGE_BEST = 6,       /* norm: per block pixel best correlation, nb*l*l float32 */
   GS_STAGE_ARG = 7,        /* norm: per block pixel argmax angle_index*n+template, nb*l*l int32 */
-  GS_STAGE_PLANES = 8      /* nonorm: peak, index, sum, sum2 float64 per [tile][template][angle][block] */
+  GS_STAGE_PLANES = 8      /* nonorm: peak, index, sum, sum2 float64 per [template][angle][tile][block] */
 } gs_stage;
 
 /* Size in bytes of a stage buffer after the last run (0 if not available). */

@@ -69,3 +69,40 @@ def test_nonorm_overlap(gs, orc):
         assert abs(p["score"] - q["score"]) <= 1e-4 * abs(q["score"])
         assert abs(p["local_x"] - q["local_x"]) < 1e-3 and abs(p["local_y"] - q["local_y"]) < 1e-3
     eng.close()
+
+
+@pytest.mark.parametrize("l,phistep,W,H", [(128, 40.0, 2000, 1500), (720, 40.0, 1440, 720)])
+def test_nonorm_tiles_batched(gs, orc, l, phistep, W, H):
+    """All tiles of a micrograph are searched together: the blocks of every tile form one batch
+    for the normalisation, rotation and correlation, and the rotated copies are made two angles
+    per complex plane.  Nine angles leave the last angle pair half empty.  At window 128 the
+    2000 x 1500 micrograph has nine overlapping tiles of 36 blocks (324 blocks), so the angles
+    run in groups of six and three; at window 720 two tiles of one block."""
+    import synth
+
+    print(f"norm_type=0 window {l}, {W}x{H}, phistep {phistep}", flush=True)
+    cfg = gs.SearchConfig(phistep=phistep, window_size=l, norm_type=0, phase_flip=1, threshold=3.0,
+                          diameter=64.0 if l == 128 else 128.0, nms_radius=0.0)
+    ds = synth.make_dataset(9, W, H, n_templates=3, tsize=64 if l == 128 else 128, n_particles=6, snr=0.5,
+                            defocus=1.5)
+    _, orecs = orc.search_nonorm(cfg, ds.image, ds.templates, ds.ctf)
+    eng = gs.Engine(cfg)
+    eng.load_templates(ds.templates)
+    g = eng.search(ds.image, ds.ctf)
+    assert len(orecs) > 0
+    key = lambda p: (int(p["block"]), int(p["angle_index"]), int(p["template_id"]))
+    assert len({int(p["block"]) for p in orecs}) > 1  # records from more than one tile
+    oi = {key(p): p for p in orecs}
+    gi = {key(p): p for p in g}
+    # a record may be absent on one side only where its oracle score lies within the score
+    # tolerance of the threshold
+    for k in set(oi) ^ set(gi):
+        q = oi.get(k, gi.get(k))
+        assert abs(q["score"] - 3.0) < 1e-4 * max(1.0, abs(q["score"])), k
+    for k in set(oi) & set(gi):
+        p, q = gi[k], oi[k]
+        assert int(p["pixel"]) == int(q["pixel"]), k
+        assert abs(p["score"] - q["score"]) <= 1e-4 * abs(q["score"]), k
+        assert abs(p["local_x"] - q["local_x"]) < 1e-3 and abs(p["local_y"] - q["local_y"]) < 1e-3, k
+    print(f"{len(orecs)} records, tiles {sorted({int(p['block']) for p in orecs})}", flush=True)
+    eng.close()
