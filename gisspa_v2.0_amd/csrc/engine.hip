@@ -1302,11 +1302,13 @@ class Engine {
       launch_scale_planes(blk, B, (long long)L2, m, 1.f, stream_);
       fft2_planes(blk, S, B, +1, 1.f);
     }
+    float* blkr = d_blkreal_.ensure((size_t)B * L2);
+    launch_extract_real(blk, blkr, (long long)B * L2, stream_);
     for (int a0 = 0; a0 < na; a0 += ag_max) {
       const int ag = std::min(ag_max, na - a0), nbp = ag * B, np2 = (ag + 1) / 2 * B;
       // rotate_subIMG of every block for angles a0..a0+ag (k:534-603), two angles per plane, and
       // the forward transforms: half spectra [angle][block] scaled by 1/l^2 (src/nonorm.cu:457-470)
-      launch_rotate_real_pairs(blk, B, l, dcs + a0, ag, rotb, stream_);
+      launch_rotate_real_pairs(blkr, B, l, dcs + a0, ag, rotb, stream_);
       fft2_planes(rotb, S, np2, -1, 1.f);
       launch_unpack_pair_spectra(rotb, ag, B, l, nullptr, (float)(1.0 / (double)L2), ihat, stream_);
       for (int t0 = 0; t0 < n; t0 += tc_max) {
@@ -1426,7 +1428,7 @@ class Engine {
 
   DevBuf<float> d_rmask_;
   DevBuf<float2> d_rph_;
-  DevBuf<float> d_raw_, d_img_, d_imgpre_, d_tprep_, d_stats_, d_best_;
+  DevBuf<float> d_raw_, d_img_, d_imgpre_, d_tprep_, d_stats_, d_best_, d_blkreal_;
   DevBuf<float2> d_work_, d_work2_, d_scratch_, d_frame_, d_frame2_, d_blocks_, d_ihat_, d_that_, d_Z_, d_cs_;
   DevBuf<double> d_mom_, d_ra_, d_ra_img_, d_m_, d_part_, d_S_, d_SS_, d_mu_, d_sigma_, d_planes_, d_chpart_;
   DevBuf<float> d_plane_part_;

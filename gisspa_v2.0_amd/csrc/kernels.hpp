@@ -87,9 +87,9 @@ void launch_rotation_stats(const double* partial, int n, int l, float* stats, hi
 // null) of items i < n from the pair transforms Z[j * stride + g] holding items 2j and 2j + 1.
 void launch_unpack_pair_spectra(const float2* Z, int n, int stride, int l, const float* stats, float scale,
                                 float2* out, hipStream_t s);
-// Rotated copies of nb real blocks (real parts of in) for na angles cs[a] = (cos, sin) (device
-// array), two angles per complex plane: out[j * nb + g] = rot(in[g], cs[2j]) + i rot(in[g], cs[2j+1]).
-void launch_rotate_real_pairs(const float2* in, int nb, int l, const float2* cs, int na, float2* out,
+// Rotated copies of nb real blocks in[g] for na angles cs[a] = (cos, sin) (device array), two
+// angles per complex plane: out[j * nb + g] = rot(in[g], cs[2j]) + i rot(in[g], cs[2j+1]).
+void launch_rotate_real_pairs(const float* in, int nb, int l, const float2* cs, int na, float2* out,
                               hipStream_t s);
 
 // ---- correlation (north-star kernels b, c, d, e) ----

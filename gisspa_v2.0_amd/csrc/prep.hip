@@ -202,12 +202,21 @@ __global__ void k_whiten_planes(float2* __restrict__ F, int n, int l, const doub
   }
 }
 
-// apply_weighting_function (k:243-288): band-pass and |CTF| Wiener-like weighting.
+// apply_weighting_function (k:243-288): band-pass and |CTF| Wiener-like weighting.  The factor
+// depends on the pixel only, so each thread evaluates it once for its pixel and applies it to
+// kWeightT planes (blockIdx.y selects the group of planes).
+constexpr int kWeightT = 8;
 __global__ void k_weight_planes(float2* __restrict__ F, int n, int l, DevParams p) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long L2 = (long long)l * l;
-  if (i >= L2 * n) return;
-  const int q = (int)(i % L2);
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= L2) return;
+  const int t0 = blockIdx.y * kWeightT;
+  float2* const Fq = F + (long long)t0 * L2 + q;
+  const int nt = min(kWeightT, n - t0);
+  float2 v[kWeightT];
+#pragma unroll
+  for (int k = 0; k < kWeightT; ++k)
+    if (k < nt) v[k] = Fq[k * L2];
   const int x = q % l, y = q / l;
   const int fx = min(x, l - x), fy = min(y, l - y);
   const int rr = rbin(fx, fy);
@@ -215,12 +224,14 @@ __global__ void k_weight_planes(float2* __restrict__ F, int n, int l, DevParams 
   if (rr < l / 2 && rr >= 0) {
     const double r = hypot((double)fy, (double)fx);
     const double ss = r * (double)p.ds;
-    const double v = ctf_ast(x, (y + l / 2) % l, l, l, p, 2);
+    const double c = ctf_ast(x, (y + l / 2) % l, l, l, p, 2);
     const double signal = exp(p.bf * ss * ss + p.bf2 * ss + p.bf3) / ((double)p.kk + 1);
     const double Ncurve = exp(p.a * ss * ss + p.b * ss + p.b2) / signal;
-    f *= v * sqrt(1 / (Ncurve + (double)p.kk * v * v));
+    f *= c * sqrt(1 / (Ncurve + (double)p.kk * c * c));
   }
-  F[i] = make_float2((float)(F[i].x * f), (float)(F[i].y * f));
+#pragma unroll
+  for (int k = 0; k < kWeightT; ++k)
+    if (k < nt) Fq[k * L2] = make_float2((float)(v[k].x * f), (float)(v[k].y * f));
 }
 
 // apply_mask (k:225-241) on the real part.
@@ -575,34 +586,50 @@ __global__ void __launch_bounds__(256)
 }
 
 // rotate_subIMG (k:534-603) of real planes, two angles per complex output plane: for the
-// pair plane p = j * nb + g, out[p] = rotate(in[g].x, cs[2j]) + i rotate(in[g].x, cs[2j + 1])
-// (the imaginary part zero when 2j + 1 = na).  The blocks are real images (split_IMG of a real
-// tile; the phase-flip normalisation keeps them real up to rounding), so one forward transform
-// of the pair plane gives the spectra of both rotated copies (k_unpack_pair_spectra).
-__global__ void k_rotate_real_pairs(const float2* __restrict__ in, int nb, int l, const float2* __restrict__ cs,
-                                    int na, float2* __restrict__ out) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+// pair plane p = j * nb + g, out[p] = rotate(in[g], cs[2j]) + i rotate(in[g], cs[2j + 1]) (the
+// imaginary part zero when 2j + 1 = na).  The blocks are real images (split_IMG of a real tile;
+// the phase-flip normalisation keeps them real up to rounding), so one forward transform of the
+// pair plane gives the spectra of both rotated copies (k_unpack_pair_spectra).  Workgroup: a
+// 16 x 16 output tile, each wavefront an 8 x 8 square of it, so that the bilinear taps of a
+// wavefront fall on a compact patch of the source at every angle (a 64-pixel output row maps to a
+// source line that crosses 64 rows near 90 degrees).
+__global__ void __launch_bounds__(256)
+    k_rotate_real_pairs(const float* __restrict__ in, int nb, int l, const float2* __restrict__ cs, int na,
+                        float2* __restrict__ out) {
   const long long L2 = (long long)l * l;
-  const int npairs = (na + 1) / 2;
-  if (i >= L2 * nb * npairs) return;
-  const long long pl = i / L2;
-  const int j = (int)(pl / nb), g = (int)(pl - (long long)j * nb);
-  const int q = (int)(i - pl * L2);
-  const float2* P = in + (long long)g * L2;
-  float res[2] = {0.f, 0.f};
+  const int gx = (l + 15) / 16;
+  const int p = blockIdx.y, j = p / nb, g = p - j * nb;
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int x = ((int)blockIdx.x % gx) * 16 + (wave & 1) * 8 + (lane & 7);
+  const int y = ((int)blockIdx.x / gx) * 16 + (wave >> 1) * 8 + (lane >> 3);
+  if (x >= l || y >= l) return;  // no barrier follows
+  const float* P = in + (long long)g * L2;
+  Bil b[2];
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
-    const int a = 2 * j + h;
-    if (a >= na) break;
+    const int a = min(2 * j + h, na - 1);
     const float2 c = cs[a];
-    const Bil b = bil_coords(q % l, q / l, l, c.x, c.y);
-    if (b.ok) {
-      const float tt = 1.0f - b.t, uu = 1.0f - b.u;
-      const float w0 = tt * uu, w1 = b.t * uu, w2 = b.t * b.u, w3 = tt * b.u;
-      res[h] = P[b.k0].x * w0 + P[b.k1].x * w1 + P[b.k2].x * w2 + P[b.k3].x * w3;
-    }
+    b[h] = bil_coords(x, y, l, c.x, c.y);
+    if (2 * j + h >= na) b[h].ok = false;
   }
-  st_stream(out + i, make_float2(res[0], res[1]));  // read once, by the next transform
+  float v[2][4];
+#pragma unroll
+  for (int h = 0; h < 2; ++h)
+    if (b[h].ok) {
+      v[h][0] = P[b[h].k0];
+      v[h][1] = P[b[h].k1];
+      v[h][2] = P[b[h].k2];
+      v[h][3] = P[b[h].k3];
+    }
+  float res[2] = {0.f, 0.f};
+#pragma unroll
+  for (int h = 0; h < 2; ++h)
+    if (b[h].ok) {
+      const float tt = 1.0f - b[h].t, uu = 1.0f - b[h].u;
+      const float w0 = tt * uu, w1 = b[h].t * uu, w2 = b[h].t * b[h].u, w3 = tt * b[h].u;
+      res[h] = v[h][0] * w0 + v[h][1] * w1 + v[h][2] * w2 + v[h][3] * w3;
+    }
+  st_stream(out + p * L2 + (long long)y * l + x, make_float2(res[0], res[1]));  // read once, by the transform
 }
 
 }  // namespace
@@ -639,7 +666,8 @@ void launch_whiten_planes(float2* F, int n, int l, const double* ra, const doubl
   GS_CHECK_LAUNCH();
 }
 void launch_weight_planes(float2* F, int n, int l, const DevParams& p, hipStream_t s) {
-  k_weight_planes<<<nblk((long long)n * l * l), BS, 0, s>>>(F, n, l, p);
+  GS_REQUIRE((n + kWeightT - 1) / kWeightT < 65536, kInvalid, "template count out of range");
+  k_weight_planes<<<dim3(nblk((long long)l * l), (unsigned)((n + kWeightT - 1) / kWeightT)), BS, 0, s>>>(F, n, l, p);
   GS_CHECK_LAUNCH();
 }
 void launch_mask_planes(float2* T, int n, int l, float d_m, float ehw, hipStream_t s) {
@@ -736,9 +764,12 @@ void launch_unpack_pair_spectra(const float2* Z, int n, int stride, int l, const
   k_unpack_pair_spectra<<<grid, 256, 0, s>>>(Z, n, stride, l, stats, scale, out);
   GS_CHECK_LAUNCH();
 }
-void launch_rotate_real_pairs(const float2* in, int nb, int l, const float2* cs, int na, float2* out,
+void launch_rotate_real_pairs(const float* in, int nb, int l, const float2* cs, int na, float2* out,
                               hipStream_t s) {
-  k_rotate_real_pairs<<<nblk((long long)l * l * nb * ((na + 1) / 2)), BS, 0, s>>>(in, nb, l, cs, na, out);
+  const long long pairs = (long long)nb * ((na + 1) / 2);
+  GS_REQUIRE(pairs >= 1 && pairs < 65536, kInvalid, "plane count out of range");
+  const int gx = (l + 15) / 16;
+  k_rotate_real_pairs<<<dim3((unsigned)(gx * gx), (unsigned)pairs), 256, 0, s>>>(in, nb, l, cs, na, out);
   GS_CHECK_LAUNCH();
 }
 
