@@ -9,16 +9,24 @@ A step is one full search (template and image preprocessing, 72 x 1000 x 234 cor
 per-pixel statistics, records and NMS) with the image and templates already resident in HBM.
 value = template.angle.pixel correlations per second = 1000 * 72 * 5760 * 4092 / s.
 
-The roofline entry is for the dominant correlation kernel: its algorithmic bytes per launch
-(template half-spectrum read + inter-pass buffer write for the column pass; inter-pass buffer
-read for the row pass) over its average launch time, from HIP events recorded around every
-launch on the engine's stream during the timed steps.  The CPU baseline is the FP64 oracle on
-a bounded sample of the same workload (see cpu_baseline.sample).
+The roofline entry is for the dominant correlation kernel (the column pass): its algorithmic
+bytes per launch (the inter-pass buffer written once, every template and block spectrum of the
+launch read once) over its average launch time, from HIP events recorded around every launch on
+the engine's stream during the timed steps; `kernels` gives the same for the row pass (buffer
+read once plus one read-modify-write of the per-pixel records), `traffic` and
+`hbm_frac_counters` the copy-calibrated HBM bytes rocprofv3 counted for the same kernels
+(profiles/corr_c1_traffic.json), and `limiter` their shader counters (profiles/corr_c1_valu.json).
+The CPU baseline is the FP64 oracle on a bounded sample of the same workload (see
+cpu_baseline.sample).
 
-The second half of the metric ("refined peaks/s at tol") is the `refine` object: the top 2000
-block-pixel records of the same search, perturbed, refined by 20 ascent steps on the GPU, with
-a sample refined by the FP64 oracle from the same starts to check the tolerance.
-`--batch K` measures a batch of K uneven micrographs instead (its own JSON line).
+The second half of the metric ("refined peaks/s at tol") is the `refine` object: the 2000 best
+picks at z >= 5 after NMS (64 px) of the same search, perturbed, refined by 20 ascent steps on
+the GPU, with a sample refined by the FP64 oracle from the same starts to check the tolerance.
+The `batch_refine` object times a batch of 4 uneven micrographs searched and refined by
+gs_search_batch_refine against the per-micrograph form; the `c0` object the norm_type=0
+configuration (1440 x 1440, 8 templates, 36 angles, window 720) with its oracle comparison.
+`--batch K` measures a batch of K uneven micrographs instead, `--c0` the norm_type=0
+configuration (each its own JSON line).
 """
 import argparse
 import json
@@ -58,6 +66,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-refine", type=int, default=4,
                     help="K uneven micrographs for the batched + refined leg of the default line (0: skip)")
+    ap.add_argument("--c0-leg", type=int, default=1,
+                    help="1: add the norm_type=0 configuration as the c0 object of the default line")
     ap.add_argument("--cpu-templates", type=int, default=4)
     ap.add_argument("--cpu-angles", type=int, default=4)
     return ap.parse_args()
@@ -386,11 +396,11 @@ def batch_main(a):
     print(json.dumps(out), flush=True)
 
 
-def c0_main(a):
+def c0_leg(steps, warmup, no_cpu):
     """norm_type=0 configuration: 1440x1440 micrograph with 40 particles at SNR 0.1, 8 templates
     of 128x128, 36 angles (phistep 10), window 720 (4 tiles of one block), phase_flip=1,
-    defocus 1.5 um.  value = correlations/s = 8 * 36 * 1440^2 / s; the FP64 oracle runs the
-    same search in full for the CPU baseline."""
+    defocus 1.5 um.  Rate = correlations/s = 8 * 36 * 1440^2 / s per search; without no_cpu the
+    FP64 oracle runs the same search in full (CPU baseline and pick comparison)."""
     import gisspa
     import synth
 
@@ -401,25 +411,20 @@ def c0_main(a):
     eng = gisspa.Engine(cfg)
     eng.load_templates(ds.templates)
     eng.set_image(ds.image, ds.ctf)
-    for _ in range(a.warmup):
+    for _ in range(warmup):
         eng.run()
     t0 = time.perf_counter()
-    for _ in range(a.steps):
+    for _ in range(steps):
         eng.run()
-    dt = (time.perf_counter() - t0) / a.steps
+    dt = (time.perf_counter() - t0) / steps
     total_ms, corr_ms = eng.timing()
     recs = eng.records()
+    eng.close()
     work = 8 * 36 * 1440 * 1440
-    out = {
-        "metric": "template·angle·pixel correlations/s (norm_type=0 configuration)",
-        "value": work / dt, "unit": "correlations/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
-        "ms_per_step": dt * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "fp32", "data": "synthetic (seeded blob-density projections, CTF, N(0,1) noise)",
-        "config": {"workload": "norm_type=0: 1440x1440, 8 templates 128x128, 36 angles, window 720, "
-                               "phase_flip=1", "picks": int(len(recs))},
-        "pipeline_ms": {"device_total": total_ms, "tile_angle_loop": corr_ms},
-    }
-    if not a.no_cpu:
+    out = {"workload": "norm_type=0: 1440x1440, 8 templates 128x128, 36 angles, window 720, phase_flip=1",
+           "correlations_per_s": work / dt, "ms_per_search": dt * 1e3, "steps": steps, "warmup": warmup,
+           "picks": int(len(recs)), "pipeline_ms": {"device_total": total_ms, "tile_angle_loop": corr_ms}}
+    if not no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle
 
@@ -429,9 +434,26 @@ def c0_main(a):
         cdt = time.perf_counter() - t0
         out["cpu_baseline"] = {"value": work / cdt, "unit": "correlations/s", "cores": oracle.num_threads(),
                                "kind": "port", "sample": f"the whole search on the FP64 oracle ({cdt:.1f} s)"}
-        out["config"]["picks_identical_to_oracle"] = bool(
+        out["picks_identical_to_oracle"] = bool(
             [(int(p["template_id"]), int(p["angle_index"]), int(p["block"])) for p in recs] ==
             [(int(p["template_id"]), int(p["angle_index"]), int(p["block"])) for p in opicks])
+    return out
+
+
+def c0_main(a):
+    """The norm_type=0 configuration as its own JSON line (see c0_leg)."""
+    leg = c0_leg(a.steps, a.warmup, a.no_cpu)
+    out = {
+        "metric": "template·angle·pixel correlations/s (norm_type=0 configuration)",
+        "value": leg["correlations_per_s"], "unit": "correlations/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": leg["ms_per_search"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32", "data": "synthetic (seeded blob-density projections, CTF, N(0,1) noise)",
+        "config": {"workload": leg["workload"], "picks": leg["picks"]},
+        "pipeline_ms": leg["pipeline_ms"],
+    }
+    if "cpu_baseline" in leg:
+        out["cpu_baseline"] = leg["cpu_baseline"]
+        out["config"]["picks_identical_to_oracle"] = leg["picks_identical_to_oracle"]
     print(json.dumps(out), flush=True)
 
 
@@ -601,6 +623,8 @@ def main():
     eng.close()
     if not a.angles and a.templates == 1000 and a.batch_refine > 0:
         out["batch_refine"] = batch_refine_leg(a.batch_refine, a.no_cpu)
+    if not a.angles and a.templates == 1000 and a.c0_leg:
+        out["c0"] = c0_leg(20, 3, a.no_cpu)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cfg, ds, a.cpu_templates, a.cpu_angles)
     else:

@@ -1246,8 +1246,16 @@ __device__ __forceinline__ bool zero_row(int y, int L, int band) { return min(y,
 // Rows of `in` transformed into `out` (the same buffer for an in-place transform: every row a
 // workgroup reads is staged in LDS before it writes any, so the pointers carry no restrict).  A
 // workgroup whose rows are all zero by `band` writes nothing: those rows of `out` are left as
-// they were and must only be read through the column pass with the same band.
-template <int L, int SIGN>
+// they were and must only be read through the column pass with the same band.  PAIRS writes
+// each row in mirror-pair order (pair_column below) for k_plane_cols_pairs.
+template <int L>
+__device__ __forceinline__ int pair_column(int s) {
+  // slot 0: column 0, slot 1: column L/2, slots 2k and 2k + 1: columns k and L - k (0 < k < L/2)
+  const int k = s >> 1;
+  return k == 0 ? ((s & 1) ? L / 2 : 0) : ((s & 1) ? L - k : k);
+}
+
+template <int L, int SIGN, bool PAIRS = false>
 __global__ void __launch_bounds__(256)
     k_plane_rows(const float2* in, float2* out, long long nrows, const float2* __restrict__ twL, float scale,
                  int band) {
@@ -1278,7 +1286,12 @@ __global__ void __launch_bounds__(256)
                                                typename C::ColPlan{});
   __syncthreads();
   for (int e = tid; e < nr * L; e += 256) {
-    const float2 v = buf[e];
+    int src_e = e;
+    if constexpr (PAIRS) {
+      const int r = e / L;
+      src_e = r * L + pair_column<L>(e - r * L);
+    }
+    const float2 v = buf[src_e];
     dst[e] = make_float2(v.x * scale, v.y * scale);
   }
 }
@@ -1320,6 +1333,76 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// Column pass of the forward transform of real-pair planes (item 2j in the real part, 2j + 1 in
+// the imaginary part) fused with their split into half spectra.  The row pass wrote each row in
+// mirror-pair order (pair_column), so workgroup (group of HC column pairs, pair plane p = j *
+// stride + g) reads one contiguous run of 2 HC slots per row: columns k and L - k for k in
+// [kx0, kx0 + HC) (group 0: columns 0 and L/2, each its own mirror, instead of k = 0 and L).  It
+// transforms them, forms A = (Z(k) + conj Z(-k)) / 2 and B = (Z(k) - conj Z(-k)) / 2i per
+// (kx, ky) and writes out[o][kx][ky] (column-major, ky contiguous) for items i = 2j, 2j + 1 < n
+// at o = i * stride + g, times scale; with stats the template standardisation (mean at the DC
+// term times L^2, division by the standard deviation) as in k_unpack_pair_spectra.  The
+// transformed columns never leave LDS.
+template <int L>
+__global__ void __launch_bounds__(256)
+    k_plane_cols_pairs(const float2* __restrict__ data, const float2* __restrict__ twL, int n, int stride,
+                       const float* __restrict__ stats, float scale, float2* __restrict__ out) {
+  using C = CorrCfg<L>;
+  constexpr int T = C::TPC, COLS = 256 / T, HC = COLS / 2, LP = L + 1, HX = L / 2 + 1;
+  static_assert((L / 2) % HC == 0, "pair groups");
+  __shared__ float2 buf[COLS * LP];
+  __shared__ float2 tw[L];
+  const int tid = threadIdx.x;
+  const int kx0 = blockIdx.x * HC;
+  const int p = blockIdx.y, jp = p / stride, g = p - jp * stride;
+  dev::RegStage<L> twr;
+  twr.load(twL, tid);
+  const float2* P = data + (long long)p * L * L + 2 * kx0;
+  // slot 2 jc + m of the run -> LDS column jc (m = 0) or HC + jc (m = 1)
+  dev::burst_copy<COLS * L, 24>(
+      tid, [&](int e) { return P[(long long)(e / COLS) * L + e % COLS]; },
+      [&](int e, float2 v) {
+        const int c = e % COLS;
+        buf[((c & 1) * HC + (c >> 1)) * LP + e / COLS] = v;
+      });
+  twr.store(tw, tid);
+  __syncthreads();
+  ct::run_stockham<L, T, -1, ct::Sync::WAVE>(buf + (tid / T) * LP, tw, tid % T, typename C::ColPlan{});
+  __syncthreads();
+  float mean[2] = {0.f, 0.f}, sd[2] = {1.f, 1.f};
+  if (stats) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int o = min(2 * jp + h, n - 1) * stride + g;
+      mean[h] = stats[2 * o];
+      const float var = stats[2 * o + 1];
+      sd[h] = var != 0 ? sqrtf(var) : 1.f;
+    }
+  }
+  // output column kx from LDS column cu and its mirror (row -ky) from LDS column cw
+  auto emit = [&](int kx, int cu, int cw, int ky) {
+    const float2 u = buf[cu * LP + ky], w = buf[cw * LP + (ky ? L - ky : 0)];
+    const float2 v[2] = {make_float2(0.5f * (u.x + w.x), 0.5f * (u.y - w.y)),
+                         make_float2(0.5f * (u.y + w.y), 0.5f * (w.x - u.x))};
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int i = 2 * jp + h;
+      if (i >= n) break;
+      float2 x = v[h];
+      if (kx == 0 && ky == 0 && mean[h] != 0) x.x -= mean[h] * (float)((long long)L * L);
+      x.x = x.x / sd[h] * scale;
+      x.y = x.y / sd[h] * scale;
+      out[((long long)(i * stride + g) * HX + kx) * L + ky] = x;
+    }
+  };
+  for (int e = tid; e < HC * L; e += 256) {
+    const int jc = e / L, ky = e - jc * L, kx = kx0 + jc;
+    emit(kx, jc, kx == 0 ? 0 : HC + jc, ky);
+  }
+  if (kx0 == 0)  // column L/2, held in LDS column HC of group 0
+    for (int ky = tid; ky < L; ky += 256) emit(L / 2, HC, HC, ky);
+}
+
 template <int L>
 void plane_fft2_impl(const float2* in, float2* data, int batch, int sign, float scale, const float2* tw,
                      int band, int clo, int chi, hipStream_t s) {
@@ -1358,6 +1441,38 @@ bool launch_plane_fft2_from(const float2* in, float2* out, int batch, int L, int
 
 bool launch_plane_fft2(float2* data, int batch, int L, int sign, float scale, hipStream_t s) {
   return launch_plane_fft2_from(data, data, batch, L, sign, scale, s);
+}
+
+namespace {
+template <int L>
+void plane_fft2_pairs_impl(const float2* in, float2* work, int npairs, int n, int stride, const float* stats,
+                           float scale, float2* out, const float2* tw, hipStream_t s) {
+  using C = CorrCfg<L>;
+  constexpr int RPB = 4 * (64 / C::TPC), HC = 256 / C::TPC / 2;
+  const long long nrows = (long long)npairs * L;
+  const long long gr = (nrows + RPB - 1) / RPB;
+  GS_REQUIRE(npairs > 0 && gr < (1LL << 31) && npairs < 65536 && stride >= 1 &&
+                 (long long)((n + 1) / 2) * stride == npairs,
+             kInvalid, "pair transform batch out of range");
+  k_plane_rows<L, -1, true><<<(unsigned)gr, 256, 0, s>>>(in, work, nrows, tw, 1.f, L / 2);
+  k_plane_cols_pairs<L><<<dim3((unsigned)(L / 2 / HC), (unsigned)npairs), 256, 0, s>>>(work, tw, n, stride, stats,
+                                                                                        scale, out);
+  GS_CHECK_LAUNCH();
+}
+}  // namespace
+
+bool launch_plane_fft2_pairs(const float2* in, float2* work, int npairs, int L, int n, int stride,
+                             const float* stats, float scale, float2* out, hipStream_t s) {
+  TwTables& T = tw_tables(current_device());
+  if (L == 320) {
+    plane_fft2_pairs_impl<320>(in, work, npairs, n, stride, stats, scale, out, T.t320, s);
+    return true;
+  }
+  if (L == 720) {
+    plane_fft2_pairs_impl<720>(in, work, npairs, n, stride, stats, scale, out, T.t720, s);
+    return true;
+  }
+  return false;
 }
 
 const float2* plane_twiddles(int L) {

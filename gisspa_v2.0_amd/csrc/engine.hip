@@ -1060,8 +1060,11 @@ class Engine {
         launch_rotation_stats(part, nt, l, stats, stream_);
         // one forward transform per pair; the split, the standardisation (affine: applied to the
         // spectra) and the 1/l^2 scale follow in one pass that writes the half spectra
-        fft2_planes(work, tscr, (nt + 1) / 2, -1, 1.f);
-        launch_unpack_pair_spectra(work, nt, 1, l, stats, (float)(1.0 / (double)L2), that, stream_);
+        if (!launch_plane_fft2_pairs(work, work, (nt + 1) / 2, l, nt, 1, stats, (float)(1.0 / (double)L2), that,
+                                     stream_)) {
+          fft2_planes(work, tscr, (nt + 1) / 2, -1, 1.f);
+          launch_unpack_pair_spectra(work, nt, 1, l, stats, (float)(1.0 / (double)L2), that, stream_);
+        }
         // correlation + per-pixel statistics (ComputeCCGSum / PickParticles in one pass)
         for (int t0 = 0; t0 < nt; t0 += tc_max, ++chunk) {
           const int tc = std::min(tc_max, nt - t0);
@@ -1160,8 +1163,11 @@ class Engine {
       const float sine = (float)std::sin((double)angles[a] * kRefPi / 180);
       launch_rotate_mask_pairs(d_tprep_.p, nt, l, cose, sine, P_.diameter, 4.0f, work, part, stream_);
       launch_rotation_stats(part, nt, l, stats, stream_);
-      fft2_planes(work, tscr, (nt + 1) / 2, -1, 1.f);
-      launch_unpack_pair_spectra(work, nt, 1, l, stats, (float)(1.0 / (double)L2), that, stream_);
+      if (!launch_plane_fft2_pairs(work, work, (nt + 1) / 2, l, nt, 1, stats, (float)(1.0 / (double)L2), that,
+                                   stream_)) {
+        fft2_planes(work, tscr, (nt + 1) / 2, -1, 1.f);
+        launch_unpack_pair_spectra(work, nt, 1, l, stats, (float)(1.0 / (double)L2), that, stream_);
+      }
       for (int t0 = 0; t0 < nt; t0 += tc2) {
         const CorrArgs ca{that, ihat, P2, t0, std::min(tc2, nt - t0), nb_};
         launch_corr_generic_records(fft_, ca, l, P2, W2, d_mu_.p, d_sigma_.p, a, P_.threshold, doffx, doffy, bx_,
@@ -1309,8 +1315,10 @@ class Engine {
       // rotate_subIMG of every block for angles a0..a0+ag (k:534-603), two angles per plane, and
       // the forward transforms: half spectra [angle][block] scaled by 1/l^2 (src/nonorm.cu:457-470)
       launch_rotate_real_pairs(blkr, B, l, dcs + a0, ag, rotb, stream_);
-      fft2_planes(rotb, S, np2, -1, 1.f);
-      launch_unpack_pair_spectra(rotb, ag, B, l, nullptr, (float)(1.0 / (double)L2), ihat, stream_);
+      if (!launch_plane_fft2_pairs(rotb, rotb, np2, l, ag, B, nullptr, (float)(1.0 / (double)L2), ihat, stream_)) {
+        fft2_planes(rotb, S, np2, -1, 1.f);
+        launch_unpack_pair_spectra(rotb, ag, B, l, nullptr, (float)(1.0 / (double)L2), ihat, stream_);
+      }
       for (int t0 = 0; t0 < n; t0 += tc_max) {
         const int tc = std::min(tc_max, n - t0);
         CorrArgs ca{d_that_.p, ihat, Z, t0, tc, nbp};

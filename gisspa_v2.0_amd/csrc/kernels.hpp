@@ -153,6 +153,13 @@ bool launch_plane_fft2(float2* data, int batch, int L, int sign, float scale, hi
 // the groups of columns outside).
 bool launch_plane_fft2_from(const float2* in, float2* out, int batch, int L, int sign, float scale, hipStream_t s,
                             int band = -1, int clo = 0, int chi = -1);
+// Forward transforms of npairs real-pair planes `in` (items 2j and 2j + 1 of group g in the real
+// and imaginary parts of plane j * stride + g; npairs = ceil(n / 2) * stride) split into the half
+// spectra out[i * stride + g][kx 0..L/2][ky] of the n items per group, times scale and, with
+// stats (mean, variance per item), standardised as by launch_unpack_pair_spectra.  work holds
+// npairs planes (may be `in`).  L = 320, 720 (returns false otherwise).
+bool launch_plane_fft2_pairs(const float2* in, float2* work, int npairs, int L, int n, int stride,
+                             const float* stats, float scale, float2* out, hipStream_t s);
 // The flat twiddle table exp(-2 pi i e / L) of the plane transforms on the current device
 // (L = 320, 720; nullptr otherwise).
 const float2* plane_twiddles(int L);
