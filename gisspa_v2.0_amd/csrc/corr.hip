@@ -48,7 +48,7 @@ template <>
 struct CorrCfg<720> {
   static constexpr int H = 360;
   static constexpr int COLS = 8, CT = 256, TPC = 32;
-  using ColPlan = ct::RadixList<4, 4, 5, 9>;
+  using ColPlan = ct::RadixList<10, 6, 12>;
   static constexpr int ROWS = 4, TPI = 2, RT = 360, TPT = 45;
   using RowPlan = ct::RadixList<4, 9, 10>;
 };
@@ -688,19 +688,20 @@ __global__ void __launch_bounds__(CorrCfg<L>::RT)
 }
 
 // Column pass of the norm_type=0 correlation at l = 720, one column per wavefront: workgroup
-// (template, block, group of 8 columns) with 8 waves; each wave forms the product of its
-// spectrum column (column 0 packs the Nyquist column as its imaginary part), transforms it in
-// its own LDS slice (64 lanes, radix 4 * 4 * 5 * 9) and the workgroup writes the group's 8
-// columns row-shifted by l/2 into the column-grouped inter-pass layout
-// Zg[t][b][group][y][8], one contiguous 46 KB region per workgroup.  Against 8 columns over 4
-// waves of 32 lanes (46 KB of LDS per 4 waves, 64-byte row segments scattered over the plane)
-// this holds twice the wavefronts per CU and writes whole lines.
+// (block, group of 8 columns) with 8 waves loops over the templates of the launch.  Each wave
+// forms the product of its spectrum column with the template's (column 0 packs the Nyquist
+// column as its imaginary part), transforms it in its own LDS slice (64 lanes, radix 10 * 6 * 12)
+// and the workgroup writes the group's 8 columns row-shifted by l/2 into the column-grouped
+// inter-pass layout Zg[t][b][group][y][8], one contiguous 46 KB region per plane.  The block's
+// columns are loaded once per workgroup and the next template's columns are loaded while the
+// current one is transformed and written: with one column per wave the LDS holds at most 28
+// columns per CU, too few waves to hide the loads' latency otherwise.
 template <int L>
 struct ColGroupCfg;
 template <>
 struct ColGroupCfg<720> {
   static constexpr int H = 360, COLS = 8;
-  using Plan = ct::RadixList<4, 4, 5, 9>;
+  using Plan = ct::RadixList<10, 6, 12>;
 };
 
 template <int L>
@@ -715,65 +716,69 @@ __global__ void __launch_bounds__(64 * ColGroupCfg<L>::COLS)
   static_assert(H % COLS == 0 && COLS % 2 == 0, "column groups");
   __shared__ float2 buf[COLS * L + L];
   float2* const tw = buf + COLS * L;
-  // XCD-aware order: each XCD takes a contiguous range of (group, block, template), templates
-  // fastest, so an XCD's L2 holds the group's spectrum columns of the launch
+  // XCD-aware order: each XCD takes a contiguous range of (block, group), blocks fastest, so an
+  // XCD's L2 holds the group's template columns while the blocks stream past
   const unsigned G = gridDim.x, bid = blockIdx.x;
   const unsigned xcd = bid % 8u, idx = bid / 8u, per = G / 8u, rem = G % 8u;
   const unsigned lg = (xcd < rem ? xcd * (per + 1u) : rem * (per + 1u) + (xcd - rem) * per) + idx;
-  const int tl = (int)(lg % (unsigned)tc);
-  const int b = (int)((lg / (unsigned)tc) % (unsigned)nb);
-  const int cg = (int)(lg / ((unsigned)tc * (unsigned)nb));
+  const int b = (int)(lg % (unsigned)nb);
+  const int cg = (int)(lg / (unsigned)nb);
   const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
   const int kx = cg * COLS + wave;
-  const float2* Tp = that + (long long)(t0 + tl) * PLANE;
+  const bool dc = kx == 0;  // column 0 carries the Nyquist column as its imaginary part
   const float2* Ip = ihat + (long long)b * PLANE;
-  float2 tv[NLD], iv[NLD], tn[NLD], in_[NLD];
+  // registers hold the block's column and the template's (the next template's once consumed);
+  // the Nyquist columns of the one wave holding column 0 are read when used, so that no wave
+  // reserves registers for them
+  float2 iv[NLD], tv[NLD];
+  auto load_t = [&](int tl) {
+    const float2* Tp = that + (long long)(t0 + tl) * PLANE;
+#pragma unroll
+    for (int i = 0; i < NLD; ++i) {
+      const int ky = lane + 64 * i;
+      if (L % 64 == 0 || ky < L) tv[i] = Tp[(long long)kx * L + ky];
+    }
+  };
 #pragma unroll
   for (int i = 0; i < NLD; ++i) {
     const int ky = lane + 64 * i;
-    if (L % 64 == 0 || ky < L) {
-      tv[i] = Tp[(long long)kx * L + ky];
-      iv[i] = Ip[(long long)kx * L + ky];
-    }
+    if (L % 64 == 0 || ky < L) iv[i] = Ip[(long long)kx * L + ky];
   }
-  const bool dc = kx == 0;  // column 0 carries the Nyquist column as its imaginary part
-  if (dc) {
+  load_t(0);
+  for (int i = tid; i < L; i += NT) tw[i] = twL[i];
+  float2* const col = buf + wave * L;
+  for (int tl = 0; tl < tc; ++tl) {
+    if (tl > 0) __syncthreads();  // the previous template's write-out has read every column
 #pragma unroll
     for (int i = 0; i < NLD; ++i) {
       const int ky = lane + 64 * i;
       if (L % 64 == 0 || ky < L) {
-        tn[i] = Tp[(long long)H * L + ky];
-        in_[i] = Ip[(long long)H * L + ky];
+        float2 p = c_mulconj(iv[i], tv[i]);
+        if (dc) {
+          const float2 pn = c_mulconj(Ip[(long long)H * L + ky], that[(t0 + tl) * PLANE + (long long)H * L + ky]);
+          p = make_float2(p.x - pn.y, p.y + pn.x);
+        }
+        col[ky] = p;
       }
     }
-  }
-  for (int i = tid; i < L; i += NT) tw[i] = twL[i];
-  float2* const col = buf + wave * L;
+    if (tl + 1 < tc) load_t(tl + 1);  // in flight during this template's transform and write-out
+    if (tl == 0)
+      __syncthreads();  // the twiddle table
+    else
+      ct::lds_sync<ct::Sync::WAVE>();
+    ct::run_stockham<L, 64, +1, ct::Sync::WAVE>(col, tw, lane, typename C::Plan{});
+    __syncthreads();  // rows of the group gather all its columns
+    float4* const Zg = reinterpret_cast<float4*>(Z + (((long long)tl * nb + b) * NCG + cg) * (long long)L * COLS);
+    constexpr int NQ = L * COLS / 2;  // 16-byte words of the group's region
 #pragma unroll
-  for (int i = 0; i < NLD; ++i) {
-    const int ky = lane + 64 * i;
-    if (L % 64 == 0 || ky < L) {
-      float2 p = c_mulconj(iv[i], tv[i]);
-      if (dc) {
-        const float2 pn = c_mulconj(in_[i], tn[i]);
-        p = make_float2(p.x - pn.y, p.y + pn.x);
+    for (int e0 = 0; e0 < NQ; e0 += NT) {
+      const int e = e0 + tid;
+      if (NQ % NT == 0 || e < NQ) {
+        const int yd = e / (COLS / 2), cp = e % (COLS / 2);
+        const int y = yd < L / 2 ? yd + L / 2 : yd - L / 2;  // output row yd holds transform row y
+        const float2 v0 = buf[(2 * cp) * L + y], v1 = buf[(2 * cp + 1) * L + y];
+        st_stream(Zg + e, make_float4(v0.x, v0.y, v1.x, v1.y));
       }
-      col[ky] = p;
-    }
-  }
-  __syncthreads();  // the twiddle table
-  ct::run_stockham<L, 64, +1, ct::Sync::WAVE>(col, tw, lane, typename C::Plan{});
-  __syncthreads();  // rows of the group gather all its columns
-  float4* const Zg = reinterpret_cast<float4*>(Z + (((long long)tl * nb + b) * NCG + cg) * (long long)L * COLS);
-  constexpr int NQ = L * COLS / 2;  // 16-byte words of the group's region
-#pragma unroll
-  for (int e0 = 0; e0 < NQ; e0 += NT) {
-    const int e = e0 + tid;
-    if (NQ % NT == 0 || e < NQ) {
-      const int yd = e / (COLS / 2), cp = e % (COLS / 2);
-      const int y = yd < L / 2 ? yd + L / 2 : yd - L / 2;  // output row yd holds transform row y
-      const float2 v0 = buf[(2 * cp) * L + y], v1 = buf[(2 * cp + 1) * L + y];
-      st_stream(Zg + e, make_float4(v0.x, v0.y, v1.x, v1.y));
     }
   }
 }
@@ -1533,7 +1538,7 @@ bool launch_colpass_plane(int L, const CorrArgs& a, int variant, hipStream_t s) 
   if (L != 720) return launch_colpass(L, a, variant, s);
   TwTables& T = tw_tables(current_device());
   using C = ColGroupCfg<720>;
-  const long long G = (long long)(C::H / C::COLS) * a.tc * a.nb;
+  const long long G = (long long)(C::H / C::COLS) * a.nb;
   GS_REQUIRE(G > 0 && G < (1LL << 31), kInvalid, "column pass grid out of range");
   k_colpass_grouped<720><<<(unsigned)G, 64 * C::COLS, 0, s>>>(a.that, a.ihat, T.t720, a.Z, a.t0, a.tc, a.nb);
   GS_CHECK_LAUNCH();
