@@ -460,8 +460,8 @@ def c0_main(a):
 def main():
     a = parse()
     if a.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported in this build "
-              "(the search shards by micrograph or block with no exchange; see DESIGN.md)",
+        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported "
+              "(the search shards by micrograph or block with no exchange between GPUs)",
               file=sys.stderr)
         sys.exit(2)
     if a.batch:

@@ -92,7 +92,7 @@ void launch_unpack_pair_spectra(const float2* Z, int n, int stride, int l, const
 void launch_rotate_real_pairs(const float* in, int nb, int l, const float2* cs, int na, float2* out,
                               hipStream_t s);
 
-// ---- correlation (north-star kernels b, c, d, e) ----
+// ---- correlation: product, inverse column and row transforms, per-pixel or per-plane reductions ----
 struct CorrArgs {
   const float2* that;  // [>=t0+tc][H+1][L] template spectra
   const float2* ihat;  // [nb][H+1][L] image spectra

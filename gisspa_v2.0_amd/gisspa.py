@@ -136,7 +136,7 @@ def load(path: str = LIB_PATH):
     if _lib is not None:
         return _lib
     if not os.path.exists(path):
-        raise OSError(f"{path} not built: run __graft_entry__.build() or make -C gisspa_v2.0_amd")
+        raise OSError(f"{path} not built: run make -C gisspa_v2.0_amd")
     lib = C.CDLL(path)
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)

@@ -28,7 +28,7 @@ typedef enum {
   GS_OK = 0,
   GS_ERR_INVALID = 1,     /* bad argument or configuration */
   GS_ERR_HIP = 2,         /* a HIP runtime call failed */
-  GS_ERR_UNSUPPORTED = 3, /* valid request this build does not handle */
+  GS_ERR_UNSUPPORTED = 3, /* valid request the library does not handle (e.g. an unsupported size) */
   GS_ERR_NOMEM = 4,       /* device or host allocation failed */
   GS_ERR_STATE = 5        /* call out of order (e.g. search before templates) */
 } gs_status;

@@ -1,6 +1,6 @@
 """GPU parity of libgisspa.so against the FP64 oracle (through the C ABI).
 
-Tolerances (north star): picks identical in (x, y, template, angle) above threshold, scores
+Tolerances: picks identical in (x, y, template, angle) above threshold, scores
 within 1e-4 relative (FP32 path vs FP64 oracle).  Intermediate stages are compared with
 relative L2 errors stated per test.
 """
