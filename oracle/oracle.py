@@ -1,8 +1,8 @@
 """ctypes binding of the FP64 oracle (oracle/lib/libgisspa_oracle.so).
 
-TEST INFRASTRUCTURE ONLY: imported by tests/, by the smoke check and by the CPU-baseline leg of
-bench.py, always to check results, never as the thing measured.  The product path (libgisspa.so)
-never loads it.
+TEST INFRASTRUCTURE ONLY: imported by tests/ and the smoke check (to check the GPU path's
+results) and by the CPU-baseline leg of bench.py (timed as the CPU reference point).  The product
+path (libgisspa.so) never loads it.
 """
 from __future__ import annotations
 
