@@ -123,6 +123,10 @@ struct ColStageMaps {
 //     transform (zpack2 above) to each pair and stores the two packed values next to each other:
 //     Zp[..][y][2j + 0/1] = packed column j / H - j (H/2 for j = 0).  The row pass then starts
 //     from packed rows.
+//  5: as 4 with every twiddle of the lane (22 for the 4*4*20 plan: they depend only on the lane's
+//     place in its column) loaded from the global table into registers with the spectra, instead
+//     of formed by recurrence in every stage; more registers, which the LDS-bound occupancy (four
+//     workgroups of four wavefronts per CU) leaves unused.
 // Z is written with streaming stores: it is read back only by the next kernel, and keeping it
 // out of L2 keeps the spectra columns that neighbouring workgroups are still reading there.
 // BT x TT: tile of the logical order (blocks x templates).  At 512 templates per launch, tiles
@@ -134,7 +138,8 @@ __global__ void __launch_bounds__(CorrCfg<L>::CT)
               const float2* __restrict__ twL, float2* __restrict__ Z, int t0, int tc, int nb) {
   using C = CorrCfg<L>;
   using P = typename ColPlanOf<L, Plan>::type;
-  constexpr bool BK = LAY == 1, CM = LAY >= 2, RC = LAY >= 3, PK = LAY == 4;
+  constexpr bool BK = LAY == 1, CM = LAY >= 2, RC = LAY == 3 || LAY == 4, PK = LAY == 4 || LAY == 5;
+  constexpr bool LW = LAY == 5;
   constexpr int H = C::H, COLS = C::COLS, NT = C::CT, TPC = C::TPC;
   constexpr int LP = BK ? L + 17 : (CM ? L : L + 1);
   static_assert(COLS * TPC == NT, "column pass thread mapping");
@@ -143,8 +148,8 @@ __global__ void __launch_bounds__(CorrCfg<L>::CT)
   static_assert(!BK || ct::stage_twiddle_count(P{}) <= L, "per-stage twiddles fit the table slot");
   // twiddles: each lane's own in registers where the plan allows it (BK), else a table in LDS
   // (per-stage layout for BK, flat otherwise), or the global table (CM)
-  constexpr bool LT = (BK || RC) && ct::lane_twiddles_fit<TPC>(P{});
-  static_assert(!RC || LT, "recurrence twiddles need T % NS == 0");
+  constexpr bool LT = (BK || RC || LW) && ct::lane_twiddles_fit<TPC>(P{});
+  static_assert(!(RC || LW) || LT, "per-lane twiddles need T % NS == 0");
   constexpr ct::TwSrc TS = RC ? ct::TwSrc::RECUR : (LT ? ct::TwSrc::LANE : (BK ? ct::TwSrc::STAGE : ct::TwSrc::FLAT));
   constexpr long long PLANE = (long long)(H + 1) * L;
   __shared__ float2 smem[COLS * LP + (LT || CM ? 0 : L)];
@@ -252,10 +257,10 @@ __global__ void __launch_bounds__(CorrCfg<L>::CT)
   const int col = tid / TPC;
   if constexpr (BK)
     ct::run_stockham<L, TPC, +1, ct::Sync::WAVE, ct::Swz16, TS>(buf + col * LP, tw, tid % TPC, P{});
-  else if constexpr (RC) {
+  else if constexpr (RC || LW) {
     static_assert(std::is_same_v<P, ct::RadixList<4, 4, 20>> && TPC == 16, "stage layouts of the 4*4*20 plan");
-    ct::run_stockham_maps<L, TPC, +1, ct::Sync::WAVE, ct::TwSrc::RECUR>(buf + col * LP, tw, tid % TPC, P{},
-                                                                         ColStageMaps{cmask(col)});
+    ct::run_stockham_maps<L, TPC, +1, ct::Sync::WAVE, LW ? ct::TwSrc::LANE : ct::TwSrc::RECUR>(
+        buf + col * LP, tw, tid % TPC, P{}, ColStageMaps{cmask(col)});
   }
   else if constexpr (CM)
     ct::run_stockham_fm<L, TPC, +1, ct::Sync::WAVE, ct::IdentityMap, ct::LowXor16, ct::TwSrc::FLAT>(
@@ -1512,9 +1517,12 @@ static void colpass_impl(const CorrArgs& a, const float2* twL, int variant, hipS
       case 6: colpass_tiled<L, 1, 2, 8, 16>(a, twL, s); break;
       case 7:  // recurrence twiddles need 16 lanes per column and a plan with T % NS == 0 (l = 320)
       case 8:  // the same with the row-packed layout (the row pass must be variant 7)
+      case 10:  // 8 with the lane twiddles loaded into registers (the row pass must be variant 7)
         if constexpr (ct::lane_twiddles_fit<CorrCfg<L>::TPC>(typename ColPlanOf<L, 1>::type{}) &&
                       CorrCfg<L>::COLS == 16) {
-          if (variant == 8)
+          if (variant == 10)
+            colpass_tiled<L, 1, 5, 8, 16>(a, twL, s);
+          else if (variant == 8)
             colpass_tiled<L, 1, 4, 8, 16>(a, twL, s);
           else
             colpass_tiled<L, 1, 3, 8, 16>(a, twL, s);

@@ -487,7 +487,7 @@ class Engine {
   }
   void set_chunk(int tc) { chunk_ = tc; }
   void set_variant(int kernel, int variant) {
-    GS_REQUIRE((kernel == 0 && variant >= 0 && variant <= 9) || (kernel == 1 && variant >= 0 && variant <= 7) ||
+    GS_REQUIRE((kernel == 0 && variant >= 0 && variant <= 10) || (kernel == 1 && variant >= 0 && variant <= 7) ||
                    (kernel == 2 && (variant == 0 || variant == 1)),
                kInvalid, "unknown kernel variant");
     if (kernel == 0)
@@ -1034,10 +1034,11 @@ class Engine {
       ensure_pipeline();
       rs = stream2_;
     }
-    // column variant 8 writes the row-packed layout that only row variant 7 reads (l = 320; other
-    // sizes use one unpacked layout)
-    GS_REQUIRE(l != 320 || (col_variant_ == 8 || col_variant_ == 9) == (row_variant_ == 7), kInvalid,
-               "column variant 8 (row-packed inter-pass layout) and row variant 7 go together");
+    // column variants 8, 9 and 10 write the row-packed layout that only row variant 7 reads
+    // (l = 320; other sizes use one unpacked layout)
+    const bool packed_col = col_variant_ == 8 || col_variant_ == 9 || col_variant_ == 10;
+    GS_REQUIRE(l != 320 || packed_col == (row_variant_ == 7), kInvalid,
+               "column variants 8, 9, 10 (row-packed inter-pass layout) and row variant 7 go together");
     float2* Zb = d_Z_.ensure(zsz * (overlap ? 2 : 1));
     double* part = d_part_.ensure((size_t)n * rotation_groups(l) * 2);
     float* stats = d_stats_.ensure((size_t)n * 2);

@@ -31,6 +31,7 @@ def _key(a):
     (7, 6, 0, 0),
     (8, 7, 0, 0), (8, 7, 4, 0),  # row-packed inter-pass layout (also with a partial last launch)
     (9, 7, 0, 0), (9, 7, 4, 0),  # the same with 8 templates per workgroup (one partial chunk)
+    (10, 7, 0, 0), (10, 7, 4, 0),  # row-packed, lane twiddles loaded into registers
     (0, 1, 0, 0), (0, 2, 0, 0), (0, 3, 0, 0), (0, 4, 0, 0), (0, 5, 0, 0), (0, 6, 0, 0),
     (3, 6, 4, 0),   # eight templates per iteration with a partial last iteration (4 + 2 templates)
     (3, 1, 4, 0),   # two launches per angle, the second partial (4 + 2 templates)
