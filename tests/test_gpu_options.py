@@ -71,24 +71,27 @@ def test_nonorm_overlap(gs, orc):
     eng.close()
 
 
-@pytest.mark.parametrize("l,phistep,W,H", [(128, 40.0, 2000, 1500), (720, 40.0, 1440, 720)])
-def test_nonorm_tiles_batched(gs, orc, l, phistep, W, H):
+@pytest.mark.parametrize("l,phistep,W,H,pf,inv", [(128, 40.0, 2000, 1500, 1, 0), (720, 40.0, 1440, 720, 1, 0),
+                                                   (720, 30.0, 1440, 720, 0, 1)])
+def test_nonorm_tiles_batched(gs, orc, l, phistep, W, H, pf, inv):
     """All tiles of a micrograph are searched together: the blocks of every tile form one batch
     for the normalisation, rotation and correlation, and the rotated copies are made two angles
     per complex plane.  Nine angles leave the last angle pair half empty.  At window 128 the
     2000 x 1500 micrograph has nine overlapping tiles of 36 blocks (324 blocks), so the angles
-    run in groups of six and three; at window 720 two tiles of one block."""
+    run in groups of six and three; at window 720 two tiles of one block, also without the phase
+    flip (raw tiles split into blocks) and with inverted contrast."""
     import synth
 
-    print(f"norm_type=0 window {l}, {W}x{H}, phistep {phistep}", flush=True)
-    cfg = gs.SearchConfig(phistep=phistep, window_size=l, norm_type=0, phase_flip=1, threshold=3.0,
+    print(f"norm_type=0 window {l}, {W}x{H}, phistep {phistep}, phase_flip {pf}, invert {inv}", flush=True)
+    cfg = gs.SearchConfig(phistep=phistep, window_size=l, norm_type=0, phase_flip=pf, invert=inv, threshold=3.0,
                           diameter=64.0 if l == 128 else 128.0, nms_radius=0.0)
     ds = synth.make_dataset(9, W, H, n_templates=3, tsize=64 if l == 128 else 128, n_particles=6, snr=0.5,
                             defocus=1.5)
-    _, orecs = orc.search_nonorm(cfg, ds.image, ds.templates, ds.ctf)
+    image = -ds.image if inv else ds.image
+    _, orecs = orc.search_nonorm(cfg, image, ds.templates, ds.ctf)
     eng = gs.Engine(cfg)
     eng.load_templates(ds.templates)
-    g = eng.search(ds.image, ds.ctf)
+    g = eng.search(image, ds.ctf)
     assert len(orecs) > 0
     key = lambda p: (int(p["block"]), int(p["angle_index"]), int(p["template_id"]))
     assert len({int(p["block"]) for p in orecs}) > 1  # records from more than one tile
