@@ -100,6 +100,52 @@ __global__ void k_fft_stage(const float2* __restrict__ src, float2* __restrict__
   }
 }
 
+// A stage of any other radix R (the primes 11, 31, 53, 101 ... of frames such as 4092, 3710 or
+// 3838 px) as direct sums, one thread per output: a workgroup takes NB = 256 / R butterflies
+// (consecutive in the row-major (row, j) order), stages their R pre-twiddled inputs in LDS
+// (each input read and twiddled once) with the table W_R^k, and thread (b, k) forms output k of
+// butterfly b as the sum over r in order of A[b][r] W_R^(r k mod R).  The same arithmetic, term
+// for term, as one thread per butterfly, with R times the parallelism and R times fewer reads.
+template <int SIGN>
+__global__ void __launch_bounds__(256)
+    k_fft_stage_direct(const float2* __restrict__ src, float2* __restrict__ dst, const float2* __restrict__ tw,
+                       int n, int R, int Ns, long long nbfly, float scale) {
+  extern __shared__ float2 sm[];
+  const int NB = 256 / R;
+  float2* const A = sm;           // [NB][R]
+  float2* const WR = sm + NB * R;  // W_R^k, k < R
+  const int nb = n / R, step = n / (Ns * R), nR = n / R;
+  const long long b0 = (long long)blockIdx.x * NB;
+  const int tid = threadIdx.x;
+  for (int k = tid; k < R; k += 256) WR[k] = twv<SIGN>(tw, (long long)k * nR);
+  for (int idx = tid; idx < NB * R; idx += 256) {
+    const int bl = idx / R, r = idx - bl * R;
+    const long long g = b0 + bl;
+    if (g >= nbfly) continue;
+    const long long row = g / nb;
+    const int j = (int)(g - row * nb), jm = j % Ns;
+    float2 a = src[row * n + j + (long long)r * nb];
+    if (r > 0 && jm > 0) a = c_mul(a, twv<SIGN>(tw, (long long)r * jm * step));
+    A[idx] = a;
+  }
+  __syncthreads();
+  if (tid >= NB * R) return;  // no barrier follows
+  const int bl = tid / R, k = tid - bl * R;
+  const long long g = b0 + bl;
+  if (g >= nbfly) return;
+  const long long row = g / nb;
+  const int j = (int)(g - row * nb), jm = j % Ns;
+  const float2* Ab = A + bl * R;
+  float2 acc = make_float2(0.f, 0.f);
+  int e = 0;  // r k mod R
+  for (int r = 0; r < R; ++r) {
+    acc = c_add(acc, c_mul(Ab[r], WR[e]));
+    e += k;
+    if (e >= R) e -= R;
+  }
+  dst[row * n + (j / Ns) * Ns * R + jm + (long long)k * Ns] = c_scale(acc, scale);
+}
+
 __global__ void k_copy_scale(const float2* __restrict__ src, float2* __restrict__ dst, long long n,
                              float scale) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -139,10 +185,19 @@ void fft_rows_table(const int* radices, int nrad, const float2* tw, float2* data
     const int R = radices[st];
     const float sc = (st == nrad - 1) ? scale : 1.f;
     const long long work = nrows * (n / R);
-    if (sign < 0)
+    if (R > 7 && R <= 256) {  // direct sums, one thread per output
+      const int NB = 256 / R;
+      const size_t lds = sizeof(float2) * (size_t)(NB * R + R);
+      const unsigned grid = nblk(work, NB);
+      if (sign < 0)
+        k_fft_stage_direct<-1><<<grid, 256, lds, s>>>(src, dst, tw, n, R, Ns, work, sc);
+      else
+        k_fft_stage_direct<+1><<<grid, 256, lds, s>>>(src, dst, tw, n, R, Ns, work, sc);
+    } else if (sign < 0) {
       k_fft_stage<-1><<<nblk(work, 256), 256, 0, s>>>(src, dst, tw, n, R, Ns, nrows, sc);
-    else
+    } else {
       k_fft_stage<+1><<<nblk(work, 256), 256, 0, s>>>(src, dst, tw, n, R, Ns, nrows, sc);
+    }
     GS_CHECK_LAUNCH();
     std::swap(src, dst);
     Ns *= R;
