@@ -84,7 +84,7 @@ __device__ __forceinline__ double block_sum(double v) {
 }
 
 // Output tile of the template rotations: kRotTX x kRotTY pixels per workgroup of 256 threads, two
-// pixels per thread (rows y and y + 8).  A compact tile keeps the rotated source footprint of a
+// pixels per thread (each wavefront a 16 x 8 part of the tile, as two 8 x 8 squares).  A compact tile keeps the rotated source footprint of a
 // workgroup compact at every angle (a full-row strip fans out over many source rows near 45
 // degrees).
 constexpr int kRotTX = 32, kRotTY = 16;

@@ -456,11 +456,15 @@ __global__ void __launch_bounds__(256)
   const bool hb = tb < n;
   const float* A = tp + (long long)ta * l * l;
   const float* B = tp + (long long)(hb ? tb : ta) * l * l;
-  const int i = (g % gx) * kRotTX + threadIdx.x % kRotTX;
+  // each wavefront covers a 16 x 8 part of the tile as two 8 x 8 squares, so that the bilinear
+  // taps of one load instruction fall on a compact source patch at every angle
+  static_assert(kRotTX == 32 && kRotTY == 16, "wavefront squares");
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int j = (g / gx) * kRotTY + (wave >> 1) * 8 + (lane >> 3);
   double sa = 0, sa2 = 0, sb = 0, sb2 = 0;
 #pragma unroll
-  for (int h = 0; h < kRotTY / 8; ++h) {
-    const int j = (g / gx) * kRotTY + threadIdx.x / kRotTX + 8 * h;
+  for (int h = 0; h < 2; ++h) {
+    const int i = (g % gx) * kRotTX + (wave & 1) * 16 + h * 8 + (lane & 7);
     if (i >= l || j >= l) continue;
     float ra = 0.f, rb = 0.f;
     const float r = hypotf((float)(i - l / 2), (float)(j - l / 2));
