@@ -1625,10 +1625,24 @@ gs_status gs_fft2_c2c(int device, float* data, int ny, int nx, int batch, int si
 }
 
 namespace {
+// Streaming copy: four 16-byte loads in flight per thread, non-temporal loads and stores (the
+// fastest of the plain, unrolled and non-temporal forms measured on MI355X at 1 and 4 GiB).
+constexpr int kCopyUnroll = 4, kCopyGrid = 16384;
 __global__ void k_stream_copy(const float4* __restrict__ a, float4* __restrict__ b, long long n) {
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (long long)gridDim.x * blockDim.x)
-    b[i] = a[i];
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i0 = (long long)blockIdx.x * blockDim.x + threadIdx.x; i0 < n; i0 += stride * kCopyUnroll) {
+    float4 v[kCopyUnroll];
+#pragma unroll
+    for (int u = 0; u < kCopyUnroll; ++u) {
+      const long long i = i0 + u * stride;
+      if (i < n) v[u] = ld_stream(a + i);
+    }
+#pragma unroll
+    for (int u = 0; u < kCopyUnroll; ++u) {
+      const long long i = i0 + u * stride;
+      if (i < n) st_stream(b + i, v[u]);
+    }
+  }
 }
 }  // namespace
 
@@ -1644,10 +1658,10 @@ gs_status gs_copy_bandwidth(int device, size_t bytes, int reps, double* gbps) {
     hipEvent_t e0, e1;
     GS_HIP(hipEventCreate(&e0));
     GS_HIP(hipEventCreate(&e1));
-    k_stream_copy<<<4096, 256>>>(a.p, b.p, n);
+    k_stream_copy<<<kCopyGrid, 256>>>(a.p, b.p, n);
     GS_CHECK_LAUNCH();
     GS_HIP(hipEventRecord(e0, nullptr));
-    for (int r = 0; r < reps; ++r) k_stream_copy<<<4096, 256>>>(a.p, b.p, n);
+    for (int r = 0; r < reps; ++r) k_stream_copy<<<kCopyGrid, 256>>>(a.p, b.p, n);
     GS_CHECK_LAUNCH();
     GS_HIP(hipEventRecord(e1, nullptr));
     GS_HIP(hipEventSynchronize(e1));
