@@ -1434,7 +1434,7 @@ class Engine {
   std::vector<gs_record> records_;
   std::vector<double> planes_host_;
   double total_n_ = 0, refine_ms_ = 0;
-  static constexpr size_t kRefineChunk = 1024;  // picks per refinement batch (about 4 MB each at l=320)
+  static constexpr size_t kRefineChunk = 2048;  // picks per refinement batch (about 4 MB each at l=320)
   bool white_ok_ = false;
   DevBuf<float2> d_twhite_, d_rH_, d_rT_, d_rAB_, d_rS_, d_rIT_;
   DevBuf<int32_t> d_ridx_;

@@ -147,23 +147,23 @@ def test_select_reproduces_and_extends_the_records(case):
 
 
 def test_refinement_beyond_one_device_batch(case, orc):
-    """More picks than one device batch of the refinement (1024): the second batch reuses the
+    """More picks than one device batch of the refinement (2048): the second batch reuses the
     buffers of the first with a smaller count, and its picks must come out exactly as a
     separate call on those picks alone, and agree with the oracle on a sample."""
     eng = case["eng"]
     base = _perturbed(case, 6, seed=11)
-    k = 1100
+    k = 2100
     pk = np.concatenate([base] * (k // len(base) + 1))[:k].copy()
     rng = np.random.default_rng(12)
     pk["dx"] += rng.uniform(-0.2, 0.2, k)
     pk["theta"] += rng.uniform(-0.5, 0.5, k)
     out = eng.refine(pk, iterations=3)
-    tail = eng.refine(pk[1024:].copy(), iterations=3)
-    assert np.array_equal(out["score"][1024:], tail["score"])
-    assert np.array_equal(out["peak"][1024:], tail["peak"])
+    tail = eng.refine(pk[2048:].copy(), iterations=3)
+    assert np.array_equal(out["score"][2048:], tail["score"])
+    assert np.array_equal(out["peak"][2048:], tail["peak"])
     head = eng.refine(pk[:8].copy(), iterations=3)
     np.testing.assert_allclose(out["score"][:8], head["score"], rtol=1e-6)
-    idx = np.array([0, 1023, 1024, 1099])
+    idx = np.array([0, 2047, 2048, 2099])
     mu, sg = _frozen(case, pk[idx])
     o, zo, _, _ = case["R"].run(_as_oracle(orc, pk[idx]), mu, sg, iters=3, step=0.1,
                                 scale=(2.0, 2.0, 3.0, 500.0))
