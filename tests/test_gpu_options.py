@@ -71,21 +71,22 @@ def test_nonorm_overlap(gs, orc):
     eng.close()
 
 
-@pytest.mark.parametrize("l,phistep,W,H,pf,inv", [(128, 40.0, 2000, 1500, 1, 0), (720, 40.0, 1440, 720, 1, 0),
-                                                   (720, 30.0, 1440, 720, 0, 1)])
-def test_nonorm_tiles_batched(gs, orc, l, phistep, W, H, pf, inv):
+@pytest.mark.parametrize("l,phistep,W,H,pf,inv,nt", [(128, 40.0, 2000, 1500, 1, 0, 3), (720, 40.0, 1440, 720, 1, 0, 3),
+                                                      (720, 30.0, 1440, 720, 0, 1, 3), (720, 30.0, 1440, 720, 1, 0, 1)])
+def test_nonorm_tiles_batched(gs, orc, l, phistep, W, H, pf, inv, nt):
     """All tiles of a micrograph are searched together: the blocks of every tile form one batch
     for the normalisation, rotation and correlation, and the rotated copies are made two angles
     per complex plane.  Nine angles leave the last angle pair half empty.  At window 128 the
     2000 x 1500 micrograph has nine overlapping tiles of 36 blocks (324 blocks), so the angles
     run in groups of six and three; at window 720 two tiles of one block, also without the phase
-    flip (raw tiles split into blocks) and with inverted contrast."""
+    flip (raw tiles split into blocks) and with inverted contrast, and with a single template
+    (one template per correlation launch)."""
     import synth
 
     print(f"norm_type=0 window {l}, {W}x{H}, phistep {phistep}, phase_flip {pf}, invert {inv}", flush=True)
     cfg = gs.SearchConfig(phistep=phistep, window_size=l, norm_type=0, phase_flip=pf, invert=inv, threshold=3.0,
                           diameter=64.0 if l == 128 else 128.0, nms_radius=0.0)
-    ds = synth.make_dataset(9, W, H, n_templates=3, tsize=64 if l == 128 else 128, n_particles=6, snr=0.5,
+    ds = synth.make_dataset(9, W, H, n_templates=nt, tsize=64 if l == 128 else 128, n_particles=6, snr=0.5,
                             defocus=1.5)
     image = -ds.image if inv else ds.image
     _, orecs = orc.search_nonorm(cfg, image, ds.templates, ds.ctf)
