@@ -171,27 +171,61 @@ def refine_leg(eng, cfg, ds, no_cpu, sample=16):
     return res
 
 
-def batch_items(k, seed=2000):
-    """K uneven micrographs: sizes drawn from {3710x3838, 4096x4096, 5760x4092}, defocus
-    U(0.8, 3.0) um, N_t ~ U{200..1000} templates of a size drawn from {128, 180, 256}, each with
-    its own template set and 100 particles at SNR 0.05."""
-    import synth
-
+def batch_specs(k, seed=2000):
+    """K uneven micrographs as the batching configuration draws them: size from {3710x3838,
+    4096x4096, 5760x4092}, N_t ~ U{200..1000} templates of a size drawn from {128, 180, 256},
+    defocus U(0.8, 3.0) um and U{100..500} particles at SNR 0.05, each with its own template set."""
     rng = np.random.default_rng(seed)
     sizes = [(3710, 3838), (4096, 4096), (5760, 4092)]
-    items, work, desc, dss = [], 0, [], []
+    specs = []
     for i in range(k):
         W, H = sizes[int(rng.integers(3))]
         nt = int(rng.integers(200, 1001))
         ts = int(rng.choice([128, 180, 256]))
         df = float(rng.uniform(0.8, 3.0))
-        ds = synth.make_dataset(1000 + i, W, H, n_templates=nt, tsize=ts, n_particles=100,
-                                snr=0.05, defocus=df)
-        items.append({"image": ds.image, "ctf": ds.ctf, "templates": ds.templates})
-        dss.append(ds)
-        work += nt * 72 * W * H
-        desc.append(f"{W}x{H}/{nt}x{ts}/df{df:.2f}")
-        log(f"micrograph {i}: {desc[-1]}")
+        npart = int(rng.integers(100, 501))
+        specs.append((1000 + i, W, H, nt, ts, df, npart))
+    return specs
+
+
+def make_item(spec):
+    """One micrograph of batch_specs: (item for the search API, dataset, description)."""
+    import synth
+
+    seed, W, H, nt, ts, df, npart = spec
+    ds = synth.make_dataset(seed, W, H, n_templates=nt, tsize=ts, n_particles=npart, snr=0.05, defocus=df)
+    item = {"image": ds.image, "ctf": ds.ctf, "templates": ds.templates}
+    return item, ds, f"{W}x{H}/{nt}x{ts}/df{df:.2f}/{npart}p"
+
+
+def _one_thread():
+    # pool workers: one BLAS/FFT thread each, the pool supplies the parallelism
+    from threadpoolctl import threadpool_limits
+
+    threadpool_limits(1)
+
+
+def batch_items(k, seed=2000, workers=1):
+    """Build the K micrographs of batch_specs, in a process pool when workers > 1 (only before
+    anything has used the GPU: the pool forks this process)."""
+    specs = batch_specs(k, seed)
+    if workers > 1:
+        import multiprocessing as mp
+
+        with mp.get_context("fork").Pool(min(workers, k), initializer=_one_thread) as pool:
+            made = []
+            for i, m in enumerate(pool.imap(make_item, specs)):
+                made.append(m)
+                log(f"micrograph {i}: {m[2]}")
+    else:
+        made = []
+        for i, spec in enumerate(specs):
+            made.append(make_item(spec))
+            log(f"micrograph {i}: {made[-1][2]}")
+    items = [m[0] for m in made]
+    dss = [m[1] for m in made]
+    desc = [m[2] for m in made]
+    work = sum(nt * 72 * W * H for (_, W, H, nt, _, _, _) in specs)
     return items, work, desc, dss
 
 
@@ -277,30 +311,18 @@ def c1_config(gisspa):
 def batch_main(a):
     """Uneven-micrograph batch: K micrographs with sizes drawn from {3710x3838, 4096x4096,
     5760x4092}, defocus U(0.8, 3.0) um, N_t ~ U{200..1000} templates of a size drawn from
-    {128, 180, 256}, each micrograph with its own template set.  Three ways to run them:
+    {128, 180, 256}, U{100..500} particles, each micrograph with its own template set.  Three ways
+    to run them (the second only for K <= 8):
     `per_engine` = the reference's driver loop (an engine created, loaded, run and destroyed per
     micrograph), `one_engine` = one engine and gs_search per micrograph, `batch` =
     gs_search_batch (staging thread + graph replay).  value = correlations/s of `batch`.  The
     batch + refinement form (gs_search_batch_refine, top REFINE_TOP picks per micrograph) is
     compared with the same work run per micrograph (engine, gs_search, gs_refine)."""
-    import gisspa
-    import synth
-
-    rng = np.random.default_rng(2000)
-    sizes = [(3710, 3838), (4096, 4096), (5760, 4092)]
-    items, work, desc = [], 0, []
     t0 = time.perf_counter()
-    for i in range(a.batch):
-        W, H = sizes[int(rng.integers(3))]
-        nt = int(rng.integers(200, 1001))
-        ts = int(rng.choice([128, 180, 256]))
-        df = float(rng.uniform(0.8, 3.0))
-        ds = synth.make_dataset(1000 + i, W, H, n_templates=nt, tsize=ts, n_particles=100,
-                                snr=0.05, defocus=df)
-        items.append({"image": ds.image, "ctf": ds.ctf, "templates": ds.templates})
-        work += nt * 72 * W * H
-        desc.append(f"{W}x{H}/{nt}x{ts}")
-        log(f"micrograph {i}: {desc[-1]}, defocus {df:.2f}")
+    # the micrographs are synthesised in a process pool before this process touches the GPU
+    items, work, desc, _ = batch_items(a.batch, workers=min(16, os.cpu_count() or 1))
+    import gisspa
+
     log(f"batch data ready in {time.perf_counter() - t0:.1f} s")
     cfg = c1_config(gisspa)
     times = {}
@@ -312,20 +334,23 @@ def batch_main(a):
     recs = {}
     t1 = time.perf_counter()
     recs["per_engine"] = []
-    for it in items:
+    for i, it in enumerate(items):
         with gisspa.Engine(cfg) as e:
             e.load_templates(it["templates"])
             recs["per_engine"].append(e.search(it["image"], it["ctf"]))
+        if i % 16 == 15:
+            log(f"per-engine loop: {i + 1} micrographs in {time.perf_counter() - t1:.1f} s")
     times["per_engine"] = time.perf_counter() - t1
     log(f"per-engine loop: {times['per_engine']:.2f} s")
-    t1 = time.perf_counter()
-    recs["one_engine"] = []
-    with gisspa.Engine(cfg) as e:
-        for it in items:
-            e.load_templates(it["templates"])
-            recs["one_engine"].append(e.search(it["image"], it["ctf"]))
-    times["one_engine"] = time.perf_counter() - t1
-    log(f"one-engine loop: {times['one_engine']:.2f} s")
+    if a.batch <= 8:  # a third form for small batches only (it adds a full pass over the work)
+        t1 = time.perf_counter()
+        recs["one_engine"] = []
+        with gisspa.Engine(cfg) as e:
+            for it in items:
+                e.load_templates(it["templates"])
+                recs["one_engine"].append(e.search(it["image"], it["ctf"]))
+        times["one_engine"] = time.perf_counter() - t1
+        log(f"one-engine loop: {times['one_engine']:.2f} s")
     with gisspa.Engine(cfg) as e:
         e.set_graph(True)
         t1 = time.perf_counter()
@@ -345,11 +370,13 @@ def batch_main(a):
     # refinement would be: an engine per micrograph, gs_search, then gs_refine of its top picks
     t1 = time.perf_counter()
     refs_seq = []
-    for it in items:
+    for i, it in enumerate(items):
         with gisspa.Engine(cfg) as e:
             e.load_templates(it["templates"])
             r = e.search(it["image"], it["ctf"])[:REFINE_TOP]
             refs_seq.append(e.refine(e.peaks_from_records(r)) if len(r) else None)
+        if i % 16 == 15:
+            log(f"per-engine search + refinement: {i + 1} micrographs in {time.perf_counter() - t1:.1f} s")
     times["per_engine_refine"] = time.perf_counter() - t1
     log(f"per-engine search + refinement: {times['per_engine_refine']:.2f} s")
     seq_same = all((x is None and len(y) == 0) or
@@ -358,7 +385,7 @@ def batch_main(a):
                    for x, y in zip(refs_seq, refs_c4))
     same = all(
         len(x) == len(y) and all(np.array_equal(x[n], y[n]) for n in x.dtype.names)
-        for k in ("one_engine", "batch") for x, y in zip(recs["per_engine"], recs[k]))
+        for k in ("one_engine", "batch") if k in recs for x, y in zip(recs["per_engine"], recs[k]))
     out = {
         "metric": "template·angle·pixel correlations/s over an uneven micrograph batch",
         "value": work / times["batch"],
@@ -376,7 +403,7 @@ def batch_main(a):
                                "72 angles", "micrographs": desc},
         "seconds": times,
         "speedup_vs_per_engine": times["per_engine"] / times["batch"],
-        "speedup_vs_one_engine": times["one_engine"] / times["batch"],
+        "speedup_vs_one_engine": times["one_engine"] / times["batch"] if "one_engine" in times else None,
         "records_identical": bool(same),
         "graph_captures_launches": list(graphs),
         "batch_refine": {
@@ -496,11 +523,11 @@ def main():
         log(f"autotune: column variant {col}, row variant {row}, {chunk} templates per launch "
             f"({time.perf_counter() - t1:.1f} s)")
     else:
-        # an option not given keeps the library default (column variant 8 with row variant 7, the
-        # row-packed layout); row variant 7 reads only that layout, so another column variant
-        # defaults to row variant 6
+        # an option not given keeps the library default (column variant 8 with row variant 8, the
+        # row-packed layout); row variants 7 and 8 read only that layout (written by column
+        # variants 8, 9 and 10), so another column variant defaults to row variant 6
         col = a.col_variant if a.col_variant >= 0 else 8
-        row = a.row_variant if a.row_variant >= 0 else (7 if col == 8 else 6)
+        row = a.row_variant if a.row_variant >= 0 else (8 if col in (8, 9, 10) else 6)
         chunk = a.chunk
         eng.set_variant(0, col)
         eng.set_variant(1, row)
@@ -555,7 +582,8 @@ def main():
             0: "false, 4, false, false", 1: "true, 4, false, false", 2: "true, 8, false, false",
             3: "false, 8, false, false",
             4: "true, 4, true, false", 5: "false, 4, true, false", 6: "true, 8, true, false",
-            7: "true, 8, true, true"}.get(variant[1], "")
+            7: "true, 8, true, true", 8: "true, 8, true, true, true",
+            9: "true, 8, true, false, true"}.get(variant[1], "")
         recs_t = json.load(open(tpath))["kernels"]
         for key, kname in (("column_pass", col_name), ("row_pass", row_name)):
             rec = recs_t.get(kname)
