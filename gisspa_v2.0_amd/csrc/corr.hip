@@ -940,7 +940,12 @@ __global__ void __launch_bounds__(64 * PlaneWaveCfg<L>::WAVES)
 // value is read once, after the column pass wrote it with streaming stores).
 // PKR: Z holds row-packed pairs (column pass layout 4): each lane loads 16-byte pairs and writes
 // them to their two slots, with no packing arithmetic here.
-template <int L, bool PF, int TPI_ = CorrCfg<L>::TPI, bool BK = false, bool PKR = false>
+// LZ: the running maximum starts from the pixel's record of the earlier launches, and an
+// iteration's TPI values take the per-template strict-maximum update only where their maximum
+// exceeds it (per pixel: the sums, one max chain and one compare instead of a compare and two
+// selects per template).  Once a few thousand hypotheses have been folded, new maxima are rare
+// and most waves skip the update: the result is the same first strict maximum in template order.
+template <int L, bool PF, int TPI_ = CorrCfg<L>::TPI, bool BK = false, bool PKR = false, bool LZ = false>
 __global__ void __launch_bounds__(256)
     k_rowpass_pix_wave(const float2* __restrict__ Z, const float2* __restrict__ twL,
                        const float2* __restrict__ twH, int tc, int nb, double* __restrict__ S,
@@ -980,11 +985,19 @@ __global__ void __launch_bounds__(256)
 
   float s[PPL], ss[PPL], bs[PPL];
   int ag[PPL];
+  // record index of the lane's pixel i: lane + 64 i of the wave's rows, x shifted by L/2
+  auto record = [&](int i) {
+    const int q = lane + 64 * i;
+    const int r2 = q / L, x = q - r2 * L;
+    int px = x + L / 2;
+    if (px >= L) px -= L;
+    return ((long long)b * L + (y0 + r2)) * L + px;
+  };
 #pragma unroll
   for (int i = 0; i < PPL; ++i) {
     s[i] = 0.f;
     ss[i] = 0.f;
-    bs[i] = -INFINITY;
+    bs[i] = LZ ? best[record(i)] : -INFINITY;
     ag[i] = -1;
   }
 
@@ -1081,15 +1094,37 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
     for (int i = 0; i < PPL; ++i) {
       const int r2 = i / (L / 64), xo = 64 * (i % (L / 64)) + lx;
+      if constexpr (LZ) {
+        float v[TPI];
+        float m = bs[i];
 #pragma unroll
-      for (int u = 0; u < TPI; ++u) {
-        if (FULL || t + u < tc) {
-          const float v = out[(r2 * TPI + u) * 2 * HS + xo];
-          s[i] += v;
-          ss[i] = fmaf(v, v, ss[i]);
-          const bool gt = v > bs[i];  // strict: the first maximum in template order is kept
-          bs[i] = gt ? v : bs[i];
-          ag[i] = gt ? t + u : ag[i];
+        for (int u = 0; u < TPI; ++u) {
+          v[u] = (FULL || t + u < tc) ? out[(r2 * TPI + u) * 2 * HS + xo] : -INFINITY;
+          if (FULL || t + u < tc) {
+            s[i] += v[u];
+            ss[i] = fmaf(v[u], v[u], ss[i]);
+          }
+          m = fmaxf(m, v[u]);
+        }
+        if (m > bs[i]) {  // a value of this iteration beats the running maximum: locate the first
+#pragma unroll
+          for (int u = 0; u < TPI; ++u) {
+            const bool gt = v[u] > bs[i];
+            bs[i] = gt ? v[u] : bs[i];
+            ag[i] = gt ? t + u : ag[i];
+          }
+        }
+      } else {
+#pragma unroll
+        for (int u = 0; u < TPI; ++u) {
+          if (FULL || t + u < tc) {
+            const float v = out[(r2 * TPI + u) * 2 * HS + xo];
+            s[i] += v;
+            ss[i] = fmaf(v, v, ss[i]);
+            const bool gt = v > bs[i];  // strict: the first maximum in template order is kept
+            bs[i] = gt ? v : bs[i];
+            ag[i] = gt ? t + u : ag[i];
+          }
         }
       }
     }
@@ -1103,14 +1138,11 @@ __global__ void __launch_bounds__(256)
 
 #pragma unroll
   for (int i = 0; i < PPL; ++i) {
-    const int q = lane + 64 * i;
-    const int r2 = q / L, x = q - r2 * L;
-    int px = x + L / 2;
-    if (px >= L) px -= L;
-    const long long gi = ((long long)b * L + (y0 + r2)) * L + px;
+    const long long gi = record(i);
     S[gi] += (double)s[i];
     SS[gi] += (double)ss[i];
-    if (bs[i] > best[gi]) {
+    // LZ: bs started at best[gi], so it moved only if a template of this launch beat it
+    if (LZ ? ag[i] >= 0 : bs[i] > best[gi]) {
       best[gi] = bs[i];
       arg[gi] = arg_base + ag[i];
     }
@@ -1594,8 +1626,9 @@ bool launch_rowpass_pixel(int L, const float2* Z, int tc, int nb, double* S, dou
   if (L == 320) {
     // variants 0/1: 2 rows x 4 templates per wave iteration without/with prefetch; 2/3: 1 row x
     // 8 templates with/without prefetch; 4/5/6: as 1/0/2 with the bank-conflict-free LDS layout;
-    // 7: as 6 reading the row-packed layout (column pass variant 8)
-    const int tpi = (variant == 2 || variant == 3 || variant == 6 || variant == 7) ? 8 : CorrCfg<320>::TPI;
+    // 7: as 6 reading the row-packed layout (column pass variant 8); 8 and 9: 7 and 6 with the
+    // lazy argmax
+    const int tpi = (variant == 2 || variant == 3 || (variant >= 6 && variant <= 9)) ? 8 : CorrCfg<320>::TPI;
     const int rows = 4 * (64 / CorrCfg<320>::TPT) / tpi;
     dim3 grid(320 / rows, nb);
     if (variant == 1)
@@ -1612,6 +1645,10 @@ bool launch_rowpass_pixel(int L, const float2* Z, int tc, int nb, double* S, dou
       k_rowpass_pix_wave<320, true, 8, true><<<grid, 256, 0, s>>>(Z, T.t320, T.t160, tc, nb, S, SS, best, arg, arg_base);
     else if (variant == 7)
       k_rowpass_pix_wave<320, true, 8, true, true><<<grid, 256, 0, s>>>(Z, T.t320, T.t160, tc, nb, S, SS, best, arg, arg_base);
+    else if (variant == 8)
+      k_rowpass_pix_wave<320, true, 8, true, true, true><<<grid, 256, 0, s>>>(Z, T.t320, T.t160, tc, nb, S, SS, best, arg, arg_base);
+    else if (variant == 9)
+      k_rowpass_pix_wave<320, true, 8, true, false, true><<<grid, 256, 0, s>>>(Z, T.t320, T.t160, tc, nb, S, SS, best, arg, arg_base);
     else
       k_rowpass_pix_wave<320, false><<<grid, 256, 0, s>>>(Z, T.t320, T.t160, tc, nb, S, SS, best, arg, arg_base);
     GS_CHECK_LAUNCH();

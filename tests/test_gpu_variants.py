@@ -1,5 +1,5 @@
 """Every result-invariant option of the norm_type=1 correlation kernels against the FP64 oracle:
-column-pass variants 0-8, row-pass variants 0-7, templates per launch (including a partial last
+column-pass variants 0-10, row-pass variants 0-9, templates per launch (including a partial last
 launch) and the two-stream pipeline.  Each must give the oracle's per-pixel z of the best
 record within 1e-4 and the oracle's pick list."""
 import numpy as np
@@ -30,6 +30,8 @@ def _key(a):
     (0, 0, 0, 0), (1, 0, 0, 0), (2, 0, 0, 0), (3, 0, 0, 0), (4, 0, 0, 0), (5, 0, 0, 0), (6, 0, 0, 0),
     (7, 6, 0, 0),
     (8, 7, 0, 0), (8, 7, 4, 0),  # row-packed inter-pass layout (also with a partial last launch)
+    (8, 8, 0, 0), (8, 8, 4, 0),  # the same with the lazy argmax (running maximum across launches)
+    (7, 9, 0, 0), (7, 9, 4, 0),  # unpacked layout, packing in the row pass, lazy argmax
     (9, 7, 0, 0), (9, 7, 4, 0),  # the same with 8 templates per workgroup (one partial chunk)
     (10, 7, 0, 0), (10, 7, 4, 0),  # row-packed, lane twiddles loaded into registers
     (0, 1, 0, 0), (0, 2, 0, 0), (0, 3, 0, 0), (0, 4, 0, 0), (0, 5, 0, 0), (0, 6, 0, 0),
@@ -60,12 +62,12 @@ def test_variant_matches_oracle(gs, case, col, row, chunk, overlap):
 
 
 def test_packed_layout_needs_its_row_pass(gs, case):
-    """Column variants 8 and 9 write the row-packed layout that only row variant 7 reads; a mismatched
-    pair is rejected before any kernel runs."""
+    """Column variants 8, 9 and 10 write the row-packed layout that only row variants 7 and 8 read; a
+    mismatched pair is rejected before any kernel runs."""
     cfg, ds, _, _ = case
     eng = gs.Engine(cfg)
     eng.load_templates(ds.templates)
-    for col, row in ((8, 6), (9, 6), (7, 7)):
+    for col, row in ((8, 6), (9, 6), (7, 7), (7, 8), (8, 9)):
         eng.set_variant(0, col)
         eng.set_variant(1, row)
         with pytest.raises(gs.GisspaError, match="go together"):
