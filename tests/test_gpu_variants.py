@@ -40,6 +40,7 @@ def _key(a):
     (4, 4, 4, 0),   # the same with the bank-conflict-free layouts of both passes
     (0, 1, 2, 1),   # three launches per angle through the double-buffered two-stream pipeline
     (5, 5, 2, 1),
+    (8, 8, 2, 1),   # row-packed layout and running maximum through the two-stream pipeline
 ])
 def test_variant_matches_oracle(gs, case, col, row, chunk, overlap):
     cfg, ds, picks, z_o = case
