@@ -335,7 +335,10 @@ __global__ void __launch_bounds__(CorrCfg<L>::CT)
 // k_colpass<320, 1, 4>: product with the Nyquist column packed into column 0, radix 4 * 4 * 20
 // over the ColStageMaps layouts, real-to-complex pair packing in the write-out, streaming stores.
 // Logical order: tiles of BT blocks x TT template chunks per column group, XCD-contiguous.
-template <int L, int TL, int BT, int TT>
+// PF: the block's columns are loaded once and held in registers, and the next template's
+// columns are loaded into the registers the product has just consumed, so that they arrive
+// during this template's transform and write-out.
+template <int L, int TL, int BT, int TT, bool PF = false>
 __global__ void __launch_bounds__(CorrCfg<L>::CT)
     k_colpass_tl(const float2* __restrict__ that, const float2* __restrict__ ihat,
                  const float2* __restrict__ twL, float2* __restrict__ Z, int t0, int tc, int nb) {
@@ -385,27 +388,46 @@ __global__ void __launch_bounds__(CorrCfg<L>::CT)
   const int j = 8 * cg + jl;
   const float2 w1 = c_conj(twL[j]);  // exp(+2 pi i j / L)
   const int tl1 = min(tc, (tch + 1) * TL);
-  for (int tl = tch * TL; tl < tl1; ++tl) {
+  float2 tv[NLD], iv[NLD], tn[NDC], in_[NDC];
+  auto load_t = [&](int tl) {
     const float2* Tp = that + (long long)(t0 + tl) * PLANE;
-    float2 tv[NLD], iv[NLD];
 #pragma unroll
     for (int i = 0; i < NLD; ++i) {
       const int q = lane + 64 * i;
       const int cl = q / L, ky = q - cl * L;
-      const long long e = (long long)column(cg, cw0 + cl) * L + ky;
-      tv[i] = Tp[e];
-      iv[i] = Ip[e];
+      tv[i] = Tp[(long long)column(cg, cw0 + cl) * L + ky];
     }
-    float2 tn[NDC], in_[NDC];
     if (dc) {
 #pragma unroll
       for (int i = 0; i < NDC; ++i) {
         const int ky = lane + 64 * i;
-        if (ky < L) {
-          tn[i] = Tp[(long long)H * L + ky];
-          in_[i] = Ip[(long long)H * L + ky];
-        }
+        if (ky < L) tn[i] = Tp[(long long)H * L + ky];
       }
+    }
+  };
+  auto load_i = [&]() {
+#pragma unroll
+    for (int i = 0; i < NLD; ++i) {
+      const int q = lane + 64 * i;
+      const int cl = q / L, ky = q - cl * L;
+      iv[i] = Ip[(long long)column(cg, cw0 + cl) * L + ky];
+    }
+    if (dc) {
+#pragma unroll
+      for (int i = 0; i < NDC; ++i) {
+        const int ky = lane + 64 * i;
+        if (ky < L) in_[i] = Ip[(long long)H * L + ky];
+      }
+    }
+  };
+  if constexpr (PF) {
+    load_i();
+    load_t(tch * TL);  // tch * TL < tl1: the workgroup's chunk is not empty
+  }
+  for (int tl = tch * TL; tl < tl1; ++tl) {
+    if constexpr (!PF) {
+      load_t(tl);
+      load_i();
     }
 #pragma unroll
     for (int i = 0; i < NLD; ++i) {
@@ -417,6 +439,9 @@ __global__ void __launch_bounds__(CorrCfg<L>::CT)
         p = make_float2(p.x - pn.y, p.y + pn.x);
       }
       buf[(cw0 + cl) * LP + ky] = p;
+    }
+    if constexpr (PF) {
+      if (tl + 1 < tl1) load_t(tl + 1);  // in flight during the transform and write-out
     }
     ct::lds_sync<ct::Sync::WAVE>();
     ct::run_stockham_maps<L, TPC, +1, ct::Sync::WAVE, ct::TwSrc::LANE>(buf + col * LP, wl, tid % TPC, P{},
@@ -1548,8 +1573,8 @@ static void colpass_impl(const CorrArgs& a, const float2* twL, int variant, hipS
       case 5: colpass_tiled<L, 0, 1, 8, 16>(a, twL, s); break;
       case 6: colpass_tiled<L, 1, 2, 8, 16>(a, twL, s); break;
       case 7:  // recurrence twiddles need 16 lanes per column and a plan with T % NS == 0 (l = 320)
-      case 8:  // the same with the row-packed layout (the row pass must be variant 7)
-      case 10:  // 8 with the lane twiddles loaded into registers (the row pass must be variant 7)
+      case 8:  // the same with the row-packed layout (the row pass must be variant 7 or 8)
+      case 10:  // 8 with the lane twiddles loaded into registers
         if constexpr (ct::lane_twiddles_fit<CorrCfg<L>::TPC>(typename ColPlanOf<L, 1>::type{}) &&
                       CorrCfg<L>::COLS == 16) {
           if (variant == 10)
@@ -1587,14 +1612,20 @@ bool launch_colpass_plane(int L, const CorrArgs& a, int variant, hipStream_t s) 
 
 bool launch_colpass(int L, const CorrArgs& a, int variant, hipStream_t s) {
   TwTables& T = tw_tables(current_device());
-  if (L == 320 && variant == 9) {  // 8 templates per workgroup, lane twiddles in registers
+  if (L == 320 && (variant == 9 || variant == 11)) {
+    // 8 templates per workgroup, lane twiddles in registers (11: block columns held, next
+    // template's columns prefetched)
     constexpr int TL = 8, BT = 8, TT = 8;
     const long long ntc = (a.tc + TL - 1) / TL;
     const long long G = (long long)(CorrCfg<320>::H / CorrCfg<320>::COLS) * ((ntc + TT - 1) / TT * TT) *
                         ((a.nb + BT - 1) / BT * BT);
     GS_REQUIRE(G > 0 && G < (1LL << 31), kInvalid, "column pass grid out of range");
-    k_colpass_tl<320, TL, BT, TT><<<(unsigned)G, CorrCfg<320>::CT, 0, s>>>(a.that, a.ihat, T.t320, a.Z, a.t0, a.tc,
-                                                                         a.nb);
+    if (variant == 11)
+      k_colpass_tl<320, TL, BT, TT, true><<<(unsigned)G, CorrCfg<320>::CT, 0, s>>>(a.that, a.ihat, T.t320, a.Z,
+                                                                                 a.t0, a.tc, a.nb);
+    else
+      k_colpass_tl<320, TL, BT, TT><<<(unsigned)G, CorrCfg<320>::CT, 0, s>>>(a.that, a.ihat, T.t320, a.Z, a.t0,
+                                                                           a.tc, a.nb);
     GS_CHECK_LAUNCH();
     return true;
   }

@@ -501,7 +501,7 @@ class Engine {
   }
   void set_chunk(int tc) { chunk_ = tc; }
   void set_variant(int kernel, int variant) {
-    GS_REQUIRE((kernel == 0 && variant >= 0 && variant <= 10) || (kernel == 1 && variant >= 0 && variant <= 9) ||
+    GS_REQUIRE((kernel == 0 && variant >= 0 && variant <= 11) || (kernel == 1 && variant >= 0 && variant <= 9) ||
                    (kernel == 2 && (variant == 0 || variant == 1)),
                kInvalid, "unknown kernel variant");
     if (kernel == 0)
@@ -1049,11 +1049,11 @@ class Engine {
       ensure_pipeline();
       rs = stream2_;
     }
-    // column variants 8, 9 and 10 write the row-packed layout that only row variants 7 and 8 read
+    // column variants 8 to 11 write the row-packed layout that only row variants 7 and 8 read
     // (l = 320; other sizes use one unpacked layout)
-    const bool packed_col = col_variant_ == 8 || col_variant_ == 9 || col_variant_ == 10;
+    const bool packed_col = col_variant_ >= 8 && col_variant_ <= 11;
     GS_REQUIRE(l != 320 || packed_col == (row_variant_ == 7 || row_variant_ == 8), kInvalid,
-               "column variants 8, 9, 10 (row-packed inter-pass layout) and row variants 7, 8 go together");
+               "column variants 8 to 11 (row-packed inter-pass layout) and row variants 7, 8 go together");
     float2* Zb = d_Z_.ensure(zsz * (overlap ? 2 : 1));
     double* part = d_part_.ensure((size_t)n * rotation_groups(l) * 2);
     float* stats = d_stats_.ensure((size_t)n * 2);
