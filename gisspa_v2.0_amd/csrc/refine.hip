@@ -255,6 +255,18 @@ struct RefRowCfg<720> {
   using Plan = ct::RadixList<4, 4, 5, 9>;
 };
 
+// Row layout of the A and B planes between k_ref_rotate_rows and k_ref_cols_dot: position
+// q = 2 NU g + s of a row holds column NU g + s (s < NU) or that column's mirror (s >= NU;
+// l - u, and l/2 for u = 0), so that the NU column pairs of a k_ref_cols_dot workgroup are one
+// contiguous run of 2 NU values per row (128 bytes at l = 320) rather than two runs of NU.
+template <int L>
+__device__ __forceinline__ int ref_pair_column(int q) {
+  constexpr int NU = 256 / RefRowCfg<L>::T / 2;
+  const int g = q / (2 * NU), s = q % (2 * NU);
+  const int u = g * NU + s % NU;
+  return s < NU ? u : (u == 0 ? L / 2 : L - u);
+}
+
 // k_ref_rotate and the row pass of the forward transform in one kernel: one workgroup = RPB rows
 // of one output plane.  For an A plane (pick p) it samples r and dr/dtheta of its rows, for a B
 // plane m = p - K the defocus-derivative fields of picks 2m and 2m + 1 (the same float
@@ -426,8 +438,12 @@ __global__ void __launch_bounds__(256)
     ct::run_stockham<L, T, -1, ct::Sync::WAVE>(buf + (wave * RPW + lane / T) * L, tw, lane % T, RowPlan{});
   }
   __syncthreads();
+  // rows written in the column-pair order of ref_pair_column
   float2* dst = AB + (long long)p * L2 + (long long)y0 * L;
-  for (int e = tid; e < RPB * L; e += 256) dst[e] = buf[e];
+  for (int e = tid; e < RPB * L; e += 256) {
+    const int r = e / L, q = e - r * L;
+    dst[e] = buf[r * L + ref_pair_column<L>(q)];
+  }
   if constexpr (RT) {
     __syncthreads();  // buf's rows are written: its first bytes hold the reduction
     store_sums(reinterpret_cast<double(*)[4]>(buf));
@@ -693,11 +709,13 @@ __global__ void __launch_bounds__(256)
     asm volatile("" : "+v"(t));
     return t;
   };
-  // a thread's elements of a plane lie in one column (256 is a multiple of COLS), rows YS apart
+  // a thread's elements of a plane lie in one column (256 is a multiple of COLS), rows YS apart;
+  // the workgroup's COLS columns are contiguous in each row of AB (ref_pair_column order)
   constexpr int NE = (COLS * L + 255) / 256, YS = 256 / COLS;
+  static_assert(COLS == 2 * NU && 2 * NU * G == L, "pair runs tile the row");
   auto load_plane = [&](const float2* P, int t) __attribute__((always_inline)) {
     const int lcol = t % COLS, ybase = t / COLS;
-    const float2* src = P + col_of(lcol);
+    const float2* src = P + COLS * g + lcol;
     float2 v[NE];
 #pragma unroll
     for (int k = 0; k < NE; ++k) {
