@@ -523,11 +523,11 @@ def main():
         log(f"autotune: column variant {col}, row variant {row}, {chunk} templates per launch "
             f"({time.perf_counter() - t1:.1f} s)")
     else:
-        # an option not given keeps the library default (column variant 8 with row variant 8, the
-        # row-packed layout); row variants 7 and 8 read only that layout (written by column
-        # variants 8, 9 and 10), so another column variant defaults to row variant 6
-        col = a.col_variant if a.col_variant >= 0 else 8
-        row = a.row_variant if a.row_variant >= 0 else (8 if col in (8, 9, 10) else 6)
+        # an option not given keeps the library default (column variant 12 with row variant 8,
+        # the row-packed layout); row variants 7 and 8 read only that layout (written by column
+        # variants 8 to 13), so another column variant defaults to row variant 6
+        col = a.col_variant if a.col_variant >= 0 else 12
+        row = a.row_variant if a.row_variant >= 0 else (8 if 8 <= col <= 13 else 6)
         chunk = a.chunk
         eng.set_variant(0, col)
         eng.set_variant(1, row)
@@ -577,7 +577,8 @@ def main():
         col_name = {0: "k_colpass<320, 0, 0, 8, 16>", 3: "k_colpass<320, 1, 0, 8, 16>",
                     4: "k_colpass<320, 1, 1, 8, 16>", 5: "k_colpass<320, 0, 1, 8, 16>",
                     6: "k_colpass<320, 1, 2, 8, 16>", 7: "k_colpass<320, 1, 3, 8, 16>",
-                    8: "k_colpass<320, 1, 4, 8, 16>"}.get(variant[0], "")
+                    8: "k_colpass<320, 1, 4, 8, 16>", 12: "k_colpass<320, 1, 4, 4, 4, true>",
+                    13: "k_colpass<320, 1, 4, 2, 8, true>"}.get(variant[0], "")
         row_name = "k_rowpass_pix_wave<320, %s>" % {
             0: "false, 4, false, false", 1: "true, 4, false, false", 2: "true, 8, false, false",
             3: "false, 8, false, false",

@@ -132,7 +132,11 @@ struct ColStageMaps {
 // BT x TT: tile of the logical order (blocks x templates).  At 512 templates per launch, tiles
 // of 32 x 16 or 32 x 32 fetch 1.8 GB per launch from beyond L2 instead of 6.4 GB but ran 4 %
 // slower on MI355X; 16 x 16 and 16 x 32 ran as 8 x 16.
-template <int L, int Plan = 0, int LAY = 0, int BT = 8, int TT = 16>
+// CGF: the column group is the fastest index of the logical order instead of the slowest, so
+// that the H / COLS workgroups of a plane run together and write its rows as whole runs of
+// 8 H bytes rather than one COLS-column piece per row at a time (at the cost of reading each
+// tile's spectra columns once per plane group instead of once per tile).
+template <int L, int Plan = 0, int LAY = 0, int BT = 8, int TT = 16, bool CGF = false>
 __global__ void __launch_bounds__(CorrCfg<L>::CT)
     k_colpass(const float2* __restrict__ that, const float2* __restrict__ ihat,
               const float2* __restrict__ twL, float2* __restrict__ Z, int t0, int tc, int nb) {
@@ -180,6 +184,11 @@ __global__ void __launch_bounds__(CorrCfg<L>::CT)
   const unsigned xcd = bid % 8u, idx = bid / 8u, per = G / 8u, rem = G % 8u;
   unsigned logical = (xcd < rem ? xcd * (per + 1u) : rem * (per + 1u) + (xcd - rem) * per) + idx;
   const unsigned ntb = (unsigned)(nb + BT - 1) / BT, ntt = (unsigned)(tc + TT - 1) / TT;
+  int cgf = 0;
+  if constexpr (CGF) {
+    cgf = (int)(logical % (unsigned)(H / COLS));
+    logical /= (unsigned)(H / COLS);
+  }
   const int bi = (int)(logical % BT);
   logical /= BT;
   const int ti = (int)(logical % TT);
@@ -187,7 +196,7 @@ __global__ void __launch_bounds__(CorrCfg<L>::CT)
   const int tt = (int)(logical % ntt);
   logical /= ntt;
   const int btile = (int)(logical % ntb);
-  const int cg = (int)(logical / ntb);
+  const int cg = CGF ? cgf : (int)(logical / ntb);
   const int b = btile * BT + bi;
   const int tl = tt * TT + ti;
   if (b >= nb || tl >= tc) return;  // padding of the last tiles (uniform per workgroup)
@@ -1555,12 +1564,12 @@ int rowpass_groups(int L) {
   return 0;
 }
 
-template <int L, int Plan, int LAY, int BT, int TT>
+template <int L, int Plan, int LAY, int BT, int TT, bool CGF = false>
 static void colpass_tiled(const CorrArgs& a, const float2* twL, hipStream_t s) {
   using C = CorrCfg<L>;
   const long long G = (long long)(C::H / C::COLS) * ((a.tc + TT - 1) / TT * TT) * ((a.nb + BT - 1) / BT * BT);
   GS_REQUIRE(G > 0 && G < (1LL << 31), kInvalid, "column pass grid out of range");
-  k_colpass<L, Plan, LAY, BT, TT><<<(unsigned)G, C::CT, 0, s>>>(a.that, a.ihat, twL, a.Z, a.t0, a.tc, a.nb);
+  k_colpass<L, Plan, LAY, BT, TT, CGF><<<(unsigned)G, C::CT, 0, s>>>(a.that, a.ihat, twL, a.Z, a.t0, a.tc, a.nb);
 }
 
 template <int L>
@@ -1575,9 +1584,15 @@ static void colpass_impl(const CorrArgs& a, const float2* twL, int variant, hipS
       case 7:  // recurrence twiddles need 16 lanes per column and a plan with T % NS == 0 (l = 320)
       case 8:  // the same with the row-packed layout (the row pass must be variant 7 or 8)
       case 10:  // 8 with the lane twiddles loaded into registers
+      case 12:  // 8 with the column group the fastest index (4 x 4 tiles)
+      case 13:  // 8 with the column group the fastest index (2 x 8 tiles)
         if constexpr (ct::lane_twiddles_fit<CorrCfg<L>::TPC>(typename ColPlanOf<L, 1>::type{}) &&
                       CorrCfg<L>::COLS == 16) {
-          if (variant == 10)
+          if (variant == 12)
+            colpass_tiled<L, 1, 4, 4, 4, true>(a, twL, s);
+          else if (variant == 13)
+            colpass_tiled<L, 1, 4, 2, 8, true>(a, twL, s);
+          else if (variant == 10)
             colpass_tiled<L, 1, 5, 8, 16>(a, twL, s);
           else if (variant == 8)
             colpass_tiled<L, 1, 4, 8, 16>(a, twL, s);

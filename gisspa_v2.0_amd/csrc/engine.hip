@@ -501,7 +501,7 @@ class Engine {
   }
   void set_chunk(int tc) { chunk_ = tc; }
   void set_variant(int kernel, int variant) {
-    GS_REQUIRE((kernel == 0 && variant >= 0 && variant <= 11) || (kernel == 1 && variant >= 0 && variant <= 9) ||
+    GS_REQUIRE((kernel == 0 && variant >= 0 && variant <= 13) || (kernel == 1 && variant >= 0 && variant <= 9) ||
                    (kernel == 2 && (variant == 0 || variant == 1)),
                kInvalid, "unknown kernel variant");
     if (kernel == 0)
@@ -551,8 +551,8 @@ class Engine {
     };
     try {
       // candidates, as (column pass, row pass) pairs timed together: the row-packed layout (column
-      // pass 8 packs the rows, row pass 8 reads them and keeps the running maximum across
-      // launches), the unpacked layout with the compact recurrence-twiddle column pass and the
+      // pass 12 packs the rows, the column group the fastest index of its order; row pass 8 reads
+      // them and keeps the running maximum across launches), the unpacked layout with the compact recurrence-twiddle column pass and the
       // bank-conflict-free row pass (7, 6), and the natural layouts (3, 1); the other variants
       // measured slower on MI355X at every size tried.  A later candidate replaces the current
       // pick only when more than 3 % faster, so that timing noise between near-equal options
@@ -561,7 +561,7 @@ class Engine {
       double best = 0;
       bool first = true;
       int pick_col = col_variant_, pick_row = row_variant_;
-      for (const auto& c : {std::pair<int, int>{8, 8}, std::pair<int, int>{7, 6}, std::pair<int, int>{3, 1}}) {
+      for (const auto& c : {std::pair<int, int>{12, 8}, std::pair<int, int>{7, 6}, std::pair<int, int>{3, 1}}) {
         col_variant_ = c.first;
         row_variant_ = c.second;
         const double ms = measure(2);
@@ -1049,11 +1049,11 @@ class Engine {
       ensure_pipeline();
       rs = stream2_;
     }
-    // column variants 8 to 11 write the row-packed layout that only row variants 7 and 8 read
+    // column variants 8 to 13 write the row-packed layout that only row variants 7 and 8 read
     // (l = 320; other sizes use one unpacked layout)
-    const bool packed_col = col_variant_ >= 8 && col_variant_ <= 11;
+    const bool packed_col = col_variant_ >= 8 && col_variant_ <= 13;
     GS_REQUIRE(l != 320 || packed_col == (row_variant_ == 7 || row_variant_ == 8), kInvalid,
-               "column variants 8 to 11 (row-packed inter-pass layout) and row variants 7, 8 go together");
+               "column variants 8 to 13 (row-packed inter-pass layout) and row variants 7, 8 go together");
     float2* Zb = d_Z_.ensure(zsz * (overlap ? 2 : 1));
     double* part = d_part_.ensure((size_t)n * rotation_groups(l) * 2);
     float* stats = d_stats_.ensure((size_t)n * 2);
@@ -1428,7 +1428,7 @@ class Engine {
   std::vector<int> offx_, offy_, ulx_, uly_;
   std::vector<float> angles_;
   // kernel variants default to the fastest measured on the production shape (l = 320)
-  int lim_angles_ = 0, lim_templates_ = 0, chunk_ = 0, row_variant_ = 8, col_variant_ = 8;
+  int lim_angles_ = 0, lim_templates_ = 0, chunk_ = 0, row_variant_ = 8, col_variant_ = 12;
   bool graph_allowed_ = true;  // false while a batch runs an item whose shape occurs only once
   double mem_budget_ = 0;  // bytes the automatic templates-per-launch may give the inter-pass buffer
   double total_ms_ = 0, corr_ms_ = 0;

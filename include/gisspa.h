@@ -173,16 +173,18 @@ gs_status gs_set_chunk(gs_engine* e, int templates_per_launch);
  * prefetch, 3 as 0 with radix 4*4*20 (window 720: as 0), 4/5 as 3/0 with the
  * bank-conflict-free LDS layout (swizzled slots; per-lane twiddles in registers for 4), 6 as 3
  * with an unpadded column buffer and twiddles from global memory, 7 as 6 with twiddles formed
- * per lane by recurrence, 8 (default) as 7 writing the row-packed inter-pass layout, 9 as 8
+ * per lane by recurrence, 8 as 7 writing the row-packed inter-pass layout, 9 as 8
  * looping each workgroup over 8 templates with the lane twiddles in registers, 10 as 8 with the
  * lane twiddles loaded into registers, 11 as 9 with the block's columns held in registers and
- * the next template's columns loaded during the current transform (window 720: 7 and above as 3);
+ * the next template's columns loaded during the current transform, 12 (default) and 13 as 8 with
+ * the column group the fastest index of the workgroup order, tiles of 4 x 4 and 2 x 8 (block,
+ * template) planes (window 720: 7 and above as 3);
  * kernel 1 = row pass: 0 plain, 1 software-prefetched loads, 2/3 as 1/0 with one row and eight
  * templates per wave iteration, 4/5/6 as 1/0/2 with the bank-conflict-free LDS layout and
  * streaming loads, 7 as 6 reading the row-packed layout, 8 (default) as 7 with the running
  * maximum started from the earlier launches' record and the per-template argmax update taken
  * only where a value beats it, 9 as 6 with that update (window 720: one variant).
- * Column variants 8 to 11 (row-packed layout) and row variants 7, 8 go together (window 320,
+ * Column variants 8 to 13 (row-packed layout) and row variants 7, 8 go together (window 320,
  * norm_type=1): a search with only one side of a pair fails with GS_ERR_INVALID;
  * kernel 2 = pipeline: 0 both passes on one stream (default), 1 row pass on a second stream
  * overlapping the next column pass through a double-buffered inter-pass buffer. */

@@ -1,5 +1,5 @@
 """Every result-invariant option of the norm_type=1 correlation kernels against the FP64 oracle:
-column-pass variants 0-11, row-pass variants 0-9, templates per launch (including a partial last
+column-pass variants 0-13, row-pass variants 0-9, templates per launch (including a partial last
 launch) and the two-stream pipeline.  Each must give the oracle's per-pixel z of the best
 record within 1e-4 and the oracle's pick list."""
 import numpy as np
@@ -35,6 +35,7 @@ def _key(a):
     (9, 7, 0, 0), (9, 7, 4, 0),  # the same with 8 templates per workgroup (one partial chunk)
     (10, 7, 0, 0), (10, 7, 4, 0),  # row-packed, lane twiddles loaded into registers
     (11, 8, 0, 0), (11, 8, 4, 0),  # 9 with the block columns held and the next template prefetched
+    (12, 8, 0, 0), (13, 8, 4, 0),  # 8 with the column group the fastest index of the order
     (0, 1, 0, 0), (0, 2, 0, 0), (0, 3, 0, 0), (0, 4, 0, 0), (0, 5, 0, 0), (0, 6, 0, 0),
     (3, 6, 4, 0),   # eight templates per iteration with a partial last iteration (4 + 2 templates)
     (3, 1, 4, 0),   # two launches per angle, the second partial (4 + 2 templates)
@@ -64,7 +65,7 @@ def test_variant_matches_oracle(gs, case, col, row, chunk, overlap):
 
 
 def test_packed_layout_needs_its_row_pass(gs, case):
-    """Column variants 8 to 11 write the row-packed layout that only row variants 7 and 8 read; a
+    """Column variants 8 to 13 write the row-packed layout that only row variants 7 and 8 read; a
     mismatched pair is rejected before any kernel runs."""
     cfg, ds, _, _ = case
     eng = gs.Engine(cfg)

@@ -45,7 +45,7 @@ def test_norm_window_size(gs, orc, l):
     np.testing.assert_allclose(g["score"], opicks["score"], rtol=1e-4)
     # the autotuner has nothing to choose for the generic path and keeps the settings
     col, row, chunk = eng.autotune()
-    assert (col, row) == (8, 8)
+    assert (col, row) == (12, 8)
     eng.close()
 
 
