@@ -859,6 +859,7 @@ __global__ void __launch_bounds__(64 * PlaneWaveCfg<L>::WAVES)
   constexpr int SLICE = (TPW + (DUMMY ? 1 : 0)) * HS;
   static_assert(L % RCH == 0 && RCH % TPW == 0 && TPW * TPT <= 64, "row mapping");
   static_assert(!DUMMY || TPW * TPT + TPT > 64, "the spare slice covers the remaining lanes");
+  static_assert(L % 4 == 0, "the half-row shift is a whole number of output slots");
   __shared__ float2 smem[WAVES * SLICE + H + H / 2 + 1];
   float2* const twh = smem + WAVES * SLICE;  // exp(-2 pi i e / H)
   float2* const twc = twh + H;               // exp(+2 pi i k / L), k <= H/2
@@ -923,23 +924,25 @@ __global__ void __launch_bounds__(64 * PlaneWaveCfg<L>::WAVES)
       const int py = y0 + it * TPW + tr;
       const bool yedge = py < border || py > L - border;
       float rs = 0.f, rs2 = 0.f;
+      // Lane element e holds the record pixels px = 2e, 2e + 1: they sit in output slot
+      // e + L/4 (mod L/2), the half-row shift of the correlation read as a rotation of the slots.
+      // A lane thus meets its pixels in ascending index (rows ascend across iterations), so a
+      // strict > keeps the lowest index among equal values, and vbest >= 0 makes v > vbest imply
+      // the reference's v > 0.
 #pragma unroll
       for (int i = 0; i < NO; ++i) {
-        const int sl = sub + TPT * i;
-        if (H % TPT == 0 || sl < H) {
-          const float2 v2 = dst[sl];
+        const int e = sub + TPT * i;
+        if (H % TPT == 0 || e < H) {
+          const float2 v2 = dst[e < H / 2 ? e + H / 2 : e - H / 2];
 #pragma unroll
           for (int h = 0; h < 2; ++h) {
             const float v = h ? v2.y : v2.x;
-            const int x = 2 * sl + h;
-            const int px = x < L / 2 ? x + L / 2 : x - L / 2;
+            const int px = 2 * e + h;
             rs += v;
             rs2 = fmaf(v, v, rs2);
-            const int id = py * L + px;
-            const bool ok = !yedge && !(px < border || px > L - border) && v > 0.f &&
-                            (v > vbest || (v == vbest && id < ibest));
+            const bool ok = !yedge && !(px < border || px > L - border) && v > vbest;
             vbest = ok ? v : vbest;
-            ibest = ok ? id : ibest;
+            ibest = ok ? py * L + px : ibest;
           }
         }
       }
