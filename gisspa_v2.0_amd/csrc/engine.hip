@@ -873,7 +873,7 @@ class Engine {
     launch_phase_flip(F, nx, ny, nf, dp_, stream_);
     const int nbins = std::max(nx, ny) / 2;
     double* ra = d_ra_img_.ensure((size_t)nf * nbins);
-    double* part = d_chpart_.ensure((size_t)nf * radial_image_groups(nf) * nbins);
+    double* part = d_ichpart_.ensure((size_t)nf * radial_image_groups(nf) * nbins);
     launch_radial_power_image(F, nx, ny, nf, part, ra, stream_);
     launch_whiten_filter_image(F, nx, ny, nf, ra, radial_counts_dev(nx, ny), dp_, stream_);
     if (dc_zero) launch_zero_dc(F, stream_);
@@ -968,12 +968,12 @@ class Engine {
     k.used += 1;
   }
 
-  // Second stream (default priority) for the row pass, and the events of the double-buffered
-  // inter-pass handoff; created on first use.
+  // Second stream (default priority) for the row pass (and the norm_type=0 template
+  // preparation), and the events of the double-buffered inter-pass handoff; created on first use.
   void ensure_pipeline() {
     if (stream2_) return;
     GS_HIP(hipStreamCreateWithFlags(&stream2_, hipStreamNonBlocking));
-    for (hipEvent_t* e : {&ev_zready_[0], &ev_zready_[1], &ev_zfree_[0], &ev_zfree_[1], &ev_sync_})
+    for (hipEvent_t* e : {&ev_zready_[0], &ev_zready_[1], &ev_zfree_[0], &ev_zfree_[1], &ev_sync_, &ev_tprep_})
       GS_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming));
   }
 
@@ -1258,7 +1258,26 @@ class Engine {
   void run_nonorm() {
     const int l = P_.window_size, hx = l / 2 + 1, n = n_;
     const size_t L2 = (size_t)l * l, T2 = (size_t)kTile * kTile;
-    prep_templates();  // -> d_that_ [n][hx][l]
+    // The template preparation (-> d_that_ [n][hx][l]) runs on the second stream beside the
+    // tiles' preparation on the main one: both are chains of small launches that leave most of
+    // the GPU idle.  The two chains share no buffer (the tiles use d_scratch2_, d_m2_,
+    // d_apart2_ and d_ichpart_), and the main stream waits for the templates before the
+    // correlation.  The second stream first waits for the uploads already on the main one.
+    ensure_pipeline();
+    GS_HIP(hipEventRecord(ev_sync_, stream_));
+    GS_HIP(hipStreamWaitEvent(stream2_, ev_sync_, 0));
+    {
+      const hipStream_t main = stream_;
+      stream_ = stream2_;
+      try {
+        prep_templates();
+      } catch (...) {
+        stream_ = main;
+        throw;
+      }
+      stream_ = main;
+    }
+    GS_HIP(hipEventRecord(ev_tprep_, stream2_));
     ulx_.clear();
     uly_.clear();
     tile_geometry(W_, H_, ulx_, uly_);
@@ -1303,8 +1322,8 @@ class Engine {
     const size_t npair_max = (size_t)(ag_max + 1) / 2 * B;
     float2* rotb = d_work2_.ensure(npair_max * L2);
     float2* ihat = d_ihat_.ensure((size_t)nbp_max * hx * l);
-    float2* S = d_scratch_.ensure(std::max((size_t)B, npair_max) * L2);
-    double* m = d_m_.ensure(std::max(n, B));
+    float2* S = d_scratch2_.ensure(std::max((size_t)B, npair_max) * L2);
+    double* m = d_m2_.ensure(B);
     std::vector<float2> cs(na);
     for (int a = 0; a < na; ++a)
       cs[a] = make_float2((float)std::cos((double)angles[a] * kRefPi / 180),
@@ -1320,12 +1339,13 @@ class Engine {
     launch_split_tile_blocks(F, kTile, l, doff, nbk_, ntiles, blk, stream_);
     if (P_.phase_flip == 1) {  // src/nonorm.cu:405-447
       fft2_planes(blk, S, B, -1, (float)(1.0 / (double)L2));
-      launch_area_power_full(blk, B, l, d_apart_.ensure((size_t)B * plane_reduction_chunks(B)), m, stream_);
+      launch_area_power_full(blk, B, l, d_apart2_.ensure((size_t)B * plane_reduction_chunks(B)), m, stream_);
       launch_scale_planes(blk, B, (long long)L2, m, 1.f, stream_);
       fft2_planes(blk, S, B, +1, 1.f);
     }
     float* blkr = d_blkreal_.ensure((size_t)B * L2);
     launch_extract_real(blk, blkr, (long long)B * L2, stream_);
+    GS_HIP(hipStreamWaitEvent(stream_, ev_tprep_, 0));  // the template spectra d_that_
     for (int a0 = 0; a0 < na; a0 += ag_max) {
       const int ag = std::min(ag_max, na - a0), nbp = ag * B, np2 = (ag + 1) / 2 * B;
       // rotate_subIMG of every block for angles a0..a0+ag (k:534-603), two angles per plane, and
@@ -1413,6 +1433,7 @@ class Engine {
   int device_;
   hipStream_t stream_ = nullptr, stream2_ = nullptr;
   hipEvent_t ev_zready_[2] = {nullptr, nullptr}, ev_zfree_[2] = {nullptr, nullptr}, ev_sync_ = nullptr;
+  hipEvent_t ev_tprep_ = nullptr;
   bool overlap_ = false;
   hipEvent_t ev_start_ = nullptr, ev_corr0_ = nullptr, ev_corr1_ = nullptr, ev_end_ = nullptr;
   KernelClock clk_col_, clk_row_;
@@ -1441,7 +1462,7 @@ class Engine {
   DevBuf<int32_t> d_ridx_;
   DevBuf<RefWeight> d_rtab_;
   DevBuf<RefPeak> d_rpk_;
-  DevBuf<double2> d_rpart_, d_apart_;
+  DevBuf<double2> d_rpart_, d_apart_, d_apart2_;
   DevBuf<double> d_rpart6_, d_rstats_, d_rcg_, d_rz_, d_rz0_, d_rgrad_;
   static constexpr size_t kMaxGraphs = 8;
   bool use_graph_ = false;
@@ -1453,8 +1474,10 @@ class Engine {
   DevBuf<float> d_rmask_;
   DevBuf<float2> d_rph_;
   DevBuf<float> d_raw_, d_img_, d_imgpre_, d_tprep_, d_stats_, d_best_, d_blkreal_;
-  DevBuf<float2> d_work_, d_work2_, d_scratch_, d_frame_, d_frame2_, d_blocks_, d_ihat_, d_that_, d_Z_, d_cs_;
-  DevBuf<double> d_mom_, d_ra_, d_ra_img_, d_m_, d_part_, d_S_, d_SS_, d_mu_, d_sigma_, d_planes_, d_chpart_;
+  DevBuf<float2> d_work_, d_work2_, d_scratch_, d_scratch2_, d_frame_, d_frame2_, d_blocks_, d_ihat_, d_that_, d_Z_,
+      d_cs_;
+  DevBuf<double> d_mom_, d_ra_, d_ra_img_, d_m_, d_m2_, d_part_, d_S_, d_SS_, d_mu_, d_sigma_, d_planes_, d_chpart_,
+      d_ichpart_;
   DevBuf<float> d_plane_part_;
   DevBuf<int32_t> d_arg_;
   DevBuf<int> d_offx_, d_offy_, d_count_, d_tilegeo_;
