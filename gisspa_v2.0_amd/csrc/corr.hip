@@ -744,7 +744,7 @@ struct ColGroupCfg<720> {
 };
 
 template <int L>
-__global__ void __launch_bounds__(64 * ColGroupCfg<L>::COLS)
+__global__ void __launch_bounds__(64 * ColGroupCfg<L>::COLS) __attribute__((amdgpu_waves_per_eu(4)))
     k_colpass_grouped(const float2* __restrict__ that, const float2* __restrict__ ihat,
                       const float2* __restrict__ twL, float2* __restrict__ Z, int t0, int tc, int nb) {
   using C = ColGroupCfg<L>;
@@ -769,35 +769,45 @@ __global__ void __launch_bounds__(64 * ColGroupCfg<L>::COLS)
   // registers hold the block's column and the template's (the next template's once consumed);
   // the Nyquist columns of the one wave holding column 0 are read when used, so that no wave
   // reserves registers for them
+  // Addresses come from an opaque copy of the lane index taken where they are used: hoisted out
+  // of the template loop, the 64-bit address of every unrolled load stayed live across it
+  // (about 120 registers in all), which left room for one workgroup per CU.
+  auto opaque_lane = [&]() {
+    int ln = lane;
+    asm volatile("" : "+v"(ln));
+    return ln & 63;
+  };
   float2 iv[NLD], tv[NLD];
   auto load_t = [&](int tl) {
-    const float2* Tp = that + (long long)(t0 + tl) * PLANE;
+    const float2* Tp = that + (long long)(t0 + tl) * PLANE + (long long)kx * L + opaque_lane();
 #pragma unroll
-    for (int i = 0; i < NLD; ++i) {
-      const int ky = lane + 64 * i;
-      if (L % 64 == 0 || ky < L) tv[i] = Tp[(long long)kx * L + ky];
-    }
+    for (int i = 0; i < NLD; ++i)
+      if (L % 64 == 0 || opaque_lane() + 64 * i < L) tv[i] = Tp[64 * i];
   };
+  {
+    const float2* Ic = Ip + (long long)kx * L + opaque_lane();
 #pragma unroll
-  for (int i = 0; i < NLD; ++i) {
-    const int ky = lane + 64 * i;
-    if (L % 64 == 0 || ky < L) iv[i] = Ip[(long long)kx * L + ky];
+    for (int i = 0; i < NLD; ++i)
+      if (L % 64 == 0 || opaque_lane() + 64 * i < L) iv[i] = Ic[64 * i];
   }
   load_t(0);
   for (int i = tid; i < L; i += NT) tw[i] = twL[i];
   float2* const col = buf + wave * L;
   for (int tl = 0; tl < tc; ++tl) {
     if (tl > 0) __syncthreads();  // the previous template's write-out has read every column
+    {
+      const int ln = opaque_lane();
 #pragma unroll
-    for (int i = 0; i < NLD; ++i) {
-      const int ky = lane + 64 * i;
-      if (L % 64 == 0 || ky < L) {
-        float2 p = c_mulconj(iv[i], tv[i]);
-        if (dc) {
-          const float2 pn = c_mulconj(Ip[(long long)H * L + ky], that[(t0 + tl) * PLANE + (long long)H * L + ky]);
-          p = make_float2(p.x - pn.y, p.y + pn.x);
+      for (int i = 0; i < NLD; ++i) {
+        const int ky = ln + 64 * i;
+        if (L % 64 == 0 || ky < L) {
+          float2 p = c_mulconj(iv[i], tv[i]);
+          if (dc) {
+            const float2 pn = c_mulconj(Ip[(long long)H * L + ky], that[(t0 + tl) * PLANE + (long long)H * L + ky]);
+            p = make_float2(p.x - pn.y, p.y + pn.x);
+          }
+          col[ky] = p;
         }
-        col[ky] = p;
       }
     }
     if (tl + 1 < tc) load_t(tl + 1);  // in flight during this template's transform and write-out
@@ -809,9 +819,10 @@ __global__ void __launch_bounds__(64 * ColGroupCfg<L>::COLS)
     __syncthreads();  // rows of the group gather all its columns
     float4* const Zg = reinterpret_cast<float4*>(Z + (((long long)tl * nb + b) * NCG + cg) * (long long)L * COLS);
     constexpr int NQ = L * COLS / 2;  // 16-byte words of the group's region
+    const int wtid = 64 * (tid / 64) + opaque_lane();  // tid, opaque as above
 #pragma unroll
     for (int e0 = 0; e0 < NQ; e0 += NT) {
-      const int e = e0 + tid;
+      const int e = e0 + wtid;
       if (NQ % NT == 0 || e < NQ) {
         const int yd = e / (COLS / 2), cp = e % (COLS / 2);
         const int y = yd < L / 2 ? yd + L / 2 : yd - L / 2;  // output row yd holds transform row y
