@@ -993,12 +993,31 @@ __global__ void __launch_bounds__(64 * PlaneWaveCfg<L>::WAVES)
 // exceeds it (per pixel: the sums, one max chain and one compare instead of a compare and two
 // selects per template).  Once a few thousand hypotheses have been folded, new maxima are rare
 // and most waves skip the update: the result is the same first strict maximum in template order.
-template <int L, bool PF, int TPI_ = CorrCfg<L>::TPI, bool BK = false, bool PKR = false, bool LZ = false>
+// CR: instead of the per-pixel statistics, the reference's records (second pass of
+// src/norm.cu:560-600 with mu and sigma known): z per (pixel, template) as update_CCG (k:709-721),
+// and for every template whose z reaches the threshold somewhere in a wavefront's row, the best
+// value at or above the threshold of each part of a 256-element chunk the row holds (a row of
+// 320 pixels meets two chunks; the workgroup's 4 rows are 5 whole chunks), ties to the smaller
+// bit-reversed offset in the chunk (the order get_peak_pos's tree keeps, k:766-793).  Each part
+// is appended to ChunkRecArgs::rec; the host merges the parts of a chunk and applies the centre
+// range to the winner.  A part with no value at or above the threshold cannot win a chunk whose
+// maximum reaches it, so the parts skipped lose nothing.
+struct ChunkRecArgs {
+  const double* mu;
+  const double* sigma;
+  float thresh;
+  int angle, t0, max_out;
+  RefRec* rec;
+  int* count;
+};
+
+template <int L, bool PF, int TPI_ = CorrCfg<L>::TPI, bool BK = false, bool PKR = false, bool LZ = false,
+          bool CR = false>
 __global__ void __launch_bounds__(256)
     k_rowpass_pix_wave(const float2* __restrict__ Z, const float2* __restrict__ twL,
                        const float2* __restrict__ twH, int tc, int nb, double* __restrict__ S,
                        double* __restrict__ SS, float* __restrict__ best, int32_t* __restrict__ arg,
-                       int arg_base) {
+                       int arg_base, ChunkRecArgs cr = {}) {
   using C = CorrCfg<L>;
   constexpr int H = C::H, TPI = TPI_, TPT = C::TPT;
   constexpr int TPW = 64 / TPT;  // transforms per wave
@@ -1041,6 +1060,24 @@ __global__ void __launch_bounds__(256)
     if (px >= L) px -= L;
     return ((long long)b * L + (y0 + r2)) * L + px;
   };
+  static_assert(!CR || (RW == 1 && (4 * L) % 256 == 0), "chunk records: one row per wave, whole chunks per workgroup");
+  double cmu[CR ? PPL : 1], cis[CR ? PPL : 1];  // mu and 1/sigma (0 where sigma <= 0: z = 0)
+  int cpix[CR ? PPL : 1], ckey[CR ? PPL : 1];
+  bool cseg[CR ? PPL : 1];
+  if constexpr (CR) {
+    const int part0 = (wave * L) / 256;  // first chunk part of this row, in workgroup chunks
+#pragma unroll
+    for (int i = 0; i < PPL; ++i) {
+      const long long gi = record(i);
+      const double sg = cr.sigma[gi];
+      cmu[i] = cr.mu[gi];
+      cis[i] = sg > 0 ? 1.0 / sg : 0.0;
+      cpix[i] = (int)(gi - (long long)b * L * L);  // y * L + x in the block
+      const int q = wave * L + cpix[i] % L;         // index in the workgroup's 4 * L
+      ckey[i] = (int)(__builtin_bitreverse32((unsigned)(q % 256)) >> 24);
+      cseg[i] = q / 256 != part0;
+    }
+  }
 #pragma unroll
   for (int i = 0; i < PPL; ++i) {
     s[i] = 0.f;
@@ -1139,8 +1176,48 @@ __global__ void __launch_bounds__(256)
     // row i / (L/64); its real output sits in complex slot x / 2 = ln / 2 + 32 (i % (L/64)), and
     // map(a + 32 c) = map(a) + 32 c, so one lane part serves every pixel.
     const int lx = 2 * map(ln >> 1) + (ln & 1);
+    if constexpr (CR) {
 #pragma unroll
-    for (int i = 0; i < PPL; ++i) {
+      for (int u = 0; u < TPI; ++u) {
+        if (!(FULL || t + u < tc)) continue;
+        float z[PPL];
+        bool hit = false;
+#pragma unroll
+        for (int i = 0; i < PPL; ++i) {
+          const float v = out[u * 2 * HS + 64 * i + lx];
+          z[i] = cis[i] != 0.0 ? (float)(((double)v - cmu[i]) * cis[i]) : 0.f;
+          hit = hit || z[i] >= cr.thresh;
+        }
+        if (!__any(hit)) continue;  // no part of this template reaches the threshold (uniform)
+#pragma unroll
+        for (int sg = 0; sg < 2; ++sg) {
+          float bz = -INFINITY;
+          int bk = 1 << 16, bp = 0;
+#pragma unroll
+          for (int i = 0; i < PPL; ++i) {
+            const bool take = cseg[i] == (sg == 1) && z[i] >= cr.thresh &&
+                              (z[i] > bz || (z[i] == bz && ckey[i] < bk));
+            bz = take ? z[i] : bz;
+            bk = take ? ckey[i] : bk;
+            bp = take ? cpix[i] : bp;
+          }
+          for (int o = 32; o > 0; o >>= 1) {
+            const float oz = __shfl_xor(bz, o, 64);
+            const int ok = __shfl_xor(bk, o, 64), op = __shfl_xor(bp, o, 64);
+            const bool take = oz > bz || (oz == bz && ok < bk);
+            bz = take ? oz : bz;
+            bk = take ? ok : bk;
+            bp = take ? op : bp;
+          }
+          if (ln == 0 && bz >= cr.thresh) {
+            const int slot = atomicAdd(cr.count, 1);
+            if (slot < cr.max_out) cr.rec[slot] = RefRec{cr.angle, b, cr.t0 + t + u, bp / 256, bp, bz};
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < (CR ? 0 : PPL); ++i) {
       const int r2 = i / (L / 64), xo = 64 * (i % (L / 64)) + lx;
       if constexpr (LZ) {
         float v[TPI];
@@ -1185,7 +1262,7 @@ __global__ void __launch_bounds__(256)
   if (tfull < tc) iteration(tfull, std::false_type{});
 
 #pragma unroll
-  for (int i = 0; i < PPL; ++i) {
+  for (int i = 0; i < (CR ? 0 : PPL); ++i) {
     const long long gi = record(i);
     S[gi] += (double)s[i];
     SS[gi] += (double)ss[i];
@@ -1719,6 +1796,20 @@ bool launch_rowpass_pixel(int L, const float2* Z, int tc, int nb, double* S, dou
     return true;
   }
   return false;
+}
+
+bool launch_rowpass_chunk_records(int L, const float2* Z, int tc, int nb, const double* mu, const double* sigma,
+                                  float thresh, int angle, int t0, RefRec* rec, int* count, int max_out,
+                                  hipStream_t s) {
+  if (L != 320) return false;
+  TwTables& T = tw_tables(current_device());
+  constexpr int rows = 4 * (64 / CorrCfg<320>::TPT) / 8;
+  dim3 grid(320 / rows, nb);
+  const ChunkRecArgs cr{mu, sigma, thresh, angle, t0, max_out, rec, count};
+  k_rowpass_pix_wave<320, true, 8, true, true, false, true><<<grid, 256, 0, s>>>(
+      Z, T.t320, T.t160, tc, nb, nullptr, nullptr, nullptr, nullptr, 0, cr);
+  GS_CHECK_LAUNCH();
+  return true;
 }
 
 bool launch_rowpass_plane(int L, const float2* Z, int tc, int nb, float4* partial, float border,

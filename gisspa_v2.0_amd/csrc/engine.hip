@@ -1162,18 +1162,27 @@ class Engine {
     launch_finalize_norm(d_S_.p, d_SS_.p, d_best_.p, d_arg_.p, nb_, l, total_n, P_.threshold, doffx,
                          doffy, bx_, W_, H_, P_.diameter, d_mu_.ensure(npix), d_sigma_.ensure(npix),
                          cand, count, max_cand, stream_);
+    ref_parts_ = false;
     if (!P_.reference_records) return;
     // Second pass of the reference (PickParticles, src/norm.cu:560-600): every correlation again,
-    // now with mu and sigma known, reduced to the reference's per-256-chunk records.  It runs on
-    // the generic correlation, whose two buffers take the inter-pass buffer's memory.
-    corr_generic_prepare(fft_, l);
-    const size_t per_t = 2 * corr_generic_buffer(l, 1, nb_);
-    const int tc2 = (int)std::max<size_t>(1, std::min<size_t>((size_t)nt, zsz * (overlap ? 2 : 1) / per_t));
-    float2* P2 = d_Z_.ensure(per_t * tc2);
-    float2* W2 = P2 + per_t / 2 * tc2;
+    // now with mu and sigma known, reduced to the reference's per-256-chunk records.  At l = 320
+    // it runs on the tiled correlation (a row-packed column pass, then the row pass's record
+    // form, which appends chunk parts that collect_reference merges); at other sizes on the
+    // generic correlation, whose two buffers take the inter-pass buffer's memory.
     RefRec* rec = d_refrec_.ensure(kMaxRefRecords);
     int* rcount = d_refcount_.ensure(1);
     GS_HIP(hipMemsetAsync(rcount, 0, sizeof(int), stream_));
+    ref_parts_ = !gen && l == 320;
+    const int colv = col_variant_ >= 8 && col_variant_ <= 13 ? col_variant_ : 12;  // a row-packed layout
+    int tc2 = tc_max;
+    float2 *P2 = nullptr, *W2 = nullptr;
+    if (!ref_parts_) {
+      corr_generic_prepare(fft_, l);
+      const size_t per_t = 2 * corr_generic_buffer(l, 1, nb_);
+      tc2 = (int)std::max<size_t>(1, std::min<size_t>((size_t)nt, zsz * (overlap ? 2 : 1) / per_t));
+      P2 = d_Z_.ensure(per_t * tc2);
+      W2 = P2 + per_t / 2 * tc2;
+    }
     for (int a = 0; a < na; ++a) {
       const float cose = (float)std::cos((double)angles[a] * kRefPi / 180);
       const float sine = (float)std::sin((double)angles[a] * kRefPi / 180);
@@ -1185,7 +1194,16 @@ class Engine {
         launch_unpack_pair_spectra(work, nt, 1, l, stats, (float)(1.0 / (double)L2), that, stream_);
       }
       for (int t0 = 0; t0 < nt; t0 += tc2) {
-        const CorrArgs ca{that, ihat, P2, t0, std::min(tc2, nt - t0), nb_};
+        const int tc = std::min(tc2, nt - t0);
+        if (ref_parts_) {
+          const CorrArgs ca{that, ihat, Zb, t0, tc, nb_};
+          GS_REQUIRE(launch_colpass(l, ca, colv, stream_), kUnsupported, "window_size not supported");
+          GS_REQUIRE(launch_rowpass_chunk_records(l, Zb, tc, nb_, d_mu_.p, d_sigma_.p, P_.threshold, a, t0, rec,
+                                                  rcount, kMaxRefRecords, stream_),
+                     kUnsupported, "window_size not supported");
+          continue;
+        }
+        const CorrArgs ca{that, ihat, P2, t0, tc, nb_};
         launch_corr_generic_records(fft_, ca, l, P2, W2, d_mu_.p, d_sigma_.p, a, P_.threshold, doffx, doffy, bx_,
                                     W_, H_, P_.diameter, rec, rcount, kMaxRefRecords, stream_);
       }
@@ -1209,6 +1227,26 @@ class Engine {
       return a.chunk < b.chunk;
     });
     const int l = P_.window_size;
+    if (ref_parts_) {
+      // Parts of one (angle, block, template, chunk): the best score wins, ties to the smaller
+      // bit-reversed offset in the chunk (get_peak_pos's tree, k:766-793); then the centre range
+      // of src/norm.cu:583-590 applies to the winner.
+      auto key = [](const RefRec& r) { return __builtin_bitreverse32((unsigned)(r.pixel % 256)) >> 24; };
+      std::vector<RefRec> m;
+      for (size_t i = 0; i < c.size();) {
+        size_t j = i;
+        RefRec w = c[i];
+        for (; j < c.size() && c[j].angle == c[i].angle && c[j].block == c[i].block && c[j].templ == c[i].templ &&
+               c[j].chunk == c[i].chunk;
+             ++j)
+          if (c[j].score > w.score || (c[j].score == w.score && key(c[j]) < key(w))) w = c[j];
+        const int cx = offx_[w.block % bx_] + w.pixel % l, cy = offy_[w.block / bx_] + w.pixel / l;
+        const float dm = P_.diameter;
+        if (cx >= dm && cx < W_ - dm && cy >= dm && cy < H_ - dm) m.push_back(w);
+        i = j;
+      }
+      c.swap(m);
+    }
     records_.reserve(count);
     for (const RefRec& k : c) {
       gs_record r{};
@@ -1435,6 +1473,7 @@ class Engine {
   hipEvent_t ev_zready_[2] = {nullptr, nullptr}, ev_zfree_[2] = {nullptr, nullptr}, ev_sync_ = nullptr;
   hipEvent_t ev_tprep_ = nullptr;
   bool overlap_ = false;
+  bool ref_parts_ = false;  // the reference records of the last run are chunk parts (tiled pass)
   hipEvent_t ev_start_ = nullptr, ev_corr0_ = nullptr, ev_corr1_ = nullptr, ev_end_ = nullptr;
   KernelClock clk_col_, clk_row_;
   bool profile_ = false;

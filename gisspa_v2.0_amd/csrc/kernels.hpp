@@ -112,6 +112,14 @@ bool launch_rowpass_pixel(int L, const float2* Z, int tc, int nb, double* S, dou
 // launch_rowpass_plane reads (variant ignored there).
 bool launch_colpass_plane(int L, const CorrArgs& a, int variant, hipStream_t s);
 // Row pass, norm_type=0: C2R rows + per-(template,block,row group) peak/sum partials.
+struct RefRec;
+// The reference's records of one launch from the row-packed inter-pass buffer (l = 320; false
+// elsewhere): per template and 256-element chunk part of each row, the best z at or above
+// thresh, appended to rec (RefRec::chunk, pixel, score); the parts of a chunk are merged, and
+// the centre range applied, by the caller.
+bool launch_rowpass_chunk_records(int L, const float2* Z, int tc, int nb, const double* mu, const double* sigma,
+                                  float thresh, int angle, int t0, RefRec* rec, int* count, int max_out,
+                                  hipStream_t s);
 bool launch_rowpass_plane(int L, const float2* Z, int tc, int nb, float4* partial, float border,
                           hipStream_t s);
 int rowpass_groups(int L);

@@ -13,8 +13,10 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("l", [320, 256])
-def test_reference_records_match_the_oracle(gs, orc, l):
+@pytest.mark.parametrize("l,chunk", [(320, 0), (320, 4), (256, 0)])
+def test_reference_records_match_the_oracle(gs, orc, l, chunk):
+    """l = 320 runs the tiled second pass (chunk parts merged on the host; chunk 4: two launches
+    per angle, the second partial), l = 256 the generic one."""
     import synth
 
     cfg = gs.SearchConfig(angpix=1.0, phistep=20.0, kk=3.0, energy=300.0, cs=2.7, highres=8.0,
@@ -24,6 +26,8 @@ def test_reference_records_match_the_oracle(gs, orc, l):
                             defocus=1.8, dfdiff=0.003, dfang=15.0)
     o = orc.search_norm_records(cfg, ds.image, ds.templates, ds.ctf)
     with gs.Engine(cfg) as eng:
+        if chunk:
+            eng.set_chunk(chunk)
         eng.load_templates(ds.templates)
         g = eng.search(ds.image, ds.ctf)
     assert len(o) > 20
@@ -48,4 +52,4 @@ def test_reference_records_match_the_oracle(gs, orc, l):
             near_ties += 1
         assert int(b["pixel"]) // 256 == k[3] and k[3] < chunks
         assert b["x"] == a["x"] or int(b["pixel"]) != int(a["pixel"])
-    print(f"l={l}: {len(o)} reference records, {len(both)} matched, {near_ties} near-tie pixels")
+    print(f"l={l} chunk={chunk}: {len(o)} reference records, {len(both)} matched, {near_ties} near-tie pixels")
