@@ -321,15 +321,22 @@ __global__ void __launch_bounds__(256)
   else
     twr.load(twL, tid);
   double a[4] = {0, 0, 0, 0};
-  // pixels e = tid + 256 k in batches of BT, all gathers of a batch in flight together; the mask
-  // factors of the next batch are loaded while the current one is processed
+  // work items n = tid + 256 k in batches of BT, all gathers of a batch in flight together; the
+  // mask factors of the next batch are loaded while the current one is processed.  Item n is
+  // pixel sq(n): each wavefront's 64 items are an 8 x 8 square of the workgroup's rows, so that
+  // its bilinear taps fall on a compact patch of the source plane at every angle.
   constexpr int N = RPB * L, NE = (N + 255) / 256, BT = RT ? 3 : 4;
+  static_assert(RPB % 8 == 0 && L % 8 == 0, "8 x 8 squares");
+  auto sq = [](int n) {
+    const int s = n / 64, ln = n % 64, sr = s / (L / 8), sc = s - sr * (L / 8);
+    return (8 * sr + ln / 8) * L + 8 * sc + ln % 8;
+  };
   const float* Mrows = mask + (long long)y0 * L;
   float mnext[BT];
 #pragma unroll
   for (int b = 0; b < BT; ++b) {
-    const int e = tid + 256 * b;
-    mnext[b] = (b < NE && e < N) ? Mrows[e] : 0.f;
+    const int n = tid + 256 * b;
+    mnext[b] = (b < NE && n < N) ? Mrows[sq(n)] : 0.f;
   }
   for (int k0 = 0; k0 < NE; k0 += BT) {
     float2 h[BT][2][4];
@@ -339,7 +346,7 @@ __global__ void __launch_bounds__(256)
     for (int b = 0; b < BT; ++b) mk[b] = mnext[b];
 #pragma unroll
     for (int b = 0; b < BT; ++b) {
-      const int e = tid + 256 * (k0 + b);
+      const int e = sq(tid + 256 * (k0 + b));
       const int r = e / L, i = e - r * L, jrow = y0 + r;
 #pragma unroll
       for (int q = 0; q < 2; ++q) {
@@ -361,13 +368,14 @@ __global__ void __launch_bounds__(256)
     }
 #pragma unroll
     for (int b = 0; b < BT; ++b) {
-      const int e = tid + 256 * (k0 + BT + b);
-      mnext[b] = (k0 + BT + b < NE && e < N) ? Mrows[e] : 0.f;
+      const int n = tid + 256 * (k0 + BT + b);
+      mnext[b] = (k0 + BT + b < NE && n < N) ? Mrows[sq(n)] : 0.f;
     }
 #pragma unroll
     for (int b = 0; b < BT; ++b) {
-      const int e = tid + 256 * (k0 + b);
-      if (!(k0 + b < NE && e < N)) continue;
+      const int n = tid + 256 * (k0 + b);
+      if (!(k0 + b < NE && n < N)) continue;
+      const int e = sq(n);
       const int r = e / L, i = e - r * L, jrow = y0 + r;
       const float m = mk[b];
       float2 out = make_float2(0.f, 0.f);
