@@ -603,17 +603,18 @@ def main():
     tpath = os.path.join(ROOT, "profiles", "corr_c1_traffic.json")
     traffic = {"column_pass": None, "row_pass": None}
     if os.path.exists(tpath) and l == 320:
-        col_name = {0: "k_colpass<320, 0, 0, 8, 16>", 3: "k_colpass<320, 1, 0, 8, 16>",
-                    4: "k_colpass<320, 1, 1, 8, 16>", 5: "k_colpass<320, 0, 1, 8, 16>",
-                    6: "k_colpass<320, 1, 2, 8, 16>", 7: "k_colpass<320, 1, 3, 8, 16>",
-                    8: "k_colpass<320, 1, 4, 8, 16>", 12: "k_colpass<320, 1, 4, 4, 4, true>",
+        # demangled names, every template argument spelled out as rocprofv3 prints them
+        col_name = {0: "k_colpass<320, 0, 0, 8, 16, false>", 3: "k_colpass<320, 1, 0, 8, 16, false>",
+                    4: "k_colpass<320, 1, 1, 8, 16, false>", 5: "k_colpass<320, 0, 1, 8, 16, false>",
+                    6: "k_colpass<320, 1, 2, 8, 16, false>", 7: "k_colpass<320, 1, 3, 8, 16, false>",
+                    8: "k_colpass<320, 1, 4, 8, 16, false>", 12: "k_colpass<320, 1, 4, 4, 4, true>",
                     13: "k_colpass<320, 1, 4, 2, 8, true>"}.get(variant[0], "")
-        row_name = "k_rowpass_pix_wave<320, %s>" % {
-            0: "false, 4, false, false", 1: "true, 4, false, false", 2: "true, 8, false, false",
-            3: "false, 8, false, false",
-            4: "true, 4, true, false", 5: "false, 4, true, false", 6: "true, 8, true, false",
-            7: "true, 8, true, true", 8: "true, 8, true, true, true",
-            9: "true, 8, true, false, true"}.get(variant[1], "")
+        row_name = "k_rowpass_pix_wave<320, %s, false>" % {
+            0: "false, 4, false, false, false", 1: "true, 4, false, false, false",
+            2: "true, 8, false, false, false", 3: "false, 8, false, false, false",
+            4: "true, 4, true, false, false", 5: "false, 4, true, false, false",
+            6: "true, 8, true, false, false", 7: "true, 8, true, true, false",
+            8: "true, 8, true, true, true", 9: "true, 8, true, false, true"}.get(variant[1], "")
         recs_t = json.load(open(tpath))["kernels"]
         for key, kname in (("column_pass", col_name), ("row_pass", row_name)):
             rec = recs_t.get(kname)
