@@ -252,7 +252,8 @@ class Engine {
     clk_row_.destroy();
     if (stream2_) {
       (void)hipStreamSynchronize(stream2_);
-      for (hipEvent_t e : {ev_zready_[0], ev_zready_[1], ev_zfree_[0], ev_zfree_[1], ev_sync_})
+      for (hipEvent_t e :
+           {ev_zready_[0], ev_zready_[1], ev_zfree_[0], ev_zfree_[1], ev_sync_, ev_tprep_, ev_rfork_, ev_rjoin_})
         if (e) (void)hipEventDestroy(e);
       (void)hipStreamDestroy(stream2_);
     }
@@ -688,87 +689,128 @@ class Engine {
     launch_ref_table(didx, ntab, l, dp_, tab, stream_);
     float2* ihat_t = d_rIT_.ensure((size_t)nb_ * hx * l);
     transpose_batched(d_ihat_.p, ihat_t, nb_, hx, l, l, stream_);
-    const int Kc = (int)std::min<size_t>(k, kRefineChunk);
-    // weighted spectra (written on the band-pass support only, zero elsewhere for the whole call)
-    // and their inverse transforms
-    float2* H = d_rH_.ensure((size_t)Kc * L2);
-    launch_ref_clear(H, (long long)Kc * L2, stream_);
-    float2* Tt = d_rT_.ensure((size_t)Kc * L2);
-    // K planes A (r + i dr/dtheta) and (K + 1) / 2 planes holding the B fields of two picks each
-    float2* AB = d_rAB_.ensure((size_t)(Kc + (Kc + 1) / 2) * L2);
-    // scratch of the runtime-planned transform, needed only where no LDS plane transform exists
-    float2* S = corr_supported(l) ? nullptr : d_rS_.ensure((size_t)(Kc + (Kc + 1) / 2) * L2);
-    RefPeak* rp = d_rpk_.ensure(Kc);
-    double2* part = d_rpart_.ensure((size_t)Kc * 16);
-    double* part6 = d_rpart6_.ensure((size_t)Kc * rotation_groups(l) * 6);
-    double* stats = d_rstats_.ensure((size_t)Kc * 8);
-    double* cg = d_rcg_.ensure((size_t)Kc * std::max(kRefDotSplit, ref_cols_dot_parts(l)) * 5);
-    float2* phs = d_rph_.ensure((size_t)Kc * (2 * l + 2 * hx));
-    double* dz = d_rz_.ensure(Kc);
-    double* dz0 = d_rz0_.ensure(Kc);
-    double* dg = d_rgrad_.ensure((size_t)Kc * 4);
     // the rotation's mask table and the rows it leaves non-zero (the fused rotation kernels)
     int ylo = 0, yhi = l - 1;
     float* mtab = d_rmask_.ensure((size_t)L2);
     launch_ref_mask(l, P_.diameter, 4.0f, mtab, &ylo, &yhi, stream_);
-    std::vector<RefPeak> hp(Kc);
-    std::vector<double> hz(Kc), hz0(Kc), hg((size_t)Kc * 4);
-    for (size_t c0 = 0; c0 < k; c0 += Kc) {
-      const int K = (int)std::min<size_t>(Kc, k - c0);
+    // Picks are independent but for the B planes, which hold the defocus fields of picks 2m and
+    // 2m + 1: batches start at even indices, so a pick's partner (and its result) does not
+    // depend on the batching.  Where the plane transforms are the stateless LDS kernels
+    // (l = 320, 720), two batches run side by side on two streams, each with its own buffers, so
+    // that one batch's transforms overlap the other's rotation and dot and each fills the
+    // other's launch gaps and kernel tails.
+    const int nl = corr_supported(l) && k >= 2 * kRefineMinLane ? 2 : 1;
+    size_t per = (k + nl - 1) / nl;
+    per += per & 1;
+    const int Kc = (int)std::min<size_t>(per, kRefineChunk);
+    hipStream_t ls[2] = {stream_, stream_};
+    if (nl == 2) {
+      ensure_pipeline();
+      ls[1] = stream2_;
+      GS_HIP(hipEventRecord(ev_rfork_, stream_));  // tables, mask and block spectra are ready
+      GS_HIP(hipStreamWaitEvent(stream2_, ev_rfork_, 0));
+    }
+    for (int ln = 0; ln < nl; ++ln) {
+      RefLane& R = rlane_[ln];
+      // weighted spectra (written on the band-pass support only, zero elsewhere for the whole
+      // call) and their inverse transforms
+      launch_ref_clear(R.H.ensure((size_t)Kc * L2), (long long)Kc * L2, ls[ln]);
+      R.T.ensure((size_t)Kc * L2);
+      // K planes A (r + i dr/dtheta) and (K + 1) / 2 planes holding the B fields of two picks each
+      R.AB.ensure((size_t)(Kc + (Kc + 1) / 2) * L2);
+      R.pk.ensure(Kc);
+      R.part.ensure((size_t)Kc * 16);
+      R.part6.ensure((size_t)Kc * rotation_groups(l) * 6);
+      R.stats.ensure((size_t)Kc * 8);
+      R.cg.ensure((size_t)Kc * std::max(kRefDotSplit, ref_cols_dot_parts(l)) * 5);
+      R.ph.ensure((size_t)Kc * (2 * l + 2 * hx));
+      R.z.ensure(Kc);
+      R.z0.ensure(Kc);
+      R.grad.ensure((size_t)Kc * 4);
+      R.hp.resize(Kc);
+      R.hz.resize(Kc);
+      R.hz0.resize(Kc);
+      R.hg.resize((size_t)Kc * 4);
+    }
+    // scratch of the runtime-planned transform, needed only where no LDS plane transform exists
+    float2* S = corr_supported(l) ? nullptr : d_rS_.ensure((size_t)(Kc + (Kc + 1) / 2) * L2);
+    // one batch of K picks from c0 on lane ln: upload, iters + 1 evaluations, download queued
+    auto enqueue = [&](int ln, size_t c0, int K) {
+      RefLane& R = rlane_[ln];
+      const hipStream_t st = ls[ln];
+      RefPeak* rp = R.pk.p;
+      float2 *H = R.H.p, *Tt = R.T.p, *AB = R.AB.p;
       for (int j = 0; j < K; ++j) {
         const gs_peak& p = in[c0 + j];
-        hp[j] = RefPeak{p.block, p.pixel, p.template_id, 0, p.dx, p.dy, p.theta, p.ddf};
+        R.hp[j] = RefPeak{p.block, p.pixel, p.template_id, 0, p.dx, p.dy, p.theta, p.ddf};
       }
-      GS_HIP(hipMemcpyAsync(rp, hp.data(), sizeof(RefPeak) * K, hipMemcpyHostToDevice, stream_));
+      GS_HIP(hipMemcpyAsync(rp, R.hp.data(), sizeof(RefPeak) * K, hipMemcpyHostToDevice, st));
       // LDS plane transforms where built (l = 320, 720), the runtime-planned FFT otherwise
       auto fft2 = [&](float2* d, int batch, int sign, float scale) {
-        if (!launch_plane_fft2(d, batch, l, sign, scale, stream_))
-          fft2_batched(fft_, d, S, batch, l, l, sign, scale, stream_);
+        if (!launch_plane_fft2(d, batch, l, sign, scale, st)) fft2_batched(fft_, d, S, batch, l, l, sign, scale, st);
       };
       for (int it = 0; it <= iters; ++it) {
-        launch_ref_template(d_twhite_.p, tab, ntab, rp, K, l, dp_, part, count, H, stream_);
+        launch_ref_template(d_twhite_.p, tab, ntab, rp, K, l, dp_, R.part.p, count, H, st);
         // the rotation's bilinear taps read only columns (and rows) [ylo - 1, yhi + 1] of the
         // processed templates: rotation keeps the distance to the centre, which the mask bounds
-        if (!launch_plane_fft2_from(H, Tt, K, l, +1, 1.f, stream_, band, ylo - 2, yhi + 2)) {
-          GS_HIP(hipMemcpyAsync(Tt, H, sizeof(float2) * (size_t)K * L2, hipMemcpyDeviceToDevice, stream_));
-          fft2_batched(fft_, Tt, S, K, l, l, +1, 1.f, stream_);
+        if (!launch_plane_fft2_from(H, Tt, K, l, +1, 1.f, st, band, ylo - 2, yhi + 2)) {
+          GS_HIP(hipMemcpyAsync(Tt, H, sizeof(float2) * (size_t)K * L2, hipMemcpyDeviceToDevice, st));
+          fft2_batched(fft_, Tt, S, K, l, l, +1, 1.f, st);
         }
         // l = 320, 720: the rotation writes rows already transformed and the column pass of the
         // forward transform is fused with the dot, so the spectra of AB are never stored
         int parts = kRefDotSplit;
-        if (launch_ref_rotate_rows(Tt, rp, K, l, mtab, ylo, yhi, part6, stats, AB, stream_)) {
-          GS_REQUIRE(launch_ref_cols_dot(d_ihat_.p, AB, rp, stats, K, l, ylo, yhi, (float)(1.0 / (double)L2), phs,
-                                         cg, stream_),
+        if (launch_ref_rotate_rows(Tt, rp, K, l, mtab, ylo, yhi, R.part6.p, R.stats.p, AB, st)) {
+          GS_REQUIRE(launch_ref_cols_dot(d_ihat_.p, AB, rp, R.stats.p, K, l, ylo, yhi, (float)(1.0 / (double)L2),
+                                         R.ph.p, R.cg.p, st),
                      kUnsupported, "window_size not supported");
           parts = ref_cols_dot_parts(l);
         } else {
-          launch_ref_rotate(Tt, rp, K, l, P_.diameter, 4.0f, part6, stats, AB, stream_);
+          launch_ref_rotate(Tt, rp, K, l, P_.diameter, 4.0f, R.part6.p, R.stats.p, AB, st);
           fft2(AB, K + (K + 1) / 2, -1, (float)(1.0 / (double)L2));
-          GS_REQUIRE(launch_ref_dot(ihat_t, AB, rp, stats, K, l, cg, stream_), kUnsupported,
+          GS_REQUIRE(launch_ref_dot(ihat_t, AB, rp, R.stats.p, K, l, R.cg.p, st), kUnsupported,
                      "window_size not supported");
         }
-        launch_ref_update(rp, K, cg, parts, d_mu_.p, d_sigma_.p, l, step, scale, it == 0, it < iters, dz, dz0, dg,
-                          stream_);
+        launch_ref_update(rp, K, R.cg.p, parts, d_mu_.p, d_sigma_.p, l, step, scale, it == 0, it < iters, R.z.p,
+                          R.z0.p, R.grad.p, st);
       }
-      GS_HIP(hipMemcpyAsync(hp.data(), rp, sizeof(RefPeak) * K, hipMemcpyDeviceToHost, stream_));
-      GS_HIP(hipMemcpyAsync(hz.data(), dz, sizeof(double) * K, hipMemcpyDeviceToHost, stream_));
-      GS_HIP(hipMemcpyAsync(hz0.data(), dz0, sizeof(double) * K, hipMemcpyDeviceToHost, stream_));
-      GS_HIP(hipMemcpyAsync(hg.data(), dg, sizeof(double) * 4 * K, hipMemcpyDeviceToHost, stream_));
-      GS_HIP(hipStreamSynchronize(stream_));
+    };
+    // results of the batch of K picks from c0 on lane ln (waits for the lane)
+    auto collect = [&](int ln, size_t c0, int K) {
+      RefLane& R = rlane_[ln];
+      const hipStream_t st = ls[ln];
+      GS_HIP(hipMemcpyAsync(R.hp.data(), R.pk.p, sizeof(RefPeak) * K, hipMemcpyDeviceToHost, st));
+      GS_HIP(hipMemcpyAsync(R.hz.data(), R.z.p, sizeof(double) * K, hipMemcpyDeviceToHost, st));
+      GS_HIP(hipMemcpyAsync(R.hz0.data(), R.z0.p, sizeof(double) * K, hipMemcpyDeviceToHost, st));
+      GS_HIP(hipMemcpyAsync(R.hg.data(), R.grad.p, sizeof(double) * 4 * K, hipMemcpyDeviceToHost, st));
+      GS_HIP(hipStreamSynchronize(st));
       for (int j = 0; j < K; ++j) {
         const size_t i = c0 + j;
         if (out_pk) {
           out_pk[i] = in[i];
-          out_pk[i].dx = hp[j].dx;
-          out_pk[i].dy = hp[j].dy;
-          out_pk[i].theta = hp[j].theta;
-          out_pk[i].ddf = hp[j].ddf;
+          out_pk[i].dx = R.hp[j].dx;
+          out_pk[i].dy = R.hp[j].dy;
+          out_pk[i].theta = R.hp[j].theta;
+          out_pk[i].ddf = R.hp[j].ddf;
         }
-        if (z) z[i] = hz[j];
-        if (z0) z0[i] = hz0[j];
+        if (z) z[i] = R.hz[j];
+        if (z0) z0[i] = R.hz0[j];
         if (grad)
-          for (int q = 0; q < 4; ++q) grad[4 * i + q] = hg[4 * j + q];
+          for (int q = 0; q < 4; ++q) grad[4 * i + q] = R.hg[4 * j + q];
       }
+    };
+    for (size_t c0 = 0; c0 < k; c0 += (size_t)nl * Kc) {
+      int Kl[2] = {0, 0};
+      for (int ln = 0; ln < nl && c0 + (size_t)ln * Kc < k; ++ln) {
+        Kl[ln] = (int)std::min<size_t>(Kc, k - (c0 + (size_t)ln * Kc));
+        enqueue(ln, c0 + (size_t)ln * Kc, Kl[ln]);
+      }
+      for (int ln = 0; ln < nl; ++ln)
+        if (Kl[ln]) collect(ln, c0 + (size_t)ln * Kc, Kl[ln]);
+    }
+    if (nl == 2) {  // the end event on stream_ covers the second lane's work
+      GS_HIP(hipEventRecord(ev_rjoin_, stream2_));
+      GS_HIP(hipStreamWaitEvent(stream_, ev_rjoin_, 0));
     }
     GS_HIP(hipEventRecord(ev_end_, stream_));
     GS_HIP(hipEventSynchronize(ev_end_));
@@ -973,7 +1015,8 @@ class Engine {
   void ensure_pipeline() {
     if (stream2_) return;
     GS_HIP(hipStreamCreateWithFlags(&stream2_, hipStreamNonBlocking));
-    for (hipEvent_t* e : {&ev_zready_[0], &ev_zready_[1], &ev_zfree_[0], &ev_zfree_[1], &ev_sync_, &ev_tprep_})
+    for (hipEvent_t* e : {&ev_zready_[0], &ev_zready_[1], &ev_zfree_[0], &ev_zfree_[1], &ev_sync_, &ev_tprep_,
+                          &ev_rfork_, &ev_rjoin_})
       GS_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming));
   }
 
@@ -1471,7 +1514,7 @@ class Engine {
   int device_;
   hipStream_t stream_ = nullptr, stream2_ = nullptr;
   hipEvent_t ev_zready_[2] = {nullptr, nullptr}, ev_zfree_[2] = {nullptr, nullptr}, ev_sync_ = nullptr;
-  hipEvent_t ev_tprep_ = nullptr;
+  hipEvent_t ev_tprep_ = nullptr, ev_rfork_ = nullptr, ev_rjoin_ = nullptr;
   bool overlap_ = false;
   bool ref_parts_ = false;  // the reference records of the last run are chunk parts (tiled pass)
   hipEvent_t ev_start_ = nullptr, ev_corr0_ = nullptr, ev_corr1_ = nullptr, ev_end_ = nullptr;
@@ -1497,12 +1540,21 @@ class Engine {
   double total_n_ = 0, refine_ms_ = 0;
   static constexpr size_t kRefineChunk = 2048;  // picks per refinement batch (about 4 MB each at l=320)
   bool white_ok_ = false;
-  DevBuf<float2> d_twhite_, d_rH_, d_rT_, d_rAB_, d_rS_, d_rIT_;
+  DevBuf<float2> d_twhite_, d_rS_, d_rIT_;
   DevBuf<int32_t> d_ridx_;
   DevBuf<RefWeight> d_rtab_;
-  DevBuf<RefPeak> d_rpk_;
-  DevBuf<double2> d_rpart_, d_apart_, d_apart2_;
-  DevBuf<double> d_rpart6_, d_rstats_, d_rcg_, d_rz_, d_rz0_, d_rgrad_;
+  DevBuf<double2> d_apart_, d_apart2_;
+  // buffers of one refinement batch (one per stream lane) and its host copies
+  struct RefLane {
+    DevBuf<float2> H, T, AB, ph;
+    DevBuf<RefPeak> pk;
+    DevBuf<double2> part;
+    DevBuf<double> part6, stats, cg, z, z0, grad;
+    std::vector<RefPeak> hp;
+    std::vector<double> hz, hz0, hg;
+  };
+  RefLane rlane_[2];
+  static constexpr size_t kRefineMinLane = 64;  // picks per lane before a second lane is used
   static constexpr size_t kMaxGraphs = 8;
   bool use_graph_ = false;
   std::map<std::vector<long long>, CapturedGraph> graphs_;
@@ -1511,7 +1563,6 @@ class Engine {
   std::map<long long, std::unique_ptr<DevBuf<double>>> rb_cache_;
 
   DevBuf<float> d_rmask_;
-  DevBuf<float2> d_rph_;
   DevBuf<float> d_raw_, d_img_, d_imgpre_, d_tprep_, d_stats_, d_best_, d_blkreal_;
   DevBuf<float2> d_work_, d_work2_, d_scratch_, d_scratch2_, d_frame_, d_frame2_, d_blocks_, d_ihat_, d_that_, d_Z_,
       d_cs_;

@@ -147,23 +147,26 @@ def test_select_reproduces_and_extends_the_records(case):
 
 
 def test_refinement_beyond_one_device_batch(case, orc):
-    """More picks than one device batch of the refinement (2048): the second batch reuses the
-    buffers of the first with a smaller count, and its picks must come out exactly as a
-    separate call on those picks alone, and agree with the oracle on a sample."""
+    """More picks than two device batches of the refinement (2048 each, one per stream lane):
+    the first two batches run side by side on two streams, the third reuses the first lane's
+    buffers with a smaller count.  Picks of the second and third batches must come out exactly
+    as a separate call on those picks alone, and a sample of every batch must agree with the
+    oracle."""
     eng = case["eng"]
     base = _perturbed(case, 6, seed=11)
-    k = 2100
+    k = 4200
     pk = np.concatenate([base] * (k // len(base) + 1))[:k].copy()
     rng = np.random.default_rng(12)
     pk["dx"] += rng.uniform(-0.2, 0.2, k)
     pk["theta"] += rng.uniform(-0.5, 0.5, k)
     out = eng.refine(pk, iterations=3)
-    tail = eng.refine(pk[2048:].copy(), iterations=3)
-    assert np.array_equal(out["score"][2048:], tail["score"])
-    assert np.array_equal(out["peak"][2048:], tail["peak"])
+    for lo, hi in ((4096, 4200), (2048, 2100)):
+        part = eng.refine(pk[lo:hi].copy(), iterations=3)
+        assert np.array_equal(out["score"][lo:hi], part["score"])
+        assert np.array_equal(out["peak"][lo:hi], part["peak"])
     head = eng.refine(pk[:8].copy(), iterations=3)
     np.testing.assert_allclose(out["score"][:8], head["score"], rtol=1e-6)
-    idx = np.array([0, 2047, 2048, 2099])
+    idx = np.array([0, 2047, 2048, 4095, 4096, 4199])
     mu, sg = _frozen(case, pk[idx])
     o, zo, _, _ = case["R"].run(_as_oracle(orc, pk[idx]), mu, sg, iters=3, step=0.1,
                                 scale=(2.0, 2.0, 3.0, 500.0))
