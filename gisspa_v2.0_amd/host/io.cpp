@@ -135,6 +135,17 @@ static bool read_mrc_sections(const std::string& path, int first, int count, Mrc
     return false;
   }
   const size_t sec = (size_t)nx * ny, n = sec * count;
+  // the sections must lie inside the file before anything is allocated for them (a header with
+  // huge dimensions is a truncated or foreign file, not a request for terabytes)
+  f.seekg(0, std::ios::end);
+  const unsigned long long size = (unsigned long long)(std::streamoff)f.tellg();
+  const unsigned long long head = 1024ull + (unsigned long long)ext;
+  const unsigned long long per = (unsigned long long)sec * bytes;  // < 2^64: nx, ny < 2^31
+  if (size < head || (unsigned long long)(first + count) > (size - head) / per) {
+    err = "truncated MRC data: " + path + " (" + std::to_string(size) + " bytes for " +
+          std::to_string(first + count) + " sections of " + std::to_string(per) + ")";
+    return false;
+  }
   f.seekg(1024 + (std::streamoff)ext + (std::streamoff)(sec * first * bytes));
   std::vector<char> raw(n * bytes);
   if (!f.read(raw.data(), raw.size())) {
