@@ -1,12 +1,25 @@
 // Shared helpers for the GisSPA HIP library: error handling and complex arithmetic.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <rocprofiler-sdk-roctx/roctx.h>
 
 #include <cstdint>
+#include <optional>
 #include <stdexcept>
 #include <string>
 
 namespace gs {
+
+// Host-side roctx range around a stage of the search (template and image preparation, the
+// angle loop, finalisation, record collection, refinement); `rocprofv3 --marker-trace` shows
+// them beside the kernels.  Kernel launches are asynchronous, so a range covers the stage's
+// enqueue time unless the stage ends in a synchronisation (the whole search does).
+struct Range {
+  explicit Range(const char* name) { roctxRangePushA(name); }
+  ~Range() { roctxRangePop(); }
+  Range(const Range&) = delete;
+  Range& operator=(const Range&) = delete;
+};
 
 struct Error : std::runtime_error {
   int code;

@@ -414,6 +414,7 @@ class Engine {
   void run() {
     GS_REQUIRE(n_ > 0, kState, "no templates loaded");
     GS_REQUIRE(have_image_, kState, "no image set");
+    Range whole("gs_search");
     bind();
     set_params();
     records_.clear();
@@ -431,6 +432,7 @@ class Engine {
     corr_ms_ = ms;
     clk_col_.harvest();
     clk_row_.harvest();
+    Range rec("records");
     if (P_.norm_type && P_.reference_records) {
       collect_reference();  // the reference's own records, unsuppressed, in its order
     } else {
@@ -651,6 +653,7 @@ class Engine {
   // gradient (k x 4: per px, px, deg, Angstrom).
   void refine(const gs_peak* in, size_t k, int iters, double step, const double* scale, gs_peak* out_pk,
               double* z, double* z0, double* grad) {
+    Range range("refine");
     GS_REQUIRE(P_.norm_type == 1, kUnsupported, "refinement is defined for norm_type=1");
     GS_REQUIRE(ran_ && have_image_ && n_ > 0 && nb_ > 0, kState, "refinement needs a completed norm_type=1 run");
     GS_REQUIRE(in && k > 0, kInvalid, "no picks");
@@ -1048,7 +1051,11 @@ class Engine {
   void run_norm() {
     const int l = P_.window_size, hx = l / 2 + 1, n = n_;
     const size_t L2 = (size_t)l * l;
-    prep_templates();
+    {
+      Range r("templates");
+      prep_templates();
+    }
+    std::optional<Range> img_range(std::in_place, "image_and_blocks");
     float* pre = d_imgpre_.ensure((size_t)W_ * H_);
     prep_frame(d_img_.p, W_, H_, true, pre);
     // SplitImage (src/norm.cu:404-477)
@@ -1104,6 +1111,8 @@ class Engine {
     // template pairs share one complex plane per angle (real and imaginary parts)
     float2* work = d_work_.ensure((size_t)((n + 1) / 2) * L2);
     float2* tscr = d_scratch_.ensure((size_t)((n + 1) / 2) * L2);
+    img_range.reset();
+    std::optional<Range> loop_range(std::in_place, "angle_loop");
     GS_HIP(hipEventRecord(ev_corr0_, stream_));
     if (overlap) {  // accumulators initialised on stream_ before the first row pass
       GS_HIP(hipEventRecord(ev_sync_, stream_));
@@ -1194,6 +1203,8 @@ class Engine {
       GS_HIP(hipStreamWaitEvent(stream_, ev_sync_, 0));
     }
     GS_HIP(hipEventRecord(ev_corr1_, stream_));
+    loop_range.reset();
+    Range fin_range("finalize");
     // set_CCG_mean: n = N_t * int(360/phistep) (src/norm.cu:549-550)
     double total_n = (double)n * (int)(360 / P_.phistep);
     if (lim_angles_ > 0 || lim_templates_ > 0) total_n = (double)nt * na;
@@ -1412,6 +1423,7 @@ class Engine {
     float2* dcs = d_cs_.ensure(na);
     GS_HIP(hipMemcpyAsync(dcs, cs.data(), sizeof(float2) * na, hipMemcpyHostToDevice, stream_));
     const float border = P_.diameter / 4;
+    std::optional<Range> loop_range(std::in_place, "angle_loop");
     GS_HIP(hipEventRecord(ev_corr0_, stream_));
     // TileImages (src/utils/tileimages.cpp:66-71), the per-tile preparation (src/nonorm.cu:346-389)
     // and SplitImage (src/nonorm.cu:391-450) of every tile
@@ -1458,6 +1470,7 @@ class Engine {
       }
     }
     GS_HIP(hipEventRecord(ev_corr1_, stream_));
+    loop_range.reset();
     planes_host_.resize((size_t)n * na * B * 4);
   }
 

@@ -19,7 +19,7 @@ Config::Config() {
       {"GPU_ID", -1},       {"window_size", 320}, {"phase_flip", 0},    {"overlap", 0},
       {"norm_type", 0},     {"invert", 0},
       // new keys
-      {"nms_radius", 0.0f}, {"tile_local_centers", 0}, {"reference_records", 0},
+      {"nms_radius", 0.0f}, {"tile_local_centers", 0}, {"reference_records", 0}, {"resume", 0},
   };
 }
 

@@ -10,7 +10,7 @@ namespace gsio {
 
 // `key = value` config (reference Config, include/EMReader/DataReader2.h:17-71 and
 // src/EMReader/DataReader2.cpp:6-53): keys with defaults, types from the defaults, unknown keys
-// ignored.  Three keys are new: nms_radius, tile_local_centers and reference_records.
+// ignored.  Four keys are new: nms_radius, tile_local_centers, reference_records and resume.
 struct Config {
   using Node = std::variant<int, float, std::string>;
   std::map<std::string, Node> value;

@@ -7,10 +7,20 @@
 // are reported with a non-zero exit status instead of being printed and ignored, and records
 // pass through NMS when nms_radius > 0.  With reference_records = 1 (norm_type=1) the records are
 // the reference's own per-chunk records, in its order and without NMS.
+//
+// Checkpoint and resume (the reference has neither: its output is truncated on start,
+// src/main.cu:34).  Beside the output, `<output>.done` lists every finished micrograph as
+// "index bytes path", bytes being the output's size once that micrograph's records are flushed.
+// A run with resume = 1 cuts the output back to the last listed size (dropping the lines of a
+// micrograph that was being written when a run stopped), appends to it and skips the listed
+// micrographs; with resume = 0 both files start empty.
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
+#include <filesystem>
 #include <fstream>
+#include <set>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -82,15 +92,44 @@ int main(int argc, char** argv) {
     gs_destroy(eng);
     return die(e);
   }
-  std::ofstream out(conf.gets("output"), std::ios::out | std::ios::trunc);
-  if (!out) {
+  const std::string out_path = conf.gets("output"), done_path = out_path + ".done";
+  std::set<int> done;
+  if (conf.geti("resume")) {
+    unsigned long long keep = 0;
+    std::ifstream mf(done_path);
+    std::string line;
+    while (std::getline(mf, line)) {
+      std::istringstream ls(line);
+      int idx;
+      unsigned long long bytes;
+      if (ls >> idx >> bytes) {
+        done.insert(idx);
+        keep = bytes;
+      }
+    }
+    std::error_code ec;
+    if (std::filesystem::exists(out_path, ec) && std::filesystem::file_size(out_path, ec) > keep)
+      std::filesystem::resize_file(out_path, keep, ec);
+    if (ec) {
+      gs_destroy(eng);
+      return die("cannot cut " + out_path + " back to its last finished micrograph: " + ec.message());
+    }
+  }
+  const auto mode = conf.geti("resume") ? std::ios::out | std::ios::app : std::ios::out | std::ios::trunc;
+  std::ofstream out(out_path, mode);
+  std::ofstream manifest(done_path, mode);
+  if (!out || !manifest) {
     gs_destroy(eng);
-    return die("cannot write output " + conf.gets("output"));
+    return die("cannot write output " + out_path + " or " + done_path);
   }
   int status = 0;
   long long lines = 0;
   for (int i = first; i < last; ++i) {
     const auto& e = lst[i];
+    if (done.count(i)) {
+      std::printf("%s: done in an earlier run, skipped\n", e.rpath.c_str());
+      continue;
+    }
     gsio::Mrc img;
     if (!gsio::read_image(e.rpath, e.unused, img, err)) {  // image number = LST index
       status = die(err);
@@ -111,6 +150,21 @@ int main(int argc, char** argv) {
       ++lines;
     }
     gs_free(recs);
+    out.flush();
+    if (!out) {
+      status = die("cannot write output " + out_path);
+      break;
+    }
+    // the size on disk (in append mode the stream's own position is not the file's size before
+    // its first write)
+    std::error_code ec;
+    const unsigned long long size = std::filesystem::file_size(out_path, ec);
+    if (ec) {
+      status = die("cannot stat output " + out_path + ": " + ec.message());
+      break;
+    }
+    manifest << i << " " << size << " " << e.rpath << "\n";
+    manifest.flush();
     auto ms = std::chrono::duration_cast<std::chrono::milliseconds>(std::chrono::steady_clock::now() - t0).count();
     std::printf("%s: %zu records in %lld ms\n", e.rpath.c_str(), nrec, (long long)ms);
   }

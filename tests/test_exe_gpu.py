@@ -97,3 +97,49 @@ def test_executable_reads_hdf5_templates_and_stack_sections(tmp_path, gs):
     for ga, gb in zip(a, b):  # identical except the micrograph index and path columns
         assert ga[2:] == gb[2:]
     assert [g[0] for g in b] == sorted(g[0] for g in b) and {g[0] for g in b} == {"0", "1"}
+
+
+def test_executable_resumes_from_its_manifest(tmp_path, gs):
+    """A run cut short and resumed with resume = 1 writes the same output file as one whole run:
+    finished micrographs are skipped, and lines past the last finished one are cut away."""
+    import synth
+
+    mics = []
+    for k, seed in enumerate((31, 32, 33)):
+        ds = synth.make_dataset(seed, 640, 640, n_templates=4, tsize=128, n_particles=3, snr=0.6,
+                                defocus=1.2 + 0.2 * k)
+        path = tmp_path / f"mic_{k}.mrc"
+        synth.write_mrc(str(path), ds.image)
+        mics.append((path, ds))
+    synth.write_mrc(str(tmp_path / "templates.mrc"), mics[0][1].templates)
+    (tmp_path / "euler.txt").write_text("".join(f"{j} {a:.4f} {b:.4f} 0.0\n"
+                                               for j, (a, b, _) in enumerate(mics[0][1].eulers)))
+    (tmp_path / "list.lst").write_text("".join(
+        f"{k} {p} defocus={d.ctf[0]:.6f} dfdiff={d.ctf[1]:.6f} dfang={d.ctf[2]:.6f}\n"
+        for k, (p, d) in enumerate(mics)))
+
+    def run(last, resume, out):
+        (tmp_path / "config.txt").write_text(
+            "input = list.lst\ntemplate = templates.mrc\neulerfile = euler.txt\nangpix = 1.0\n"
+            "phistep = 10\nkk = 3\nenergy = 300\ncs = 2.7\nHighres = 8\nLowres = 100\n"
+            f"diameter = 128\nthreshold = 6.0\noutput = {out}\nfirst = 0\nlast = {last}\nGPU_ID = 0\n"
+            f"window_size = 320\nnorm_type = 1\nnms_radius = 64\nresume = {resume}\n")
+        r = subprocess.run([EXE, "config.txt"], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, r.stderr
+        return r.stdout
+
+    run(3, 0, "whole.txt")
+    whole = (tmp_path / "whole.txt").read_text()
+    assert len((tmp_path / "whole.txt.done").read_text().splitlines()) == 3
+    assert len(whole.splitlines()) > 0
+
+    run(1, 0, "part.txt")
+    first = (tmp_path / "part.txt").read_text()
+    assert whole.startswith(first)
+    with open(tmp_path / "part.txt", "a") as f:  # lines of a micrograph that was being written
+        f.write("1\tmic_1.mrc\tdefocus=1.4")
+    out = run(3, 1, "part.txt")
+    assert "mic_0.mrc: done in an earlier run, skipped" in out
+    assert (tmp_path / "part.txt").read_text() == whole
+    out = run(3, 1, "part.txt")  # nothing left to do
+    assert out.count("skipped") == 3 and (tmp_path / "part.txt").read_text() == whole
