@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <array>
 #include <functional>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -114,10 +115,18 @@ static void clip_image_host(float* img, size_t N, int nx, int ny, int invert) {
       sum_s2 += q2[i];
     }
   }
+  // a non-finite pixel (or one whose square overflows float) makes the sums non-finite; the
+  // reference then searches NaN planes, here it is an input error
+  GS_REQUIRE(std::isfinite(sum) && std::isfinite(sum_s2), kInvalid,
+             "image holds non-finite pixels (NaN or Inf) or values whose squares overflow float");
   const float avg = sum;
   const float var = std::sqrt(sum_s2 - avg * avg);
-  const int up = (int)(avg + 6 * var);
-  const int low = (int)(avg - 6 * var);
+  // the reference's (int) casts, with the result x86 gives for a value outside int's range (the
+  // "integer indefinite" INT_MIN) stated instead of left undefined; sqrt of a negative
+  // difference is NaN, which also gives INT_MIN
+  auto to_int = [](float v) { return (v >= -2147483648.f && v < 2147483648.f) ? (int)v : INT_MIN; };
+  const int up = to_int(avg + 6 * var);
+  const int low = to_int(avg - 6 * var);
   if (sum_s2 > avg * avg)
     for (size_t i = 0; i < N; ++i)
       if (img[i] > up || img[i] < low) img[i] = avg;
@@ -295,7 +304,13 @@ class Engine {
   // and invert there.  Makes no HIP call, so it may run on a helper thread.
   void stage_image(const float* img, int W, int H, float* dst) const {
     std::memcpy(dst, img, sizeof(float) * (size_t)W * H);
-    if (P_.norm_type) clip_image_host(dst, (size_t)W * H, W, H, P_.invert);
+    if (P_.norm_type) {
+      clip_image_host(dst, (size_t)W * H, W, H, P_.invert);  // checks finiteness through its sums
+    } else {
+      bool finite = true;
+      for (size_t i = 0, n = (size_t)W * H; i < n; ++i) finite &= std::isfinite(dst[i]);
+      GS_REQUIRE(finite, kInvalid, "image holds non-finite pixels (NaN or Inf)");
+    }
   }
 
   // Uploads a staged image (the output of stage_image).

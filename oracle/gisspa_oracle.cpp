@@ -7,6 +7,7 @@
 #include "gisspa_oracle.h"
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -334,8 +335,11 @@ void clip_image(float* img, size_t N, int nx, int ny, int invert) {
   }
   const float avg = sum;
   const float var = std::sqrt(sum_s2 - avg * avg);
-  const int up = (int)(avg + 6 * var);
-  const int low = (int)(avg - 6 * var);
+  // (int) as x86 converts: a value outside int's range (or NaN) gives INT_MIN, stated here
+  // instead of left undefined
+  auto to_int = [](float v) { return (v >= -2147483648.f && v < 2147483648.f) ? (int)v : INT_MIN; };
+  const int up = to_int(avg + 6 * var);
+  const int low = to_int(avg - 6 * var);
   if (sum_s2 > avg * avg)
     for (size_t i = 0; i < N; ++i)
       if (img[i] > up || img[i] < low) img[i] = avg;

@@ -50,6 +50,13 @@ def test_bad_calls_leave_the_engine_usable(gs, case):
     print("image smaller than the window", flush=True)
     with pytest.raises(gs.GisspaError, match="at least window_size"):
         eng.set_image(np.zeros((300, 640), np.float32), ds.ctf)
+    print("non-finite pixels", flush=True)
+    for v in (np.nan, np.inf, -np.inf, 3e19):
+        img = ds.image.copy()
+        img[17, 301] = v
+        with pytest.raises(gs.GisspaError, match="non-finite") as ei:
+            eng.set_image(img, ds.ctf)
+        assert ei.value.status == 1
     print("unknown kernel variants and chunk", flush=True)
     with pytest.raises(gs.GisspaError, match="unknown kernel variant"):
         eng.set_variant(1, 10)
